@@ -1,0 +1,214 @@
+/*
+ * skm.h -- C ABI of the MI355X (gfx950) SKM kernel-sum / metric / regularizer
+ * library (libskm.so).
+ *
+ * The boundary replaces the C++ class GpuKernelEvaluator of the reference
+ * (include/cuda/GpuEvaluator.h:69-179).  Each entry point names the member it
+ * replaces.  Arrays are plain pointers; host-pointer entry points copy in and
+ * out, *_device entry points take device pointers and enqueue work on the
+ * evaluator's HIP stream without synchronising.
+ *
+ * Layouts (as in the reference):
+ *   images           float, x fastest, geometry = origin + spacing*index,
+ *                    direction cosines are not part of the geometry
+ *                    (include/itkGlue.h:59-63);
+ *   control points   AoS float, D values per control point (GpuEvaluator.cu:104);
+ *   derivative       SoA float, index i + N(j + M(k + O d))
+ *                    (itkTransformAdapter.hxx:202-236, GpuEvaluatorMSE.cu:165-169);
+ *   weight tensor    D*D floats per voxel, row-major.
+ *
+ * Errors: every call returns an skm_status; skm_last_error() describes the
+ * last failure of the calling thread.  An evaluator is not re-entrant: calls
+ * on one evaluator must not overlap (GpuKernelEvaluator is not either).
+ */
+#ifndef SKM_H
+#define SKM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKM_API __attribute__((visibility("default")))
+
+typedef enum skm_status {
+    SKM_SUCCESS = 0,
+    SKM_ERROR_INVALID_ARGUMENT = 1,
+    SKM_ERROR_HIP = 2,
+    SKM_ERROR_OUT_OF_MEMORY = 3,
+    SKM_ERROR_NOT_SUPPORTED = 4
+} skm_status;
+
+/* GpuKernelEvaluator::MEASURE (GpuEvaluator.h:160) */
+typedef enum skm_measure { SKM_MEASURE_MSE = 0, SKM_MEASURE_NCC = 1, SKM_MEASURE_LCC = 2 } skm_measure;
+
+/* Kernel modes selected by GKernel::useWeightImage / useWeightTensor (Kernel.h:188-196). */
+typedef enum skm_kernel_mode {
+    SKM_KERNEL_STATIONARY = 0,
+    SKM_KERNEL_WEIGHT_IMAGE = 1,
+    SKM_KERNEL_WEIGHT_TENSOR = 2
+} skm_kernel_mode;
+
+/* SKM_FLAG_EXACT_GRADIENT: derivative windows that cover every sample in the
+ * kernel support, so the derivative is the exact gradient of the value.  Off
+ * by default: the reference windows are used, including the 2D subsampled
+ * lower bound (GpuEvaluatorMSE.cu:267-273) and the LCC centre truncation
+ * (GpuEvaluatorLCC.cu:239-245). */
+#define SKM_FLAG_EXACT_GRADIENT 0x1u
+
+/* Origin, spacing and size of an image (ImageBase, include/cuda/ImageBase.h:25-50). */
+typedef struct skm_grid {
+    int32_t size[3];      /* size[2] is ignored when dim == 2 */
+    double origin[3];
+    double spacing[3];
+} skm_grid;
+
+/* Construction arguments of GpuKernelEvaluator (GpuEvaluator.h:147-154) and
+ * of the GKernel it holds (Kernel.h:134-146). */
+typedef struct skm_desc {
+    int32_t dim;                              /* 2 or 3 (SpaceDimensions)              */
+    skm_grid fixed;  const float *fixed_data; /* target image T                        */
+    skm_grid moving; const float *moving_data;/* reference image R, raw intensities;   */
+                                              /* the cubic B-spline prefilter runs     */
+                                              /* inside skm_create                      */
+    skm_grid cp;                              /* control-point grid                    */
+    double sigma_c0, sigma_c4;                /* Wendland C0 / C4 supports             */
+    int32_t kernel_mode;                      /* skm_kernel_mode                       */
+    double maximum_weight;                    /* max of the weight image               */
+    double maximum_eigenvalue_covariance[9];  /* D*D row-major                         */
+    skm_grid weight; const float *weight_data;/* weight image / tensor field, or NULL  */
+    int32_t subsample;                        /* sampling rate (>= 1)                  */
+    int32_t subsample_neighborhood;           /* LCC neighbourhood stride (>= 1)       */
+    uint64_t rng_state, rng_sequence;         /* PCG32 state (UniformRng.h:32)         */
+    uint32_t flags;                           /* SKM_FLAG_*                            */
+} skm_desc;
+
+/* GpuKernelEvaluator::EvaluationResult (GpuEvaluator.h:138-145). */
+typedef struct skm_result {
+    double measure;
+    double reg_value_rkhs;
+    double reg_value_rd;
+    double reg_value_pg;
+    uint32_t num_pixels_counted;
+    uint32_t reserved;
+} skm_result;
+
+typedef struct skm_evaluator skm_evaluator;
+
+/* Fills the reference defaults: subsample 1, neighbourhood 1, stationary kernel,
+ * identity maximum covariance, PCG32 state 0xBA5EBA11DEADBEEF / sequence 0. */
+SKM_API void skm_desc_init(skm_desc *desc);
+
+/* GpuKernelEvaluator::GpuKernelEvaluator (GpuEvaluator.h:147-154, .cu:30-91),
+ * including the host prefilter done by TransformAdapter::Initialize
+ * (itkTransformAdapter.hxx:435-439). */
+SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out);
+SKM_API void skm_destroy(skm_evaluator *ev);
+SKM_API const char *skm_last_error(void);
+
+/* SetRegularizerRKHS / RD / PG (GpuEvaluator.h:156-158). */
+SKM_API skm_status skm_set_regularizer_rkhs(skm_evaluator *ev, double weight);
+SKM_API skm_status skm_set_regularizer_rd(skm_evaluator *ev, double weight, double scaling);
+SKM_API skm_status skm_set_regularizer_pg(skm_evaluator *ev, double weight, double scaling);
+
+/* Previous-scale displacement field (the ImageNearest<VecF> argument of
+ * getValue / getValueAndDerivative / evaluateDisplacementField, cached by the
+ * reference on pointer identity, GpuEvaluator.cu:120-126).  Host AoS data;
+ * grid == NULL or data == NULL clears it. */
+SKM_API skm_status skm_set_displacement_field(skm_evaluator *ev, const skm_grid *grid, const float *df_aos);
+
+/* GpuKernelEvaluator::getValue (GpuEvaluator.h:162-166). */
+SKM_API skm_status skm_get_value(skm_evaluator *ev, int32_t measure, const float *cp_aos,
+                                 const float *cpw, int32_t do_resampling, skm_result *out);
+
+/* GpuKernelEvaluator::getValueAndDerivative (GpuEvaluator.h:168-173). */
+SKM_API skm_status skm_get_value_and_derivative(skm_evaluator *ev, int32_t measure, const float *cp_aos,
+                                                const float *cpw, float *derivative_soa,
+                                                int32_t do_resampling, skm_result *out);
+
+/* GpuKernelEvaluator::evaluateDisplacementField (GpuEvaluator.h:175-178):
+ * displacement at every fixed voxel, AoS. */
+SKM_API skm_status skm_evaluate_displacement_field(skm_evaluator *ev, const float *cp_aos,
+                                                   const float *cpw, float *dst_aos);
+
+/* Device-pointer forms of the two calls above (no counterpart in the
+ * reference, which copies through the host every call).  Work is enqueued on
+ * the evaluator's stream; d_result (device memory, may be NULL) receives the
+ * skm_result when the work completes. */
+SKM_API skm_status skm_set_stream(skm_evaluator *ev, void *hip_stream);
+SKM_API skm_status skm_get_value_and_derivative_device(skm_evaluator *ev, int32_t measure,
+                                                       const float *d_cp_aos, const float *d_cpw,
+                                                       float *d_derivative_soa, int32_t do_resampling,
+                                                       skm_result *d_result);
+SKM_API skm_status skm_evaluate_displacement_field_device(skm_evaluator *ev, const float *d_cp_aos,
+                                                          const float *d_cpw, float *d_dst_aos);
+SKM_API skm_status skm_synchronize(skm_evaluator *ev);
+
+/* Jitter offsets of the last evaluation, D ints per subsampled cell
+ * (GpuEvaluator.cu:107-118); n_cells receives prod(fixed.size / subsample). */
+SKM_API skm_status skm_get_cube_offsets(skm_evaluator *ev, int32_t *out, int64_t *n_cells);
+
+/* TransformAdapter::ComputeControlPointImage geometry (itkTransformAdapter.hxx:246-308). */
+SKM_API skm_status skm_control_point_grid(int32_t dim, const skm_grid *image, const double *cp_spacing,
+                                          skm_grid *cp_out);
+
+/* ---- metric assembly and optimisation, device resident ---------------- */
+
+/* RegularizedImageToImageMetricv4 switches (itkRegularizedImageToImageMetricv4.h:124-160). */
+typedef struct skm_metric_options {
+    int32_t measure;
+    double reg_l1, reg_l21, reg_l2;
+    int32_t use_negative_gradient;   /* default 1 (itkRegularizedImageToImageMetricv4.hxx:47) */
+    int32_t do_resampling;           /* default 1 */
+} skm_metric_options;
+
+/* Optimizer kinds: Standard (itkStandardGradientDescentOptimizerv4.hxx:159-166),
+ * AdaptiveStep (itkAdaptiveStepGradientDescentOptimizerv4.hxx:86-128) and a
+ * fixed step of size a. */
+typedef enum skm_optimizer_kind { SKM_OPT_STANDARD = 0, SKM_OPT_ADAPTIVE = 1, SKM_OPT_FIXED = 2 } skm_optimizer_kind;
+
+typedef struct skm_optimizer_options {
+    int32_t kind;
+    double a, A, alpha;              /* step schedule                                  */
+    double fmax, fmin, omega;        /* adaptive time function                          */
+    int32_t orthant_projection;
+    int32_t iterations;
+} skm_optimizer_options;
+
+/* Metric value per iteration as reported by
+ * RegularizedImageToImageMetricv4::GetValueAndDerivative (measure + weighted
+ * regularizers, .hxx:531-572). */
+typedef struct skm_iteration_record {
+    double value;          /* measure + regularizers */
+    double measure;
+    double learning_rate;
+    uint32_t num_pixels_counted;
+    uint32_t reserved;
+} skm_iteration_record;
+
+/* Runs `iterations` steps of gradient descent entirely on the device:
+ * GetValueAndDerivative, host-side regularizers L1/L21/L2 and sign flip
+ * (itkRegularizedImageToImageMetricv4.hxx:430-529), then the optimizer
+ * update theta_{k+1} = theta_k + eta_k d_k with optional orthant projection
+ * (itkStandardGradientDescentOptimizerv4.hxx:116-157).  cpw (host) holds the
+ * control-point weights; params_soa (host)
+ * holds the start point and receives the result; records (host, may be NULL)
+ * receives one entry per iteration. */
+SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *metric,
+                                const skm_optimizer_options *opt, const float *cpw, float *params_soa,
+                                skm_iteration_record *records);
+
+/* Adds L1 / L21 / L2 values and subgradients to a host derivative and applies
+ * the sign flip exactly as the metric does on the host
+ * (itkRegularizedImageToImageMetricv4.hxx:430-529).  Returns the weighted sum
+ * of the three regularizers in *reg_value. */
+SKM_API skm_status skm_host_regularizers(int32_t dim, int64_t n_cp, const double *params_soa,
+                                         double reg_l1, double reg_l21, double reg_l2,
+                                         int32_t use_negative_gradient, double *derivative_soa,
+                                         double *reg_value, double *l1, double *l21, double *l2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,17 @@
+"""MI355X (gfx950) implementation of the SKM kernel-sum / metric / regularizer path.
+
+The compute path is libskm.so (HIP kernels + C ABI, include/skm.h); this
+package is the host-side mirror of the reference's GpuKernelEvaluator
+interface over that ABI.
+"""
+from . import _capi
+from .evaluator import (MEASURE, EvaluationResult, GpuKernelEvaluator, Grid, Image, Kernel,
+                        control_point_grid)
+
+__all__ = ["MEASURE", "EvaluationResult", "GpuKernelEvaluator", "Grid", "Image", "Kernel",
+           "control_point_grid", "load_library"]
+
+
+def load_library():
+    """Loads the in-tree libskm.so (raises if it was not built)."""
+    return _capi.load()

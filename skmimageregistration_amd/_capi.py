@@ -1,0 +1,157 @@
+"""ctypes declarations of the C ABI in include/skm.h (libskm.so).
+
+Loading fails loudly when the in-tree library is missing: there is no CPU or
+PyTorch fallback for any entry point.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "libskm.so")
+
+SKM_SUCCESS = 0
+SKM_MEASURE_MSE, SKM_MEASURE_NCC, SKM_MEASURE_LCC = 0, 1, 2
+SKM_KERNEL_STATIONARY, SKM_KERNEL_WEIGHT_IMAGE, SKM_KERNEL_WEIGHT_TENSOR = 0, 1, 2
+SKM_FLAG_EXACT_GRADIENT = 0x1
+SKM_OPT_STANDARD, SKM_OPT_ADAPTIVE, SKM_OPT_FIXED = 0, 1, 2
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+
+
+class SkmGrid(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int32 * 3), ("origin", ctypes.c_double * 3), ("spacing", ctypes.c_double * 3)]
+
+
+class SkmDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("fixed", SkmGrid), ("fixed_data", c_float_p),
+        ("moving", SkmGrid), ("moving_data", c_float_p),
+        ("cp", SkmGrid),
+        ("sigma_c0", ctypes.c_double), ("sigma_c4", ctypes.c_double),
+        ("kernel_mode", ctypes.c_int32),
+        ("maximum_weight", ctypes.c_double),
+        ("maximum_eigenvalue_covariance", ctypes.c_double * 9),
+        ("weight", SkmGrid), ("weight_data", c_float_p),
+        ("subsample", ctypes.c_int32), ("subsample_neighborhood", ctypes.c_int32),
+        ("rng_state", ctypes.c_uint64), ("rng_sequence", ctypes.c_uint64),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+class SkmResult(ctypes.Structure):
+    _fields_ = [("measure", ctypes.c_double), ("reg_value_rkhs", ctypes.c_double),
+                ("reg_value_rd", ctypes.c_double), ("reg_value_pg", ctypes.c_double),
+                ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class SkmMetricOptions(ctypes.Structure):
+    _fields_ = [("measure", ctypes.c_int32), ("reg_l1", ctypes.c_double), ("reg_l21", ctypes.c_double),
+                ("reg_l2", ctypes.c_double), ("use_negative_gradient", ctypes.c_int32),
+                ("do_resampling", ctypes.c_int32)]
+
+
+class SkmOptimizerOptions(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("a", ctypes.c_double), ("A", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("fmax", ctypes.c_double), ("fmin", ctypes.c_double),
+                ("omega", ctypes.c_double), ("orthant_projection", ctypes.c_int32),
+                ("iterations", ctypes.c_int32)]
+
+
+class SkmIterationRecord(ctypes.Structure):
+    _fields_ = [("value", ctypes.c_double), ("measure", ctypes.c_double), ("learning_rate", ctypes.c_double),
+                ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+# Every symbol include/skm.h declares, with (restype, argtypes).
+_SIGNATURES = {
+    "skm_desc_init": (None, [ctypes.POINTER(SkmDesc)]),
+    "skm_create": (ctypes.c_int, [ctypes.POINTER(SkmDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "skm_destroy": (None, [ctypes.c_void_p]),
+    "skm_last_error": (ctypes.c_char_p, []),
+    "skm_set_regularizer_rkhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "skm_set_regularizer_rd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "skm_set_regularizer_pg": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "skm_set_displacement_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmGrid), c_float_p]),
+    "skm_get_value": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_float_p, ctypes.c_int32,
+                                     ctypes.POINTER(SkmResult)]),
+    "skm_get_value_and_derivative": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_float_p,
+                                                    c_float_p, ctypes.c_int32, ctypes.POINTER(SkmResult)]),
+    "skm_evaluate_displacement_field": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p]),
+    "skm_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "skm_get_value_and_derivative_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                           ctypes.c_void_p]),
+    "skm_evaluate_displacement_field_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                              ctypes.c_void_p]),
+    "skm_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "skm_get_cube_offsets": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, ctypes.POINTER(ctypes.c_int64)]),
+    "skm_control_point_grid": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_double_p,
+                                              ctypes.POINTER(SkmGrid)]),
+    "skm_optimize": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmMetricOptions),
+                                    ctypes.POINTER(SkmOptimizerOptions), c_float_p, c_float_p,
+                                    ctypes.POINTER(SkmIterationRecord)]),
+    "skm_host_regularizers": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, c_double_p, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_int32, c_double_p,
+                                             c_double_p, c_double_p, c_double_p, c_double_p]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+_lib = None
+
+
+def load():
+    """Returns the loaded libskm.so; raises if the in-tree build is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libskm.so not found at {LIB_PATH}; build it with "
+            "`python -m skmimageregistration_amd.build` (there is no fallback path)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class SkmError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"skm status {status}: {message}")
+        self.status = status
+
+
+def check(status):
+    if status != SKM_SUCCESS:
+        msg = load().skm_last_error()
+        raise SkmError(status, msg.decode() if msg else "")
+
+
+def fptr(a):
+    return a.ctypes.data_as(c_float_p)
+
+
+def dptr(a):
+    return a.ctypes.data_as(c_double_p)
+
+
+def make_grid(size, origin=None, spacing=None):
+    g = SkmGrid()
+    dim = len(size)
+    for d in range(3):
+        g.size[d] = int(size[d]) if d < dim else 1
+        g.origin[d] = float(origin[d]) if origin is not None and d < dim else 0.0
+        g.spacing[d] = float(spacing[d]) if spacing is not None and d < dim else 1.0
+    return g
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)

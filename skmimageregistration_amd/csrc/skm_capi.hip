@@ -1,0 +1,823 @@
+// skm_capi.hip -- C ABI of libskm.so: the evaluator object that replaces
+// GpuKernelEvaluator (include/cuda/GpuEvaluator.h:69-179), its host-side
+// set-up (B-spline prefilter, static windows, per-control-point kernel state)
+// and the orchestration of the device passes on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "skm.h"
+#include "skm_internal.hpp"
+
+using namespace skm;
+
+namespace {
+
+thread_local std::string g_error;
+
+skm_status fail(skm_status st, const std::string &msg)
+{
+    g_error = msg;
+    return st;
+}
+
+#define SKM_HIP(expr)                                                                                              \
+    do {                                                                                                           \
+        hipError_t e_ = (expr);                                                                                    \
+        if (e_ != hipSuccess)                                                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? SKM_ERROR_OUT_OF_MEMORY : SKM_ERROR_HIP,                      \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                                        \
+    } while (0)
+
+#define SKM_TRY(expr)                                                                                              \
+    do {                                                                                                           \
+        skm_status s_ = (expr);                                                                                    \
+        if (s_ != SKM_SUCCESS) return s_;                                                                          \
+    } while (0)
+
+long long grid_count(const skm_grid &g, int dim)
+{
+    long long n = 1;
+    for (int d = 0; d < dim; ++d) n *= g.size[d];
+    return n;
+}
+
+Grid to_grid(const skm_grid &g, int dim)
+{
+    Grid r;
+    for (int d = 0; d < 3; ++d) {
+        r.n[d] = d < dim ? g.size[d] : 1;
+        r.o[d] = d < dim ? (float)g.origin[d] : 0.0f;
+        r.h[d] = d < dim ? (float)g.spacing[d] : 1.0f;
+        r.ih[d] = (float)(1.0 / (d < dim ? g.spacing[d] : 1.0));
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------- prefilter
+
+// Unser-Thevenaz cubic B-spline prefilter along one line, pole sqrt(3)-2,
+// mirror boundary (BsplineImage.h:37-70).
+void prefilter_line(double *c, int n, long long stride, double z)
+{
+    double lambda = 2.0 - z - 1.0 / z;
+    for (int i = 0; i < n; ++i) c[i * stride] *= lambda;
+    // causal initial value: sum over the mirrored signal
+    double zn = z, iz = 1.0 / z, z2n = std::pow(z, (double)(n - 1));
+    double sum = c[0] + z2n * c[(long long)(n - 1) * stride];
+    z2n *= z2n * iz;
+    for (int i = 1; i < n - 1; ++i) {
+        sum += (zn + z2n) * c[i * stride];
+        zn *= z;
+        z2n *= iz;
+    }
+    c[0] = sum / (1.0 - zn * zn);
+    for (int i = 1; i < n; ++i) c[i * stride] += z * c[(i - 1) * stride];
+    c[(long long)(n - 1) * stride] =
+        (z / (z * z - 1.0)) * (z * c[(long long)(n - 2) * stride] + c[(long long)(n - 1) * stride]);
+    for (int i = n - 2; i >= 0; --i) c[i * stride] = z * (c[(i + 1) * stride] - c[i * stride]);
+}
+
+// Interpolation coefficients of the moving image, shifted by one and padded
+// to (n+3)^D with mirrored borders (BsplineImage.h:72-108, 174-221).
+std::vector<float> bspline_coefficients(int dim, const int *size, const float *src)
+{
+    long long n = 1;
+    for (int d = 0; d < dim; ++d) n *= size[d];
+    std::vector<double> c(src, src + n);
+    const double pole = std::sqrt(3.0) - 2.0;
+    if (dim == 2) {
+        for (int y = 0; y < size[1]; ++y) prefilter_line(c.data() + (long long)y * size[0], size[0], 1, pole);
+        for (int x = 0; x < size[0]; ++x) prefilter_line(c.data() + x, size[1], size[0], pole);
+    } else {
+        long long sxy = (long long)size[0] * size[1];
+        for (int y = 0; y < size[1]; ++y)
+            for (int x = 0; x < size[0]; ++x) prefilter_line(c.data() + x + (long long)y * size[0], size[2], sxy, pole);
+        for (int z = 0; z < size[2]; ++z)
+            for (int x = 0; x < size[0]; ++x) prefilter_line(c.data() + x + z * sxy, size[1], size[0], pole);
+        for (int z = 0; z < size[2]; ++z)
+            for (int y = 0; y < size[1]; ++y)
+                prefilter_line(c.data() + (long long)size[0] * (y + (long long)z * size[1]), size[0], 1, pole);
+    }
+    int ps[3] = {size[0] + 3, size[1] + 3, dim == 3 ? size[2] + 3 : 1};
+    std::vector<float> out((size_t)ps[0] * ps[1] * ps[2]);
+    auto mirror = [](int x, int len) { return (len - 1) - std::abs((len - 1) - std::abs(x - 1)); };
+    for (int z = 0; z < ps[2]; ++z)
+        for (int y = 0; y < ps[1]; ++y)
+            for (int x = 0; x < ps[0]; ++x) {
+                int xi = mirror(x, size[0]), yi = mirror(y, size[1]), zi = dim == 3 ? mirror(z, size[2]) : 0;
+                out[x + (size_t)ps[0] * (y + (size_t)ps[1] * z)] =
+                    (float)c[xi + (size_t)size[0] * (yi + (size_t)size[1] * zi)];
+            }
+    return out;
+}
+
+// ---------------------------------------------------------------- host math
+
+double det_h(int dim, const double *m)
+{
+    if (dim == 2) return m[0] * m[3] - m[2] * m[1];
+    return m[0] * m[4] * m[8] + m[3] * m[7] * m[2] + m[6] * m[1] * m[5] - m[0] * m[7] * m[5] -
+           m[6] * m[4] * m[2] - m[3] * m[1] * m[8];
+}
+
+double max_eig_h(int dim, const double *m)
+{
+    if (dim == 2) {
+        double tr = m[0] + m[3], det = det_h(2, m);
+        return 0.5 * (tr + std::sqrt(std::max(tr * tr - 4.0 * det, 0.0)));
+    }
+    double tri = m[1] * m[1] + m[2] * m[2] + m[5] * m[5];
+    if (tri == 0.0) return std::max(m[0], std::max(m[4], m[8]));
+    double q = (1.0 / 3.0) * (m[0] + m[4] + m[8]);
+    double psq = 2.0 * tri + (m[0] - q) * (m[0] - q) + (m[4] - q) * (m[4] - q) + (m[8] - q) * (m[8] - q);
+    double p = std::sqrt(psq * (1.0 / 6.0));
+    double b[9];
+    std::memcpy(b, m, sizeof(b));
+    b[0] -= q; b[4] -= q; b[8] -= q;
+    double r = det_h(3, b) * (0.5 / (p * p * p));
+    double phi = (1.0 / 3.0) * std::acos(std::min(1.0, std::max(-1.0, r)));
+    return q + 2.0 * p * std::cos(phi);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- evaluator
+
+struct skm_evaluator {
+    skm_desc desc{};
+    int dim = 3;
+    Problem pr{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int max_partial_blocks = 0;
+
+    float *d_fixed = nullptr, *d_coeffs = nullptr, *d_weight = nullptr, *d_df = nullptr;
+    float4 *d_cpc = nullptr;
+    float *d_cp_state = nullptr;
+    int *d_cp_window = nullptr, *d_reg_window = nullptr, *d_lcc_window = nullptr, *d_dense_window = nullptr;
+    int *d_offsets = nullptr;
+    float4 *d_sg = nullptr, *d_sv = nullptr;
+    float *d_sb = nullptr;
+    double *d_partials = nullptr, *d_reg_partials = nullptr, *d_hreg_partials = nullptr, *d_dot_partials = nullptr;
+    Scalars *d_sc = nullptr;
+    float *d_deriv = nullptr, *d_cp_in = nullptr, *d_cpw_in = nullptr, *d_dst = nullptr;
+    float4 *d_dense = nullptr, *d_dense_q = nullptr, *d_ccoef = nullptr, *d_gamma = nullptr;
+    float *d_params = nullptr, *d_prev_g = nullptr;
+    OptState *d_opt = nullptr;
+    skm_iteration_record *d_records = nullptr;
+    int records_cap = 0;
+
+    uint64_t rng_state = 0, rng_inc = 1;
+    bool evaluated_once = false;
+    double reg_w[3] = {0, 0, 0};
+    double rd_scaling = 1.0, pg_scaling = 1.0;
+
+    ~skm_evaluator()
+    {
+        void *ptrs[] = {d_fixed, d_coeffs, d_weight, d_df, d_cpc, d_cp_state, d_cp_window, d_reg_window,
+                        d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
+                        d_hreg_partials, d_dot_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
+                        d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_opt, d_records};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+skm_status dev_alloc(T **p, size_t count)
+{
+    if (count == 0) count = 1;
+    SKM_HIP(hipMalloc((void **)p, sizeof(T) * count));
+    return SKM_SUCCESS;
+}
+
+template <typename T>
+skm_status dev_upload(T **p, const T *src, size_t count)
+{
+    SKM_TRY(dev_alloc(p, count));
+    if (src && count) SKM_HIP(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    return SKM_SUCCESS;
+}
+
+skm_status validate_grid(const skm_grid &g, int dim, const char *name, int min_size)
+{
+    for (int d = 0; d < dim; ++d) {
+        if (g.size[d] < min_size) return fail(SKM_ERROR_INVALID_ARGUMENT, std::string(name) + ": size too small");
+        if (!(g.spacing[d] > 0.0)) return fail(SKM_ERROR_INVALID_ARGUMENT, std::string(name) + ": spacing must be > 0");
+    }
+    return SKM_SUCCESS;
+}
+
+// Static per-control-point data: kernel state at p_j (getSigmaAtPoint /
+// getCovarianceAtPoint and getRegionSupport, Kernel.h:148-171) and the three
+// index windows the derivative passes use, all in FP64 as the reference
+// computes them with ScalarType = double.
+skm_status build_static(skm_evaluator *ev)
+{
+    const skm_desc &ds = ev->desc;
+    const int dim = ev->dim;
+    const long long ncp = grid_count(ds.cp, dim);
+    const double smin = std::min(ds.sigma_c4, ds.sigma_c0);
+    std::vector<float> state((size_t)ncp * kCpStateStride, 0.0f);
+    std::vector<double> support(ncp, smin);
+    std::vector<int> win((size_t)ncp * 6), lwin((size_t)ncp * 6), dwin((size_t)ncp * 6), rwin((size_t)ncp * 6);
+    int ns[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) ns[d] = ds.fixed.size[d] / ds.subsample;
+    const int nn = dim * dim;
+    for (long long j = 0; j < ncp; ++j) {
+        int jc[3] = {0, 0, 0};
+        long long r = j;
+        for (int d = 0; d < dim; ++d) { jc[d] = (int)(r % ds.cp.size[d]); r /= ds.cp.size[d]; }
+        double p[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) p[d] = ds.cp.origin[d] + ds.cp.spacing[d] * jc[d];
+        float *st = &state[(size_t)j * kCpStateStride];
+        if (ds.kernel_mode != SKM_KERNEL_STATIONARY) {
+            long long f = 0, stride = 1;
+            bool in = true;
+            for (int d = 0; d < dim; ++d) {
+                int i = (int)((1.0 / ds.weight.spacing[d]) * (p[d] - ds.weight.origin[d]) + 0.5);
+                in = in && i >= 0 && i < ds.weight.size[d];
+                f += stride * i;
+                stride *= ds.weight.size[d];
+            }
+            if (ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE) {
+                double s = in ? ds.weight_data[f] : 0.0;
+                st[0] = (float)s;
+                st[1] = (float)std::sqrt(std::fabs(s));
+                support[j] = smin / std::sqrt(2.0 / (ds.maximum_weight + s));
+            } else {
+                double cov[9] = {0}, sum[9] = {0};
+                for (int i = 0; i < nn; ++i) cov[i] = in ? ds.weight_data[f * nn + i] : ((i % (dim + 1)) == 0 ? 1.0 : 0.0);
+                for (int i = 0; i < nn; ++i) st[2 + i] = (float)cov[i];
+                st[1] = (float)std::sqrt(det_h(dim, cov));
+                for (int i = 0; i < nn; ++i) sum[i] = ds.maximum_eigenvalue_covariance[i] + cov[i];
+                support[j] = smin / std::sqrt(2.0 / max_eig_h(dim, sum));
+            }
+        }
+        st[11] = (float)support[j];
+        const bool exact = (ds.flags & SKM_FLAG_EXACT_GRADIENT) != 0;
+        for (int d = 0; d < 3; ++d) {
+            int *w = &win[(size_t)j * 6], *lw = &lwin[(size_t)j * 6], *dw = &dwin[(size_t)j * 6], *rw = &rwin[(size_t)j * 6];
+            if (d >= dim) { w[d] = w[d + 3] = lw[d] = lw[d + 3] = dw[d] = dw[d + 3] = rw[d] = rw[d + 3] = 0; continue; }
+            double idx = (1.0 / ds.fixed.spacing[d]) * (p[d] - ds.fixed.origin[d]);
+            double sp = support[j] / ds.fixed.spacing[d];
+            double sub = (double)ds.subsample;
+            int lim = (ds.subsample > 1 ? ns[d] : ds.fixed.size[d]) - 1;
+            int lo, hi;
+            if (dim == 3 || exact) {  // GpuEvaluatorMSE.cu:126-132 (3D)
+                lo = (int)((idx - sp) / sub);
+                hi = (int)((idx + sp) / sub + 1.0);
+            } else {                  // GpuEvaluatorMSE.cu:267-273 (2D)
+                lo = (int)((idx - sp) / sub + 1.0);
+                hi = (int)((idx + sp) / sub);
+            }
+            if (exact) { lo -= 1; hi += 1; }
+            w[d] = std::min(std::max(lo, 0), lim);
+            w[d + 3] = std::min(std::max(hi, 0), lim);
+            // LCC centre window, tight in 2D and 3D (GpuEvaluatorLCC.cu:239-245, 724-730)
+            lw[d] = std::min(std::max((int)((idx - sp) / sub + 1.0), 0), lim);
+            lw[d + 3] = std::min(std::max((int)((idx + sp) / sub), 0), lim);
+            // dense voxel window for the exact LCC gather
+            dw[d] = std::min(std::max((int)(idx - sp) - 1, 0), ds.fixed.size[d] - 1);
+            dw[d + 3] = std::min(std::max((int)(idx + sp + 1.0) + 1, 0), ds.fixed.size[d] - 1);
+            // regularizer window on the cp grid (GpuEvaluatorRegularizers.cu:52-54)
+            double csp = support[j] / ds.cp.spacing[d];
+            rw[d] = std::min(std::max((int)((double)jc[d] - csp + 1.0), 0), ds.cp.size[d] - 1);
+            rw[d + 3] = std::min(std::max((int)((double)jc[d] + csp), 0), ds.cp.size[d] - 1);
+        }
+    }
+    SKM_TRY(dev_upload(&ev->d_cp_state, state.data(), state.size()));
+    SKM_TRY(dev_upload(&ev->d_cp_window, win.data(), win.size()));
+    SKM_TRY(dev_upload(&ev->d_lcc_window, lwin.data(), lwin.size()));
+    SKM_TRY(dev_upload(&ev->d_dense_window, dwin.data(), dwin.size()));
+    SKM_TRY(dev_upload(&ev->d_reg_window, rwin.data(), rwin.size()));
+    return SKM_SUCCESS;
+}
+
+skm_status ensure_lcc_buffers(skm_evaluator *ev)
+{
+    if (!ev->d_dense) SKM_TRY(dev_alloc(&ev->d_dense, ev->pr.n_vox));
+    if (!ev->d_ccoef) SKM_TRY(dev_alloc(&ev->d_ccoef, ev->pr.n_sub));
+    return SKM_SUCCESS;
+}
+
+// One evaluation on the device: value (+ derivative) of the measure and the
+// regularizers, everything left in d_sc / deriv (GpuEvaluator.cu:94-357, 361-714).
+skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *deriv)
+{
+    Problem &pr = ev->pr;
+    hipStream_t s = ev->stream;
+    if (pr.sub > 1 && (!ev->evaluated_once || do_resampling)) {
+        launch_offsets(pr, ev->d_offsets, ev->rng_state, ev->rng_inc, s);
+        ev->rng_state = lcg_advance(ev->rng_state, (uint64_t)pr.n_sub * pr.dim, ev->rng_inc);
+        ev->evaluated_once = true;
+    }
+    int blocks = 0;
+    if (measure == SKM_MEASURE_MSE || measure == SKM_MEASURE_NCC) {
+        launch_value(pr, measure, ev->d_sg, ev->d_sb, ev->d_partials, &blocks, s);
+        launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s);
+        if (deriv) {
+            launch_prepare(pr, measure, ev->d_sg, ev->d_sb, ev->d_sc, ev->d_sv, s);
+            launch_gather(pr, ev->d_sv, deriv, s);
+        }
+    } else if (measure == SKM_MEASURE_LCC) {
+        SKM_TRY(ensure_lcc_buffers(ev));
+        launch_dense_warp(pr, 0, ev->d_dense, s);
+        launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s);
+        launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s);
+        if (deriv) {
+            if (pr.exact) {
+                if (!ev->d_gamma) SKM_TRY(dev_alloc(&ev->d_gamma, pr.n_vox));
+                SKM_HIP(hipMemsetAsync(ev->d_gamma, 0, sizeof(float4) * pr.n_vox, s));
+                launch_lcc_scatter(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_gamma, s);
+                Problem dp = pr;  // gather over the dense voxel grid
+                dp.sub = 1;
+                for (int d = 0; d < 3; ++d) dp.nsub[d] = pr.fixed.n[d];
+                dp.n_sub = pr.n_vox;
+                dp.cp_window = ev->d_dense_window;
+                launch_gather(dp, ev->d_gamma, deriv, s);
+            } else {
+                const float4 *dense = ev->d_dense;
+                Problem lp = pr;
+                lp.cp_window = ev->d_lcc_window;
+                if (pr.dim == 3 && pr.kc.mode != SKM_KERNEL_STATIONARY) {
+                    // the 3D derivative kernel warps neighbours with sigma(y) twice
+                    if (!ev->d_dense_q) SKM_TRY(dev_alloc(&ev->d_dense_q, pr.n_vox));
+                    launch_dense_warp(pr, 1, ev->d_dense_q, s);
+                    int b2 = 0;
+                    launch_lcc_centres(pr, ev->d_dense_q, ev->d_ccoef, ev->d_partials + (size_t)blocks * kPartialsK,
+                                       &b2, s);
+                    dense = ev->d_dense_q;
+                }
+                launch_lcc_gather_reference(lp, dense, ev->d_ccoef, ev->d_sc, deriv, s);
+            }
+        }
+    } else {
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "unknown measure");
+    }
+    if (ev->reg_w[0] > 0 || ev->reg_w[1] > 0 || ev->reg_w[2] > 0)
+        launch_regularizers(pr, ev->reg_w, ev->pg_scaling, deriv != nullptr, deriv, ev->d_reg_partials, ev->d_sc, s);
+    SKM_HIP(hipGetLastError());
+    return SKM_SUCCESS;
+}
+
+skm_status upload_cp(skm_evaluator *ev, const float *cp_aos, const float *cpw)
+{
+    if (!cp_aos || !cpw) return fail(SKM_ERROR_INVALID_ARGUMENT, "null control-point data");
+    size_t n = (size_t)ev->pr.n_cp;
+    SKM_HIP(hipMemcpyAsync(ev->d_cp_in, cp_aos, sizeof(float) * n * ev->dim, hipMemcpyHostToDevice, ev->stream));
+    SKM_HIP(hipMemcpyAsync(ev->d_cpw_in, cpw, sizeof(float) * n, hipMemcpyHostToDevice, ev->stream));
+    launch_pack_cp(ev->pr, ev->d_cp_in, ev->d_cpw_in, 0, ev->d_cpc, ev->stream);
+    return SKM_SUCCESS;
+}
+
+skm_status read_result(skm_evaluator *ev, skm_result *out)
+{
+    Scalars sc;
+    SKM_HIP(hipMemcpyAsync(&sc, ev->d_sc, sizeof(Scalars), hipMemcpyDeviceToHost, ev->stream));
+    SKM_HIP(hipStreamSynchronize(ev->stream));
+    if (out) {
+        out->measure = sc.measure;
+        out->reg_value_rkhs = sc.reg[0];
+        out->reg_value_rd = sc.reg[1];
+        out->reg_value_pg = sc.reg[2];
+        out->num_pixels_counted = (uint32_t)sc.count;
+        out->reserved = 0;
+    }
+    return SKM_SUCCESS;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" {
+
+SKM_API void skm_desc_init(skm_desc *desc)
+{
+    if (!desc) return;
+    std::memset(desc, 0, sizeof(*desc));
+    desc->dim = 3;
+    for (int d = 0; d < 3; ++d) {
+        desc->fixed.spacing[d] = desc->moving.spacing[d] = desc->cp.spacing[d] = desc->weight.spacing[d] = 1.0;
+        desc->fixed.size[d] = desc->moving.size[d] = desc->cp.size[d] = desc->weight.size[d] = 1;
+    }
+    desc->sigma_c0 = desc->sigma_c4 = 1.0;
+    desc->kernel_mode = SKM_KERNEL_STATIONARY;
+    desc->maximum_weight = 1.0;
+    for (int i = 0; i < 3; ++i) desc->maximum_eigenvalue_covariance[i * 3 + i] = 1.0;
+    desc->subsample = 1;
+    desc->subsample_neighborhood = 1;
+    desc->rng_state = 0xBA5EBA11DEADBEEFULL;
+    desc->rng_sequence = 0;
+}
+
+SKM_API const char *skm_last_error(void) { return g_error.c_str(); }
+
+SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
+{
+    if (!desc || !out) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    const int dim = desc->dim;
+    if (dim != 2 && dim != 3) return fail(SKM_ERROR_INVALID_ARGUMENT, "dim must be 2 or 3");
+    SKM_TRY(validate_grid(desc->fixed, dim, "fixed", 1));
+    SKM_TRY(validate_grid(desc->moving, dim, "moving", 3));
+    SKM_TRY(validate_grid(desc->cp, dim, "cp", 1));
+    if (!desc->fixed_data || !desc->moving_data) return fail(SKM_ERROR_INVALID_ARGUMENT, "image data missing");
+    if (!(desc->sigma_c0 > 0) || !(desc->sigma_c4 > 0))
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "Kernel parameters have to be positive.");
+    if (desc->subsample < 1 || desc->subsample > 100)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "Sampling rate inproper.");
+    if (desc->subsample_neighborhood < 1 || desc->subsample_neighborhood > 100)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "Neighborhood sampling rate inproper.");
+    if (desc->kernel_mode < 0 || desc->kernel_mode > 2) return fail(SKM_ERROR_INVALID_ARGUMENT, "bad kernel mode");
+    for (int d = 0; d < dim; ++d)
+        if (desc->fixed.size[d] / desc->subsample < 1)
+            return fail(SKM_ERROR_INVALID_ARGUMENT, "subsample larger than the fixed image");
+    const long long nw = grid_count(desc->weight, dim);
+    if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
+        SKM_TRY(validate_grid(desc->weight, dim, "weight", 1));
+        if (!desc->weight_data) return fail(SKM_ERROR_INVALID_ARGUMENT, "Weight image not provided.");
+        if (desc->kernel_mode == SKM_KERNEL_WEIGHT_IMAGE) {
+            for (long long i = 0; i < nw; ++i)
+                if (!(desc->weight_data[i] > 1e-8))
+                    return fail(SKM_ERROR_INVALID_ARGUMENT, "Weighting image has to be strictly positive everywhere.");
+        } else {
+            const int nn = dim * dim;
+            for (long long i = 0; i < nw; ++i) {
+                double m[9];
+                for (int k = 0; k < nn; ++k) m[k] = desc->weight_data[i * nn + k];
+                if (!(det_h(dim, m) > 0))
+                    return fail(SKM_ERROR_INVALID_ARGUMENT, "Weighting tensors must be positive definite.");
+            }
+        }
+    }
+    if (desc->moving.size[0] > 0xffff || desc->subsample > 255)
+        return fail(SKM_ERROR_NOT_SUPPORTED, "image or subsample too large");
+
+    skm_evaluator *ev = new skm_evaluator();
+    ev->desc = *desc;
+    ev->desc.fixed_data = ev->desc.moving_data = ev->desc.weight_data = nullptr;
+    ev->dim = dim;
+    ev->rng_state = desc->rng_state;
+    ev->rng_inc = desc->rng_sequence | 1ULL;
+    auto cleanup = [&](skm_status st) {
+        delete ev;
+        return st;
+    };
+    skm_status st;
+    if (hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(SKM_ERROR_HIP, "hipStreamCreate failed"));
+    ev->own_stream = true;
+
+    Problem &pr = ev->pr;
+    pr.dim = dim;
+    pr.fixed = to_grid(desc->fixed, dim);
+    pr.moving = to_grid(desc->moving, dim);
+    pr.cp = to_grid(desc->cp, dim);
+    pr.weight = to_grid(desc->weight, dim);
+    pr.df = to_grid(desc->fixed, dim);
+    pr.kc.mode = desc->kernel_mode;
+    pr.kc.inv_s4 = (float)(1.0 / desc->sigma_c4);
+    pr.kc.inv_s0 = (float)(1.0 / desc->sigma_c0);
+    pr.kc.smin = (float)std::min(desc->sigma_c4, desc->sigma_c0);
+    pr.kc.max_weight = (float)desc->maximum_weight;
+    for (int i = 0; i < 9; ++i) pr.kc.max_eig_cov[i] = (float)desc->maximum_eigenvalue_covariance[i];
+    pr.sub = desc->subsample;
+    pr.subn = desc->subsample_neighborhood;
+    for (int d = 0; d < 3; ++d) pr.nsub[d] = d < dim ? desc->fixed.size[d] / desc->subsample : 1;
+    pr.n_sub = (long long)pr.nsub[0] * pr.nsub[1] * pr.nsub[2];
+    pr.n_cp = grid_count(desc->cp, dim);
+    pr.n_vox = grid_count(desc->fixed, dim);
+    pr.exact = (desc->flags & SKM_FLAG_EXACT_GRADIENT) ? 1 : 0;
+
+    // images
+    int msize[3] = {desc->moving.size[0], desc->moving.size[1], dim == 3 ? desc->moving.size[2] : 1};
+    std::vector<float> coeffs = bspline_coefficients(dim, msize, desc->moving_data);
+    if ((st = dev_upload(&ev->d_coeffs, coeffs.data(), coeffs.size())) != SKM_SUCCESS) return cleanup(st);
+    if ((st = dev_upload(&ev->d_fixed, desc->fixed_data, (size_t)pr.n_vox)) != SKM_SUCCESS) return cleanup(st);
+    if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
+        size_t per = desc->kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 1 : (size_t)dim * dim;
+        std::vector<float> w(desc->weight_data, desc->weight_data + nw * per);
+        if ((st = dev_upload(&ev->d_weight, w.data(), w.size())) != SKM_SUCCESS) return cleanup(st);
+    }
+    {
+        // static tables need the weight data on the host
+        skm_desc tmp = ev->desc;
+        tmp.weight_data = desc->weight_data;
+        ev->desc = tmp;
+        st = build_static(ev);
+        ev->desc.weight_data = nullptr;
+        if (st != SKM_SUCCESS) return cleanup(st);
+    }
+
+    // per-sample and per-cp buffers
+    int tiles = 1;
+    {
+        int tx = dim == 3 ? 8 : 16, ty = dim == 3 ? 8 : 16, tz = dim == 3 ? 4 : 1;
+        long long a = (long long)((pr.nsub[0] + tx - 1) / tx) * ((pr.nsub[1] + ty - 1) / ty) * ((pr.nsub[2] + tz - 1) / tz);
+        long long b = (pr.n_sub + kThreads - 1) / kThreads;
+        tiles = (int)std::max(a, b) * 2 + 8;
+    }
+    ev->max_partial_blocks = tiles;
+    if ((st = dev_alloc(&ev->d_cpc, pr.n_cp)) || (st = dev_alloc(&ev->d_cp_in, pr.n_cp * dim)) ||
+        (st = dev_alloc(&ev->d_cpw_in, pr.n_cp)) || (st = dev_alloc(&ev->d_deriv, pr.n_cp * dim)) ||
+        (st = dev_alloc(&ev->d_sg, pr.n_sub)) || (st = dev_alloc(&ev->d_sb, pr.n_sub)) ||
+        (st = dev_alloc(&ev->d_sv, pr.n_sub)) || (st = dev_alloc(&ev->d_partials, (size_t)tiles * kPartialsK)) ||
+        (st = dev_alloc(&ev->d_reg_partials, (size_t)((pr.n_cp + kThreads - 1) / kThreads) * 4 + 4)) ||
+        (st = dev_alloc(&ev->d_hreg_partials, (size_t)((pr.n_cp + kThreads - 1) / kThreads) * 4 + 4)) ||
+        (st = dev_alloc(&ev->d_dot_partials, 64)) || (st = dev_alloc(&ev->d_sc, 1)) ||
+        (st = dev_alloc(&ev->d_offsets, pr.sub > 1 ? pr.n_sub * dim : 1)))
+        return cleanup(st);
+    pr.fixed_data = ev->d_fixed;
+    pr.coeffs = ev->d_coeffs;
+    pr.weight_data = ev->d_weight;
+    pr.df_data = nullptr;
+    pr.has_df = 0;
+    pr.cpc = ev->d_cpc;
+    pr.cp_state = ev->d_cp_state;
+    pr.cp_window = ev->d_cp_window;
+    pr.reg_window = ev->d_reg_window;
+    pr.offsets = ev->d_offsets;
+    if (hipMemset(ev->d_offsets, 0, sizeof(int) * (pr.sub > 1 ? pr.n_sub * dim : 1)) != hipSuccess)
+        return cleanup(fail(SKM_ERROR_HIP, "hipMemset failed"));
+    *out = ev;
+    return SKM_SUCCESS;
+}
+
+SKM_API void skm_destroy(skm_evaluator *ev)
+{
+    if (!ev) return;
+    if (ev->stream) (void)hipStreamSynchronize(ev->stream);
+    delete ev;
+}
+
+// SetRegularizerRKHS / RD / PG (GpuEvaluator.cu:786-831).  The scaling of RD
+// is stored but, as in the reference, not used by the RD term (:813).
+SKM_API skm_status skm_set_regularizer_rkhs(skm_evaluator *ev, double weight)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    ev->reg_w[0] = weight;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_set_regularizer_rd(skm_evaluator *ev, double weight, double scaling)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    if (!(scaling > 0.0)) return fail(SKM_ERROR_INVALID_ARGUMENT, "scaling of regularizer must be strictly positive");
+    ev->reg_w[1] = weight;
+    ev->rd_scaling = scaling;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_set_regularizer_pg(skm_evaluator *ev, double weight, double scaling)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    if (!(scaling > 0.0)) return fail(SKM_ERROR_INVALID_ARGUMENT, "scaling of regularizer must be strictly positive");
+    ev->reg_w[2] = weight;
+    ev->pg_scaling = scaling;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_set_displacement_field(skm_evaluator *ev, const skm_grid *grid, const float *df_aos)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    SKM_HIP(hipStreamSynchronize(ev->stream));
+    if (ev->d_df) {
+        SKM_HIP(hipFree(ev->d_df));
+        ev->d_df = nullptr;
+    }
+    ev->pr.has_df = 0;
+    ev->pr.df_data = nullptr;
+    if (!grid || !df_aos) return SKM_SUCCESS;
+    SKM_TRY(validate_grid(*grid, ev->dim, "displacement field", 1));
+    size_t n = (size_t)grid_count(*grid, ev->dim) * ev->dim;
+    SKM_TRY(dev_upload(&ev->d_df, df_aos, n));
+    ev->pr.df = to_grid(*grid, ev->dim);
+    ev->pr.df_data = ev->d_df;
+    ev->pr.has_df = 1;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_set_stream(skm_evaluator *ev, void *hip_stream)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    SKM_HIP(hipStreamSynchronize(ev->stream));
+    if (ev->own_stream) SKM_HIP(hipStreamDestroy(ev->stream));
+    ev->own_stream = false;
+    ev->stream = (hipStream_t)hip_stream;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_synchronize(skm_evaluator *ev)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    SKM_HIP(hipStreamSynchronize(ev->stream));
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_get_value(skm_evaluator *ev, int32_t measure, const float *cp_aos, const float *cpw,
+                                 int32_t do_resampling, skm_result *out)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    SKM_TRY(upload_cp(ev, cp_aos, cpw));
+    SKM_TRY(run_eval(ev, measure, do_resampling, nullptr));
+    return read_result(ev, out);
+}
+
+SKM_API skm_status skm_get_value_and_derivative(skm_evaluator *ev, int32_t measure, const float *cp_aos,
+                                                const float *cpw, float *derivative_soa, int32_t do_resampling,
+                                                skm_result *out)
+{
+    if (!ev || !derivative_soa) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    SKM_TRY(upload_cp(ev, cp_aos, cpw));
+    SKM_TRY(run_eval(ev, measure, do_resampling, ev->d_deriv));
+    SKM_HIP(hipMemcpyAsync(derivative_soa, ev->d_deriv, sizeof(float) * ev->pr.n_cp * ev->dim,
+                           hipMemcpyDeviceToHost, ev->stream));
+    return read_result(ev, out);
+}
+
+SKM_API skm_status skm_get_value_and_derivative_device(skm_evaluator *ev, int32_t measure, const float *d_cp_aos,
+                                                       const float *d_cpw, float *d_derivative_soa,
+                                                       int32_t do_resampling, skm_result *d_result)
+{
+    if (!ev || !d_cp_aos || !d_cpw || !d_derivative_soa) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    launch_pack_cp(ev->pr, d_cp_aos, d_cpw, 0, ev->d_cpc, ev->stream);
+    SKM_TRY(run_eval(ev, measure, do_resampling, d_derivative_soa));
+    if (d_result) launch_result(ev->d_sc, d_result, ev->stream);
+    SKM_HIP(hipGetLastError());
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_evaluate_displacement_field(skm_evaluator *ev, const float *cp_aos, const float *cpw,
+                                                   float *dst_aos)
+{
+    if (!ev || !dst_aos) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    SKM_TRY(upload_cp(ev, cp_aos, cpw));
+    if (!ev->d_dst) SKM_TRY(dev_alloc(&ev->d_dst, ev->pr.n_vox * ev->dim));
+    launch_displacement(ev->pr, ev->d_dst, ev->stream);
+    SKM_HIP(hipGetLastError());
+    SKM_HIP(hipMemcpyAsync(dst_aos, ev->d_dst, sizeof(float) * ev->pr.n_vox * ev->dim, hipMemcpyDeviceToHost,
+                           ev->stream));
+    SKM_HIP(hipStreamSynchronize(ev->stream));
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_evaluate_displacement_field_device(skm_evaluator *ev, const float *d_cp_aos,
+                                                          const float *d_cpw, float *d_dst_aos)
+{
+    if (!ev || !d_cp_aos || !d_cpw || !d_dst_aos) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    launch_pack_cp(ev->pr, d_cp_aos, d_cpw, 0, ev->d_cpc, ev->stream);
+    launch_displacement(ev->pr, d_dst_aos, ev->stream);
+    SKM_HIP(hipGetLastError());
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_get_cube_offsets(skm_evaluator *ev, int32_t *out, int64_t *n_cells)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    if (n_cells) *n_cells = ev->pr.n_sub;
+    if (!out) return SKM_SUCCESS;
+    if (ev->pr.sub <= 1) {
+        std::memset(out, 0, sizeof(int32_t) * ev->pr.n_sub * ev->dim);
+        return SKM_SUCCESS;
+    }
+    SKM_HIP(hipMemcpyAsync(out, ev->d_offsets, sizeof(int32_t) * ev->pr.n_sub * ev->dim, hipMemcpyDeviceToHost,
+                           ev->stream));
+    SKM_HIP(hipStreamSynchronize(ev->stream));
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_control_point_grid(int32_t dim, const skm_grid *image, const double *cp_spacing,
+                                          skm_grid *cp_out)
+{
+    if (!image || !cp_spacing || !cp_out || (dim != 2 && dim != 3))
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "bad argument");
+    skm_grid g;
+    std::memset(&g, 0, sizeof(g));
+    for (int d = 0; d < 3; ++d) { g.size[d] = 1; g.spacing[d] = 1.0; }
+    for (int d = 0; d < dim; ++d) {
+        if (!(cp_spacing[d] > 0)) return fail(SKM_ERROR_INVALID_ARGUMENT, "control-point spacing must be > 0");
+        double lower = image->origin[d];
+        double upper = image->origin[d] + image->spacing[d] * (image->size[d] - 1);
+        g.spacing[d] = cp_spacing[d];
+        g.size[d] = (int)std::ceil(std::fabs((upper - lower) / cp_spacing[d]));
+        double cp_upper = image->origin[d] + cp_spacing[d] * (g.size[d] - 1);
+        g.origin[d] = image->origin[d] + std::fabs(upper - cp_upper) / 2.0;
+    }
+    *cp_out = g;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_host_regularizers(int32_t dim, int64_t n_cp, const double *params_soa, double reg_l1,
+                                         double reg_l21, double reg_l2, int32_t use_negative_gradient,
+                                         double *derivative_soa, double *reg_value, double *l1, double *l21,
+                                         double *l2)
+{
+    if (!params_soa || (dim != 2 && dim != 3)) return fail(SKM_ERROR_INVALID_ARGUMENT, "bad argument");
+    double v1 = 0, v21 = 0, v2 = 0;
+    const long long np = n_cp * dim;
+    if (reg_l1 > 0) {
+        for (long long p = 0; p < np; ++p) v1 += std::fabs(params_soa[p]);
+        if (derivative_soa)
+            for (long long p = 0; p < np; ++p) {
+                double par = params_soa[p];
+                double &g = derivative_soa[p];
+                if (std::fabs(par) > 0) g += reg_l1 * (par < 0 ? -1.0 : 1.0);
+                else if (g < -reg_l1) g += reg_l1;
+                else if (g > reg_l1) g -= reg_l1;
+                else if (-reg_l1 <= g && g <= reg_l1) g = 0.0;
+                else return fail(SKM_ERROR_INVALID_ARGUMENT, "Error in calculating subgradient");
+            }
+    }
+    if (reg_l2 > 0)
+        for (long long j = 0; j < n_cp; ++j) {
+            double n2 = 0;
+            for (int d = 0; d < dim; ++d) n2 += params_soa[j + d * n_cp] * params_soa[j + d * n_cp];
+            v2 += n2;
+            if (derivative_soa)
+                for (int d = 0; d < dim; ++d) derivative_soa[j + d * n_cp] += reg_l2 * 2.0 * params_soa[j + d * n_cp];
+        }
+    if (reg_l21 > 0)
+        for (long long j = 0; j < n_cp; ++j) {
+            double n2 = 0;
+            for (int d = 0; d < dim; ++d) n2 += params_soa[j + d * n_cp] * params_soa[j + d * n_cp];
+            double nrm = std::sqrt(n2);
+            v21 += nrm;
+            if (!derivative_soa) continue;
+            for (int d = 0; d < dim; ++d) {
+                double &g = derivative_soa[j + d * n_cp];
+                if (nrm > 0) g += reg_l21 * params_soa[j + d * n_cp] / nrm;
+                else if (g < -reg_l21) g += reg_l21;
+                else if (g > reg_l21) g -= reg_l21;
+                else if (-reg_l21 <= g && g <= reg_l21) g = 0.0;
+                else return fail(SKM_ERROR_INVALID_ARGUMENT, "Error in calculating subgradient");
+            }
+        }
+    if (use_negative_gradient && derivative_soa)
+        for (long long p = 0; p < np; ++p) derivative_soa[p] = -derivative_soa[p];
+    double reg = 0;
+    if (reg_l1 > 0) reg += reg_l1 * v1;
+    if (reg_l21 > 0) reg += reg_l21 * v21;
+    if (reg_l2 > 0) reg += reg_l2 * v2;
+    if (reg_value) *reg_value = reg;
+    if (l1) *l1 = v1;
+    if (l21) *l21 = v21;
+    if (l2) *l2 = v2;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *metric,
+                                const skm_optimizer_options *opt, const float *cpw, float *params_soa,
+                                skm_iteration_record *records)
+{
+    if (!ev || !metric || !opt || !cpw || !params_soa) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    if (opt->iterations < 0) return fail(SKM_ERROR_INVALID_ARGUMENT, "iterations must be >= 0");
+    if (opt->kind < 0 || opt->kind > 2) return fail(SKM_ERROR_INVALID_ARGUMENT, "bad optimizer kind");
+    if (metric->reg_l1 < 0 || metric->reg_l21 < 0 || metric->reg_l2 < 0)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "Regularization weights must be positive.");
+    Problem &pr = ev->pr;
+    hipStream_t s = ev->stream;
+    const size_t np = (size_t)pr.n_cp * pr.dim;
+    if (!ev->d_params) SKM_TRY(dev_alloc(&ev->d_params, np));
+    if (!ev->d_prev_g) SKM_TRY(dev_alloc(&ev->d_prev_g, np));
+    if (!ev->d_opt) SKM_TRY(dev_alloc(&ev->d_opt, 1));
+    if (ev->records_cap < opt->iterations) {
+        if (ev->d_records) SKM_HIP(hipFree(ev->d_records));
+        ev->d_records = nullptr;
+        SKM_TRY(dev_alloc(&ev->d_records, (size_t)opt->iterations));
+        ev->records_cap = opt->iterations;
+    }
+    SKM_HIP(hipMemcpyAsync(ev->d_params, params_soa, sizeof(float) * np, hipMemcpyHostToDevice, s));
+    SKM_HIP(hipMemcpyAsync(ev->d_cpw_in, cpw, sizeof(float) * pr.n_cp, hipMemcpyHostToDevice, s));
+    SKM_HIP(hipMemsetAsync(ev->d_prev_g, 0, sizeof(float) * np, s));
+    SKM_HIP(hipMemsetAsync(ev->d_opt, 0, sizeof(OptState), s));
+    for (int it = 0; it < opt->iterations; ++it) {
+        launch_soa_to_cpc(pr, ev->d_params, ev->d_cpw_in, ev->d_cpc, s);
+        SKM_TRY(run_eval(ev, metric->measure, metric->do_resampling, ev->d_deriv));
+        launch_host_regs_device(pr, ev->d_params, metric->reg_l1, metric->reg_l21, metric->reg_l2,
+                                metric->use_negative_gradient, ev->d_deriv, ev->d_hreg_partials, s);
+        launch_opt_step(pr, opt->kind, opt->a, opt->A, opt->alpha, opt->fmax, opt->fmin, opt->omega,
+                        opt->orthant_projection, ev->d_params, ev->d_deriv, ev->d_prev_g, ev->d_dot_partials,
+                        ev->d_opt, ev->d_sc, ev->d_hreg_partials, metric->reg_l1, metric->reg_l21, metric->reg_l2,
+                        ev->reg_w, ev->d_records, s);
+    }
+    SKM_HIP(hipGetLastError());
+    SKM_HIP(hipMemcpyAsync(params_soa, ev->d_params, sizeof(float) * np, hipMemcpyDeviceToHost, s));
+    if (records && opt->iterations > 0)
+        SKM_HIP(hipMemcpyAsync(records, ev->d_records, sizeof(skm_iteration_record) * opt->iterations,
+                               hipMemcpyDeviceToHost, s));
+    SKM_HIP(hipStreamSynchronize(s));
+    return SKM_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,335 @@
+// skm_device.hpp -- device-side math of the SKM path for gfx950.
+//
+// Geometry, Wendland kernels in the three modes, cubic B-spline sampling and
+// PCG32 jump-ahead.  All per-pair arithmetic is FP32; sums that leave a
+// kernel are FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace skm {
+
+constexpr int kThreads = 256;   // 4 wave64 per workgroup
+
+// Image geometry in FP32 (origin + spacing * index), x fastest.
+struct Grid {
+    int n[3];
+    float o[3];
+    float h[3];
+    float ih[3];   // 1 / spacing
+};
+
+struct KernelConst {
+    int mode;            // 0 stationary, 1 weight image, 2 weight tensor
+    float inv_s4, inv_s0;
+    float smin;          // min(sigma_C4, sigma_C0)
+    float max_weight;
+    float max_eig_cov[9];
+};
+
+// Kernel-side state of one point: sigma (mode 1) or covariance (mode 2),
+// sdet = |sigma|^(1/2) or det(cov)^(1/2) (the square of the F factor's
+// per-point part, Kernel.h:107-109, 121-123), and the region support.
+struct PState {
+    float sig;
+    float sdet;
+    float cov[9];
+    float support;
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+template <int DIM>
+__device__ __forceinline__ void to_global(const Grid &g, const float *idx, float *p)
+{
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) p[d] = g.o[d] + g.h[d] * idx[d];
+}
+
+template <int DIM>
+__device__ __forceinline__ void to_local(const Grid &g, const float *p, float *idx)
+{
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) idx[d] = g.ih[d] * (p[d] - g.o[d]);
+}
+
+// Nearest-neighbour index, toNearest = int(idx + 0.5) (ImageBase.h:64-67).
+template <int DIM>
+__device__ __forceinline__ bool nearest(const Grid &g, const float *p, long long &flat)
+{
+    float idx[3];
+    to_local<DIM>(g, p, idx);
+    long long f = 0, stride = 1;
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        int i = (int)(idx[d] + 0.5f);
+        in = in && i >= 0 && i < g.n[d];
+        f += stride * (long long)i;
+        stride *= g.n[d];
+    }
+    flat = f;
+    return in;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Wendland C4 with t = r / sigma_C4 (Kernel.h:72-77): (1-t)_+^6 (3 + 18t + 35t^2) / 3.
+__device__ __forceinline__ float wendland_c4(float t)
+{
+    float f = fmaxf(0.0f, 1.0f - t);
+    float f2 = f * f;
+    return f2 * f2 * f2 * fmaf(fmaf(35.0f, t, 18.0f), t, 3.0f) * (1.0f / 3.0f);
+}
+
+// Wendland C0 (Kernel.h:48-53): (1-t)_+^2.
+__device__ __forceinline__ float wendland_c0(float t)
+{
+    float f = fmaxf(0.0f, 1.0f - t);
+    return f * f;
+}
+
+template <int DIM>
+__device__ __forceinline__ float det_m(const float *m)
+{
+    if (DIM == 2) return m[0] * m[3] - m[2] * m[1];
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) + m[1] * (m[5] * m[6] - m[3] * m[8]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+// Largest eigenvalue, trigonometric closed form for 3x3 (Mat.h:259-281).
+template <int DIM>
+__device__ __forceinline__ float max_eigenvalue(const float *m)
+{
+    if (DIM == 2) {
+        float tr = m[0] + m[3];
+        float det = det_m<2>(m);
+        return 0.5f * (tr + sqrtf(fmaxf(tr * tr - 4.0f * det, 0.0f)));
+    }
+    float tri = m[1] * m[1] + m[2] * m[2] + m[5] * m[5];
+    if (tri == 0.0f) return fmaxf(m[0], fmaxf(m[4], m[8]));
+    float q = (1.0f / 3.0f) * (m[0] + m[4] + m[8]);
+    float a = m[0] - q, b = m[4] - q, c = m[8] - q;
+    float psq = 2.0f * tri + a * a + b * b + c * c;
+    float p = sqrtf(psq * (1.0f / 6.0f));
+    float s[9] = {a, m[1], m[2], m[3], b, m[5], m[6], m[7], c};
+    float r = det_m<3>(s) * (0.5f / (p * p * p));
+    float phi = (1.0f / 3.0f) * acosf(fminf(1.0f, fmaxf(-1.0f, r)));
+    return q + 2.0f * p * cosf(phi);
+}
+
+// State of a point from the weight image / tensor field (getSigmaAtPoint /
+// getCovarianceAtPoint, Kernel.h:148-156, NN with 0 / identity outside) and
+// its region support (Kernel.h:158-171).
+template <int DIM, int MODE>
+__device__ __forceinline__ void point_state(const KernelConst &kc, const Grid &wg, const float *wdata,
+                                            const float *p, PState &st)
+{
+    if (MODE == 0) {
+        st.support = kc.smin;
+        return;
+    }
+    long long f;
+    bool in = nearest<DIM>(wg, p, f);
+    if (MODE == 1) {
+        float s = in ? wdata[f] : 0.0f;
+        st.sig = s;
+        st.sdet = sqrtf(fabsf(s));
+        st.support = kc.smin / sqrtf(2.0f / (kc.max_weight + s));
+    } else {
+        constexpr int NN = DIM * DIM;
+#pragma unroll
+        for (int i = 0; i < NN; ++i) st.cov[i] = in ? wdata[f * NN + i] : ((i % (DIM + 1)) == 0 ? 1.0f : 0.0f);
+        st.sdet = sqrtf(det_m<DIM>(st.cov));
+        float sum[9];
+#pragma unroll
+        for (int i = 0; i < NN; ++i) sum[i] = kc.max_eig_cov[i] + st.cov[i];
+        st.support = kc.smin / sqrtf(2.0f / max_eigenvalue<DIM>(sum));
+    }
+}
+
+// k(p, q) for v = p - q with the states of both points (Kernel.h:96-131);
+// the weighted modes carry F twice, F^2 = sdet_p * sdet_q / |avg|.
+template <int DIM, int MODE>
+__device__ __forceinline__ float kernel_pair(const KernelConst &kc, const float *v, const PState &a, const PState &b)
+{
+    float r2 = v[0] * v[0] + v[1] * v[1];
+    if (DIM == 3) r2 = fmaf(v[2], v[2], r2);
+    if (MODE == 0) {
+        float r = sqrtf(r2);
+        return wendland_c4(r * kc.inv_s4) * wendland_c0(r * kc.inv_s0);
+    } else if (MODE == 1) {
+        float sum = a.sig + b.sig;
+        float F2 = a.sdet * b.sdet / fabsf(0.5f * sum);
+        float rs = sqrtf(2.0f / sum) * sqrtf(r2);
+        return F2 * wendland_c4(rs * kc.inv_s4) * wendland_c0(rs * kc.inv_s0);
+    } else {
+        float m[9];
+        constexpr int NN = DIM * DIM;
+#pragma unroll
+        for (int i = 0; i < NN; ++i) m[i] = 0.5f * (a.cov[i] + b.cov[i]);
+        float quad, det;
+        if (DIM == 2) {
+            det = m[0] * m[3] - m[2] * m[1];
+            quad = v[0] * v[0] * m[3] - v[0] * v[1] * (m[1] + m[2]) + v[1] * v[1] * m[0];
+        } else {
+            float a0 = m[4] * m[8] - m[5] * m[7], a1 = m[2] * m[7] - m[1] * m[8], a2 = m[1] * m[5] - m[2] * m[4];
+            float a3 = m[5] * m[6] - m[3] * m[8], a4 = m[0] * m[8] - m[2] * m[6], a5 = m[2] * m[3] - m[0] * m[5];
+            float a6 = m[3] * m[7] - m[4] * m[6], a7 = m[1] * m[6] - m[0] * m[7], a8 = m[0] * m[4] - m[1] * m[3];
+            det = m[0] * a0 + m[1] * a3 + m[2] * a6;
+            float w0 = v[0] * a0 + v[1] * a3 + v[2] * a6;
+            float w1 = v[0] * a1 + v[1] * a4 + v[2] * a7;
+            float w2 = v[0] * a2 + v[1] * a5 + v[2] * a8;
+            quad = w0 * v[0] + w1 * v[1] + w2 * v[2];
+        }
+        float F2 = a.sdet * b.sdet / det;
+        float rs = sqrtf(quad / det);
+        return F2 * wendland_c4(rs * kc.inv_s4) * wendland_c0(rs * kc.inv_s0);
+    }
+}
+
+// ---------------------------------------------------------------- B-spline
+
+// Value and physical gradient of the cubic B-spline at p, coefficient grid
+// padded by 3 and shifted by one; false outside [-0.5, n-0.5)
+// (BsplineImage.h:247-296).  `g` is the raw image geometry.
+template <int DIM>
+__device__ __forceinline__ bool bspline_sample(const float *__restrict__ coef, const Grid &g, const float *p,
+                                               float &val, float *grad)
+{
+    float idx[3];
+    to_local<DIM>(g, p, idx);
+    bool in = true;
+    // written so that a NaN position is outside (the reference's negated test
+    // would let it through and index with an undefined integer)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) in = in && (idx[d] >= -0.5f && idx[d] + 3.5f < (float)(g.n[d] + 3));
+    if (!in) return false;
+    int base[3] = {0, 0, 0};
+    float w[3][4], dw[3][4];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        base[d] = (int)idx[d];
+        float t = idx[d] - (float)base[d];
+        float t2 = t * t;
+        w[d][3] = (1.0f / 6.0f) * t2 * t;
+        w[d][0] = (1.0f / 6.0f) + 0.5f * t * (t - 1.0f) - w[d][3];
+        w[d][2] = t + w[d][0] - 2.0f * w[d][3];
+        w[d][1] = 1.0f - w[d][0] - w[d][2] - w[d][3];
+        dw[d][3] = 0.5f * t2;
+        dw[d][0] = t - 0.5f - dw[d][3];
+        dw[d][2] = 1.0f + dw[d][0] - 2.0f * dw[d][3];
+        dw[d][1] = -dw[d][0] - dw[d][2] - dw[d][3];
+    }
+    const long long sx = g.n[0] + 3;
+    if (DIM == 2) {
+        float v = 0.0f, gx = 0.0f, gy = 0.0f;
+        const float *row = coef + (long long)base[0] + (long long)base[1] * sx;
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const float *c = row + y * sx;
+            float r = 0.0f, rd = 0.0f;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                r = fmaf(w[0][x], c[x], r);
+                rd = fmaf(dw[0][x], c[x], rd);
+            }
+            v = fmaf(w[1][y], r, v);
+            gx = fmaf(w[1][y], rd, gx);
+            gy = fmaf(dw[1][y], r, gy);
+        }
+        val = v;
+        grad[0] = gx * g.ih[0];
+        grad[1] = gy * g.ih[1];
+    } else {
+        const long long sy = g.n[1] + 3;
+        float v = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        const float *blk = coef + (long long)base[0] + (long long)base[1] * sx + (long long)base[2] * sx * sy;
+#pragma unroll
+        for (int z = 0; z < 4; ++z) {
+            float az = 0.0f, axz = 0.0f, ayz = 0.0f;
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                const float *c = blk + z * sx * sy + y * sx;
+                float r = 0.0f, rd = 0.0f;
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    r = fmaf(w[0][x], c[x], r);
+                    rd = fmaf(dw[0][x], c[x], rd);
+                }
+                az = fmaf(w[1][y], r, az);
+                axz = fmaf(w[1][y], rd, axz);
+                ayz = fmaf(dw[1][y], r, ayz);
+            }
+            v = fmaf(w[2][z], az, v);
+            gx = fmaf(w[2][z], axz, gx);
+            gy = fmaf(w[2][z], ayz, gy);
+            gz = fmaf(dw[2][z], az, gz);
+        }
+        val = v;
+        grad[0] = gx * g.ih[0];
+        grad[1] = gy * g.ih[1];
+        grad[2] = gz * g.ih[2];
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------- PCG32
+
+constexpr uint64_t kPcgMult = 6364136223846793005ULL;
+
+// Output permutation XSH-RR of the PCG32 generator (UniformRng.h:34-41).
+__host__ __device__ __forceinline__ uint32_t pcg32_output(uint64_t old)
+{
+    uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xorshifted >> rot) | (xorshifted << ((uint32_t)(-(int32_t)rot) & 31));
+}
+
+// State after `delta` LCG steps, O(log delta) (O'Neill 2014, "PCG", sec. 4.3.1).
+__host__ __device__ __forceinline__ uint64_t lcg_advance(uint64_t state, uint64_t delta, uint64_t inc)
+{
+    uint64_t acc_mult = 1, acc_plus = 0, cur_mult = kPcgMult, cur_plus = inc;
+    while (delta > 0) {
+        if (delta & 1) {
+            acc_mult *= cur_mult;
+            acc_plus = acc_plus * cur_mult + cur_plus;
+        }
+        cur_plus = (cur_mult + 1) * cur_plus;
+        cur_mult *= cur_mult;
+        delta >>= 1;
+    }
+    return acc_mult * state + acc_plus;
+}
+
+// ---------------------------------------------------------------- reductions
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sum of K doubles in a fixed order; result valid in thread 0.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K * 4 */)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[wave * K + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < kThreads / 64; ++w) s += lds[w * K + k];
+            v[k] = s;
+        }
+    }
+}
+
+}  // namespace skm

@@ -1,0 +1,88 @@
+// skm_internal.hpp -- argument blocks shared by the kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "skm_device.hpp"
+
+namespace skm {
+
+// Per-control-point kernel state (static for an evaluator): sig, sdet, cov[9], support.
+constexpr int kCpStateStride = 12;
+
+// Everything a kernel needs to know about one registration problem.
+struct Problem {
+    int dim;
+    Grid fixed, moving, cp, weight, df;
+    KernelConst kc;
+    int sub, subn;
+    int nsub[3];
+    long long n_sub, n_cp, n_vox;
+    int has_df;
+    int exact;
+    const float *fixed_data;
+    const float *coeffs;       // moving B-spline coefficients, (n+3)^D
+    const float *weight_data;  // weight image / tensor field
+    const float *df_data;      // previous displacement, AoS
+    const float4 *cpc;         // (c, w) per control point
+    const float *cp_state;     // kCpStateStride floats per control point
+    const int *cp_window;      // per cp: lo[3], hi[3] in sample space (derivative gather)
+    const int *reg_window;     // per cp: lo[3], hi[3] in cp space (regularizers)
+    const int *offsets;        // D ints per subsampled cell
+};
+
+// Device-resident scalars of one evaluation (all FP64).
+struct Scalars {
+    double count;
+    double sums[6];        // MSE: sum d^2 | NCC: sf, sm, sff, smm, sfm | LCC: sum cc
+    double measure;
+    double mean_f, mean_m, rho, inv_denom;  // NCC derivative coefficients
+    double coef_scale;     // MSE: 2/N, LCC: 1/N
+    double reg[3];         // RKHS, RD, PG values
+};
+
+enum { kPartialsK = 8 };
+
+// ---- launchers (skm_kernels.hip) ----
+void launch_offsets(const Problem &pr, int *offsets, uint64_t state, uint64_t inc, hipStream_t s);
+void launch_pack_cp(const Problem &pr, const float *cp, const float *cpw, int soa, float4 *cpc, hipStream_t s);
+void launch_value(const Problem &pr, int measure, float4 *sg, float *sb, double *partials, int *blocks_out,
+                  hipStream_t s);
+void launch_reduce(const Problem &pr, int measure, const double *partials, int blocks, Scalars *sc,
+                   hipStream_t s);
+void launch_prepare(const Problem &pr, int measure, const float4 *sg, const float *sb, const Scalars *sc,
+                    float4 *sv, hipStream_t s);
+void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s);
+void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, int with_derivative,
+                         float *deriv_soa, double *partials, Scalars *sc, hipStream_t s);
+void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
+void launch_result(const Scalars *sc, void *result, hipStream_t s);
+
+// LCC (skm_lcc.hip)
+void launch_dense_warp(const Problem &pr, int quirk3d, float4 *dense_g, hipStream_t s);
+void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre_coef, double *partials,
+                        int *blocks_out, hipStream_t s);
+void launch_lcc_gather_reference(const Problem &pr, const float4 *dense_g, const float4 *centre_coef,
+                                 const Scalars *sc, float *deriv_soa, hipStream_t s);
+void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
+                        float4 *gamma, hipStream_t s);
+
+// optimizer (skm_kernels.hip)
+struct OptState {
+    double time;           // adaptive time
+    double rate;
+    double prev_dot;
+    long long iteration;
+};
+void launch_host_regs_device(const Problem &pr, const float *params_soa, double l1, double l21, double l2,
+                             int negate, float *deriv_soa, double *partials, hipStream_t s);
+void launch_opt_step(const Problem &pr, int kind, double a, double A, double alpha, double fmax, double fmin,
+                     double omega, int orthant, float *params_soa, const float *grad_soa, float *prev_grad,
+                     double *partials, OptState *st, const Scalars *sc, const double *reg_partials,
+                     double l1w, double l21w, double l2w, const double *gpu_reg_weights, void *record,
+                     hipStream_t s);
+void launch_soa_to_cpc(const Problem &pr, const float *params_soa, const float *cpw, float4 *cpc,
+                       hipStream_t s);
+
+}  // namespace skm

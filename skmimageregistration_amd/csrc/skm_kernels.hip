@@ -1,0 +1,794 @@
+// skm_kernels.hip -- gfx950 kernels of the SKM kernel-sum / MSE / NCC /
+// regularizer path and the device-resident optimizer step.
+//
+// Reference counterparts (CUDA, one thread per voxel or per control point on
+// 4x4x4 / 16x16 blocks): include/cuda/GpuEvaluatorMSE.cu, GpuEvaluatorNCC.cu,
+// GpuEvaluatorRegularizers.cu, GpuEvaluatorDisplacementField.cu and the host
+// loops of GpuEvaluator.cu.  Here the value pass runs on 256-lane workgroups
+// whose four wave64 each cover a 4x4x4 (3D) or 16x4 (2D) sample brick, the
+// derivative is a per-control-point gather by a whole workgroup over the
+// sample window with rows pruned to the kernel ball, and every reduction is a
+// fixed-order FP64 tree (per-block partials, then one finishing block).
+#include "skm_internal.hpp"
+
+#include "skm.h"
+
+namespace skm {
+
+// ------------------------------------------------------------ tiling helpers
+
+template <int DIM> struct Tile;
+template <> struct Tile<2> { static constexpr int X = 16, Y = 16, Z = 1; };
+template <> struct Tile<3> { static constexpr int X = 8, Y = 8, Z = 4; };
+
+// Thread -> offset inside the block tile; each wave64 gets a compact brick.
+template <int DIM>
+__device__ __forceinline__ void tile_coords(int t, int *l)
+{
+    const int lane = t & 63, wave = t >> 6;
+    if (DIM == 3) {
+        l[0] = (lane & 3) + (wave & 1) * 4;
+        l[1] = ((lane >> 2) & 3) + (wave >> 1) * 4;
+        l[2] = lane >> 4;
+    } else {
+        l[0] = lane & 15;
+        l[1] = (lane >> 4) + wave * 4;
+        l[2] = 0;
+    }
+}
+
+__device__ __forceinline__ long long block_linear()
+{
+    return blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+}
+
+template <int DIM>
+__device__ __forceinline__ void sample_position(const Problem &pr, const int *m, long long s, int *pos)
+{
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) pos[d] = m[d];
+    if (pr.sub > 1) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) pos[d] = m[d] * pr.sub + pr.offsets[s * DIM + d];
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ float fixed_value(const Problem &pr, const int *pos)
+{
+    // A jitter offset equal to `subsample` (nextF() == 1.0) can step past the
+    // last voxel; the index is clamped instead of reading out of bounds.
+    long long f = 0, stride = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        int i = clampi(pos[d], 0, pr.fixed.n[d] - 1);
+        f += stride * i;
+        stride *= pr.fixed.n[d];
+    }
+    return pr.fixed_data[f];
+}
+
+template <int DIM>
+__device__ __forceinline__ void df_lookup(const Problem &pr, const float *x, float *u)
+{
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) u[d] = 0.0f;
+    if (!pr.has_df) return;
+    long long f;
+    if (nearest<DIM>(pr.df, x, f)) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) u[d] = pr.df_data[f * DIM + d];
+    }
+}
+
+template <int DIM, int MODE>
+__device__ __forceinline__ void cp_state_load(const Problem &pr, long long j, PState &st)
+{
+    if (MODE == 0) return;
+    const float *c = pr.cp_state + j * kCpStateStride;
+    st.sig = c[0];
+    st.sdet = c[1];
+    if (MODE == 2) {
+#pragma unroll
+        for (int i = 0; i < DIM * DIM; ++i) st.cov[i] = c[2 + i];
+    }
+}
+
+// Kernel sum u(x) = sum_j w_j k(x, p_j) c_j over the control points in the
+// support ball of x (reference window GpuEvaluatorMSE.cu:61-80; rows whose
+// distance to x exceeds the support only hold k == 0 and are skipped).
+template <int DIM, int MODE>
+__device__ __forceinline__ void kernel_sum(const Problem &pr, const float *x, const PState &st, float *u)
+{
+    float ci[3];
+    to_local<DIM>(pr.cp, x, ci);
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        float r = st.support * pr.cp.ih[d];
+        lo[d] = clampi((int)(ci[d] - r + 1.0f), 0, pr.cp.n[d] - 1);
+        hi[d] = clampi((int)(ci[d] + r), 0, pr.cp.n[d] - 1);
+    }
+    const float s2 = st.support * st.support;
+    const int z0 = DIM == 3 ? lo[2] : 0, z1 = DIM == 3 ? hi[2] : 0;
+    for (int cz = z0; cz <= z1; ++cz) {
+        float dz = DIM == 3 ? x[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
+        for (int cy = lo[1]; cy <= hi[1]; ++cy) {
+            float dy = x[1] - (pr.cp.o[1] + pr.cp.h[1] * cy);
+            float rem = s2 - dy * dy - dz * dz;
+            if (rem <= 0.0f) continue;
+            float rx = sqrtf(rem) * pr.cp.ih[0];
+            int xa = max(lo[0], (int)floorf(ci[0] - rx));
+            int xb = min(hi[0], (int)ceilf(ci[0] + rx));
+            long long row = (long long)pr.cp.n[0] * (cy + (long long)pr.cp.n[1] * cz);
+            for (int cx = xa; cx <= xb; ++cx) {
+                float v[3] = {x[0] - (pr.cp.o[0] + pr.cp.h[0] * cx), dy, dz};
+                PState cst;
+                cp_state_load<DIM, MODE>(pr, row + cx, cst);
+                float k = kernel_pair<DIM, MODE>(pr.kc, v, st, cst);
+                float4 c = pr.cpc[row + cx];
+                float wk = c.w * k;
+                u[0] = fmaf(wk, c.x, u[0]);
+                u[1] = fmaf(wk, c.y, u[1]);
+                if (DIM == 3) u[2] = fmaf(wk, c.z, u[2]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ value pass
+
+// MSE / NCC value pass (evaluateValueMSE, GpuEvaluatorMSE.cu:29-95, 174-237;
+// evaluateValueNCC, GpuEvaluatorNCC.cu:37-109, 194-263).  Per sample it stores
+// (gradM, a) and b with a = M - F (MSE) or a = F, b = M (NCC), zeros outside,
+// and adds its share of the FP64 sums to the block partials.
+template <int DIM, int MODE, int MEASURE>
+__global__ __launch_bounds__(kThreads) void k_value(Problem pr, float4 *__restrict__ sg, float *__restrict__ sb,
+                                                    double *__restrict__ partials)
+{
+    __shared__ double lds[kPartialsK * 4];
+    int l[3];
+    tile_coords<DIM>(threadIdx.x, l);
+    int m[3] = {(int)blockIdx.x * Tile<DIM>::X + l[0], (int)blockIdx.y * Tile<DIM>::Y + l[1],
+                DIM == 3 ? (int)blockIdx.z * Tile<DIM>::Z + l[2] : 0};
+    bool valid = m[0] < pr.nsub[0] && m[1] < pr.nsub[1] && (DIM == 2 || m[2] < pr.nsub[2]);
+    double acc[kPartialsK];
+#pragma unroll
+    for (int k = 0; k < kPartialsK; ++k) acc[k] = 0.0;
+    if (valid) {
+        long long s = m[0] + (long long)pr.nsub[0] * (m[1] + (long long)pr.nsub[1] * m[2]);
+        int pos[3] = {0, 0, 0};
+        sample_position<DIM>(pr, m, s, pos);
+        float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, x[3];
+        to_global<DIM>(pr.fixed, fp, x);
+        PState st;
+        point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+        float u[3];
+        df_lookup<DIM>(pr, x, u);
+        kernel_sum<DIM, MODE>(pr, x, st, u);
+        float y[3];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) y[d] = x[d] + u[d];
+        float val = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
+        bool in = bspline_sample<DIM>(pr.coeffs, pr.moving, y, val, g);
+        float a = 0.0f, b = 0.0f;
+        if (in) {
+            float f = fixed_value<DIM>(pr, pos);
+            if (MEASURE == SKM_MEASURE_MSE) a = val - f;
+            else { a = f; b = val; }
+        } else {
+            g[0] = g[1] = g[2] = 0.0f;
+        }
+        sg[s] = make_float4(g[0], g[1], DIM == 3 ? g[2] : 0.0f, a);
+        if (MEASURE == SKM_MEASURE_NCC) sb[s] = b;
+        double da = a, db = b;
+        if (MEASURE == SKM_MEASURE_MSE) {
+            acc[0] = da * da;
+        } else {
+            acc[0] = da; acc[1] = db; acc[2] = da * da; acc[3] = db * db; acc[4] = da * db;
+        }
+        acc[6] = in ? 1.0 : 0.0;
+    }
+    block_sum<kPartialsK>(acc, lds);
+    if (threadIdx.x == 0) {
+        long long b = block_linear();
+#pragma unroll
+        for (int k = 0; k < kPartialsK; ++k) partials[b * kPartialsK + k] = acc[k];
+    }
+}
+
+// Host reduction of GpuEvaluator.cu:192-200 (MSE), 229-262 / 501-534 (NCC) and
+// 287-297 (LCC), done by one block in a fixed order.
+__global__ __launch_bounds__(kThreads) void k_reduce(int measure, const double *__restrict__ partials, int blocks,
+                                                     Scalars *sc)
+{
+    __shared__ double lds[kPartialsK * 4];
+    double acc[kPartialsK];
+#pragma unroll
+    for (int k = 0; k < kPartialsK; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += kThreads)
+#pragma unroll
+        for (int k = 0; k < kPartialsK; ++k) acc[k] += partials[(long long)b * kPartialsK + k];
+    block_sum<kPartialsK>(acc, lds);
+    if (threadIdx.x != 0) return;
+    double n = acc[6];
+    sc->count = n;
+    for (int k = 0; k < 6; ++k) sc->sums[k] = acc[k];
+    sc->mean_f = sc->mean_m = sc->rho = sc->inv_denom = 0.0;
+    if (measure == SKM_MEASURE_MSE) {
+        sc->measure = acc[0] / n;
+        sc->coef_scale = 2.0 / n;
+    } else if (measure == SKM_MEASURE_NCC) {
+        double sf = acc[0], sm = acc[1], sff = acc[2], smm = acc[3], sfm = acc[4];
+        sff -= sf * sf / n;
+        smm -= sm * sm / n;
+        sfm -= sf * sm / n;
+        double denom = -1.0 * sqrt(sff * smm);
+        sc->measure = denom != 0.0 ? sfm / denom : 0.0;
+        sc->mean_f = sf / n;
+        sc->mean_m = sm / n;
+        if (denom != 0.0) {
+            sc->rho = sfm / smm;
+            sc->inv_denom = 1.0 / denom;
+        }
+        sc->coef_scale = 0.0;
+    } else {
+        sc->measure = acc[0] / n;
+        sc->coef_scale = 1.0 / n;
+    }
+    sc->reg[0] = sc->reg[1] = sc->reg[2] = 0.0;
+}
+
+// Per-sample derivative vector v = coef * gradM with coef folding the host
+// normalisation (GpuEvaluator.cu:602-603) or the NCC combination
+// (dF - sfm/smm dM)/denom (GpuEvaluator.cu:628-633; GpuEvaluatorNCC.cu:177-179).
+// The jitter offset rides in .w as packed bytes.
+template <int DIM, int MEASURE>
+__global__ __launch_bounds__(kThreads) void k_prepare(Problem pr, const float4 *__restrict__ sg,
+                                                      const float *__restrict__ sb, const Scalars *__restrict__ sc,
+                                                      float4 *__restrict__ sv)
+{
+    long long s = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (s >= pr.n_sub) return;
+    float4 g = sg[s];
+    float coef;
+    if (MEASURE == SKM_MEASURE_MSE) {
+        coef = (float)(sc->coef_scale * (double)g.w);
+    } else {
+        double a = g.w, b = sb[s];
+        coef = (float)(((a - sc->mean_f) - sc->rho * (b - sc->mean_m)) * sc->inv_denom);
+    }
+    unsigned packed = 0;
+    if (pr.sub > 1) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) packed |= (unsigned)pr.offsets[s * DIM + d] << (8 * d);
+    }
+    sv[s] = make_float4(coef * g.x, coef * g.y, coef * g.z, __uint_as_float(packed));
+}
+
+// ------------------------------------------------------------ derivative gather
+
+// d_j = w_j * sum_{samples m in window(j)} k(p_j, x_m) v_m
+// (evaluateDerivativeMSE, GpuEvaluatorMSE.cu:97-170, 239-309;
+// evaluateDerivativeNCC, GpuEvaluatorNCC.cu:111-190, 265-340).  One workgroup
+// per control point; the window (reference or exact) was computed on the host
+// in FP64.  Lanes work in groups of 8 along x, 32 rows at a time; rows and
+// row ends outside the support ball are skipped (k == 0 there).
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *__restrict__ sv,
+                                                     float *__restrict__ deriv)
+{
+    __shared__ double lds[4 * 4];
+    const long long j = blockIdx.x;
+    int jc[3] = {0, 0, 0};
+    {
+        long long r = j;
+        jc[0] = (int)(r % pr.cp.n[0]); r /= pr.cp.n[0];
+        jc[1] = (int)(r % pr.cp.n[1]); r /= pr.cp.n[1];
+        jc[2] = (int)r;
+    }
+    float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]}, p[3];
+    to_global<DIM>(pr.cp, fj, p);
+    PState st;
+    cp_state_load<DIM, MODE>(pr, j, st);
+    st.support = MODE == 0 ? pr.kc.smin : pr.cp_state[j * kCpStateStride + 11];
+    const int *win = pr.cp_window + j * 6;
+    int lo[3] = {win[0], win[1], DIM == 3 ? win[2] : 0}, hi[3] = {win[3], win[4], DIM == 3 ? win[5] : 0};
+    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
+    const int rows = by * bz;
+    const int tx = threadIdx.x & 7, tr = threadIdx.x >> 3;
+    const float s2 = st.support * st.support;
+    const int sub = pr.sub;
+    const float span = sub > 1 ? (float)sub : 0.0f;  // jitter range of a cell
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    float pc[3];
+    to_local<DIM>(pr.fixed, p, pc);  // cp position in fixed index units
+    for (int r = tr; r < rows; r += kThreads / 8) {
+        int my = lo[1] + r % by, mz = lo[2] + r / by;
+        // distance from p to the cell row's y/z extent (jitter included)
+        float y0 = pr.fixed.o[1] + pr.fixed.h[1] * (float)(my * sub);
+        float y1 = y0 + pr.fixed.h[1] * span;
+        float dy = fmaxf(0.0f, fmaxf(y0 - p[1], p[1] - y1));
+        float dz = 0.0f;
+        if (DIM == 3) {
+            float z0 = pr.fixed.o[2] + pr.fixed.h[2] * (float)(mz * sub);
+            float z1 = z0 + pr.fixed.h[2] * span;
+            dz = fmaxf(0.0f, fmaxf(z0 - p[2], p[2] - z1));
+        }
+        float rem = s2 - dy * dy - dz * dz;
+        if (rem <= 0.0f) continue;
+        float rx = sqrtf(rem) * pr.fixed.ih[0];
+        int xa = max(lo[0], (int)floorf((pc[0] - rx - span) / (float)sub) - 1);
+        int xb = min(hi[0], (int)ceilf((pc[0] + rx) / (float)sub) + 1);
+        long long rowbase = (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
+        for (int mx = xa + tx; mx <= xb; mx += 8) {
+            float4 v4 = sv[rowbase + mx];
+            int pos[3] = {mx, my, mz};
+            if (sub > 1) {
+                unsigned pk = __float_as_uint(v4.w);
+                pos[0] = mx * sub + (int)(pk & 0xff);
+                pos[1] = my * sub + (int)((pk >> 8) & 0xff);
+                pos[2] = mz * sub + (int)((pk >> 16) & 0xff);
+            }
+            float fpos[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, xs[3];
+            to_global<DIM>(pr.fixed, fpos, xs);
+            float v[3] = {p[0] - xs[0], p[1] - xs[1], DIM == 3 ? p[2] - xs[2] : 0.0f};
+            PState sst;
+            if (MODE != 0) point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, xs, sst);
+            float k = kernel_pair<DIM, MODE>(pr.kc, v, st, sst);
+            acc[0] = fmaf(k, v4.x, acc[0]);
+            acc[1] = fmaf(k, v4.y, acc[1]);
+            if (DIM == 3) acc[2] = fmaf(k, v4.z, acc[2]);
+        }
+    }
+    double dacc[4] = {acc[0], acc[1], acc[2], 0.0};
+    block_sum<4>(dacc, lds);
+    if (threadIdx.x == 0) {
+        double w = pr.cpc[j].w;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) deriv[j + d * pr.n_cp] = (float)(w * dacc[d]);
+    }
+}
+
+// ------------------------------------------------------------ regularizers
+
+// RKHS, RD and PG in one pass over the cp window (regularizerRKHS / RD / PG,
+// GpuEvaluatorRegularizers.cu:24-255, 259-472).  Values are unweighted per-cp
+// sums; derivatives are weighted and added to the normalised metric
+// derivative (GpuEvaluator.cu:652-711).
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_regularizers(Problem pr, double w_rkhs, double w_rd, double w_pg,
+                                                           double pg_scale, int with_derivative,
+                                                           float *__restrict__ deriv, double *__restrict__ partials)
+{
+    __shared__ double lds[4 * 4];
+    long long j = blockIdx.x * (long long)kThreads + threadIdx.x;
+    double vals[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < pr.n_cp) {
+        int jc[3] = {0, 0, 0};
+        {
+            long long r = j;
+            jc[0] = (int)(r % pr.cp.n[0]); r /= pr.cp.n[0];
+            jc[1] = (int)(r % pr.cp.n[1]); r /= pr.cp.n[1];
+            jc[2] = (int)r;
+        }
+        float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]}, p[3];
+        to_global<DIM>(pr.cp, fj, p);
+        PState st;
+        cp_state_load<DIM, MODE>(pr, j, st);
+        float4 cj4 = pr.cpc[j];
+        float cj[3] = {cj4.x, cj4.y, cj4.z};
+        float wj = cj4.w;
+        const int *win = pr.reg_window + j * 6;
+        float s = (float)pg_scale;
+        float vr = 0.0f, vd = 0.0f, vp = 0.0f;
+        float dr[3] = {0, 0, 0}, dd[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+        int z0 = DIM == 3 ? win[2] : 0, z1 = DIM == 3 ? win[5] : 0;
+        for (int iz = z0; iz <= z1; ++iz)
+            for (int iy = win[1]; iy <= win[4]; ++iy)
+                for (int ix = win[0]; ix <= win[3]; ++ix) {
+                    long long i = ix + (long long)pr.cp.n[0] * (iy + (long long)pr.cp.n[1] * iz);
+                    float fi[3] = {(float)ix, (float)iy, (float)iz}, q[3];
+                    to_global<DIM>(pr.cp, fi, q);
+                    float v[3] = {p[0] - q[0], p[1] - q[1], DIM == 3 ? p[2] - q[2] : 0.0f};
+                    PState ist;
+                    cp_state_load<DIM, MODE>(pr, i, ist);
+                    float k = kernel_pair<DIM, MODE>(pr.kc, v, st, ist);
+                    if (k == 0.0f) continue;
+                    float4 ci4 = pr.cpc[i];
+                    float ci[3] = {ci4.x, ci4.y, ci4.z};
+                    float kw = k * wj;
+                    if (w_rkhs > 0) {
+                        float dot = ci[0] * cj[0] + ci[1] * cj[1] + (DIM == 3 ? ci[2] * cj[2] : 0.0f);
+                        vr = fmaf(kw, dot, vr);
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) dr[d] = fmaf(2.0f * kw, ci[d], dr[d]);
+                    }
+                    if (w_rd > 0) {
+                        float e[3] = {cj[0] - ci[0], cj[1] - ci[1], DIM == 3 ? cj[2] - ci[2] : 0.0f};
+                        float l2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+                        vd = fmaf(kw, l2, vd);
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) dd[d] = fmaf(4.0f * kw, e[d], dd[d]);
+                    }
+                    if (w_pg > 0) {
+                        if (DIM == 3) {
+                            float c0 = ci[1] * cj[2] - ci[2] * cj[1];
+                            float c1 = ci[2] * cj[0] - ci[0] * cj[2];
+                            float c2 = ci[0] * cj[1] - ci[1] * cj[0];
+                            float n2 = c0 * c0 + c1 * c1 + c2 * c2;
+                            vp = fmaf(kw, s * s * 0.5f * logf(1.0f + n2 / s), vp);
+                            float den = 2.0f * (n2 / s + 1.0f);
+                            float q0 = 2.0f * ci[2] * (ci[2] * cj[0] - ci[0] * cj[2]) - 2.0f * ci[1] * (ci[0] * cj[1] - ci[1] * cj[0]);
+                            float q1 = 2.0f * ci[0] * (ci[0] * cj[1] - ci[1] * cj[0]) - 2.0f * ci[2] * (ci[1] * cj[2] - ci[2] * cj[1]);
+                            float q2 = 2.0f * ci[1] * (ci[1] * cj[2] - ci[2] * cj[1]) - 2.0f * ci[0] * (ci[2] * cj[0] - ci[0] * cj[2]);
+                            dp[0] = fmaf(2.0f * kw, s * q0 / den, dp[0]);
+                            dp[1] = fmaf(2.0f * kw, s * q1 / den, dp[1]);
+                            dp[2] = fmaf(2.0f * kw, s * q2 / den, dp[2]);
+                        } else {
+                            float val = ci[0] * cj[1] - ci[1] * cj[0];
+                            float val2 = val * val;
+                            vp = fmaf(kw, s * s * 0.5f * logf(1.0f + val2 / s), vp);
+                            float factor = val2 / s + 1.0f;
+                            dp[0] = fmaf(2.0f * kw, -s * ci[1] * val / factor, dp[0]);
+                            dp[1] = fmaf(2.0f * kw, s * ci[0] * val / factor, dp[1]);
+                        }
+                    }
+                }
+        vals[0] = vr; vals[1] = vd; vals[2] = vp;
+        if (with_derivative) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                double add = w_rkhs * (double)dr[d] + w_rd * (double)dd[d] + w_pg * (double)dp[d];
+                deriv[j + d * pr.n_cp] = (float)((double)deriv[j + d * pr.n_cp] + add);
+            }
+        }
+    }
+    block_sum<4>(vals, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4LL + k] = vals[k];
+}
+
+__global__ __launch_bounds__(kThreads) void k_reg_reduce(const double *__restrict__ partials, int blocks,
+                                                         Scalars *sc)
+{
+    __shared__ double lds[4 * 4];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < blocks; b += kThreads)
+        for (int k = 0; k < 4; ++k) acc[k] += partials[b * 4LL + k];
+    block_sum<4>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) sc->reg[k] = acc[k];
+}
+
+// ------------------------------------------------------------ displacement
+
+// Dense u(x) + previous field on the full fixed grid, AoS
+// (evaluateDisplacement, GpuEvaluatorDisplacementField.cu:24-127).
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_displacement(Problem pr, float *__restrict__ dst)
+{
+    int l[3];
+    tile_coords<DIM>(threadIdx.x, l);
+    int m[3] = {(int)blockIdx.x * Tile<DIM>::X + l[0], (int)blockIdx.y * Tile<DIM>::Y + l[1],
+                DIM == 3 ? (int)blockIdx.z * Tile<DIM>::Z + l[2] : 0};
+    if (m[0] >= pr.fixed.n[0] || m[1] >= pr.fixed.n[1] || (DIM == 3 && m[2] >= pr.fixed.n[2])) return;
+    float fm[3] = {(float)m[0], (float)m[1], (float)m[2]}, x[3];
+    to_global<DIM>(pr.fixed, fm, x);
+    PState st;
+    point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+    float u[3];
+    df_lookup<DIM>(pr, x, u);
+    kernel_sum<DIM, MODE>(pr, x, st, u);
+    long long v = m[0] + (long long)pr.fixed.n[0] * (m[1] + (long long)pr.fixed.n[1] * m[2]);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) dst[v * DIM + d] = u[d];
+}
+
+// ------------------------------------------------------------ small kernels
+
+// PCG32 cube offsets: cell i draws numbers i*D .. i*D+D-1 of the stream, so
+// every thread jumps ahead in O(log n) and the result is bit-identical to the
+// serial host loop (GpuEvaluator.cu:107-113; UniformRng.h:34-55).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_offsets(long long n, int sub, uint64_t state, uint64_t inc,
+                                                      int *__restrict__ out)
+{
+    long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i >= n) return;
+    uint64_t st = lcg_advance(state, (uint64_t)i * DIM, inc);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        uint32_t r = pcg32_output(st);
+        st = st * kPcgMult + inc;
+        double f = (double)r / 4294967295.0;
+        out[i * DIM + d] = (int)(f * (double)sub);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_pack_cp(long long n, int dim, const float *__restrict__ cp,
+                                                      const float *__restrict__ cpw, int soa,
+                                                      float4 *__restrict__ cpc)
+{
+    long long j = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (j >= n) return;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    for (int d = 0; d < dim; ++d) c[d] = soa ? cp[j + d * n] : cp[j * dim + d];
+    cpc[j] = make_float4(c[0], c[1], c[2], cpw[j]);
+}
+
+__global__ void k_result(const Scalars *__restrict__ sc, skm_result *__restrict__ out)
+{
+    out->measure = sc->measure;
+    out->reg_value_rkhs = sc->reg[0];
+    out->reg_value_rd = sc->reg[1];
+    out->reg_value_pg = sc->reg[2];
+    out->num_pixels_counted = (uint32_t)sc->count;
+    out->reserved = 0;
+}
+
+// ------------------------------------------------------------ launchers
+
+template <int DIM>
+static dim3 tile_grid(const int *n)
+{
+    return dim3((n[0] + Tile<DIM>::X - 1) / Tile<DIM>::X, (n[1] + Tile<DIM>::Y - 1) / Tile<DIM>::Y,
+                DIM == 3 ? (n[2] + Tile<DIM>::Z - 1) / Tile<DIM>::Z : 1);
+}
+
+static unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+#define SKM_DISPATCH_DIM_MODE(pr, FN, ...)                                                                         \
+    do {                                                                                                           \
+        if ((pr).dim == 2) {                                                                                       \
+            if ((pr).kc.mode == 0) FN<2, 0>(__VA_ARGS__);                                                          \
+            else if ((pr).kc.mode == 1) FN<2, 1>(__VA_ARGS__);                                                     \
+            else FN<2, 2>(__VA_ARGS__);                                                                            \
+        } else {                                                                                                   \
+            if ((pr).kc.mode == 0) FN<3, 0>(__VA_ARGS__);                                                          \
+            else if ((pr).kc.mode == 1) FN<3, 1>(__VA_ARGS__);                                                     \
+            else FN<3, 2>(__VA_ARGS__);                                                                            \
+        }                                                                                                          \
+    } while (0)
+
+void launch_offsets(const Problem &pr, int *offsets, uint64_t state, uint64_t inc, hipStream_t s)
+{
+    if (pr.dim == 2) k_offsets<2><<<blocks_for(pr.n_sub), kThreads, 0, s>>>(pr.n_sub, pr.sub, state, inc, offsets);
+    else k_offsets<3><<<blocks_for(pr.n_sub), kThreads, 0, s>>>(pr.n_sub, pr.sub, state, inc, offsets);
+}
+
+void launch_pack_cp(const Problem &pr, const float *cp, const float *cpw, int soa, float4 *cpc, hipStream_t s)
+{
+    k_pack_cp<<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr.n_cp, pr.dim, cp, cpw, soa, cpc);
+}
+
+template <int DIM, int MODE>
+static void value_impl(const Problem &pr, int measure, float4 *sg, float *sb, double *partials, int *blocks,
+                       hipStream_t s)
+{
+    dim3 grid = tile_grid<DIM>(pr.nsub);
+    *blocks = (int)(grid.x * grid.y * grid.z);
+    if (measure == SKM_MEASURE_MSE)
+        k_value<DIM, MODE, SKM_MEASURE_MSE><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
+    else
+        k_value<DIM, MODE, SKM_MEASURE_NCC><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
+}
+
+void launch_value(const Problem &pr, int measure, float4 *sg, float *sb, double *partials, int *blocks_out,
+                  hipStream_t s)
+{
+    SKM_DISPATCH_DIM_MODE(pr, value_impl, pr, measure, sg, sb, partials, blocks_out, s);
+}
+
+void launch_reduce(const Problem &pr, int measure, const double *partials, int blocks, Scalars *sc, hipStream_t s)
+{
+    k_reduce<<<1, kThreads, 0, s>>>(measure, partials, blocks, sc);
+}
+
+void launch_prepare(const Problem &pr, int measure, const float4 *sg, const float *sb, const Scalars *sc,
+                    float4 *sv, hipStream_t s)
+{
+    unsigned g = blocks_for(pr.n_sub);
+    if (pr.dim == 2) {
+        if (measure == SKM_MEASURE_MSE) k_prepare<2, SKM_MEASURE_MSE><<<g, kThreads, 0, s>>>(pr, sg, sb, sc, sv);
+        else k_prepare<2, SKM_MEASURE_NCC><<<g, kThreads, 0, s>>>(pr, sg, sb, sc, sv);
+    } else {
+        if (measure == SKM_MEASURE_MSE) k_prepare<3, SKM_MEASURE_MSE><<<g, kThreads, 0, s>>>(pr, sg, sb, sc, sv);
+        else k_prepare<3, SKM_MEASURE_NCC><<<g, kThreads, 0, s>>>(pr, sg, sb, sc, sv);
+    }
+}
+
+template <int DIM, int MODE>
+static void gather_impl(const Problem &pr, const float4 *sv, float *deriv, hipStream_t s)
+{
+    k_gather<DIM, MODE><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
+}
+
+void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s)
+{
+    SKM_DISPATCH_DIM_MODE(pr, gather_impl, pr, sv, deriv_soa, s);
+}
+
+template <int DIM, int MODE>
+static void reg_impl(const Problem &pr, const double *w, double pg, int with_derivative, float *deriv,
+                     double *partials, hipStream_t s)
+{
+    k_regularizers<DIM, MODE><<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr, w[0], w[1], w[2], pg, with_derivative,
+                                                                       deriv, partials);
+}
+
+void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, int with_derivative,
+                         float *deriv_soa, double *partials, Scalars *sc, hipStream_t s)
+{
+    SKM_DISPATCH_DIM_MODE(pr, reg_impl, pr, weights, pg_scaling, with_derivative, deriv_soa, partials, s);
+    k_reg_reduce<<<1, kThreads, 0, s>>>(partials, (int)blocks_for(pr.n_cp), sc);
+}
+
+template <int DIM, int MODE>
+static void displacement_impl(const Problem &pr, float *dst, hipStream_t s)
+{
+    k_displacement<DIM, MODE><<<tile_grid<DIM>(pr.fixed.n), kThreads, 0, s>>>(pr, dst);
+}
+
+void launch_displacement(const Problem &pr, float *dst, hipStream_t s)
+{
+    SKM_DISPATCH_DIM_MODE(pr, displacement_impl, pr, dst, s);
+}
+
+void launch_result(const Scalars *sc, void *result, hipStream_t s)
+{
+    k_result<<<1, 1, 0, s>>>(sc, (skm_result *)result);
+}
+
+// ------------------------------------------------------------ optimizer
+
+// L1 / L2 / L21 values and subgradients and the sign flip of
+// RegularizedImageToImageMetricv4::GetValueAndDerivative
+// (itkRegularizedImageToImageMetricv4.hxx:430-529), one thread per cp.
+__global__ __launch_bounds__(kThreads) void k_host_regs(long long ncp, int dim, const float *__restrict__ th,
+                                                        double l1w, double l21w, double l2w, int negate,
+                                                        float *__restrict__ der, double *__restrict__ partials)
+{
+    __shared__ double lds[4 * 4];
+    long long j = blockIdx.x * (long long)kThreads + threadIdx.x;
+    double vals[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < ncp) {
+        double t[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) { t[d] = th[j + d * ncp]; g[d] = der[j + d * ncp]; }
+        if (l1w > 0) {
+            for (int d = 0; d < dim; ++d) {
+                vals[0] += fabs(t[d]);
+                if (fabs(t[d]) > 0) g[d] += l1w * (t[d] < 0 ? -1.0 : 1.0);
+                else if (g[d] < -l1w) g[d] += l1w;
+                else if (g[d] > l1w) g[d] -= l1w;
+                else g[d] = 0.0;
+            }
+        }
+        double n2 = 0.0;
+        for (int d = 0; d < dim; ++d) n2 += t[d] * t[d];
+        if (l2w > 0) {
+            vals[2] = n2;
+            for (int d = 0; d < dim; ++d) g[d] += l2w * 2.0 * t[d];
+        }
+        if (l21w > 0) {
+            double nrm = sqrt(n2);
+            vals[1] = nrm;
+            for (int d = 0; d < dim; ++d) {
+                if (nrm > 0) g[d] += l21w * t[d] / nrm;
+                else if (g[d] < -l21w) g[d] += l21w;
+                else if (g[d] > l21w) g[d] -= l21w;
+                else g[d] = 0.0;
+            }
+        }
+        for (int d = 0; d < dim; ++d) der[j + d * ncp] = (float)(negate ? -g[d] : g[d]);
+    }
+    block_sum<4>(vals, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4LL + k] = vals[k];
+}
+
+void launch_host_regs_device(const Problem &pr, const float *params_soa, double l1, double l21, double l2,
+                             int negate, float *deriv_soa, double *partials, hipStream_t s)
+{
+    k_host_regs<<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr.n_cp, pr.dim, params_soa, l1, l21, l2, negate,
+                                                         deriv_soa, partials);
+}
+
+// <g_k, g_{k-1}> partials for the adaptive step (CalculateScalarProduct,
+// itkAdaptiveStepGradientDescentOptimizerv4.hxx:100-116).
+__global__ __launch_bounds__(kThreads) void k_dot(long long n, const float *__restrict__ a,
+                                                  const float *__restrict__ b, double *__restrict__ partials)
+{
+    __shared__ double lds[1 * 4];
+    long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    double v[1] = {0.0};
+    for (; i < n; i += (long long)gridDim.x * kThreads) v[0] += (double)a[i] * (double)b[i];
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+
+struct StepParams {
+    int kind;
+    double a, A, alpha, fmax, fmin, omega;
+    double l1w, l21w, l2w;
+    double rw[3];
+    int dot_blocks, reg_blocks;
+};
+
+// Learning rate for this step and the adaptive time update
+// (EstimateLearningRate, Standard...hxx:159-166, Adaptive...hxx:86-97;
+// ModifyCurrentTime, Adaptive...hxx:118-128), plus the iteration record.
+__global__ void k_step_scalars(StepParams sp, const double *__restrict__ dot_partials,
+                               const double *__restrict__ reg_partials, OptState *st, const Scalars *sc,
+                               skm_iteration_record *rec)
+{
+    double dot = 0.0;
+    for (int b = 0; b < sp.dot_blocks; ++b) dot += dot_partials[b];
+    double rate;
+    if (sp.kind == SKM_OPT_STANDARD) rate = pow(sp.a / (sp.A + (double)st->iteration + 1.0), sp.alpha);
+    else if (sp.kind == SKM_OPT_ADAPTIVE) rate = sp.a / pow(sp.A + st->time + 1.0, sp.alpha);
+    else rate = sp.a;
+    st->rate = rate;
+    if (sp.kind == SKM_OPT_ADAPTIVE) {
+        double x = -dot;
+        double f = sp.fmin + (sp.fmax - sp.fmin) / (1.0 - (sp.fmax / sp.fmin) * exp(-x / sp.omega));
+        st->time = fmax(0.0, st->time + f);
+    }
+    st->prev_dot = dot;
+    double l1 = 0, l21 = 0, l2 = 0;
+    for (int b = 0; b < sp.reg_blocks; ++b) {
+        l1 += reg_partials[b * 4LL + 0];
+        l21 += reg_partials[b * 4LL + 1];
+        l2 += reg_partials[b * 4LL + 2];
+    }
+    double reg = 0.0;
+    if (sp.l1w > 0) reg += sp.l1w * l1;
+    if (sp.l21w > 0) reg += sp.l21w * l21;
+    if (sp.l2w > 0) reg += sp.l2w * l2;
+    for (int k = 0; k < 3; ++k)
+        if (sp.rw[k] > 0) reg += sp.rw[k] * sc->reg[k];
+    if (rec) {
+        rec[st->iteration].value = sc->measure + reg;
+        rec[st->iteration].measure = sc->measure;
+        rec[st->iteration].learning_rate = rate;
+        rec[st->iteration].num_pixels_counted = (uint32_t)sc->count;
+        rec[st->iteration].reserved = 0;
+    }
+    st->iteration += 1;
+}
+
+// theta <- theta + rate * d with orthant projection; d is kept for the next
+// scalar product (ModifyGradientByLearningRateOverSubRange, Standard...hxx:133-143).
+__global__ __launch_bounds__(kThreads) void k_update(long long n, float *__restrict__ th, const float *__restrict__ g,
+                                                     float *__restrict__ prev_g, const OptState *st, int orthant)
+{
+    long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i >= n) return;
+    double rate = st->rate;
+    double t = th[i];
+    double v = t + (double)g[i] * rate;
+    if (orthant && v * t < 0) v = 0.0;
+    th[i] = (float)v;
+    prev_g[i] = g[i];
+}
+
+void launch_opt_step(const Problem &pr, int kind, double a, double A, double alpha, double fmax, double fmin,
+                     double omega, int orthant, float *params_soa, const float *grad_soa, float *prev_grad,
+                     double *partials, OptState *st, const Scalars *sc, const double *reg_partials, double l1w,
+                     double l21w, double l2w, const double *gpu_reg_weights, void *record, hipStream_t s)
+{
+    long long n = pr.n_cp * pr.dim;
+    int dot_blocks = 64;
+    k_dot<<<dot_blocks, kThreads, 0, s>>>(n, grad_soa, prev_grad, partials);
+    StepParams sp{kind, a, A, alpha, fmax, fmin, omega, l1w, l21w, l2w,
+                  {gpu_reg_weights[0], gpu_reg_weights[1], gpu_reg_weights[2]}, dot_blocks, (int)blocks_for(pr.n_cp)};
+    k_step_scalars<<<1, 1, 0, s>>>(sp, partials, reg_partials, st, sc, (skm_iteration_record *)record);
+    k_update<<<blocks_for(n), kThreads, 0, s>>>(n, params_soa, grad_soa, prev_grad, st, orthant);
+}
+
+void launch_soa_to_cpc(const Problem &pr, const float *params_soa, const float *cpw, float4 *cpc, hipStream_t s)
+{
+    k_pack_cp<<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr.n_cp, pr.dim, params_soa, cpw, 1, cpc);
+}
+
+}  // namespace skm

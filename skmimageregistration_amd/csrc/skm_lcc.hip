@@ -1,0 +1,459 @@
+// skm_lcc.hip -- local cross-correlation (LCC) value and derivative on gfx950.
+//
+// The reference (include/cuda/GpuEvaluatorLCC.cu) recomputes the kernel sum
+// u(y) for every neighbour y of every centre, inside every control point's
+// derivative thread (NOAPPROX, include/CommonConfig.h:41).  Here u(y), the
+// warped value M(y + u(y)) and its gradient are computed once per fixed voxel
+// (dense warp), the per-centre correlation sums are taken from that field,
+// and each centre's derivative of -v^2/u is folded into three coefficients
+// A, B, C with  d cc_x / d(kw M(y')) = A kw F(y) + B kw M(y') + C.
+// The derivative then only needs k(p_j, y) per (cp, centre, neighbour).
+#include "skm_internal.hpp"
+
+#include "skm.h"
+
+namespace skm {
+
+template <int DIM> struct LTile;
+template <> struct LTile<2> { static constexpr int X = 16, Y = 16, Z = 1; };
+template <> struct LTile<3> { static constexpr int X = 8, Y = 8, Z = 4; };
+
+template <int DIM>
+__device__ __forceinline__ void ltile_coords(int t, int *l)
+{
+    const int lane = t & 63, wave = t >> 6;
+    if (DIM == 3) {
+        l[0] = (lane & 3) + (wave & 1) * 4;
+        l[1] = ((lane >> 2) & 3) + (wave >> 1) * 4;
+        l[2] = lane >> 4;
+    } else {
+        l[0] = lane & 15;
+        l[1] = (lane >> 4) + wave * 4;
+        l[2] = 0;
+    }
+}
+
+template <int DIM, int MODE, int QUIRK>
+__device__ __forceinline__ void lcc_kernel_sum(const Problem &pr, const float *y, const PState &st, float *u)
+{
+    float ci[3];
+    to_local<DIM>(pr.cp, y, ci);
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        float r = st.support * pr.cp.ih[d];
+        lo[d] = clampi((int)(ci[d] - r + 1.0f), 0, pr.cp.n[d] - 1);
+        hi[d] = clampi((int)(ci[d] + r), 0, pr.cp.n[d] - 1);
+    }
+    const float s2 = st.support * st.support;
+    const int z0 = DIM == 3 ? lo[2] : 0, z1 = DIM == 3 ? hi[2] : 0;
+    for (int cz = z0; cz <= z1; ++cz) {
+        float dz = DIM == 3 ? y[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
+        for (int cy = lo[1]; cy <= hi[1]; ++cy) {
+            float dy = y[1] - (pr.cp.o[1] + pr.cp.h[1] * cy);
+            float rem = s2 - dy * dy - dz * dz;
+            if (rem <= 0.0f) continue;
+            float rx = sqrtf(rem) * pr.cp.ih[0];
+            int xa = max(lo[0], (int)floorf(ci[0] - rx));
+            int xb = min(hi[0], (int)ceilf(ci[0] + rx));
+            long long row = (long long)pr.cp.n[0] * (cy + (long long)pr.cp.n[1] * cz);
+            for (int cx = xa; cx <= xb; ++cx) {
+                float v[3] = {y[0] - (pr.cp.o[0] + pr.cp.h[0] * cx), dy, dz};
+                float k;
+                if (QUIRK) {
+                    // evaluate(cp_i, y, sigma(y)) looks the second sigma up at y too
+                    // (GpuEvaluatorLCC.cu:428-443, 3D derivative kernel)
+                    k = kernel_pair<DIM, MODE>(pr.kc, v, st, st);
+                } else {
+                    PState cst;
+                    if (MODE != 0) {
+                        const float *c = pr.cp_state + (row + cx) * kCpStateStride;
+                        cst.sig = c[0];
+                        cst.sdet = c[1];
+#pragma unroll
+                        for (int i = 0; i < DIM * DIM; ++i) cst.cov[i] = c[2 + i];
+                    }
+                    k = kernel_pair<DIM, MODE>(pr.kc, v, st, cst);
+                }
+                float4 c = pr.cpc[row + cx];
+                float wk = c.w * k;
+                u[0] = fmaf(wk, c.x, u[0]);
+                u[1] = fmaf(wk, c.y, u[1]);
+                if (DIM == 3) u[2] = fmaf(wk, c.z, u[2]);
+            }
+        }
+    }
+}
+
+// Dense warp: for every fixed voxel y, (grad M, M) at y + df(y) + u(y); .w is
+// NaN where y + u(y) leaves the moving image (insideGlobal, BsplineImage.h:285-296).
+template <int DIM, int MODE, int QUIRK>
+__global__ __launch_bounds__(kThreads) void k_dense_warp(Problem pr, float4 *__restrict__ out)
+{
+    int l[3];
+    ltile_coords<DIM>(threadIdx.x, l);
+    int m[3] = {(int)blockIdx.x * LTile<DIM>::X + l[0], (int)blockIdx.y * LTile<DIM>::Y + l[1],
+                DIM == 3 ? (int)blockIdx.z * LTile<DIM>::Z + l[2] : 0};
+    if (m[0] >= pr.fixed.n[0] || m[1] >= pr.fixed.n[1] || (DIM == 3 && m[2] >= pr.fixed.n[2])) return;
+    float fm[3] = {(float)m[0], (float)m[1], (float)m[2]}, y[3];
+    to_global<DIM>(pr.fixed, fm, y);
+    PState st;
+    point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, y, st);
+    float u[3] = {0.0f, 0.0f, 0.0f};
+    if (pr.has_df) {
+        long long f;
+        if (nearest<DIM>(pr.df, y, f))
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) u[d] = pr.df_data[f * DIM + d];
+    }
+    lcc_kernel_sum<DIM, MODE, QUIRK>(pr, y, st, u);
+    float w[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) w[d] = y[d] + u[d];
+    float val = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
+    bool in = bspline_sample<DIM>(pr.coeffs, pr.moving, w, val, g);
+    long long v = m[0] + (long long)pr.fixed.n[0] * (m[1] + (long long)pr.fixed.n[1] * m[2]);
+    out[v] = in ? make_float4(g[0], g[1], DIM == 3 ? g[2] : 0.0f, val) : make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0x7fc00000));
+}
+
+// neighborhoodWeight(p, q, support, 10) (GpuEvaluatorLCC.cu:28-35).
+__device__ __forceinline__ float neighborhood_weight(float r2, float support)
+{
+    float q = r2 / (support * support);
+    float q2 = q * q;
+    float r = q2 * q2 * q;  // (|p-q|/support)^10
+    float f = fmaxf(0.0f, 1.0f - r);
+    return f * f;
+}
+
+// Neighbour box of a centre: tight window with the centre's support on the
+// fixed grid, FP64 so the stride lattice starts where the reference's does
+// (GpuEvaluatorLCC.cu:95-97).
+template <int DIM>
+__device__ __forceinline__ void neighbour_box(const Problem &pr, const int *pos, double support, int *lo, int *hi)
+{
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        double idx = (double)pos[d];  // centre index: toLocal(toGlobal(pos)) == pos
+        double r = support / (double)pr.fixed.h[d];
+        lo[d] = clampi((int)(idx - r + 1.0), 0, pr.fixed.n[d] - 1);
+        hi[d] = clampi((int)(idx + r), 0, pr.fixed.n[d] - 1);
+    }
+}
+
+template <int DIM>
+__device__ __forceinline__ void centre_position(const Problem &pr, long long s, int *pos)
+{
+    long long r = s;
+    int m[3] = {0, 0, 0};
+    m[0] = (int)(r % pr.nsub[0]); r /= pr.nsub[0];
+    m[1] = (int)(r % pr.nsub[1]); r /= pr.nsub[1];
+    m[2] = (int)r;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) pos[d] = pr.sub > 1 ? m[d] * pr.sub + pr.offsets[s * DIM + d] : m[d];
+}
+
+template <int DIM>
+__device__ __forceinline__ long long voxel_index(const Problem &pr, const int *pos)
+{
+    long long f = 0, stride = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        f += stride * clampi(pos[d], 0, pr.fixed.n[d] - 1);
+        stride *= pr.fixed.n[d];
+    }
+    return f;
+}
+
+// Per-centre LCC (evaluateValueLCC, GpuEvaluatorLCC.cu:38-204, 536-694) from the
+// dense warp, one thread per centre.  Writes the derivative coefficients
+// (A, B, C) of the centre (zero where the centre contributes no derivative:
+// reference |u| > 1e-16 guard, :517-518, or d1 > 0 && d2 > 0 in exact mode)
+// and adds cc and the centre's inside flag to the block partials.
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const float4 *__restrict__ dense,
+                                                          float4 *__restrict__ coef, double *__restrict__ partials)
+{
+    __shared__ double lds[kPartialsK * 4];
+    long long s = blockIdx.x * (long long)kThreads + threadIdx.x;
+    double acc[kPartialsK];
+#pragma unroll
+    for (int k = 0; k < kPartialsK; ++k) acc[k] = 0.0;
+    if (s < pr.n_sub) {
+        int pos[3] = {0, 0, 0};
+        centre_position<DIM>(pr, s, pos);
+        float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, x[3];
+        to_global<DIM>(pr.fixed, fp, x);
+        PState st;
+        point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        neighbour_box<DIM>(pr, pos, (double)st.support, lo, hi);
+        double sf = 0, sm = 0, sff = 0, smm = 0, sfm = 0;
+        int count = 0;
+        const int sn = pr.subn;
+        const int z0 = DIM == 3 ? lo[2] : 0, z1 = DIM == 3 ? hi[2] : 0;
+        for (int fz = z0; fz <= z1; fz += sn)
+            for (int fy = lo[1]; fy <= hi[1]; fy += sn)
+                for (int fx = lo[0]; fx <= hi[0]; fx += sn) {
+                    long long v = fx + (long long)pr.fixed.n[0] * (fy + (long long)pr.fixed.n[1] * fz);
+                    float4 g4 = dense[v];
+                    if (isnan(g4.w)) continue;
+                    float fy3[3] = {(float)fx, (float)fy, (float)fz}, y[3];
+                    to_global<DIM>(pr.fixed, fy3, y);
+                    float sup = pr.kc.smin;
+                    if (MODE != 0) {
+                        PState sty;
+                        point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, y, sty);
+                        sup = sty.support;
+                    }
+                    float r2 = 0.0f;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) r2 += (x[d] - y[d]) * (x[d] - y[d]);
+                    float kw = neighborhood_weight(r2, sup);
+                    if (!(kw > 0.0f)) continue;
+                    double mv = (double)(kw * g4.w);
+                    double fv = (double)(kw * pr.fixed_data[v]);
+                    sf += fv; sm += mv; sff += fv * fv; smm += mv * mv; sfm += fv * mv;
+                    ++count;
+                }
+        double cc = 0.0;
+        float A = 0.0f, B = 0.0f, C = 0.0f;
+        if (count > 0) {
+            double msf = sf / count, msm = sm / count;
+            double d1 = sff - 2.0 * msf * sf + count * msf * msf;
+            double d2 = smm - 2.0 * msm * sm + count * msm * msm;
+            double vv = sfm - msm * sf - msf * sm + count * msf * msm;
+            if (d1 > 0 && d2 > 0) cc = -vv * vv / (d1 * d2);
+            double u = d1 * d2;
+            bool use = pr.exact ? (d1 > 0 && d2 > 0) : (fabs(u) > 1e-16);
+            if (use) {
+                // beta(y) = -(2 v (fv - msf) u - 2 d1 (mv - msm) v^2) / u^2
+                double inv_u = 1.0 / u;
+                A = (float)(-2.0 * vv * inv_u);
+                B = (float)(2.0 * d1 * vv * vv * inv_u * inv_u);
+                C = (float)(2.0 * vv * msf * inv_u - 2.0 * d1 * msm * vv * vv * inv_u * inv_u);
+            }
+        }
+        coef[s] = make_float4(A, B, C, 0.0f);
+        acc[0] = cc;
+        long long cv = voxel_index<DIM>(pr, pos);
+        acc[6] = isnan(dense[cv].w) ? 0.0 : 1.0;
+    }
+    block_sum<kPartialsK>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < kPartialsK; ++k) partials[blockIdx.x * (long long)kPartialsK + k] = acc[k];
+}
+
+// Reference-mode derivative (evaluateDerivativeLCC, GpuEvaluatorLCC.cu:206-530,
+// 696-961): for cp j, centres in the tight window of p_j, all their
+// neighbours, ka = k(p_j, y) with the neighbour's sigma / covariance for both
+// points (:446-460), no w_j, divided by N on the host (:645-646).
+// One workgroup per control point, one wave per centre, lanes over neighbours.
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_lcc_gather_ref(Problem pr, const float4 *__restrict__ dense,
+                                                             const float4 *__restrict__ coef,
+                                                             const Scalars *__restrict__ sc,
+                                                             float *__restrict__ deriv)
+{
+    __shared__ double lds[4 * 4];
+    const long long j = blockIdx.x;
+    int jc[3] = {0, 0, 0};
+    {
+        long long r = j;
+        jc[0] = (int)(r % pr.cp.n[0]); r /= pr.cp.n[0];
+        jc[1] = (int)(r % pr.cp.n[1]); r /= pr.cp.n[1];
+        jc[2] = (int)r;
+    }
+    float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]}, p[3];
+    to_global<DIM>(pr.cp, fj, p);
+    const int *win = pr.cp_window + j * 6;
+    int lo[3] = {win[0], win[1], DIM == 3 ? win[2] : 0}, hi[3] = {win[3], win[4], DIM == 3 ? win[5] : 0};
+    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
+    const int ncent = bx * by * bz;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    const int sn = pr.subn;
+    for (int c = wave; c < ncent; c += kThreads / 64) {
+        int mx = lo[0] + c % bx, my = lo[1] + (c / bx) % by, mz = lo[2] + c / (bx * by);
+        long long s = mx + (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
+        float4 abc = coef[s];
+        if (abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f) continue;
+        int pos[3] = {0, 0, 0};
+        centre_position<DIM>(pr, s, pos);
+        float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, x[3];
+        to_global<DIM>(pr.fixed, fp, x);
+        PState st;
+        point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+        int nlo[3] = {0, 0, 0}, nhi[3] = {0, 0, 0};
+        neighbour_box<DIM>(pr, pos, (double)st.support, nlo, nhi);
+        int nx = (nhi[0] - nlo[0]) / sn + 1, ny = (nhi[1] - nlo[1]) / sn + 1;
+        int nz = DIM == 3 ? (nhi[2] - nlo[2]) / sn + 1 : 1;
+        int nn = nx * ny * nz;
+        for (int t = lane; t < nn; t += 64) {
+            int fx = nlo[0] + (t % nx) * sn, fy = nlo[1] + ((t / nx) % ny) * sn;
+            int fz = DIM == 3 ? nlo[2] + (t / (nx * ny)) * sn : 0;
+            long long v = fx + (long long)pr.fixed.n[0] * (fy + (long long)pr.fixed.n[1] * fz);
+            float4 g4 = dense[v];
+            if (isnan(g4.w)) continue;
+            float fy3[3] = {(float)fx, (float)fy, (float)fz}, y[3];
+            to_global<DIM>(pr.fixed, fy3, y);
+            PState sty;
+            point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, y, sty);
+            float r2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) r2 += (x[d] - y[d]) * (x[d] - y[d]);
+            float kw = neighborhood_weight(r2, MODE == 0 ? pr.kc.smin : sty.support);
+            if (!(kw > 0.0f)) continue;
+            float vv[3] = {p[0] - y[0], p[1] - y[1], DIM == 3 ? p[2] - y[2] : 0.0f};
+            float ka = kernel_pair<DIM, MODE>(pr.kc, vv, sty, sty);
+            if (ka == 0.0f) continue;
+            float fv = kw * pr.fixed_data[v];
+            float mv = kw * g4.w;
+            float beta = fmaf(abc.x, fv, fmaf(abc.y, mv, abc.z));
+            float w = beta * kw * ka;
+            acc[0] = fmaf(w, g4.x, acc[0]);
+            acc[1] = fmaf(w, g4.y, acc[1]);
+            if (DIM == 3) acc[2] = fmaf(w, g4.z, acc[2]);
+        }
+    }
+    double dacc[4] = {acc[0], acc[1], acc[2], 0.0};
+    block_sum<4>(dacc, lds);
+    if (threadIdx.x == 0) {
+        double scale = sc->coef_scale;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) deriv[j + d * pr.n_cp] = (float)(dacc[d] * scale);
+    }
+}
+
+// Exact mode: every valid centre scatters beta kw gradM onto its neighbours,
+// pre-scaled by 1/N; the control points then gather sum_y w_j k(p_j, y) gamma(y).
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_lcc_scatter(Problem pr, const float4 *__restrict__ dense,
+                                                          const float4 *__restrict__ coef,
+                                                          const Scalars *__restrict__ sc, float4 *__restrict__ gamma)
+{
+    long long s = blockIdx.x * (long long)(kThreads / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= pr.n_sub) return;
+    float4 abc = coef[s];
+    if (abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f) return;
+    float scale = (float)sc->coef_scale;
+    int pos[3] = {0, 0, 0};
+    centre_position<DIM>(pr, s, pos);
+    float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, x[3];
+    to_global<DIM>(pr.fixed, fp, x);
+    PState st;
+    point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+    int nlo[3] = {0, 0, 0}, nhi[3] = {0, 0, 0};
+    neighbour_box<DIM>(pr, pos, (double)st.support, nlo, nhi);
+    const int sn = pr.subn;
+    int nx = (nhi[0] - nlo[0]) / sn + 1, ny = (nhi[1] - nlo[1]) / sn + 1;
+    int nz = DIM == 3 ? (nhi[2] - nlo[2]) / sn + 1 : 1;
+    int nn = nx * ny * nz;
+    for (int t = lane; t < nn; t += 64) {
+        int fx = nlo[0] + (t % nx) * sn, fy = nlo[1] + ((t / nx) % ny) * sn;
+        int fz = DIM == 3 ? nlo[2] + (t / (nx * ny)) * sn : 0;
+        long long v = fx + (long long)pr.fixed.n[0] * (fy + (long long)pr.fixed.n[1] * fz);
+        float4 g4 = dense[v];
+        if (isnan(g4.w)) continue;
+        float fy3[3] = {(float)fx, (float)fy, (float)fz}, y[3];
+        to_global<DIM>(pr.fixed, fy3, y);
+        float sup = pr.kc.smin;
+        if (MODE != 0) {
+            PState sty;
+            point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, y, sty);
+            sup = sty.support;
+        }
+        float r2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) r2 += (x[d] - y[d]) * (x[d] - y[d]);
+        float kw = neighborhood_weight(r2, sup);
+        if (!(kw > 0.0f)) continue;
+        float fv = kw * pr.fixed_data[v];
+        float mv = kw * g4.w;
+        float beta = fmaf(abc.x, fv, fmaf(abc.y, mv, abc.z)) * kw * scale;
+        atomicAdd(&gamma[v].x, beta * g4.x);
+        atomicAdd(&gamma[v].y, beta * g4.y);
+        if (DIM == 3) atomicAdd(&gamma[v].z, beta * g4.z);
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+template <int DIM>
+static dim3 ltile_grid(const int *n)
+{
+    return dim3((n[0] + LTile<DIM>::X - 1) / LTile<DIM>::X, (n[1] + LTile<DIM>::Y - 1) / LTile<DIM>::Y,
+                DIM == 3 ? (n[2] + LTile<DIM>::Z - 1) / LTile<DIM>::Z : 1);
+}
+
+template <int DIM, int MODE>
+static void dense_impl(const Problem &pr, int quirk, float4 *out, hipStream_t s)
+{
+    dim3 g = ltile_grid<DIM>(pr.fixed.n);
+    if (quirk) k_dense_warp<DIM, MODE, 1><<<g, kThreads, 0, s>>>(pr, out);
+    else k_dense_warp<DIM, MODE, 0><<<g, kThreads, 0, s>>>(pr, out);
+}
+
+#define SKM_LCC_DISPATCH(pr, FN, ...)                                                                              \
+    do {                                                                                                           \
+        if ((pr).dim == 2) {                                                                                       \
+            if ((pr).kc.mode == 0) FN<2, 0>(__VA_ARGS__);                                                          \
+            else if ((pr).kc.mode == 1) FN<2, 1>(__VA_ARGS__);                                                     \
+            else FN<2, 2>(__VA_ARGS__);                                                                            \
+        } else {                                                                                                   \
+            if ((pr).kc.mode == 0) FN<3, 0>(__VA_ARGS__);                                                          \
+            else if ((pr).kc.mode == 1) FN<3, 1>(__VA_ARGS__);                                                     \
+            else FN<3, 2>(__VA_ARGS__);                                                                            \
+        }                                                                                                          \
+    } while (0)
+
+void launch_dense_warp(const Problem &pr, int quirk3d, float4 *dense_g, hipStream_t s)
+{
+    SKM_LCC_DISPATCH(pr, dense_impl, pr, quirk3d, dense_g, s);
+}
+
+template <int DIM, int MODE>
+static void centres_impl(const Problem &pr, const float4 *dense, float4 *coef, double *partials, int *blocks,
+                         hipStream_t s)
+{
+    unsigned g = (unsigned)((pr.n_sub + kThreads - 1) / kThreads);
+    *blocks = (int)g;
+    k_lcc_centres<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, partials);
+}
+
+void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre_coef, double *partials,
+                        int *blocks_out, hipStream_t s)
+{
+    SKM_LCC_DISPATCH(pr, centres_impl, pr, dense_g, centre_coef, partials, blocks_out, s);
+}
+
+template <int DIM, int MODE>
+static void gather_ref_impl(const Problem &pr, const float4 *dense, const float4 *coef, const Scalars *sc,
+                            float *deriv, hipStream_t s)
+{
+    k_lcc_gather_ref<DIM, MODE><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, dense, coef, sc, deriv);
+}
+
+void launch_lcc_gather_reference(const Problem &pr, const float4 *dense_g, const float4 *centre_coef,
+                                 const Scalars *sc, float *deriv_soa, hipStream_t s)
+{
+    SKM_LCC_DISPATCH(pr, gather_ref_impl, pr, dense_g, centre_coef, sc, deriv_soa, s);
+}
+
+template <int DIM, int MODE>
+static void scatter_impl(const Problem &pr, const float4 *dense, const float4 *coef, const Scalars *sc,
+                         float4 *gamma, hipStream_t s)
+{
+    unsigned g = (unsigned)((pr.n_sub + 3) / 4);
+    k_lcc_scatter<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, sc, gamma);
+}
+
+void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
+                        float4 *gamma, hipStream_t s)
+{
+    SKM_LCC_DISPATCH(pr, scatter_impl, pr, dense_g, centre_coef, sc, gamma, s);
+}
+
+}  // namespace skm

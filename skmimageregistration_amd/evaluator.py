@@ -1,0 +1,270 @@
+"""Host-side mirror of the reference's GpuKernelEvaluator interface over libskm.so.
+
+Names, argument meaning and order follow include/cuda/GpuEvaluator.h:138-178
+and include/cuda/Kernel.h:134-146 of the reference; arrays are numpy.  Images
+are numpy arrays in C order with x fastest (shape (ny, nx) or (nz, ny, nx)),
+geometry tuples are in (x, y[, z]) order.
+"""
+import ctypes
+import enum
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, f32, fptr, make_grid
+
+
+class MEASURE(enum.IntEnum):
+    """GpuKernelEvaluator::MEASURE (GpuEvaluator.h:160)."""
+    MSE = _capi.SKM_MEASURE_MSE
+    NCC = _capi.SKM_MEASURE_NCC
+    LCC = _capi.SKM_MEASURE_LCC
+
+
+class Image:
+    """An image with geometry: origin + spacing * index (ImageBase.h:25-57)."""
+
+    def __init__(self, data, origin=None, spacing=None, dim=None):
+        self.data = np.asarray(data)
+        # vector-valued images (a displacement field) pass dim; the trailing
+        # axis then holds the components
+        self.dim = int(dim) if dim is not None else self.data.ndim
+        self.origin = tuple(float(v) for v in (origin if origin is not None else (0.0,) * self.dim))
+        self.spacing = tuple(float(v) for v in (spacing if spacing is not None else (1.0,) * self.dim))
+
+    def size(self):
+        return tuple(reversed(self.data.shape[: self.dim]))
+
+    def grid(self):
+        return make_grid(self.size(), self.origin, self.spacing)
+
+
+class Grid:
+    """Geometry without data (the control-point image of the reference)."""
+
+    def __init__(self, size, origin, spacing):
+        self.size_ = tuple(int(v) for v in size)
+        self.dim = len(self.size_)
+        self.origin = tuple(float(v) for v in origin)
+        self.spacing = tuple(float(v) for v in spacing)
+
+    def size(self):
+        return self.size_
+
+    def grid(self):
+        return make_grid(self.size_, self.origin, self.spacing)
+
+    def count(self):
+        return int(np.prod(self.size_))
+
+
+def image_geometry(size, origin, spacing):
+    return Grid(size, origin, spacing)
+
+
+class Kernel:
+    """GKernel construction arguments (Kernel.h:134-146).
+
+    wImage: Image of scalars (useWeightImage); wTensor: (grid Grid, data
+    array of shape (..., D, D)) for useWeightTensor.
+    """
+
+    def __init__(self, sigmaC0, sigmaC4, useWeightImage=False, maximumWeight=1.0, wImage=None,
+                 useWeightTensor=False, maximumEigenvalueCovariance=None, wTensor=None):
+        self.sigmaC0 = float(sigmaC0)
+        self.sigmaC4 = float(sigmaC4)
+        self.useWeightImage = bool(useWeightImage)
+        self.maximumWeight = float(maximumWeight)
+        self.wImage = wImage
+        self.useWeightTensor = bool(useWeightTensor)
+        self.maximumEigenvalueCovariance = maximumEigenvalueCovariance
+        self.wTensor = wTensor
+
+
+class EvaluationResult:
+    """GpuKernelEvaluator::EvaluationResult (GpuEvaluator.h:138-145)."""
+
+    def __init__(self, r):
+        self.measure = r.measure
+        self.reg_value_rkhs = r.reg_value_rkhs
+        self.reg_value_rd = r.reg_value_rd
+        self.reg_value_pg = r.reg_value_pg
+        self.numPixelsCountedGpu = r.num_pixels_counted
+
+    def __repr__(self):
+        return (f"EvaluationResult(measure={self.measure!r}, rkhs={self.reg_value_rkhs!r}, "
+                f"rd={self.reg_value_rd!r}, pg={self.reg_value_pg!r}, "
+                f"numPixelsCountedGpu={self.numPixelsCountedGpu})")
+
+
+class GpuKernelEvaluator:
+    """Drop-in for GpuKernelEvaluator (GpuEvaluator.h:69-179) on libskm.so.
+
+    movingImage holds raw intensities; the cubic B-spline prefilter that the
+    reference runs in TransformAdapter::Initialize (itkTransformAdapter.hxx:
+    435-439) runs inside the library.  cpImage is a Grid (geometry of the
+    control-point image).  exact_gradient selects derivative windows that
+    cover the whole kernel support.
+    """
+
+    def __init__(self, numParameters, subsample, subsampleNeighborhood, kernel, movingImage, fixedImage,
+                 cpImage, cpwImage=None, exact_gradient=False, rng_state=None, rng_sequence=None):
+        self._lib = _capi.load()
+        dim = fixedImage.dim
+        self.dim = dim
+        self.cp_grid = cpImage
+        self.n_cp = cpImage.count()
+        if numParameters != self.n_cp * dim:
+            raise ValueError("numParameters does not match the control-point grid")
+        desc = _capi.SkmDesc()
+        self._lib.skm_desc_init(ctypes.byref(desc))
+        desc.dim = dim
+        self._fixed = f32(fixedImage.data)
+        self._moving = f32(movingImage.data)
+        desc.fixed = fixedImage.grid()
+        desc.fixed_data = fptr(self._fixed)
+        desc.moving = movingImage.grid()
+        desc.moving_data = fptr(self._moving)
+        desc.cp = cpImage.grid()
+        desc.sigma_c0 = kernel.sigmaC0
+        desc.sigma_c4 = kernel.sigmaC4
+        desc.maximum_weight = kernel.maximumWeight
+        self._weight = None
+        if kernel.useWeightImage:
+            desc.kernel_mode = _capi.SKM_KERNEL_WEIGHT_IMAGE
+            self._weight = f32(kernel.wImage.data)
+            desc.weight = kernel.wImage.grid()
+            desc.weight_data = fptr(self._weight)
+        elif kernel.useWeightTensor:
+            desc.kernel_mode = _capi.SKM_KERNEL_WEIGHT_TENSOR
+            tgrid, tdata = kernel.wTensor
+            self._weight = f32(tdata)
+            desc.weight = tgrid.grid()
+            desc.weight_data = fptr(self._weight)
+        if kernel.maximumEigenvalueCovariance is not None:
+            m = np.asarray(kernel.maximumEigenvalueCovariance, dtype=np.float64).reshape(dim, dim)
+            for i in range(9):
+                desc.maximum_eigenvalue_covariance[i] = 0.0
+            for i in range(dim):
+                for k in range(dim):
+                    desc.maximum_eigenvalue_covariance[i * dim + k] = m[i, k]
+        desc.subsample = int(subsample)
+        desc.subsample_neighborhood = int(subsampleNeighborhood)
+        if rng_state is not None:
+            desc.rng_state = int(rng_state)
+        if rng_sequence is not None:
+            desc.rng_sequence = int(rng_sequence)
+        desc.flags = _capi.SKM_FLAG_EXACT_GRADIENT if exact_gradient else 0
+        handle = ctypes.c_void_p()
+        check(self._lib.skm_create(ctypes.byref(desc), ctypes.byref(handle)))
+        self._h = handle
+        self._df_key = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.skm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- GpuEvaluator.h:156-158
+    def SetRegularizerRKHS(self, weight):
+        check(self._lib.skm_set_regularizer_rkhs(self._h, float(weight)))
+
+    def SetRegularizerRD(self, weight, scaling=1.0):
+        check(self._lib.skm_set_regularizer_rd(self._h, float(weight), float(scaling)))
+
+    def SetRegularizerPG(self, weight, scaling=1.0):
+        check(self._lib.skm_set_regularizer_pg(self._h, float(weight), float(scaling)))
+
+    def _set_df(self, displacementField):
+        """The previous-scale field is uploaded when it changes (the reference
+        keys its device copy on the host pointer, GpuEvaluator.cu:120-126)."""
+        key = None if displacementField is None else id(displacementField.data)
+        if key == self._df_key:
+            return
+        if displacementField is None:
+            check(self._lib.skm_set_displacement_field(self._h, None, None))
+        else:
+            self._df = f32(displacementField.data)
+            g = displacementField.grid()
+            check(self._lib.skm_set_displacement_field(self._h, ctypes.byref(g), fptr(self._df)))
+        self._df_key = key
+
+    def _cp(self, cpData, cpwData):
+        cp = f32(cpData).reshape(self.n_cp, self.dim)
+        cpw = f32(cpwData if cpwData is not None else np.ones(self.n_cp, np.float32)).reshape(self.n_cp)
+        return cp, cpw
+
+    # -- GpuEvaluator.h:162-166
+    def getValue(self, metric, cpData, cpwData=None, displacementField=None, do_resampling=True):
+        cp, cpw = self._cp(cpData, cpwData)
+        self._set_df(displacementField)
+        r = _capi.SkmResult()
+        check(self._lib.skm_get_value(self._h, int(metric), fptr(cp), fptr(cpw), int(bool(do_resampling)),
+                                      ctypes.byref(r)))
+        return EvaluationResult(r)
+
+    # -- GpuEvaluator.h:168-173; returns (result, derivative SoA)
+    def getValueAndDerivative(self, metric, cpData, cpwData=None, displacementField=None, derivatives=None,
+                              do_resampling=True):
+        cp, cpw = self._cp(cpData, cpwData)
+        self._set_df(displacementField)
+        out = derivatives if derivatives is not None else np.empty(self.n_cp * self.dim, np.float32)
+        if out.dtype != np.float32 or not out.flags["C_CONTIGUOUS"] or out.size != self.n_cp * self.dim:
+            raise ValueError("derivatives must be a contiguous float32 array of numParameters values")
+        r = _capi.SkmResult()
+        check(self._lib.skm_get_value_and_derivative(self._h, int(metric), fptr(cp), fptr(cpw), fptr(out),
+                                                     int(bool(do_resampling)), ctypes.byref(r)))
+        return EvaluationResult(r), out
+
+    # -- GpuEvaluator.h:175-178; returns the AoS field (N_vox, D)
+    def evaluateDisplacementField(self, cpData, cpwData=None, displacementField=None):
+        cp, cpw = self._cp(cpData, cpwData)
+        self._set_df(displacementField)
+        n = int(np.prod(self._fixed.shape))
+        dst = np.empty((n, self.dim), np.float32)
+        check(self._lib.skm_evaluate_displacement_field(self._h, fptr(cp), fptr(cpw), fptr(dst)))
+        return dst
+
+    def cube_offsets(self):
+        n = ctypes.c_int64()
+        check(self._lib.skm_get_cube_offsets(self._h, None, ctypes.byref(n)))
+        out = np.empty((n.value, self.dim), np.int32)
+        check(self._lib.skm_get_cube_offsets(self._h, out.ctypes.data_as(_capi.c_int32_p), None))
+        return out
+
+    # -- device-resident optimisation loop
+    def optimize(self, params_soa, metric=MEASURE.MSE, reg_l1=0.0, reg_l21=0.0, reg_l2=0.0,
+                 use_negative_gradient=True, do_resampling=True, kind=_capi.SKM_OPT_ADAPTIVE, a=0.1, A=20.0,
+                 alpha=1.0, fmax=1e7, fmin=-1e6, omega=1.0, orthant_projection=False, iterations=100,
+                 cpwData=None):
+        params = f32(params_soa).copy()
+        cpw = f32(cpwData if cpwData is not None else np.ones(self.n_cp, np.float32))
+        mo = _capi.SkmMetricOptions(int(metric), reg_l1, reg_l21, reg_l2, int(bool(use_negative_gradient)),
+                                    int(bool(do_resampling)))
+        oo = _capi.SkmOptimizerOptions(int(kind), a, A, alpha, fmax, fmin, omega, int(bool(orthant_projection)),
+                                       int(iterations))
+        recs = (_capi.SkmIterationRecord * max(1, iterations))()
+        check(self._lib.skm_optimize(self._h, ctypes.byref(mo), ctypes.byref(oo), fptr(cpw), fptr(params), recs))
+        records = [(r.value, r.measure, r.learning_rate, r.num_pixels_counted) for r in recs[:iterations]]
+        return params, records
+
+
+def control_point_grid(image, spacing):
+    """TransformAdapter::ComputeControlPointImage geometry (itkTransformAdapter.hxx:246-308)."""
+    lib = _capi.load()
+    dim = image.dim
+    g_in = image.grid()
+    sp = (ctypes.c_double * 3)(*([float(s) for s in spacing] + [1.0] * (3 - len(spacing))))
+    out = _capi.SkmGrid()
+    check(lib.skm_control_point_grid(dim, ctypes.byref(g_in), sp, ctypes.byref(out)))
+    return Grid(tuple(out.size[:dim]), tuple(out.origin[:dim]), tuple(out.spacing[:dim]))

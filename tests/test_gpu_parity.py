@@ -1,0 +1,138 @@
+"""HIP path (libskm.so, through the C ABI) against the FP64 oracle on small inputs.
+
+Tolerances (FP32 per pair, FP64 final sums):
+  metric value       relative 1e-5
+  derivative         max-norm relative 1e-4  (max|d_gpu - d_oracle| / max|d_oracle|)
+  regularizer values relative 1e-5
+  displacement       max-norm relative 1e-5
+"""
+import numpy as np
+import pytest
+
+from helpers import Case, rel_err, scalar_rel
+from skmimageregistration_amd import MEASURE
+
+pytestmark = pytest.mark.gpu
+
+VALUE_TOL = 1e-5
+DERIV_TOL = 1e-4
+
+
+def _compare(case, measure, regs=(0.0, 0.0, 0.0), pg_scaling=1.0, calls=1, value_only=False):
+    ev = case.gpu()
+    if regs[0] > 0:
+        ev.SetRegularizerRKHS(regs[0])
+    if regs[1] > 0:
+        ev.SetRegularizerRD(regs[1])
+    if regs[2] > 0:
+        ev.SetRegularizerPG(regs[2], pg_scaling)
+    df = case.df_image()
+    for _ in range(calls - 1):
+        ev.getValue(measure, case.cp, case.cpw, df, do_resampling=True)
+    if value_only:
+        res = ev.getValue(measure, case.cp, case.cpw, df, do_resampling=True)
+        der = None
+    else:
+        res, der = ev.getValueAndDerivative(measure, case.cp, case.cpw, df, do_resampling=True)
+    offsets = None
+    if case.sub > 1:
+        offsets = case.offsets(calls)
+        np.testing.assert_array_equal(ev.cube_offsets(), offsets)  # bit-exact PCG32 jitter
+    orc = case.oracle(offsets)
+    ores, oder = orc.evaluate(int(measure), not value_only, *regs, pg_scaling)
+    assert res.numPixelsCountedGpu == ores["pixels_counted"]
+    assert scalar_rel(res.measure, ores["measure"]) < VALUE_TOL, (res.measure, ores["measure"])
+    for name, gv in (("reg_rkhs", res.reg_value_rkhs), ("reg_rd", res.reg_value_rd), ("reg_pg", res.reg_value_pg)):
+        assert scalar_rel(gv, ores[name]) < VALUE_TOL, (name, gv, ores[name])
+    if der is not None:
+        assert rel_err(der, oder) < DERIV_TOL, rel_err(der, oder)
+    ev.close()
+
+
+@pytest.mark.parametrize("measure", [MEASURE.MSE, MEASURE.NCC])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_2d_modes(measure, mode):
+    _compare(Case(dim=2, mode=mode, seed=3), measure)
+
+
+@pytest.mark.parametrize("measure", [MEASURE.MSE, MEASURE.NCC])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_3d_modes(measure, mode):
+    _compare(Case(dim=3, shape=(18, 20, 22), cp_spacing=5.0, sigma=12.0, mode=mode, seed=5), measure)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("sub", [2, 3])
+@pytest.mark.parametrize("measure", [MEASURE.MSE, MEASURE.NCC])
+def test_subsampled(dim, sub, measure):
+    shape = (48, 52) if dim == 2 else (18, 20, 22)
+    _compare(Case(dim=dim, shape=shape, cp_spacing=5.0, sigma=12.0, subsample=sub, seed=11), measure, calls=2)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_exact_windows(dim):
+    shape = (48, 52) if dim == 2 else (18, 20, 22)
+    _compare(Case(dim=dim, shape=shape, cp_spacing=5.0, sigma=12.0, subsample=2, exact=True, seed=13), MEASURE.MSE)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_regularizers(dim, mode):
+    shape = (40, 44) if dim == 2 else (16, 18, 20)
+    case = Case(dim=dim, shape=shape, cp_spacing=4.0, sigma=10.0, mode=mode, seed=17, coef_scale=0.8)
+    _compare(case, MEASURE.MSE, regs=(0.3, 0.2, 0.1), pg_scaling=0.7)
+    _compare(case, MEASURE.NCC, regs=(0.3, 0.2, 0.1), pg_scaling=1.3, value_only=True)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_previous_displacement(dim):
+    shape = (40, 44) if dim == 2 else (16, 18, 20)
+    _compare(Case(dim=dim, shape=shape, cp_spacing=5.0, sigma=12.0, with_df=True, seed=19), MEASURE.NCC)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_geometry(dim):
+    origin = (-3.5, 2.25, 1.0)[:dim]
+    spacing = (1.25, 0.8, 1.5)[:dim]
+    shape = (36, 40) if dim == 2 else (16, 18, 20)
+    _compare(Case(dim=dim, shape=shape, cp_spacing=6.0, sigma=14.0, origin=origin, spacing=spacing, seed=23),
+             MEASURE.MSE)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_displacement_field(dim, mode):
+    shape = (40, 44) if dim == 2 else (16, 18, 20)
+    case = Case(dim=dim, shape=shape, cp_spacing=5.0, sigma=12.0, mode=mode, with_df=True, seed=29)
+    ev = case.gpu()
+    u = ev.evaluateDisplacementField(case.cp, case.cpw, case.df_image())
+    ref = case.oracle().displacement()
+    assert rel_err(u, ref) < 1e-5
+    ev.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("exact", [False, True])
+def test_lcc(dim, exact):
+    shape = (28, 30) if dim == 2 else (12, 13, 14)
+    case = Case(dim=dim, shape=shape, cp_spacing=4.0, sigma=7.0, subsample=2, subsample_neighborhood=2,
+                exact=exact, seed=31, coef_scale=0.8)
+    _compare(case, MEASURE.LCC)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_lcc_weighted(mode):
+    case = Case(dim=3, shape=(12, 13, 14), cp_spacing=4.0, sigma=7.0, mode=mode, subsample=2,
+                subsample_neighborhood=2, seed=37, coef_scale=0.8)
+    _compare(case, MEASURE.LCC)
+
+
+def test_value_matches_value_and_derivative():
+    """GetValue == GetValueAndDerivative value (itkFiniteDifferenceTest.cpp:188-195)."""
+    case = Case(dim=3, shape=(18, 20, 22), cp_spacing=5.0, sigma=12.0, subsample=2, seed=41)
+    ev = case.gpu()
+    for measure in (MEASURE.MSE, MEASURE.NCC, MEASURE.LCC):
+        r1, _ = ev.getValueAndDerivative(measure, case.cp, case.cpw, do_resampling=False)
+        r2 = ev.getValue(measure, case.cp, case.cpw, do_resampling=False)
+        assert r1.measure == r2.measure
+    ev.close()
