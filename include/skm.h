@@ -145,6 +145,21 @@ SKM_API skm_status skm_evaluate_displacement_field_device(skm_evaluator *ev, con
                                                           const float *d_cpw, float *d_dst_aos);
 SKM_API skm_status skm_synchronize(skm_evaluator *ev);
 
+/* Device time per pass, accumulated with hipEvents on the evaluator's stream
+ * while timing is enabled (no counterpart in the reference, which prints host
+ * wall-clock only, itkIterationObserver.h:59-63).  Passes: 0 jitter offsets,
+ * 1 value, 2 reduction, 3 derivative preparation, 4 derivative gather,
+ * 5 regularizers, 6 LCC dense warp, 7 LCC centres, 8 LCC derivative,
+ * 9 displacement field. */
+#define SKM_NUM_PASSES 10
+typedef struct skm_pass_times {
+    double ms[SKM_NUM_PASSES];
+    uint32_t launches[SKM_NUM_PASSES];
+} skm_pass_times;
+SKM_API skm_status skm_enable_timing(skm_evaluator *ev, int32_t enable);
+/* Synchronises the stream, returns the totals since the last read and resets them. */
+SKM_API skm_status skm_read_timing(skm_evaluator *ev, skm_pass_times *out);
+
 /* Jitter offsets of the last evaluation, D ints per subsampled cell
  * (GpuEvaluator.cu:107-118); n_cells receives prod(fixed.size / subsample). */
 SKM_API skm_status skm_get_cube_offsets(skm_evaluator *ev, int32_t *out, int64_t *n_cells);

@@ -62,6 +62,15 @@ class SkmOptimizerOptions(ctypes.Structure):
                 ("iterations", ctypes.c_int32)]
 
 
+SKM_NUM_PASSES = 10
+PASS_NAMES = ("offsets", "value", "reduce", "prepare", "gather", "regularizers", "lcc_dense", "lcc_centres",
+              "lcc_derivative", "displacement")
+
+
+class SkmPassTimes(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * SKM_NUM_PASSES), ("launches", ctypes.c_uint32 * SKM_NUM_PASSES)]
+
+
 class SkmIterationRecord(ctypes.Structure):
     _fields_ = [("value", ctypes.c_double), ("measure", ctypes.c_double), ("learning_rate", ctypes.c_double),
                 ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
@@ -89,6 +98,8 @@ _SIGNATURES = {
     "skm_evaluate_displacement_field_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                               ctypes.c_void_p]),
     "skm_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "skm_enable_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "skm_read_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmPassTimes)]),
     "skm_get_cube_offsets": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, ctypes.POINTER(ctypes.c_int64)]),
     "skm_control_point_grid": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_double_p,
                                               ctypes.POINTER(SkmGrid)]),

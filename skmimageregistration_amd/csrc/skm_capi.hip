@@ -175,6 +175,14 @@ struct skm_evaluator {
 
     uint64_t rng_state = 0, rng_inc = 1;
     bool evaluated_once = false;
+
+    // pass timing: events recorded around each pass while enabled
+    bool timing = false;
+    struct Pending { int pass; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double pass_ms[SKM_NUM_PASSES] = {0};
+    uint32_t pass_launches[SKM_NUM_PASSES] = {0};
     double reg_w[3] = {0, 0, 0};
     double rd_scaling = 1.0, pg_scaling = 1.0;
 
@@ -186,11 +194,46 @@ struct skm_evaluator {
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_opt, d_records};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
+        for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+        for (auto e : event_pool) (void)hipEventDestroy(e);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
 
 namespace {
+
+hipEvent_t take_event(skm_evaluator *ev)
+{
+    if (!ev->event_pool.empty()) {
+        hipEvent_t e = ev->event_pool.back();
+        ev->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets the launches of one pass with events when timing is on.
+struct PassTimer {
+    skm_evaluator *ev;
+    int pass;
+    hipEvent_t a = nullptr;
+    PassTimer(skm_evaluator *e, int p) : ev(e), pass(p)
+    {
+        if (!ev->timing) return;
+        a = take_event(ev);
+        if (a) (void)hipEventRecord(a, ev->stream);
+    }
+    ~PassTimer()
+    {
+        if (!ev->timing || !a) return;
+        hipEvent_t b = take_event(ev);
+        if (!b) return;
+        (void)hipEventRecord(b, ev->stream);
+        ev->pending.push_back({pass, a, b});
+    }
+};
 
 template <typename T>
 skm_status dev_alloc(T **p, size_t count)
@@ -317,24 +360,26 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
     Problem &pr = ev->pr;
     hipStream_t s = ev->stream;
     if (pr.sub > 1 && (!ev->evaluated_once || do_resampling)) {
+        PassTimer t(ev, 0);
         launch_offsets(pr, ev->d_offsets, ev->rng_state, ev->rng_inc, s);
         ev->rng_state = lcg_advance(ev->rng_state, (uint64_t)pr.n_sub * pr.dim, ev->rng_inc);
         ev->evaluated_once = true;
     }
     int blocks = 0;
     if (measure == SKM_MEASURE_MSE || measure == SKM_MEASURE_NCC) {
-        launch_value(pr, measure, ev->d_sg, ev->d_sb, ev->d_partials, &blocks, s);
-        launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s);
+        { PassTimer t(ev, 1); launch_value(pr, measure, ev->d_sg, ev->d_sb, ev->d_partials, &blocks, s); }
+        { PassTimer t(ev, 2); launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s); }
         if (deriv) {
-            launch_prepare(pr, measure, ev->d_sg, ev->d_sb, ev->d_sc, ev->d_sv, s);
-            launch_gather(pr, ev->d_sv, deriv, s);
+            { PassTimer t(ev, 3); launch_prepare(pr, measure, ev->d_sg, ev->d_sb, ev->d_sc, ev->d_sv, s); }
+            { PassTimer t(ev, 4); launch_gather(pr, ev->d_sv, deriv, s); }
         }
     } else if (measure == SKM_MEASURE_LCC) {
         SKM_TRY(ensure_lcc_buffers(ev));
-        launch_dense_warp(pr, 0, ev->d_dense, s);
-        launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s);
-        launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s);
+        { PassTimer t(ev, 6); launch_dense_warp(pr, 0, ev->d_dense, s); }
+        { PassTimer t(ev, 7); launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s); }
+        { PassTimer t(ev, 2); launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s); }
         if (deriv) {
+            PassTimer t(ev, 8);
             if (pr.exact) {
                 if (!ev->d_gamma) SKM_TRY(dev_alloc(&ev->d_gamma, pr.n_vox));
                 SKM_HIP(hipMemsetAsync(ev->d_gamma, 0, sizeof(float4) * pr.n_vox, s));
@@ -364,8 +409,10 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
     } else {
         return fail(SKM_ERROR_INVALID_ARGUMENT, "unknown measure");
     }
-    if (ev->reg_w[0] > 0 || ev->reg_w[1] > 0 || ev->reg_w[2] > 0)
+    if (ev->reg_w[0] > 0 || ev->reg_w[1] > 0 || ev->reg_w[2] > 0) {
+        PassTimer t(ev, 5);
         launch_regularizers(pr, ev->reg_w, ev->pg_scaling, deriv != nullptr, deriv, ev->d_reg_partials, ev->d_sc, s);
+    }
     SKM_HIP(hipGetLastError());
     return SKM_SUCCESS;
 }
@@ -664,7 +711,10 @@ SKM_API skm_status skm_evaluate_displacement_field(skm_evaluator *ev, const floa
     if (!ev || !dst_aos) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
     SKM_TRY(upload_cp(ev, cp_aos, cpw));
     if (!ev->d_dst) SKM_TRY(dev_alloc(&ev->d_dst, ev->pr.n_vox * ev->dim));
-    launch_displacement(ev->pr, ev->d_dst, ev->stream);
+    {
+        PassTimer t(ev, 9);
+        launch_displacement(ev->pr, ev->d_dst, ev->stream);
+    }
     SKM_HIP(hipGetLastError());
     SKM_HIP(hipMemcpyAsync(dst_aos, ev->d_dst, sizeof(float) * ev->pr.n_vox * ev->dim, hipMemcpyDeviceToHost,
                            ev->stream));
@@ -677,8 +727,40 @@ SKM_API skm_status skm_evaluate_displacement_field_device(skm_evaluator *ev, con
 {
     if (!ev || !d_cp_aos || !d_cpw || !d_dst_aos) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
     launch_pack_cp(ev->pr, d_cp_aos, d_cpw, 0, ev->d_cpc, ev->stream);
-    launch_displacement(ev->pr, d_dst_aos, ev->stream);
+    {
+        PassTimer t(ev, 9);
+        launch_displacement(ev->pr, d_dst_aos, ev->stream);
+    }
     SKM_HIP(hipGetLastError());
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_enable_timing(skm_evaluator *ev, int32_t enable)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    ev->timing = enable != 0;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_read_timing(skm_evaluator *ev, skm_pass_times *out)
+{
+    if (!ev || !out) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    SKM_HIP(hipStreamSynchronize(ev->stream));
+    for (auto &p : ev->pending) {
+        float ms = 0.0f;
+        SKM_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        ev->pass_ms[p.pass] += ms;
+        ev->pass_launches[p.pass] += 1;
+        ev->event_pool.push_back(p.a);
+        ev->event_pool.push_back(p.b);
+    }
+    ev->pending.clear();
+    for (int i = 0; i < SKM_NUM_PASSES; ++i) {
+        out->ms[i] = ev->pass_ms[i];
+        out->launches[i] = ev->pass_launches[i];
+        ev->pass_ms[i] = 0.0;
+        ev->pass_launches[i] = 0;
+    }
     return SKM_SUCCESS;
 }
 

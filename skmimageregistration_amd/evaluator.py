@@ -235,6 +235,28 @@ class GpuKernelEvaluator:
         check(self._lib.skm_evaluate_displacement_field(self._h, fptr(cp), fptr(cpw), fptr(dst)))
         return dst
 
+    def enable_timing(self, enable=True):
+        check(self._lib.skm_enable_timing(self._h, int(bool(enable))))
+
+    def read_timing(self):
+        """{pass name: (ms, launches)} accumulated since the last read."""
+        t = _capi.SkmPassTimes()
+        check(self._lib.skm_read_timing(self._h, ctypes.byref(t)))
+        return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(_capi.PASS_NAMES)}
+
+    def set_stream(self, stream_handle):
+        check(self._lib.skm_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+
+    def get_value_and_derivative_device(self, metric, d_cp_aos, d_cpw, d_derivative, do_resampling=True,
+                                        d_result=None):
+        """Device pointers (ints); enqueued on the evaluator's stream."""
+        check(self._lib.skm_get_value_and_derivative_device(
+            self._h, int(metric), ctypes.c_void_p(d_cp_aos), ctypes.c_void_p(d_cpw), ctypes.c_void_p(d_derivative),
+            int(bool(do_resampling)), ctypes.c_void_p(d_result) if d_result else None))
+
+    def synchronize(self):
+        check(self._lib.skm_synchronize(self._h))
+
     def cube_offsets(self):
         n = ctypes.c_int64()
         check(self._lib.skm_get_cube_offsets(self._h, None, ctypes.byref(n)))
