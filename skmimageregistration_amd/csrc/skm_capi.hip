@@ -394,7 +394,7 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
                 const float4 *dense = ev->d_dense;
                 Problem lp = pr;
                 lp.cp_window = ev->d_lcc_window;
-                if (pr.dim == 3 && pr.kc.mode != SKM_KERNEL_STATIONARY) {
+                if (pr.dim == 3 && !is_stationary(pr.kc.mode)) {
                     // the 3D derivative kernel warps neighbours with sigma(y) twice
                     if (!ev->d_dense_q) SKM_TRY(dev_alloc(&ev->d_dense_q, pr.n_vox));
                     launch_dense_warp(pr, 1, ev->d_dense_q, s);
@@ -534,6 +534,8 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
     pr.weight = to_grid(desc->weight, dim);
     pr.df = to_grid(desc->fixed, dim);
     pr.kc.mode = desc->kernel_mode;
+    if (desc->kernel_mode == SKM_KERNEL_STATIONARY && desc->sigma_c0 == desc->sigma_c4)
+        pr.kc.mode = kModeStationaryEqual;  // C4 * C0 share one (1 - t) factor
     pr.kc.inv_s4 = (float)(1.0 / desc->sigma_c4);
     pr.kc.inv_s0 = (float)(1.0 / desc->sigma_c0);
     pr.kc.smin = (float)std::min(desc->sigma_c4, desc->sigma_c0);
@@ -580,7 +582,7 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
         (st = dev_alloc(&ev->d_cpw_in, pr.n_cp)) || (st = dev_alloc(&ev->d_deriv, pr.n_cp * dim)) ||
         (st = dev_alloc(&ev->d_sg, pr.n_sub)) || (st = dev_alloc(&ev->d_sb, pr.n_sub)) ||
         (st = dev_alloc(&ev->d_sv, pr.n_sub)) || (st = dev_alloc(&ev->d_partials, (size_t)tiles * kPartialsK)) ||
-        (st = dev_alloc(&ev->d_reg_partials, (size_t)((pr.n_cp + kThreads - 1) / kThreads) * 4 + 4)) ||
+        (st = dev_alloc(&ev->d_reg_partials, (size_t)reg_blocks(pr.n_cp) * 4 + 4)) ||
         (st = dev_alloc(&ev->d_hreg_partials, (size_t)((pr.n_cp + kThreads - 1) / kThreads) * 4 + 4)) ||
         (st = dev_alloc(&ev->d_dot_partials, 64)) || (st = dev_alloc(&ev->d_sc, 1)) ||
         (st = dev_alloc(&ev->d_offsets, pr.sub > 1 ? pr.n_sub * dim : 1)))

@@ -75,12 +75,37 @@ __device__ __forceinline__ bool nearest(const Grid &g, const float *p, long long
 
 // ---------------------------------------------------------------- kernels
 
+// Kernel modes inside the library: the three reference modes plus mode 3,
+// the stationary kernel with sigma_C4 == sigma_C0, where C4 * C0 collapses
+// to (1-t)^8 (1 + 6t + 35/3 t^2).
+constexpr int kModeStationaryEqual = 3;
+__host__ __device__ constexpr bool is_stationary(int mode) { return mode == 0 || mode == kModeStationaryEqual; }
+
+// Hardware square root (v_sqrt_f32, ~1 ulp); the IEEE-rounded sqrtf expands
+// to a dozen instructions per pair.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Wendland C4 with t = r / sigma_C4 (Kernel.h:72-77): (1-t)_+^6 (3 + 18t + 35t^2) / 3.
 __device__ __forceinline__ float wendland_c4(float t)
 {
     float f = fmaxf(0.0f, 1.0f - t);
     float f2 = f * f;
-    return f2 * f2 * f2 * fmaf(fmaf(35.0f, t, 18.0f), t, 3.0f) * (1.0f / 3.0f);
+    return f2 * f2 * f2 * fmaf(fmaf(35.0f / 3.0f, t, 6.0f), t, 1.0f);
+}
+
+// Stationary k(r) from r^2 (Kernel.h:96-103, 173-176).
+template <int MODE>
+__device__ __forceinline__ float stationary_k(float r2, float inv_s4, float inv_s0)
+{
+    float r = fast_sqrt(r2);
+    float t = r * inv_s4;
+    float f = fmaxf(0.0f, fmaf(-inv_s4, r, 1.0f));
+    float f2 = f * f;
+    float f4 = f2 * f2;
+    float poly = fmaf(fmaf(35.0f / 3.0f, t, 6.0f), t, 1.0f);
+    if (MODE == kModeStationaryEqual) return f4 * f4 * poly;
+    float g = fmaxf(0.0f, fmaf(-inv_s0, r, 1.0f));
+    return f4 * f2 * poly * (g * g);
 }
 
 // Wendland C0 (Kernel.h:48-53): (1-t)_+^2.
@@ -125,7 +150,7 @@ template <int DIM, int MODE>
 __device__ __forceinline__ void point_state(const KernelConst &kc, const Grid &wg, const float *wdata,
                                             const float *p, PState &st)
 {
-    if (MODE == 0) {
+    if (is_stationary(MODE)) {
         st.support = kc.smin;
         return;
     }
@@ -155,13 +180,12 @@ __device__ __forceinline__ float kernel_pair(const KernelConst &kc, const float 
 {
     float r2 = v[0] * v[0] + v[1] * v[1];
     if (DIM == 3) r2 = fmaf(v[2], v[2], r2);
-    if (MODE == 0) {
-        float r = sqrtf(r2);
-        return wendland_c4(r * kc.inv_s4) * wendland_c0(r * kc.inv_s0);
+    if (is_stationary(MODE)) {
+        return stationary_k<MODE>(r2, kc.inv_s4, kc.inv_s0);
     } else if (MODE == 1) {
         float sum = a.sig + b.sig;
         float F2 = a.sdet * b.sdet / fabsf(0.5f * sum);
-        float rs = sqrtf(2.0f / sum) * sqrtf(r2);
+        float rs = fast_sqrt(2.0f / sum * r2);
         return F2 * wendland_c4(rs * kc.inv_s4) * wendland_c0(rs * kc.inv_s0);
     } else {
         float m[9];
@@ -183,7 +207,7 @@ __device__ __forceinline__ float kernel_pair(const KernelConst &kc, const float 
             quad = w0 * v[0] + w1 * v[1] + w2 * v[2];
         }
         float F2 = a.sdet * b.sdet / det;
-        float rs = sqrtf(quad / det);
+        float rs = fast_sqrt(quad / det);
         return F2 * wendland_c4(rs * kc.inv_s4) * wendland_c0(rs * kc.inv_s0);
     }
 }

@@ -56,6 +56,7 @@ void launch_prepare(const Problem &pr, int measure, const float4 *sg, const floa
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s);
 void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, int with_derivative,
                          float *deriv_soa, double *partials, Scalars *sc, hipStream_t s);
+unsigned reg_blocks(long long n_cp);  // workgroups (and 4-double partial slots) of launch_regularizers
 void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
 void launch_result(const Scalars *sc, void *result, hipStream_t s);
 

@@ -84,7 +84,7 @@ __device__ __forceinline__ void df_lookup(const Problem &pr, const float *x, flo
 template <int DIM, int MODE>
 __device__ __forceinline__ void cp_state_load(const Problem &pr, long long j, PState &st)
 {
-    if (MODE == 0) return;
+    if (is_stationary(MODE)) return;
     const float *c = pr.cp_state + j * kCpStateStride;
     st.sig = c[0];
     st.sdet = c[1];
@@ -117,10 +117,25 @@ __device__ __forceinline__ void kernel_sum(const Problem &pr, const float *x, co
             float dy = x[1] - (pr.cp.o[1] + pr.cp.h[1] * cy);
             float rem = s2 - dy * dy - dz * dz;
             if (rem <= 0.0f) continue;
-            float rx = sqrtf(rem) * pr.cp.ih[0];
-            int xa = max(lo[0], (int)floorf(ci[0] - rx));
-            int xb = min(hi[0], (int)ceilf(ci[0] + rx));
+            // lo >= 0 makes truncation act as floor; +1 covers the ceiling
+            float rx = fast_sqrt(rem) * pr.cp.ih[0];
+            int xa = max(lo[0], (int)(ci[0] - rx));
+            int xb = min(hi[0], (int)(ci[0] + rx) + 1);
             long long row = (long long)pr.cp.n[0] * (cy + (long long)pr.cp.n[1] * cz);
+            if constexpr (is_stationary(MODE)) {
+                const float c_row = fmaf(dy, dy, dz * dz);
+                float dx = x[0] - (pr.cp.o[0] + pr.cp.h[0] * (float)xa);
+                const float4 *src = pr.cpc + row;
+                for (int cx = xa; cx <= xb; ++cx, dx -= pr.cp.h[0]) {
+                    float k = stationary_k<MODE>(fmaf(dx, dx, c_row), pr.kc.inv_s4, pr.kc.inv_s0);
+                    float4 c = src[cx];
+                    float wk = c.w * k;
+                    u[0] = fmaf(wk, c.x, u[0]);
+                    u[1] = fmaf(wk, c.y, u[1]);
+                    if (DIM == 3) u[2] = fmaf(wk, c.z, u[2]);
+                }
+                continue;
+            }
             for (int cx = xa; cx <= xb; ++cx) {
                 float v[3] = {x[0] - (pr.cp.o[0] + pr.cp.h[0] * cx), dy, dz};
                 PState cst;
@@ -273,8 +288,10 @@ __global__ __launch_bounds__(kThreads) void k_prepare(Problem pr, const float4 *
 // evaluateDerivativeNCC, GpuEvaluatorNCC.cu:111-190, 265-340).  One workgroup
 // per control point; the window (reference or exact) was computed on the host
 // in FP64.  Lanes work in groups of 8 along x, 32 rows at a time; rows and
-// row ends outside the support ball are skipped (k == 0 there).
-template <int DIM, int MODE>
+// row ends outside the support ball are skipped (k == 0 there).  DENSE
+// (subsample 1, stationary kernel) walks a row with an incremental offset and
+// the row's constant y/z distance.
+template <int DIM, int MODE, bool DENSE>
 __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *__restrict__ sv,
                                                      float *__restrict__ deriv)
 {
@@ -291,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
     to_global<DIM>(pr.cp, fj, p);
     PState st;
     cp_state_load<DIM, MODE>(pr, j, st);
-    st.support = MODE == 0 ? pr.kc.smin : pr.cp_state[j * kCpStateStride + 11];
+    st.support = is_stationary(MODE) ? pr.kc.smin : pr.cp_state[j * kCpStateStride + 11];
     const int *win = pr.cp_window + j * 6;
     int lo[3] = {win[0], win[1], DIM == 3 ? win[2] : 0}, hi[3] = {win[3], win[4], DIM == 3 ? win[5] : 0};
     const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
@@ -303,8 +320,13 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
     float acc[3] = {0.0f, 0.0f, 0.0f};
     float pc[3];
     to_local<DIM>(pr.fixed, p, pc);  // cp position in fixed index units
-    for (int r = tr; r < rows; r += kThreads / 8) {
-        int my = lo[1] + r % by, mz = lo[2] + r / by;
+    const float inv_sub = 1.0f / (float)sub;
+    int my = lo[1] + tr % by, mz = lo[2] + tr / by;
+    for (int r = tr; r < rows; r += kThreads / 8, my += kThreads / 8) {
+        while (my > hi[1]) {
+            my -= by;
+            ++mz;
+        }
         // distance from p to the cell row's y/z extent (jitter included)
         float y0 = pr.fixed.o[1] + pr.fixed.h[1] * (float)(my * sub);
         float y1 = y0 + pr.fixed.h[1] * span;
@@ -317,10 +339,29 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
         }
         float rem = s2 - dy * dy - dz * dz;
         if (rem <= 0.0f) continue;
-        float rx = sqrtf(rem) * pr.fixed.ih[0];
-        int xa = max(lo[0], (int)floorf((pc[0] - rx - span) / (float)sub) - 1);
-        int xb = min(hi[0], (int)ceilf((pc[0] + rx) / (float)sub) + 1);
+        // x cells that can hold in-support samples, one cell of margin each side
+        float rx = fast_sqrt(rem) * pr.fixed.ih[0];
+        int xa = max(lo[0], (int)((pc[0] - rx - span) * inv_sub) - 1);
+        int xb = min(hi[0], (int)((pc[0] + rx) * inv_sub) + 1);
         long long rowbase = (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
+        if constexpr (DENSE && is_stationary(MODE)) {
+            // dense rows: y and z of the row are exact sample coordinates
+            float ry = p[1] - (pr.fixed.o[1] + pr.fixed.h[1] * (float)my);
+            float rz = DIM == 3 ? p[2] - (pr.fixed.o[2] + pr.fixed.h[2] * (float)mz) : 0.0f;
+            const float c_row = fmaf(ry, ry, rz * rz);
+            const float step = -8.0f * pr.fixed.h[0];
+            int mx = xa + tx;
+            float dx = p[0] - (pr.fixed.o[0] + pr.fixed.h[0] * (float)mx);
+            const float4 *src = sv + rowbase;
+            for (; mx <= xb; mx += 8, dx += step) {
+                float4 v4 = src[mx];
+                float k = stationary_k<MODE>(fmaf(dx, dx, c_row), pr.kc.inv_s4, pr.kc.inv_s0);
+                acc[0] = fmaf(k, v4.x, acc[0]);
+                acc[1] = fmaf(k, v4.y, acc[1]);
+                if (DIM == 3) acc[2] = fmaf(k, v4.z, acc[2]);
+            }
+            continue;
+        }
         for (int mx = xa + tx; mx <= xb; mx += 8) {
             float4 v4 = sv[rowbase + mx];
             int pos[3] = {mx, my, mz};
@@ -334,7 +375,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
             to_global<DIM>(pr.fixed, fpos, xs);
             float v[3] = {p[0] - xs[0], p[1] - xs[1], DIM == 3 ? p[2] - xs[2] : 0.0f};
             PState sst;
-            if (MODE != 0) point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, xs, sst);
+            if (!is_stationary(MODE)) point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, xs, sst);
             float k = kernel_pair<DIM, MODE>(pr.kc, v, st, sst);
             acc[0] = fmaf(k, v4.x, acc[0]);
             acc[1] = fmaf(k, v4.y, acc[1]);
@@ -355,14 +396,16 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
 // RKHS, RD and PG in one pass over the cp window (regularizerRKHS / RD / PG,
 // GpuEvaluatorRegularizers.cu:24-255, 259-472).  Values are unweighted per-cp
 // sums; derivatives are weighted and added to the normalised metric
-// derivative (GpuEvaluator.cu:652-711).
+// derivative (GpuEvaluator.cu:652-711).  One wave per control point, lanes
+// over the (x, y) plane of its window, z looped; wave sums in FP64.
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_regularizers(Problem pr, double w_rkhs, double w_rd, double w_pg,
                                                            double pg_scale, int with_derivative,
                                                            float *__restrict__ deriv, double *__restrict__ partials)
 {
     __shared__ double lds[4 * 4];
-    long long j = blockIdx.x * (long long)kThreads + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    long long j = blockIdx.x * (long long)(kThreads / 64) + (threadIdx.x >> 6);
     double vals[4] = {0.0, 0.0, 0.0, 0.0};
     if (j < pr.n_cp) {
         int jc[3] = {0, 0, 0};
@@ -384,9 +427,10 @@ __global__ __launch_bounds__(kThreads) void k_regularizers(Problem pr, double w_
         float vr = 0.0f, vd = 0.0f, vp = 0.0f;
         float dr[3] = {0, 0, 0}, dd[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
         int z0 = DIM == 3 ? win[2] : 0, z1 = DIM == 3 ? win[5] : 0;
-        for (int iz = z0; iz <= z1; ++iz)
-            for (int iy = win[1]; iy <= win[4]; ++iy)
-                for (int ix = win[0]; ix <= win[3]; ++ix) {
+        const int wx = win[3] - win[0] + 1, plane = wx * (win[4] - win[1] + 1);
+        for (int t = lane; t < plane; t += 64)
+            for (int iz = z0; iz <= z1; ++iz) {
+                    const int ix = win[0] + t % wx, iy = win[1] + t / wx;
                     long long i = ix + (long long)pr.cp.n[0] * (iy + (long long)pr.cp.n[1] * iz);
                     float fi[3] = {(float)ix, (float)iy, (float)iz}, q[3];
                     to_global<DIM>(pr.cp, fi, q);
@@ -437,11 +481,14 @@ __global__ __launch_bounds__(kThreads) void k_regularizers(Problem pr, double w_
                 }
         vals[0] = vr; vals[1] = vd; vals[2] = vp;
         if (with_derivative) {
+            double add[3];
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                double add = w_rkhs * (double)dr[d] + w_rd * (double)dd[d] + w_pg * (double)dp[d];
-                deriv[j + d * pr.n_cp] = (float)((double)deriv[j + d * pr.n_cp] + add);
-            }
+            for (int d = 0; d < DIM; ++d)
+                add[d] = wave_sum(w_rkhs * (double)dr[d] + w_rd * (double)dd[d] + w_pg * (double)dp[d]);
+            if (lane == 0)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                    deriv[j + d * pr.n_cp] = (float)((double)deriv[j + d * pr.n_cp] + add[d]);
         }
     }
     block_sum<4>(vals, lds);
@@ -543,11 +590,13 @@ static unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) /
         if ((pr).dim == 2) {                                                                                       \
             if ((pr).kc.mode == 0) FN<2, 0>(__VA_ARGS__);                                                          \
             else if ((pr).kc.mode == 1) FN<2, 1>(__VA_ARGS__);                                                     \
-            else FN<2, 2>(__VA_ARGS__);                                                                            \
+            else if ((pr).kc.mode == 2) FN<2, 2>(__VA_ARGS__);                                                     \
+            else FN<2, kModeStationaryEqual>(__VA_ARGS__);                                                         \
         } else {                                                                                                   \
             if ((pr).kc.mode == 0) FN<3, 0>(__VA_ARGS__);                                                          \
             else if ((pr).kc.mode == 1) FN<3, 1>(__VA_ARGS__);                                                     \
-            else FN<3, 2>(__VA_ARGS__);                                                                            \
+            else if ((pr).kc.mode == 2) FN<3, 2>(__VA_ARGS__);                                                     \
+            else FN<3, kModeStationaryEqual>(__VA_ARGS__);                                                         \
         }                                                                                                          \
     } while (0)
 
@@ -601,7 +650,8 @@ void launch_prepare(const Problem &pr, int measure, const float4 *sg, const floa
 template <int DIM, int MODE>
 static void gather_impl(const Problem &pr, const float4 *sv, float *deriv, hipStream_t s)
 {
-    k_gather<DIM, MODE><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
+    if (pr.sub == 1) k_gather<DIM, MODE, true><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
+    else k_gather<DIM, MODE, false><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
 }
 
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s)
@@ -613,15 +663,18 @@ template <int DIM, int MODE>
 static void reg_impl(const Problem &pr, const double *w, double pg, int with_derivative, float *deriv,
                      double *partials, hipStream_t s)
 {
-    k_regularizers<DIM, MODE><<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr, w[0], w[1], w[2], pg, with_derivative,
+    k_regularizers<DIM, MODE><<<reg_blocks(pr.n_cp), kThreads, 0, s>>>(pr, w[0], w[1], w[2], pg, with_derivative,
                                                                        deriv, partials);
 }
+
+// one wave per control point: 4 control points per workgroup
+unsigned reg_blocks(long long n_cp) { return (unsigned)((n_cp + kThreads / 64 - 1) / (kThreads / 64)); }
 
 void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, int with_derivative,
                          float *deriv_soa, double *partials, Scalars *sc, hipStream_t s)
 {
     SKM_DISPATCH_DIM_MODE(pr, reg_impl, pr, weights, pg_scaling, with_derivative, deriv_soa, partials, s);
-    k_reg_reduce<<<1, kThreads, 0, s>>>(partials, (int)blocks_for(pr.n_cp), sc);
+    k_reg_reduce<<<1, kThreads, 0, s>>>(partials, (int)reg_blocks(pr.n_cp), sc);
 }
 
 template <int DIM, int MODE>

@@ -396,14 +396,15 @@ static void dense_impl(const Problem &pr, int quirk, float4 *out, hipStream_t s)
     else k_dense_warp<DIM, MODE, 0><<<g, kThreads, 0, s>>>(pr, out);
 }
 
+// Both stationary modes (0 and 3) run the general stationary instantiation.
 #define SKM_LCC_DISPATCH(pr, FN, ...)                                                                              \
     do {                                                                                                           \
         if ((pr).dim == 2) {                                                                                       \
-            if ((pr).kc.mode == 0) FN<2, 0>(__VA_ARGS__);                                                          \
+            if (is_stationary((pr).kc.mode)) FN<2, 0>(__VA_ARGS__);                                                \
             else if ((pr).kc.mode == 1) FN<2, 1>(__VA_ARGS__);                                                     \
             else FN<2, 2>(__VA_ARGS__);                                                                            \
         } else {                                                                                                   \
-            if ((pr).kc.mode == 0) FN<3, 0>(__VA_ARGS__);                                                          \
+            if (is_stationary((pr).kc.mode)) FN<3, 0>(__VA_ARGS__);                                                \
             else if ((pr).kc.mode == 1) FN<3, 1>(__VA_ARGS__);                                                     \
             else FN<3, 2>(__VA_ARGS__);                                                                            \
         }                                                                                                          \
