@@ -76,7 +76,8 @@ typedef struct skm_desc {
     double sigma_c0, sigma_c4;                /* Wendland C0 / C4 supports             */
     int32_t kernel_mode;                      /* skm_kernel_mode                       */
     double maximum_weight;                    /* max of the weight image               */
-    double maximum_eigenvalue_covariance[9];  /* D*D row-major                         */
+    double maximum_eigenvalue_covariance[9];  /* 3x3 row-major; 2D uses the upper-left
+                                                 2x2 block                              */
     skm_grid weight; const float *weight_data;/* weight image / tensor field, or NULL  */
     int32_t subsample;                        /* sampling rate (>= 1)                  */
     int32_t subsample_neighborhood;           /* LCC neighbourhood stride (>= 1)       */
@@ -167,6 +168,39 @@ SKM_API skm_status skm_get_cube_offsets(skm_evaluator *ev, int32_t *out, int64_t
 /* TransformAdapter::ComputeControlPointImage geometry (itkTransformAdapter.hxx:246-308). */
 SKM_API skm_status skm_control_point_grid(int32_t dim, const skm_grid *image, const double *cp_spacing,
                                           skm_grid *cp_out);
+
+/* ---- image preprocessing ----------------------------------------------- */
+
+/* GaussianSmoothing(image, variance) (include/itkUtils.h:140-151):
+ * itk::DiscreteGaussianImageFilter with the variance in physical units
+ * (UseImageSpacing), maximum error 0.01, maximum kernel width 64 and edge
+ * replication at the border.  Host pointers; in and out may be the same
+ * array.  A per-axis variance beyond ~709 pixels^2, where the discrete
+ * kernel's exp(t) overflows, is rejected. */
+SKM_API skm_status skm_gaussian_smooth(int32_t dim, const skm_grid *grid, const float *in, double variance,
+                                       float *out);
+
+/* Arguments of ComputeAnisotropicTensor (include/AnisotropicTensorUtils.h:37-39). */
+typedef struct skm_tensor_options {
+    double sigma;                   /* sigma_C4 of the scale: image smoothing variance; sigma/4
+                                       smooths the structure tensor                            */
+    double weight;                  /* tensor weight (gradient scale), tensor_weights[scale]   */
+    int32_t alignment_direction;    /* 0 x, 1 y, 2 z, negative none; SKMReg.cxx:271 passes 2   */
+    int32_t anisotropic_filtering;  /* must be 0; SKMReg.cxx:272 passes false                  */
+} skm_tensor_options;
+
+/* ComputeAnisotropicTensor(image, sigma, weight, alignment, maximum_tilt,
+ * false) (AnisotropicTensorUtils.h:37-355) followed by
+ * TransformAdapter::SetWeightingTensor(tensor, maximum_tilt)
+ * (itkTransformAdapter.hxx:335-366), on the GPU.  tensor_out receives D*D
+ * floats per voxel on the image grid (the layout of skm_desc.weight_data);
+ * maximum_tilt the value the reference returns (the lowest double when no
+ * voxel is re-weighted); maximum_weight and maximum_eigenvalue_covariance
+ * (9 doubles, 3x3 row-major like skm_desc) the values skm_desc takes.  Output pointers other than
+ * tensor_out may be NULL. */
+SKM_API skm_status skm_anisotropic_tensor(int32_t dim, const skm_grid *grid, const float *image,
+                                          const skm_tensor_options *opt, float *tensor_out, double *maximum_tilt,
+                                          double *maximum_weight, double *maximum_eigenvalue_covariance);
 
 /* ---- metric assembly and optimisation, device resident ---------------- */
 

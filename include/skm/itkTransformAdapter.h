@@ -240,7 +240,9 @@ public:
         desc.sigma_c4 = sigma_c4_;
         desc.maximum_weight = maximum_weight_;
         for (int i = 0; i < 9; ++i) desc.maximum_eigenvalue_covariance[i] = 0.0;
-        for (unsigned i = 0; i < NDimensions * NDimensions; ++i) desc.maximum_eigenvalue_covariance[i] = max_eig_cov_[i];
+        for (unsigned i = 0; i < NDimensions; ++i)
+            for (unsigned k = 0; k < NDimensions; ++k)
+                desc.maximum_eigenvalue_covariance[i * 3 + k] = max_eig_cov_[i * NDimensions + k];
         if (use_weight_image_) {
             desc.kernel_mode = SKM_KERNEL_WEIGHT_IMAGE;
             fill_grid(*weight_image_, desc.weight);

@@ -5,11 +5,11 @@ package is the host-side mirror of the reference's GpuKernelEvaluator
 interface over that ABI.
 """
 from . import _capi
-from .evaluator import (MEASURE, EvaluationResult, GpuKernelEvaluator, Grid, Image, Kernel,
-                        control_point_grid)
+from .evaluator import (MEASURE, ComputeAnisotropicTensor, EvaluationResult, GaussianSmoothing,
+                        GpuKernelEvaluator, Grid, Image, Kernel, control_point_grid)
 
-__all__ = ["MEASURE", "EvaluationResult", "GpuKernelEvaluator", "Grid", "Image", "Kernel",
-           "control_point_grid", "load_library"]
+__all__ = ["MEASURE", "ComputeAnisotropicTensor", "EvaluationResult", "GaussianSmoothing", "GpuKernelEvaluator",
+           "Grid", "Image", "Kernel", "control_point_grid", "load_library"]
 
 
 def load_library():

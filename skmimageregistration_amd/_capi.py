@@ -76,6 +76,11 @@ class SkmIterationRecord(ctypes.Structure):
                 ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
+class SkmTensorOptions(ctypes.Structure):
+    _fields_ = [("sigma", ctypes.c_double), ("weight", ctypes.c_double), ("alignment_direction", ctypes.c_int32),
+                ("anisotropic_filtering", ctypes.c_int32)]
+
+
 # Every symbol include/skm.h declares, with (restype, argtypes).
 _SIGNATURES = {
     "skm_desc_init": (None, [ctypes.POINTER(SkmDesc)]),
@@ -106,6 +111,11 @@ _SIGNATURES = {
     "skm_optimize": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmMetricOptions),
                                     ctypes.POINTER(SkmOptimizerOptions), c_float_p, c_float_p,
                                     ctypes.POINTER(SkmIterationRecord)]),
+    "skm_gaussian_smooth": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p, ctypes.c_double,
+                                           c_float_p]),
+    "skm_anisotropic_tensor": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p,
+                                              ctypes.POINTER(SkmTensorOptions), c_float_p, c_double_p, c_double_p,
+                                              c_double_p]),
     "skm_host_regularizers": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, c_double_p, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_int32, c_double_p,
                                              c_double_p, c_double_p, c_double_p, c_double_p]),

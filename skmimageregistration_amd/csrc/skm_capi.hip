@@ -445,6 +445,8 @@ skm_status read_result(skm_evaluator *ev, skm_result *out)
 
 }  // namespace
 
+skm_status skm::set_last_error(skm_status st, const std::string &msg) { return fail(st, msg); }
+
 // ---------------------------------------------------------------- C ABI
 
 extern "C" {
@@ -514,6 +516,12 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
     skm_evaluator *ev = new skm_evaluator();
     ev->desc = *desc;
     ev->desc.fixed_data = ev->desc.moving_data = ev->desc.weight_data = nullptr;
+    // The descriptor holds the maximum-eigenvalue covariance as 3x3 row-major
+    // (a 2D kernel uses the upper-left block); internally it is packed D*D.
+    double mec[9] = {0};
+    for (int i = 0; i < dim; ++i)
+        for (int k = 0; k < dim; ++k) mec[i * dim + k] = desc->maximum_eigenvalue_covariance[i * 3 + k];
+    std::memcpy(ev->desc.maximum_eigenvalue_covariance, mec, sizeof(mec));
     ev->dim = dim;
     ev->rng_state = desc->rng_state;
     ev->rng_inc = desc->rng_sequence | 1ULL;
@@ -540,7 +548,7 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
     pr.kc.inv_s0 = (float)(1.0 / desc->sigma_c0);
     pr.kc.smin = (float)std::min(desc->sigma_c4, desc->sigma_c0);
     pr.kc.max_weight = (float)desc->maximum_weight;
-    for (int i = 0; i < 9; ++i) pr.kc.max_eig_cov[i] = (float)desc->maximum_eigenvalue_covariance[i];
+    for (int i = 0; i < 9; ++i) pr.kc.max_eig_cov[i] = (float)ev->desc.maximum_eigenvalue_covariance[i];
     pr.sub = desc->subsample;
     pr.subn = desc->subsample_neighborhood;
     for (int d = 0; d < 3; ++d) pr.nsub[d] = d < dim ? desc->fixed.size[d] / desc->subsample : 1;

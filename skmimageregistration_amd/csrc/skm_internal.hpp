@@ -4,9 +4,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
+#include "skm.h"
 #include "skm_device.hpp"
 
 namespace skm {
+
+// Stores msg as the calling thread's last error (skm_last_error) and returns st.
+skm_status set_last_error(skm_status st, const std::string &msg);
 
 // Per-control-point kernel state (static for an evaluator): sig, sdet, cov[9], support.
 constexpr int kCpStateStride = 12;

@@ -147,7 +147,7 @@ class GpuKernelEvaluator:
                 desc.maximum_eigenvalue_covariance[i] = 0.0
             for i in range(dim):
                 for k in range(dim):
-                    desc.maximum_eigenvalue_covariance[i * dim + k] = m[i, k]
+                    desc.maximum_eigenvalue_covariance[i * 3 + k] = m[i, k]
         desc.subsample = int(subsample)
         desc.subsample_neighborhood = int(subsampleNeighborhood)
         if rng_state is not None:
@@ -290,3 +290,39 @@ def control_point_grid(image, spacing):
     out = _capi.SkmGrid()
     check(lib.skm_control_point_grid(dim, ctypes.byref(g_in), sp, ctypes.byref(out)))
     return Grid(tuple(out.size[:dim]), tuple(out.origin[:dim]), tuple(out.spacing[:dim]))
+
+
+def GaussianSmoothing(image, variance=1.0):
+    """GaussianSmoothing(image, variance) (include/itkUtils.h:140-151) on the GPU:
+    discrete Gaussian, variance in physical units, edge replication.  Returns
+    an Image with the same geometry."""
+    lib = _capi.load()
+    src = f32(image.data)
+    out = np.empty_like(src)
+    g = image.grid()
+    check(lib.skm_gaussian_smooth(image.dim, ctypes.byref(g), fptr(src), float(variance), fptr(out)))
+    return Image(out, image.origin, image.spacing)
+
+
+def ComputeAnisotropicTensor(image, sigma, weight, alignment_direction=2, anisotropic_filtering=False):
+    """ComputeAnisotropicTensor (include/AnisotropicTensorUtils.h:37-355) with the
+    maximum-tilt handling of TransformAdapter::SetWeightingTensor
+    (itkTransformAdapter.hxx:335-366), on the GPU.
+
+    Returns (tensor, maximum_tilt, maximum_weight, maximum_eigenvalue_covariance):
+    tensor has shape image.data.shape + (D, D) on the image grid (the wTensor
+    data of Kernel), maximum_tilt is the value the reference returns (the
+    lowest double when no voxel is re-weighted), and the last two are what
+    Kernel(maximumWeight=..., maximumEigenvalueCovariance=...) takes."""
+    lib = _capi.load()
+    dim = image.dim
+    src = f32(image.data)
+    out = np.empty(src.shape + (dim, dim), dtype=np.float32)
+    g = image.grid()
+    opt = _capi.SkmTensorOptions(float(sigma), float(weight), int(alignment_direction), int(bool(anisotropic_filtering)))
+    tilt, mw = ctypes.c_double(), ctypes.c_double()
+    cov = (ctypes.c_double * 9)()
+    check(lib.skm_anisotropic_tensor(dim, ctypes.byref(g), fptr(src), ctypes.byref(opt), fptr(out),
+                                     ctypes.byref(tilt), ctypes.byref(mw), cov))
+    mcov = np.array(cov[:], dtype=np.float64).reshape(3, 3)[:dim, :dim].copy()
+    return out, tilt.value, mw.value, mcov

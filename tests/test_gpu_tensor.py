@@ -1,0 +1,107 @@
+"""GPU parity of the image preprocessing of the anisotropic kernel: discrete
+Gaussian smoothing and ComputeAnisotropicTensor (libskm.so through the C ABI)
+against the oracle (oracle/tensor_oracle.py), then the generated tensor field
+driving the tensor-kernel evaluator against the FP64 evaluator oracle.
+
+Tolerances: the smoothing keeps float images between passes like the
+reference, so results agree to float rounding (2e-6 of the image range).  The
+covariance entries lie in (0, 2]; the GPU's FP64 Jacobi and the oracle's
+LAPACK eigensolver agree to 1e-12 on the same float tensor, and the float
+storage of the intermediate planes leaves differences of a few 1e-6 (bound:
+1e-4 absolute).
+"""
+import numpy as np
+import pytest
+
+import tensor_oracle as tor
+from helpers import Case, rel_err, scalar_rel
+from skmimageregistration_amd import ComputeAnisotropicTensor, GaussianSmoothing, Image, MEASURE
+from skmimageregistration_amd._capi import SkmError
+
+pytestmark = pytest.mark.gpu
+
+
+def _blobs(shape, seed, noise=0.02):
+    rng = np.random.default_rng(seed)
+    axes = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    img = np.zeros(shape)
+    for _ in range(5):
+        c = [rng.uniform(0, n - 1) for n in shape]
+        s = rng.uniform(2.0, 5.0)
+        img += rng.uniform(0.3, 1.0) * np.exp(-sum((a - ci) ** 2 for a, ci in zip(axes, c)) / (2 * s * s))
+    return (img + noise * rng.normal(size=shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,spacing,variance", [
+    ((16, 20, 24), (1.5, 1.0, 2.0), 2.0),
+    ((16, 20, 24), (1.5, 1.0, 2.0), 16.0),
+    ((9, 70, 5), (1.0, 0.5, 1.0), 64.0),        # kernel wider than two axes
+    ((36, 40), (1.0, 1.0), 4.0),
+    ((36, 40), (0.7, 1.3), 30.0),
+])
+def test_gaussian_smoothing_matches_oracle(shape, spacing, variance):
+    img = _blobs(shape, 1)
+    print(f"smoothing shape={shape} spacing={spacing} variance={variance}", flush=True)
+    out = GaussianSmoothing(Image(img, spacing=spacing), variance).data
+    ref = tor.gaussian_smooth(img, spacing, variance)
+    assert np.max(np.abs(out - ref)) <= 2e-6 * np.max(np.abs(ref))
+
+
+def test_gaussian_smoothing_rejects_overflowing_variance():
+    img = _blobs((8, 9, 10), 2)
+    with pytest.raises(SkmError):
+        GaussianSmoothing(Image(img, spacing=(0.5, 1.0, 1.0)), 400.0)   # 1600 pixels^2 along x
+
+
+@pytest.mark.parametrize("dim,shape,spacing,sigma,weight,alignment", [
+    (3, (16, 18, 20), (1.5, 1.0, 2.0), 8.0, 50.0, 2),
+    (3, (16, 18, 20), (1.0, 1.0, 1.0), 16.0, 400.0, 2),
+    (3, (12, 14, 16), (1.0, 1.2, 0.8), 8.0, 50.0, -1),
+    (3, (12, 14, 16), (1.0, 1.2, 0.8), 8.0, 50.0, 0),
+    (2, (40, 44), (1.0, 1.0), 16.0, 400.0, 2),
+    (2, (40, 44), (1.2, 0.9), 8.0, 50.0, 0),
+])
+def test_anisotropic_tensor_matches_oracle(dim, shape, spacing, sigma, weight, alignment):
+    img = _blobs(shape, 3 + dim)
+    print(f"tensor dim={dim} shape={shape} sigma={sigma} weight={weight} alignment={alignment}", flush=True)
+    ten, tilt, mw, mcov = ComputeAnisotropicTensor(Image(img, spacing=spacing), sigma, weight, alignment)
+    rten, rtilt, rmw, rmcov = tor.anisotropic_tensor(img, spacing, sigma, weight, alignment)
+    assert ten.shape == rten.shape
+    diff = np.abs(ten.astype(np.float64) - rten)
+    assert diff.max() < 1e-4, f"max |diff| {diff.max()}"
+    assert rtilt > 0 and scalar_rel(tilt, rtilt) < 1e-5
+    assert scalar_rel(mw, rmw) < 1e-5
+    np.testing.assert_array_equal(mcov, rmcov)
+
+
+def test_constant_image_gives_identity_and_lowest_tilt():
+    img = np.full((10, 11, 12), 7.0, np.float32)
+    ten, tilt, mw, mcov = ComputeAnisotropicTensor(Image(img), 8.0, 400.0, 2)
+    np.testing.assert_array_equal(ten, np.broadcast_to(np.eye(3, dtype=np.float32), ten.shape))
+    assert tilt == tor.LOWEST and mw == 1.0
+    np.testing.assert_array_equal(mcov, np.eye(3))
+
+
+def test_anisotropic_filtering_not_supported():
+    img = _blobs((8, 9, 10), 4)
+    with pytest.raises(SkmError):
+        ComputeAnisotropicTensor(Image(img), 8.0, 400.0, 2, anisotropic_filtering=True)
+
+
+@pytest.mark.parametrize("dim,measure", [(3, MEASURE.NCC), (3, MEASURE.MSE), (2, MEASURE.MSE)])
+def test_generated_tensor_drives_the_evaluator(dim, measure):
+    """SKMReg case 3 (src/SKMReg.cxx:268-273): tensor from the target image,
+    then the anisotropic kernel evaluates value and derivative."""
+    case = Case(dim=dim, mode=2, cp_spacing=5.0, sigma=12.0, subsample=1, seed=11)
+    ten, tilt, mw, mcov = ComputeAnisotropicTensor(Image(case.fixed, case.origin, case.spacing), 12.0, 400.0, 2)
+    assert tilt > 0
+    case.weight = ten
+    case.max_weight = mw
+    case.max_eig = mcov
+    print(f"tensor-driven evaluator dim={dim} measure={measure.name} tilt={tilt:.4g}", flush=True)
+    ev = case.gpu()
+    res, der = ev.getValueAndDerivative(measure, case.cp, case.cpw)
+    ev.close()
+    ores, oder = case.oracle().evaluate(int(measure), True)
+    assert scalar_rel(res.measure, ores["measure"]) < 1e-5
+    assert rel_err(der, oder) < 1e-4
