@@ -145,6 +145,24 @@ double max_eig_h(int dim, const double *m)
     return q + 2.0 * p * std::cos(phi);
 }
 
+// Symmetric part (m + m^T) / 2 of a row-major dim x dim matrix.
+void symmetric_part(int dim, const double *m, double *out)
+{
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j) out[i * dim + j] = 0.5 * (m[i * dim + j] + m[j * dim + i]);
+}
+
+// pstate_pack layout of a mode-2 point: {up0, up1, up2, sdet}, {up3, up4, up5, lam}.
+void pack_upper(int dim, const double *sym, float sdet, float lam, float *ps)
+{
+    int k = 0;
+    float up[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < dim; ++i)
+        for (int j = i; j < dim; ++j) up[k++] = (float)sym[i * dim + j];
+    ps[0] = up[0]; ps[1] = up[1]; ps[2] = up[2]; ps[3] = sdet;
+    ps[4] = up[3]; ps[5] = up[4]; ps[6] = up[5]; ps[7] = lam;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- evaluator
@@ -160,6 +178,7 @@ struct skm_evaluator {
     float *d_fixed = nullptr, *d_coeffs = nullptr, *d_weight = nullptr, *d_df = nullptr;
     float4 *d_cpc = nullptr;
     float *d_cp_state = nullptr;
+    float4 *d_cp_pstate = nullptr, *d_smp_pstate = nullptr, *d_vox_pstate = nullptr;
     int *d_cp_window = nullptr, *d_reg_window = nullptr, *d_lcc_window = nullptr, *d_dense_window = nullptr;
     int *d_offsets = nullptr;
     float4 *d_sg = nullptr, *d_sv = nullptr;
@@ -188,7 +207,8 @@ struct skm_evaluator {
 
     ~skm_evaluator()
     {
-        void *ptrs[] = {d_fixed, d_coeffs, d_weight, d_df, d_cpc, d_cp_state, d_cp_window, d_reg_window,
+        void *ptrs[] = {d_fixed, d_coeffs, d_weight, d_df, d_cpc, d_cp_state, d_cp_pstate, d_smp_pstate,
+                        d_vox_pstate, d_cp_window, d_reg_window,
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_dot_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_opt, d_records};
@@ -271,6 +291,8 @@ skm_status build_static(skm_evaluator *ev)
     const long long ncp = grid_count(ds.cp, dim);
     const double smin = std::min(ds.sigma_c4, ds.sigma_c0);
     std::vector<float> state((size_t)ncp * kCpStateStride, 0.0f);
+    std::vector<float> pstate((size_t)ncp * 8, 0.0f);  // pstate_pack layout, two float4 per cp
+    double lam_cp_max = 0.0;
     std::vector<double> support(ncp, smin);
     std::vector<int> win((size_t)ncp * 6), lwin((size_t)ncp * 6), dwin((size_t)ncp * 6), rwin((size_t)ncp * 6);
     int ns[3] = {1, 1, 1};
@@ -292,11 +314,16 @@ skm_status build_static(skm_evaluator *ev)
                 f += stride * i;
                 stride *= ds.weight.size[d];
             }
+            float *ps = &pstate[(size_t)j * 8];
             if (ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE) {
                 double s = in ? ds.weight_data[f] : 0.0;
                 st[0] = (float)s;
                 st[1] = (float)std::sqrt(std::fabs(s));
                 support[j] = smin / std::sqrt(2.0 / (ds.maximum_weight + s));
+                ps[0] = st[0];
+                ps[1] = st[1];
+                ps[7] = st[0];
+                lam_cp_max = std::max(lam_cp_max, (double)ps[7]);
             } else {
                 double cov[9] = {0}, sum[9] = {0};
                 for (int i = 0; i < nn; ++i) cov[i] = in ? ds.weight_data[f * nn + i] : ((i % (dim + 1)) == 0 ? 1.0 : 0.0);
@@ -304,6 +331,12 @@ skm_status build_static(skm_evaluator *ev)
                 st[1] = (float)std::sqrt(det_h(dim, cov));
                 for (int i = 0; i < nn; ++i) sum[i] = ds.maximum_eigenvalue_covariance[i] + cov[i];
                 support[j] = smin / std::sqrt(2.0 / max_eig_h(dim, sum));
+                double sym[9];
+                symmetric_part(dim, cov, sym);
+                const double lam = max_eig_h(dim, sym);
+                st[0] = (float)lam;
+                pack_upper(dim, sym, st[1], (float)lam, ps);
+                lam_cp_max = std::max(lam_cp_max, lam);
             }
         }
         st[11] = (float)support[j];
@@ -339,6 +372,26 @@ skm_status build_static(skm_evaluator *ev)
         }
     }
     SKM_TRY(dev_upload(&ev->d_cp_state, state.data(), state.size()));
+    if (ds.kernel_mode != SKM_KERNEL_STATIONARY) {
+        SKM_TRY(dev_upload(&ev->d_cp_pstate, reinterpret_cast<const float4 *>(pstate.data()), (size_t)ncp * 2));
+        // largest lam a sample can look up: every weight voxel, and the
+        // outside value (sigma 0 / identity covariance)
+        const long long nw = grid_count(ds.weight, dim);
+        double lam_w = ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 0.0 : 1.0;
+        for (long long i = 0; i < nw; ++i) {
+            if (ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE) {
+                lam_w = std::max(lam_w, (double)ds.weight_data[i]);
+            } else {
+                double cov[9] = {0}, sym[9];
+                for (int k = 0; k < nn; ++k) cov[k] = ds.weight_data[i * nn + k];
+                symmetric_part(dim, cov, sym);
+                lam_w = std::max(lam_w, max_eig_h(dim, sym));
+            }
+        }
+        // rounded up so that the FP32 values stay upper bounds
+        ev->pr.lam_cp_max = (float)(lam_cp_max * (1.0 + 1e-6));
+        ev->pr.lam_w_max = (float)(lam_w * (1.0 + 1e-6));
+    }
     SKM_TRY(dev_upload(&ev->d_cp_window, win.data(), win.size()));
     SKM_TRY(dev_upload(&ev->d_lcc_window, lwin.data(), lwin.size()));
     SKM_TRY(dev_upload(&ev->d_dense_window, dwin.data(), dwin.size()));
@@ -385,6 +438,13 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
                 SKM_HIP(hipMemsetAsync(ev->d_gamma, 0, sizeof(float4) * pr.n_vox, s));
                 launch_lcc_scatter(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_gamma, s);
                 Problem dp = pr;  // gather over the dense voxel grid
+                if (!is_stationary(pr.kc.mode)) {
+                    if (!ev->d_vox_pstate) {
+                        SKM_TRY(dev_alloc(&ev->d_vox_pstate, (size_t)pr.n_vox * 2));
+                        launch_voxel_states(pr, ev->d_vox_pstate, s);
+                    }
+                    dp.smp_pstate = ev->d_vox_pstate;
+                }
                 dp.sub = 1;
                 for (int d = 0; d < 3; ++d) dp.nsub[d] = pr.fixed.n[d];
                 dp.n_sub = pr.n_vox;
@@ -602,6 +662,11 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
     pr.has_df = 0;
     pr.cpc = ev->d_cpc;
     pr.cp_state = ev->d_cp_state;
+    pr.cp_pstate = ev->d_cp_pstate;
+    if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
+        if ((st = dev_alloc(&ev->d_smp_pstate, (size_t)pr.n_sub * 2)) != SKM_SUCCESS) return cleanup(st);
+    }
+    pr.smp_pstate = ev->d_smp_pstate;
     pr.cp_window = ev->d_cp_window;
     pr.reg_window = ev->d_reg_window;
     pr.offsets = ev->d_offsets;

@@ -28,15 +28,19 @@ struct KernelConst {
     float max_eig_cov[9];
 };
 
-// Kernel-side state of one point: sigma (mode 1) or covariance (mode 2),
+// Kernel-side state of one point: sigma (mode 1) or the symmetric part of
+// the covariance (mode 2, upper triangle xx, xy, (xz,) yy, (yz, zz)),
 // sdet = |sigma|^(1/2) or det(cov)^(1/2) (the square of the F factor's
 // per-point part, Kernel.h:107-109, 121-123), and the region support.
 struct PState {
     float sig;
     float sdet;
-    float cov[9];
+    float up[6];
     float support;
 };
+
+// Number of upper-triangle entries of a symmetric DIM x DIM matrix.
+template <int DIM> constexpr int kUpper = DIM * (DIM + 1) / 2;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
@@ -143,6 +147,78 @@ __device__ __forceinline__ float max_eigenvalue(const float *m)
     return q + 2.0f * p * cosf(phi);
 }
 
+// Symmetric part of a row-major DIM x DIM matrix, upper triangle.
+template <int DIM>
+__device__ __forceinline__ void sym_upper(const float *m, float *up)
+{
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = i; j < DIM; ++j) up[k++] = i == j ? m[i * DIM + i] : 0.5f * (m[i * DIM + j] + m[j * DIM + i]);
+}
+
+// Largest eigenvalue of the symmetric matrix held as its upper triangle.
+template <int DIM>
+__device__ __forceinline__ float max_eig_upper(const float *up)
+{
+    float m[9];
+    if (DIM == 2) {
+        m[0] = up[0]; m[1] = m[2] = up[1]; m[3] = up[2];
+    } else {
+        m[0] = up[0]; m[1] = m[3] = up[1]; m[2] = m[6] = up[2];
+        m[4] = up[3]; m[5] = m[7] = up[4]; m[8] = up[5];
+    }
+    return max_eigenvalue<DIM>(m);
+}
+
+// Packed per-point state of the weighted modes, two float4 per point:
+//   mode 1: {sigma, sdet, 0, 0}, {0, 0, 0, lam}
+//   mode 2: {up0, up1, up2, sdet}, {up3, up4, up5, lam}
+// lam (sigma, or the largest covariance eigenvalue) bounds the distance at
+// which a pair with this point can still have k != 0.
+template <int DIM, int MODE>
+__device__ __forceinline__ void pstate_unpack(const float4 a, const float4 b, PState &st)
+{
+    if (MODE == 1) {
+        st.sig = a.x;
+        st.sdet = a.y;
+    } else {
+        st.up[0] = a.x; st.up[1] = a.y; st.up[2] = a.z;
+        st.up[3] = b.x; st.up[4] = b.y; st.up[5] = b.z;
+        st.sdet = a.w;
+    }
+}
+
+template <int DIM, int MODE>
+__device__ __forceinline__ void pstate_pack(const PState &st, float lam, float4 &a, float4 &b)
+{
+    if (MODE == 1) {
+        a = make_float4(st.sig, st.sdet, 0.0f, 0.0f);
+        b = make_float4(0.0f, 0.0f, 0.0f, lam);
+    } else {
+        a = make_float4(st.up[0], st.up[1], st.up[2], st.sdet);
+        b = make_float4(DIM == 3 ? st.up[3] : 0.0f, DIM == 3 ? st.up[4] : 0.0f, DIM == 3 ? st.up[5] : 0.0f, lam);
+    }
+}
+
+// lam of a point state: sigma (mode 1) or the largest covariance eigenvalue.
+template <int DIM, int MODE>
+__device__ __forceinline__ float pstate_lam(const PState &st)
+{
+    return MODE == 1 ? st.sig : max_eig_upper<DIM>(st.up);
+}
+
+// Squared distance beyond which k(p, q) == 0 for every partner q: the kernel
+// vanishes for r >= min(sigma_C4, sigma_C0), and r^2 >= |v|^2 2 / (lam_p +
+// lam_q) in both weighted modes (mode 2: lambda_max((S_p + S_q) / 2) <=
+// (lam_p + lam_q) / 2).  lam_other bounds lam_q over all partners; the
+// relative margin covers FP32 rounding of lam.
+__device__ __forceinline__ float pair_reach2(float smin, float lam_self, float lam_other)
+{
+    return smin * smin * 0.5f * (lam_self + lam_other) * 1.0001f;
+}
+
 // State of a point from the weight image / tensor field (getSigmaAtPoint /
 // getCovarianceAtPoint, Kernel.h:148-156, NN with 0 / identity outside) and
 // its region support (Kernel.h:158-171).
@@ -163,51 +239,58 @@ __device__ __forceinline__ void point_state(const KernelConst &kc, const Grid &w
         st.support = kc.smin / sqrtf(2.0f / (kc.max_weight + s));
     } else {
         constexpr int NN = DIM * DIM;
+        float cov[9];
 #pragma unroll
-        for (int i = 0; i < NN; ++i) st.cov[i] = in ? wdata[f * NN + i] : ((i % (DIM + 1)) == 0 ? 1.0f : 0.0f);
-        st.sdet = sqrtf(det_m<DIM>(st.cov));
+        for (int i = 0; i < NN; ++i) cov[i] = in ? wdata[f * NN + i] : ((i % (DIM + 1)) == 0 ? 1.0f : 0.0f);
+        st.sdet = sqrtf(det_m<DIM>(cov));
+        sym_upper<DIM>(cov, st.up);
         float sum[9];
 #pragma unroll
-        for (int i = 0; i < NN; ++i) sum[i] = kc.max_eig_cov[i] + st.cov[i];
+        for (int i = 0; i < NN; ++i) sum[i] = kc.max_eig_cov[i] + cov[i];
         st.support = kc.smin / sqrtf(2.0f / max_eigenvalue<DIM>(sum));
     }
 }
 
 // k(p, q) for v = p - q with the states of both points (Kernel.h:96-131);
 // the weighted modes carry F twice, F^2 = sdet_p * sdet_q / |avg|.
+// Mode 2 works on S = S_p + S_q: with m = S / 2, v^T m^-1 v = 2 v^T adj(S) v /
+// det S and det m = det S / 2^D, so one reciprocal serves both terms.
 template <int DIM, int MODE>
 __device__ __forceinline__ float kernel_pair(const KernelConst &kc, const float *v, const PState &a, const PState &b)
 {
-    float r2 = v[0] * v[0] + v[1] * v[1];
-    if (DIM == 3) r2 = fmaf(v[2], v[2], r2);
     if (is_stationary(MODE)) {
+        float r2 = v[0] * v[0] + v[1] * v[1];
+        if (DIM == 3) r2 = fmaf(v[2], v[2], r2);
         return stationary_k<MODE>(r2, kc.inv_s4, kc.inv_s0);
     } else if (MODE == 1) {
+        float r2 = v[0] * v[0] + v[1] * v[1];
+        if (DIM == 3) r2 = fmaf(v[2], v[2], r2);
         float sum = a.sig + b.sig;
-        float F2 = a.sdet * b.sdet / fabsf(0.5f * sum);
-        float rs = fast_sqrt(2.0f / sum * r2);
+        float inv = __builtin_amdgcn_rcpf(sum);
+        float F2 = 2.0f * a.sdet * b.sdet * fabsf(inv);
+        float rs = fast_sqrt(2.0f * inv * r2);
         return F2 * wendland_c4(rs * kc.inv_s4) * wendland_c0(rs * kc.inv_s0);
     } else {
-        float m[9];
-        constexpr int NN = DIM * DIM;
-#pragma unroll
-        for (int i = 0; i < NN; ++i) m[i] = 0.5f * (a.cov[i] + b.cov[i]);
         float quad, det;
         if (DIM == 2) {
-            det = m[0] * m[3] - m[2] * m[1];
-            quad = v[0] * v[0] * m[3] - v[0] * v[1] * (m[1] + m[2]) + v[1] * v[1] * m[0];
+            const float s00 = a.up[0] + b.up[0], s01 = a.up[1] + b.up[1], s11 = a.up[2] + b.up[2];
+            det = fmaf(s00, s11, -s01 * s01);
+            quad = fmaf(v[0] * v[0], s11, fmaf(v[1] * v[1], s00, -2.0f * s01 * v[0] * v[1]));
         } else {
-            float a0 = m[4] * m[8] - m[5] * m[7], a1 = m[2] * m[7] - m[1] * m[8], a2 = m[1] * m[5] - m[2] * m[4];
-            float a3 = m[5] * m[6] - m[3] * m[8], a4 = m[0] * m[8] - m[2] * m[6], a5 = m[2] * m[3] - m[0] * m[5];
-            float a6 = m[3] * m[7] - m[4] * m[6], a7 = m[1] * m[6] - m[0] * m[7], a8 = m[0] * m[4] - m[1] * m[3];
-            det = m[0] * a0 + m[1] * a3 + m[2] * a6;
-            float w0 = v[0] * a0 + v[1] * a3 + v[2] * a6;
-            float w1 = v[0] * a1 + v[1] * a4 + v[2] * a7;
-            float w2 = v[0] * a2 + v[1] * a5 + v[2] * a8;
-            quad = w0 * v[0] + w1 * v[1] + w2 * v[2];
+            const float s00 = a.up[0] + b.up[0], s01 = a.up[1] + b.up[1], s02 = a.up[2] + b.up[2];
+            const float s11 = a.up[3] + b.up[3], s12 = a.up[4] + b.up[4], s22 = a.up[5] + b.up[5];
+            const float c00 = fmaf(s11, s22, -s12 * s12), c01 = fmaf(s02, s12, -s01 * s22);
+            const float c02 = fmaf(s01, s12, -s02 * s11), c11 = fmaf(s00, s22, -s02 * s02);
+            const float c12 = fmaf(s01, s02, -s00 * s12), c22 = fmaf(s00, s11, -s01 * s01);
+            det = fmaf(s00, c00, fmaf(s01, c01, s02 * c02));
+            const float t0 = fmaf(c00, v[0], fmaf(c01, v[1], c02 * v[2]));
+            const float t1 = fmaf(c01, v[0], fmaf(c11, v[1], c12 * v[2]));
+            const float t2 = fmaf(c02, v[0], fmaf(c12, v[1], c22 * v[2]));
+            quad = fmaf(t0, v[0], fmaf(t1, v[1], t2 * v[2]));
         }
-        float F2 = a.sdet * b.sdet / det;
-        float rs = fast_sqrt(quad / det);
+        const float inv = __builtin_amdgcn_rcpf(det);
+        const float F2 = (DIM == 3 ? 8.0f : 4.0f) * a.sdet * b.sdet * inv;
+        const float rs = fast_sqrt(fmaxf(0.0f, 2.0f * quad * inv));
         return F2 * wendland_c4(rs * kc.inv_s4) * wendland_c0(rs * kc.inv_s0);
     }
 }

@@ -36,6 +36,11 @@ struct Problem {
     const int *cp_window;      // per cp: lo[3], hi[3] in sample space (derivative gather)
     const int *reg_window;     // per cp: lo[3], hi[3] in cp space (regularizers)
     const int *offsets;        // D ints per subsampled cell
+    // weighted modes (1, 2): packed point states, two float4 per point (pstate_pack)
+    const float4 *cp_pstate;   // per control point, static
+    float4 *smp_pstate;        // per sample of the current evaluation, written by the value pass
+    float lam_cp_max;          // largest lam over the control points
+    float lam_w_max;           // largest lam over the points a sample can look up (weight image, outside)
 };
 
 // Device-resident scalars of one evaluation (all FP64).
@@ -64,6 +69,8 @@ void launch_regularizers(const Problem &pr, const double *weights, double pg_sca
                          float *deriv_soa, double *partials, Scalars *sc, hipStream_t s);
 unsigned reg_blocks(long long n_cp);  // workgroups (and 4-double partial slots) of launch_regularizers
 void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
+// packed states of the weighted modes at every voxel of the dense fixed grid
+void launch_voxel_states(const Problem &pr, float4 *out, hipStream_t s);
 void launch_result(const Scalars *sc, void *result, hipStream_t s);
 
 // LCC (skm_lcc.hip)

@@ -81,22 +81,20 @@ __device__ __forceinline__ void df_lookup(const Problem &pr, const float *x, flo
     }
 }
 
+// State of control point j in the weighted modes; returns its lam.
 template <int DIM, int MODE>
-__device__ __forceinline__ void cp_state_load(const Problem &pr, long long j, PState &st)
+__device__ __forceinline__ float cp_state_load(const Problem &pr, long long j, PState &st)
 {
-    if (is_stationary(MODE)) return;
-    const float *c = pr.cp_state + j * kCpStateStride;
-    st.sig = c[0];
-    st.sdet = c[1];
-    if (MODE == 2) {
-#pragma unroll
-        for (int i = 0; i < DIM * DIM; ++i) st.cov[i] = c[2 + i];
-    }
+    if (is_stationary(MODE)) return 0.0f;
+    const float4 b = pr.cp_pstate[2 * j + 1];
+    pstate_unpack<DIM, MODE>(pr.cp_pstate[2 * j], b, st);
+    return b.w;
 }
 
-// Kernel sum u(x) = sum_j w_j k(x, p_j) c_j over the control points in the
-// support ball of x (reference window GpuEvaluatorMSE.cu:61-80; rows whose
-// distance to x exceeds the support only hold k == 0 and are skipped).
+// Kernel sum u(x) = sum_j w_j k(x, p_j) c_j over the control points of the
+// reference window of x (GpuEvaluatorMSE.cu:61-80).  Rows farther from x than
+// any partner's kernel reaches (the support in the stationary modes,
+// pair_reach2 in the weighted modes) only hold k == 0 and are skipped.
 template <int DIM, int MODE>
 __device__ __forceinline__ void kernel_sum(const Problem &pr, const float *x, const PState &st, float *u)
 {
@@ -109,7 +107,8 @@ __device__ __forceinline__ void kernel_sum(const Problem &pr, const float *x, co
         lo[d] = clampi((int)(ci[d] - r + 1.0f), 0, pr.cp.n[d] - 1);
         hi[d] = clampi((int)(ci[d] + r), 0, pr.cp.n[d] - 1);
     }
-    const float s2 = st.support * st.support;
+    const float s2 = is_stationary(MODE) ? st.support * st.support
+                                         : pair_reach2(pr.kc.smin, pstate_lam<DIM, MODE>(st), pr.lam_cp_max);
     const int z0 = DIM == 3 ? lo[2] : 0, z1 = DIM == 3 ? hi[2] : 0;
     for (int cz = z0; cz <= z1; ++cz) {
         float dz = DIM == 3 ? x[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
@@ -178,6 +177,13 @@ __global__ __launch_bounds__(kThreads) void k_value(Problem pr, float4 *__restri
         to_global<DIM>(pr.fixed, fp, x);
         PState st;
         point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+        if (!is_stationary(MODE)) {
+            // the derivative gather reads the sample's state from here
+            float4 a, b;
+            pstate_pack<DIM, MODE>(st, 0.0f, a, b);
+            pr.smp_pstate[2 * s] = a;
+            if (MODE == 2) pr.smp_pstate[2 * s + 1] = b;
+        }
         float u[3];
         df_lookup<DIM>(pr, x, u);
         kernel_sum<DIM, MODE>(pr, x, st, u);
@@ -288,9 +294,10 @@ __global__ __launch_bounds__(kThreads) void k_prepare(Problem pr, const float4 *
 // evaluateDerivativeNCC, GpuEvaluatorNCC.cu:111-190, 265-340).  One workgroup
 // per control point; the window (reference or exact) was computed on the host
 // in FP64.  Lanes work in groups of 8 along x, 32 rows at a time; rows and
-// row ends outside the support ball are skipped (k == 0 there).  DENSE
-// (subsample 1, stationary kernel) walks a row with an incremental offset and
-// the row's constant y/z distance.
+// row ends beyond the kernel's reach are skipped (k == 0 there).  The
+// weighted modes read each sample's state from the value pass (smp_pstate).
+// DENSE (subsample 1, stationary kernel) walks a row with an incremental
+// offset and the row's constant y/z distance.
 template <int DIM, int MODE, bool DENSE>
 __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *__restrict__ sv,
                                                      float *__restrict__ deriv)
@@ -307,14 +314,14 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
     float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]}, p[3];
     to_global<DIM>(pr.cp, fj, p);
     PState st;
-    cp_state_load<DIM, MODE>(pr, j, st);
+    const float lam_p = cp_state_load<DIM, MODE>(pr, j, st);
     st.support = is_stationary(MODE) ? pr.kc.smin : pr.cp_state[j * kCpStateStride + 11];
     const int *win = pr.cp_window + j * 6;
     int lo[3] = {win[0], win[1], DIM == 3 ? win[2] : 0}, hi[3] = {win[3], win[4], DIM == 3 ? win[5] : 0};
     const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
     const int rows = by * bz;
     const int tx = threadIdx.x & 7, tr = threadIdx.x >> 3;
-    const float s2 = st.support * st.support;
+    const float s2 = is_stationary(MODE) ? st.support * st.support : pair_reach2(pr.kc.smin, lam_p, pr.lam_w_max);
     const int sub = pr.sub;
     const float span = sub > 1 ? (float)sub : 0.0f;  // jitter range of a cell
     float acc[3] = {0.0f, 0.0f, 0.0f};
@@ -375,7 +382,12 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
             to_global<DIM>(pr.fixed, fpos, xs);
             float v[3] = {p[0] - xs[0], p[1] - xs[1], DIM == 3 ? p[2] - xs[2] : 0.0f};
             PState sst;
-            if (!is_stationary(MODE)) point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, xs, sst);
+            if (!is_stationary(MODE)) {
+                const long long si = rowbase + mx;
+                pstate_unpack<DIM, MODE>(pr.smp_pstate[2 * si],
+                                         MODE == 2 ? pr.smp_pstate[2 * si + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f),
+                                         sst);
+            }
             float k = kernel_pair<DIM, MODE>(pr.kc, v, st, sst);
             acc[0] = fmaf(k, v4.x, acc[0]);
             acc[1] = fmaf(k, v4.y, acc[1]);
@@ -532,6 +544,26 @@ __global__ __launch_bounds__(kThreads) void k_displacement(Problem pr, float *__
     for (int d = 0; d < DIM; ++d) dst[v * DIM + d] = u[d];
 }
 
+// Packed states (pstate_pack) of the weighted modes at every fixed voxel,
+// for gathers over the dense grid (the exact LCC derivative).
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_voxel_states(Problem pr, float4 *__restrict__ out)
+{
+    const long long v = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (v >= pr.n_vox) return;
+    if (is_stationary(MODE)) return;
+    int m[3] = {(int)(v % pr.fixed.n[0]), (int)((v / pr.fixed.n[0]) % pr.fixed.n[1]),
+                (int)(v / ((long long)pr.fixed.n[0] * pr.fixed.n[1]))};
+    float fm[3] = {(float)m[0], (float)m[1], (float)m[2]}, x[3];
+    to_global<DIM>(pr.fixed, fm, x);
+    PState st;
+    point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+    float4 a, b;
+    pstate_pack<DIM, MODE>(st, 0.0f, a, b);
+    out[2 * v] = a;
+    out[2 * v + 1] = b;
+}
+
 // ------------------------------------------------------------ small kernels
 
 // PCG32 cube offsets: cell i draws numbers i*D .. i*D+D-1 of the stream, so
@@ -686,6 +718,17 @@ static void displacement_impl(const Problem &pr, float *dst, hipStream_t s)
 void launch_displacement(const Problem &pr, float *dst, hipStream_t s)
 {
     SKM_DISPATCH_DIM_MODE(pr, displacement_impl, pr, dst, s);
+}
+
+template <int DIM, int MODE>
+static void voxel_states_impl(const Problem &pr, float4 *out, hipStream_t s)
+{
+    k_voxel_states<DIM, MODE><<<(unsigned)((pr.n_vox + kThreads - 1) / kThreads), kThreads, 0, s>>>(pr, out);
+}
+
+void launch_voxel_states(const Problem &pr, float4 *out, hipStream_t s)
+{
+    SKM_DISPATCH_DIM_MODE(pr, voxel_states_impl, pr, out, s);
 }
 
 void launch_result(const Scalars *sc, void *result, hipStream_t s)

@@ -45,7 +45,13 @@ __device__ __forceinline__ void lcc_kernel_sum(const Problem &pr, const float *y
         lo[d] = clampi((int)(ci[d] - r + 1.0f), 0, pr.cp.n[d] - 1);
         hi[d] = clampi((int)(ci[d] + r), 0, pr.cp.n[d] - 1);
     }
-    const float s2 = st.support * st.support;
+    // rows beyond the kernel's reach hold k == 0 (with sigma(y) twice the
+    // pair's lam sum is 2 lam(y))
+    float s2 = st.support * st.support;
+    if (MODE != 0) {
+        const float lam_y = pstate_lam<DIM, MODE>(st);
+        s2 = pair_reach2(pr.kc.smin, lam_y, QUIRK ? lam_y : pr.lam_cp_max);
+    }
     const int z0 = DIM == 3 ? lo[2] : 0, z1 = DIM == 3 ? hi[2] : 0;
     for (int cz = z0; cz <= z1; ++cz) {
         float dz = DIM == 3 ? y[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
@@ -66,13 +72,8 @@ __device__ __forceinline__ void lcc_kernel_sum(const Problem &pr, const float *y
                     k = kernel_pair<DIM, MODE>(pr.kc, v, st, st);
                 } else {
                     PState cst;
-                    if (MODE != 0) {
-                        const float *c = pr.cp_state + (row + cx) * kCpStateStride;
-                        cst.sig = c[0];
-                        cst.sdet = c[1];
-#pragma unroll
-                        for (int i = 0; i < DIM * DIM; ++i) cst.cov[i] = c[2 + i];
-                    }
+                    if (MODE != 0)
+                        pstate_unpack<DIM, MODE>(pr.cp_pstate[2 * (row + cx)], pr.cp_pstate[2 * (row + cx) + 1], cst);
                     k = kernel_pair<DIM, MODE>(pr.kc, v, st, cst);
                 }
                 float4 c = pr.cpc[row + cx];

@@ -105,3 +105,42 @@ def test_generated_tensor_drives_the_evaluator(dim, measure):
     ores, oder = case.oracle().evaluate(int(measure), True)
     assert scalar_rel(res.measure, ores["measure"]) < 1e-5
     assert rel_err(der, oder) < 1e-4
+
+
+@pytest.mark.parametrize("dim,mode", [(3, 2), (2, 2), (3, 1), (2, 1)])
+@pytest.mark.parametrize("measure", [MEASURE.MSE, MEASURE.NCC])
+def test_kernel_reach_beyond_the_support_estimate(dim, mode, measure):
+    """Control points whose weight (16) exceeds maximum_weight /
+    maximum_eigenvalue_covariance (1): the reference window of a sample
+    (support from its own state) then holds pairs with k != 0 beyond that
+    support.  Every such pair must be evaluated: the window
+    (GpuEvaluatorMSE.cu:61-80, sized by Kernel.h:158-171) decides, not the
+    support estimate."""
+    case = Case(dim=dim, mode=mode, cp_spacing=4.0, sigma=8.0, subsample=1, seed=21)
+    big = np.zeros(case.shape, bool)
+    for j in range(case.n_cp):
+        idx, r = [], j
+        for d in range(dim):
+            idx.append(r % case.cp_size[d])
+            r //= case.cp_size[d]
+        vox = [int(case.cp_origin[d] + case.cp_spacing[d] * idx[d] + 0.5) for d in range(dim)]
+        if all(0 <= vox[d] < case.shape[dim - 1 - d] for d in range(dim)):
+            big[tuple(reversed(vox))] = True
+    if mode == 2:
+        w = np.broadcast_to(np.eye(dim, dtype=np.float32), case.shape + (dim, dim)).copy()
+        w[big] = 16.0 * np.eye(dim, dtype=np.float32)
+        case.weight = w
+        case.max_eig = np.eye(dim)
+    else:
+        case.weight = np.where(big, 16.0, 1.0).astype(np.float32)
+        case.max_weight = 1.0
+    print(f"reach dim={dim} mode={mode} measure={measure.name} big={int(big.sum())}", flush=True)
+    ev = case.gpu()
+    res, der = ev.getValueAndDerivative(measure, case.cp, case.cpw)
+    disp = ev.evaluateDisplacementField(case.cp, case.cpw)
+    ev.close()
+    orc_pr = case.oracle()
+    ores, oder = orc_pr.evaluate(int(measure), True)
+    assert scalar_rel(res.measure, ores["measure"]) < 1e-5
+    assert rel_err(der, oder) < 1e-4
+    assert rel_err(disp.reshape(-1), orc_pr.displacement().reshape(-1)) < 1e-5
