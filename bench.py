@@ -3,8 +3,11 @@
 Workload (default): the 3D registration of 128x128x96 float32 volumes
 (random smoothed ellipsoids, target warped by a smooth field, noise), control
 points every 6 voxels (22x22x16 = 7,744, 23,232 parameters), Wendland support
-24 voxels, NCC metric, RD regularizer, dense sampling.  One step is one
+24 voxels, anisotropic kernel tensors computed from the target's structure
+(ComputeAnisotropicTensor with tensor weight 400, alignment z, as SKMReg does
+by default), NCC metric, RD regularizer, dense sampling.  One step is one
 value-and-derivative evaluation with every input resident in HBM.
+`--kernel stationary` runs the isotropic kernel instead.
 
 metric: voxel x control-point pair evaluations per second (GVox*ctrl/s),
 counting the pairs enumerated by the reference windows in the value and the
@@ -23,7 +26,10 @@ sys.path.insert(0, REPO)
 
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-FLOPS_PER_PAIR = 32  # Wendland C4*C0 evaluation + weighted accumulate, stationary kernel
+# algorithmic FP32 flops per (voxel, control point) pair: Wendland C4*C0 and
+# the weighted accumulate (32); the anisotropic kernel adds the Mahalanobis
+# distance and the F factor (75 more)
+FLOPS_PER_PAIR = {"stationary": 32, "anisotropic": 32 + 75}
 
 
 def parse():
@@ -34,53 +40,22 @@ def parse():
     ap.add_argument("--shape", default="96,128,128", help="z,y,x voxels")
     ap.add_argument("--cp-spacing", type=float, default=6.0)
     ap.add_argument("--sigma", type=float, default=24.0)
+    ap.add_argument("--kernel", default="anisotropic", choices=["anisotropic", "stationary"])
+    ap.add_argument("--tensor-weight", type=float, default=400.0)
     ap.add_argument("--subsample", type=int, default=1)
     ap.add_argument("--measure", default="ncc", choices=["mse", "ncc", "lcc"])
     ap.add_argument("--reg-rd", type=float, default=1e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-crop", default="64,96,96", help="z,y,x voxels of the CPU baseline sample")
+    ap.add_argument("--no-pair-count", action="store_true", help="skip the exact k != 0 pair count")
+    ap.add_argument("--cpu-crop", default=None, help="z,y,x voxels of the CPU baseline sample")
     return ap.parse_args()
 
 
-def window_counts(n, origin, spacing, cp_n, cp_origin, cp_spacing, support, sub, tight):
-    """Per-axis counts of the reference windows: for value (tight=True) the cp
-    window around each sample; for the derivative the sample window per cp."""
-    if tight:
-        idx = (origin + spacing * np.arange(0, (n // sub) * sub, sub) - cp_origin) / cp_spacing
-        r = support / cp_spacing
-        lo = np.clip(np.trunc(idx - r + 1.0), 0, cp_n - 1)
-        hi = np.clip(np.trunc(idx + r), 0, cp_n - 1)
-        return hi - lo + 1
-    pc = cp_origin + cp_spacing * np.arange(cp_n)
-    idx = (pc - origin) / spacing
-    r = support / spacing
-    lim = (n // sub) - 1
-    lo = np.clip(np.trunc((idx - r) / sub), 0, lim)
-    hi = np.clip(np.trunc((idx + r) / sub + 1.0), 0, lim)
-    return hi - lo + 1
-
-
-def support_pairs(sample_axes, cp_axes, support):
-    """Exact number of (sample, cp) pairs with |x - p| < support (k != 0)."""
-    d2 = [(s[:, None] - c[None, :]) ** 2 for s, c in zip(sample_axes, cp_axes)]
-    s2 = support * support
-    dx = np.sort(d2[0].ravel())
-    rest = d2[1].ravel()
-    if len(d2) == 3:
-        rest = (d2[1].ravel()[:, None] + d2[2].ravel()[None, :]).ravel()
-    t = s2 - rest
-    t = t[t > 0]
-    return int(np.searchsorted(dx, t, side="left").sum())
-
-
-def workload(args):
-    from skmimageregistration_amd import synthetic
-    shape = tuple(int(v) for v in args.shape.split(","))
-    mov, fix, _ = synthetic.ellipsoids_3d(shape, n=20, smooth=1.5, max_disp=6.0, noise=0.02)
-    return mov, fix
-
+# ------------------------------------------------------------ geometry
 
 def geometry(shape, cp_spacing):
+    """Control-point grid of TransformAdapter::ComputeControlPointImage for a
+    unit-spacing image at the origin: (image size xyz, cp size xyz, cp origin xyz)."""
     size = tuple(reversed(shape))
     cs, co = [], []
     for d in range(3):
@@ -91,38 +66,216 @@ def geometry(shape, cp_spacing):
     return size, tuple(cs), tuple(co)
 
 
-def pair_counts(shape, cp_spacing, sigma, sub):
-    size, cs, co = geometry(shape, cp_spacing)
-    val = np.prod([window_counts(size[d], 0.0, 1.0, cs[d], co[d], cp_spacing, sigma, sub, True).sum()
-                   for d in range(3)])
-    der = np.prod([window_counts(size[d], 0.0, 1.0, cs[d], co[d], cp_spacing, sigma, sub, False).sum()
-                   for d in range(3)])
-    samples = [np.arange(0, (size[d] // sub) * sub, sub, dtype=np.float64) for d in range(3)]
-    cps = [co[d] + cp_spacing * np.arange(cs[d]) for d in range(3)]
-    insup = support_pairs(samples, cps, sigma) if sub == 1 else None
-    return int(val), int(der), insup
+def value_windows(size, cs, co, h, support, sub=1):
+    """Reference cp window of every sample (GpuEvaluatorMSE.cu:61-80), samples
+    at the cell origins: per-axis lo/hi arrays (N, 3); support per sample (N,)."""
+    nx, ny, nz = size
+    z, y, x = np.meshgrid(*[np.arange(0, (n // sub) * sub, sub) for n in (nz, ny, nx)], indexing="ij")
+    pos = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.float64)
+    ci = (pos - np.asarray(co)) / h
+    r = (np.asarray(support, np.float64) / h).reshape(-1, 1) * np.ones((1, 3))
+    lo = np.clip(np.trunc(ci - r + 1.0), 0, np.asarray(cs) - 1).astype(np.int64)
+    hi = np.clip(np.trunc(ci + r), 0, np.asarray(cs) - 1).astype(np.int64)
+    return lo, hi
 
 
-def cpu_baseline(args, mov, fix):
+def derivative_windows(size, cs, co, h, support, sub=1):
+    """Sample-cell window of every control point (3D derivative kernels,
+    GpuEvaluatorMSE.cu:239-309): lo/hi arrays (N_cp, 3)."""
+    ix, iy, iz = (np.arange(c) for c in cs)
+    z, y, x = np.meshgrid(iz, iy, ix, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1) * h + np.asarray(co)
+    r = np.asarray(support, np.float64).reshape(-1, 1) * np.ones((1, 3))
+    lim = np.asarray(size) // sub - 1
+    lo = np.clip(np.trunc((p - r) / sub), 0, lim).astype(np.int64)
+    hi = np.clip(np.trunc((p + r) / sub + 1.0), 0, lim).astype(np.int64)
+    return lo, hi, p
+
+
+def box_pairs(lo, hi):
+    return int(np.prod(hi - lo + 1, axis=1).sum())
+
+
+def support_pairs(sample_axes, cp_axes, support):
+    """Exact number of (sample, cp) pairs with |x - p| < support (stationary k != 0)."""
+    d2 = [(s[:, None] - c[None, :]) ** 2 for s, c in zip(sample_axes, cp_axes)]
+    s2 = support * support
+    dx = np.sort(d2[0].ravel())
+    rest = (d2[1].ravel()[:, None] + d2[2].ravel()[None, :]).ravel()
+    t = s2 - rest
+    t = t[t > 0]
+    return int(np.searchsorted(dx, t, side="left").sum())
+
+
+# ------------------------------------------------------------ anisotropic pair counts
+
+def upper(t):
+    """Symmetric part of (..., 3, 3) as its upper triangle (..., 6): xx xy xz yy yz zz."""
+    s = 0.5 * (t + np.swapaxes(t, -1, -2))
+    return np.stack([s[..., 0, 0], s[..., 0, 1], s[..., 0, 2], s[..., 1, 1], s[..., 1, 2], s[..., 2, 2]], axis=-1)
+
+
+def tensor_supports(ten, mcov, smin):
+    """Region support smin / sqrt(2 / lambda_max(maxEigCov + Sigma)) per voxel (Kernel.h:166-171)."""
+    lam = np.linalg.eigvalsh(ten.reshape(-1, 3, 3).astype(np.float64) + mcov[None])[:, -1]
+    return smin / np.sqrt(2.0 / lam)
+
+
+def _nonzero(torch, s, v, smin):
+    """k != 0  <=>  r = sqrt(2 v^T adj(S) v / det S) < smin for S = S_p + S_q."""
+    s00, s01, s02, s11, s12, s22 = (s[..., i] for i in range(6))
+    c00 = s11 * s22 - s12 * s12
+    c01 = s02 * s12 - s01 * s22
+    c02 = s01 * s12 - s02 * s11
+    c11 = s00 * s22 - s02 * s02
+    c12 = s01 * s02 - s00 * s12
+    c22 = s00 * s11 - s01 * s01
+    det = s00 * c00 + s01 * c01 + s02 * c02
+    vx, vy, vz = v
+    quad = (c00 * vx * vx + c11 * vy * vy + c22 * vz * vz
+            + 2.0 * (c01 * vx * vy + c02 * vx * vz + c12 * vy * vz))
+    return 2.0 * quad < smin * smin * det
+
+
+def count_nonzero_pairs(ten, size, cs, co, h, smin, vlo, vhi, dlo, dhi, dev):
+    """Exact numbers of pairs with k != 0 inside the value windows and inside
+    the derivative windows, counted with torch on the GPU (float64)."""
+    import torch
+    nx, ny, nz = size
+    up = torch.tensor(upper(ten.astype(np.float64)).reshape(-1, 6), device=dev)
+    # control-point states: the tensor at the nearest voxel
+    ix, iy, iz = (np.arange(c) for c in cs)
+    z, y, x = np.meshgrid(iz, iy, ix, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1) * h + np.asarray(co)
+    nv = np.clip((p + 0.5).astype(np.int64), 0, np.asarray(size) - 1)
+    cp_up = up[torch.tensor(nv[:, 0] + nx * (nv[:, 1] + ny * nv[:, 2]), device=dev)]
+    cp_p = torch.tensor(p, device=dev)
+
+    # derivative windows: per cp, voxels of its box
+    dlo_t, dhi_t = torch.tensor(dlo, device=dev), torch.tensor(dhi, device=dev)
+    W = (dhi - dlo + 1).max(axis=0)
+    off = [torch.arange(int(W[d]), device=dev) for d in range(3)]
+    der = 0
+    B = 8
+    for j0 in range(0, len(p), B):
+        lo, hi = dlo_t[j0:j0 + B], dhi_t[j0:j0 + B]
+        ax = [lo[:, d, None] + off[d][None, :] for d in range(3)]               # (B, W_d)
+        ok = [a <= hi[:, d, None] for d, a in enumerate(ax)]
+        axc = [torch.clamp(a, max=n - 1) for a, n in zip(ax, (nx, ny, nz))]
+        flat = (axc[2][:, :, None, None] * ny + axc[1][:, None, :, None]) * nx + axc[0][:, None, None, :]
+        s = up[flat] + cp_up[j0:j0 + B][:, None, None, None, :]
+        pp = cp_p[j0:j0 + B]
+        v = (pp[:, 0, None, None, None] - ax[0][:, None, None, :].double(),
+             pp[:, 1, None, None, None] - ax[1][:, None, :, None].double(),
+             pp[:, 2, None, None, None] - ax[2][:, :, None, None].double())
+        mask = ok[2][:, :, None, None] & ok[1][:, None, :, None] & ok[0][:, None, None, :]
+        der += int((_nonzero(torch, s, v, smin) & mask).sum())
+
+    # value windows: per sample, control points of its box
+    vlo_t, vhi_t = torch.tensor(vlo, device=dev), torch.tensor(vhi, device=dev)
+    Wc = (vhi - vlo + 1).max(axis=0)
+    offc = [torch.arange(int(Wc[d]), device=dev) for d in range(3)]
+    ncx, ncy, ncz = cs
+    val = 0
+    S = 4096
+    n = len(vlo)
+    for i0 in range(0, n, S):
+        idx = torch.arange(i0, min(i0 + S, n), device=dev)
+        xs = torch.stack([idx % nx, (idx // nx) % ny, idx // (nx * ny)], dim=1).double()
+        lo, hi = vlo_t[i0:i0 + S], vhi_t[i0:i0 + S]
+        ax = [lo[:, d, None] + offc[d][None, :] for d in range(3)]
+        ok = [a <= hi[:, d, None] for d, a in enumerate(ax)]
+        axc = [torch.clamp(a, max=m - 1) for a, m in zip(ax, (ncx, ncy, ncz))]
+        flat = (axc[2][:, :, None, None] * ncy + axc[1][:, None, :, None]) * ncx + axc[0][:, None, None, :]
+        s = cp_up[flat] + up[idx][:, None, None, None, :]
+        v = (xs[:, 0, None, None, None] - (co[0] + h * ax[0][:, None, None, :].double()),
+             xs[:, 1, None, None, None] - (co[1] + h * ax[1][:, None, :, None].double()),
+             xs[:, 2, None, None, None] - (co[2] + h * ax[2][:, :, None, None].double()))
+        mask = ok[2][:, :, None, None] & ok[1][:, None, :, None] & ok[0][:, None, None, :]
+        val += int((_nonzero(torch, s, v, smin) & mask).sum())
+    return val, der
+
+
+# ------------------------------------------------------------ workload
+
+def workload(args):
+    from skmimageregistration_amd import synthetic
+    shape = tuple(int(v) for v in args.shape.split(","))
+    mov, fix, _ = synthetic.ellipsoids_3d(shape, n=20, smooth=1.5, max_disp=6.0, noise=0.02)
+    return mov, fix
+
+
+def kernel_for(args, fix):
+    """Kernel of the workload; anisotropic: ComputeAnisotropicTensor on the
+    target (SKMReg.cxx:268-273) and SetWeightingTensor's maximum values."""
+    from skmimageregistration_amd import ComputeAnisotropicTensor, Grid, Image, Kernel
+    if args.kernel == "stationary":
+        return Kernel(args.sigma, args.sigma), None, None, None
+    t0 = time.perf_counter()
+    ten, tilt, mw, mcov = ComputeAnisotropicTensor(Image(fix), args.sigma, args.tensor_weight, 2)
+    tensor_ms = (time.perf_counter() - t0) * 1e3
+    size = tuple(reversed(fix.shape))
+    k = Kernel(args.sigma, args.sigma, useWeightTensor=True, maximumWeight=mw, maximumEigenvalueCovariance=mcov,
+               wTensor=(Grid(size, (0.0,) * 3, (1.0,) * 3), ten))
+    return k, ten, mcov, tensor_ms
+
+
+def pair_counts(args, shape, ten, mcov, dev, exact_nonzero):
+    """(value box pairs, derivative box pairs, value k!=0 pairs, derivative k!=0 pairs)."""
+    size, cs, co = geometry(shape, args.cp_spacing)
+    h = args.cp_spacing
+    sub = args.subsample
+    n_cells = int(np.prod([n // sub for n in size]))
+    if ten is None:
+        sv = np.full(n_cells, args.sigma)
+        sd = np.full(int(np.prod(cs)), args.sigma)
+    else:
+        sup = tensor_supports(ten, mcov, args.sigma).reshape(tuple(reversed(size)))
+        sv = sup[::sub, ::sub, ::sub][: size[2] // sub, : size[1] // sub, : size[0] // sub].ravel()
+        _, _, p = derivative_windows(size, cs, co, h, np.zeros(int(np.prod(cs))))
+        nv = np.clip((p + 0.5).astype(np.int64), 0, np.asarray(size) - 1)
+        sd = sup.ravel()[nv[:, 0] + size[0] * (nv[:, 1] + size[1] * nv[:, 2])]
+    vlo, vhi = value_windows(size, cs, co, h, sv, sub)
+    dlo, dhi, _ = derivative_windows(size, cs, co, h, sd, sub)
+    vbox, dbox = box_pairs(vlo, vhi), box_pairs(dlo, dhi)
+    vnz = dnz = None
+    if exact_nonzero:
+        if ten is None:
+            samples = [np.arange(size[d], dtype=np.float64) for d in range(3)]
+            cps = [co[d] + h * np.arange(cs[d]) for d in range(3)]
+            vnz = dnz = support_pairs(samples, cps, args.sigma)
+        else:
+            vnz, dnz = count_nonzero_pairs(ten, size, cs, co, h, args.sigma, vlo, vhi, dlo, dhi, dev)
+    return vbox, dbox, vnz, dnz
+
+
+def cpu_baseline(args, mov, fix, ten, mcov):
     """FP64 oracle (single thread) on a crop of the same volumes, same kernel
     and control-point spacing; reported in the same unit."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
-    crop = tuple(int(v) for v in args.cpu_crop.split(","))
-    f = fix[: crop[0], : crop[1], : crop[2]]
-    m = mov[: crop[0], : crop[1], : crop[2]]
+    default = "64,96,96" if ten is None else "48,64,64"
+    crop = tuple(int(v) for v in (args.cpu_crop or default).split(","))
+    f = np.ascontiguousarray(fix[: crop[0], : crop[1], : crop[2]])
+    m = np.ascontiguousarray(mov[: crop[0], : crop[1], : crop[2]])
     size, cs, co = geometry(crop, args.cp_spacing)
     rng = np.random.default_rng(0)
     cp = rng.uniform(-1, 1, size=(int(np.prod(cs)), 3))
-    pr = orc.OracleProblem(f, m, (cs, co, (args.cp_spacing,) * 3), cp, sigma_c0=args.sigma, sigma_c4=args.sigma)
+    kw = {}
+    tc = None
+    if ten is not None:
+        tc = np.ascontiguousarray(ten[: crop[0], : crop[1], : crop[2]])
+        kw = dict(kernel_mode=2, max_eig_cov=mcov, weight=(size, (0.0,) * 3, (1.0,) * 3, tc))
+    pr = orc.OracleProblem(f, m, (cs, co, (args.cp_spacing,) * 3), cp, sigma_c0=args.sigma, sigma_c4=args.sigma,
+                           **kw)
     measure = {"mse": 0, "ncc": 1, "lcc": 2}[args.measure]
     t0 = time.perf_counter()
     pr.evaluate(measure, True, 0.0, args.reg_rd, 0.0)
     dt = time.perf_counter() - t0
-    v, d, _ = pair_counts(crop, args.cp_spacing, args.sigma, 1)
+    v, d, _, _ = pair_counts(args, crop, tc, mcov, None, False)
     return {"value": (v + d) / dt / 1e9, "unit": "GVox*ctrl/s", "cores": 1, "kind": "port",
             "sample": f"FP64 oracle, one GetValueAndDerivative on a {crop[2]}x{crop[1]}x{crop[0]} crop "
-                      f"(same sigma and cp spacing), {dt:.1f} s"}
+                      f"(same kernel, sigma and cp spacing), {dt:.1f} s"}
 
 
 def main():
@@ -133,14 +286,15 @@ def main():
         sys.exit(2)
     import torch
 
-    from skmimageregistration_amd import GpuKernelEvaluator, Grid, Image, Kernel, MEASURE
+    from skmimageregistration_amd import GpuKernelEvaluator, Grid, Image, MEASURE
 
     mov, fix = workload(args)
     shape = fix.shape
     size, cs, co = geometry(shape, args.cp_spacing)
     n_cp = int(np.prod(cs))
     measure = {"mse": MEASURE.MSE, "ncc": MEASURE.NCC, "lcc": MEASURE.LCC}[args.measure]
-    ev = GpuKernelEvaluator(n_cp * 3, args.subsample, 1, Kernel(args.sigma, args.sigma), Image(mov), Image(fix),
+    kernel, ten, mcov, tensor_ms = kernel_for(args, fix)
+    ev = GpuKernelEvaluator(n_cp * 3, args.subsample, 1, kernel, Image(mov), Image(fix),
                             Grid(cs, co, (args.cp_spacing,) * 3))
     if args.reg_rd > 0:
         ev.SetRegularizerRD(args.reg_rd)
@@ -175,35 +329,43 @@ def main():
         step()
     passes = ev.read_timing()
     ev.enable_timing(False)
+    ev.close()
 
-    val_pairs, der_pairs, insup = pair_counts(shape, args.cp_spacing, args.sigma, args.subsample)
-    value = (val_pairs + der_pairs) / (ms_step * 1e-3) / 1e9
+    exact = args.subsample == 1 and not args.no_pair_count
+    vbox, dbox, vnz, dnz = pair_counts(args, shape, ten, mcov, dev, exact)
+    value = (vbox + dbox) / (ms_step * 1e-3) / 1e9
     per_kernel = {k: v[0] / max(v[1], 1) for k, v in passes.items() if v[1] > 0}
     dom = max(("value", "gather"), key=lambda k: per_kernel.get(k, 0.0))
     roof = None
-    if insup is not None:
-        achieved = insup * FLOPS_PER_PAIR / (per_kernel[dom] * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": "k_value" if dom == "value" else "k_gather", "achieved": round(achieved, 3),
-                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": None, "launch_ms": round(per_kernel[dom], 4),
-                "pairs_per_launch": insup, "flops_per_pair": FLOPS_PER_PAIR}
+    fpp = FLOPS_PER_PAIR[args.kernel]
+    if vnz is not None:
+        pairs = vnz if dom == "value" else dnz
+        achieved = pairs * fpp / (per_kernel[dom] * 1e-3) / 1e12
+        roof = {"bound": "valu", "kernel": "k_value" if dom == "value" else "k_gather",
+                "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "launch_ms": round(per_kernel[dom], 4), "pairs_per_launch": pairs, "flops_per_pair": fpp}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, mov, fix)
+        cpu = cpu_baseline(args, mov, fix, ten, mcov)
+    kdesc = ("anisotropic Wendland (tensors from the target, weight %g)" % args.tensor_weight
+             if args.kernel == "anisotropic" else "stationary Wendland")
     line = {
         "metric": "GVox*ctrl/s (voxel x control-point pair evaluations per second, value + derivative pass)",
         "value": round(value, 3), "unit": "GVox*ctrl/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (FP64 final sums)", "data": "synthetic",
         "config": {"workload": f"SKM GetValueAndDerivative, 3D {size[0]}x{size[1]}x{size[2]}, cp {cs[0]}x{cs[1]}x{cs[2]} "
-                               f"spacing {args.cp_spacing:g}, stationary Wendland support {args.sigma:g}, "
+                               f"spacing {args.cp_spacing:g}, {kdesc} support {args.sigma:g}, "
                                f"{args.measure.upper()} + RD {args.reg_rd:g}, subsample {args.subsample}",
-                   "pairs_value": val_pairs, "pairs_derivative": der_pairs, "pairs_in_support": insup},
+                   "pairs_value": vbox, "pairs_derivative": dbox,
+                   "pairs_nonzero_value": vnz, "pairs_nonzero_derivative": dnz},
         "pass_ms": {k: round(v, 4) for k, v in per_kernel.items()},
         "roofline": roof, "cpu_baseline": cpu,
     }
+    if tensor_ms is not None:
+        line["tensor_generation_ms"] = round(tensor_ms, 2)
     print(json.dumps(line))
-    ev.close()
 
 
 if __name__ == "__main__":

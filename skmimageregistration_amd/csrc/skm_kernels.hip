@@ -9,6 +9,8 @@
 // derivative is a per-control-point gather by a whole workgroup over the
 // sample window with rows pruned to the kernel ball, and every reduction is a
 // fixed-order FP64 tree (per-block partials, then one finishing block).
+#include <climits>
+
 #include "skm_internal.hpp"
 
 #include "skm.h"
@@ -150,6 +152,76 @@ __device__ __forceinline__ void kernel_sum(const Problem &pr, const float *x, co
     }
 }
 
+__device__ __forceinline__ int wave_min_i(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Kernel sum of the weighted modes with a wave-uniform control-point loop:
+// the wave walks the union of its lanes' reference windows row by row, so
+// every control point's state and coefficients are scalar loads shared by
+// the 64 lanes; a lane adds a pair only inside its own window and reach.
+// Every lane of the wave must call this (inactive lanes pass active = false).
+template <int DIM, int MODE>
+__device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *x, const PState &st, bool active,
+                                                float *u)
+{
+    float ci[3];
+    to_local<DIM>(pr.cp, x, ci);
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        float r = st.support * pr.cp.ih[d];
+        lo[d] = active ? clampi((int)(ci[d] - r + 1.0f), 0, pr.cp.n[d] - 1) : INT_MAX;
+        hi[d] = active ? clampi((int)(ci[d] + r), 0, pr.cp.n[d] - 1) : INT_MIN;
+    }
+    int ulo[3] = {0, 0, 0}, uhi[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        ulo[d] = __builtin_amdgcn_readfirstlane(wave_min_i(lo[d]));
+        uhi[d] = __builtin_amdgcn_readfirstlane(wave_max_i(hi[d]));
+    }
+    const float s2 = pair_reach2(pr.kc.smin, pstate_lam<DIM, MODE>(st), pr.lam_cp_max);
+    for (int cz = ulo[2]; cz <= uhi[2]; ++cz) {
+        const float dz = DIM == 3 ? x[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
+        const bool zin = DIM == 2 || (cz >= lo[2] && cz <= hi[2]);
+        for (int cy = ulo[1]; cy <= uhi[1]; ++cy) {
+            const float dy = x[1] - (pr.cp.o[1] + pr.cp.h[1] * cy);
+            const float rem = s2 - dy * dy - dz * dz;
+            int xa = INT_MAX, xb = INT_MIN;
+            if (zin && cy >= lo[1] && cy <= hi[1] && rem > 0.0f) {
+                const float rx = fast_sqrt(rem) * pr.cp.ih[0];
+                xa = max(lo[0], (int)(ci[0] - rx));
+                xb = min(hi[0], (int)(ci[0] + rx) + 1);
+            }
+            const int uxa = __builtin_amdgcn_readfirstlane(wave_min_i(xa));
+            const int uxb = __builtin_amdgcn_readfirstlane(wave_max_i(xb));
+            const long long row = (long long)pr.cp.n[0] * (cy + (long long)pr.cp.n[1] * cz);
+            for (int cx = uxa; cx <= uxb; ++cx) {
+                PState cst;
+                cp_state_load<DIM, MODE>(pr, row + cx, cst);
+                const float4 c = pr.cpc[row + cx];
+                if (cx >= xa && cx <= xb) {
+                    const float v[3] = {x[0] - (pr.cp.o[0] + pr.cp.h[0] * cx), dy, dz};
+                    const float wk = c.w * kernel_pair<DIM, MODE>(pr.kc, v, st, cst);
+                    u[0] = fmaf(wk, c.x, u[0]);
+                    u[1] = fmaf(wk, c.y, u[1]);
+                    if (DIM == 3) u[2] = fmaf(wk, c.z, u[2]);
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------ value pass
 
 // MSE / NCC value pass (evaluateValueMSE, GpuEvaluatorMSE.cu:29-95, 174-237;
@@ -165,18 +237,25 @@ __global__ __launch_bounds__(kThreads) void k_value(Problem pr, float4 *__restri
     tile_coords<DIM>(threadIdx.x, l);
     int m[3] = {(int)blockIdx.x * Tile<DIM>::X + l[0], (int)blockIdx.y * Tile<DIM>::Y + l[1],
                 DIM == 3 ? (int)blockIdx.z * Tile<DIM>::Z + l[2] : 0};
-    bool valid = m[0] < pr.nsub[0] && m[1] < pr.nsub[1] && (DIM == 2 || m[2] < pr.nsub[2]);
+    const bool valid = m[0] < pr.nsub[0] && m[1] < pr.nsub[1] && (DIM == 2 || m[2] < pr.nsub[2]);
     double acc[kPartialsK];
 #pragma unroll
     for (int k = 0; k < kPartialsK; ++k) acc[k] = 0.0;
+    // lanes past the sample grid run the kernel sum of sample 0 (the weighted
+    // modes' wave-uniform loop needs every lane) and store nothing
+    if (!valid) m[0] = m[1] = m[2] = 0;
+    const long long s = m[0] + (long long)pr.nsub[0] * (m[1] + (long long)pr.nsub[1] * m[2]);
+    int pos[3] = {0, 0, 0};
+    sample_position<DIM>(pr, m, s, pos);
+    float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, x[3];
+    to_global<DIM>(pr.fixed, fp, x);
+    PState st;
+    point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+    float u[3];
+    df_lookup<DIM>(pr, x, u);
+    if constexpr (is_stationary(MODE)) kernel_sum<DIM, MODE>(pr, x, st, u);
+    else kernel_sum_wave<DIM, MODE>(pr, x, st, valid, u);
     if (valid) {
-        long long s = m[0] + (long long)pr.nsub[0] * (m[1] + (long long)pr.nsub[1] * m[2]);
-        int pos[3] = {0, 0, 0};
-        sample_position<DIM>(pr, m, s, pos);
-        float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, x[3];
-        to_global<DIM>(pr.fixed, fp, x);
-        PState st;
-        point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
         if (!is_stationary(MODE)) {
             // the derivative gather reads the sample's state from here
             float4 a, b;
@@ -184,9 +263,6 @@ __global__ __launch_bounds__(kThreads) void k_value(Problem pr, float4 *__restri
             pr.smp_pstate[2 * s] = a;
             if (MODE == 2) pr.smp_pstate[2 * s + 1] = b;
         }
-        float u[3];
-        df_lookup<DIM>(pr, x, u);
-        kernel_sum<DIM, MODE>(pr, x, st, u);
         float y[3];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) y[d] = x[d] + u[d];
@@ -400,6 +476,145 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
         double w = pr.cpc[j].w;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) deriv[j + d * pr.n_cp] = (float)(w * dacc[d]);
+    }
+}
+
+// Control-point tile of the weighted-mode gather.
+template <int DIM> struct CpTile;
+template <> struct CpTile<2> { static constexpr int X = 4, Y = 2, Z = 1; };
+template <> struct CpTile<3> { static constexpr int X = 2, Y = 2, Z = 2; };
+
+// Derivative gather of the weighted modes: one workgroup per tile of
+// control points.  The workgroup walks the union of the tile's reference
+// windows once; each lane loads a sample (v, state) once and adds its pair
+// with every tile control point whose window and reach hold the sample.
+// Control-point data are wave-uniform (scalar registers).  Same sums per
+// control point as k_gather, in a fixed order.
+template <int DIM, int MODE>
+__global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const float4 *__restrict__ sv,
+                                                          float *__restrict__ deriv)
+{
+    constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z, T = TX * TY * TZ;
+    __shared__ double lds[T * 3 * 4];
+    const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
+    const int t0 = (int)(blockIdx.x % ntx), t1 = (int)((blockIdx.x / ntx) % nty), t2 = (int)(blockIdx.x / (ntx * nty));
+    long long jj[T];
+    bool ok[T];
+    float p[T][3], pc0[T], r2[T];
+    int lo[T][3], hi[T][3];
+    PState st[T];
+    int ulo[3] = {INT_MAX, INT_MAX, DIM == 3 ? INT_MAX : 0}, uhi[3] = {INT_MIN, INT_MIN, DIM == 3 ? INT_MIN : 0};
+#pragma unroll
+    for (int c = 0; c < T; ++c) {
+        const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
+        ok[c] = jc[0] < pr.cp.n[0] && jc[1] < pr.cp.n[1] && (DIM == 2 || jc[2] < pr.cp.n[2]);
+        jj[c] = ok[c] ? jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]) : 0;
+        const float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]};
+        to_global<DIM>(pr.cp, fj, p[c]);
+        if (DIM == 2) p[c][2] = 0.0f;
+        float pcl[3];
+        to_local<DIM>(pr.fixed, p[c], pcl);
+        pc0[c] = pcl[0];
+        const float lam = cp_state_load<DIM, MODE>(pr, jj[c], st[c]);
+        r2[c] = pair_reach2(pr.kc.smin, lam, pr.lam_w_max);
+        const int *win = pr.cp_window + jj[c] * 6;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[c][d] = (d < DIM) ? win[d] : 0;
+            hi[c][d] = (d < DIM) ? win[d + 3] : 0;
+        }
+        if (ok[c]) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                ulo[d] = min(ulo[d], lo[c][d]);
+                uhi[d] = max(uhi[d], hi[c][d]);
+            }
+        }
+    }
+    const int bx = uhi[0] - ulo[0] + 1, by = uhi[1] - ulo[1] + 1, bz = uhi[2] - ulo[2] + 1;
+    const int rows = by * bz;
+    const int tx = threadIdx.x & 7, tr = threadIdx.x >> 3;
+    const int sub = pr.sub;
+    const float span = sub > 1 ? (float)sub : 0.0f;
+    const float inv_sub = 1.0f / (float)sub;
+    float acc[T][3];
+#pragma unroll
+    for (int c = 0; c < T; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0f;
+    int my = ulo[1] + tr % max(by, 1), mz = ulo[2] + tr / max(by, 1);
+    for (int r = tr; r < rows && bx > 0; r += kThreads / 8, my += kThreads / 8) {
+        while (my > uhi[1]) {
+            my -= by;
+            ++mz;
+        }
+        const float y0 = pr.fixed.o[1] + pr.fixed.h[1] * (float)(my * sub);
+        const float y1 = y0 + pr.fixed.h[1] * span;
+        const float z0 = DIM == 3 ? pr.fixed.o[2] + pr.fixed.h[2] * (float)(mz * sub) : 0.0f;
+        const float z1 = z0 + (DIM == 3 ? pr.fixed.h[2] * span : 0.0f);
+        // per tile control point: x cells of this row inside its window and reach
+        int xa[T], xb[T];
+        int rxa = INT_MAX, rxb = INT_MIN;
+#pragma unroll
+        for (int c = 0; c < T; ++c) {
+            xa[c] = INT_MAX;
+            xb[c] = INT_MIN;
+            const bool rin = ok[c] && my >= lo[c][1] && my <= hi[c][1] && (DIM == 2 || (mz >= lo[c][2] && mz <= hi[c][2]));
+            const float dy = fmaxf(0.0f, fmaxf(y0 - p[c][1], p[c][1] - y1));
+            const float dz = DIM == 3 ? fmaxf(0.0f, fmaxf(z0 - p[c][2], p[c][2] - z1)) : 0.0f;
+            const float rem = r2[c] - dy * dy - dz * dz;
+            if (rin && rem > 0.0f) {
+                const float rx = fast_sqrt(rem) * pr.fixed.ih[0];
+                xa[c] = max(lo[c][0], (int)((pc0[c] - rx - span) * inv_sub) - 1);
+                xb[c] = min(hi[c][0], (int)((pc0[c] + rx) * inv_sub) + 1);
+                rxa = min(rxa, xa[c]);
+                rxb = max(rxb, xb[c]);
+            }
+        }
+        // no tile control point reaches this row (rxa is still INT_MAX: rxa + tx
+        // would overflow)
+        if (rxa > rxb) continue;
+        const long long rowbase = (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
+        for (int mx = rxa + tx; mx <= rxb; mx += 8) {
+            const long long si = rowbase + mx;
+            const float4 v4 = sv[si];
+            PState sst;
+            pstate_unpack<DIM, MODE>(pr.smp_pstate[2 * si],
+                                     MODE == 2 ? pr.smp_pstate[2 * si + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f), sst);
+            int pos[3] = {mx, my, mz};
+            if (sub > 1) {
+                const unsigned pk = __float_as_uint(v4.w);
+                pos[0] = mx * sub + (int)(pk & 0xff);
+                pos[1] = my * sub + (int)((pk >> 8) & 0xff);
+                pos[2] = mz * sub + (int)((pk >> 16) & 0xff);
+            }
+            const float fpos[3] = {(float)pos[0], (float)pos[1], (float)pos[2]};
+            float xs[3];
+            to_global<DIM>(pr.fixed, fpos, xs);
+#pragma unroll
+            for (int c = 0; c < T; ++c) {
+                if (mx >= xa[c] && mx <= xb[c]) {
+                    const float v[3] = {p[c][0] - xs[0], p[c][1] - xs[1], DIM == 3 ? p[c][2] - xs[2] : 0.0f};
+                    const float k = kernel_pair<DIM, MODE>(pr.kc, v, st[c], sst);
+                    acc[c][0] = fmaf(k, v4.x, acc[c][0]);
+                    acc[c][1] = fmaf(k, v4.y, acc[c][1]);
+                    if (DIM == 3) acc[c][2] = fmaf(k, v4.z, acc[c][2]);
+                }
+            }
+        }
+    }
+    double dacc[T * 3];
+#pragma unroll
+    for (int c = 0; c < T; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dacc[c * 3 + d] = acc[c][d];
+    block_sum<T * 3>(dacc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < T; ++c) {
+            if (!ok[c]) continue;
+            const double w = pr.cpc[jj[c]].w;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) deriv[jj[c] + d * pr.n_cp] = (float)(w * dacc[c * 3 + d]);
+        }
     }
 }
 
@@ -682,8 +897,16 @@ void launch_prepare(const Problem &pr, int measure, const float4 *sg, const floa
 template <int DIM, int MODE>
 static void gather_impl(const Problem &pr, const float4 *sv, float *deriv, hipStream_t s)
 {
-    if (pr.sub == 1) k_gather<DIM, MODE, true><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
-    else k_gather<DIM, MODE, false><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
+    if constexpr (!is_stationary(MODE)) {
+        const long long tiles = (long long)((pr.cp.n[0] + CpTile<DIM>::X - 1) / CpTile<DIM>::X) *
+                                ((pr.cp.n[1] + CpTile<DIM>::Y - 1) / CpTile<DIM>::Y) *
+                                (DIM == 3 ? (pr.cp.n[2] + CpTile<DIM>::Z - 1) / CpTile<DIM>::Z : 1);
+        k_gather_tile<DIM, MODE><<<(unsigned)tiles, kThreads, 0, s>>>(pr, sv, deriv);
+    } else if (pr.sub == 1) {
+        k_gather<DIM, MODE, true><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
+    } else {
+        k_gather<DIM, MODE, false><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
+    }
 }
 
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s)
