@@ -179,6 +179,7 @@ struct skm_evaluator {
     float4 *d_cpc = nullptr;
     float *d_cp_state = nullptr;
     float4 *d_cp_pstate = nullptr, *d_smp_pstate = nullptr, *d_vox_pstate = nullptr;
+    float *d_cp_row_lam = nullptr, *d_smp_row_lam = nullptr, *d_vox_row_lam = nullptr;
     int *d_cp_window = nullptr, *d_reg_window = nullptr, *d_lcc_window = nullptr, *d_dense_window = nullptr;
     int *d_offsets = nullptr;
     float4 *d_sg = nullptr, *d_sv = nullptr;
@@ -208,7 +209,7 @@ struct skm_evaluator {
     ~skm_evaluator()
     {
         void *ptrs[] = {d_fixed, d_coeffs, d_weight, d_df, d_cpc, d_cp_state, d_cp_pstate, d_smp_pstate,
-                        d_vox_pstate, d_cp_window, d_reg_window,
+                        d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_window, d_reg_window,
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_dot_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_opt, d_records};
@@ -284,6 +285,48 @@ skm_status validate_grid(const skm_grid &g, int dim, const char *name, int min_s
 // getCovarianceAtPoint and getRegionSupport, Kernel.h:148-171) and the three
 // index windows the derivative passes use, all in FP64 as the reference
 // computes them with ScalarType = double.
+// Largest lam (sigma, or the largest covariance eigenvalue) over the points
+// every row of sample cells (my, mz) can take: voxel rows my*sub .. my*sub+sub
+// in y and z (the jitter offset reaches sub), every x; positions off the
+// weight grid take lam_out.  Uploaded as nsub[1] * nsub[2] floats.
+skm_status sample_row_lams(skm_evaluator *ev, const std::vector<double> &wl, double lam_out, int sub, float **dst)
+{
+    const skm_desc &ds = ev->desc;
+    const int dim = ev->dim;
+    int n[3] = {1, 1, 1}, ns[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) {
+        n[d] = ds.fixed.size[d];
+        ns[d] = n[d] / sub;
+    }
+    const int jit = sub > 1 ? sub : 0;
+    // nearest weight voxel of every fixed voxel along each axis (-1 outside)
+    std::vector<int> wi[3];
+    for (int d = 0; d < 3; ++d) {
+        wi[d].assign(n[d] + jit, 0);
+        for (int i = 0; i < n[d] + jit; ++i) {
+            if (d >= dim) { wi[d][i] = 0; continue; }
+            const double p = ds.fixed.origin[d] + ds.fixed.spacing[d] * i;
+            const int k = (int)((1.0 / ds.weight.spacing[d]) * (p - ds.weight.origin[d]) + 0.5);
+            wi[d][i] = (k >= 0 && k < ds.weight.size[d]) ? k : -1;
+        }
+    }
+    const long long wx = ds.weight.size[0], wy = ds.weight.size[1];
+    std::vector<float> out((size_t)ns[1] * ns[2]);
+    for (int mz = 0; mz < ns[2]; ++mz)
+        for (int my = 0; my < ns[1]; ++my) {
+            double m = 0.0;
+            const int z0 = dim == 3 ? mz * sub : 0, z1 = dim == 3 ? mz * sub + jit : 0;
+            for (int z = z0; z <= z1; ++z)
+                for (int y = my * sub; y <= my * sub + jit; ++y)
+                    for (int x = 0; x < ns[0] * sub + jit; ++x) {
+                        const int a = wi[0][x], b = wi[1][y], c = wi[2][z];
+                        m = std::max(m, (a < 0 || b < 0 || c < 0) ? lam_out : wl[a + wx * (b + wy * (long long)c)]);
+                    }
+            out[(size_t)my + (size_t)ns[1] * mz] = (float)(m * (1.0 + 1e-6));
+        }
+    return dev_upload(dst, out.data(), out.size());
+}
+
 skm_status build_static(skm_evaluator *ev)
 {
     const skm_desc &ds = ev->desc;
@@ -377,20 +420,39 @@ skm_status build_static(skm_evaluator *ev)
         // largest lam a sample can look up: every weight voxel, and the
         // outside value (sigma 0 / identity covariance)
         const long long nw = grid_count(ds.weight, dim);
-        double lam_w = ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 0.0 : 1.0;
+        const double lam_out = ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 0.0 : 1.0;
+        double lam_w = lam_out;
+        std::vector<double> wl((size_t)nw);
         for (long long i = 0; i < nw; ++i) {
             if (ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE) {
-                lam_w = std::max(lam_w, (double)ds.weight_data[i]);
+                wl[i] = ds.weight_data[i];
             } else {
                 double cov[9] = {0}, sym[9];
                 for (int k = 0; k < nn; ++k) cov[k] = ds.weight_data[i * nn + k];
                 symmetric_part(dim, cov, sym);
-                lam_w = std::max(lam_w, max_eig_h(dim, sym));
+                wl[i] = max_eig_h(dim, sym);
             }
+            lam_w = std::max(lam_w, wl[i]);
         }
         // rounded up so that the FP32 values stay upper bounds
         ev->pr.lam_cp_max = (float)(lam_cp_max * (1.0 + 1e-6));
         ev->pr.lam_w_max = (float)(lam_w * (1.0 + 1e-6));
+        // lam bound per row of control points (cy, cz): the value pass's reach
+        std::vector<float> cp_row((size_t)ds.cp.size[1] * (dim == 3 ? ds.cp.size[2] : 1), 0.0f);
+        for (long long j = 0; j < ncp; ++j) {
+            const long long row = j / ds.cp.size[0];
+            cp_row[row] = std::max(cp_row[row], (float)(pstate[(size_t)j * 8 + 7] * (1.0 + 1e-6)));
+        }
+        SKM_TRY(dev_upload(&ev->d_cp_row_lam, cp_row.data(), cp_row.size()));
+        // lam bound per row of sample cells (my, mz), jitter included, for the
+        // gather's reach; once for the evaluator's subsample and once for the
+        // dense voxel grid
+        SKM_TRY(sample_row_lams(ev, wl, lam_out, ds.subsample, &ev->d_smp_row_lam));
+        if (ds.subsample == 1) {
+            ev->d_vox_row_lam = nullptr;  // the dense table is the same
+        } else {
+            SKM_TRY(sample_row_lams(ev, wl, lam_out, 1, &ev->d_vox_row_lam));
+        }
     }
     SKM_TRY(dev_upload(&ev->d_cp_window, win.data(), win.size()));
     SKM_TRY(dev_upload(&ev->d_lcc_window, lwin.data(), lwin.size()));
@@ -444,6 +506,7 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
                         launch_voxel_states(pr, ev->d_vox_pstate, s);
                     }
                     dp.smp_pstate = ev->d_vox_pstate;
+                    if (ev->d_vox_row_lam) dp.smp_row_lam = ev->d_vox_row_lam;
                 }
                 dp.sub = 1;
                 for (int d = 0; d < 3; ++d) dp.nsub[d] = pr.fixed.n[d];
@@ -667,6 +730,8 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
         if ((st = dev_alloc(&ev->d_smp_pstate, (size_t)pr.n_sub * 2)) != SKM_SUCCESS) return cleanup(st);
     }
     pr.smp_pstate = ev->d_smp_pstate;
+    pr.cp_row_lam = ev->d_cp_row_lam;
+    pr.smp_row_lam = ev->d_smp_row_lam;
     pr.cp_window = ev->d_cp_window;
     pr.reg_window = ev->d_reg_window;
     pr.offsets = ev->d_offsets;

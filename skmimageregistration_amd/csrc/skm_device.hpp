@@ -119,6 +119,19 @@ __device__ __forceinline__ float wendland_c0(float t)
     return f * f;
 }
 
+// C4(r / sigma_C4) * C0(r / sigma_C0) of the weighted modes; with equal
+// supports the two share the (1 - t) factor: (1 - t)^8 (1 + 6t + 35/3 t^2).
+__device__ __forceinline__ float wendland_c4c0(float rs, const KernelConst &kc)
+{
+    const float t = rs * kc.inv_s4;
+    const float f = fmaxf(0.0f, 1.0f - t);
+    const float f2 = f * f, f4 = f2 * f2;
+    const float poly = fmaf(fmaf(35.0f / 3.0f, t, 6.0f), t, 1.0f);
+    if (kc.inv_s4 == kc.inv_s0) return f4 * f4 * poly;
+    const float g = fmaxf(0.0f, fmaf(-kc.inv_s0, rs, 1.0f));
+    return f4 * f2 * poly * (g * g);
+}
+
 template <int DIM>
 __device__ __forceinline__ float det_m(const float *m)
 {
@@ -269,7 +282,7 @@ __device__ __forceinline__ float kernel_pair(const KernelConst &kc, const float 
         float inv = __builtin_amdgcn_rcpf(sum);
         float F2 = 2.0f * a.sdet * b.sdet * fabsf(inv);
         float rs = fast_sqrt(2.0f * inv * r2);
-        return F2 * wendland_c4(rs * kc.inv_s4) * wendland_c0(rs * kc.inv_s0);
+        return F2 * wendland_c4c0(rs, kc);
     } else {
         float quad, det;
         if (DIM == 2) {
@@ -291,7 +304,7 @@ __device__ __forceinline__ float kernel_pair(const KernelConst &kc, const float 
         const float inv = __builtin_amdgcn_rcpf(det);
         const float F2 = (DIM == 3 ? 8.0f : 4.0f) * a.sdet * b.sdet * inv;
         const float rs = fast_sqrt(fmaxf(0.0f, 2.0f * quad * inv));
-        return F2 * wendland_c4(rs * kc.inv_s4) * wendland_c0(rs * kc.inv_s0);
+        return F2 * wendland_c4c0(rs, kc);
     }
 }
 

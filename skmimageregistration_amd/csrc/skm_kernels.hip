@@ -166,14 +166,21 @@ __device__ __forceinline__ int wave_max_i(int v)
     return v;
 }
 
+__device__ __forceinline__ float wave_max_f(float v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 // Kernel sum of the weighted modes with a wave-uniform control-point loop:
 // the wave walks the union of its lanes' reference windows row by row, so
 // every control point's state and coefficients are scalar loads shared by
 // the 64 lanes; a lane adds a pair only inside its own window and reach.
 // Every lane of the wave must call this (inactive lanes pass active = false).
 template <int DIM, int MODE>
-__device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *x, const PState &st, bool active,
-                                                float *u)
+__device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *x, const PState &st, float lam_x,
+                                                bool active, float *u)
 {
     float ci[3];
     to_local<DIM>(pr.cp, x, ci);
@@ -190,12 +197,13 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
         ulo[d] = __builtin_amdgcn_readfirstlane(wave_min_i(lo[d]));
         uhi[d] = __builtin_amdgcn_readfirstlane(wave_max_i(hi[d]));
     }
-    const float s2 = pair_reach2(pr.kc.smin, pstate_lam<DIM, MODE>(st), pr.lam_cp_max);
     for (int cz = ulo[2]; cz <= uhi[2]; ++cz) {
         const float dz = DIM == 3 ? x[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
         const bool zin = DIM == 2 || (cz >= lo[2] && cz <= hi[2]);
         for (int cy = ulo[1]; cy <= uhi[1]; ++cy) {
             const float dy = x[1] - (pr.cp.o[1] + pr.cp.h[1] * cy);
+            // reach of x over this row's control points
+            const float s2 = pair_reach2(pr.kc.smin, lam_x, pr.cp_row_lam[cy + (long long)pr.cp.n[1] * cz]);
             const float rem = s2 - dy * dy - dz * dz;
             int xa = INT_MAX, xb = INT_MIN;
             if (zin && cy >= lo[1] && cy <= hi[1] && rem > 0.0f) {
@@ -208,10 +216,12 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
             const long long row = (long long)pr.cp.n[0] * (cy + (long long)pr.cp.n[1] * cz);
             for (int cx = uxa; cx <= uxb; ++cx) {
                 PState cst;
-                cp_state_load<DIM, MODE>(pr, row + cx, cst);
+                const float lam_c = cp_state_load<DIM, MODE>(pr, row + cx, cst);
                 const float4 c = pr.cpc[row + cx];
-                if (cx >= xa && cx <= xb) {
-                    const float v[3] = {x[0] - (pr.cp.o[0] + pr.cp.h[0] * cx), dy, dz};
+                const float dx = x[0] - (pr.cp.o[0] + pr.cp.h[0] * cx);
+                // this pair's own reach (both lams) is tighter than the row's
+                if (cx >= xa && cx <= xb && dx * dx + dy * dy + dz * dz < pair_reach2(pr.kc.smin, lam_x, lam_c)) {
+                    const float v[3] = {dx, dy, dz};
                     const float wk = c.w * kernel_pair<DIM, MODE>(pr.kc, v, st, cst);
                     u[0] = fmaf(wk, c.x, u[0]);
                     u[1] = fmaf(wk, c.y, u[1]);
@@ -253,15 +263,20 @@ __global__ __launch_bounds__(kThreads) void k_value(Problem pr, float4 *__restri
     point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
     float u[3];
     df_lookup<DIM>(pr, x, u);
-    if constexpr (is_stationary(MODE)) kernel_sum<DIM, MODE>(pr, x, st, u);
-    else kernel_sum_wave<DIM, MODE>(pr, x, st, valid, u);
+    float lam_x = 0.0f;
+    if constexpr (is_stationary(MODE)) {
+        kernel_sum<DIM, MODE>(pr, x, st, u);
+    } else {
+        lam_x = pstate_lam<DIM, MODE>(st);
+        kernel_sum_wave<DIM, MODE>(pr, x, st, lam_x, valid, u);
+    }
     if (valid) {
         if (!is_stationary(MODE)) {
             // the derivative gather reads the sample's state from here
             float4 a, b;
-            pstate_pack<DIM, MODE>(st, 0.0f, a, b);
+            pstate_pack<DIM, MODE>(st, lam_x, a, b);
             pr.smp_pstate[2 * s] = a;
-            if (MODE == 2) pr.smp_pstate[2 * s + 1] = b;
+            pr.smp_pstate[2 * s + 1] = b;  // .w = lam, read by the gather's brick test
         }
         float y[3];
 #pragma unroll
@@ -479,17 +494,30 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
     }
 }
 
-// Control-point tile of the weighted-mode gather.
+// Control-point tile of the weighted-mode gather: small enough that the
+// tile's constants stay in scalar registers.
 template <int DIM> struct CpTile;
-template <> struct CpTile<2> { static constexpr int X = 4, Y = 2, Z = 1; };
-template <> struct CpTile<3> { static constexpr int X = 2, Y = 2, Z = 2; };
+template <> struct CpTile<2> { static constexpr int X = 2, Y = 2, Z = 1; };
+template <> struct CpTile<3> { static constexpr int X = 2, Y = 2, Z = 1; };
+
+// Workgroup -> tile order that gives each XCD a contiguous range of tiles:
+// workgroups are dispatched to the 8 XCDs round-robin, so neighbouring tiles
+// (whose sample windows overlap) share one XCD's L2.
+__device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned n)
+{
+    constexpr unsigned kXcds = 8;
+    const unsigned xcd = b % kXcds, slot = b / kXcds, q = n / kXcds, rm = n % kXcds;
+    return xcd * q + min(xcd, rm) + slot;
+}
 
 // Derivative gather of the weighted modes: one workgroup per tile of
 // control points.  The workgroup walks the union of the tile's reference
-// windows once; each lane loads a sample (v, state) once and adds its pair
-// with every tile control point whose window and reach hold the sample.
-// Control-point data are wave-uniform (scalar registers).  Same sums per
-// control point as k_gather, in a fixed order.
+// windows once, 8 lanes along x and 32 rows at a time; each lane loads a
+// sample (v, packed state) once and adds its pair with every tile control
+// point whose window and reach (both points' lam) hold the sample.  The tile
+// constants are wave-uniform (scalar registers); the next sample's loads are
+// issued before the current sample's pairs.  Same sums per control point as
+// k_gather, in a fixed order.
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const float4 *__restrict__ sv,
                                                           float *__restrict__ deriv)
@@ -497,31 +525,34 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
     constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z, T = TX * TY * TZ;
     __shared__ double lds[T * 3 * 4];
     const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
-    const int t0 = (int)(blockIdx.x % ntx), t1 = (int)((blockIdx.x / ntx) % nty), t2 = (int)(blockIdx.x / (ntx * nty));
+    const unsigned tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int t0 = (int)(tile % ntx), t1 = (int)((tile / ntx) % nty), t2 = (int)(tile / (ntx * nty));
+    // tile control points (uniform)
     long long jj[T];
     bool ok[T];
-    float p[T][3], pc0[T], r2[T];
+    float px[T], py[T], pz[T], pc0[T], lam[T];
     int lo[T][3], hi[T][3];
     PState st[T];
-    int ulo[3] = {INT_MAX, INT_MAX, DIM == 3 ? INT_MAX : 0}, uhi[3] = {INT_MIN, INT_MIN, DIM == 3 ? INT_MIN : 0};
+    int ulo[3] = {INT_MAX, INT_MAX, 0}, uhi[3] = {INT_MIN, INT_MIN, 0};
 #pragma unroll
     for (int c = 0; c < T; ++c) {
         const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
         ok[c] = jc[0] < pr.cp.n[0] && jc[1] < pr.cp.n[1] && (DIM == 2 || jc[2] < pr.cp.n[2]);
         jj[c] = ok[c] ? jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]) : 0;
         const float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]};
-        to_global<DIM>(pr.cp, fj, p[c]);
-        if (DIM == 2) p[c][2] = 0.0f;
-        float pcl[3];
-        to_local<DIM>(pr.fixed, p[c], pcl);
+        float pos[3] = {0.0f, 0.0f, 0.0f}, pcl[3] = {0.0f, 0.0f, 0.0f};
+        to_global<DIM>(pr.cp, fj, pos);
+        to_local<DIM>(pr.fixed, pos, pcl);
+        px[c] = pos[0];
+        py[c] = pos[1];
+        pz[c] = pos[2];
         pc0[c] = pcl[0];
-        const float lam = cp_state_load<DIM, MODE>(pr, jj[c], st[c]);
-        r2[c] = pair_reach2(pr.kc.smin, lam, pr.lam_w_max);
+        lam[c] = cp_state_load<DIM, MODE>(pr, jj[c], st[c]);
         const int *win = pr.cp_window + jj[c] * 6;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            lo[c][d] = (d < DIM) ? win[d] : 0;
-            hi[c][d] = (d < DIM) ? win[d + 3] : 0;
+            lo[c][d] = d < DIM ? win[d] : 0;
+            hi[c][d] = d < DIM ? win[d + 3] : 0;
         }
         if (ok[c]) {
 #pragma unroll
@@ -531,17 +562,19 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
             }
         }
     }
-    const int bx = uhi[0] - ulo[0] + 1, by = uhi[1] - ulo[1] + 1, bz = uhi[2] - ulo[2] + 1;
-    const int rows = by * bz;
+    const int by = uhi[1] - ulo[1] + 1, bz = uhi[2] - ulo[2] + 1;
+    // (a window can be empty along an axis: then there is nothing to gather)
+    const int rows = (uhi[0] >= ulo[0] && by > 0 && bz > 0) ? by * bz : 0;
     const int tx = threadIdx.x & 7, tr = threadIdx.x >> 3;
     const int sub = pr.sub;
     const float span = sub > 1 ? (float)sub : 0.0f;
     const float inv_sub = 1.0f / (float)sub;
+    const float smin2h = 0.5f * pr.kc.smin * pr.kc.smin * 1.0001f;
     float acc[T][3];
 #pragma unroll
     for (int c = 0; c < T; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0f;
     int my = ulo[1] + tr % max(by, 1), mz = ulo[2] + tr / max(by, 1);
-    for (int r = tr; r < rows && bx > 0; r += kThreads / 8, my += kThreads / 8) {
+    for (int r = tr; r < rows; r += kThreads / 8, my += kThreads / 8) {
         while (my > uhi[1]) {
             my -= by;
             ++mz;
@@ -550,6 +583,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
         const float y1 = y0 + pr.fixed.h[1] * span;
         const float z0 = DIM == 3 ? pr.fixed.o[2] + pr.fixed.h[2] * (float)(mz * sub) : 0.0f;
         const float z1 = z0 + (DIM == 3 ? pr.fixed.h[2] * span : 0.0f);
+        const float lam_row = pr.smp_row_lam[my + (long long)pr.nsub[1] * mz];
         // per tile control point: x cells of this row inside its window and reach
         int xa[T], xb[T];
         int rxa = INT_MAX, rxb = INT_MIN;
@@ -557,10 +591,12 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
         for (int c = 0; c < T; ++c) {
             xa[c] = INT_MAX;
             xb[c] = INT_MIN;
-            const bool rin = ok[c] && my >= lo[c][1] && my <= hi[c][1] && (DIM == 2 || (mz >= lo[c][2] && mz <= hi[c][2]));
-            const float dy = fmaxf(0.0f, fmaxf(y0 - p[c][1], p[c][1] - y1));
-            const float dz = DIM == 3 ? fmaxf(0.0f, fmaxf(z0 - p[c][2], p[c][2] - z1)) : 0.0f;
-            const float rem = r2[c] - dy * dy - dz * dz;
+            const bool rin = ok[c] && my >= lo[c][1] && my <= hi[c][1] &&
+                             (DIM == 2 || (mz >= lo[c][2] && mz <= hi[c][2]));
+            const float dy = fmaxf(0.0f, fmaxf(y0 - py[c], py[c] - y1));
+            const float dz = DIM == 3 ? fmaxf(0.0f, fmaxf(z0 - pz[c], pz[c] - z1)) : 0.0f;
+            // reach of the control point over this row's samples
+            const float rem = smin2h * (lam[c] + lam_row) - dy * dy - dz * dz;
             if (rin && rem > 0.0f) {
                 const float rx = fast_sqrt(rem) * pr.fixed.ih[0];
                 xa[c] = max(lo[c][0], (int)((pc0[c] - rx - span) * inv_sub) - 1);
@@ -573,15 +609,25 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
         // would overflow)
         if (rxa > rxb) continue;
         const long long rowbase = (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
-        for (int mx = rxa + tx; mx <= rxb; mx += 8) {
-            const long long si = rowbase + mx;
-            const float4 v4 = sv[si];
+        int mx = rxa + tx;
+        float4 cv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ca = cv, cb = cv;
+        if (mx <= rxb) {
+            cv = sv[rowbase + mx];
+            ca = pr.smp_pstate[2 * (rowbase + mx)];
+            cb = pr.smp_pstate[2 * (rowbase + mx) + 1];
+        }
+        for (; mx <= rxb; mx += 8) {
+            float4 nv = cv, na = ca, nb = cb;
+            if (mx + 8 <= rxb) {
+                nv = sv[rowbase + mx + 8];
+                na = pr.smp_pstate[2 * (rowbase + mx + 8)];
+                nb = pr.smp_pstate[2 * (rowbase + mx + 8) + 1];
+            }
             PState sst;
-            pstate_unpack<DIM, MODE>(pr.smp_pstate[2 * si],
-                                     MODE == 2 ? pr.smp_pstate[2 * si + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f), sst);
+            pstate_unpack<DIM, MODE>(ca, cb, sst);
             int pos[3] = {mx, my, mz};
             if (sub > 1) {
-                const unsigned pk = __float_as_uint(v4.w);
+                const unsigned pk = __float_as_uint(cv.w);
                 pos[0] = mx * sub + (int)(pk & 0xff);
                 pos[1] = my * sub + (int)((pk >> 8) & 0xff);
                 pos[2] = mz * sub + (int)((pk >> 16) & 0xff);
@@ -591,29 +637,44 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
             to_global<DIM>(pr.fixed, fpos, xs);
 #pragma unroll
             for (int c = 0; c < T; ++c) {
-                if (mx >= xa[c] && mx <= xb[c]) {
-                    const float v[3] = {p[c][0] - xs[0], p[c][1] - xs[1], DIM == 3 ? p[c][2] - xs[2] : 0.0f};
+                const float v[3] = {px[c] - xs[0], py[c] - xs[1], DIM == 3 ? pz[c] - xs[2] : 0.0f};
+                const float d2 = v[0] * v[0] + v[1] * v[1] + (DIM == 3 ? v[2] * v[2] : 0.0f);
+                // the pair's own reach: both points' lam
+                if (mx >= xa[c] && mx <= xb[c] && d2 < smin2h * (lam[c] + cb.w)) {
                     const float k = kernel_pair<DIM, MODE>(pr.kc, v, st[c], sst);
-                    acc[c][0] = fmaf(k, v4.x, acc[c][0]);
-                    acc[c][1] = fmaf(k, v4.y, acc[c][1]);
-                    if (DIM == 3) acc[c][2] = fmaf(k, v4.z, acc[c][2]);
+                    acc[c][0] = fmaf(k, cv.x, acc[c][0]);
+                    acc[c][1] = fmaf(k, cv.y, acc[c][1]);
+                    if (DIM == 3) acc[c][2] = fmaf(k, cv.z, acc[c][2]);
                 }
             }
+            cv = nv;
+            ca = na;
+            cb = nb;
         }
     }
-    double dacc[T * 3];
+    // wave sums (fixed butterfly, FP32), then the four waves in FP64
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < T; ++c)
 #pragma unroll
-        for (int d = 0; d < 3; ++d) dacc[c * 3 + d] = acc[c][d];
-    block_sum<T * 3>(dacc, lds);
-    if (threadIdx.x == 0) {
+        for (int d = 0; d < 3; ++d) {
+            float v = acc[c][d];
 #pragma unroll
-        for (int c = 0; c < T; ++c) {
-            if (!ok[c]) continue;
-            const double w = pr.cpc[jj[c]].w;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) deriv[jj[c] + d * pr.n_cp] = (float)(w * dacc[c * 3 + d]);
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) lds[wave * T * 3 + c * 3 + d] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < T * 3) {
+        const int c = threadIdx.x / 3, d = threadIdx.x % 3;
+        double sum = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) sum += lds[w * T * 3 + threadIdx.x];
+        if (d < DIM) {
+            const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
+            const bool inside = jc[0] < pr.cp.n[0] && jc[1] < pr.cp.n[1] && (DIM == 2 || jc[2] < pr.cp.n[2]);
+            if (inside) {
+                const long long j = jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]);
+                deriv[j + d * pr.n_cp] = (float)((double)pr.cpc[j].w * sum);
+            }
         }
     }
 }
@@ -774,7 +835,7 @@ __global__ __launch_bounds__(kThreads) void k_voxel_states(Problem pr, float4 *_
     PState st;
     point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
     float4 a, b;
-    pstate_pack<DIM, MODE>(st, 0.0f, a, b);
+    pstate_pack<DIM, MODE>(st, pstate_lam<DIM, MODE>(st), a, b);
     out[2 * v] = a;
     out[2 * v + 1] = b;
 }
