@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pair-count", action="store_true", help="skip the exact k != 0 pair count")
     ap.add_argument("--cpu-crop", default=None, help="z,y,x voxels of the CPU baseline sample")
+    ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
+                    help="run one plain device copy of MIB MiB first (reference dispatch for HBM counters)")
     return ap.parse_args()
 
 
@@ -249,6 +251,29 @@ def pair_counts(args, shape, ten, mcov, dev, exact_nonzero):
     return vbox, dbox, vnz, dnz
 
 
+def workload_key(args, shape):
+    z, y, x = shape
+    return f"{args.kernel}-{x}x{y}x{z}-h{args.cp_spacing:g}-s{args.sigma:g}-{args.measure}-sub{args.subsample}"
+
+
+def profiled_traffic(kernel, key):
+    """HBM bytes per launch of `kernel` on this workload, as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (the line
+    `traffic <kernel> <workload key> <bytes>`); None when not profiled."""
+    d = os.path.join(REPO, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for name in sorted(os.listdir(d)):
+        if not name.endswith(".txt"):
+            continue
+        with open(os.path.join(d, name)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic" and parts[1] == kernel and parts[2] == key:
+                    return float(parts[3])
+    return None
+
+
 def cpu_baseline(args, mov, fix, ten, mcov):
     """FP64 oracle (single thread) on a crop of the same volumes, same kernel
     and control-point spacing; reported in the same unit."""
@@ -310,6 +335,13 @@ def main():
     def step():
         ev.get_value_and_derivative_device(measure, cp.data_ptr(), cpw.data_ptr(), der.data_ptr(), True)
 
+    if args.calibration_copy > 0:
+        src = torch.full((args.calibration_copy * (1 << 18),), 1.0, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -341,9 +373,10 @@ def main():
     if vnz is not None:
         pairs = vnz if dom == "value" else dnz
         achieved = pairs * fpp / (per_kernel[dom] * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": "k_value" if dom == "value" else "k_gather",
+        kname = "k_value" if dom == "value" else ("k_gather" if args.kernel == "stationary" else "k_gather_tile")
+        roof = {"bound": "valu", "kernel": kname,
                 "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": profiled_traffic(kname, workload_key(args, shape)),
                 "launch_ms": round(per_kernel[dom], 4), "pairs_per_launch": pairs, "flops_per_pair": fpp}
     cpu = None
     if not args.no_cpu_baseline:

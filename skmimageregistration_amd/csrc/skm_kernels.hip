@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
 // tile's constants stay in scalar registers.
 template <int DIM> struct CpTile;
 template <> struct CpTile<2> { static constexpr int X = 2, Y = 2, Z = 1; };
-template <> struct CpTile<3> { static constexpr int X = 2, Y = 2, Z = 1; };
+template <> struct CpTile<3> { static constexpr int X = 4, Y = 1, Z = 1; };
 
 // Workgroup -> tile order that gives each XCD a contiguous range of tiles:
 // workgroups are dispatched to the 8 XCDs round-robin, so neighbouring tiles
