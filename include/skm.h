@@ -223,6 +223,9 @@ typedef struct skm_optimizer_options {
     double fmax, fmin, omega;        /* adaptive time function                          */
     int32_t orthant_projection;
     int32_t iterations;
+    int32_t return_best;             /* return the evaluated position with the lowest value
+                                        (ITK's ReturnBestParametersAndValue, set by
+                                        SKMReg.cxx:383) instead of the last step's     */
 } skm_optimizer_options;
 
 /* Metric value per iteration as reported by
@@ -256,6 +259,78 @@ SKM_API skm_status skm_host_regularizers(int32_t dim, int64_t n_cp, const double
                                          double reg_l1, double reg_l21, double reg_l2,
                                          int32_t use_negative_gradient, double *derivative_soa,
                                          double *reg_value, double *l1, double *l21, double *l2);
+
+/* ---- multi-scale registration -------------------------------------------- */
+
+/* The scale-level loop of SKMReg (src/SKMReg.cxx:123-500) over this library:
+ * per level Gaussian smoothing of both images, the weight tensor (SKMReg's
+ * cases 1-4, SKMReg.cxx:262-297), a control-point grid with spacing
+ * min(sigma_C4, sigma_C0) / cp_spacing_scaler, a new evaluator whose previous
+ * displacement is the sum of the coarser levels (SummationTransform with a
+ * nearest-neighbour displacement field), the adaptive-step optimizer of
+ * SKMReg.cxx:375-430 run on the device (a = initial_step_size, A = 20,
+ * alpha = 1, Fmax = 1e7, Fmin = -1e6, omega = 1, best parameters returned,
+ * orthant projection when L1 or L2,1 is the only regularizer), and the dense
+ * displacement of all levels so far.  Images are preprocessed as
+ * SKMReg.cxx:84-91 does: intensity thresholds, then rescaling to [0, 1]. */
+#define SKM_MAX_SCALES 8
+
+/* Per-level options, the per-scale entries of ConfigUtils.h:29-62. */
+typedef struct skm_scale_params {
+    double sigma_c4, sigma_c0;
+    double tensor_weight;            /* > 0: anisotropic tensor from the target
+                                        (or the guidance image); 0: none, or the
+                                        given weight tensor                        */
+    double reg_l1, reg_l21, reg_l2, reg_rkhs, reg_rd, reg_rd_scaling, reg_pg, reg_pg_scaling;
+    int32_t iterations;              /* num_function_evaluations                   */
+    double initial_step_size;
+    int32_t sampling_rate, neighborhood_sampling;
+    double smoothing_variance;       /* > 1e-5: GaussianSmoothing of both images   */
+} skm_scale_params;
+
+typedef struct skm_registration_desc {
+    int32_t dim;
+    skm_grid reference; const float *reference_data;   /* moving image R            */
+    skm_grid target;    const float *target_data;      /* fixed image T, the output
+                                                          grid of the displacement  */
+    int32_t num_scales;                                /* <= SKM_MAX_SCALES        */
+    skm_scale_params scale[SKM_MAX_SCALES];
+    double cp_spacing_scaler;                          /* < 1: spacing = min sigma  */
+    int32_t metric;                                    /* skm_measure               */
+    int32_t use_intensity_thresholds;                  /* intensity_thresholds set  */
+    double threshold_min, threshold_max;
+    skm_grid weight; const float *weight_image;        /* weight_filename, or NULL  */
+    const float *weight_tensor;                        /* weight_tensor_filename as a
+                                                          D*D tensor field on the
+                                                          weight grid (tensor_weight
+                                                          0), or NULL               */
+    const float *guidance_image;                       /* weight_tensor_filename as a
+                                                          guidance image on the weight
+                                                          grid (tensor_weight > 0)  */
+    int32_t restart_at;                                /* restart_at (1-based level) */
+    const float *restart_displacement;                 /* df-full of level restart_at-1,
+                                                          AoS on the target grid    */
+} skm_registration_desc;
+
+/* What one level did. */
+typedef struct skm_scale_report {
+    double value;                    /* lowest metric value (measure + regularizers) */
+    double measure;                  /* the measure at that iteration               */
+    double first_value;              /* value at the first iteration                */
+    int32_t iterations;
+    int32_t cp_size[3];
+    double maximum_tilt;             /* ComputeAnisotropicTensor's, or 0           */
+    double ms_setup, ms_optimize, ms_displacement;
+} skm_scale_report;
+
+/* ConfigUtils.h:29-62 defaults (3 levels, metric MSE, sigma_C4 64/32/16, ...). */
+SKM_API void skm_registration_desc_init(skm_registration_desc *desc);
+
+/* Runs the registration.  displacement_out (host, D floats per target voxel)
+ * receives the displacement of all levels; reports (host, num_scales entries,
+ * may be NULL) the per-level summaries (levels before restart_at untouched). */
+SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displacement_out,
+                                skm_scale_report *reports);
 
 #ifdef __cplusplus
 }

@@ -287,6 +287,10 @@ public:
     void SetNumberOfIterations(unsigned n) { iterations_ = n; }
     void CalculateMagnitudeOn() { magnitude_ = true; }
     void CountZeroParamsOn() { count_zero_ = true; }
+    // ITK's ReturnBestParametersAndValue: the result is the evaluated position
+    // with the lowest value (SKMReg.cxx:383 switches it on)
+    void ReturnBestParametersAndValueOn() { return_best_ = true; }
+    void ReturnBestParametersAndValueOff() { return_best_ = false; }
     void SetIterationCallback(std::function<void(unsigned, double)> cb) { callback_ = cb; }
     unsigned GetCurrentIteration() const { return iteration_; }
     double GetValue() const { return value_; }
@@ -299,12 +303,23 @@ public:
         position_ = metric_->GetParameters();
         previous_gradient_.assign(position_.size(), 0.0);
         start();
+        ParametersType best;
+        double best_value = 0.0;
         for (iteration_ = 0; iteration_ < iterations_; ++iteration_) {
             metric_->GetValueAndDerivative(value_, gradient_);
+            if (return_best_ && (iteration_ == 0 || value_ < best_value)) {
+                best_value = value_;
+                best = position_;
+            }
             EstimateLearningRate();
             ModifyGradientByLearningRate();
             metric_->SetParameters(position_);
             if (callback_) callback_(iteration_, value_);
+        }
+        if (return_best_ && iterations_ > 0) {
+            position_ = best;
+            value_ = best_value;
+            metric_->SetParameters(position_);
         }
     }
 
@@ -319,7 +334,7 @@ public:
         skm_metric_options mo{(int)metric_->measure(), metric_->GetRegularizerL1(), metric_->GetRegularizerL21(),
                               metric_->GetRegularizerL2(), metric_->GetUseNegativeGradient(),
                               metric_->GetDoResampling()};
-        skm_optimizer_options oo{kind, a_, A_, alpha_, fmax_, fmin_, omega_, orthant_, (int)iterations_};
+        skm_optimizer_options oo{kind, a_, A_, alpha_, fmax_, fmin_, omega_, orthant_, (int)iterations_, return_best_};
         std::vector<skm_iteration_record> rec(iterations_ ? iterations_ : 1);
         auto *ev = t->GetGpuKernelEvaluator();
         if (metric_->GetRegularizerRKHS() > 0) ev->SetRegularizerRKHS(metric_->GetRegularizerRKHS());
@@ -330,6 +345,8 @@ public:
         metric_->SetParameters(position_);
         iteration_ = iterations_;
         value_ = iterations_ ? rec[iterations_ - 1].value : 0.0;
+        if (return_best_)
+            for (unsigned k = 0; k < iterations_; ++k) value_ = k == 0 ? rec[0].value : std::min(value_, rec[k].value);
         if (callback_)
             for (unsigned k = 0; k < iterations_; ++k) callback_(k, rec[k].value);
     }
@@ -364,7 +381,7 @@ protected:
     std::shared_ptr<TMetric> metric_;
     double a_ = 1, A_ = 5, alpha_ = 1, rate_ = 0, value_ = 0;
     double fmax_ = 1, fmin_ = -0.5, omega_ = 1;
-    bool orthant_ = false, magnitude_ = false, count_zero_ = false;
+    bool orthant_ = false, magnitude_ = false, count_zero_ = false, return_best_ = false;
     unsigned iterations_ = 100, iteration_ = 0;
     ParametersType position_;
     DerivativeType gradient_, previous_gradient_;

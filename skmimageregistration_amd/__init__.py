@@ -6,10 +6,12 @@ interface over that ABI.
 """
 from . import _capi
 from .evaluator import (MEASURE, ComputeAnisotropicTensor, EvaluationResult, GaussianSmoothing,
-                        GpuKernelEvaluator, Grid, Image, Kernel, control_point_grid)
+                        GpuKernelEvaluator, Grid, Image, Kernel, RegistrationConfig, control_point_grid,
+                        register_multiscale)
 
 __all__ = ["MEASURE", "ComputeAnisotropicTensor", "EvaluationResult", "GaussianSmoothing", "GpuKernelEvaluator",
-           "Grid", "Image", "Kernel", "control_point_grid", "load_library"]
+           "Grid", "Image", "Kernel", "RegistrationConfig", "control_point_grid", "load_library",
+           "register_multiscale"]
 
 
 def load_library():

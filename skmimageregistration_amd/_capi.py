@@ -59,7 +59,7 @@ class SkmOptimizerOptions(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("a", ctypes.c_double), ("A", ctypes.c_double),
                 ("alpha", ctypes.c_double), ("fmax", ctypes.c_double), ("fmin", ctypes.c_double),
                 ("omega", ctypes.c_double), ("orthant_projection", ctypes.c_int32),
-                ("iterations", ctypes.c_int32)]
+                ("iterations", ctypes.c_int32), ("return_best", ctypes.c_int32)]
 
 
 SKM_NUM_PASSES = 10
@@ -79,6 +79,38 @@ class SkmIterationRecord(ctypes.Structure):
 class SkmTensorOptions(ctypes.Structure):
     _fields_ = [("sigma", ctypes.c_double), ("weight", ctypes.c_double), ("alignment_direction", ctypes.c_int32),
                 ("anisotropic_filtering", ctypes.c_int32)]
+
+
+SKM_MAX_SCALES = 8
+
+
+class SkmScaleParams(ctypes.Structure):
+    _fields_ = [("sigma_c4", ctypes.c_double), ("sigma_c0", ctypes.c_double), ("tensor_weight", ctypes.c_double),
+                ("reg_l1", ctypes.c_double), ("reg_l21", ctypes.c_double), ("reg_l2", ctypes.c_double),
+                ("reg_rkhs", ctypes.c_double), ("reg_rd", ctypes.c_double), ("reg_rd_scaling", ctypes.c_double),
+                ("reg_pg", ctypes.c_double), ("reg_pg_scaling", ctypes.c_double), ("iterations", ctypes.c_int32),
+                ("initial_step_size", ctypes.c_double), ("sampling_rate", ctypes.c_int32),
+                ("neighborhood_sampling", ctypes.c_int32), ("smoothing_variance", ctypes.c_double)]
+
+
+class SkmRegistrationDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32),
+                ("reference", SkmGrid), ("reference_data", c_float_p),
+                ("target", SkmGrid), ("target_data", c_float_p),
+                ("num_scales", ctypes.c_int32), ("scale", SkmScaleParams * SKM_MAX_SCALES),
+                ("cp_spacing_scaler", ctypes.c_double), ("metric", ctypes.c_int32),
+                ("use_intensity_thresholds", ctypes.c_int32), ("threshold_min", ctypes.c_double),
+                ("threshold_max", ctypes.c_double),
+                ("weight", SkmGrid), ("weight_image", c_float_p), ("weight_tensor", c_float_p),
+                ("guidance_image", c_float_p), ("restart_at", ctypes.c_int32),
+                ("restart_displacement", c_float_p)]
+
+
+class SkmScaleReport(ctypes.Structure):
+    _fields_ = [("value", ctypes.c_double), ("measure", ctypes.c_double), ("first_value", ctypes.c_double),
+                ("iterations", ctypes.c_int32), ("cp_size", ctypes.c_int32 * 3), ("maximum_tilt", ctypes.c_double),
+                ("ms_setup", ctypes.c_double), ("ms_optimize", ctypes.c_double),
+                ("ms_displacement", ctypes.c_double)]
 
 
 # Every symbol include/skm.h declares, with (restype, argtypes).
@@ -116,6 +148,8 @@ _SIGNATURES = {
     "skm_anisotropic_tensor": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p,
                                               ctypes.POINTER(SkmTensorOptions), c_float_p, c_double_p, c_double_p,
                                               c_double_p]),
+    "skm_registration_desc_init": (None, [ctypes.POINTER(SkmRegistrationDesc)]),
+    "skm_register": (ctypes.c_int, [ctypes.POINTER(SkmRegistrationDesc), c_float_p, ctypes.POINTER(SkmScaleReport)]),
     "skm_host_regularizers": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, c_double_p, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_int32, c_double_p,
                                              c_double_p, c_double_p, c_double_p, c_double_p]),

@@ -188,7 +188,7 @@ struct skm_evaluator {
     Scalars *d_sc = nullptr;
     float *d_deriv = nullptr, *d_cp_in = nullptr, *d_cpw_in = nullptr, *d_dst = nullptr;
     float4 *d_dense = nullptr, *d_dense_q = nullptr, *d_ccoef = nullptr, *d_gamma = nullptr;
-    float *d_params = nullptr, *d_prev_g = nullptr;
+    float *d_params = nullptr, *d_prev_g = nullptr, *d_best = nullptr;
     OptState *d_opt = nullptr;
     skm_iteration_record *d_records = nullptr;
     int records_cap = 0;
@@ -212,7 +212,7 @@ struct skm_evaluator {
                         d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_window, d_reg_window,
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_dot_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
-                        d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_opt, d_records};
+                        d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -1013,6 +1013,7 @@ SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *met
     if (!ev->d_params) SKM_TRY(dev_alloc(&ev->d_params, np));
     if (!ev->d_prev_g) SKM_TRY(dev_alloc(&ev->d_prev_g, np));
     if (!ev->d_opt) SKM_TRY(dev_alloc(&ev->d_opt, 1));
+    if (opt->return_best && !ev->d_best) SKM_TRY(dev_alloc(&ev->d_best, np));
     if (ev->records_cap < opt->iterations) {
         if (ev->d_records) SKM_HIP(hipFree(ev->d_records));
         ev->d_records = nullptr;
@@ -1031,10 +1032,12 @@ SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *met
         launch_opt_step(pr, opt->kind, opt->a, opt->A, opt->alpha, opt->fmax, opt->fmin, opt->omega,
                         opt->orthant_projection, ev->d_params, ev->d_deriv, ev->d_prev_g, ev->d_dot_partials,
                         ev->d_opt, ev->d_sc, ev->d_hreg_partials, metric->reg_l1, metric->reg_l21, metric->reg_l2,
-                        ev->reg_w, ev->d_records, s);
+                        ev->reg_w, ev->d_records, opt->return_best ? ev->d_best : nullptr, s);
     }
     SKM_HIP(hipGetLastError());
-    SKM_HIP(hipMemcpyAsync(params_soa, ev->d_params, sizeof(float) * np, hipMemcpyDeviceToHost, s));
+    // with return_best (and at least one evaluation) the lowest-value position
+    const float *result = (opt->return_best && opt->iterations > 0) ? ev->d_best : ev->d_params;
+    SKM_HIP(hipMemcpyAsync(params_soa, result, sizeof(float) * np, hipMemcpyDeviceToHost, s));
     if (records && opt->iterations > 0)
         SKM_HIP(hipMemcpyAsync(records, ev->d_records, sizeof(skm_iteration_record) * opt->iterations,
                                hipMemcpyDeviceToHost, s));

@@ -90,6 +90,8 @@ struct OptState {
     double rate;
     double prev_dot;
     long long iteration;
+    double best_value;     // lowest metric value evaluated so far
+    int improved;          // this iteration's value is the lowest so far
 };
 void launch_host_regs_device(const Problem &pr, const float *params_soa, double l1, double l21, double l2,
                              int negate, float *deriv_soa, double *partials, hipStream_t s);
@@ -97,7 +99,7 @@ void launch_opt_step(const Problem &pr, int kind, double a, double A, double alp
                      double omega, int orthant, float *params_soa, const float *grad_soa, float *prev_grad,
                      double *partials, OptState *st, const Scalars *sc, const double *reg_partials,
                      double l1w, double l21w, double l2w, const double *gpu_reg_weights, void *record,
-                     hipStream_t s);
+                     float *best_params, hipStream_t s);
 void launch_soa_to_cpc(const Problem &pr, const float *params_soa, const float *cpw, float4 *cpc,
                        hipStream_t s);
 

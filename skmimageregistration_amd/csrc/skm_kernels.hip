@@ -1127,14 +1127,27 @@ __global__ void k_step_scalars(StepParams sp, const double *__restrict__ dot_par
     if (sp.l2w > 0) reg += sp.l2w * l2;
     for (int k = 0; k < 3; ++k)
         if (sp.rw[k] > 0) reg += sp.rw[k] * sc->reg[k];
+    const double value = sc->measure + reg;
+    st->improved = st->iteration == 0 || value < st->best_value;
+    if (st->improved) st->best_value = value;
     if (rec) {
-        rec[st->iteration].value = sc->measure + reg;
+        rec[st->iteration].value = value;
         rec[st->iteration].measure = sc->measure;
         rec[st->iteration].learning_rate = rate;
         rec[st->iteration].num_pixels_counted = (uint32_t)sc->count;
         rec[st->iteration].reserved = 0;
     }
     st->iteration += 1;
+}
+
+// Copy of the position just evaluated when its value is the lowest so far
+// (ITK's ReturnBestParametersAndValue).
+__global__ __launch_bounds__(kThreads) void k_keep_best(long long n, const float *__restrict__ th,
+                                                        float *__restrict__ best, const OptState *st)
+{
+    long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i >= n || !st->improved) return;
+    best[i] = th[i];
 }
 
 // theta <- theta + rate * d with orthant projection; d is kept for the next
@@ -1155,7 +1168,8 @@ __global__ __launch_bounds__(kThreads) void k_update(long long n, float *__restr
 void launch_opt_step(const Problem &pr, int kind, double a, double A, double alpha, double fmax, double fmin,
                      double omega, int orthant, float *params_soa, const float *grad_soa, float *prev_grad,
                      double *partials, OptState *st, const Scalars *sc, const double *reg_partials, double l1w,
-                     double l21w, double l2w, const double *gpu_reg_weights, void *record, hipStream_t s)
+                     double l21w, double l2w, const double *gpu_reg_weights, void *record, float *best_params,
+                     hipStream_t s)
 {
     long long n = pr.n_cp * pr.dim;
     int dot_blocks = 64;
@@ -1163,6 +1177,7 @@ void launch_opt_step(const Problem &pr, int kind, double a, double A, double alp
     StepParams sp{kind, a, A, alpha, fmax, fmin, omega, l1w, l21w, l2w,
                   {gpu_reg_weights[0], gpu_reg_weights[1], gpu_reg_weights[2]}, dot_blocks, (int)blocks_for(pr.n_cp)};
     k_step_scalars<<<1, 1, 0, s>>>(sp, partials, reg_partials, st, sc, (skm_iteration_record *)record);
+    if (best_params) k_keep_best<<<blocks_for(n), kThreads, 0, s>>>(n, params_soa, best_params, st);
     k_update<<<blocks_for(n), kThreads, 0, s>>>(n, params_soa, grad_soa, prev_grad, st, orthant);
 }
 

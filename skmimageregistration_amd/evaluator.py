@@ -185,6 +185,11 @@ class GpuKernelEvaluator:
     def SetRegularizerPG(self, weight, scaling=1.0):
         check(self._lib.skm_set_regularizer_pg(self._h, float(weight), float(scaling)))
 
+    def SetDisplacementField(self, displacementField):
+        """Previous-scale displacement (skm_set_displacement_field): an Image
+        with dim set and D components per voxel, or None to clear it."""
+        self._set_df(displacementField)
+
     def _set_df(self, displacementField):
         """The previous-scale field is uploaded when it changes (the reference
         keys its device copy on the host pointer, GpuEvaluator.cu:120-126)."""
@@ -268,13 +273,13 @@ class GpuKernelEvaluator:
     def optimize(self, params_soa, metric=MEASURE.MSE, reg_l1=0.0, reg_l21=0.0, reg_l2=0.0,
                  use_negative_gradient=True, do_resampling=True, kind=_capi.SKM_OPT_ADAPTIVE, a=0.1, A=20.0,
                  alpha=1.0, fmax=1e7, fmin=-1e6, omega=1.0, orthant_projection=False, iterations=100,
-                 cpwData=None):
+                 cpwData=None, return_best=False):
         params = f32(params_soa).copy()
         cpw = f32(cpwData if cpwData is not None else np.ones(self.n_cp, np.float32))
         mo = _capi.SkmMetricOptions(int(metric), reg_l1, reg_l21, reg_l2, int(bool(use_negative_gradient)),
                                     int(bool(do_resampling)))
         oo = _capi.SkmOptimizerOptions(int(kind), a, A, alpha, fmax, fmin, omega, int(bool(orthant_projection)),
-                                       int(iterations))
+                                       int(iterations), int(bool(return_best)))
         recs = (_capi.SkmIterationRecord * max(1, iterations))()
         check(self._lib.skm_optimize(self._h, ctypes.byref(mo), ctypes.byref(oo), fptr(cpw), fptr(params), recs))
         records = [(r.value, r.measure, r.learning_rate, r.num_pixels_counted) for r in recs[:iterations]]
@@ -326,3 +331,96 @@ def ComputeAnisotropicTensor(image, sigma, weight, alignment_direction=2, anisot
                                      ctypes.byref(tilt), ctypes.byref(mw), cov))
     mcov = np.array(cov[:], dtype=np.float64).reshape(3, 3)[:dim, :dim].copy()
     return out, tilt.value, mw.value, mcov
+
+
+class RegistrationConfig:
+    """The SKMReg parameter set (include/ConfigUtils.h:29-62): per-scale lists
+    whose last entry repeats (CURRENT, src/SKMReg.cxx:48-49)."""
+
+    DEFAULTS = {
+        "sigma_C4": [64, 32, 16], "sigma_C0": [128, 64, 32], "cp_spacing_scaler": [4.0],
+        "tensor_weights": [400, 200, 100], "reg_l1": [0, 0, 0], "reg_l21": [0, 0, 0], "reg_l2": [0, 0, 0],
+        "reg_rkhs": [0, 0, 0], "reg_rd": [0, 0, 0], "reg_rd_scaling": [1, 1, 1], "reg_pg": [0, 0, 0],
+        "reg_pg_scaling": [1, 1, 1], "num_scales": [3], "restart_at": [0],
+        "num_function_evaluations": [500, 500, 500], "initial_step_size": [0.1, 0.1, 0.1],
+        "sampling_rate": [5, 4, 3], "neighborhood_sampling": [15, 12, 9], "intensity_thresholds": [-1500, 3500],
+        "smoothing_variance": [2, 1, 0], "metric": "mse",
+    }
+
+    def __init__(self, **overrides):
+        self.values = {k: (list(v) if isinstance(v, list) else v) for k, v in self.DEFAULTS.items()}
+        for k, v in overrides.items():
+            if k not in self.values:
+                raise KeyError(f"unknown option {k}")
+            self.values[k] = list(v) if isinstance(v, (list, tuple)) else ([v] if k != "metric" else v)
+
+    def CURRENT(self, key, scale):
+        v = self.values[key]
+        return v[min(scale, len(v) - 1)]
+
+    def __getitem__(self, key):
+        return self.values[key]
+
+
+def register_multiscale(reference, target, config=None, weight_image=None, weight_tensor=None, guidance_image=None,
+                        restart_displacement=None):
+    """The scale-level loop of SKMReg (src/SKMReg.cxx:84-525) through
+    skm_register.  reference (moving) and target (fixed) are Images; the
+    optional weight image / tensor / guidance image are Images on one common
+    weight grid.  Returns (displacement on the target grid, shape
+    target.data.shape + (D,), list of per-level report dicts)."""
+    lib = _capi.load()
+    cfg = config or RegistrationConfig()
+    dim = target.dim
+    d = _capi.SkmRegistrationDesc()
+    lib.skm_registration_desc_init(ctypes.byref(d))
+    d.dim = dim
+    ref = f32(reference.data)
+    tar = f32(target.data)
+    d.reference = reference.grid()
+    d.reference_data = fptr(ref)
+    d.target = target.grid()
+    d.target_data = fptr(tar)
+    n = int(cfg.CURRENT("num_scales", 0))
+    if not 1 <= n <= _capi.SKM_MAX_SCALES:
+        raise ValueError("num_scales out of range")
+    d.num_scales = n
+    for s in range(n):
+        p = d.scale[s]
+        p.sigma_c4 = cfg.CURRENT("sigma_C4", s)
+        p.sigma_c0 = cfg.CURRENT("sigma_C0", s)
+        p.tensor_weight = cfg.CURRENT("tensor_weights", s)
+        for key in ("reg_l1", "reg_l21", "reg_l2", "reg_rkhs", "reg_rd", "reg_rd_scaling", "reg_pg",
+                    "reg_pg_scaling"):
+            setattr(p, key, float(cfg.CURRENT(key, s)))
+        p.iterations = int(cfg.CURRENT("num_function_evaluations", s))
+        p.initial_step_size = float(cfg.CURRENT("initial_step_size", s))
+        p.sampling_rate = int(cfg.CURRENT("sampling_rate", s))
+        p.neighborhood_sampling = int(cfg.CURRENT("neighborhood_sampling", s))
+        p.smoothing_variance = float(cfg.CURRENT("smoothing_variance", s))
+    d.cp_spacing_scaler = float(cfg.CURRENT("cp_spacing_scaler", 0))
+    d.metric = {"mse": MEASURE.MSE, "ncc": MEASURE.NCC, "lcc": MEASURE.LCC}[cfg["metric"]]
+    th = cfg["intensity_thresholds"]
+    d.use_intensity_thresholds = int(len(th) == 2)
+    if len(th) == 2:
+        d.threshold_min, d.threshold_max = float(th[0]), float(th[1])
+    keep = []
+    for name, img in (("weight_image", weight_image), ("weight_tensor", weight_tensor),
+                      ("guidance_image", guidance_image)):
+        if img is not None:
+            arr = f32(img.data)
+            keep.append(arr)
+            d.weight = img.grid()
+            setattr(d, name, fptr(arr))
+    d.restart_at = int(cfg.CURRENT("restart_at", 0))
+    if restart_displacement is not None:
+        rd = f32(restart_displacement)
+        keep.append(rd)
+        d.restart_displacement = fptr(rd)
+    out = np.empty(tar.shape + (dim,), np.float32)
+    reps = (_capi.SkmScaleReport * n)()
+    check(lib.skm_register(ctypes.byref(d), fptr(out), reps))
+    reports = [{"value": r.value, "measure": r.measure, "first_value": r.first_value, "iterations": r.iterations,
+                "cp_size": tuple(r.cp_size[:dim]), "maximum_tilt": r.maximum_tilt, "ms_setup": r.ms_setup,
+                "ms_optimize": r.ms_optimize, "ms_displacement": r.ms_displacement} for r in reps]
+    return out, reports

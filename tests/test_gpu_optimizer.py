@@ -96,3 +96,23 @@ def test_registration_recovers_blob_warp():
     first, last = recs[0][1], recs[-1][1]
     assert np.isfinite(params).all()
     assert last < first / 3.0, (first, last)
+
+
+def test_return_best_is_the_lowest_value_position():
+    """ReturnBestParametersAndValue: the result is the evaluated position with
+    the lowest value, not the position after the last step."""
+    case = Case(dim=2, shape=(40, 44), cp_spacing=5.0, sigma=12.0, seed=53, coef_scale=0.3)
+    case.cpw[:] = 1.0
+    ev = case.gpu()
+    p0 = case.cp.T.reshape(-1).astype(np.float32)
+    opts = dict(metric=MEASURE.MSE, kind=_capi.SKM_OPT_FIXED, a=4e5, iterations=10)
+    p_last, recs = ev.optimize(p0, **opts)
+    p_best, recs_b = ev.optimize(p0, return_best=True, **opts)
+    values = [r[0] for r in recs_b]
+    assert values == [r[0] for r in recs]          # same trajectory
+    k = int(np.argmin(values))
+    p_k, _ = ev.optimize(p0, **dict(opts, iterations=k))   # the position evaluated at step k
+    ev.close()
+    np.testing.assert_array_equal(p_best, p_k if k > 0 else p0)
+    if k < len(values) - 1:
+        assert not np.array_equal(p_best, p_last)
