@@ -59,64 +59,6 @@ Grid to_grid(const skm_grid &g, int dim)
     return r;
 }
 
-// ---------------------------------------------------------------- prefilter
-
-// Unser-Thevenaz cubic B-spline prefilter along one line, pole sqrt(3)-2,
-// mirror boundary (BsplineImage.h:37-70).
-void prefilter_line(double *c, int n, long long stride, double z)
-{
-    double lambda = 2.0 - z - 1.0 / z;
-    for (int i = 0; i < n; ++i) c[i * stride] *= lambda;
-    // causal initial value: sum over the mirrored signal
-    double zn = z, iz = 1.0 / z, z2n = std::pow(z, (double)(n - 1));
-    double sum = c[0] + z2n * c[(long long)(n - 1) * stride];
-    z2n *= z2n * iz;
-    for (int i = 1; i < n - 1; ++i) {
-        sum += (zn + z2n) * c[i * stride];
-        zn *= z;
-        z2n *= iz;
-    }
-    c[0] = sum / (1.0 - zn * zn);
-    for (int i = 1; i < n; ++i) c[i * stride] += z * c[(i - 1) * stride];
-    c[(long long)(n - 1) * stride] =
-        (z / (z * z - 1.0)) * (z * c[(long long)(n - 2) * stride] + c[(long long)(n - 1) * stride]);
-    for (int i = n - 2; i >= 0; --i) c[i * stride] = z * (c[(i + 1) * stride] - c[i * stride]);
-}
-
-// Interpolation coefficients of the moving image, shifted by one and padded
-// to (n+3)^D with mirrored borders (BsplineImage.h:72-108, 174-221).
-std::vector<float> bspline_coefficients(int dim, const int *size, const float *src)
-{
-    long long n = 1;
-    for (int d = 0; d < dim; ++d) n *= size[d];
-    std::vector<double> c(src, src + n);
-    const double pole = std::sqrt(3.0) - 2.0;
-    if (dim == 2) {
-        for (int y = 0; y < size[1]; ++y) prefilter_line(c.data() + (long long)y * size[0], size[0], 1, pole);
-        for (int x = 0; x < size[0]; ++x) prefilter_line(c.data() + x, size[1], size[0], pole);
-    } else {
-        long long sxy = (long long)size[0] * size[1];
-        for (int y = 0; y < size[1]; ++y)
-            for (int x = 0; x < size[0]; ++x) prefilter_line(c.data() + x + (long long)y * size[0], size[2], sxy, pole);
-        for (int z = 0; z < size[2]; ++z)
-            for (int x = 0; x < size[0]; ++x) prefilter_line(c.data() + x + z * sxy, size[1], size[0], pole);
-        for (int z = 0; z < size[2]; ++z)
-            for (int y = 0; y < size[1]; ++y)
-                prefilter_line(c.data() + (long long)size[0] * (y + (long long)z * size[1]), size[0], 1, pole);
-    }
-    int ps[3] = {size[0] + 3, size[1] + 3, dim == 3 ? size[2] + 3 : 1};
-    std::vector<float> out((size_t)ps[0] * ps[1] * ps[2]);
-    auto mirror = [](int x, int len) { return (len - 1) - std::abs((len - 1) - std::abs(x - 1)); };
-    for (int z = 0; z < ps[2]; ++z)
-        for (int y = 0; y < ps[1]; ++y)
-            for (int x = 0; x < ps[0]; ++x) {
-                int xi = mirror(x, size[0]), yi = mirror(y, size[1]), zi = dim == 3 ? mirror(z, size[2]) : 0;
-                out[x + (size_t)ps[0] * (y + (size_t)ps[1] * z)] =
-                    (float)c[xi + (size_t)size[0] * (yi + (size_t)size[1] * zi)];
-            }
-    return out;
-}
-
 // ---------------------------------------------------------------- host math
 
 double det_h(int dim, const double *m)
@@ -285,48 +227,6 @@ skm_status validate_grid(const skm_grid &g, int dim, const char *name, int min_s
 // getCovarianceAtPoint and getRegionSupport, Kernel.h:148-171) and the three
 // index windows the derivative passes use, all in FP64 as the reference
 // computes them with ScalarType = double.
-// Largest lam (sigma, or the largest covariance eigenvalue) over the points
-// every row of sample cells (my, mz) can take: voxel rows my*sub .. my*sub+sub
-// in y and z (the jitter offset reaches sub), every x; positions off the
-// weight grid take lam_out.  Uploaded as nsub[1] * nsub[2] floats.
-skm_status sample_row_lams(skm_evaluator *ev, const std::vector<double> &wl, double lam_out, int sub, float **dst)
-{
-    const skm_desc &ds = ev->desc;
-    const int dim = ev->dim;
-    int n[3] = {1, 1, 1}, ns[3] = {1, 1, 1};
-    for (int d = 0; d < dim; ++d) {
-        n[d] = ds.fixed.size[d];
-        ns[d] = n[d] / sub;
-    }
-    const int jit = sub > 1 ? sub : 0;
-    // nearest weight voxel of every fixed voxel along each axis (-1 outside)
-    std::vector<int> wi[3];
-    for (int d = 0; d < 3; ++d) {
-        wi[d].assign(n[d] + jit, 0);
-        for (int i = 0; i < n[d] + jit; ++i) {
-            if (d >= dim) { wi[d][i] = 0; continue; }
-            const double p = ds.fixed.origin[d] + ds.fixed.spacing[d] * i;
-            const int k = (int)((1.0 / ds.weight.spacing[d]) * (p - ds.weight.origin[d]) + 0.5);
-            wi[d][i] = (k >= 0 && k < ds.weight.size[d]) ? k : -1;
-        }
-    }
-    const long long wx = ds.weight.size[0], wy = ds.weight.size[1];
-    std::vector<float> out((size_t)ns[1] * ns[2]);
-    for (int mz = 0; mz < ns[2]; ++mz)
-        for (int my = 0; my < ns[1]; ++my) {
-            double m = 0.0;
-            const int z0 = dim == 3 ? mz * sub : 0, z1 = dim == 3 ? mz * sub + jit : 0;
-            for (int z = z0; z <= z1; ++z)
-                for (int y = my * sub; y <= my * sub + jit; ++y)
-                    for (int x = 0; x < ns[0] * sub + jit; ++x) {
-                        const int a = wi[0][x], b = wi[1][y], c = wi[2][z];
-                        m = std::max(m, (a < 0 || b < 0 || c < 0) ? lam_out : wl[a + wx * (b + wy * (long long)c)]);
-                    }
-            out[(size_t)my + (size_t)ns[1] * mz] = (float)(m * (1.0 + 1e-6));
-        }
-    return dev_upload(dst, out.data(), out.size());
-}
-
 skm_status build_static(skm_evaluator *ev)
 {
     const skm_desc &ds = ev->desc;
@@ -422,17 +322,43 @@ skm_status build_static(skm_evaluator *ev)
         const long long nw = grid_count(ds.weight, dim);
         const double lam_out = ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 0.0 : 1.0;
         double lam_w = lam_out;
-        std::vector<double> wl((size_t)nw);
-        for (long long i = 0; i < nw; ++i) {
-            if (ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE) {
-                wl[i] = ds.weight_data[i];
-            } else {
-                double cov[9] = {0}, sym[9];
-                for (int k = 0; k < nn; ++k) cov[k] = ds.weight_data[i * nn + k];
-                symmetric_part(dim, cov, sym);
-                wl[i] = max_eig_h(dim, sym);
-            }
-            lam_w = std::max(lam_w, wl[i]);
+        // device scan of the weight voxels: validation (itkTransformAdapter.hxx:321-324,
+        // 346-349) and lam per voxel
+        float *wlam = nullptr;
+        double *bmax = nullptr;
+        unsigned *bbad = nullptr;
+        const unsigned nb = weight_lam_blocks(nw);
+        auto release = [&]() {
+            if (wlam) (void)hipFree(wlam);
+            if (bmax) (void)hipFree(bmax);
+            if (bbad) (void)hipFree(bbad);
+        };
+        skm_status st;
+        if ((st = dev_alloc(&wlam, (size_t)nw)) != SKM_SUCCESS || (st = dev_alloc(&bmax, nb)) != SKM_SUCCESS ||
+            (st = dev_alloc(&bbad, nb)) != SKM_SUCCESS) {
+            release();
+            return st;
+        }
+        launch_weight_lam(dim, ds.kernel_mode, nw, ev->d_weight, wlam, bmax, bbad, ev->stream);
+        std::vector<double> hmax(nb);
+        std::vector<unsigned> hbad(nb);
+        hipError_t e = hipMemcpyAsync(hmax.data(), bmax, sizeof(double) * nb, hipMemcpyDeviceToHost, ev->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hbad.data(), bbad, sizeof(unsigned) * nb, hipMemcpyDeviceToHost, ev->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ev->stream);
+        if (e != hipSuccess) {
+            release();
+            return fail(SKM_ERROR_HIP, std::string("weight scan: ") + hipGetErrorString(e));
+        }
+        unsigned long long bad = 0;
+        for (unsigned b = 0; b < nb; ++b) {
+            lam_w = std::max(lam_w, hmax[b]);
+            bad += hbad[b];
+        }
+        if (bad) {
+            release();
+            return fail(SKM_ERROR_INVALID_ARGUMENT, ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE
+                                                        ? "Weighting image has to be strictly positive everywhere."
+                                                        : "Weighting tensors must be positive definite.");
         }
         // rounded up so that the FP32 values stay upper bounds
         ev->pr.lam_cp_max = (float)(lam_cp_max * (1.0 + 1e-6));
@@ -447,12 +373,26 @@ skm_status build_static(skm_evaluator *ev)
         // lam bound per row of sample cells (my, mz), jitter included, for the
         // gather's reach; once for the evaluator's subsample and once for the
         // dense voxel grid
-        SKM_TRY(sample_row_lams(ev, wl, lam_out, ds.subsample, &ev->d_smp_row_lam));
-        if (ds.subsample == 1) {
-            ev->d_vox_row_lam = nullptr;  // the dense table is the same
-        } else {
-            SKM_TRY(sample_row_lams(ev, wl, lam_out, 1, &ev->d_vox_row_lam));
+        int cells[3] = {1, 1, 1}, vox[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) {
+            cells[d] = ds.fixed.size[d] / ds.subsample;
+            vox[d] = ds.fixed.size[d];
         }
+        st = dev_alloc(&ev->d_smp_row_lam, (size_t)cells[1] * cells[2]);
+        if (st == SKM_SUCCESS && ds.subsample > 1) st = dev_alloc(&ev->d_vox_row_lam, (size_t)vox[1] * vox[2]);
+        if (st != SKM_SUCCESS) {
+            release();
+            return st;
+        }
+        launch_sample_row_lams(dim, ev->pr.fixed, ev->pr.weight, ds.subsample, cells, wlam, (float)lam_out,
+                               ev->d_smp_row_lam, ev->stream);
+        // the dense voxel grid (exact LCC gather); the same table at subsample 1
+        if (ds.subsample > 1)
+            launch_sample_row_lams(dim, ev->pr.fixed, ev->pr.weight, 1, vox, wlam, (float)lam_out, ev->d_vox_row_lam,
+                                   ev->stream);
+        e = hipStreamSynchronize(ev->stream);
+        release();
+        if (e != hipSuccess) return fail(SKM_ERROR_HIP, std::string("row bounds: ") + hipGetErrorString(e));
     }
     SKM_TRY(dev_upload(&ev->d_cp_window, win.data(), win.size()));
     SKM_TRY(dev_upload(&ev->d_lcc_window, lwin.data(), lwin.size()));
@@ -619,19 +559,7 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
     if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
         SKM_TRY(validate_grid(desc->weight, dim, "weight", 1));
         if (!desc->weight_data) return fail(SKM_ERROR_INVALID_ARGUMENT, "Weight image not provided.");
-        if (desc->kernel_mode == SKM_KERNEL_WEIGHT_IMAGE) {
-            for (long long i = 0; i < nw; ++i)
-                if (!(desc->weight_data[i] > 1e-8))
-                    return fail(SKM_ERROR_INVALID_ARGUMENT, "Weighting image has to be strictly positive everywhere.");
-        } else {
-            const int nn = dim * dim;
-            for (long long i = 0; i < nw; ++i) {
-                double m[9];
-                for (int k = 0; k < nn; ++k) m[k] = desc->weight_data[i * nn + k];
-                if (!(det_h(dim, m) > 0))
-                    return fail(SKM_ERROR_INVALID_ARGUMENT, "Weighting tensors must be positive definite.");
-            }
-        }
+        // the values are validated on the device (build_static)
     }
     if (desc->moving.size[0] > 0xffff || desc->subsample > 255)
         return fail(SKM_ERROR_NOT_SUPPORTED, "image or subsample too large");
@@ -682,13 +610,28 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
 
     // images
     int msize[3] = {desc->moving.size[0], desc->moving.size[1], dim == 3 ? desc->moving.size[2] : 1};
-    std::vector<float> coeffs = bspline_coefficients(dim, msize, desc->moving_data);
-    if ((st = dev_upload(&ev->d_coeffs, coeffs.data(), coeffs.size())) != SKM_SUCCESS) return cleanup(st);
+    {
+        // cubic B-spline coefficients of the moving image, prefiltered on the device in FP64
+        const long long nm = (long long)msize[0] * msize[1] * msize[2];
+        const long long np = (long long)(msize[0] + 3) * (msize[1] + 3) * (dim == 3 ? msize[2] + 3 : 1);
+        float *raw = nullptr;
+        double *work = nullptr;
+        if ((st = dev_upload(&raw, desc->moving_data, (size_t)nm)) != SKM_SUCCESS ||
+            (st = dev_alloc(&work, (size_t)nm)) != SKM_SUCCESS || (st = dev_alloc(&ev->d_coeffs, (size_t)np)) != SKM_SUCCESS) {
+            if (raw) (void)hipFree(raw);
+            if (work) (void)hipFree(work);
+            return cleanup(st);
+        }
+        launch_bspline_coefficients(dim, msize, raw, work, ev->d_coeffs, ev->stream);
+        const hipError_t e = hipStreamSynchronize(ev->stream);
+        (void)hipFree(raw);
+        (void)hipFree(work);
+        if (e != hipSuccess) return cleanup(fail(SKM_ERROR_HIP, std::string("prefilter: ") + hipGetErrorString(e)));
+    }
     if ((st = dev_upload(&ev->d_fixed, desc->fixed_data, (size_t)pr.n_vox)) != SKM_SUCCESS) return cleanup(st);
     if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
         size_t per = desc->kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 1 : (size_t)dim * dim;
-        std::vector<float> w(desc->weight_data, desc->weight_data + nw * per);
-        if ((st = dev_upload(&ev->d_weight, w.data(), w.size())) != SKM_SUCCESS) return cleanup(st);
+        if ((st = dev_upload(&ev->d_weight, desc->weight_data, (size_t)nw * per)) != SKM_SUCCESS) return cleanup(st);
     }
     {
         // static tables need the weight data on the host

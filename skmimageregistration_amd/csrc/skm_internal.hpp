@@ -73,6 +73,15 @@ unsigned reg_blocks(long long n_cp);  // workgroups (and 4-double partial slots)
 void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
 // packed states of the weighted modes at every voxel of the dense fixed grid
 void launch_voxel_states(const Problem &pr, float4 *out, hipStream_t s);
+
+// set-up (skm_setup.hip)
+void launch_bspline_coefficients(int dim, const int *size, const float *d_src, double *d_work, float *d_coeffs,
+                                 hipStream_t s);
+unsigned weight_lam_blocks(long long nw);
+void launch_weight_lam(int dim, int mode, long long nw, const float *d_w, float *d_lam, double *d_blk_max,
+                       unsigned *d_blk_bad, hipStream_t s);
+void launch_sample_row_lams(int dim, const Grid &fixed, const Grid &weight, int sub, const int *ncell,
+                            const float *d_wlam, float lam_out, float *d_out, hipStream_t s);
 void launch_result(const Scalars *sc, void *result, hipStream_t s);
 
 // LCC (skm_lcc.hip)

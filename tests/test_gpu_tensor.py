@@ -144,3 +144,27 @@ def test_kernel_reach_beyond_the_support_estimate(dim, mode, measure):
     assert scalar_rel(res.measure, ores["measure"]) < 1e-5
     assert rel_err(der, oder) < 1e-4
     assert rel_err(disp.reshape(-1), orc_pr.displacement().reshape(-1)) < 1e-5
+
+
+@pytest.mark.parametrize("dim,mode", [(3, 1), (2, 1), (3, 2), (2, 2)])
+def test_invalid_weights_are_rejected(dim, mode):
+    """The weight scan on the device rejects a weight image that is not
+    strictly positive and a tensor that is not positive definite with the
+    reference's messages (itkTransformAdapter.hxx:321-324, 346-349)."""
+    case = Case(dim=dim, mode=mode, cp_spacing=5.0, sigma=8.0, subsample=1, seed=31)
+    bad = tuple(n // 2 for n in case.shape)
+    if mode == 1:
+        w = np.ones(case.shape, np.float32)
+        w[bad] = 0.0
+        case.weight = w
+        case.max_weight = 1.0
+        msg = "strictly positive"
+    else:
+        w = np.broadcast_to(np.eye(dim, dtype=np.float32), case.shape + (dim, dim)).copy()
+        w[bad] = -np.eye(dim, dtype=np.float32) if dim == 3 else np.diag([1.0, -1.0]).astype(np.float32)
+        case.weight = w
+        case.max_eig = np.eye(dim)
+        msg = "positive definite"
+    print(f"invalid weights dim={dim} mode={mode}", flush=True)
+    with pytest.raises(SkmError, match=msg):
+        case.gpu()
