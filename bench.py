@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--measure", default="ncc", choices=["mse", "ncc", "lcc"])
     ap.add_argument("--reg-rd", type=float, default=1e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-registration", action="store_true", help="skip the multi-scale registration timing")
+    ap.add_argument("--registration-repeats", type=int, default=3)
+    ap.add_argument("--registration-rd", type=float, default=1e-5,
+                    help="RD weight of the registration (the bilateral-like regularizer)")
     ap.add_argument("--no-pair-count", action="store_true", help="skip the exact k != 0 pair count")
     ap.add_argument("--cpu-crop", default=None, help="z,y,x voxels of the CPU baseline sample")
     ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
@@ -203,8 +207,7 @@ def count_nonzero_pairs(ten, size, cs, co, h, smin, vlo, vhi, dlo, dhi, dev):
 def workload(args):
     from skmimageregistration_amd import synthetic
     shape = tuple(int(v) for v in args.shape.split(","))
-    mov, fix, _ = synthetic.ellipsoids_3d(shape, n=20, smooth=1.5, max_disp=6.0, noise=0.02)
-    return mov, fix
+    return synthetic.ellipsoids_3d(shape, n=20, smooth=1.5, max_disp=6.0, noise=0.02)
 
 
 def kernel_for(args, fix):
@@ -213,6 +216,7 @@ def kernel_for(args, fix):
     from skmimageregistration_amd import ComputeAnisotropicTensor, Grid, Image, Kernel
     if args.kernel == "stationary":
         return Kernel(args.sigma, args.sigma), None, None, None
+    ComputeAnisotropicTensor(Image(fix), args.sigma, args.tensor_weight, 2)  # first HIP use, untimed
     t0 = time.perf_counter()
     ten, tilt, mw, mcov = ComputeAnisotropicTensor(Image(fix), args.sigma, args.tensor_weight, 2)
     tensor_ms = (time.perf_counter() - t0) * 1e3
@@ -303,6 +307,39 @@ def cpu_baseline(args, mov, fix, ten, mcov):
                       f"(same kernel, sigma and cp spacing), {dt:.1f} s"}
 
 
+def registration(args, mov, fix, disp):
+    """Wall time of one multi-scale registration (SKMReg's scale loop,
+    SKMReg.cxx:123-500): 3 levels with support 4s, 2s, s, control-point
+    spacing support/4, tensors from the target with weights 400/200/100,
+    NCC, sampling rates 5/4/3, 200 adaptive-step iterations per level, then
+    the dense displacement field.  Host images in, dense field out, timed
+    after one untimed run; the median of the repeats."""
+    from skmimageregistration_amd import Image, RegistrationConfig, register_multiscale
+    s = args.sigma
+    cfg = RegistrationConfig(sigma_C4=[4 * s, 2 * s, s], sigma_C0=[4 * s, 2 * s, s],
+                             tensor_weights=[args.tensor_weight, args.tensor_weight / 2, args.tensor_weight / 4]
+                             if args.kernel == "anisotropic" else [0.0],
+                             num_scales=3, num_function_evaluations=[200], initial_step_size=[1000.0],
+                             sampling_rate=[5, 4, 3], neighborhood_sampling=[1], smoothing_variance=[2, 1, 0],
+                             reg_rd=[args.registration_rd], metric="ncc")
+    register_multiscale(Image(mov), Image(fix), cfg)
+    times, reports, df = [], None, None
+    for _ in range(max(args.registration_repeats, 1)):
+        t0 = time.perf_counter()
+        df, reports = register_multiscale(Image(mov), Image(fix), cfg)
+        times.append((time.perf_counter() - t0) * 1e3)
+    inner = (slice(10, -10),) * 3
+    return {"ms_per_registration": round(float(np.median(times)), 2),
+            "levels": 3, "iterations_per_level": 200, "support": [4 * s, 2 * s, s],
+            "cp_size": [list(r["cp_size"]) for r in reports],
+            "first_value": round(reports[0]["first_value"], 6), "final_value": round(reports[-1]["value"], 6),
+            "final_ncc": round(reports[-1]["measure"], 6),
+            "level_ms": [{k: round(r[k], 2) for k in ("ms_setup", "ms_optimize", "ms_displacement")} for r in reports],
+            "mean_abs_error_vox": round(float(np.abs(df[inner] - disp[inner]).mean()), 4),
+            "mean_abs_displacement_vox": round(float(np.abs(disp[inner]).mean()), 4),
+            "repeats_ms": [round(t, 2) for t in times], "reg_rd": args.registration_rd}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -313,7 +350,7 @@ def main():
 
     from skmimageregistration_amd import GpuKernelEvaluator, Grid, Image, MEASURE
 
-    mov, fix = workload(args)
+    mov, fix, disp = workload(args)
     shape = fix.shape
     size, cs, co = geometry(shape, args.cp_spacing)
     n_cp = int(np.prod(cs))
@@ -398,6 +435,8 @@ def main():
     }
     if tensor_ms is not None:
         line["tensor_generation_ms"] = round(tensor_ms, 2)
+    if not args.no_registration:
+        line["registration"] = registration(args, mov, fix, disp)
     print(json.dumps(line))
 
 
