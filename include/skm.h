@@ -37,7 +37,8 @@ typedef enum skm_status {
     SKM_ERROR_INVALID_ARGUMENT = 1,
     SKM_ERROR_HIP = 2,
     SKM_ERROR_OUT_OF_MEMORY = 3,
-    SKM_ERROR_NOT_SUPPORTED = 4
+    SKM_ERROR_NOT_SUPPORTED = 4,
+    SKM_ERROR_IO = 5
 } skm_status;
 
 /* GpuKernelEvaluator::MEASURE (GpuEvaluator.h:160) */
@@ -202,6 +203,65 @@ SKM_API skm_status skm_anisotropic_tensor(int32_t dim, const skm_grid *grid, con
                                           const skm_tensor_options *opt, float *tensor_out, double *maximum_tilt,
                                           double *maximum_weight, double *maximum_eigenvalue_covariance);
 
+/* ---- warped images ------------------------------------------------------ */
+
+typedef enum skm_interpolation {
+    SKM_INTERPOLATION_LINEAR = 1,    /* itk::LinearInterpolateImageFunction       */
+    SKM_INTERPOLATION_BSPLINE3 = 3   /* itk::BSplineInterpolateImageFunction, order 3 */
+} skm_interpolation;
+
+/* Image warped by a dense displacement field, on the GPU: out(x) = I(x + u(x))
+ * at every voxel x of out_grid, default_value where x + u(x) leaves the
+ * image's buffer ([-0.5, n - 0.5) in index space).
+ *   LINEAR:   the final registered image of SKMReg (src/SKMReg.cxx:507-525):
+ *             ResampleImageFilter with a DisplacementFieldTransform -- u
+ *             interpolated linearly in df (zero outside its buffer), the image
+ *             interpolated linearly;
+ *   BSPLINE3: WarpImage (include/itkUtils.h:163-179): WarpImageFilter on the
+ *             field's own grid (out_grid must equal df_grid), the image
+ *             interpolated by cubic B-splines with the mirror boundary.
+ * Host pointers; df_aos holds D floats per df voxel. */
+SKM_API skm_status skm_warp_image(int32_t dim, const skm_grid *image_grid, const float *image,
+                                  const skm_grid *df_grid, const float *df_aos, const skm_grid *out_grid,
+                                  int32_t interpolation, double default_value, float *out);
+
+/* ---- image files and landmarks ------------------------------------------ */
+
+/* What an image file holds: dimension, geometry and values per voxel
+ * (1: scalar image, D: displacement field). */
+typedef struct skm_image_header {
+    int32_t dim;
+    skm_grid grid;
+    int32_t components;
+} skm_image_header;
+
+/* The files of itk::ImageFileReader / ImageFileWriter as SKMReg uses them
+ * (include/IOUtils.h:27-52): VTK legacy STRUCTURED_POINTS (.vtk; ASCII or
+ * big-endian BINARY; SCALARS, VECTORS or TENSORS point data) and MetaImage
+ * (.mhd with its data file, .mha; uncompressed, identity direction).  A VTK
+ * file with DIMENSIONS nz = 1 reads as 2D.  Stored types are converted to
+ * float; writing stores float (VTK: big-endian BINARY, 3 components as
+ * VECTORS; MetaImage: little-endian, .mhd writes <name>.raw beside it). */
+SKM_API skm_status skm_read_image_header(const char *path, skm_image_header *header);
+/* capacity: floats the buffer holds (voxels * components needed); header may be NULL. */
+SKM_API skm_status skm_read_image(const char *path, float *data, int64_t capacity, skm_image_header *header);
+SKM_API skm_status skm_write_image(const char *path, const skm_image_header *header, const float *data);
+
+/* ReadLandmarks (include/IOUtils.h:65-107): lines starting with '#' are
+ * skipped, "POINTS n ..." declares n, every complete triple of numbers on any
+ * other line is a point of which the first dim coordinates are kept.  points
+ * may be NULL to count; *declared (may be NULL) receives the POINTS count,
+ * which the reference only warns about when it differs. */
+SKM_API skm_status skm_read_landmarks(const char *path, int32_t dim, double *points, int64_t capacity,
+                                      int64_t *count, int64_t *declared);
+
+/* CalculateTRE (include/itkUtils.h:268-293): the mean over i of
+ * |m_i + u(m_i) - f_i| with u interpolated linearly in the dense field
+ * (the point is left unchanged outside the field's buffer); -1 when count is 0. */
+SKM_API skm_status skm_landmark_tre(int32_t dim, const skm_grid *df_grid, const float *df_aos,
+                                    const double *fixed_points, const double *moving_points, int64_t count,
+                                    double *tre);
+
 /* ---- metric assembly and optimisation, device resident ---------------- */
 
 /* RegularizedImageToImageMetricv4 switches (itkRegularizedImageToImageMetricv4.h:124-160). */
@@ -310,6 +370,18 @@ typedef struct skm_registration_desc {
     int32_t restart_at;                                /* restart_at (1-based level) */
     const float *restart_displacement;                 /* df-full of level restart_at-1,
                                                           AoS on the target grid    */
+    const char *temp_directory;                        /* temp_directory, or NULL: SKMReg's
+                                                          result files are written there
+                                                          (SKMReg.cxx:92-94, 150-153,
+                                                          459-525) and restart_at without
+                                                          restart_displacement reads
+                                                          df-full-<restart_at-2>.vtk
+                                                          (SKMReg.cxx:96-113)        */
+    int32_t verbosity;                                 /* >= 2: also the smoothed images,
+                                                          df-<s> and warped-<s>     */
+    int64_t num_landmarks;                             /* landmark pairs for the TRE */
+    const double *reference_landmarks;                 /* D doubles per point       */
+    const double *target_landmarks;
 } skm_registration_desc;
 
 /* What one level did. */
@@ -321,6 +393,8 @@ typedef struct skm_scale_report {
     int32_t cp_size[3];
     double maximum_tilt;             /* ComputeAnisotropicTensor's, or 0           */
     double ms_setup, ms_optimize, ms_displacement;
+    double tre;                      /* CalculateTRE of df-full, -1 without landmarks */
+    double zero_parameters;          /* percentage of parameters equal to 0          */
 } skm_scale_report;
 
 /* ConfigUtils.h:29-62 defaults (3 levels, metric MSE, sigma_C4 64/32/16, ...). */
@@ -328,7 +402,13 @@ SKM_API void skm_registration_desc_init(skm_registration_desc *desc);
 
 /* Runs the registration.  displacement_out (host, D floats per target voxel)
  * receives the displacement of all levels; reports (host, num_scales entries,
- * may be NULL) the per-level summaries (levels before restart_at untouched). */
+ * may be NULL) the per-level summaries (levels before restart_at untouched).
+ * With temp_directory set, the files written there are SKMReg's:
+ * reference-/target-preprocessed.vtk, per level s (0-based)
+ * covariance_tensor-<s>.vtk (tensor kernels), df-full-<s>.vtk,
+ * cp_image-<s>.vtk, cp_jacdet-<s>.vtk (and at verbosity >= 2
+ * smoothed_reference-<s>.vtk, smoothed_target-<s>.vtk, df-<s>.vtk,
+ * warped-<s>.vtk), then df.vtk and warped.vtk. */
 SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displacement_out,
                                 skm_scale_report *reports);
 

@@ -7,11 +7,12 @@ interface over that ABI.
 from . import _capi
 from .evaluator import (MEASURE, ComputeAnisotropicTensor, EvaluationResult, GaussianSmoothing,
                         GpuKernelEvaluator, Grid, Image, Kernel, RegistrationConfig, control_point_grid,
-                        register_multiscale)
+                        landmark_tre, read_image, read_landmarks, register_files, register_multiscale,
+                        warp_image, write_image)
 
 __all__ = ["MEASURE", "ComputeAnisotropicTensor", "EvaluationResult", "GaussianSmoothing", "GpuKernelEvaluator",
-           "Grid", "Image", "Kernel", "RegistrationConfig", "control_point_grid", "load_library",
-           "register_multiscale"]
+           "Grid", "Image", "Kernel", "RegistrationConfig", "control_point_grid", "landmark_tre", "load_library",
+           "read_image", "read_landmarks", "register_files", "register_multiscale", "warp_image", "write_image"]
 
 
 def load_library():

@@ -12,6 +12,8 @@ _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG_DIR, "libskm.so")
 
 SKM_SUCCESS = 0
+SKM_ERROR_INVALID_ARGUMENT, SKM_ERROR_HIP, SKM_ERROR_OUT_OF_MEMORY, SKM_ERROR_NOT_SUPPORTED, SKM_ERROR_IO = 1, 2, 3, 4, 5
+SKM_INTERPOLATION_LINEAR, SKM_INTERPOLATION_BSPLINE3 = 1, 3
 SKM_MEASURE_MSE, SKM_MEASURE_NCC, SKM_MEASURE_LCC = 0, 1, 2
 SKM_KERNEL_STATIONARY, SKM_KERNEL_WEIGHT_IMAGE, SKM_KERNEL_WEIGHT_TENSOR = 0, 1, 2
 SKM_FLAG_EXACT_GRADIENT = 0x1
@@ -103,14 +105,21 @@ class SkmRegistrationDesc(ctypes.Structure):
                 ("threshold_max", ctypes.c_double),
                 ("weight", SkmGrid), ("weight_image", c_float_p), ("weight_tensor", c_float_p),
                 ("guidance_image", c_float_p), ("restart_at", ctypes.c_int32),
-                ("restart_displacement", c_float_p)]
+                ("restart_displacement", c_float_p), ("temp_directory", ctypes.c_char_p),
+                ("verbosity", ctypes.c_int32), ("num_landmarks", ctypes.c_int64),
+                ("reference_landmarks", c_double_p), ("target_landmarks", c_double_p)]
 
 
 class SkmScaleReport(ctypes.Structure):
     _fields_ = [("value", ctypes.c_double), ("measure", ctypes.c_double), ("first_value", ctypes.c_double),
                 ("iterations", ctypes.c_int32), ("cp_size", ctypes.c_int32 * 3), ("maximum_tilt", ctypes.c_double),
                 ("ms_setup", ctypes.c_double), ("ms_optimize", ctypes.c_double),
-                ("ms_displacement", ctypes.c_double)]
+                ("ms_displacement", ctypes.c_double), ("tre", ctypes.c_double),
+                ("zero_parameters", ctypes.c_double)]
+
+
+class SkmImageHeader(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("grid", SkmGrid), ("components", ctypes.c_int32)]
 
 
 # Every symbol include/skm.h declares, with (restype, argtypes).
@@ -150,6 +159,16 @@ _SIGNATURES = {
                                               c_double_p]),
     "skm_registration_desc_init": (None, [ctypes.POINTER(SkmRegistrationDesc)]),
     "skm_register": (ctypes.c_int, [ctypes.POINTER(SkmRegistrationDesc), c_float_p, ctypes.POINTER(SkmScaleReport)]),
+    "skm_warp_image": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p, ctypes.POINTER(SkmGrid),
+                                      c_float_p, ctypes.POINTER(SkmGrid), ctypes.c_int32, ctypes.c_double,
+                                      c_float_p]),
+    "skm_read_image_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(SkmImageHeader)]),
+    "skm_read_image": (ctypes.c_int, [ctypes.c_char_p, c_float_p, ctypes.c_int64, ctypes.POINTER(SkmImageHeader)]),
+    "skm_write_image": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(SkmImageHeader), c_float_p]),
+    "skm_read_landmarks": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, c_double_p, ctypes.c_int64,
+                                          ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "skm_landmark_tre": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p, c_double_p, c_double_p,
+                                        ctypes.c_int64, c_double_p]),
     "skm_host_regularizers": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, c_double_p, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_int32, c_double_p,
                                              c_double_p, c_double_p, c_double_p, c_double_p]),

@@ -75,6 +75,9 @@ void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
 void launch_voxel_states(const Problem &pr, float4 *out, hipStream_t s);
 
 // set-up (skm_setup.hip)
+// cubic B-spline coefficients (mirror boundary) in FP64, unpadded, in d_work
+void launch_bspline_prefilter(int dim, const int *size, const float *d_src, double *d_work, hipStream_t s);
+// the same, then shifted by one and padded to (n+3)^D in float
 void launch_bspline_coefficients(int dim, const int *size, const float *d_src, double *d_work, float *d_coeffs,
                                  hipStream_t s);
 unsigned weight_lam_blocks(long long nw);

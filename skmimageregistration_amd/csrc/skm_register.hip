@@ -94,6 +94,52 @@ double max_eig_sym(int dim, const float *m)
     return q + 2.0 * p * std::cos(phi);
 }
 
+// DisplacementFieldJacobianDeterminantFilter (include/itkUtils.h:153-161) on
+// a D-vector image: det(I + J) with J_ij = (v_j[x + e_i] - v_j[x - e_i]) *
+// 0.5 / spacing_i, neighbours clamped to the image (zero-flux Neumann).
+std::vector<float> jacobian_determinant(int dim, const skm_grid &g, const float *v)
+{
+    const int n[3] = {g.size[0], g.size[1], dim == 3 ? g.size[2] : 1};
+    const long long nv = (long long)n[0] * n[1] * n[2];
+    std::vector<float> out((size_t)nv);
+    for (long long i = 0; i < nv; ++i) {
+        const int idx[3] = {(int)(i % n[0]), (int)((i / n[0]) % n[1]), (int)(i / ((long long)n[0] * n[1]))};
+        double J[3][3] = {{0}};
+        for (int a = 0; a < dim; ++a) {
+            int lo[3] = {idx[0], idx[1], idx[2]}, hi[3] = {idx[0], idx[1], idx[2]};
+            lo[a] = std::max(idx[a] - 1, 0);
+            hi[a] = std::min(idx[a] + 1, n[a] - 1);
+            const long long il = lo[0] + (long long)n[0] * (lo[1] + (long long)n[1] * lo[2]);
+            const long long ih = hi[0] + (long long)n[0] * (hi[1] + (long long)n[1] * hi[2]);
+            for (int c = 0; c < dim; ++c)
+                J[a][c] = 0.5 / g.spacing[a] * ((double)v[ih * dim + c] - (double)v[il * dim + c]);
+            J[a][a] += 1.0;
+        }
+        out[i] = (float)(dim == 2 ? J[0][0] * J[1][1] - J[0][1] * J[1][0]
+                                  : J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]));
+    }
+    return out;
+}
+
+// Result files of SKMReg under temp_directory.
+struct Output {
+    std::string dir;
+    int dim;
+    bool on() const { return !dir.empty(); }
+    skm_status write(const std::string &name, const skm_grid &g, int components, const float *data) const
+    {
+        if (!on()) return SKM_SUCCESS;
+        skm_image_header h;
+        std::memset(&h, 0, sizeof(h));
+        h.dim = dim;
+        h.grid = g;
+        h.components = components;
+        return skm_write_image((dir + "/" + name).c_str(), &h, data);
+    }
+};
+
 struct EvaluatorGuard {
     skm_evaluator *ev = nullptr;
     ~EvaluatorGuard()
@@ -169,13 +215,31 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
     }
     rescale_image(ref, 0.0, 1.0);
     rescale_image(tar, 0.0, 1.0);
+    const Output out{desc->temp_directory ? std::string(desc->temp_directory) : std::string(), dim};
+    SKM_R_TRY(out.write("reference-preprocessed.vtk", desc->reference, 1, ref.data()));
+    SKM_R_TRY(out.write("target-preprocessed.vtk", desc->target, 1, tar.data()));
+    const bool landmarks = desc->num_landmarks > 0 && desc->reference_landmarks && desc->target_landmarks;
 
-    // previous displacement (restart, SKMReg.cxx:96-113)
+    // previous displacement (restart, SKMReg.cxx:96-113): given, or read from
+    // df-full-<restart_at-2>.vtk; a file that cannot be read (or does not fit
+    // the target grid) starts the registration at the first level
     int start = 0;
     std::vector<float> prev;
-    if (desc->restart_at > 1 && desc->restart_at <= desc->num_scales && desc->restart_displacement) {
-        start = desc->restart_at - 1;
-        prev.assign(desc->restart_displacement, desc->restart_displacement + n_tar * dim);
+    if (desc->restart_at > 1 && desc->restart_at <= desc->num_scales) {
+        if (desc->restart_displacement) {
+            start = desc->restart_at - 1;
+            prev.assign(desc->restart_displacement, desc->restart_displacement + n_tar * dim);
+        } else if (out.on()) {
+            const std::string path = out.dir + "/df-full-" + std::to_string(std::max(0, desc->restart_at - 2)) + ".vtk";
+            skm_image_header h;
+            std::vector<float> df((size_t)n_tar * dim);
+            if (skm_read_image_header(path.c_str(), &h) == SKM_SUCCESS && h.components == dim &&
+                voxels(h.grid, dim) == n_tar &&
+                skm_read_image(path.c_str(), df.data(), (int64_t)df.size(), nullptr) == SKM_SUCCESS) {
+                start = desc->restart_at - 1;
+                prev.swap(df);
+            }
+        }
     }
     const long long n_w = voxels(desc->weight, dim);
 
@@ -193,6 +257,10 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         } else {
             ref_s = ref;
             tar_s = tar;
+        }
+        if (desc->verbosity >= 2) {
+            SKM_R_TRY(out.write("smoothed_reference-" + std::to_string(scale) + ".vtk", desc->reference, 1, ref_s.data()));
+            SKM_R_TRY(out.write("smoothed_target-" + std::to_string(scale) + ".vtk", desc->target, 1, tar_s.data()));
         }
 
         // kernel: weight image (takes precedence, GpuEvaluator.cu:311-346),
@@ -251,6 +319,9 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
             std::memcpy(ed.maximum_eigenvalue_covariance, mcov, sizeof(mcov));
             rep.maximum_tilt = tilt;
         }
+        if (ed.kernel_mode == SKM_KERNEL_WEIGHT_TENSOR)  // SKMReg.cxx:290-297
+            SKM_R_TRY(out.write("covariance_tensor-" + std::to_string(scale) + ".vtk", ed.weight, dim * dim,
+                                ed.weight_data));
 
         // control points (SKMReg.cxx:219-232)
         double sp[3];
@@ -304,10 +375,47 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         SKM_R_TRY(skm_evaluate_displacement_field(ev, cp_aos.data(), cpw.data(), df.data()));
         prev.swap(df);
         rep.ms_displacement = ms_since(t_df);
+
+        // results of the level (SKMReg.cxx:459-505)
+        const std::string s = std::to_string(scale);
+        SKM_R_TRY(out.write("df-full-" + s + ".vtk", desc->target, dim, prev.data()));
+        SKM_R_TRY(out.write("cp_image-" + s + ".vtk", ed.cp, dim, cp_aos.data()));
+        if (out.on()) {
+            const std::vector<float> jac = jacobian_determinant(dim, ed.cp, cp_aos.data());
+            SKM_R_TRY(out.write("cp_jacdet-" + s + ".vtk", ed.cp, 1, jac.data()));
+        }
+        rep.tre = -1.0;
+        if (landmarks)
+            SKM_R_TRY(skm_landmark_tre(dim, &desc->target, prev.data(), desc->reference_landmarks,
+                                       desc->target_landmarks, desc->num_landmarks, &rep.tre));
+        if (out.on() && desc->verbosity >= 2) {
+            // the level's own transform, without the coarser levels
+            std::vector<float> own((size_t)n_tar * dim);
+            SKM_R_TRY(skm_set_displacement_field(ev, nullptr, nullptr));
+            SKM_R_TRY(skm_evaluate_displacement_field(ev, cp_aos.data(), cpw.data(), own.data()));
+            SKM_R_TRY(out.write("df-" + s + ".vtk", desc->target, dim, own.data()));
+            std::vector<float> warped((size_t)n_tar);
+            SKM_R_TRY(skm_warp_image(dim, &desc->reference, ref.data(), &desc->target, prev.data(), &desc->target,
+                                     SKM_INTERPOLATION_BSPLINE3, 0.0, warped.data()));
+            SKM_R_TRY(out.write("warped-" + s + ".vtk", desc->target, 1, warped.data()));
+        }
+        long long zeros = 0;
+        for (float v : params) zeros += v == 0.0f;
+        rep.zero_parameters = params.empty() ? 0.0 : 100.0 * (double)zeros / (double)params.size();
         if (reports) reports[scale] = rep;
     }
     if (prev.empty()) prev.assign((size_t)n_tar * dim, 0.0f);
     std::memcpy(displacement_out, prev.data(), sizeof(float) * prev.size());
+
+    // final results (SKMReg.cxx:507-525): the dense field and the reference
+    // resampled onto the target grid through it, linear interpolation, 0 outside
+    if (out.on()) {
+        SKM_R_TRY(out.write("df.vtk", desc->target, dim, prev.data()));
+        std::vector<float> warped((size_t)n_tar);
+        SKM_R_TRY(skm_warp_image(dim, &desc->reference, ref.data(), &desc->target, prev.data(), &desc->target,
+                                 SKM_INTERPOLATION_LINEAR, 0.0, warped.data()));
+        SKM_R_TRY(out.write("warped.vtk", desc->target, 1, warped.data()));
+    }
     return SKM_SUCCESS;
 }
 

@@ -203,8 +203,7 @@ unsigned blocks(long long n) { return (unsigned)((n + kThreads - 1) / kThreads);
 
 }  // namespace
 
-void launch_bspline_coefficients(int dim, const int *size, const float *d_src, double *d_work, float *d_coeffs,
-                                 hipStream_t s)
+void launch_bspline_prefilter(int dim, const int *size, const float *d_src, double *d_work, hipStream_t s)
 {
     const int n0 = size[0], n1 = size[1], n2 = dim == 3 ? size[2] : 1;
     const long long n = (long long)n0 * n1 * n2;
@@ -219,6 +218,13 @@ void launch_bspline_coefficients(int dim, const int *size, const float *d_src, d
         k_prefilter_axis<<<blocks((long long)n1), kThreads, 0, s>>>(d_work, n0, n1, 1, 0, pole);
         k_prefilter_axis<<<blocks((long long)n0), kThreads, 0, s>>>(d_work, n0, n1, 1, 1, pole);
     }
+}
+
+void launch_bspline_coefficients(int dim, const int *size, const float *d_src, double *d_work, float *d_coeffs,
+                                 hipStream_t s)
+{
+    const int n0 = size[0], n1 = size[1], n2 = dim == 3 ? size[2] : 1;
+    launch_bspline_prefilter(dim, size, d_src, d_work, s);
     const long long np = (long long)(n0 + 3) * (n1 + 3) * (dim == 3 ? n2 + 3 : 1);
     k_pad_mirror<<<blocks(np), kThreads, 0, s>>>(d_work, d_coeffs, dim, n0, n1, n2);
 }

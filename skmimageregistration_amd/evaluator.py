@@ -7,6 +7,9 @@ geometry tuples are in (x, y[, z]) order.
 """
 import ctypes
 import enum
+import json
+import os
+import warnings
 
 import numpy as np
 
@@ -333,11 +336,100 @@ def ComputeAnisotropicTensor(image, sigma, weight, alignment_direction=2, anisot
     return out, tilt.value, mw.value, mcov
 
 
+# ------------------------------------------------------------ files and warped images
+
+def read_image(path):
+    """itk::ImageFileReader (include/IOUtils.h:27-37) for VTK legacy
+    (.vtk) and MetaImage (.mhd, .mha) files.  Returns an Image; vector and
+    tensor images carry their components on trailing axes ((D,) or (D, D))."""
+    lib = _capi.load()
+    h = _capi.SkmImageHeader()
+    check(lib.skm_read_image_header(os.fsencode(path), ctypes.byref(h)))
+    dim, nc = h.dim, h.components
+    size = [h.grid.size[d] for d in range(dim)]
+    n = int(np.prod(size)) * nc
+    buf = np.empty(max(n, 1), np.float32)
+    check(lib.skm_read_image(os.fsencode(path), fptr(buf), n, None))
+    trailing = () if nc == 1 else ((dim, dim) if nc == dim * dim and dim != 1 else (nc,))
+    data = buf[:n].reshape(tuple(reversed(size)) + trailing)
+    return Image(data, tuple(h.grid.origin[:dim]), tuple(h.grid.spacing[:dim]), dim=dim)
+
+
+def write_image(path, image):
+    """itk::ImageFileWriter (include/IOUtils.h:39-52): .vtk (big-endian
+    binary float), .mhd (+ .raw) or .mha."""
+    lib = _capi.load()
+    data = f32(image.data)
+    h = _capi.SkmImageHeader()
+    h.dim = image.dim
+    h.grid = image.grid()
+    h.components = int(np.prod(data.shape[image.dim:])) if data.ndim > image.dim else 1
+    check(lib.skm_write_image(os.fsencode(path), ctypes.byref(h), fptr(data)))
+
+
+def warp_image(image, displacement, interpolation="linear", out_grid=None, default_value=0.0):
+    """image(x + u(x)) on the GPU for every voxel x of out_grid (default: the
+    displacement's grid); default_value outside the image.  "linear": SKMReg's
+    final resampling (src/SKMReg.cxx:507-525); "bspline3": WarpImage
+    (include/itkUtils.h:163-179), out_grid must be the displacement's grid.
+    displacement is an Image with dim=D and a trailing axis of D components."""
+    lib = _capi.load()
+    kind = {"linear": _capi.SKM_INTERPOLATION_LINEAR, "bspline3": _capi.SKM_INTERPOLATION_BSPLINE3}[interpolation]
+    dim = image.dim
+    img = f32(image.data)
+    df = f32(displacement.data)
+    gi, gd = image.grid(), displacement.grid()
+    og = out_grid if out_grid is not None else Grid(displacement.size(), displacement.origin, displacement.spacing)
+    go = og.grid()
+    out = np.empty(tuple(reversed(og.size())), np.float32)
+    check(lib.skm_warp_image(dim, ctypes.byref(gi), fptr(img), ctypes.byref(gd), fptr(df), ctypes.byref(go), kind,
+                             float(default_value), fptr(out)))
+    return Image(out, og.origin, og.spacing)
+
+
+def read_landmarks(path, dim):
+    """ReadLandmarks (include/IOUtils.h:65-107): an (n, dim) float64 array;
+    warns, as the reference does, when n differs from the POINTS count."""
+    lib = _capi.load()
+    n = ctypes.c_int64()
+    declared = ctypes.c_int64()
+    check(lib.skm_read_landmarks(os.fsencode(path), dim, None, 0, ctypes.byref(n), ctypes.byref(declared)))
+    pts = np.zeros((max(n.value, 1), dim), np.float64)
+    check(lib.skm_read_landmarks(os.fsencode(path), dim, pts.ctypes.data_as(_capi.c_double_p), n.value,
+                                 ctypes.byref(n), ctypes.byref(declared)))
+    if n.value != declared.value:
+        warnings.warn("number of points read does not correspond to the number of points declared in the header")
+    return pts[: n.value]
+
+
+def landmark_tre(displacement, fixed_points, moving_points):
+    """CalculateTRE (include/itkUtils.h:268-293): mean |m + u(m) - f|, -1
+    without landmarks."""
+    lib = _capi.load()
+    dim = displacement.dim
+    f = np.ascontiguousarray(fixed_points, np.float64).reshape(-1, dim)
+    m = np.ascontiguousarray(moving_points, np.float64).reshape(-1, dim)
+    if len(f) != len(m):
+        return -1.0
+    df = f32(displacement.data)
+    g = displacement.grid()
+    tre = ctypes.c_double()
+    check(lib.skm_landmark_tre(dim, ctypes.byref(g), fptr(df), f.ctypes.data_as(_capi.c_double_p),
+                               m.ctypes.data_as(_capi.c_double_p), len(f), ctypes.byref(tre)))
+    return tre.value
+
+
 class RegistrationConfig:
     """The SKMReg parameter set (include/ConfigUtils.h:29-62): per-scale lists
     whose last entry repeats (CURRENT, src/SKMReg.cxx:48-49)."""
 
+    STRING_KEYS = ("metric", "reference_filename", "target_filename", "reference_landmarks", "target_landmarks",
+                   "weight_filename", "weight_tensor_filename", "temp_directory")
+
     DEFAULTS = {
+        "reference_filename": "/tmp/ref.vtk", "target_filename": "/tmp/tar.vtk", "reference_landmarks": "none",
+        "target_landmarks": "none", "weight_filename": "none", "weight_tensor_filename": "none",
+        "temp_directory": "/tmp/skm/", "verbosity": [1],
         "sigma_C4": [64, 32, 16], "sigma_C0": [128, 64, 32], "cp_spacing_scaler": [4.0],
         "tensor_weights": [400, 200, 100], "reg_l1": [0, 0, 0], "reg_l21": [0, 0, 0], "reg_l2": [0, 0, 0],
         "reg_rkhs": [0, 0, 0], "reg_rd": [0, 0, 0], "reg_rd_scaling": [1, 1, 1], "reg_pg": [0, 0, 0],
@@ -352,7 +444,17 @@ class RegistrationConfig:
         for k, v in overrides.items():
             if k not in self.values:
                 raise KeyError(f"unknown option {k}")
-            self.values[k] = list(v) if isinstance(v, (list, tuple)) else ([v] if k != "metric" else v)
+            if k in self.STRING_KEYS:
+                self.values[k] = str(v)
+            else:
+                self.values[k] = list(v) if isinstance(v, (list, tuple)) else [v]
+
+    @classmethod
+    def from_json(cls, path):
+        """load_json_config(filename) over the defaults (include/ConfigUtils.h:64-75,
+        src/SKMReg.cxx:528-560): keys of the file replace the defaults."""
+        with open(path) as f:
+            return cls(**json.load(f))
 
     def CURRENT(self, key, scale):
         v = self.values[key]
@@ -363,12 +465,16 @@ class RegistrationConfig:
 
 
 def register_multiscale(reference, target, config=None, weight_image=None, weight_tensor=None, guidance_image=None,
-                        restart_displacement=None):
+                        restart_displacement=None, temp_directory=None, reference_landmarks=None,
+                        target_landmarks=None):
     """The scale-level loop of SKMReg (src/SKMReg.cxx:84-525) through
     skm_register.  reference (moving) and target (fixed) are Images; the
     optional weight image / tensor / guidance image are Images on one common
-    weight grid.  Returns (displacement on the target grid, shape
-    target.data.shape + (D,), list of per-level report dicts)."""
+    weight grid.  With temp_directory, SKMReg's result files are written
+    there and restart_at without restart_displacement resumes from its
+    df-full file.  Landmarks ((n, D) arrays) give the TRE per level.
+    Returns (displacement on the target grid, shape target.data.shape + (D,),
+    list of per-level report dicts)."""
     lib = _capi.load()
     cfg = config or RegistrationConfig()
     dim = target.dim
@@ -417,10 +523,51 @@ def register_multiscale(reference, target, config=None, weight_image=None, weigh
         rd = f32(restart_displacement)
         keep.append(rd)
         d.restart_displacement = fptr(rd)
+    if temp_directory is not None:
+        d.temp_directory = os.fsencode(temp_directory)
+    d.verbosity = int(cfg.CURRENT("verbosity", 0))
+    if reference_landmarks is not None and target_landmarks is not None:
+        rl = np.ascontiguousarray(reference_landmarks, np.float64).reshape(-1, dim)
+        tl = np.ascontiguousarray(target_landmarks, np.float64).reshape(-1, dim)
+        if len(rl) == len(tl) and len(rl) > 0:
+            keep += [rl, tl]
+            d.num_landmarks = len(rl)
+            d.reference_landmarks = rl.ctypes.data_as(_capi.c_double_p)
+            d.target_landmarks = tl.ctypes.data_as(_capi.c_double_p)
     out = np.empty(tar.shape + (dim,), np.float32)
     reps = (_capi.SkmScaleReport * n)()
     check(lib.skm_register(ctypes.byref(d), fptr(out), reps))
     reports = [{"value": r.value, "measure": r.measure, "first_value": r.first_value, "iterations": r.iterations,
                 "cp_size": tuple(r.cp_size[:dim]), "maximum_tilt": r.maximum_tilt, "ms_setup": r.ms_setup,
-                "ms_optimize": r.ms_optimize, "ms_displacement": r.ms_displacement} for r in reps]
+                "ms_optimize": r.ms_optimize, "ms_displacement": r.ms_displacement, "tre": r.tre,
+                "zero_parameters": r.zero_parameters} for r in reps]
     return out, reports
+
+
+def register_files(config):
+    """SKMReg's run on files (src/SKMReg.cxx:55-526): reads reference_filename,
+    target_filename, the optional weight image / weight tensor (or guidance
+    image when tensor_weights > 0) and landmark files named in the config,
+    registers, and writes the result files into temp_directory (created if
+    missing).  Returns what register_multiscale returns."""
+    cfg = config if isinstance(config, RegistrationConfig) else RegistrationConfig(**dict(config))
+    reference = read_image(cfg["reference_filename"])
+    target = read_image(cfg["target_filename"])
+    dim = target.dim
+    weight_image = weight_tensor = guidance = None
+    if cfg["weight_filename"] != "none":
+        weight_image = read_image(cfg["weight_filename"])
+    if cfg["weight_tensor_filename"] != "none":
+        img = read_image(cfg["weight_tensor_filename"])
+        if img.data.ndim > img.dim:
+            weight_tensor = img
+        else:
+            guidance = img
+    rl = tl = None
+    if cfg["reference_landmarks"] != "none" and cfg["target_landmarks"] != "none":
+        rl = read_landmarks(cfg["reference_landmarks"], dim)
+        tl = read_landmarks(cfg["target_landmarks"], dim)
+    os.makedirs(cfg["temp_directory"], exist_ok=True)
+    return register_multiscale(reference, target, cfg, weight_image=weight_image, weight_tensor=weight_tensor,
+                               guidance_image=guidance, temp_directory=cfg["temp_directory"],
+                               reference_landmarks=rl, target_landmarks=tl)
