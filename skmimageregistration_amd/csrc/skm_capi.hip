@@ -121,7 +121,7 @@ struct skm_evaluator {
     float4 *d_cpc = nullptr;
     float *d_cp_state = nullptr;
     float4 *d_cp_pstate = nullptr, *d_smp_pstate = nullptr, *d_vox_pstate = nullptr;
-    float *d_cp_row_lam = nullptr, *d_smp_row_lam = nullptr, *d_vox_row_lam = nullptr;
+    float *d_cp_row_lam = nullptr, *d_smp_row_lam = nullptr, *d_vox_row_lam = nullptr, *d_cp_reach = nullptr;
     int *d_cp_window = nullptr, *d_reg_window = nullptr, *d_lcc_window = nullptr, *d_dense_window = nullptr;
     int *d_offsets = nullptr;
     float4 *d_sg = nullptr, *d_sv = nullptr;
@@ -151,7 +151,8 @@ struct skm_evaluator {
     ~skm_evaluator()
     {
         void *ptrs[] = {d_fixed, d_coeffs, d_weight, d_df, d_cpc, d_cp_state, d_cp_pstate, d_smp_pstate,
-                        d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_window, d_reg_window,
+                        d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_reach, d_cp_window,
+                        d_reg_window,
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_dot_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records};
@@ -370,6 +371,12 @@ skm_status build_static(skm_evaluator *ev)
             cp_row[row] = std::max(cp_row[row], (float)(pstate[(size_t)j * 8 + 7] * (1.0 + 1e-6)));
         }
         SKM_TRY(dev_upload(&ev->d_cp_row_lam, cp_row.data(), cp_row.size()));
+        // each control point's share of the pair reach, pair_reach2(smin, lam, 0) in float
+        const float sminf = (float)std::min(ds.sigma_c4, ds.sigma_c0);
+        std::vector<float> cp_reach((size_t)ncp);
+        for (long long j = 0; j < ncp; ++j)
+            cp_reach[(size_t)j] = sminf * sminf * 0.5f * pstate[(size_t)j * 8 + 7] * 1.0001f;
+        SKM_TRY(dev_upload(&ev->d_cp_reach, cp_reach.data(), cp_reach.size()));
         // lam bound per row of sample cells (my, mz), jitter included, for the
         // gather's reach; once for the evaluator's subsample and once for the
         // dense voxel grid
@@ -674,6 +681,7 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
     }
     pr.smp_pstate = ev->d_smp_pstate;
     pr.cp_row_lam = ev->d_cp_row_lam;
+    pr.cp_reach = ev->d_cp_reach;
     pr.smp_row_lam = ev->d_smp_row_lam;
     pr.cp_window = ev->d_cp_window;
     pr.reg_window = ev->d_reg_window;

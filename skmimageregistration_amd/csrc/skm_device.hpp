@@ -264,10 +264,77 @@ __device__ __forceinline__ void point_state(const KernelConst &kc, const Grid &w
     }
 }
 
-// k(p, q) for v = p - q with the states of both points (Kernel.h:96-131);
-// the weighted modes carry F twice, F^2 = sdet_p * sdet_q / |avg|.
-// Mode 2 works on S = S_p + S_q: with m = S / 2, v^T m^-1 v = 2 v^T adj(S) v /
-// det S and det m = det S / 2^D, so one reciprocal serves both terms.
+// Terms of v = (dx, dy, dz) that stay fixed along a row of control points
+// or samples (dy, dz fixed): 2dy, 2dz, dy^2, dz^2, 2 dy dz, dy^2 + dz^2.
+struct RowTerms {
+    float ty, tz, yy, zz, tyz, yz2;
+};
+
+template <int DIM>
+__device__ __forceinline__ RowTerms row_terms(float dy, float dz)
+{
+    RowTerms r;
+    r.ty = 2.0f * dy;
+    r.yy = dy * dy;
+    if (DIM == 3) {
+        r.tz = 2.0f * dz;
+        r.zz = dz * dz;
+        r.tyz = r.ty * dz;
+        r.yz2 = fmaf(dz, dz, r.yy);
+    } else {
+        r.tz = r.zz = r.tyz = 0.0f;
+        r.yz2 = r.yy;
+    }
+    return r;
+}
+
+// k(p, q) of the weighted modes for v = (dx, dy, dz) given as dx and the row
+// terms (Kernel.h:96-131); both modes carry F twice, F^2 = sdet_p * sdet_q /
+// |avg|.  Mode 2 works on S = S_p + S_q: with m = S / 2, v^T m^-1 v =
+// 2 v^T adj(S) v / det S and det m = det S / 2^D, so one reciprocal serves
+// both terms; v^T adj(S) v = dx (c00 dx + 2 c01 dy + 2 c02 dz) + c11 dy^2 +
+// c22 dz^2 + 2 c12 dy dz.  k = 0 unless r < smin: lanes outside return before
+// the reciprocal (the bound is loosened by 1e-4, the exact zero comes from
+// the polynomial).
+template <int DIM, int MODE>
+__device__ __forceinline__ float weighted_k_row(const KernelConst &kc, float dx, const RowTerms &r, const PState &a,
+                                                const PState &b)
+{
+    const float s2 = kc.smin * kc.smin * 1.0001f;
+    if (MODE == 1) {
+        const float r2 = fmaf(dx, dx, r.yz2);
+        const float sum = a.sig + b.sig;
+        if (!(2.0f * r2 < s2 * sum)) return 0.0f;
+        const float inv = __builtin_amdgcn_rcpf(sum);
+        const float F2 = 2.0f * a.sdet * b.sdet * fabsf(inv);
+        const float rs = fast_sqrt(2.0f * inv * r2);
+        return F2 * wendland_c4c0(rs, kc);
+    } else {
+        float quad, det;
+        if (DIM == 2) {
+            const float s00 = a.up[0] + b.up[0], s01 = a.up[1] + b.up[1], s11 = a.up[2] + b.up[2];
+            det = fmaf(s00, s11, -s01 * s01);
+            // dx (s11 dx - 2 s01 dy) + s00 dy^2
+            quad = fmaf(dx, fmaf(s11, dx, -s01 * r.ty), s00 * r.yy);
+        } else {
+            const float s00 = a.up[0] + b.up[0], s01 = a.up[1] + b.up[1], s02 = a.up[2] + b.up[2];
+            const float s11 = a.up[3] + b.up[3], s12 = a.up[4] + b.up[4], s22 = a.up[5] + b.up[5];
+            const float c00 = fmaf(s11, s22, -s12 * s12), c11 = fmaf(s00, s22, -s02 * s02);
+            const float c22 = fmaf(s00, s11, -s01 * s01);
+            const float c01 = fmaf(s02, s12, -s01 * s22), c02 = fmaf(s01, s12, -s02 * s11);
+            const float c12 = fmaf(s01, s02, -s00 * s12);
+            det = fmaf(s00, c00, fmaf(s01, c01, s02 * c02));
+            quad = fmaf(dx, fmaf(c00, dx, fmaf(c01, r.ty, c02 * r.tz)), fmaf(c11, r.yy, fmaf(c22, r.zz, c12 * r.tyz)));
+        }
+        if (!(2.0f * quad < s2 * det)) return 0.0f;
+        const float inv = __builtin_amdgcn_rcpf(det);
+        const float F2 = (DIM == 3 ? 8.0f : 4.0f) * a.sdet * b.sdet * inv;
+        const float rs = fast_sqrt(fmaxf(0.0f, 2.0f * quad * inv));
+        return F2 * wendland_c4c0(rs, kc);
+    }
+}
+
+// k(p, q) for v = p - q with the states of both points (Kernel.h:96-131).
 template <int DIM, int MODE>
 __device__ __forceinline__ float kernel_pair(const KernelConst &kc, const float *v, const PState &a, const PState &b)
 {
@@ -275,36 +342,8 @@ __device__ __forceinline__ float kernel_pair(const KernelConst &kc, const float 
         float r2 = v[0] * v[0] + v[1] * v[1];
         if (DIM == 3) r2 = fmaf(v[2], v[2], r2);
         return stationary_k<MODE>(r2, kc.inv_s4, kc.inv_s0);
-    } else if (MODE == 1) {
-        float r2 = v[0] * v[0] + v[1] * v[1];
-        if (DIM == 3) r2 = fmaf(v[2], v[2], r2);
-        float sum = a.sig + b.sig;
-        float inv = __builtin_amdgcn_rcpf(sum);
-        float F2 = 2.0f * a.sdet * b.sdet * fabsf(inv);
-        float rs = fast_sqrt(2.0f * inv * r2);
-        return F2 * wendland_c4c0(rs, kc);
     } else {
-        float quad, det;
-        if (DIM == 2) {
-            const float s00 = a.up[0] + b.up[0], s01 = a.up[1] + b.up[1], s11 = a.up[2] + b.up[2];
-            det = fmaf(s00, s11, -s01 * s01);
-            quad = fmaf(v[0] * v[0], s11, fmaf(v[1] * v[1], s00, -2.0f * s01 * v[0] * v[1]));
-        } else {
-            const float s00 = a.up[0] + b.up[0], s01 = a.up[1] + b.up[1], s02 = a.up[2] + b.up[2];
-            const float s11 = a.up[3] + b.up[3], s12 = a.up[4] + b.up[4], s22 = a.up[5] + b.up[5];
-            const float c00 = fmaf(s11, s22, -s12 * s12), c01 = fmaf(s02, s12, -s01 * s22);
-            const float c02 = fmaf(s01, s12, -s02 * s11), c11 = fmaf(s00, s22, -s02 * s02);
-            const float c12 = fmaf(s01, s02, -s00 * s12), c22 = fmaf(s00, s11, -s01 * s01);
-            det = fmaf(s00, c00, fmaf(s01, c01, s02 * c02));
-            const float t0 = fmaf(c00, v[0], fmaf(c01, v[1], c02 * v[2]));
-            const float t1 = fmaf(c01, v[0], fmaf(c11, v[1], c12 * v[2]));
-            const float t2 = fmaf(c02, v[0], fmaf(c12, v[1], c22 * v[2]));
-            quad = fmaf(t0, v[0], fmaf(t1, v[1], t2 * v[2]));
-        }
-        const float inv = __builtin_amdgcn_rcpf(det);
-        const float F2 = (DIM == 3 ? 8.0f : 4.0f) * a.sdet * b.sdet * inv;
-        const float rs = fast_sqrt(fmaxf(0.0f, 2.0f * quad * inv));
-        return F2 * wendland_c4c0(rs, kc);
+        return weighted_k_row<DIM, MODE>(kc, v[0], row_terms<DIM>(v[1], DIM == 3 ? v[2] : 0.0f), a, b);
     }
 }
 

@@ -39,6 +39,7 @@ struct Problem {
     // weighted modes (1, 2): packed point states, two float4 per point (pstate_pack)
     const float4 *cp_pstate;   // per control point, static
     float4 *smp_pstate;        // per sample of the current evaluation, written by the value pass
+    const float *cp_reach;     // per control point: pair_reach2(smin, lam, 0), its share of the pair reach
     const float *cp_row_lam;   // largest lam per row of control points (cy + n[1] cz)
     const float *smp_row_lam;  // largest lam per row of sample cells (my + nsub[1] mz)
     float lam_cp_max;          // largest lam over the control points

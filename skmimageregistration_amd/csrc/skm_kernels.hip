@@ -197,6 +197,9 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
         ulo[d] = __builtin_amdgcn_readfirstlane(wave_min_i(lo[d]));
         uhi[d] = __builtin_amdgcn_readfirstlane(wave_max_i(hi[d]));
     }
+    // pair reach: |v|^2 < smin^2 (lam_x + lam_c) / 2 (x 1.0001), as the sum of
+    // the sample's share and the control point's pre-scaled one (cp_reach)
+    const float qx = pair_reach2(pr.kc.smin, lam_x, 0.0f);
     for (int cz = ulo[2]; cz <= uhi[2]; ++cz) {
         const float dz = DIM == 3 ? x[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
         const bool zin = DIM == 2 || (cz >= lo[2] && cz <= hi[2]);
@@ -214,18 +217,21 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
             const int uxa = __builtin_amdgcn_readfirstlane(wave_min_i(xa));
             const int uxb = __builtin_amdgcn_readfirstlane(wave_max_i(xb));
             const long long row = (long long)pr.cp.n[0] * (cy + (long long)pr.cp.n[1] * cz);
-            for (int cx = uxa; cx <= uxb; ++cx) {
-                PState cst;
-                const float lam_c = cp_state_load<DIM, MODE>(pr, row + cx, cst);
-                const float4 c = pr.cpc[row + cx];
-                const float dx = x[0] - (pr.cp.o[0] + pr.cp.h[0] * cx);
-                // this pair's own reach (both lams) is tighter than the row's
-                if (cx >= xa && cx <= xb && dx * dx + dy * dy + dz * dz < pair_reach2(pr.kc.smin, lam_x, lam_c)) {
-                    const float v[3] = {dx, dy, dz};
-                    const float wk = c.w * kernel_pair<DIM, MODE>(pr.kc, v, st, cst);
-                    u[0] = fmaf(wk, c.x, u[0]);
-                    u[1] = fmaf(wk, c.y, u[1]);
-                    if (DIM == 3) u[2] = fmaf(wk, c.z, u[2]);
+            const RowTerms rt = row_terms<DIM>(dy, dz);
+            float fcx = (float)uxa;
+            for (int cx = uxa; cx <= uxb; ++cx, fcx += 1.0f) {
+                if (cx >= xa && cx <= xb) {
+                    const float dx = x[0] - fmaf(pr.cp.h[0], fcx, pr.cp.o[0]);
+                    // this pair's own reach (both lams) is tighter than the row's
+                    if (fmaf(dx, dx, rt.yz2) < qx + pr.cp_reach[row + cx]) {
+                        PState cst;
+                        cp_state_load<DIM, MODE>(pr, row + cx, cst);
+                        const float4 c = pr.cpc[row + cx];
+                        const float wk = c.w * weighted_k_row<DIM, MODE>(pr.kc, dx, rt, st, cst);
+                        u[0] = fmaf(wk, c.x, u[0]);
+                        u[1] = fmaf(wk, c.y, u[1]);
+                        if (DIM == 3) u[2] = fmaf(wk, c.z, u[2]);
+                    }
                 }
             }
         }
