@@ -127,6 +127,7 @@ struct skm_evaluator {
     float4 *d_sg = nullptr, *d_sv = nullptr;
     float *d_sb = nullptr;
     double *d_partials = nullptr, *d_reg_partials = nullptr, *d_hreg_partials = nullptr, *d_dot_partials = nullptr;
+    double *d_gather_partials = nullptr;
     Scalars *d_sc = nullptr;
     float *d_deriv = nullptr, *d_cp_in = nullptr, *d_cpw_in = nullptr, *d_dst = nullptr;
     float4 *d_dense = nullptr, *d_dense_q = nullptr, *d_ccoef = nullptr, *d_gamma = nullptr;
@@ -154,7 +155,7 @@ struct skm_evaluator {
                         d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_reach, d_cp_window,
                         d_reg_window,
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
-                        d_hreg_partials, d_dot_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
+                        d_hreg_partials, d_dot_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
@@ -678,7 +679,11 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
     pr.cp_pstate = ev->d_cp_pstate;
     if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
         if ((st = dev_alloc(&ev->d_smp_pstate, (size_t)pr.n_sub * 2)) != SKM_SUCCESS) return cleanup(st);
+        // sized for the dense voxel grid, which the exact LCC gather walks
+        const long long np = gather_partials_size(pr, pr.fixed.n[dim - 1]);
+        if ((st = dev_alloc(&ev->d_gather_partials, (size_t)np)) != SKM_SUCCESS) return cleanup(st);
     }
+    pr.gather_partials = ev->d_gather_partials;
     pr.smp_pstate = ev->d_smp_pstate;
     pr.cp_row_lam = ev->d_cp_row_lam;
     pr.cp_reach = ev->d_cp_reach;

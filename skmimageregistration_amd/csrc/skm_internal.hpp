@@ -42,6 +42,7 @@ struct Problem {
     const float *cp_reach;     // per control point: pair_reach2(smin, lam, 0), its share of the pair reach
     const float *cp_row_lam;   // largest lam per row of control points (cy + n[1] cz)
     const float *smp_row_lam;  // largest lam per row of sample cells (my + nsub[1] mz)
+    double *gather_partials;   // weighted-mode gather: FP64 sums per (slab, control-point tile)
     float lam_cp_max;          // largest lam over the control points
     float lam_w_max;           // largest lam over the points a sample can look up (weight image, outside)
 };
@@ -68,6 +69,8 @@ void launch_reduce(const Problem &pr, int measure, const double *partials, int b
 void launch_prepare(const Problem &pr, int measure, const float4 *sg, const float *sb, const Scalars *sc,
                     float4 *sv, hipStream_t s);
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s);
+// doubles of Problem::gather_partials for a sample grid with `planes` cells along the last axis
+long long gather_partials_size(const Problem &pr, long long planes);
 void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, int with_derivative,
                          float *deriv_soa, double *partials, Scalars *sc, hipStream_t s);
 unsigned reg_blocks(long long n_cp);  // workgroups (and 4-double partial slots) of launch_regularizers

@@ -506,40 +506,64 @@ template <int DIM> struct CpTile;
 template <> struct CpTile<2> { static constexpr int X = 2, Y = 2, Z = 1; };
 template <> struct CpTile<3> { static constexpr int X = 4, Y = 1, Z = 1; };
 
-// Workgroup -> tile order that gives each XCD a contiguous range of tiles:
-// workgroups are dispatched to the 8 XCDs round-robin, so neighbouring tiles
-// (whose sample windows overlap) share one XCD's L2.
-__device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned n)
+// Sample planes (last axis) per slab of the weighted-mode gather: 3D windows
+// are walked slab by slab so that the workgroups in flight share one slab of
+// samples (a few MB, resident in every XCD's L2); 2D windows in one slab.
+template <int DIM>
+__host__ __device__ __forceinline__ int gather_slab(const Problem &pr)
 {
-    constexpr unsigned kXcds = 8;
-    const unsigned xcd = b % kXcds, slot = b / kXcds, q = n / kXcds, rm = n % kXcds;
-    return xcd * q + min(xcd, rm) + slot;
+    return DIM == 3 ? 8 : max(pr.fixed.n[1], 1);
 }
 
-// Derivative gather of the weighted modes: one workgroup per tile of
-// control points.  The workgroup walks the union of the tile's reference
-// windows once, 8 lanes along x and 32 rows at a time; each lane loads a
-// sample (v, packed state) once and adds its pair with every tile control
-// point whose window and reach (both points' lam) hold the sample.  The tile
-// constants are wave-uniform (scalar registers); the next sample's loads are
-// issued before the current sample's pairs.  Same sums per control point as
+// Union of the reference windows of one control-point tile along the last
+// axis (sample cells); empty when lo > hi.
+template <int DIM>
+__device__ __forceinline__ void tile_plane_range(const Problem &pr, int t0, int t1, int t2, int &lo, int &hi)
+{
+    constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z, T = TX * TY * TZ;
+    lo = INT_MAX;
+    hi = INT_MIN;
+#pragma unroll
+    for (int c = 0; c < T; ++c) {
+        const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
+        if (jc[0] < pr.cp.n[0] && jc[1] < pr.cp.n[1] && (DIM == 2 || jc[2] < pr.cp.n[2])) {
+            const long long j = jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]);
+            lo = min(lo, pr.cp_window[j * 6 + DIM - 1]);
+            hi = max(hi, pr.cp_window[j * 6 + 3 + DIM - 1]);
+        }
+    }
+}
+
+// Derivative gather of the weighted modes: one workgroup per (slab of sample
+// planes, tile of control points), slab-major.  The workgroup walks the rows
+// of the union of the tile's reference windows inside its slab, 8 lanes
+// along x and 32 rows at a time; each lane loads a sample (v, packed state)
+// once and adds its pair with every tile control point whose window and
+// reach (both points' lam) hold the sample.  The tile constants are
+// wave-uniform (scalar registers); the next sample's loads are issued before
+// the current sample's pairs.  Each workgroup writes its FP64 sums;
+// k_gather_finish adds the slabs in order.  Same sums per control point as
 // k_gather, in a fixed order.
 template <int DIM, int MODE>
-__global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const float4 *__restrict__ sv,
-                                                          float *__restrict__ deriv)
+__global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const float4 *__restrict__ sv, int n_tiles)
 {
     constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z, T = TX * TY * TZ;
     __shared__ double lds[T * 3 * 4];
     const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
-    const unsigned tile = xcd_contiguous(blockIdx.x, gridDim.x);
-    const int t0 = (int)(tile % ntx), t1 = (int)((tile / ntx) % nty), t2 = (int)(tile / (ntx * nty));
+    const int tile = (int)(blockIdx.x % (unsigned)n_tiles), slab_id = (int)(blockIdx.x / (unsigned)n_tiles);
+    const int t0 = tile % ntx, t1 = (tile / ntx) % nty, t2 = tile / (ntx * nty);
+    const int slab = gather_slab<DIM>(pr);
+    int plo, phi;
+    tile_plane_range<DIM>(pr, t0, t1, t2, plo, phi);
+    // slabs outside the tile's windows write nothing (k_gather_finish skips them)
+    if (plo > phi || slab_id < plo / slab || slab_id > phi / slab) return;
     // tile control points (uniform)
     long long jj[T];
     bool ok[T];
     float px[T], py[T], pz[T], pc0[T], lam[T];
     int lo[T][3], hi[T][3];
     PState st[T];
-    int ulo[3] = {INT_MAX, INT_MAX, 0}, uhi[3] = {INT_MIN, INT_MIN, 0};
+    int ulo[3] = {INT_MAX, INT_MAX, DIM == 3 ? INT_MAX : 0}, uhi[3] = {INT_MIN, INT_MIN, DIM == 3 ? INT_MIN : 0};
 #pragma unroll
     for (int c = 0; c < T; ++c) {
         const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
@@ -568,6 +592,9 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
             }
         }
     }
+    // this slab's planes of the union
+    ulo[DIM - 1] = max(ulo[DIM - 1], slab_id * slab);
+    uhi[DIM - 1] = min(uhi[DIM - 1], slab_id * slab + slab - 1);
     const int by = uhi[1] - ulo[1] + 1, bz = uhi[2] - ulo[2] + 1;
     // (a window can be empty along an axis: then there is nothing to gather)
     const int rows = (uhi[0] >= ulo[0] && by > 0 && bz > 0) ? by * bz : 0;
@@ -677,12 +704,38 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
         if (d < DIM) {
             const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
             const bool inside = jc[0] < pr.cp.n[0] && jc[1] < pr.cp.n[1] && (DIM == 2 || jc[2] < pr.cp.n[2]);
-            if (inside) {
-                const long long j = jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]);
-                deriv[j + d * pr.n_cp] = (float)((double)pr.cpc[j].w * sum);
-            }
+            if (inside) pr.gather_partials[((long long)slab_id * n_tiles + tile) * (T * 3) + c * 3 + d] = sum;
         }
     }
+}
+
+// deriv_j = w_j * the sum over the slabs of j's tile, in slab order.
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_gather_finish(Problem pr, int n_tiles, float *__restrict__ deriv)
+{
+    constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z, T = TX * TY * TZ;
+    const long long j = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (j >= pr.n_cp) return;
+    const int jc[3] = {(int)(j % pr.cp.n[0]), (int)((j / pr.cp.n[0]) % pr.cp.n[1]),
+                       (int)(j / ((long long)pr.cp.n[0] * pr.cp.n[1]))};
+    const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
+    const int t0 = jc[0] / TX, t1 = jc[1] / TY, t2 = jc[2] / TZ;
+    const long long tile = t0 + (long long)ntx * (t1 + (long long)nty * t2);
+    const int c = jc[0] % TX + TX * (jc[1] % TY + TY * (jc[2] % TZ));
+    const int slab = gather_slab<DIM>(pr);
+    int plo, phi;
+    tile_plane_range<DIM>(pr, t0, t1, t2, plo, phi);
+    double sum[3] = {0.0, 0.0, 0.0};
+    if (plo <= phi) {
+        for (int s = plo / slab; s <= phi / slab; ++s) {
+            const double *p = pr.gather_partials + ((long long)s * n_tiles + tile) * (T * 3) + c * 3;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) sum[d] += p[d];
+        }
+    }
+    const double w = pr.cpc[j].w;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) deriv[j + d * pr.n_cp] = (float)(w * sum[d]);
 }
 
 // ------------------------------------------------------------ regularizers
@@ -968,12 +1021,31 @@ static void gather_impl(const Problem &pr, const float4 *sv, float *deriv, hipSt
         const long long tiles = (long long)((pr.cp.n[0] + CpTile<DIM>::X - 1) / CpTile<DIM>::X) *
                                 ((pr.cp.n[1] + CpTile<DIM>::Y - 1) / CpTile<DIM>::Y) *
                                 (DIM == 3 ? (pr.cp.n[2] + CpTile<DIM>::Z - 1) / CpTile<DIM>::Z : 1);
-        k_gather_tile<DIM, MODE><<<(unsigned)tiles, kThreads, 0, s>>>(pr, sv, deriv);
+        const int slab = gather_slab<DIM>(pr);
+        const long long slabs = (pr.nsub[DIM - 1] + slab - 1) / slab;
+        k_gather_tile<DIM, MODE><<<(unsigned)(tiles * slabs), kThreads, 0, s>>>(pr, sv, (int)tiles);
+        k_gather_finish<DIM><<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr, (int)tiles, deriv);
     } else if (pr.sub == 1) {
         k_gather<DIM, MODE, true><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
     } else {
         k_gather<DIM, MODE, false><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
     }
+}
+
+template <int DIM>
+static long long partials_size_impl(const Problem &pr, long long planes)
+{
+    constexpr int T = CpTile<DIM>::X * CpTile<DIM>::Y * CpTile<DIM>::Z;
+    const long long tiles = (long long)((pr.cp.n[0] + CpTile<DIM>::X - 1) / CpTile<DIM>::X) *
+                            ((pr.cp.n[1] + CpTile<DIM>::Y - 1) / CpTile<DIM>::Y) *
+                            (DIM == 3 ? (pr.cp.n[2] + CpTile<DIM>::Z - 1) / CpTile<DIM>::Z : 1);
+    const int slab = gather_slab<DIM>(pr);
+    return tiles * ((planes + slab - 1) / slab) * T * 3;
+}
+
+long long gather_partials_size(const Problem &pr, long long planes)
+{
+    return pr.dim == 2 ? partials_size_impl<2>(pr, planes) : partials_size_impl<3>(pr, planes);
 }
 
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s)
