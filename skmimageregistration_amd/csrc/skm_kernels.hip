@@ -220,18 +220,18 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
             const RowTerms rt = row_terms<DIM>(dy, dz);
             float fcx = (float)uxa;
             for (int cx = uxa; cx <= uxb; ++cx, fcx += 1.0f) {
-                if (cx >= xa && cx <= xb) {
-                    const float dx = x[0] - fmaf(pr.cp.h[0], fcx, pr.cp.o[0]);
-                    // this pair's own reach (both lams) is tighter than the row's
-                    if (fmaf(dx, dx, rt.yz2) < qx + pr.cp_reach[row + cx]) {
-                        PState cst;
-                        cp_state_load<DIM, MODE>(pr, row + cx, cst);
-                        const float4 c = pr.cpc[row + cx];
-                        const float wk = c.w * weighted_k_row<DIM, MODE>(pr.kc, dx, rt, st, cst);
-                        u[0] = fmaf(wk, c.x, u[0]);
-                        u[1] = fmaf(wk, c.y, u[1]);
-                        if (DIM == 3) u[2] = fmaf(wk, c.z, u[2]);
-                    }
+                const float qc = pr.cp_reach[row + cx];
+                const float dx = x[0] - fmaf(pr.cp.h[0], fcx, pr.cp.o[0]);
+                // window and this pair's own reach (both lams, tighter than the
+                // row's) as one predicate: one branch per control point
+                if ((cx >= xa) & (cx <= xb) & (fmaf(dx, dx, rt.yz2) < qx + qc)) {
+                    PState cst;
+                    cp_state_load<DIM, MODE>(pr, row + cx, cst);
+                    const float4 c = pr.cpc[row + cx];
+                    const float wk = c.w * weighted_k_row<DIM, MODE>(pr.kc, dx, rt, st, cst);
+                    u[0] = fmaf(wk, c.x, u[0]);
+                    u[1] = fmaf(wk, c.y, u[1]);
+                    if (DIM == 3) u[2] = fmaf(wk, c.z, u[2]);
                 }
             }
         }
@@ -673,7 +673,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
                 const float v[3] = {px[c] - xs[0], py[c] - xs[1], DIM == 3 ? pz[c] - xs[2] : 0.0f};
                 const float d2 = v[0] * v[0] + v[1] * v[1] + (DIM == 3 ? v[2] * v[2] : 0.0f);
                 // the pair's own reach: both points' lam
-                if (mx >= xa[c] && mx <= xb[c] && d2 < smin2h * (lam[c] + cb.w)) {
+                if ((mx >= xa[c]) & (mx <= xb[c]) & (d2 < smin2h * (lam[c] + cb.w))) {
                     const float k = kernel_pair<DIM, MODE>(pr.kc, v, st[c], sst);
                     acc[c][0] = fmaf(k, cv.x, acc[c][0]);
                     acc[c][1] = fmaf(k, cv.y, acc[c][1]);
