@@ -11,6 +11,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "libskm_oracle.so")
+# the same source built with OpenMP: the value and derivative loops run in
+# parallel (bench.py's CPU baseline); results equal the serial build's
+LIB_OMP = os.path.join(HERE, "_build", "libskm_oracle_omp.so")
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int_p = ctypes.POINTER(ctypes.c_int)
@@ -45,66 +48,80 @@ class Result(ctypes.Structure):
 
 
 _lib = None
+_lib_omp = None
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def load():
-    global _lib
+def load(parallel=False):
+    """The serial oracle; parallel=True the OpenMP build of the same source."""
+    global _lib, _lib_omp
+    if parallel:
+        if _lib_omp is None:
+            if not os.path.exists(LIB_OMP):
+                build()
+            _lib_omp = _declare(ctypes.CDLL(LIB_OMP))
+        return _lib_omp
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        lib = ctypes.CDLL(LIB)
-        lib.skmo_wendland_c4.restype = ctypes.c_double
-        lib.skmo_wendland_c4.argtypes = [ctypes.c_double]
-        lib.skmo_wendland_c0.restype = ctypes.c_double
-        lib.skmo_wendland_c0.argtypes = [ctypes.c_double]
-        lib.skmo_kernel_eval.restype = ctypes.c_double
-        lib.skmo_kernel_eval.argtypes = [ctypes.POINTER(Problem), c_double_p, c_double_p, ctypes.c_double, c_double_p]
-        lib.skmo_region_support.restype = ctypes.c_double
-        lib.skmo_region_support.argtypes = [ctypes.POINTER(Problem), c_double_p]
-        lib.skmo_max_eigenvalue.restype = ctypes.c_double
-        lib.skmo_max_eigenvalue.argtypes = [ctypes.c_int, c_double_p]
-        lib.skmo_determinant.restype = ctypes.c_double
-        lib.skmo_determinant.argtypes = [ctypes.c_int, c_double_p]
-        lib.skmo_bspline_coefficients.restype = None
-        lib.skmo_bspline_coefficients.argtypes = [ctypes.c_int, c_int_p, c_double_p, c_double_p]
-        lib.skmo_bspline_sample.restype = ctypes.c_int
-        lib.skmo_bspline_sample.argtypes = [ctypes.c_int, ctypes.POINTER(Grid), c_double_p, c_double_p,
-                                            c_double_p, c_double_p]
-        lib.skmo_pcg32_next.restype = ctypes.c_uint32
-        lib.skmo_pcg32_next.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
-        lib.skmo_cube_offsets.restype = None
-        lib.skmo_cube_offsets.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_longlong,
-                                          ctypes.c_int, ctypes.c_int, c_int_p]
-        lib.skmo_displacement.restype = None
-        lib.skmo_displacement.argtypes = [ctypes.POINTER(Problem), c_double_p]
-        lib.skmo_evaluate.restype = ctypes.c_int
-        lib.skmo_evaluate.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, c_double_p, c_double_p,
-                                      ctypes.POINTER(Result)]
-        lib.skmo_regularizer.restype = None
-        lib.skmo_regularizer.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                         c_double_p, c_double_p]
-        lib.skmo_host_regularizers.restype = ctypes.c_double
-        lib.skmo_host_regularizers.argtypes = [ctypes.c_int, ctypes.c_longlong, c_double_p, ctypes.c_double,
-                                               ctypes.c_double, ctypes.c_double, ctypes.c_int, c_double_p,
-                                               c_double_p, c_double_p, c_double_p]
-        lib.skmo_standard_rate.restype = ctypes.c_double
-        lib.skmo_standard_rate.argtypes = [ctypes.c_double] * 3 + [ctypes.c_longlong]
-        lib.skmo_adaptive_rate.restype = ctypes.c_double
-        lib.skmo_adaptive_rate.argtypes = [ctypes.c_double] * 4
-        lib.skmo_adaptive_time.restype = ctypes.c_double
-        lib.skmo_adaptive_time.argtypes = [ctypes.c_double] * 5
-        lib.skmo_gd_update.restype = None
-        lib.skmo_gd_update.argtypes = [ctypes.c_longlong, c_double_p, c_double_p, ctypes.c_double, ctypes.c_int,
-                                       c_double_p]
-        lib.skmo_control_point_grid.restype = None
-        lib.skmo_control_point_grid.argtypes = [ctypes.c_int, ctypes.POINTER(Grid), c_double_p, ctypes.POINTER(Grid)]
-        _lib = lib
+        _lib = _declare(ctypes.CDLL(LIB))
     return _lib
+
+
+def _declare(lib):
+    """Declares the signatures of the oracle's entry points."""
+    lib.skmo_wendland_c4.restype = ctypes.c_double
+    lib.skmo_wendland_c4.argtypes = [ctypes.c_double]
+    lib.skmo_wendland_c0.restype = ctypes.c_double
+    lib.skmo_wendland_c0.argtypes = [ctypes.c_double]
+    lib.skmo_kernel_eval.restype = ctypes.c_double
+    lib.skmo_kernel_eval.argtypes = [ctypes.POINTER(Problem), c_double_p, c_double_p, ctypes.c_double, c_double_p]
+    lib.skmo_region_support.restype = ctypes.c_double
+    lib.skmo_region_support.argtypes = [ctypes.POINTER(Problem), c_double_p]
+    lib.skmo_max_eigenvalue.restype = ctypes.c_double
+    lib.skmo_max_eigenvalue.argtypes = [ctypes.c_int, c_double_p]
+    lib.skmo_determinant.restype = ctypes.c_double
+    lib.skmo_determinant.argtypes = [ctypes.c_int, c_double_p]
+    lib.skmo_bspline_coefficients.restype = None
+    lib.skmo_bspline_coefficients.argtypes = [ctypes.c_int, c_int_p, c_double_p, c_double_p]
+    lib.skmo_bspline_sample.restype = ctypes.c_int
+    lib.skmo_bspline_sample.argtypes = [ctypes.c_int, ctypes.POINTER(Grid), c_double_p, c_double_p,
+                                        c_double_p, c_double_p]
+    lib.skmo_pcg32_next.restype = ctypes.c_uint32
+    lib.skmo_pcg32_next.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
+    lib.skmo_cube_offsets.restype = None
+    lib.skmo_cube_offsets.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_longlong,
+                                      ctypes.c_int, ctypes.c_int, c_int_p]
+    lib.skmo_displacement.restype = None
+    lib.skmo_displacement.argtypes = [ctypes.POINTER(Problem), c_double_p]
+    lib.skmo_evaluate.restype = ctypes.c_int
+    lib.skmo_evaluate.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, c_double_p, c_double_p,
+                                  ctypes.POINTER(Result)]
+    lib.skmo_regularizer.restype = None
+    lib.skmo_regularizer.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     c_double_p, c_double_p]
+    lib.skmo_host_regularizers.restype = ctypes.c_double
+    lib.skmo_host_regularizers.argtypes = [ctypes.c_int, ctypes.c_longlong, c_double_p, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, c_double_p,
+                                           c_double_p, c_double_p, c_double_p]
+    lib.skmo_standard_rate.restype = ctypes.c_double
+    lib.skmo_standard_rate.argtypes = [ctypes.c_double] * 3 + [ctypes.c_longlong]
+    lib.skmo_adaptive_rate.restype = ctypes.c_double
+    lib.skmo_adaptive_rate.argtypes = [ctypes.c_double] * 4
+    lib.skmo_adaptive_time.restype = ctypes.c_double
+    lib.skmo_adaptive_time.argtypes = [ctypes.c_double] * 5
+    lib.skmo_gd_update.restype = None
+    lib.skmo_gd_update.argtypes = [ctypes.c_longlong, c_double_p, c_double_p, ctypes.c_double, ctypes.c_int,
+                                   c_double_p]
+    lib.skmo_set_threads.restype = ctypes.c_int
+    lib.skmo_set_threads.argtypes = [ctypes.c_int]
+    lib.skmo_control_point_grid.restype = None
+    lib.skmo_control_point_grid.argtypes = [ctypes.c_int, ctypes.POINTER(Grid), c_double_p, ctypes.POINTER(Grid)]
+    return lib
 
 
 def dp(a):
@@ -151,8 +168,8 @@ class OracleProblem:
 
     def __init__(self, fixed, moving, cp_grid, cp_aos, cpw=None, sigma_c0=1.0, sigma_c4=1.0, kernel_mode=0,
                  max_weight=1.0, max_eig_cov=None, weight=None, df=None, subsample=1, subsample_neighborhood=1,
-                 offsets=None, exact=False, fixed_geom=None, moving_geom=None):
-        lib = load()
+                 offsets=None, exact=False, fixed_geom=None, moving_geom=None, parallel=False):
+        lib = load(parallel)
         self.lib = lib
         fixed = np.asarray(fixed)
         dim = fixed.ndim

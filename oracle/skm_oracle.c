@@ -14,6 +14,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------ */
 /* small helpers                                                       */
@@ -778,7 +781,9 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
     int *inside = (int *)calloc((size_t)nsub, sizeof(int));
     if (!a || !b || !g || !xs || !inside) return -1;
 
-    /* value pass: one iteration per subsampled sample (e.g. GpuEvaluatorMSE.cu:29-95) */
+    /* value pass: one iteration per subsampled sample (e.g. GpuEvaluatorMSE.cu:29-95);
+       iterations are independent (the OpenMP build runs them in parallel) */
+#pragma omp parallel for schedule(dynamic, 256)
     for (long long s = 0; s < nsub; ++s) {
         int m[3] = {0, 0, 0}, pos[3] = {0, 0, 0};
         long long r = s;
@@ -834,6 +839,8 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
     if (with_derivative) {
         double mean_f = sf / count, mean_m = sm / count;
         for (long long jj = 0; jj < ncp * n; ++jj) deriv_soa[jj] = 0.0;
+        /* one control point per iteration, independent (parallel in the OpenMP build) */
+#pragma omp parallel for schedule(dynamic, 4)
         for (long long jj = 0; jj < ncp; ++jj) {
             int j[3] = {0, 0, 0};
             long long r = jj;
@@ -1098,4 +1105,16 @@ void skmo_control_point_grid(int dim, const skmo_grid *image, const double *sigm
         double cp_upper = image->origin[d] + sigma[d] * (cp->size[d] - 1);
         cp->origin[d] = image->origin[d] + fabs(upper - cp_upper) / 2.0;
     }
+}
+
+/* Threads of the OpenMP build (1 in the serial build); n > 0 sets them. */
+int skmo_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }

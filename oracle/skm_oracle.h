@@ -105,6 +105,9 @@ void skmo_gd_update(long long n, const double *prev, const double *grad, double 
 /* a14 control-point grid */
 void skmo_control_point_grid(int dim, const skmo_grid *image, const double *sigma, skmo_grid *cp);
 
+/* Threads of the OpenMP build of this file (1 in the serial build); n > 0 sets them. */
+int skmo_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

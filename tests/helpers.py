@@ -105,7 +105,7 @@ class Case:
         return Image(self.df, self.origin, self.spacing, dim=self.dim)
 
     # ---------------------------------------------------------------- oracle
-    def oracle(self, offsets=None, cp=None):
+    def oracle(self, offsets=None, cp=None, parallel=False):
         weight = None
         if self.mode != 0:
             weight = (tuple(reversed(self.shape)), self.origin, self.spacing, self.weight)
@@ -117,7 +117,7 @@ class Case:
             self.cp if cp is None else cp, self.cpw, sigma_c0=self.sigma_c0, sigma_c4=self.sigma_c4,
             kernel_mode=self.mode, max_weight=self.max_weight, max_eig_cov=self.max_eig, weight=weight, df=df,
             subsample=self.sub, subsample_neighborhood=self.subn, offsets=offsets, exact=self.exact,
-            fixed_geom=(self.origin, self.spacing), moving_geom=(self.origin, self.spacing))
+            fixed_geom=(self.origin, self.spacing), moving_geom=(self.origin, self.spacing), parallel=parallel)
 
     def n_cells(self):
         return int(np.prod([s // self.sub for s in self.shape]))

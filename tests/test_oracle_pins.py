@@ -508,3 +508,19 @@ def test_gd_update_orthant_projection_and_quadratic_bowl():
         rate = L.skmo_standard_rate(1.0, 1.0, 1.0, k)
         L.skmo_gd_update(3, orc.dp(th.copy()), orc.dp(g), rate, 0, orc.dp(th))
     np.testing.assert_allclose(th, th0 / 201.0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("mode,measure", [(0, 1), (2, 1), (1, 0), (2, 0)])
+def test_openmp_build_equals_serial_build(mode, measure):
+    """The OpenMP build of the oracle (bench.py's CPU baseline) runs the
+    per-sample and per-control-point loops in parallel with the reductions
+    left serial, so its results equal the serial build's exactly."""
+    case = Case(dim=3, mode=mode, cp_spacing=5.0, sigma=10.0, subsample=1, seed=41)
+    serial = case.oracle()
+    assert orc.load(parallel=True).skmo_set_threads(4) == 4
+    assert orc.load().skmo_set_threads(4) == 1
+    par = case.oracle(parallel=True)
+    rs, ds = serial.evaluate(measure, True, 0.0, 1e-3, 0.0)
+    rp, dpar = par.evaluate(measure, True, 0.0, 1e-3, 0.0)
+    assert rs == rp
+    np.testing.assert_array_equal(ds, dpar)
