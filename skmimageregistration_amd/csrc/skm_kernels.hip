@@ -244,9 +244,11 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
 // evaluateValueNCC, GpuEvaluatorNCC.cu:37-109, 194-263).  Per sample it stores
 // (gradM, a) and b with a = M - F (MSE) or a = F, b = M (NCC), zeros outside,
 // and adds its share of the FP64 sums to the block partials.
+// Eight waves per SIMD: the per-pair loop is latency bound (measured faster
+// than the compiler's own register budget, which stops at seven).
 template <int DIM, int MODE, int MEASURE>
-__global__ __launch_bounds__(kThreads) void k_value(Problem pr, float4 *__restrict__ sg, float *__restrict__ sb,
-                                                    double *__restrict__ partials)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_value(
+    Problem pr, float4 *__restrict__ sg, float *__restrict__ sb, double *__restrict__ partials)
 {
     __shared__ double lds[kPartialsK * 4];
     int l[3];
@@ -501,10 +503,12 @@ __global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *_
 }
 
 // Control-point tile of the weighted-mode gather: small enough that the
-// tile's constants stay in scalar registers.
+// tile's constants stay in scalar registers.  In 3D the four points share
+// x and y, so their chords along a sample row nearly coincide and the lanes
+// of a row serve all four (measured faster than 4x1x1, 1x4x1 and 2x2x1).
 template <int DIM> struct CpTile;
 template <> struct CpTile<2> { static constexpr int X = 2, Y = 2, Z = 1; };
-template <> struct CpTile<3> { static constexpr int X = 4, Y = 1, Z = 1; };
+template <> struct CpTile<3> { static constexpr int X = 1, Y = 1, Z = 4; };
 
 // Sample planes (last axis) per slab of the weighted-mode gather: 3D windows
 // are walked slab by slab so that the workgroups in flight share one slab of
