@@ -657,7 +657,9 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
         int tx = dim == 3 ? 8 : 16, ty = dim == 3 ? 8 : 16, tz = dim == 3 ? 4 : 1;
         long long a = (long long)((pr.nsub[0] + tx - 1) / tx) * ((pr.nsub[1] + ty - 1) / ty) * ((pr.nsub[2] + tz - 1) / tz);
         long long b = (pr.n_sub + kThreads - 1) / kThreads;
-        tiles = (int)std::max(a, b) * 2 + 8;
+        // the value pass writes one partial per block tile, or per wave brick
+        // (four per tile) when it splits control-point rows over the waves
+        tiles = (int)std::max(4 * a, b) * 2 + 8;
     }
     ev->max_partial_blocks = tiles;
     if ((st = dev_alloc(&ev->d_cpc, pr.n_cp)) || (st = dev_alloc(&ev->d_cp_in, pr.n_cp * dim)) ||

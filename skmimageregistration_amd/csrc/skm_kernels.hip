@@ -9,6 +9,7 @@
 // derivative is a per-control-point gather by a whole workgroup over the
 // sample window with rows pruned to the kernel ball, and every reduction is a
 // fixed-order FP64 tree (per-block partials, then one finishing block).
+#include <cfloat>
 #include <climits>
 
 #include "skm_internal.hpp"
@@ -22,6 +23,10 @@ namespace skm {
 template <int DIM> struct Tile;
 template <> struct Tile<2> { static constexpr int X = 16, Y = 16, Z = 1; };
 template <> struct Tile<3> { static constexpr int X = 8, Y = 8, Z = 4; };
+// One wave's brick of samples (the block tile holds four).
+template <int DIM> struct WaveBrick;
+template <> struct WaveBrick<2> { static constexpr int X = 16, Y = 4, Z = 1; };
+template <> struct WaveBrick<3> { static constexpr int X = 4, Y = 4, Z = 4; };
 
 // Thread -> offset inside the block tile; each wave64 gets a compact brick.
 template <int DIM>
@@ -98,7 +103,8 @@ __device__ __forceinline__ float cp_state_load(const Problem &pr, long long j, P
 // any partner's kernel reaches (the support in the stationary modes,
 // pair_reach2 in the weighted modes) only hold k == 0 and are skipped.
 template <int DIM, int MODE>
-__device__ __forceinline__ void kernel_sum(const Problem &pr, const float *x, const PState &st, float *u)
+__device__ __forceinline__ void kernel_sum(const Problem &pr, const float *x, const PState &st, float *u,
+                                           int split = 1, int part = 0)
 {
     float ci[3];
     to_local<DIM>(pr.cp, x, ci);
@@ -112,9 +118,12 @@ __device__ __forceinline__ void kernel_sum(const Problem &pr, const float *x, co
     const float s2 = is_stationary(MODE) ? st.support * st.support
                                          : pair_reach2(pr.kc.smin, pstate_lam<DIM, MODE>(st), pr.lam_cp_max);
     const int z0 = DIM == 3 ? lo[2] : 0, z1 = DIM == 3 ? hi[2] : 0;
+    int row_index = 0;
     for (int cz = z0; cz <= z1; ++cz) {
         float dz = DIM == 3 ? x[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
         for (int cy = lo[1]; cy <= hi[1]; ++cy) {
+            // with split > 1 this call takes every split-th row of the window
+            if (split > 1 && row_index++ % split != part) continue;
             float dy = x[1] - (pr.cp.o[1] + pr.cp.h[1] * cy);
             float rem = s2 - dy * dy - dz * dz;
             if (rem <= 0.0f) continue;
@@ -180,7 +189,7 @@ __device__ __forceinline__ float wave_max_f(float v)
 // Every lane of the wave must call this (inactive lanes pass active = false).
 template <int DIM, int MODE>
 __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *x, const PState &st, float lam_x,
-                                                bool active, float *u)
+                                                bool active, float *u, int split = 1, int part = 0)
 {
     float ci[3];
     to_local<DIM>(pr.cp, x, ci);
@@ -200,22 +209,41 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
     // pair reach: |v|^2 < smin^2 (lam_x + lam_c) / 2 (x 1.0001), as the sum of
     // the sample's share and the control point's pre-scaled one (cp_reach)
     const float qx = pair_reach2(pr.kc.smin, lam_x, 0.0f);
+    // the wave's samples: bounding box and largest reach share (once per wave;
+    // each row's range of control points follows from them without per-row
+    // cross-lane reductions)
+    float bmin[3] = {0.0f, 0.0f, 0.0f}, bmax[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        bmin[d] = -wave_max_f(active ? -x[d] : -FLT_MAX);
+        bmax[d] = wave_max_f(active ? x[d] : -FLT_MAX);
+    }
+    const float qx_max = wave_max_f(active ? qx : 0.0f);
+    int row_index = 0;
     for (int cz = ulo[2]; cz <= uhi[2]; ++cz) {
-        const float dz = DIM == 3 ? x[2] - (pr.cp.o[2] + pr.cp.h[2] * cz) : 0.0f;
+        const float pz = DIM == 3 ? pr.cp.o[2] + pr.cp.h[2] * cz : 0.0f;
+        const float dz = DIM == 3 ? x[2] - pz : 0.0f;
+        const float dzb = DIM == 3 ? fmaxf(0.0f, fmaxf(bmin[2] - pz, pz - bmax[2])) : 0.0f;
         const bool zin = DIM == 2 || (cz >= lo[2] && cz <= hi[2]);
         for (int cy = ulo[1]; cy <= uhi[1]; ++cy) {
-            const float dy = x[1] - (pr.cp.o[1] + pr.cp.h[1] * cy);
-            // reach of x over this row's control points
-            const float s2 = pair_reach2(pr.kc.smin, lam_x, pr.cp_row_lam[cy + (long long)pr.cp.n[1] * cz]);
-            const float rem = s2 - dy * dy - dz * dz;
-            int xa = INT_MAX, xb = INT_MIN;
-            if (zin && cy >= lo[1] && cy <= hi[1] && rem > 0.0f) {
-                const float rx = fast_sqrt(rem) * pr.cp.ih[0];
-                xa = max(lo[0], (int)(ci[0] - rx));
-                xb = min(hi[0], (int)(ci[0] + rx) + 1);
-            }
-            const int uxa = __builtin_amdgcn_readfirstlane(wave_min_i(xa));
-            const int uxb = __builtin_amdgcn_readfirstlane(wave_max_i(xb));
+            // with split > 1 this wave takes every split-th row of the union
+            if (split > 1 && row_index++ % split != part) continue;
+            const float py = pr.cp.o[1] + pr.cp.h[1] * cy;
+            const float dyb = fmaxf(0.0f, fmaxf(bmin[1] - py, py - bmax[1]));
+            // largest pair reach in this row, from the box: control points
+            // farther along x than rx from the box are out of every lane's reach
+            const float rem = qx_max + pair_reach2(pr.kc.smin, 0.0f, pr.cp_row_lam[cy + (long long)pr.cp.n[1] * cz]) -
+                              dyb * dyb - dzb * dzb;
+            if (!(rem > 0.0f)) continue;
+            const float rx = fast_sqrt(rem);
+            const int uxa = __builtin_amdgcn_readfirstlane(
+                max(ulo[0], (int)floorf((bmin[0] - rx - pr.cp.o[0]) * pr.cp.ih[0])));
+            const int uxb = __builtin_amdgcn_readfirstlane(
+                min(uhi[0], (int)ceilf((bmax[0] + rx - pr.cp.o[0]) * pr.cp.ih[0])));
+            // this lane's window along x, empty outside its rows
+            const bool rin = active && zin && cy >= lo[1] && cy <= hi[1];
+            const int xa = rin ? lo[0] : INT_MAX, xb = rin ? hi[0] : INT_MIN;
+            const float dy = x[1] - py;
             const long long row = (long long)pr.cp.n[0] * (cy + (long long)pr.cp.n[1] * cz);
             const RowTerms rt = row_terms<DIM>(dy, dz);
             float fcx = (float)uxa;
@@ -246,16 +274,24 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
 // and adds its share of the FP64 sums to the block partials.
 // Eight waves per SIMD: the per-pair loop is latency bound (measured faster
 // than the compiler's own register budget, which stops at seven).
-template <int DIM, int MODE, int MEASURE>
+// SPLIT == 1: each wave a brick of its own.  SPLIT == 4 (small sample grids,
+// too few waves to fill the chip): the four waves share one brick, each
+// takes every fourth row of control points, and their partial sums are
+// added in wave order before wave 0 finishes the samples.
+template <int DIM, int MODE, int MEASURE, int SPLIT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_value(
     Problem pr, float4 *__restrict__ sg, float *__restrict__ sb, double *__restrict__ partials)
 {
+    static_assert(SPLIT == 1 || SPLIT == kThreads / 64, "a split uses every wave of the workgroup");
     __shared__ double lds[kPartialsK * 4];
+    __shared__ float lds_u[SPLIT > 1 ? SPLIT * 64 * 3 : 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int l[3];
-    tile_coords<DIM>(threadIdx.x, l);
-    int m[3] = {(int)blockIdx.x * Tile<DIM>::X + l[0], (int)blockIdx.y * Tile<DIM>::Y + l[1],
-                DIM == 3 ? (int)blockIdx.z * Tile<DIM>::Z + l[2] : 0};
-    const bool valid = m[0] < pr.nsub[0] && m[1] < pr.nsub[1] && (DIM == 2 || m[2] < pr.nsub[2]);
+    tile_coords<DIM>(SPLIT > 1 ? lane : (int)threadIdx.x, l);
+    constexpr int BX = SPLIT > 1 ? WaveBrick<DIM>::X : Tile<DIM>::X, BY = SPLIT > 1 ? WaveBrick<DIM>::Y : Tile<DIM>::Y,
+                  BZ = SPLIT > 1 ? WaveBrick<DIM>::Z : Tile<DIM>::Z;
+    int m[3] = {(int)blockIdx.x * BX + l[0], (int)blockIdx.y * BY + l[1], DIM == 3 ? (int)blockIdx.z * BZ + l[2] : 0};
+    bool valid = m[0] < pr.nsub[0] && m[1] < pr.nsub[1] && (DIM == 2 || m[2] < pr.nsub[2]);
     double acc[kPartialsK];
 #pragma unroll
     for (int k = 0; k < kPartialsK; ++k) acc[k] = 0.0;
@@ -271,12 +307,26 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
     point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
     float u[3];
     df_lookup<DIM>(pr, x, u);
+    // with a split the previous displacement is wave 0's share only
+    if (SPLIT > 1 && wave > 0) u[0] = u[1] = u[2] = 0.0f;
     float lam_x = 0.0f;
     if constexpr (is_stationary(MODE)) {
-        kernel_sum<DIM, MODE>(pr, x, st, u);
+        kernel_sum<DIM, MODE>(pr, x, st, u, SPLIT, wave);
     } else {
         lam_x = pstate_lam<DIM, MODE>(st);
-        kernel_sum_wave<DIM, MODE>(pr, x, st, lam_x, valid, u);
+        kernel_sum_wave<DIM, MODE>(pr, x, st, lam_x, valid, u, SPLIT, wave);
+    }
+    if constexpr (SPLIT > 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) lds_u[(wave * 64 + lane) * 3 + d] = u[d];
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int w = 1; w < SPLIT; ++w)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) u[d] += lds_u[(w * 64 + lane) * 3 + d];
+        }
+        valid = valid && wave == 0;
     }
     if (valid) {
         if (!is_stationary(MODE)) {
@@ -987,11 +1037,25 @@ static void value_impl(const Problem &pr, int measure, float4 *sg, float *sb, do
                        hipStream_t s)
 {
     dim3 grid = tile_grid<DIM>(pr.nsub);
+    constexpr int kSplit = kThreads / 64;
+    // fewer than four waves per SIMD (1024 SIMDs on the chip): the waves of a
+    // workgroup share a brick and split its control-point rows
+    if ((long long)grid.x * grid.y * grid.z * kSplit < 4096) {
+        grid = dim3((pr.nsub[0] + WaveBrick<DIM>::X - 1) / WaveBrick<DIM>::X,
+                    (pr.nsub[1] + WaveBrick<DIM>::Y - 1) / WaveBrick<DIM>::Y,
+                    DIM == 3 ? (pr.nsub[2] + WaveBrick<DIM>::Z - 1) / WaveBrick<DIM>::Z : 1);
+        *blocks = (int)(grid.x * grid.y * grid.z);
+        if (measure == SKM_MEASURE_MSE)
+            k_value<DIM, MODE, SKM_MEASURE_MSE, kSplit><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
+        else
+            k_value<DIM, MODE, SKM_MEASURE_NCC, kSplit><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
+        return;
+    }
     *blocks = (int)(grid.x * grid.y * grid.z);
     if (measure == SKM_MEASURE_MSE)
-        k_value<DIM, MODE, SKM_MEASURE_MSE><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
+        k_value<DIM, MODE, SKM_MEASURE_MSE, 1><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
     else
-        k_value<DIM, MODE, SKM_MEASURE_NCC><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
+        k_value<DIM, MODE, SKM_MEASURE_NCC, 1><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
 }
 
 void launch_value(const Problem &pr, int measure, float4 *sg, float *sb, double *partials, int *blocks_out,
