@@ -471,8 +471,8 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // Block sum of K doubles in a fixed order; result valid in thread 0.
-template <int K>
-__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K * 4 */)
+template <int K, int WAVES = kThreads / 64>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K * WAVES */)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -485,7 +485,7 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *lds /* K * 4 *
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             double s = 0.0;
-            for (int w = 0; w < kThreads / 64; ++w) s += lds[w * K + k];
+            for (int w = 0; w < WAVES; ++w) s += lds[w * K + k];
             v[k] = s;
         }
     }

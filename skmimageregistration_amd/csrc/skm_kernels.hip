@@ -11,6 +11,7 @@
 // fixed-order FP64 tree (per-block partials, then one finishing block).
 #include <cfloat>
 #include <climits>
+#include <type_traits>
 
 #include "skm_internal.hpp"
 
@@ -274,16 +275,16 @@ __device__ __forceinline__ void kernel_sum_wave(const Problem &pr, const float *
 // and adds its share of the FP64 sums to the block partials.
 // Eight waves per SIMD: the per-pair loop is latency bound (measured faster
 // than the compiler's own register budget, which stops at seven).
-// SPLIT == 1: each wave a brick of its own.  SPLIT == 4 (small sample grids,
-// too few waves to fill the chip): the four waves share one brick, each
-// takes every fourth row of control points, and their partial sums are
-// added in wave order before wave 0 finishes the samples.
+// SPLIT == 1: 256 threads, each wave a brick of its own.  SPLIT > 1 (small
+// sample grids, too few waves to fill the chip): SPLIT waves share one brick,
+// each takes every SPLIT-th row of control points, and their partial sums
+// are added in wave order before wave 0 finishes the samples.
 template <int DIM, int MODE, int MEASURE, int SPLIT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_value(
+__global__ __launch_bounds__(SPLIT > 1 ? 64 * SPLIT : kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_value(
     Problem pr, float4 *__restrict__ sg, float *__restrict__ sb, double *__restrict__ partials)
 {
-    static_assert(SPLIT == 1 || SPLIT == kThreads / 64, "a split uses every wave of the workgroup");
-    __shared__ double lds[kPartialsK * 4];
+    constexpr int WAVES = SPLIT > 1 ? SPLIT : kThreads / 64;
+    __shared__ double lds[kPartialsK * WAVES];
     __shared__ float lds_u[SPLIT > 1 ? SPLIT * 64 * 3 : 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int l[3];
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
         }
         acc[6] = in ? 1.0 : 0.0;
     }
-    block_sum<kPartialsK>(acc, lds);
+    block_sum<kPartialsK, WAVES>(acc, lds);
     if (threadIdx.x == 0) {
         long long b = block_linear();
 #pragma unroll
@@ -1037,18 +1038,27 @@ static void value_impl(const Problem &pr, int measure, float4 *sg, float *sb, do
                        hipStream_t s)
 {
     dim3 grid = tile_grid<DIM>(pr.nsub);
-    constexpr int kSplit = kThreads / 64;
     // fewer than four waves per SIMD (1024 SIMDs on the chip): the waves of a
-    // workgroup share a brick and split its control-point rows
-    if ((long long)grid.x * grid.y * grid.z * kSplit < 4096) {
+    // workgroup share a brick and split its control-point rows, 4, 8 or 16
+    // ways, the fewest that reach 4096 waves
+    constexpr long long kWavesWanted = 4096;
+    if ((long long)grid.x * grid.y * grid.z * (kThreads / 64) < kWavesWanted) {
         grid = dim3((pr.nsub[0] + WaveBrick<DIM>::X - 1) / WaveBrick<DIM>::X,
                     (pr.nsub[1] + WaveBrick<DIM>::Y - 1) / WaveBrick<DIM>::Y,
                     DIM == 3 ? (pr.nsub[2] + WaveBrick<DIM>::Z - 1) / WaveBrick<DIM>::Z : 1);
-        *blocks = (int)(grid.x * grid.y * grid.z);
-        if (measure == SKM_MEASURE_MSE)
-            k_value<DIM, MODE, SKM_MEASURE_MSE, kSplit><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
-        else
-            k_value<DIM, MODE, SKM_MEASURE_NCC, kSplit><<<grid, kThreads, 0, s>>>(pr, sg, sb, partials);
+        const long long bricks = (long long)grid.x * grid.y * grid.z;
+        *blocks = (int)bricks;
+        const int split = bricks * 4 >= kWavesWanted ? 4 : bricks * 8 >= kWavesWanted ? 8 : 16;
+        auto go = [&](auto split_c) {
+            constexpr int S = decltype(split_c)::value;
+            if (measure == SKM_MEASURE_MSE)
+                k_value<DIM, MODE, SKM_MEASURE_MSE, S><<<grid, 64 * S, 0, s>>>(pr, sg, sb, partials);
+            else
+                k_value<DIM, MODE, SKM_MEASURE_NCC, S><<<grid, 64 * S, 0, s>>>(pr, sg, sb, partials);
+        };
+        if (split == 4) go(std::integral_constant<int, 4>());
+        else if (split == 8) go(std::integral_constant<int, 8>());
+        else go(std::integral_constant<int, 16>());
         return;
     }
     *blocks = (int)(grid.x * grid.y * grid.z);
