@@ -216,6 +216,16 @@ skm_status dev_upload(T **p, const T *src, size_t count)
     return SKM_SUCCESS;
 }
 
+// A device buffer holding a copy of src (a host or, with device, a device array).
+template <typename T>
+skm_status dev_copy_in(T **p, const T *src, size_t count, bool device)
+{
+    if (!device) return dev_upload(p, src, count);
+    SKM_TRY(dev_alloc(p, count));
+    if (src && count) SKM_HIP(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyDeviceToDevice));
+    return SKM_SUCCESS;
+}
+
 skm_status validate_grid(const skm_grid &g, int dim, const char *name, int min_size)
 {
     for (int d = 0; d < dim; ++d) {
@@ -243,6 +253,43 @@ skm_status build_static(skm_evaluator *ev)
     int ns[3] = {1, 1, 1};
     for (int d = 0; d < dim; ++d) ns[d] = ds.fixed.size[d] / ds.subsample;
     const int nn = dim * dim;
+    // the weight value / tensor at every control point's nearest voxel
+    // (ImageNearest, index = int(... + 0.5)), gathered on the device
+    std::vector<float> cpw_vals;
+    if (ds.kernel_mode != SKM_KERNEL_STATIONARY) {
+        const int per = ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 1 : nn;
+        std::vector<long long> idx((size_t)ncp);
+        for (long long j = 0; j < ncp; ++j) {
+            int jc[3] = {0, 0, 0};
+            long long r = j;
+            for (int d = 0; d < dim; ++d) { jc[d] = (int)(r % ds.cp.size[d]); r /= ds.cp.size[d]; }
+            long long f = 0, stride = 1;
+            bool in = true;
+            for (int d = 0; d < dim; ++d) {
+                const double p = ds.cp.origin[d] + ds.cp.spacing[d] * jc[d];
+                const int i = (int)((1.0 / ds.weight.spacing[d]) * (p - ds.weight.origin[d]) + 0.5);
+                in = in && i >= 0 && i < ds.weight.size[d];
+                f += stride * i;
+                stride *= ds.weight.size[d];
+            }
+            idx[(size_t)j] = in ? f : -1;
+        }
+        long long *d_idx = nullptr;
+        float *d_vals = nullptr;
+        skm_status st = dev_upload(&d_idx, idx.data(), idx.size());
+        if (st == SKM_SUCCESS) st = dev_alloc(&d_vals, (size_t)ncp * per);
+        if (st == SKM_SUCCESS) {
+            launch_gather_values(ev->d_weight, d_idx, per, ncp, d_vals, ev->stream);
+            cpw_vals.resize((size_t)ncp * per);
+            hipError_t e = hipMemcpyAsync(cpw_vals.data(), d_vals, sizeof(float) * cpw_vals.size(),
+                                          hipMemcpyDeviceToHost, ev->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ev->stream);
+            if (e != hipSuccess) st = fail(SKM_ERROR_HIP, std::string("weight gather: ") + hipGetErrorString(e));
+        }
+        if (d_idx) (void)hipFree(d_idx);
+        if (d_vals) (void)hipFree(d_vals);
+        if (st != SKM_SUCCESS) return st;
+    }
     for (long long j = 0; j < ncp; ++j) {
         int jc[3] = {0, 0, 0};
         long long r = j;
@@ -251,17 +298,9 @@ skm_status build_static(skm_evaluator *ev)
         for (int d = 0; d < dim; ++d) p[d] = ds.cp.origin[d] + ds.cp.spacing[d] * jc[d];
         float *st = &state[(size_t)j * kCpStateStride];
         if (ds.kernel_mode != SKM_KERNEL_STATIONARY) {
-            long long f = 0, stride = 1;
-            bool in = true;
-            for (int d = 0; d < dim; ++d) {
-                int i = (int)((1.0 / ds.weight.spacing[d]) * (p[d] - ds.weight.origin[d]) + 0.5);
-                in = in && i >= 0 && i < ds.weight.size[d];
-                f += stride * i;
-                stride *= ds.weight.size[d];
-            }
             float *ps = &pstate[(size_t)j * 8];
             if (ds.kernel_mode == SKM_KERNEL_WEIGHT_IMAGE) {
-                double s = in ? ds.weight_data[f] : 0.0;
+                double s = cpw_vals[(size_t)j];
                 st[0] = (float)s;
                 st[1] = (float)std::sqrt(std::fabs(s));
                 support[j] = smin / std::sqrt(2.0 / (ds.maximum_weight + s));
@@ -271,7 +310,7 @@ skm_status build_static(skm_evaluator *ev)
                 lam_cp_max = std::max(lam_cp_max, (double)ps[7]);
             } else {
                 double cov[9] = {0}, sum[9] = {0};
-                for (int i = 0; i < nn; ++i) cov[i] = in ? ds.weight_data[f * nn + i] : ((i % (dim + 1)) == 0 ? 1.0 : 0.0);
+                for (int i = 0; i < nn; ++i) cov[i] = cpw_vals[(size_t)j * nn + i];
                 for (int i = 0; i < nn; ++i) st[2 + i] = (float)cov[i];
                 st[1] = (float)std::sqrt(det_h(dim, cov));
                 for (int i = 0; i < nn; ++i) sum[i] = ds.maximum_eigenvalue_covariance[i] + cov[i];
@@ -545,6 +584,15 @@ SKM_API const char *skm_last_error(void) { return g_error.c_str(); }
 
 SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
 {
+    return skm::create_evaluator(desc, false, out);
+}
+
+}  // extern "C"
+
+// skm_create; with device_data the image, weight and tensor pointers of the
+// descriptor are device pointers (copied on the device, no PCIe transfer).
+skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out)
+{
     if (!desc || !out) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     const int dim = desc->dim;
@@ -624,32 +672,26 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
         const long long np = (long long)(msize[0] + 3) * (msize[1] + 3) * (dim == 3 ? msize[2] + 3 : 1);
         float *raw = nullptr;
         double *work = nullptr;
-        if ((st = dev_upload(&raw, desc->moving_data, (size_t)nm)) != SKM_SUCCESS ||
+        if ((!device_data && (st = dev_upload(&raw, desc->moving_data, (size_t)nm)) != SKM_SUCCESS) ||
             (st = dev_alloc(&work, (size_t)nm)) != SKM_SUCCESS || (st = dev_alloc(&ev->d_coeffs, (size_t)np)) != SKM_SUCCESS) {
             if (raw) (void)hipFree(raw);
             if (work) (void)hipFree(work);
             return cleanup(st);
         }
-        launch_bspline_coefficients(dim, msize, raw, work, ev->d_coeffs, ev->stream);
+        launch_bspline_coefficients(dim, msize, device_data ? desc->moving_data : raw, work, ev->d_coeffs, ev->stream);
         const hipError_t e = hipStreamSynchronize(ev->stream);
         (void)hipFree(raw);
         (void)hipFree(work);
         if (e != hipSuccess) return cleanup(fail(SKM_ERROR_HIP, std::string("prefilter: ") + hipGetErrorString(e)));
     }
-    if ((st = dev_upload(&ev->d_fixed, desc->fixed_data, (size_t)pr.n_vox)) != SKM_SUCCESS) return cleanup(st);
+    if ((st = dev_copy_in(&ev->d_fixed, desc->fixed_data, (size_t)pr.n_vox, device_data)) != SKM_SUCCESS)
+        return cleanup(st);
     if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
         size_t per = desc->kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 1 : (size_t)dim * dim;
-        if ((st = dev_upload(&ev->d_weight, desc->weight_data, (size_t)nw * per)) != SKM_SUCCESS) return cleanup(st);
+        if ((st = dev_copy_in(&ev->d_weight, desc->weight_data, (size_t)nw * per, device_data)) != SKM_SUCCESS)
+            return cleanup(st);
     }
-    {
-        // static tables need the weight data on the host
-        skm_desc tmp = ev->desc;
-        tmp.weight_data = desc->weight_data;
-        ev->desc = tmp;
-        st = build_static(ev);
-        ev->desc.weight_data = nullptr;
-        if (st != SKM_SUCCESS) return cleanup(st);
-    }
+    if ((st = build_static(ev)) != SKM_SUCCESS) return cleanup(st);
 
     // per-sample and per-cp buffers
     int tiles = 1;
@@ -699,6 +741,8 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
     return SKM_SUCCESS;
 }
 
+extern "C" {
+
 SKM_API void skm_destroy(skm_evaluator *ev)
 {
     if (!ev) return;
@@ -735,6 +779,18 @@ SKM_API skm_status skm_set_regularizer_pg(skm_evaluator *ev, double weight, doub
 
 SKM_API skm_status skm_set_displacement_field(skm_evaluator *ev, const skm_grid *grid, const float *df_aos)
 {
+    return skm::set_displacement_field_impl(ev, grid, df_aos, false);
+}
+
+}  // extern "C"
+
+skm_status skm::set_displacement_field_device(skm_evaluator *ev, const skm_grid *grid, const float *d_df_aos)
+{
+    return set_displacement_field_impl(ev, grid, d_df_aos, true);
+}
+
+skm_status skm::set_displacement_field_impl(skm_evaluator *ev, const skm_grid *grid, const float *df_aos, bool device)
+{
     if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
     SKM_HIP(hipStreamSynchronize(ev->stream));
     if (ev->d_df) {
@@ -746,12 +802,14 @@ SKM_API skm_status skm_set_displacement_field(skm_evaluator *ev, const skm_grid 
     if (!grid || !df_aos) return SKM_SUCCESS;
     SKM_TRY(validate_grid(*grid, ev->dim, "displacement field", 1));
     size_t n = (size_t)grid_count(*grid, ev->dim) * ev->dim;
-    SKM_TRY(dev_upload(&ev->d_df, df_aos, n));
+    SKM_TRY(dev_copy_in(&ev->d_df, df_aos, n, device));
     ev->pr.df = to_grid(*grid, ev->dim);
     ev->pr.df_data = ev->d_df;
     ev->pr.has_df = 1;
     return SKM_SUCCESS;
 }
+
+extern "C" {
 
 SKM_API skm_status skm_set_stream(skm_evaluator *ev, void *hip_stream)
 {

@@ -78,12 +78,27 @@ void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
 // packed states of the weighted modes at every voxel of the dense fixed grid
 void launch_voxel_states(const Problem &pr, float4 *out, hipStream_t s);
 
+// skm_create with, when device_data, device pointers in the descriptor (skm_capi.hip)
+skm_status create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out);
+// skm_set_displacement_field from a device array
+skm_status set_displacement_field_device(skm_evaluator *ev, const skm_grid *grid, const float *d_df_aos);
+skm_status set_displacement_field_impl(skm_evaluator *ev, const skm_grid *grid, const float *df_aos, bool device);
+// GaussianSmoothing / ComputeAnisotropicTensor on device arrays (skm_tensor.hip)
+skm_status gaussian_smooth_device(int dim, const skm_grid &grid, const float *d_in, double variance, float *d_out);
+skm_status anisotropic_tensor_device(int dim, const skm_grid &grid, const float *d_image,
+                                     const skm_tensor_options &opt, float *d_tensor_out, double *maximum_tilt,
+                                     double *maximum_weight, double *maximum_eigenvalue_covariance);
+
 // set-up (skm_setup.hip)
 // cubic B-spline coefficients (mirror boundary) in FP64, unpadded, in d_work
 void launch_bspline_prefilter(int dim, const int *size, const float *d_src, double *d_work, hipStream_t s);
 // the same, then shifted by one and padded to (n+3)^D in float
 void launch_bspline_coefficients(int dim, const int *size, const float *d_src, double *d_work, float *d_coeffs,
                                  hipStream_t s);
+// dst[i * per + k] = src[idx[i] * per + k], or (idx < 0) 0 for per == 1 and
+// the identity's entries for per == D * D
+void launch_gather_values(const float *d_src, const long long *d_idx, int per, long long n, float *d_dst,
+                          hipStream_t s);
 unsigned weight_lam_blocks(long long nw);
 void launch_weight_lam(int dim, int mode, long long nw, const float *d_w, float *d_lam, double *d_blk_max,
                        unsigned *d_blk_bad, hipStream_t s);

@@ -3,6 +3,11 @@
 // smoothing and weight tensor, a new evaluator per level whose previous
 // displacement is the sum of the coarser levels, the device-resident
 // adaptive-step optimizer, and the dense displacement of all levels.
+//
+// Images, tensors and displacement fields stay in device memory from level to
+// level: the preprocessed images (and a given weight image, weight tensor or
+// guidance image) cross PCIe once, and only result files, landmark errors and
+// the final field are copied back.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -140,6 +145,49 @@ struct Output {
     }
 };
 
+// A device array owned by the driver.
+template <typename T>
+struct DeviceArray {
+    T *p = nullptr;
+    size_t n = 0;
+    DeviceArray() = default;
+    DeviceArray(const DeviceArray &) = delete;
+    DeviceArray &operator=(const DeviceArray &) = delete;
+    ~DeviceArray()
+    {
+        if (p) (void)hipFree(p);
+    }
+    skm_status alloc(size_t count)
+    {
+        if (p && n == count) return SKM_SUCCESS;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return set_last_error(SKM_ERROR_HIP, std::string("registration buffer: ") + hipGetErrorString(e));
+        }
+        n = count;
+        return SKM_SUCCESS;
+    }
+    skm_status upload(const T *src, size_t count)
+    {
+        skm_status st = alloc(count);
+        if (st != SKM_SUCCESS) return st;
+        hipError_t e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("upload: ") + hipGetErrorString(e));
+        return SKM_SUCCESS;
+    }
+    skm_status download(std::vector<T> &dst) const
+    {
+        dst.resize(n);
+        hipError_t e = hipMemcpy(dst.data(), p, n * sizeof(T), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("download: ") + hipGetErrorString(e));
+        return SKM_SUCCESS;
+    }
+};
+
 struct EvaluatorGuard {
     skm_evaluator *ev = nullptr;
     ~EvaluatorGuard()
@@ -243,7 +291,22 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
     }
     const long long n_w = voxels(desc->weight, dim);
 
-    std::vector<float> ref_s(n_ref), tar_s(n_tar), tensor;
+    // device copies of the inputs, made once
+    DeviceArray<float> d_ref, d_tar, d_ref_s, d_tar_s, d_weight, d_guide, d_tensor, d_prev, d_cp, d_cpw;
+    SKM_R_TRY(d_ref.upload(ref.data(), ref.size()));
+    SKM_R_TRY(d_tar.upload(tar.data(), tar.size()));
+    if (desc->weight_image)
+        SKM_R_TRY(d_weight.upload(desc->weight_image, (size_t)n_w));
+    else if (desc->weight_tensor)
+        SKM_R_TRY(d_weight.upload(desc->weight_tensor, (size_t)n_w * dim * dim));
+    if (desc->guidance_image) SKM_R_TRY(d_guide.upload(desc->guidance_image, (size_t)n_w));
+    bool has_prev = !prev.empty();
+    if (has_prev)
+        SKM_R_TRY(d_prev.upload(prev.data(), prev.size()));
+    else
+        SKM_R_TRY(d_prev.alloc((size_t)n_tar * dim));
+
+    std::vector<float> host_tmp;
     for (int scale = start; scale < desc->num_scales; ++scale) {
         const skm_scale_params &P = desc->scale[scale];
         skm_scale_report rep;
@@ -251,16 +314,24 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         const auto t_setup = Clock::now();
 
         // smoothing (SKMReg.cxx:140-150)
+        const float *ref_s = d_ref.p, *tar_s = d_tar.p;
         if (P.smoothing_variance > 1e-5) {
-            SKM_R_TRY(skm_gaussian_smooth(dim, &desc->reference, ref.data(), P.smoothing_variance, ref_s.data()));
-            SKM_R_TRY(skm_gaussian_smooth(dim, &desc->target, tar.data(), P.smoothing_variance, tar_s.data()));
-        } else {
-            ref_s = ref;
-            tar_s = tar;
+            SKM_R_TRY(d_ref_s.alloc(d_ref.n));
+            SKM_R_TRY(d_tar_s.alloc(d_tar.n));
+            SKM_R_TRY(skm::gaussian_smooth_device(dim, desc->reference, d_ref.p, P.smoothing_variance, d_ref_s.p));
+            SKM_R_TRY(skm::gaussian_smooth_device(dim, desc->target, d_tar.p, P.smoothing_variance, d_tar_s.p));
+            ref_s = d_ref_s.p;
+            tar_s = d_tar_s.p;
         }
-        if (desc->verbosity >= 2) {
-            SKM_R_TRY(out.write("smoothed_reference-" + std::to_string(scale) + ".vtk", desc->reference, 1, ref_s.data()));
-            SKM_R_TRY(out.write("smoothed_target-" + std::to_string(scale) + ".vtk", desc->target, 1, tar_s.data()));
+        if (out.on() && desc->verbosity >= 2) {
+            const DeviceArray<float> &rs = ref_s == d_ref.p ? d_ref : d_ref_s;
+            const DeviceArray<float> &ts = tar_s == d_tar.p ? d_tar : d_tar_s;
+            SKM_R_TRY(rs.download(host_tmp));
+            SKM_R_TRY(out.write("smoothed_reference-" + std::to_string(scale) + ".vtk", desc->reference, 1,
+                                host_tmp.data()));
+            SKM_R_TRY(ts.download(host_tmp));
+            SKM_R_TRY(out.write("smoothed_target-" + std::to_string(scale) + ".vtk", desc->target, 1,
+                                host_tmp.data()));
         }
 
         // kernel: weight image (takes precedence, GpuEvaluator.cu:311-346),
@@ -269,19 +340,20 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         skm_desc_init(&ed);
         ed.dim = dim;
         ed.fixed = desc->target;
-        ed.fixed_data = tar_s.data();
+        ed.fixed_data = tar_s;
         ed.moving = desc->reference;
-        ed.moving_data = ref_s.data();
+        ed.moving_data = ref_s;
         ed.sigma_c0 = P.sigma_c0;
         ed.sigma_c4 = P.sigma_c4;
         ed.subsample = P.sampling_rate;
         ed.subsample_neighborhood = P.neighborhood_sampling;
+        const float *host_tensor = nullptr;  // the level's weight tensor in host memory, if any
         if (desc->weight_image) {
             double mx = std::numeric_limits<double>::lowest();
             for (long long i = 0; i < n_w; ++i) mx = std::max(mx, (double)desc->weight_image[i]);
             ed.kernel_mode = SKM_KERNEL_WEIGHT_IMAGE;
             ed.weight = desc->weight;
-            ed.weight_data = desc->weight_image;
+            ed.weight_data = d_weight.p;
             ed.maximum_weight = mx;
         } else if (desc->weight_tensor && P.tensor_weight == 0) {
             // SetWeightingTensor(tensor, -1): largest |lambda_max| and its tensor
@@ -298,30 +370,35 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
             }
             ed.kernel_mode = SKM_KERNEL_WEIGHT_TENSOR;
             ed.weight = desc->weight;
-            ed.weight_data = desc->weight_tensor;
+            ed.weight_data = d_weight.p;
             ed.maximum_weight = best;
             for (int i = 0; i < 9; ++i) ed.maximum_eigenvalue_covariance[i] = 0.0;
             for (int i = 0; i < dim; ++i)
                 for (int j = 0; j < dim; ++j)
                     ed.maximum_eigenvalue_covariance[i * 3 + j] = desc->weight_tensor[at * nn + i * dim + j];
+            host_tensor = desc->weight_tensor;
         } else if (P.tensor_weight > 0) {
             const bool guided = desc->guidance_image != nullptr;
             const skm_grid &g = guided ? desc->weight : desc->target;
-            tensor.resize((size_t)voxels(g, dim) * dim * dim);
+            SKM_R_TRY(d_tensor.alloc((size_t)voxels(g, dim) * dim * dim));
             skm_tensor_options to{P.sigma_c4, P.tensor_weight, 2, 0};
             double tilt = 0, mw = 0, mcov[9];
-            SKM_R_TRY(skm_anisotropic_tensor(dim, &g, guided ? desc->guidance_image : tar_s.data(), &to,
-                                             tensor.data(), &tilt, &mw, mcov));
+            SKM_R_TRY(skm::anisotropic_tensor_device(dim, g, guided ? d_guide.p : tar_s, to, d_tensor.p, &tilt, &mw,
+                                                     mcov));
             ed.kernel_mode = SKM_KERNEL_WEIGHT_TENSOR;
             ed.weight = g;
-            ed.weight_data = tensor.data();
+            ed.weight_data = d_tensor.p;
             ed.maximum_weight = mw;
             std::memcpy(ed.maximum_eigenvalue_covariance, mcov, sizeof(mcov));
             rep.maximum_tilt = tilt;
+            if (out.on()) {
+                SKM_R_TRY(d_tensor.download(host_tmp));
+                host_tensor = host_tmp.data();
+            }
         }
         if (ed.kernel_mode == SKM_KERNEL_WEIGHT_TENSOR)  // SKMReg.cxx:290-297
             SKM_R_TRY(out.write("covariance_tensor-" + std::to_string(scale) + ".vtk", ed.weight, dim * dim,
-                                ed.weight_data));
+                                host_tensor));
 
         // control points (SKMReg.cxx:219-232)
         double sp[3];
@@ -335,12 +412,12 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         }
 
         EvaluatorGuard guard;
-        SKM_R_TRY(skm_create(&ed, &guard.ev));
+        SKM_R_TRY(skm::create_evaluator(&ed, true, &guard.ev));
         skm_evaluator *ev = guard.ev;
         if (P.reg_rkhs > 0) SKM_R_TRY(skm_set_regularizer_rkhs(ev, P.reg_rkhs));
         if (P.reg_rd > 0) SKM_R_TRY(skm_set_regularizer_rd(ev, P.reg_rd, P.reg_rd_scaling));
         if (P.reg_pg > 0) SKM_R_TRY(skm_set_regularizer_pg(ev, P.reg_pg, P.reg_pg_scaling));
-        if (!prev.empty()) SKM_R_TRY(skm_set_displacement_field(ev, &desc->target, prev.data()));
+        if (has_prev) SKM_R_TRY(skm::set_displacement_field_device(ev, &desc->target, d_prev.p));
         rep.ms_setup = ms_since(t_setup);
 
         // optimisation (SKMReg.cxx:375-430)
@@ -366,14 +443,18 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         }
         rep.ms_optimize = ms_since(t_opt);
 
-        // displacement of all levels so far (SKMReg.cxx:436-447)
+        // displacement of all levels so far (SKMReg.cxx:436-447), written over
+        // the previous field, which the evaluator holds its own copy of
         const auto t_df = Clock::now();
         std::vector<float> cp_aos((size_t)n_cp * dim);
         for (long long j = 0; j < n_cp; ++j)
             for (int d = 0; d < dim; ++d) cp_aos[j * dim + d] = params[d * n_cp + j];
-        std::vector<float> df((size_t)n_tar * dim);
-        SKM_R_TRY(skm_evaluate_displacement_field(ev, cp_aos.data(), cpw.data(), df.data()));
-        prev.swap(df);
+        SKM_R_TRY(d_cp.upload(cp_aos.data(), cp_aos.size()));
+        SKM_R_TRY(d_cpw.upload(cpw.data(), cpw.size()));
+        SKM_R_TRY(skm_evaluate_displacement_field_device(ev, d_cp.p, d_cpw.p, d_prev.p));
+        SKM_R_TRY(skm_synchronize(ev));
+        has_prev = true;
+        if (out.on() || landmarks) SKM_R_TRY(d_prev.download(prev));
         rep.ms_displacement = ms_since(t_df);
 
         // results of the level (SKMReg.cxx:459-505)
@@ -404,7 +485,10 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         rep.zero_parameters = params.empty() ? 0.0 : 100.0 * (double)zeros / (double)params.size();
         if (reports) reports[scale] = rep;
     }
-    if (prev.empty()) prev.assign((size_t)n_tar * dim, 0.0f);
+    if (!has_prev)
+        prev.assign((size_t)n_tar * dim, 0.0f);
+    else if (prev.size() != (size_t)n_tar * dim || !(out.on() || landmarks))
+        SKM_R_TRY(d_prev.download(prev));
     std::memcpy(displacement_out, prev.data(), sizeof(float) * prev.size());
 
     // final results (SKMReg.cxx:507-525): the dense field and the reference

@@ -199,6 +199,22 @@ __global__ __launch_bounds__(kThreads) void k_sample_row_lams(int dim, Grid fixe
     out[r] = m;
 }
 
+// dst[i * per + k] = src[idx[i] * per + k] (idx >= 0), fill[k] otherwise.
+__global__ __launch_bounds__(kThreads) void k_gather_values(const float *__restrict__ src,
+                                                            const long long *__restrict__ idx, int per, long long n,
+                                                            float fill_diag, float *__restrict__ dst)
+{
+    const long long t = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (t >= n * per) return;
+    const long long i = t / per;
+    const int k = (int)(t % per);
+    const long long f = idx[i];
+    // outside the grid: identity (tensor, diagonal entries) or 0 (weight image)
+    const int d = per == 4 ? 2 : 3;
+    const float outside = per == 1 ? 0.0f : ((k % (d + 1)) == 0 ? fill_diag : 0.0f);
+    dst[t] = f >= 0 ? src[f * per + k] : outside;
+}
+
 unsigned blocks(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
 }  // namespace
@@ -227,6 +243,13 @@ void launch_bspline_coefficients(int dim, const int *size, const float *d_src, d
     launch_bspline_prefilter(dim, size, d_src, d_work, s);
     const long long np = (long long)(n0 + 3) * (n1 + 3) * (dim == 3 ? n2 + 3 : 1);
     k_pad_mirror<<<blocks(np), kThreads, 0, s>>>(d_work, d_coeffs, dim, n0, n1, n2);
+}
+
+void launch_gather_values(const float *d_src, const long long *d_idx, int per, long long n, float *d_dst,
+                          hipStream_t s)
+{
+    if (n <= 0) return;
+    k_gather_values<<<blocks(n * per), kThreads, 0, s>>>(d_src, d_idx, per, n, 1.0f, d_dst);
 }
 
 unsigned weight_lam_blocks(long long nw) { return blocks(nw); }

@@ -467,17 +467,12 @@ SKM_API skm_status skm_gaussian_smooth(int32_t dim, const skm_grid *grid, const 
 {
     skm_status st = validate_image(dim, grid, in, out);
     if (st != SKM_SUCCESS) return st;
-    if (!(variance >= 0.0)) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "variance must not be negative");
     const Dims d = make_dims(dim, *grid);
-    AxisKernels k;
-    if ((st = make_axis_kernels(dim, *grid, variance, k)) != SKM_SUCCESS) return st;
-    DevBuf<float> a, b;
+    DevBuf<float> a;
     SKM_T_HIP(a.alloc(d.n_vox));
-    SKM_T_HIP(b.alloc(d.n_vox));
     SKM_T_HIP(hipMemcpy(a.p, in, d.n_vox * sizeof(float), hipMemcpyHostToDevice));
-    float *res = nullptr;
-    if ((st = gaussian_passes(dim, d, 1, k, 1.0f, a.p, b.p, &res, nullptr)) != SKM_SUCCESS) return st;
-    SKM_T_HIP(hipMemcpy(out, res, d.n_vox * sizeof(float), hipMemcpyDeviceToHost));
+    if ((st = gaussian_smooth_device(dim, *grid, a.p, variance, a.p)) != SKM_SUCCESS) return st;
+    SKM_T_HIP(hipMemcpy(out, a.p, d.n_vox * sizeof(float), hipMemcpyDeviceToHost));
     return SKM_SUCCESS;
 }
 
@@ -488,6 +483,49 @@ SKM_API skm_status skm_anisotropic_tensor(int32_t dim, const skm_grid *grid, con
     skm_status st = validate_image(dim, grid, image, tensor_out);
     if (st != SKM_SUCCESS) return st;
     if (!opt) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "null options");
+    const Dims d = make_dims(dim, *grid);
+    DevBuf<float> img, ten;
+    SKM_T_HIP(img.alloc(d.n_vox));
+    SKM_T_HIP(ten.alloc(d.n_vox * dim * dim));
+    SKM_T_HIP(hipMemcpy(img.p, image, d.n_vox * sizeof(float), hipMemcpyHostToDevice));
+    if ((st = anisotropic_tensor_device(dim, *grid, img.p, *opt, ten.p, maximum_tilt, maximum_weight,
+                                        maximum_eigenvalue_covariance)) != SKM_SUCCESS)
+        return st;
+    SKM_T_HIP(hipMemcpy(tensor_out, ten.p, d.n_vox * dim * dim * sizeof(float), hipMemcpyDeviceToHost));
+    return SKM_SUCCESS;
+}
+
+}  // extern "C"
+
+// GaussianSmoothing on device arrays; d_in and d_out may be the same array.
+skm_status skm::gaussian_smooth_device(int dim, const skm_grid &grid, const float *d_in, double variance,
+                                       float *d_out)
+{
+    skm_status st = validate_image(dim, &grid, d_in, d_out);
+    if (st != SKM_SUCCESS) return st;
+    if (!(variance >= 0.0)) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "variance must not be negative");
+    const Dims d = make_dims(dim, grid);
+    AxisKernels k;
+    if ((st = make_axis_kernels(dim, grid, variance, k)) != SKM_SUCCESS) return st;
+    DevBuf<float> a, b;
+    SKM_T_HIP(a.alloc(d.n_vox));
+    SKM_T_HIP(b.alloc(d.n_vox));
+    SKM_T_HIP(hipMemcpy(a.p, d_in, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice));
+    float *res = nullptr;
+    if ((st = gaussian_passes(dim, d, 1, k, 1.0f, a.p, b.p, &res, nullptr)) != SKM_SUCCESS) return st;
+    SKM_T_HIP(hipMemcpy(d_out, res, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice));
+    return SKM_SUCCESS;
+}
+
+skm_status skm::anisotropic_tensor_device(int dim, const skm_grid &grid_ref, const float *d_image,
+                                          const skm_tensor_options &opt_ref, float *d_tensor_out,
+                                          double *maximum_tilt, double *maximum_weight,
+                                          double *maximum_eigenvalue_covariance)
+{
+    const skm_grid *grid = &grid_ref;
+    const skm_tensor_options *opt = &opt_ref;
+    skm_status st = validate_image(dim, grid, d_image, d_tensor_out);
+    if (st != SKM_SUCCESS) return st;
     if (opt->anisotropic_filtering)
         return set_last_error(SKM_ERROR_NOT_SUPPORTED,
                               "anisotropic pre-filtering (curvature diffusion) is not supported; SKMReg passes false");
@@ -499,19 +537,18 @@ SKM_API skm_status skm_anisotropic_tensor(int32_t dim, const skm_grid *grid, con
     if ((st = make_axis_kernels(dim, *grid, opt->sigma, k_img)) != SKM_SUCCESS) return st;
     if ((st = make_axis_kernels(dim, *grid, opt->sigma / 4.0, k_ten)) != SKM_SUCCESS) return st;
 
-    DevBuf<float> a, b, ca, cb, ten;
+    DevBuf<float> a, b, ca, cb;
     SKM_T_HIP(a.alloc(d.n_vox));
     SKM_T_HIP(b.alloc(d.n_vox));
     SKM_T_HIP(ca.alloc(d.n_vox * nc));
     SKM_T_HIP(cb.alloc(d.n_vox * nc));
-    SKM_T_HIP(ten.alloc(d.n_vox * dim * dim));
     const int fin_blocks = std::min(launch_blocks(d.n_vox), 4096);
     DevBuf<float> bmax;
     DevBuf<unsigned> bcnt, bbad;
     SKM_T_HIP(bmax.alloc(fin_blocks));
     SKM_T_HIP(bcnt.alloc(fin_blocks));
     SKM_T_HIP(bbad.alloc(fin_blocks));
-    SKM_T_HIP(hipMemcpy(a.p, image, d.n_vox * sizeof(float), hipMemcpyHostToDevice));
+    SKM_T_HIP(hipMemcpy(a.p, d_image, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice));
 
     hipStream_t stream = nullptr;
     float *smoothed = nullptr, *comps = nullptr;
@@ -529,10 +566,10 @@ SKM_API skm_status skm_anisotropic_tensor(int32_t dim, const skm_grid *grid, con
     const int align = opt->alignment_direction;
     if (dim == 3)
         hipLaunchKernelGGL(k_tensor_finish<3>, dim3(fin_blocks), dim3(kThreads), 0, stream, comps, d.n_vox, align,
-                           ten.p, bmax.p, bcnt.p, bbad.p);
+                           d_tensor_out, bmax.p, bcnt.p, bbad.p);
     else
         hipLaunchKernelGGL(k_tensor_finish<2>, dim3(fin_blocks), dim3(kThreads), 0, stream, comps, d.n_vox, align,
-                           ten.p, bmax.p, bcnt.p, bbad.p);
+                           d_tensor_out, bmax.p, bcnt.p, bbad.p);
     SKM_T_HIP(hipGetLastError());
 
     std::vector<float> hmax(fin_blocks);
@@ -550,7 +587,6 @@ SKM_API skm_status skm_anisotropic_tensor(int32_t dim, const skm_grid *grid, con
     if (bad)
         return set_last_error(SKM_ERROR_INVALID_ARGUMENT,
                               std::to_string(bad) + " voxels have a non-finite structure tensor (non-finite image?)");
-    SKM_T_HIP(hipMemcpy(tensor_out, ten.p, d.n_vox * dim * dim * sizeof(float), hipMemcpyDeviceToHost));
 
     // ComputeAnisotropicTensor returns the largest tilt, or the lowest double
     // when no voxel was re-weighted; SetWeightingTensor then takes the tilt as
@@ -564,4 +600,3 @@ SKM_API skm_status skm_anisotropic_tensor(int32_t dim, const skm_grid *grid, con
     return SKM_SUCCESS;
 }
 
-}  // extern "C"
