@@ -484,7 +484,8 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
             PassTimer t(ev, 8);
             if (pr.exact) {
                 if (!ev->d_gamma) SKM_TRY(dev_alloc(&ev->d_gamma, pr.n_vox));
-                SKM_HIP(hipMemsetAsync(ev->d_gamma, 0, sizeof(float4) * pr.n_vox, s));
+                launch_zero_words(ev->d_gamma, 4LL * pr.n_vox, s);
+                SKM_HIP(hipGetLastError());
                 launch_lcc_scatter(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_gamma, s);
                 Problem dp = pr;  // gather over the dense voxel grid
                 if (!is_stationary(pr.kc.mode)) {

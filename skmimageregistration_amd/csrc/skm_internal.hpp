@@ -97,6 +97,7 @@ void launch_bspline_coefficients(int dim, const int *size, const float *d_src, d
                                  hipStream_t s);
 // dst[i * per + k] = src[idx[i] * per + k], or (idx < 0) 0 for per == 1 and
 // the identity's entries for per == D * D
+void launch_zero_words(void *d_ptr, long long n_words, hipStream_t s);
 void launch_gather_values(const float *d_src, const long long *d_idx, int per, long long n, float *d_dst,
                           hipStream_t s);
 unsigned weight_lam_blocks(long long nw);

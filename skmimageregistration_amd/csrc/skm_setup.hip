@@ -4,6 +4,7 @@
 // itkTransformAdapter.hxx:310-366, and the lam bounds of the pair reach).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "skm_internal.hpp"
@@ -215,9 +216,24 @@ __global__ __launch_bounds__(kThreads) void k_gather_values(const float *__restr
     dst[t] = f >= 0 ? src[f * per + k] : outside;
 }
 
+// Zeroes n words with vector stores (a kernel rather than a memset node, so
+// that a stream capture of the caller records only kernels).
+__global__ __launch_bounds__(kThreads) void k_zero_words(unsigned *__restrict__ p, long long n)
+{
+    const long long stride = (long long)gridDim.x * kThreads;
+    for (long long t = blockIdx.x * (long long)kThreads + threadIdx.x; t < n; t += stride) p[t] = 0u;
+}
+
 unsigned blocks(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
 }  // namespace
+
+void launch_zero_words(void *d_ptr, long long n_words, hipStream_t s)
+{
+    if (n_words <= 0) return;
+    const unsigned g = (unsigned)std::min<long long>(blocks(n_words), 8192);
+    k_zero_words<<<g, kThreads, 0, s>>>(static_cast<unsigned *>(d_ptr), n_words);
+}
 
 void launch_bspline_prefilter(int dim, const int *size, const float *d_src, double *d_work, hipStream_t s)
 {
