@@ -281,14 +281,14 @@ def profiled_traffic(kernel, key):
 def cpu_baseline(args, mov, fix, ten, mcov):
     """The FP64 oracle, OpenMP build (per-sample and per-control-point loops
     in parallel over the host's cores, at most 16: one GPU's share of the
-    host), on the same volumes with the same kernel and control-point
-    spacing, on a 96x96x64 crop (about 0.4 of the volume; --cpu-crop sets
-    another); reported in the same unit."""
+    host), one GetValueAndDerivative of the same workload (whole volumes,
+    same kernel and control-point spacing; --cpu-crop z,y,x takes a corner
+    crop instead); reported in the same unit."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
     threads = min(16, os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     threads = orc.load(parallel=True).skmo_set_threads(threads)
-    crop = tuple(int(v) for v in (args.cpu_crop or "64,96,96").split(","))
+    crop = tuple(int(v) for v in args.cpu_crop.split(",")) if args.cpu_crop else tuple(fix.shape)
     crop = tuple(min(c, n) for c, n in zip(crop, fix.shape))
     f = np.ascontiguousarray(fix[: crop[0], : crop[1], : crop[2]])
     m = np.ascontiguousarray(mov[: crop[0], : crop[1], : crop[2]])

@@ -595,12 +595,13 @@ __device__ __forceinline__ void tile_plane_range(const Problem &pr, int t0, int 
 // along x and 32 rows at a time; each lane loads a sample (v, packed state)
 // once and adds its pair with every tile control point whose window and
 // reach (both points' lam) hold the sample.  The tile constants are
-// wave-uniform (scalar registers); the next sample's loads are issued before
-// the current sample's pairs.  Each workgroup writes its FP64 sums;
+// wave-uniform (scalar registers).  Six waves per SIMD (measured faster than
+// five with the next sample's loads issued early, and than seven or eight,
+// which spill).  Each workgroup writes its FP64 sums;
 // k_gather_finish adds the slabs in order.  Same sums per control point as
 // k_gather, in a fixed order.
 template <int DIM, int MODE>
-__global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const float4 *__restrict__ sv, int n_tiles)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_gather_tile(Problem pr, const float4 *__restrict__ sv, int n_tiles)
 {
     constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z, T = TX * TY * TZ;
     __shared__ double lds[T * 3 * 4];
@@ -697,20 +698,9 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
         // would overflow)
         if (rxa > rxb) continue;
         const long long rowbase = (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
-        int mx = rxa + tx;
-        float4 cv = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ca = cv, cb = cv;
-        if (mx <= rxb) {
-            cv = sv[rowbase + mx];
-            ca = pr.smp_pstate[2 * (rowbase + mx)];
-            cb = pr.smp_pstate[2 * (rowbase + mx) + 1];
-        }
-        for (; mx <= rxb; mx += 8) {
-            float4 nv = cv, na = ca, nb = cb;
-            if (mx + 8 <= rxb) {
-                nv = sv[rowbase + mx + 8];
-                na = pr.smp_pstate[2 * (rowbase + mx + 8)];
-                nb = pr.smp_pstate[2 * (rowbase + mx + 8) + 1];
-            }
+        for (int mx = rxa + tx; mx <= rxb; mx += 8) {
+            const float4 cv = sv[rowbase + mx], ca = pr.smp_pstate[2 * (rowbase + mx)],
+                         cb = pr.smp_pstate[2 * (rowbase + mx) + 1];
             PState sst;
             pstate_unpack<DIM, MODE>(ca, cb, sst);
             int pos[3] = {mx, my, mz};
@@ -735,9 +725,6 @@ __global__ __launch_bounds__(kThreads) void k_gather_tile(Problem pr, const floa
                     if (DIM == 3) acc[c][2] = fmaf(k, cv.z, acc[c][2]);
                 }
             }
-            cv = nv;
-            ca = na;
-            cb = nb;
         }
     }
     // wave sums (fixed butterfly, FP32), then the four waves in FP64
