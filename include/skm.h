@@ -286,6 +286,11 @@ typedef struct skm_optimizer_options {
     int32_t return_best;             /* return the evaluated position with the lowest value
                                         (ITK's ReturnBestParametersAndValue, set by
                                         SKMReg.cxx:383) instead of the last step's     */
+    int32_t graph;                   /* 1: the first iteration is launched kernel by
+                                        kernel, the others replay it as one captured HIP
+                                        graph (one host call per iteration; same results);
+                                        0: every iteration kernel by kernel, as also while
+                                        pass timing is on                               */
 } skm_optimizer_options;
 
 /* Metric value per iteration as reported by

@@ -334,7 +334,8 @@ public:
         skm_metric_options mo{(int)metric_->measure(), metric_->GetRegularizerL1(), metric_->GetRegularizerL21(),
                               metric_->GetRegularizerL2(), metric_->GetUseNegativeGradient(),
                               metric_->GetDoResampling()};
-        skm_optimizer_options oo{kind, a_, A_, alpha_, fmax_, fmin_, omega_, orthant_, (int)iterations_, return_best_};
+        skm_optimizer_options oo{kind, a_, A_, alpha_, fmax_, fmin_, omega_, orthant_, (int)iterations_, return_best_,
+                                 0};
         std::vector<skm_iteration_record> rec(iterations_ ? iterations_ : 1);
         auto *ev = t->GetGpuKernelEvaluator();
         if (metric_->GetRegularizerRKHS() > 0) ev->SetRegularizerRKHS(metric_->GetRegularizerRKHS());

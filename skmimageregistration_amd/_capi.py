@@ -61,7 +61,7 @@ class SkmOptimizerOptions(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("a", ctypes.c_double), ("A", ctypes.c_double),
                 ("alpha", ctypes.c_double), ("fmax", ctypes.c_double), ("fmin", ctypes.c_double),
                 ("omega", ctypes.c_double), ("orthant_projection", ctypes.c_int32),
-                ("iterations", ctypes.c_int32), ("return_best", ctypes.c_int32)]
+                ("iterations", ctypes.c_int32), ("return_best", ctypes.c_int32), ("graph", ctypes.c_int32)]
 
 
 SKM_NUM_PASSES = 10

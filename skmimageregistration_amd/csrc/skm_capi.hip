@@ -124,9 +124,10 @@ struct skm_evaluator {
     float *d_cp_row_lam = nullptr, *d_smp_row_lam = nullptr, *d_vox_row_lam = nullptr, *d_cp_reach = nullptr;
     int *d_cp_window = nullptr, *d_reg_window = nullptr, *d_lcc_window = nullptr, *d_dense_window = nullptr;
     int *d_offsets = nullptr;
+    uint64_t *d_rng = nullptr;  // PCG32 state of the cube offsets
     float4 *d_sg = nullptr, *d_sv = nullptr;
     float *d_sb = nullptr;
-    double *d_partials = nullptr, *d_reg_partials = nullptr, *d_hreg_partials = nullptr, *d_dot_partials = nullptr;
+    double *d_partials = nullptr, *d_reg_partials = nullptr, *d_hreg_partials = nullptr;
     double *d_gather_partials = nullptr;
     Scalars *d_sc = nullptr;
     float *d_deriv = nullptr, *d_cp_in = nullptr, *d_cpw_in = nullptr, *d_dst = nullptr;
@@ -136,7 +137,7 @@ struct skm_evaluator {
     skm_iteration_record *d_records = nullptr;
     int records_cap = 0;
 
-    uint64_t rng_state = 0, rng_inc = 1;
+    uint64_t rng_inc = 1;
     bool evaluated_once = false;
 
     // pass timing: events recorded around each pass while enabled
@@ -155,8 +156,8 @@ struct skm_evaluator {
                         d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_reach, d_cp_window,
                         d_reg_window,
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
-                        d_hreg_partials, d_dot_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
-                        d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records};
+                        d_hreg_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
+                        d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -216,13 +217,18 @@ skm_status dev_upload(T **p, const T *src, size_t count)
     return SKM_SUCCESS;
 }
 
-// A device buffer holding a copy of src (a host or, with device, a device array).
+// A device buffer holding a copy of src (a host or, with device, a device
+// array); the copy has completed on return (the evaluator's own stream does
+// not wait for the null stream).
 template <typename T>
 skm_status dev_copy_in(T **p, const T *src, size_t count, bool device)
 {
     if (!device) return dev_upload(p, src, count);
     SKM_TRY(dev_alloc(p, count));
-    if (src && count) SKM_HIP(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyDeviceToDevice));
+    if (src && count) {
+        SKM_HIP(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyDeviceToDevice));
+        SKM_HIP(hipStreamSynchronize(nullptr));
+    }
     return SKM_SUCCESS;
 }
 
@@ -461,16 +467,20 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
 {
     Problem &pr = ev->pr;
     hipStream_t s = ev->stream;
+    uint64_t drawn = 0;  // random numbers drawn for the offsets of this evaluation
     if (pr.sub > 1 && (!ev->evaluated_once || do_resampling)) {
         PassTimer t(ev, 0);
-        launch_offsets(pr, ev->d_offsets, ev->rng_state, ev->rng_inc, s);
-        ev->rng_state = lcg_advance(ev->rng_state, (uint64_t)pr.n_sub * pr.dim, ev->rng_inc);
+        launch_offsets(pr, ev->d_offsets, ev->d_rng, ev->rng_inc, s);
+        drawn = (uint64_t)pr.n_sub * pr.dim;
         ev->evaluated_once = true;
     }
     int blocks = 0;
     if (measure == SKM_MEASURE_MSE || measure == SKM_MEASURE_NCC) {
         { PassTimer t(ev, 1); launch_value(pr, measure, ev->d_sg, ev->d_sb, ev->d_partials, &blocks, s); }
-        { PassTimer t(ev, 2); launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s); }
+        {
+            PassTimer t(ev, 2);
+            launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s, ev->d_rng, drawn, ev->rng_inc);
+        }
         if (deriv) {
             { PassTimer t(ev, 3); launch_prepare(pr, measure, ev->d_sg, ev->d_sb, ev->d_sc, ev->d_sv, s); }
             { PassTimer t(ev, 4); launch_gather(pr, ev->d_sv, deriv, s); }
@@ -479,7 +489,10 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
         SKM_TRY(ensure_lcc_buffers(ev));
         { PassTimer t(ev, 6); launch_dense_warp(pr, 0, ev->d_dense, s); }
         { PassTimer t(ev, 7); launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s); }
-        { PassTimer t(ev, 2); launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s); }
+        {
+            PassTimer t(ev, 2);
+            launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s, ev->d_rng, drawn, ev->rng_inc);
+        }
         if (deriv) {
             PassTimer t(ev, 8);
             if (pr.exact) {
@@ -631,7 +644,6 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         for (int k = 0; k < dim; ++k) mec[i * dim + k] = desc->maximum_eigenvalue_covariance[i * 3 + k];
     std::memcpy(ev->desc.maximum_eigenvalue_covariance, mec, sizeof(mec));
     ev->dim = dim;
-    ev->rng_state = desc->rng_state;
     ev->rng_inc = desc->rng_sequence | 1ULL;
     auto cleanup = [&](skm_status st) {
         delete ev;
@@ -711,8 +723,9 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         (st = dev_alloc(&ev->d_sv, pr.n_sub)) || (st = dev_alloc(&ev->d_partials, (size_t)tiles * kPartialsK)) ||
         (st = dev_alloc(&ev->d_reg_partials, (size_t)reg_blocks(pr.n_cp) * 4 + 4)) ||
         (st = dev_alloc(&ev->d_hreg_partials, (size_t)((pr.n_cp + kThreads - 1) / kThreads) * 4 + 4)) ||
-        (st = dev_alloc(&ev->d_dot_partials, 64)) || (st = dev_alloc(&ev->d_sc, 1)) ||
-        (st = dev_alloc(&ev->d_offsets, pr.sub > 1 ? pr.n_sub * dim : 1)))
+        (st = dev_alloc(&ev->d_sc, 1)) ||
+        (st = dev_alloc(&ev->d_offsets, pr.sub > 1 ? pr.n_sub * dim : 1)) ||
+        (st = dev_upload(&ev->d_rng, &desc->rng_state, 1)))
         return cleanup(st);
     pr.fixed_data = ev->d_fixed;
     pr.coeffs = ev->d_coeffs;
@@ -1041,15 +1054,52 @@ SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *met
     SKM_HIP(hipMemcpyAsync(ev->d_cpw_in, cpw, sizeof(float) * pr.n_cp, hipMemcpyHostToDevice, s));
     SKM_HIP(hipMemsetAsync(ev->d_prev_g, 0, sizeof(float) * np, s));
     SKM_HIP(hipMemsetAsync(ev->d_opt, 0, sizeof(OptState), s));
-    for (int it = 0; it < opt->iterations; ++it) {
-        launch_soa_to_cpc(pr, ev->d_params, ev->d_cpw_in, ev->d_cpc, s);
+    // one iteration: every value it needs lives on the device (parameters and
+    // their packed control points, optimizer state and iteration count,
+    // random stream), so the launches of one iteration serve every other one;
+    // the update packs the control points of the next evaluation
+    launch_soa_to_cpc(pr, ev->d_params, ev->d_cpw_in, ev->d_cpc, s);
+    auto iteration = [&]() -> skm_status {
         SKM_TRY(run_eval(ev, metric->measure, metric->do_resampling, ev->d_deriv));
         launch_host_regs_device(pr, ev->d_params, metric->reg_l1, metric->reg_l21, metric->reg_l2,
-                                metric->use_negative_gradient, ev->d_deriv, ev->d_hreg_partials, s);
+                                metric->use_negative_gradient, ev->d_deriv, ev->d_prev_g, ev->d_hreg_partials, s);
         launch_opt_step(pr, opt->kind, opt->a, opt->A, opt->alpha, opt->fmax, opt->fmin, opt->omega,
-                        opt->orthant_projection, ev->d_params, ev->d_deriv, ev->d_prev_g, ev->d_dot_partials,
-                        ev->d_opt, ev->d_sc, ev->d_hreg_partials, metric->reg_l1, metric->reg_l21, metric->reg_l2,
-                        ev->reg_w, ev->d_records, opt->return_best ? ev->d_best : nullptr, s);
+                        opt->orthant_projection, ev->d_params, ev->d_deriv, ev->d_prev_g, ev->d_opt, ev->d_sc,
+                        ev->d_hreg_partials, metric->reg_l1, metric->reg_l21, metric->reg_l2, ev->reg_w,
+                        ev->d_records, opt->return_best ? ev->d_best : nullptr, ev->d_cpw_in, ev->d_cpc, s);
+        return SKM_SUCCESS;
+    };
+    // with graph replay the first iteration runs kernel by kernel (it
+    // allocates what a measure needs on first use and draws the first
+    // offsets) and the rest replay a capture of the second one; pass timing
+    // needs the kernel-by-kernel launches
+    const bool replay = opt->graph && !ev->timing && opt->iterations > 2;
+    int it = 0;
+    for (; it < (replay ? 1 : opt->iterations); ++it) SKM_TRY(iteration());
+    if (replay) {
+        SKM_HIP(hipGetLastError());
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        SKM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const skm_status captured = iteration();
+        const hipError_t ended = hipStreamEndCapture(s, &graph);
+        auto release = [&]() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+        };
+        if (captured != SKM_SUCCESS) {
+            release();
+            return captured;
+        }
+        if (ended != hipSuccess) {
+            release();
+            return fail(SKM_ERROR_HIP, std::string("optimizer capture: ") + hipGetErrorString(ended));
+        }
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        for (; e == hipSuccess && it < opt->iterations; ++it) e = hipGraphLaunch(exec, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        release();
+        if (e != hipSuccess) return fail(SKM_ERROR_HIP, std::string("optimizer replay: ") + hipGetErrorString(e));
     }
     SKM_HIP(hipGetLastError());
     // with return_best (and at least one evaluation) the lowest-value position

@@ -60,12 +60,15 @@ struct Scalars {
 enum { kPartialsK = 8 };
 
 // ---- launchers (skm_kernels.hip) ----
-void launch_offsets(const Problem &pr, int *offsets, uint64_t state, uint64_t inc, hipStream_t s);
+// Draws the cube offsets from the PCG32 state at *state (device); the
+// evaluation's launch_reduce advances the state past them.
+void launch_offsets(const Problem &pr, int *offsets, uint64_t *state, uint64_t inc, hipStream_t s);
 void launch_pack_cp(const Problem &pr, const float *cp, const float *cpw, int soa, float4 *cpc, hipStream_t s);
 void launch_value(const Problem &pr, int measure, float4 *sg, float *sb, double *partials, int *blocks_out,
                   hipStream_t s);
+// Also moves the PCG32 state at *rng rng_delta draws ahead (rng_delta 0: untouched).
 void launch_reduce(const Problem &pr, int measure, const double *partials, int blocks, Scalars *sc,
-                   hipStream_t s);
+                   hipStream_t s, uint64_t *rng = nullptr, uint64_t rng_delta = 0, uint64_t rng_inc = 1);
 void launch_prepare(const Problem &pr, int measure, const float4 *sg, const float *sb, const Scalars *sc,
                     float4 *sv, hipStream_t s);
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s);
@@ -125,13 +128,17 @@ struct OptState {
     double best_value;     // lowest metric value evaluated so far
     int improved;          // this iteration's value is the lowest so far
 };
+// L1 / L2,1 / L2 subgradients and the sign flip on the device derivative,
+// with block partials of the three values and of <g, prev_grad>.
 void launch_host_regs_device(const Problem &pr, const float *params_soa, double l1, double l21, double l2,
-                             int negate, float *deriv_soa, double *partials, hipStream_t s);
+                             int negate, float *deriv_soa, const float *prev_grad, double *partials, hipStream_t s);
+// Step scalars and record, then the update of params_soa (best position kept
+// when best_params is set) and the packed control points of the next step.
 void launch_opt_step(const Problem &pr, int kind, double a, double A, double alpha, double fmax, double fmin,
                      double omega, int orthant, float *params_soa, const float *grad_soa, float *prev_grad,
-                     double *partials, OptState *st, const Scalars *sc, const double *reg_partials,
-                     double l1w, double l21w, double l2w, const double *gpu_reg_weights, void *record,
-                     float *best_params, hipStream_t s);
+                     OptState *st, const Scalars *sc, const double *reg_partials, double l1w, double l21w,
+                     double l2w, const double *gpu_reg_weights, void *record, float *best_params, const float *cpw,
+                     float4 *cpc, hipStream_t s);
 void launch_soa_to_cpc(const Problem &pr, const float *params_soa, const float *cpw, float4 *cpc,
                        hipStream_t s);
 

@@ -371,7 +371,7 @@ __global__ __launch_bounds__(SPLIT > 1 ? 64 * SPLIT : kThreads) __attribute__((a
 // Host reduction of GpuEvaluator.cu:192-200 (MSE), 229-262 / 501-534 (NCC) and
 // 287-297 (LCC), done by one block in a fixed order.
 __global__ __launch_bounds__(kThreads) void k_reduce(int measure, const double *__restrict__ partials, int blocks,
-                                                     Scalars *sc)
+                                                     Scalars *sc, uint64_t *rng, uint64_t rng_delta, uint64_t rng_inc)
 {
     __shared__ double lds[kPartialsK * 4];
     double acc[kPartialsK];
@@ -408,6 +408,9 @@ __global__ __launch_bounds__(kThreads) void k_reduce(int measure, const double *
         sc->coef_scale = 1.0 / n;
     }
     sc->reg[0] = sc->reg[1] = sc->reg[2] = 0.0;
+    // past the offsets this evaluation drew (k_offsets has finished: the
+    // value pass ran in between)
+    if (rng_delta) *rng = lcg_advance(*rng, rng_delta, rng_inc);
 }
 
 // Per-sample derivative vector v = coef * gradM with coef folding the host
@@ -945,14 +948,16 @@ __global__ __launch_bounds__(kThreads) void k_voxel_states(Problem pr, float4 *_
 
 // PCG32 cube offsets: cell i draws numbers i*D .. i*D+D-1 of the stream, so
 // every thread jumps ahead in O(log n) and the result is bit-identical to the
-// serial host loop (GpuEvaluator.cu:107-113; UniformRng.h:34-55).
+// serial host loop (GpuEvaluator.cu:107-113; UniformRng.h:34-55).  The stream
+// state lives in device memory (k_reduce of the same evaluation moves it past
+// the n*D draws), so every launch of a recorded evaluation draws anew.
 template <int DIM>
-__global__ __launch_bounds__(kThreads) void k_offsets(long long n, int sub, uint64_t state, uint64_t inc,
-                                                      int *__restrict__ out)
+__global__ __launch_bounds__(kThreads) void k_offsets(long long n, int sub, const uint64_t *__restrict__ state,
+                                                      uint64_t inc, int *__restrict__ out)
 {
     long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
     if (i >= n) return;
-    uint64_t st = lcg_advance(state, (uint64_t)i * DIM, inc);
+    uint64_t st = lcg_advance(*state, (uint64_t)i * DIM, inc);
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         uint32_t r = pcg32_output(st);
@@ -1009,7 +1014,7 @@ static unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) /
         }                                                                                                          \
     } while (0)
 
-void launch_offsets(const Problem &pr, int *offsets, uint64_t state, uint64_t inc, hipStream_t s)
+void launch_offsets(const Problem &pr, int *offsets, uint64_t *state, uint64_t inc, hipStream_t s)
 {
     if (pr.dim == 2) k_offsets<2><<<blocks_for(pr.n_sub), kThreads, 0, s>>>(pr.n_sub, pr.sub, state, inc, offsets);
     else k_offsets<3><<<blocks_for(pr.n_sub), kThreads, 0, s>>>(pr.n_sub, pr.sub, state, inc, offsets);
@@ -1061,9 +1066,10 @@ void launch_value(const Problem &pr, int measure, float4 *sg, float *sb, double 
     SKM_DISPATCH_DIM_MODE(pr, value_impl, pr, measure, sg, sb, partials, blocks_out, s);
 }
 
-void launch_reduce(const Problem &pr, int measure, const double *partials, int blocks, Scalars *sc, hipStream_t s)
+void launch_reduce(const Problem &pr, int measure, const double *partials, int blocks, Scalars *sc, hipStream_t s,
+                   uint64_t *rng, uint64_t rng_delta, uint64_t rng_inc)
 {
-    k_reduce<<<1, kThreads, 0, s>>>(measure, partials, blocks, sc);
+    k_reduce<<<1, kThreads, 0, s>>>(measure, partials, blocks, sc, rng, rng_delta, rng_inc);
 }
 
 void launch_prepare(const Problem &pr, int measure, const float4 *sg, const float *sb, const Scalars *sc,
@@ -1170,7 +1176,8 @@ void launch_result(const Scalars *sc, void *result, hipStream_t s)
 // (itkRegularizedImageToImageMetricv4.hxx:430-529), one thread per cp.
 __global__ __launch_bounds__(kThreads) void k_host_regs(long long ncp, int dim, const float *__restrict__ th,
                                                         double l1w, double l21w, double l2w, int negate,
-                                                        float *__restrict__ der, double *__restrict__ partials)
+                                                        float *__restrict__ der, const float *__restrict__ prev_g,
+                                                        double *__restrict__ partials)
 {
     __shared__ double lds[4 * 4];
     long long j = blockIdx.x * (long long)kThreads + threadIdx.x;
@@ -1203,7 +1210,13 @@ __global__ __launch_bounds__(kThreads) void k_host_regs(long long ncp, int dim, 
                 else g[d] = 0.0;
             }
         }
-        for (int d = 0; d < dim; ++d) der[j + d * ncp] = (float)(negate ? -g[d] : g[d]);
+        for (int d = 0; d < dim; ++d) {
+            const float gd = (float)(negate ? -g[d] : g[d]);
+            der[j + d * ncp] = gd;
+            // <g_k, g_{k-1}> of the adaptive step (CalculateScalarProduct,
+            // itkAdaptiveStepGradientDescentOptimizerv4.hxx:100-116)
+            vals[3] += (double)gd * (double)prev_g[j + d * ncp];
+        }
     }
     block_sum<4>(vals, lds);
     if (threadIdx.x == 0)
@@ -1211,23 +1224,10 @@ __global__ __launch_bounds__(kThreads) void k_host_regs(long long ncp, int dim, 
 }
 
 void launch_host_regs_device(const Problem &pr, const float *params_soa, double l1, double l21, double l2,
-                             int negate, float *deriv_soa, double *partials, hipStream_t s)
+                             int negate, float *deriv_soa, const float *prev_grad, double *partials, hipStream_t s)
 {
     k_host_regs<<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr.n_cp, pr.dim, params_soa, l1, l21, l2, negate,
-                                                         deriv_soa, partials);
-}
-
-// <g_k, g_{k-1}> partials for the adaptive step (CalculateScalarProduct,
-// itkAdaptiveStepGradientDescentOptimizerv4.hxx:100-116).
-__global__ __launch_bounds__(kThreads) void k_dot(long long n, const float *__restrict__ a,
-                                                  const float *__restrict__ b, double *__restrict__ partials)
-{
-    __shared__ double lds[1 * 4];
-    long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
-    double v[1] = {0.0};
-    for (; i < n; i += (long long)gridDim.x * kThreads) v[0] += (double)a[i] * (double)b[i];
-    block_sum<1>(v, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+                                                         deriv_soa, prev_grad, partials);
 }
 
 struct StepParams {
@@ -1235,18 +1235,27 @@ struct StepParams {
     double a, A, alpha, fmax, fmin, omega;
     double l1w, l21w, l2w;
     double rw[3];
-    int dot_blocks, reg_blocks;
+    int reg_blocks;
 };
 
 // Learning rate for this step and the adaptive time update
 // (EstimateLearningRate, Standard...hxx:159-166, Adaptive...hxx:86-97;
 // ModifyCurrentTime, Adaptive...hxx:118-128), plus the iteration record.
-__global__ void k_step_scalars(StepParams sp, const double *__restrict__ dot_partials,
-                               const double *__restrict__ reg_partials, OptState *st, const Scalars *sc,
-                               skm_iteration_record *rec)
+// One wave: the lanes add the block partials of k_host_regs (L1, L2,1, L2
+// and the scalar product) in a fixed order, lane 0 does the rest.
+__global__ __launch_bounds__(64) void k_step_scalars(StepParams sp, const double *__restrict__ reg_partials,
+                                                     OptState *st, const Scalars *sc, skm_iteration_record *rec)
 {
-    double dot = 0.0;
-    for (int b = 0; b < sp.dot_blocks; ++b) dot += dot_partials[b];
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < sp.reg_blocks; b += 64)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[k] += reg_partials[b * 4LL + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) part[k] += __shfl_xor(part[k], off, 64);
+    if (threadIdx.x != 0) return;
+    const double dot = part[3];
     double rate;
     if (sp.kind == SKM_OPT_STANDARD) rate = pow(sp.a / (sp.A + (double)st->iteration + 1.0), sp.alpha);
     else if (sp.kind == SKM_OPT_ADAPTIVE) rate = sp.a / pow(sp.A + st->time + 1.0, sp.alpha);
@@ -1258,12 +1267,7 @@ __global__ void k_step_scalars(StepParams sp, const double *__restrict__ dot_par
         st->time = fmax(0.0, st->time + f);
     }
     st->prev_dot = dot;
-    double l1 = 0, l21 = 0, l2 = 0;
-    for (int b = 0; b < sp.reg_blocks; ++b) {
-        l1 += reg_partials[b * 4LL + 0];
-        l21 += reg_partials[b * 4LL + 1];
-        l2 += reg_partials[b * 4LL + 2];
-    }
+    const double l1 = part[0], l21 = part[1], l2 = part[2];
     double reg = 0.0;
     if (sp.l1w > 0) reg += sp.l1w * l1;
     if (sp.l21w > 0) reg += sp.l21w * l21;
@@ -1283,45 +1287,46 @@ __global__ void k_step_scalars(StepParams sp, const double *__restrict__ dot_par
     st->iteration += 1;
 }
 
-// Copy of the position just evaluated when its value is the lowest so far
-// (ITK's ReturnBestParametersAndValue).
-__global__ __launch_bounds__(kThreads) void k_keep_best(long long n, const float *__restrict__ th,
-                                                        float *__restrict__ best, const OptState *st)
-{
-    long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
-    if (i >= n || !st->improved) return;
-    best[i] = th[i];
-}
-
 // theta <- theta + rate * d with orthant projection; d is kept for the next
-// scalar product (ModifyGradientByLearningRateOverSubRange, Standard...hxx:133-143).
-__global__ __launch_bounds__(kThreads) void k_update(long long n, float *__restrict__ th, const float *__restrict__ g,
-                                                     float *__restrict__ prev_g, const OptState *st, int orthant)
+// scalar product (ModifyGradientByLearningRateOverSubRange, Standard...hxx:
+// 133-143).  One thread per control point: first the copy of the position
+// just evaluated when its value is the lowest so far (ITK's
+// ReturnBestParametersAndValue; best may be null), then the step, then the
+// new position packed as the next evaluation's control points.
+__global__ __launch_bounds__(kThreads) void k_update(long long ncp, int dim, float *__restrict__ th,
+                                                     const float *__restrict__ g, float *__restrict__ prev_g,
+                                                     const OptState *st, int orthant, float *__restrict__ best,
+                                                     const float *__restrict__ cpw, float4 *__restrict__ cpc)
 {
-    long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
-    if (i >= n) return;
-    double rate = st->rate;
-    double t = th[i];
-    double v = t + (double)g[i] * rate;
-    if (orthant && v * t < 0) v = 0.0;
-    th[i] = (float)v;
-    prev_g[i] = g[i];
+    const long long j = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (j >= ncp) return;
+    const double rate = st->rate;
+    const bool keep = best && st->improved;
+    float c[3] = {0.0f, 0.0f, 0.0f};
+    for (int d = 0; d < dim; ++d) {
+        const long long i = j + d * ncp;
+        const double t = th[i];
+        if (keep) best[i] = th[i];
+        double v = t + (double)g[i] * rate;
+        if (orthant && v * t < 0) v = 0.0;
+        th[i] = (float)v;
+        c[d] = (float)v;
+        prev_g[i] = g[i];
+    }
+    cpc[j] = make_float4(c[0], c[1], c[2], cpw[j]);
 }
 
 void launch_opt_step(const Problem &pr, int kind, double a, double A, double alpha, double fmax, double fmin,
                      double omega, int orthant, float *params_soa, const float *grad_soa, float *prev_grad,
-                     double *partials, OptState *st, const Scalars *sc, const double *reg_partials, double l1w,
-                     double l21w, double l2w, const double *gpu_reg_weights, void *record, float *best_params,
-                     hipStream_t s)
+                     OptState *st, const Scalars *sc, const double *reg_partials, double l1w, double l21w,
+                     double l2w, const double *gpu_reg_weights, void *record, float *best_params, const float *cpw,
+                     float4 *cpc, hipStream_t s)
 {
-    long long n = pr.n_cp * pr.dim;
-    int dot_blocks = 64;
-    k_dot<<<dot_blocks, kThreads, 0, s>>>(n, grad_soa, prev_grad, partials);
     StepParams sp{kind, a, A, alpha, fmax, fmin, omega, l1w, l21w, l2w,
-                  {gpu_reg_weights[0], gpu_reg_weights[1], gpu_reg_weights[2]}, dot_blocks, (int)blocks_for(pr.n_cp)};
-    k_step_scalars<<<1, 1, 0, s>>>(sp, partials, reg_partials, st, sc, (skm_iteration_record *)record);
-    if (best_params) k_keep_best<<<blocks_for(n), kThreads, 0, s>>>(n, params_soa, best_params, st);
-    k_update<<<blocks_for(n), kThreads, 0, s>>>(n, params_soa, grad_soa, prev_grad, st, orthant);
+                  {gpu_reg_weights[0], gpu_reg_weights[1], gpu_reg_weights[2]}, (int)blocks_for(pr.n_cp)};
+    k_step_scalars<<<1, 64, 0, s>>>(sp, reg_partials, st, sc, (skm_iteration_record *)record);
+    k_update<<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr.n_cp, pr.dim, params_soa, grad_soa, prev_grad, st, orthant,
+                                                      best_params, cpw, cpc);
 }
 
 void launch_soa_to_cpc(const Problem &pr, const float *params_soa, const float *cpw, float4 *cpc, hipStream_t s)

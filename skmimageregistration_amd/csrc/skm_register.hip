@@ -428,7 +428,7 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
                               P.reg_pg <= 0;
         skm_metric_options mo{desc->metric, P.reg_l1, P.reg_l21, P.reg_l2, 1, 1};
         skm_optimizer_options oo{SKM_OPT_ADAPTIVE, P.initial_step_size, 20.0, 1.0, 1e7, -1e6, 1.0,
-                                 (only_l1 || only_l21) ? 1 : 0, P.iterations, 1};
+                                 (only_l1 || only_l21) ? 1 : 0, P.iterations, 1, 0};
         std::vector<float> params((size_t)n_cp * dim, 0.0f), cpw((size_t)n_cp, 1.0f);
         std::vector<skm_iteration_record> rec(std::max(1, P.iterations));
         SKM_R_TRY(skm_optimize(ev, &mo, &oo, cpw.data(), params.data(), rec.data()));

@@ -514,6 +514,7 @@ skm_status skm::gaussian_smooth_device(int dim, const skm_grid &grid, const floa
     float *res = nullptr;
     if ((st = gaussian_passes(dim, d, 1, k, 1.0f, a.p, b.p, &res, nullptr)) != SKM_SUCCESS) return st;
     SKM_T_HIP(hipMemcpy(d_out, res, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice));
+    SKM_T_HIP(hipStreamSynchronize(nullptr));  // complete for callers on other streams
     return SKM_SUCCESS;
 }
 

@@ -276,13 +276,13 @@ class GpuKernelEvaluator:
     def optimize(self, params_soa, metric=MEASURE.MSE, reg_l1=0.0, reg_l21=0.0, reg_l2=0.0,
                  use_negative_gradient=True, do_resampling=True, kind=_capi.SKM_OPT_ADAPTIVE, a=0.1, A=20.0,
                  alpha=1.0, fmax=1e7, fmin=-1e6, omega=1.0, orthant_projection=False, iterations=100,
-                 cpwData=None, return_best=False):
+                 cpwData=None, return_best=False, graph=False):
         params = f32(params_soa).copy()
         cpw = f32(cpwData if cpwData is not None else np.ones(self.n_cp, np.float32))
         mo = _capi.SkmMetricOptions(int(metric), reg_l1, reg_l21, reg_l2, int(bool(use_negative_gradient)),
                                     int(bool(do_resampling)))
         oo = _capi.SkmOptimizerOptions(int(kind), a, A, alpha, fmax, fmin, omega, int(bool(orthant_projection)),
-                                       int(iterations), int(bool(return_best)))
+                                       int(iterations), int(bool(return_best)), int(bool(graph)))
         recs = (_capi.SkmIterationRecord * max(1, iterations))()
         check(self._lib.skm_optimize(self._h, ctypes.byref(mo), ctypes.byref(oo), fptr(cpw), fptr(params), recs))
         records = [(r.value, r.measure, r.learning_rate, r.num_pixels_counted) for r in recs[:iterations]]
