@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -39,41 +40,121 @@ long long voxels(const skm_grid &g, int dim)
     return n;
 }
 
-// ThresholdImage (include/itkUtils.h:45-100): values at or above thresh_max
-// become the largest value below it, values at or below thresh_min the
-// smallest value above it (two ThresholdOutside passes bounded by the image's
-// own minimum and maximum).
-void threshold_image(std::vector<float> &img, double thresh_max, double thresh_min)
+// ---- preprocessing (SKMReg.cxx:84-91) on the device
+
+constexpr int kPreThreads = 256;
+constexpr int kPreBlocks = 512;
+
+// ThresholdImage's four bounds, then RescaleImage's map.
+struct PreStats {
+    double below_max, above_min, lo, hi;
+    double scale, shift;
+};
+
+// std::min / std::max of the host restatement (NaN compares false either way).
+__device__ __forceinline__ double min_as_std(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double max_as_std(double a, double b) { return a < b ? b : a; }
+
+// Block partials of the image's minimum and maximum and, with thresholds, of
+// the largest value below thresh_max and the smallest above thresh_min.
+__global__ __launch_bounds__(kPreThreads) void k_pre_partials(const float *__restrict__ v, long long n,
+                                                              double thresh_max, double thresh_min,
+                                                              PreStats *__restrict__ part)
 {
-    double below_max = std::numeric_limits<double>::lowest(), above_min = std::numeric_limits<double>::max();
-    double lo = std::numeric_limits<double>::max(), hi = std::numeric_limits<double>::lowest();
-    for (float v : img) {
-        const double x = v;
-        if (below_max < x && x < thresh_max) below_max = x;
-        if (above_min > x && x > thresh_min) above_min = x;
-        lo = std::min(lo, x);
-        hi = std::max(hi, x);
+    __shared__ double s[4][kPreThreads];
+    double bm = -DBL_MAX, am = DBL_MAX, lo = DBL_MAX, hi = -DBL_MAX;
+    for (long long i = blockIdx.x * (long long)kPreThreads + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kPreThreads) {
+        const double x = v[i];
+        if (bm < x && x < thresh_max) bm = x;
+        if (am > x && x > thresh_min) am = x;
+        lo = min_as_std(lo, x);
+        hi = max_as_std(hi, x);
     }
-    for (float &v : img) {
-        if (v < lo || v > below_max) v = (float)below_max;  // ThresholdOutside(min, max), outside value max
+    s[0][threadIdx.x] = bm;
+    s[1][threadIdx.x] = am;
+    s[2][threadIdx.x] = lo;
+    s[3][threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = kPreThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s[0][threadIdx.x] = max_as_std(s[0][threadIdx.x], s[0][threadIdx.x + w]);
+            s[1][threadIdx.x] = min_as_std(s[1][threadIdx.x], s[1][threadIdx.x + w]);
+            s[2][threadIdx.x] = min_as_std(s[2][threadIdx.x], s[2][threadIdx.x + w]);
+            s[3][threadIdx.x] = max_as_std(s[3][threadIdx.x], s[3][threadIdx.x + w]);
+        }
+        __syncthreads();
     }
-    for (float &v : img) {
-        if (v < above_min || v > hi) v = (float)above_min;  // ThresholdOutside(min, image max), value min
+    if (threadIdx.x == 0) part[blockIdx.x] = PreStats{s[0][0], s[1][0], s[2][0], s[3][0], 0.0, 0.0};
+}
+
+// One thread: the bounds of the whole image into part[blocks], and the
+// rescale map [lo, hi] -> [0, 1] (itkUtils.h:103-112), the product and the
+// sum rounded one at a time as on the host.
+__global__ void k_pre_final(PreStats *part, int blocks)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    PreStats r{-DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, 0.0, 0.0};
+    for (int b = 0; b < blocks; ++b) {
+        r.below_max = max_as_std(r.below_max, part[b].below_max);
+        r.above_min = min_as_std(r.above_min, part[b].above_min);
+        r.lo = min_as_std(r.lo, part[b].lo);
+        r.hi = max_as_std(r.hi, part[b].hi);
+    }
+    if (r.hi != r.lo) r.scale = 1.0 / (r.hi - r.lo);
+    else if (r.hi != 0.0) r.scale = 1.0 / r.hi;
+    {
+#pragma clang fp contract(off)
+        r.shift = 0.0 - r.lo * r.scale;
+    }
+    part[blocks] = r;
+}
+
+// ThresholdImage's two ThresholdOutside passes (include/itkUtils.h:45-100):
+// values outside [min, largest below thresh_max] become that largest value,
+// then values outside [smallest above thresh_min, max] that smallest value.
+__global__ __launch_bounds__(kPreThreads) void k_pre_threshold(float *__restrict__ v, long long n,
+                                                               const PreStats *__restrict__ st)
+{
+    const PreStats r = *st;
+    for (long long i = blockIdx.x * (long long)kPreThreads + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kPreThreads) {
+        float x = v[i];
+        if (x < r.lo || x > r.below_max) x = (float)r.below_max;
+        if (x < r.above_min || x > r.hi) x = (float)r.above_min;
+        v[i] = x;
     }
 }
 
-// RescaleImage(image, 0, 1) (itkUtils.h:103-112): RescaleIntensityImageFilter,
-// out = in * scale + shift with scale = (1 - 0) / (max - min).
-void rescale_image(std::vector<float> &img, double out_min, double out_max)
+// RescaleImage(image, 0, 1): out = clamp(in * scale + shift, 0, 1).
+__global__ __launch_bounds__(kPreThreads) void k_pre_rescale(float *__restrict__ v, long long n,
+                                                             const PreStats *__restrict__ st)
 {
-    if (img.empty()) return;
-    const auto mm = std::minmax_element(img.begin(), img.end());
-    const double lo = *mm.first, hi = *mm.second;
-    double scale = 0.0;
-    if (hi != lo) scale = (out_max - out_min) / (hi - lo);
-    else if (hi != 0.0) scale = (out_max - out_min) / hi;
-    const double shift = out_min - lo * scale;
-    for (float &v : img) v = (float)std::min(out_max, std::max(out_min, (double)v * scale + shift));
+#pragma clang fp contract(off)  // a product and a sum, each rounded, as on the host
+    const double scale = st->scale, shift = st->shift;
+    for (long long i = blockIdx.x * (long long)kPreThreads + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kPreThreads)
+        v[i] = (float)min_as_std(1.0, max_as_std(0.0, (double)v[i] * scale + shift));
+}
+
+// ThresholdImage (with thresholds) then RescaleImage(0, 1) of a device image;
+// stats holds kPreBlocks + 1 entries.
+skm_status preprocess_device(float *d_img, long long n, bool thresholds, double thresh_max, double thresh_min,
+                             PreStats *stats)
+{
+    const int blocks = (int)std::min<long long>(kPreBlocks, (n + kPreThreads - 1) / kPreThreads);
+    if (thresholds) {
+        k_pre_partials<<<blocks, kPreThreads>>>(d_img, n, thresh_max, thresh_min, stats);
+        k_pre_final<<<1, 64>>>(stats, blocks);
+        k_pre_threshold<<<blocks, kPreThreads>>>(d_img, n, stats + blocks);
+    }
+    k_pre_partials<<<blocks, kPreThreads>>>(d_img, n, 0.0, 0.0, stats);
+    k_pre_final<<<1, 64>>>(stats, blocks);
+    k_pre_rescale<<<blocks, kPreThreads>>>(d_img, n, stats + blocks);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("preprocessing: ") + hipGetErrorString(e));
+    return SKM_SUCCESS;
 }
 
 // Largest eigenvalue of the symmetric part of a dim x dim matrix (FP64).
@@ -254,16 +335,23 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
     const long long n_ref = voxels(desc->reference, dim), n_tar = voxels(desc->target, dim);
     if (n_ref < 1 || n_tar < 1) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "empty image");
 
-    // preprocessing (SKMReg.cxx:84-91)
-    std::vector<float> ref(desc->reference_data, desc->reference_data + n_ref);
-    std::vector<float> tar(desc->target_data, desc->target_data + n_tar);
-    if (desc->use_intensity_thresholds) {
-        threshold_image(ref, desc->threshold_max, desc->threshold_min);
-        threshold_image(tar, desc->threshold_max, desc->threshold_min);
-    }
-    rescale_image(ref, 0.0, 1.0);
-    rescale_image(tar, 0.0, 1.0);
+    // preprocessing (SKMReg.cxx:84-91) on the device copies of the images; the
+    // host copies (result files and warped outputs only) are read back when
+    // files are written
     const Output out{desc->temp_directory ? std::string(desc->temp_directory) : std::string(), dim};
+    DeviceArray<float> d_ref, d_tar, d_ref_s, d_tar_s, d_weight, d_guide, d_tensor, d_prev, d_cp, d_cpw;
+    DeviceArray<PreStats> d_pre;
+    SKM_R_TRY(d_ref.upload(desc->reference_data, (size_t)n_ref));
+    SKM_R_TRY(d_tar.upload(desc->target_data, (size_t)n_tar));
+    SKM_R_TRY(d_pre.alloc(kPreBlocks + 1));
+    const bool thresholds = desc->use_intensity_thresholds != 0;
+    SKM_R_TRY(preprocess_device(d_ref.p, n_ref, thresholds, desc->threshold_max, desc->threshold_min, d_pre.p));
+    SKM_R_TRY(preprocess_device(d_tar.p, n_tar, thresholds, desc->threshold_max, desc->threshold_min, d_pre.p));
+    std::vector<float> ref, tar;
+    if (out.on()) {
+        SKM_R_TRY(d_ref.download(ref));
+        SKM_R_TRY(d_tar.download(tar));
+    }
     SKM_R_TRY(out.write("reference-preprocessed.vtk", desc->reference, 1, ref.data()));
     SKM_R_TRY(out.write("target-preprocessed.vtk", desc->target, 1, tar.data()));
     const bool landmarks = desc->num_landmarks > 0 && desc->reference_landmarks && desc->target_landmarks;
@@ -291,10 +379,7 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
     }
     const long long n_w = voxels(desc->weight, dim);
 
-    // device copies of the inputs, made once
-    DeviceArray<float> d_ref, d_tar, d_ref_s, d_tar_s, d_weight, d_guide, d_tensor, d_prev, d_cp, d_cpw;
-    SKM_R_TRY(d_ref.upload(ref.data(), ref.size()));
-    SKM_R_TRY(d_tar.upload(tar.data(), tar.size()));
+    // device copies of the other inputs, made once
     if (desc->weight_image)
         SKM_R_TRY(d_weight.upload(desc->weight_image, (size_t)n_w));
     else if (desc->weight_tensor)

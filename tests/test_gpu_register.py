@@ -135,3 +135,29 @@ def test_registration_recovers_the_warp():
     err_after = np.abs(df[inner] - disp[inner]).mean()
     print(f"mean |d| {err_before:.3f} -> mean |u - d| {err_after:.3f}", flush=True)
     assert err_after < 0.7 * err_before
+
+
+@pytest.mark.parametrize("thresholds", [[0.05, 0.8], []], ids=["clipping", "none"])
+def test_preprocessed_images_equal_the_host_restatement(tmp_path, thresholds):
+    """The driver thresholds and rescales on the device; its
+    reference-/target-preprocessed files must equal ThresholdImage and
+    RescaleImage restated on the host, bit for bit (comparisons and the
+    rescale map in double, one rounding per operation)."""
+    from skmimageregistration_amd import read_image
+    reference, target, _ = _case3d()
+    rng = np.random.default_rng(8)
+    # intensities off [0, 1] and with repeats, so that both bounds matter
+    ref = (reference.data * 700.0 - 120.0 + rng.normal(size=reference.data.shape) * 3.0).astype(np.float32)
+    tar = (target.data * 650.0 - 90.0).astype(np.float32)
+    cfg = RegistrationConfig(sigma_C4=[16], sigma_C0=[16], tensor_weights=[0], num_scales=1,
+                             num_function_evaluations=[2], initial_step_size=[0.5], sampling_rate=[2],
+                             smoothing_variance=[0], metric="ncc",
+                             intensity_thresholds=[t * 700.0 - 100.0 for t in thresholds])
+    print(f"preprocessing thresholds={thresholds}", flush=True)
+    register_multiscale(Image(ref), Image(tar), cfg, temp_directory=str(tmp_path))
+    th = cfg["intensity_thresholds"]
+    for name, img in (("reference", ref), ("target", tar)):
+        want = threshold_image(img, th[1], th[0]) if len(th) == 2 else img
+        want = rescale_image(want)
+        got = read_image(str(tmp_path / f"{name}-preprocessed.vtk")).data
+        np.testing.assert_array_equal(got, want)
