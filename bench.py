@@ -52,6 +52,7 @@ def parse():
                     help="RD weight of the registration (the bilateral-like regularizer)")
     ap.add_argument("--no-pair-count", action="store_true", help="skip the exact k != 0 pair count")
     ap.add_argument("--cpu-crop", default=None, help="z,y,x voxels of the CPU baseline sample")
+    ap.add_argument("--cpu-repeats", type=int, default=3, help="timed CPU baseline evaluations (median)")
     ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (reference dispatch for HBM counters)")
     return ap.parse_args()
@@ -286,7 +287,8 @@ def cpu_baseline(args, mov, fix, ten, mcov):
     crop instead); reported in the same unit."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
-    threads = min(16, os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = min(16, avail, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     threads = orc.load(parallel=True).skmo_set_threads(threads)
     crop = tuple(int(v) for v in args.cpu_crop.split(",")) if args.cpu_crop else tuple(fix.shape)
     crop = tuple(min(c, n) for c, n in zip(crop, fix.shape))
@@ -303,13 +305,32 @@ def cpu_baseline(args, mov, fix, ten, mcov):
     pr = orc.OracleProblem(f, m, (cs, co, (args.cp_spacing,) * 3), cp, sigma_c0=args.sigma, sigma_c4=args.sigma,
                            parallel=True, **kw)
     measure = {"mse": 0, "ncc": 1, "lcc": 2}[args.measure]
-    t0 = time.perf_counter()
-    pr.evaluate(measure, True, 0.0, args.reg_rd, 0.0)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(max(args.cpu_repeats, 1)):
+        t0 = time.perf_counter()
+        pr.evaluate(measure, True, 0.0, args.reg_rd, 0.0)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     v, d, _, _ = pair_counts(args, crop, tc, mcov, None, False)
     return {"value": (v + d) / dt / 1e9, "unit": "GVox*ctrl/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle (OpenMP, {threads} threads), one GetValueAndDerivative on "
-                      f"{crop[2]}x{crop[1]}x{crop[0]} voxels (same kernel, sigma and cp spacing), {dt:.1f} s"}
+            "sample": f"FP64 oracle (OpenMP, {threads} threads), GetValueAndDerivative on "
+                      f"{crop[2]}x{crop[1]}x{crop[0]} voxels (same kernel, sigma and cp spacing), median of "
+                      f"{len(times)}: {dt:.1f} s; host {host_cpu()}"}
+
+
+def host_cpu():
+    """CPU model and the logical CPUs this process may use."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return f"{model}, {n} logical CPUs available"
 
 
 def registration(args, mov, fix, disp):
