@@ -8,7 +8,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "skm.h"
@@ -109,12 +111,66 @@ void pack_upper(int dim, const double *sym, float sdet, float lam, float *ps)
 
 // ---------------------------------------------------------------- evaluator
 
+namespace {
+
+// Streams of evaluators.  Creating and destroying a HIP stream costs about
+// 3 ms on MI355X, more than the rest of an evaluator's set-up, so a
+// destroyed evaluator's stream (synchronised, default priority, non-blocking)
+// is kept for the next evaluator on the same device.
+struct StreamPool {
+    std::mutex mutex;
+    std::vector<std::pair<int, hipStream_t>> idle;  // (device, stream)
+};
+
+StreamPool &stream_pool()
+{
+    static StreamPool pool;
+    return pool;
+}
+
+hipError_t take_stream(hipStream_t *out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    {
+        StreamPool &p = stream_pool();
+        std::lock_guard<std::mutex> lock(p.mutex);
+        for (size_t i = p.idle.size(); i-- > 0;) {
+            if (p.idle[i].first == dev) {
+                *out = p.idle[i].second;
+                p.idle.erase(p.idle.begin() + (std::ptrdiff_t)i);
+                return hipSuccess;
+            }
+        }
+    }
+    return hipStreamCreateWithFlags(out, hipStreamNonBlocking);
+}
+
+void return_stream(int dev, hipStream_t s)
+{
+    if (!s) return;
+    constexpr size_t kKeep = 16;  // idle streams kept per process
+    if (hipStreamSynchronize(s) == hipSuccess) {
+        StreamPool &p = stream_pool();
+        std::lock_guard<std::mutex> lock(p.mutex);
+        if (p.idle.size() < kKeep) {
+            p.idle.emplace_back(dev, s);
+            return;
+        }
+    }
+    (void)hipStreamDestroy(s);
+}
+
+}  // namespace
+
 struct skm_evaluator {
     skm_desc desc{};
     int dim = 3;
     Problem pr{};
     hipStream_t stream = nullptr;
-    bool own_stream = false;
+    bool own_stream = false;  // taken from the stream pool (returned on destruction)
+    int stream_device = 0;
     int max_partial_blocks = 0;
 
     float *d_fixed = nullptr, *d_coeffs = nullptr, *d_weight = nullptr, *d_df = nullptr;
@@ -152,6 +208,8 @@ struct skm_evaluator {
 
     ~skm_evaluator()
     {
+        // work queued on the stream may still read these buffers
+        if (stream) (void)hipStreamSynchronize(stream);
         void *ptrs[] = {d_fixed, d_coeffs, d_weight, d_df, d_cpc, d_cp_state, d_cp_pstate, d_smp_pstate,
                         d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_reach, d_cp_window,
                         d_reg_window,
@@ -162,7 +220,7 @@ struct skm_evaluator {
             if (p) (void)hipFree(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
         for (auto e : event_pool) (void)hipEventDestroy(e);
-        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        if (own_stream && stream) return_stream(stream_device, stream);
     }
 };
 
@@ -650,7 +708,7 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         return st;
     };
     skm_status st;
-    if (hipStreamCreateWithFlags(&ev->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipGetDevice(&ev->stream_device) != hipSuccess || take_stream(&ev->stream) != hipSuccess)
         return cleanup(fail(SKM_ERROR_HIP, "hipStreamCreate failed"));
     ev->own_stream = true;
 
@@ -829,7 +887,7 @@ SKM_API skm_status skm_set_stream(skm_evaluator *ev, void *hip_stream)
 {
     if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
     SKM_HIP(hipStreamSynchronize(ev->stream));
-    if (ev->own_stream) SKM_HIP(hipStreamDestroy(ev->stream));
+    if (ev->own_stream) return_stream(ev->stream_device, ev->stream);
     ev->own_stream = false;
     ev->stream = (hipStream_t)hip_stream;
     return SKM_SUCCESS;
