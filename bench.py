@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--reg-rd", type=float, default=1e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-registration", action="store_true", help="skip the multi-scale registration timing")
+    ap.add_argument("--no-lcc", action="store_true", help="skip the LCC timing on the sliding-motion volumes")
     ap.add_argument("--registration-repeats", type=int, default=3)
     ap.add_argument("--registration-rd", type=float, default=1e-5,
                     help="RD weight of the registration (the bilateral-like regularizer)")
@@ -333,6 +334,40 @@ def host_cpu():
     return f"{model}, {n} logical CPUs available"
 
 
+def lcc_timing():
+    """One LCC GetValueAndDerivative on the sliding-motion workload
+    (256x256x128 two halves shifted 4 voxels plus breathing motion,
+    control-point spacing 5 -> 51x51x26, isotropic Wendland support 20, RD
+    1e-3, subsample 2, neighbourhood stride 6 -> 7^3 neighbours per centre):
+    the reference's windowed derivative (GpuEvaluatorLCC.cu:206-530) and the
+    exact gradient (every centre's neighbours scattered once, then one
+    kernel-sum gather); median of 3 after one untimed evaluation."""
+    from skmimageregistration_amd import MEASURE, GpuKernelEvaluator, Grid, Image, Kernel, synthetic
+    shape, h, s = (128, 256, 256), 5.0, 20.0
+    mov, fix, _ = synthetic.sliding_3d(shape)
+    size, cs, co = geometry(shape, h)
+    n = int(np.prod(cs))
+    cp = np.random.default_rng(0).uniform(-1, 1, (n, 3)).astype(np.float32)
+    out = {"workload": "LCC GetValueAndDerivative, 3D 256x256x128 sliding motion, cp 51x51x26 spacing 5, "
+                       "isotropic Wendland support 20, RD 0.001, subsample 2, neighbourhood stride 6"}
+    for name, exact in (("exact_gradient", True), ("reference_windows", False)):
+        ev = GpuKernelEvaluator(n * 3, 2, 6, Kernel(s, s), Image(mov), Image(fix), Grid(cs, co, (h,) * 3),
+                                exact_gradient=exact)
+        ev.SetRegularizerRD(1e-3)
+        ev.getValueAndDerivative(MEASURE.LCC, cp)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            ev.getValueAndDerivative(MEASURE.LCC, cp)
+            times.append((time.perf_counter() - t0) * 1e3)
+        ev.enable_timing(True)
+        r, _ = ev.getValueAndDerivative(MEASURE.LCC, cp)
+        passes = {k: round(v[0] / max(v[1], 1), 3) for k, v in ev.read_timing().items() if v[1]}
+        ev.close()
+        out[name] = {"ms": round(float(np.median(times)), 2), "pass_ms": passes, "measure": round(r.measure, 6)}
+    return out
+
+
 def registration(args, mov, fix, disp):
     """Wall time of one multi-scale registration (SKMReg's scale loop,
     SKMReg.cxx:123-500): 3 levels with support 4s, 2s, s, control-point
@@ -463,6 +498,8 @@ def main():
         line["tensor_generation_ms"] = round(tensor_ms, 2)
     if not args.no_registration:
         line["registration"] = registration(args, mov, fix, disp)
+    if not args.no_lcc:
+        line["lcc"] = lcc_timing()
     print(json.dumps(line))
 
 
