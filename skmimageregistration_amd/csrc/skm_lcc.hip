@@ -246,37 +246,77 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
         for (int k = 0; k < kPartialsK; ++k) partials[blockIdx.x * (long long)kPartialsK + k] = acc[k];
 }
 
+// Control points per workgroup of the reference-mode derivative: the tile's
+// points share every loaded neighbour (its dense warp, fixed value and
+// neighbourhood weight), which cuts the re-reads of the dense field that
+// bound a one-point-per-workgroup walk (each point's centres and neighbours
+// cover ~80^3 voxels, more than an XCD's L2).  In 3D 16 points per tile
+// (measured faster than 4, 8, 12 and 32 on the 256x256x128 sliding-motion
+// workload: 534, 274, 203, 192 and 240 ms).
+template <int DIM> struct LcTile;
+template <> struct LcTile<2> { static constexpr int X = 2, Y = 2, Z = 1; };
+template <> struct LcTile<3> { static constexpr int X = 2, Y = 2, Z = 4; };
+
 // Reference-mode derivative (evaluateDerivativeLCC, GpuEvaluatorLCC.cu:206-530,
 // 696-961): for cp j, centres in the tight window of p_j, all their
 // neighbours, ka = k(p_j, y) with the neighbour's sigma / covariance for both
 // points (:446-460), no w_j, divided by N on the host (:645-646).
-// One workgroup per control point, one wave per centre, lanes over neighbours.
+// One workgroup per tile of control points: the waves walk the centres of
+// the union of the tile's windows (one centre per wave at a time), the lanes
+// a centre's neighbours, and each pair is added to every tile point whose
+// window holds the centre.  Sums: FP32 per lane, FP32 butterfly per wave,
+// FP64 across the waves in order.
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_lcc_gather_ref(Problem pr, const float4 *__restrict__ dense,
                                                              const float4 *__restrict__ coef,
                                                              const Scalars *__restrict__ sc,
                                                              float *__restrict__ deriv)
 {
-    __shared__ double lds[4 * 4];
-    const long long j = blockIdx.x;
-    int jc[3] = {0, 0, 0};
-    {
-        long long r = j;
-        jc[0] = (int)(r % pr.cp.n[0]); r /= pr.cp.n[0];
-        jc[1] = (int)(r % pr.cp.n[1]); r /= pr.cp.n[1];
-        jc[2] = (int)r;
+    constexpr int TX = LcTile<DIM>::X, TY = LcTile<DIM>::Y, TZ = LcTile<DIM>::Z, T = TX * TY * TZ;
+    __shared__ double lds[(kThreads / 64) * T * 3];
+    const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
+    const int tile = (int)blockIdx.x;
+    const int t0 = tile % ntx, t1 = (tile / ntx) % nty, t2 = tile / (ntx * nty);
+    float p[T][3];
+    int lo[T][3], hi[T][3];
+    bool ok[T];
+    int ulo[3] = {INT_MAX, INT_MAX, DIM == 3 ? INT_MAX : 0}, uhi[3] = {INT_MIN, INT_MIN, DIM == 3 ? INT_MIN : 0};
+#pragma unroll
+    for (int c = 0; c < T; ++c) {
+        const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
+        ok[c] = jc[0] < pr.cp.n[0] && jc[1] < pr.cp.n[1] && (DIM == 2 || jc[2] < pr.cp.n[2]);
+        const long long j = ok[c] ? jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]) : 0;
+        const float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]};
+        p[c][0] = p[c][1] = p[c][2] = 0.0f;
+        to_global<DIM>(pr.cp, fj, p[c]);
+        const int *win = pr.cp_window + j * 6;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[c][d] = d < DIM ? win[d] : 0;
+            hi[c][d] = d < DIM ? win[d + 3] : 0;
+            if (ok[c] && d < DIM) {
+                ulo[d] = min(ulo[d], lo[c][d]);
+                uhi[d] = max(uhi[d], hi[c][d]);
+            }
+        }
     }
-    float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]}, p[3];
-    to_global<DIM>(pr.cp, fj, p);
-    const int *win = pr.cp_window + j * 6;
-    int lo[3] = {win[0], win[1], DIM == 3 ? win[2] : 0}, hi[3] = {win[3], win[4], DIM == 3 ? win[5] : 0};
-    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
-    const int ncent = bx * by * bz;
+    const int bx = uhi[0] - ulo[0] + 1, by = uhi[1] - ulo[1] + 1, bz = uhi[2] - ulo[2] + 1;
+    const int ncent = (bx > 0 && by > 0 && bz > 0) ? bx * by * bz : 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    float acc[3] = {0.0f, 0.0f, 0.0f};
+    float acc[T][3];
+#pragma unroll
+    for (int c = 0; c < T; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0f;
     const int sn = pr.subn;
-    for (int c = wave; c < ncent; c += kThreads / 64) {
-        int mx = lo[0] + c % bx, my = lo[1] + (c / bx) % by, mz = lo[2] + c / (bx * by);
+    for (int ci = wave; ci < ncent; ci += kThreads / 64) {
+        const int mx = ulo[0] + ci % bx, my = ulo[1] + (ci / bx) % by, mz = ulo[2] + ci / (bx * by);
+        // tile points whose window holds this centre (wave-uniform)
+        unsigned mask = 0;
+#pragma unroll
+        for (int c = 0; c < T; ++c)
+            if (ok[c] && mx >= lo[c][0] && mx <= hi[c][0] && my >= lo[c][1] && my <= hi[c][1] &&
+                (DIM == 2 || (mz >= lo[c][2] && mz <= hi[c][2])))
+                mask |= 1u << c;
+        if (!mask) continue;
         long long s = mx + (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
         float4 abc = coef[s];
         if (abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f) continue;
@@ -306,24 +346,41 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gather_ref(Problem pr, const f
             for (int d = 0; d < DIM; ++d) r2 += (x[d] - y[d]) * (x[d] - y[d]);
             float kw = neighborhood_weight(r2, MODE == 0 ? pr.kc.smin : sty.support);
             if (!(kw > 0.0f)) continue;
-            float vv[3] = {p[0] - y[0], p[1] - y[1], DIM == 3 ? p[2] - y[2] : 0.0f};
-            float ka = kernel_pair<DIM, MODE>(pr.kc, vv, sty, sty);
-            if (ka == 0.0f) continue;
             float fv = kw * pr.fixed_data[v];
             float mv = kw * g4.w;
             float beta = fmaf(abc.x, fv, fmaf(abc.y, mv, abc.z));
-            float w = beta * kw * ka;
-            acc[0] = fmaf(w, g4.x, acc[0]);
-            acc[1] = fmaf(w, g4.y, acc[1]);
-            if (DIM == 3) acc[2] = fmaf(w, g4.z, acc[2]);
+            float bk = beta * kw;
+#pragma unroll
+            for (int c = 0; c < T; ++c) {
+                if (!((mask >> c) & 1u)) continue;
+                float vv[3] = {p[c][0] - y[0], p[c][1] - y[1], DIM == 3 ? p[c][2] - y[2] : 0.0f};
+                float ka = kernel_pair<DIM, MODE>(pr.kc, vv, sty, sty);
+                float w = bk * ka;
+                acc[c][0] = fmaf(w, g4.x, acc[c][0]);
+                acc[c][1] = fmaf(w, g4.y, acc[c][1]);
+                if (DIM == 3) acc[c][2] = fmaf(w, g4.z, acc[c][2]);
+            }
         }
     }
-    double dacc[4] = {acc[0], acc[1], acc[2], 0.0};
-    block_sum<4>(dacc, lds);
-    if (threadIdx.x == 0) {
-        double scale = sc->coef_scale;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) deriv[j + d * pr.n_cp] = (float)(dacc[d] * scale);
+    for (int c = 0; c < T; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            float v = acc[c][d];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) lds[wave * T * 3 + c * 3 + d] = v;
+        }
+    __syncthreads();
+    if (threadIdx.x < T * 3) {
+        const int c = threadIdx.x / 3, d = threadIdx.x % 3;
+        double sum = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) sum += lds[w * T * 3 + threadIdx.x];
+        if (ok[c] && d < DIM) {
+            const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
+            const long long j = jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]);
+            deriv[j + d * pr.n_cp] = (float)(sum * sc->coef_scale);
+        }
     }
 }
 
@@ -435,7 +492,10 @@ template <int DIM, int MODE>
 static void gather_ref_impl(const Problem &pr, const float4 *dense, const float4 *coef, const Scalars *sc,
                             float *deriv, hipStream_t s)
 {
-    k_lcc_gather_ref<DIM, MODE><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, dense, coef, sc, deriv);
+    constexpr int TX = LcTile<DIM>::X, TY = LcTile<DIM>::Y, TZ = LcTile<DIM>::Z;
+    const long long tiles = (long long)((pr.cp.n[0] + TX - 1) / TX) * ((pr.cp.n[1] + TY - 1) / TY) *
+                            (DIM == 3 ? (pr.cp.n[2] + TZ - 1) / TZ : 1);
+    k_lcc_gather_ref<DIM, MODE><<<(unsigned)tiles, kThreads, 0, s>>>(pr, dense, coef, sc, deriv);
 }
 
 void launch_lcc_gather_reference(const Problem &pr, const float4 *dense_g, const float4 *centre_coef,
