@@ -793,8 +793,9 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
     pr.cpc = ev->d_cpc;
     pr.cp_state = ev->d_cp_state;
     pr.cp_pstate = ev->d_cp_pstate;
-    if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
+    if (desc->kernel_mode != SKM_KERNEL_STATIONARY)
         if ((st = dev_alloc(&ev->d_smp_pstate, (size_t)pr.n_sub * 2)) != SKM_SUCCESS) return cleanup(st);
+    {
         // sized for the dense voxel grid, which the exact LCC gather walks
         const long long np = gather_partials_size(pr, pr.fixed.n[dim - 1]);
         if ((st = dev_alloc(&ev->d_gather_partials, (size_t)np)) != SKM_SUCCESS) return cleanup(st);

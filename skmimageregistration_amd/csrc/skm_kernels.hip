@@ -442,121 +442,7 @@ __global__ __launch_bounds__(kThreads) void k_prepare(Problem pr, const float4 *
 
 // ------------------------------------------------------------ derivative gather
 
-// d_j = w_j * sum_{samples m in window(j)} k(p_j, x_m) v_m
-// (evaluateDerivativeMSE, GpuEvaluatorMSE.cu:97-170, 239-309;
-// evaluateDerivativeNCC, GpuEvaluatorNCC.cu:111-190, 265-340).  One workgroup
-// per control point; the window (reference or exact) was computed on the host
-// in FP64.  Lanes work in groups of 8 along x, 32 rows at a time; rows and
-// row ends beyond the kernel's reach are skipped (k == 0 there).  The
-// weighted modes read each sample's state from the value pass (smp_pstate).
-// DENSE (subsample 1, stationary kernel) walks a row with an incremental
-// offset and the row's constant y/z distance.
-template <int DIM, int MODE, bool DENSE>
-__global__ __launch_bounds__(kThreads) void k_gather(Problem pr, const float4 *__restrict__ sv,
-                                                     float *__restrict__ deriv)
-{
-    __shared__ double lds[4 * 4];
-    const long long j = blockIdx.x;
-    int jc[3] = {0, 0, 0};
-    {
-        long long r = j;
-        jc[0] = (int)(r % pr.cp.n[0]); r /= pr.cp.n[0];
-        jc[1] = (int)(r % pr.cp.n[1]); r /= pr.cp.n[1];
-        jc[2] = (int)r;
-    }
-    float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]}, p[3];
-    to_global<DIM>(pr.cp, fj, p);
-    PState st;
-    const float lam_p = cp_state_load<DIM, MODE>(pr, j, st);
-    st.support = is_stationary(MODE) ? pr.kc.smin : pr.cp_state[j * kCpStateStride + 11];
-    const int *win = pr.cp_window + j * 6;
-    int lo[3] = {win[0], win[1], DIM == 3 ? win[2] : 0}, hi[3] = {win[3], win[4], DIM == 3 ? win[5] : 0};
-    const int bx = hi[0] - lo[0] + 1, by = hi[1] - lo[1] + 1, bz = hi[2] - lo[2] + 1;
-    const int rows = by * bz;
-    const int tx = threadIdx.x & 7, tr = threadIdx.x >> 3;
-    const float s2 = is_stationary(MODE) ? st.support * st.support : pair_reach2(pr.kc.smin, lam_p, pr.lam_w_max);
-    const int sub = pr.sub;
-    const float span = sub > 1 ? (float)sub : 0.0f;  // jitter range of a cell
-    float acc[3] = {0.0f, 0.0f, 0.0f};
-    float pc[3];
-    to_local<DIM>(pr.fixed, p, pc);  // cp position in fixed index units
-    const float inv_sub = 1.0f / (float)sub;
-    int my = lo[1] + tr % by, mz = lo[2] + tr / by;
-    for (int r = tr; r < rows; r += kThreads / 8, my += kThreads / 8) {
-        while (my > hi[1]) {
-            my -= by;
-            ++mz;
-        }
-        // distance from p to the cell row's y/z extent (jitter included)
-        float y0 = pr.fixed.o[1] + pr.fixed.h[1] * (float)(my * sub);
-        float y1 = y0 + pr.fixed.h[1] * span;
-        float dy = fmaxf(0.0f, fmaxf(y0 - p[1], p[1] - y1));
-        float dz = 0.0f;
-        if (DIM == 3) {
-            float z0 = pr.fixed.o[2] + pr.fixed.h[2] * (float)(mz * sub);
-            float z1 = z0 + pr.fixed.h[2] * span;
-            dz = fmaxf(0.0f, fmaxf(z0 - p[2], p[2] - z1));
-        }
-        float rem = s2 - dy * dy - dz * dz;
-        if (rem <= 0.0f) continue;
-        // x cells that can hold in-support samples, one cell of margin each side
-        float rx = fast_sqrt(rem) * pr.fixed.ih[0];
-        int xa = max(lo[0], (int)((pc[0] - rx - span) * inv_sub) - 1);
-        int xb = min(hi[0], (int)((pc[0] + rx) * inv_sub) + 1);
-        long long rowbase = (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
-        if constexpr (DENSE && is_stationary(MODE)) {
-            // dense rows: y and z of the row are exact sample coordinates
-            float ry = p[1] - (pr.fixed.o[1] + pr.fixed.h[1] * (float)my);
-            float rz = DIM == 3 ? p[2] - (pr.fixed.o[2] + pr.fixed.h[2] * (float)mz) : 0.0f;
-            const float c_row = fmaf(ry, ry, rz * rz);
-            const float step = -8.0f * pr.fixed.h[0];
-            int mx = xa + tx;
-            float dx = p[0] - (pr.fixed.o[0] + pr.fixed.h[0] * (float)mx);
-            const float4 *src = sv + rowbase;
-            for (; mx <= xb; mx += 8, dx += step) {
-                float4 v4 = src[mx];
-                float k = stationary_k<MODE>(fmaf(dx, dx, c_row), pr.kc.inv_s4, pr.kc.inv_s0);
-                acc[0] = fmaf(k, v4.x, acc[0]);
-                acc[1] = fmaf(k, v4.y, acc[1]);
-                if (DIM == 3) acc[2] = fmaf(k, v4.z, acc[2]);
-            }
-            continue;
-        }
-        for (int mx = xa + tx; mx <= xb; mx += 8) {
-            float4 v4 = sv[rowbase + mx];
-            int pos[3] = {mx, my, mz};
-            if (sub > 1) {
-                unsigned pk = __float_as_uint(v4.w);
-                pos[0] = mx * sub + (int)(pk & 0xff);
-                pos[1] = my * sub + (int)((pk >> 8) & 0xff);
-                pos[2] = mz * sub + (int)((pk >> 16) & 0xff);
-            }
-            float fpos[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, xs[3];
-            to_global<DIM>(pr.fixed, fpos, xs);
-            float v[3] = {p[0] - xs[0], p[1] - xs[1], DIM == 3 ? p[2] - xs[2] : 0.0f};
-            PState sst;
-            if (!is_stationary(MODE)) {
-                const long long si = rowbase + mx;
-                pstate_unpack<DIM, MODE>(pr.smp_pstate[2 * si],
-                                         MODE == 2 ? pr.smp_pstate[2 * si + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f),
-                                         sst);
-            }
-            float k = kernel_pair<DIM, MODE>(pr.kc, v, st, sst);
-            acc[0] = fmaf(k, v4.x, acc[0]);
-            acc[1] = fmaf(k, v4.y, acc[1]);
-            if (DIM == 3) acc[2] = fmaf(k, v4.z, acc[2]);
-        }
-    }
-    double dacc[4] = {acc[0], acc[1], acc[2], 0.0};
-    block_sum<4>(dacc, lds);
-    if (threadIdx.x == 0) {
-        double w = pr.cpc[j].w;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) deriv[j + d * pr.n_cp] = (float)(w * dacc[d]);
-    }
-}
-
-// Control-point tile of the weighted-mode gather: small enough that the
+// Control-point tile of the derivative gather: small enough that the
 // tile's constants stay in scalar registers.  In 3D the four points share
 // x and y, so their chords along a sample row nearly coincide and the lanes
 // of a row serve all four (measured faster than 4x1x1, 1x4x1 and 2x2x1).
@@ -564,7 +450,7 @@ template <int DIM> struct CpTile;
 template <> struct CpTile<2> { static constexpr int X = 2, Y = 2, Z = 1; };
 template <> struct CpTile<3> { static constexpr int X = 1, Y = 1, Z = 4; };
 
-// Sample planes (last axis) per slab of the weighted-mode gather: 3D windows
+// Sample planes (last axis) per slab of the derivative gather: 3D windows
 // are walked slab by slab so that the workgroups in flight share one slab of
 // samples (a few MB, resident in every XCD's L2); 2D windows in one slab.
 template <int DIM>
@@ -592,17 +478,21 @@ __device__ __forceinline__ void tile_plane_range(const Problem &pr, int t0, int 
     }
 }
 
-// Derivative gather of the weighted modes: one workgroup per (slab of sample
-// planes, tile of control points), slab-major.  The workgroup walks the rows
-// of the union of the tile's reference windows inside its slab, 8 lanes
-// along x and 32 rows at a time; each lane loads a sample (v, packed state)
-// once and adds its pair with every tile control point whose window and
-// reach (both points' lam) hold the sample.  The tile constants are
-// wave-uniform (scalar registers).  Six waves per SIMD (measured faster than
-// five with the next sample's loads issued early, and than seven or eight,
-// which spill).  Each workgroup writes its FP64 sums;
-// k_gather_finish adds the slabs in order.  Same sums per control point as
-// k_gather, in a fixed order.
+// Derivative gather, d_j = w_j * sum_{samples m in window(j)} k(p_j, x_m) v_m
+// (evaluateDerivativeMSE, GpuEvaluatorMSE.cu:97-170, 239-309;
+// evaluateDerivativeNCC, GpuEvaluatorNCC.cu:111-190, 265-340; the windows,
+// reference or exact, computed on the host in FP64).  One workgroup per
+// (slab of sample planes, tile of control points), slab-major.  The
+// workgroup walks the rows of the union of the tile's windows inside its
+// slab; in 3D each wave takes 4 x 4 (y, z) rows with 4 lanes along x (a
+// 4 x 4 x 4 block of samples per step), in 2D 8 rows with 8 lanes along x.
+// Each lane loads a sample (v, and in the weighted modes the state the value
+// pass stored) once and adds its pair with every tile control point whose
+// window and reach hold it (weighted modes: both points' lam; stationary
+// modes: the support).  The tile constants are wave-uniform (scalar
+// registers).  Six waves per SIMD (measured faster than five with the next
+// sample's loads issued early, and than seven or eight, which spill).  Each
+// workgroup writes its FP64 sums; k_gather_finish adds the slabs in order.
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_gather_tile(Problem pr, const float4 *__restrict__ sv, int n_tiles)
 {
@@ -636,7 +526,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6))
         py[c] = pos[1];
         pz[c] = pos[2];
         pc0[c] = pcl[0];
-        lam[c] = cp_state_load<DIM, MODE>(pr, jj[c], st[c]);
+        // stationary modes: a reach share of 1 for every point makes the
+        // reach tests |v|^2 < smin^2 (x 1.0001), where their kernel ends
+        lam[c] = is_stationary(MODE) ? 1.0f : cp_state_load<DIM, MODE>(pr, jj[c], st[c]);
         const int *win = pr.cp_window + jj[c] * 6;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -657,7 +549,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6))
     const int by = uhi[1] - ulo[1] + 1, bz = uhi[2] - ulo[2] + 1;
     // (a window can be empty along an axis: then there is nothing to gather)
     const int rows = (uhi[0] >= ulo[0] && by > 0 && bz > 0) ? by * bz : 0;
-    const int tx = threadIdx.x & 7, tr = threadIdx.x >> 3;
+    // lanes: LX along x, the rest over rows; in 3D a wave takes 4 x 4 (y, z)
+    // rows at a time, so its 64 samples form a 4 x 4 x 4 block
+    constexpr int LX = DIM == 3 ? 4 : 8;
+    const int tx = threadIdx.x % LX;
+    const int wv = threadIdx.x >> 6, lr = (threadIdx.x & 63) / LX;
     const int sub = pr.sub;
     const float span = sub > 1 ? (float)sub : 0.0f;
     const float inv_sub = 1.0f / (float)sub;
@@ -665,17 +561,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6))
     float acc[T][3];
 #pragma unroll
     for (int c = 0; c < T; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0f;
-    int my = ulo[1] + tr % max(by, 1), mz = ulo[2] + tr / max(by, 1);
-    for (int r = tr; r < rows; r += kThreads / 8, my += kThreads / 8) {
-        while (my > uhi[1]) {
-            my -= by;
-            ++mz;
-        }
+    constexpr int RY = DIM == 3 ? 4 : 64 / LX, RZ = 64 / LX / RY;
+    const int nby = rows ? (by + RY - 1) / RY : 0, nbz = rows ? (bz + RZ - 1) / RZ : 0;
+    for (int blk = wv; blk < nby * nbz; blk += kThreads / 64) {
+        const int my = ulo[1] + (blk % nby) * RY + lr % RY, mz = ulo[2] + (blk / nby) * RZ + lr / RY;
+        if (my > uhi[1] || mz > uhi[2]) continue;
         const float y0 = pr.fixed.o[1] + pr.fixed.h[1] * (float)(my * sub);
         const float y1 = y0 + pr.fixed.h[1] * span;
         const float z0 = DIM == 3 ? pr.fixed.o[2] + pr.fixed.h[2] * (float)(mz * sub) : 0.0f;
         const float z1 = z0 + (DIM == 3 ? pr.fixed.h[2] * span : 0.0f);
-        const float lam_row = pr.smp_row_lam[my + (long long)pr.nsub[1] * mz];
+        const float lam_row = is_stationary(MODE) ? 1.0f : pr.smp_row_lam[my + (long long)pr.nsub[1] * mz];
         // per tile control point: x cells of this row inside its window and reach
         int xa[T], xb[T];
         int rxa = INT_MAX, rxb = INT_MIN;
@@ -701,11 +596,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6))
         // would overflow)
         if (rxa > rxb) continue;
         const long long rowbase = (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
-        for (int mx = rxa + tx; mx <= rxb; mx += 8) {
-            const float4 cv = sv[rowbase + mx], ca = pr.smp_pstate[2 * (rowbase + mx)],
-                         cb = pr.smp_pstate[2 * (rowbase + mx) + 1];
+        for (int mx = rxa + tx; mx <= rxb; mx += LX) {
+            const float4 cv = sv[rowbase + mx];
+            float4 cb = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // .w: the sample's reach share
             PState sst;
-            pstate_unpack<DIM, MODE>(ca, cb, sst);
+            if constexpr (!is_stationary(MODE)) {
+                const float4 ca = pr.smp_pstate[2 * (rowbase + mx)];
+                cb = pr.smp_pstate[2 * (rowbase + mx) + 1];
+                pstate_unpack<DIM, MODE>(ca, cb, sst);
+            }
             int pos[3] = {mx, my, mz};
             if (sub > 1) {
                 const unsigned pk = __float_as_uint(cv.w);
@@ -1088,7 +987,7 @@ void launch_prepare(const Problem &pr, int measure, const float4 *sg, const floa
 template <int DIM, int MODE>
 static void gather_impl(const Problem &pr, const float4 *sv, float *deriv, hipStream_t s)
 {
-    if constexpr (!is_stationary(MODE)) {
+    {
         const long long tiles = (long long)((pr.cp.n[0] + CpTile<DIM>::X - 1) / CpTile<DIM>::X) *
                                 ((pr.cp.n[1] + CpTile<DIM>::Y - 1) / CpTile<DIM>::Y) *
                                 (DIM == 3 ? (pr.cp.n[2] + CpTile<DIM>::Z - 1) / CpTile<DIM>::Z : 1);
@@ -1096,10 +995,6 @@ static void gather_impl(const Problem &pr, const float4 *sv, float *deriv, hipSt
         const long long slabs = (pr.nsub[DIM - 1] + slab - 1) / slab;
         k_gather_tile<DIM, MODE><<<(unsigned)(tiles * slabs), kThreads, 0, s>>>(pr, sv, (int)tiles);
         k_gather_finish<DIM><<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr, (int)tiles, deriv);
-    } else if (pr.sub == 1) {
-        k_gather<DIM, MODE, true><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
-    } else {
-        k_gather<DIM, MODE, false><<<(unsigned)pr.n_cp, kThreads, 0, s>>>(pr, sv, deriv);
     }
 }
 
