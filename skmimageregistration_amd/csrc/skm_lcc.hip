@@ -251,8 +251,8 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
 // neighbourhood weight), which cuts the re-reads of the dense field that
 // bound a one-point-per-workgroup walk (each point's centres and neighbours
 // cover ~80^3 voxels, more than an XCD's L2).  In 3D 16 points per tile
-// (measured faster than 4, 8, 12 and 32 on the 256x256x128 sliding-motion
-// workload: 534, 274, 203, 192 and 240 ms).
+// (measured faster than 8 and 32 on the 256x256x128 sliding-motion workload:
+// 137, 90 and 97 ms).
 template <int DIM> struct LcTile;
 template <> struct LcTile<2> { static constexpr int X = 2, Y = 2, Z = 1; };
 template <> struct LcTile<3> { static constexpr int X = 2, Y = 2, Z = 4; };
@@ -261,9 +261,9 @@ template <> struct LcTile<3> { static constexpr int X = 2, Y = 2, Z = 4; };
 // 696-961): for cp j, centres in the tight window of p_j, all their
 // neighbours, ka = k(p_j, y) with the neighbour's sigma / covariance for both
 // points (:446-460), no w_j, divided by N on the host (:645-646).
-// One workgroup per tile of control points: the waves walk the centres of
-// the union of the tile's windows (one centre per wave at a time), the lanes
-// a centre's neighbours, and each pair is added to every tile point whose
+// One workgroup per tile of control points: each thread takes a centre of
+// the union of the tile's windows (256 at a time), the wave walks the
+// neighbour offsets in step, and each pair is added to every tile point whose
 // window holds the centre.  Sums: FP32 per lane, FP32 butterfly per wave,
 // FP64 across the waves in order.
 template <int DIM, int MODE>
@@ -307,33 +307,57 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gather_ref(Problem pr, const f
 #pragma unroll
     for (int c = 0; c < T; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0f;
     const int sn = pr.subn;
-    for (int ci = wave; ci < ncent; ci += kThreads / 64) {
-        const int mx = ulo[0] + ci % bx, my = ulo[1] + (ci / bx) % by, mz = ulo[2] + ci / (bx * by);
-        // tile points whose window holds this centre (wave-uniform)
+    // one centre per thread (consecutive threads: consecutive centres along
+    // x, so a step's neighbour loads of a wave fall on few cache lines); the
+    // neighbour offsets are walked in step by the whole wave
+    for (int cb = 0; cb < ncent; cb += kThreads) {
+        const int ci = cb + (int)threadIdx.x;
         unsigned mask = 0;
+        float4 abc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float x[3] = {0.0f, 0.0f, 0.0f};
+        int nlo[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
+        if (ci < ncent) {
+            const int mx = ulo[0] + ci % bx, my = ulo[1] + (ci / bx) % by, mz = ulo[2] + ci / (bx * by);
+            // tile points whose window holds this centre
 #pragma unroll
-        for (int c = 0; c < T; ++c)
-            if (ok[c] && mx >= lo[c][0] && mx <= hi[c][0] && my >= lo[c][1] && my <= hi[c][1] &&
-                (DIM == 2 || (mz >= lo[c][2] && mz <= hi[c][2])))
-                mask |= 1u << c;
-        if (!mask) continue;
-        long long s = mx + (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
-        float4 abc = coef[s];
-        if (abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f) continue;
-        int pos[3] = {0, 0, 0};
-        centre_position<DIM>(pr, s, pos);
-        float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, x[3];
-        to_global<DIM>(pr.fixed, fp, x);
-        PState st;
-        point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
-        int nlo[3] = {0, 0, 0}, nhi[3] = {0, 0, 0};
-        neighbour_box<DIM>(pr, pos, (double)st.support, nlo, nhi);
-        int nx = (nhi[0] - nlo[0]) / sn + 1, ny = (nhi[1] - nlo[1]) / sn + 1;
-        int nz = DIM == 3 ? (nhi[2] - nlo[2]) / sn + 1 : 1;
-        int nn = nx * ny * nz;
-        for (int t = lane; t < nn; t += 64) {
-            int fx = nlo[0] + (t % nx) * sn, fy = nlo[1] + ((t / nx) % ny) * sn;
-            int fz = DIM == 3 ? nlo[2] + (t / (nx * ny)) * sn : 0;
+            for (int c = 0; c < T; ++c)
+                if (ok[c] && mx >= lo[c][0] && mx <= hi[c][0] && my >= lo[c][1] && my <= hi[c][1] &&
+                    (DIM == 2 || (mz >= lo[c][2] && mz <= hi[c][2])))
+                    mask |= 1u << c;
+            if (mask) {
+                const long long s = mx + (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
+                abc = coef[s];
+                if (abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f) mask = 0;
+            }
+            if (mask) {
+                const long long s = mx + (long long)pr.nsub[0] * (my + (long long)pr.nsub[1] * mz);
+                int pos[3] = {0, 0, 0};
+                centre_position<DIM>(pr, s, pos);
+                float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]};
+                to_global<DIM>(pr.fixed, fp, x);
+                PState st;
+                point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+                int nhi[3] = {0, 0, 0};
+                neighbour_box<DIM>(pr, pos, (double)st.support, nlo, nhi);
+                cnt[0] = (nhi[0] - nlo[0]) / sn + 1;
+                cnt[1] = (nhi[1] - nlo[1]) / sn + 1;
+                cnt[2] = DIM == 3 ? (nhi[2] - nlo[2]) / sn + 1 : 1;
+            }
+        }
+        // the wave's largest neighbour box
+        int wn[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int m = cnt[d];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+            wn[d] = __builtin_amdgcn_readfirstlane(m);
+        }
+        for (int t = 0; t < wn[0] * wn[1] * wn[2]; ++t) {
+            const int kx = t % wn[0], ky = (t / wn[0]) % wn[1], kz = t / (wn[0] * wn[1]);
+            if (kx >= cnt[0] || ky >= cnt[1] || kz >= cnt[2]) continue;
+            int fx = nlo[0] + kx * sn, fy = nlo[1] + ky * sn;
+            int fz = DIM == 3 ? nlo[2] + kz * sn : 0;
             long long v = fx + (long long)pr.fixed.n[0] * (fy + (long long)pr.fixed.n[1] * fz);
             float4 g4 = dense[v];
             if (isnan(g4.w)) continue;
