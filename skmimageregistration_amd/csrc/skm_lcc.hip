@@ -415,27 +415,43 @@ __global__ __launch_bounds__(kThreads) void k_lcc_scatter(Problem pr, const floa
                                                           const float4 *__restrict__ coef,
                                                           const Scalars *__restrict__ sc, float4 *__restrict__ gamma)
 {
-    long long s = blockIdx.x * (long long)(kThreads / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (s >= pr.n_sub) return;
-    float4 abc = coef[s];
-    if (abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f) return;
-    float scale = (float)sc->coef_scale;
-    int pos[3] = {0, 0, 0};
-    centre_position<DIM>(pr, s, pos);
-    float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]}, x[3];
-    to_global<DIM>(pr.fixed, fp, x);
-    PState st;
-    point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
-    int nlo[3] = {0, 0, 0}, nhi[3] = {0, 0, 0};
-    neighbour_box<DIM>(pr, pos, (double)st.support, nlo, nhi);
+    // one centre per thread (consecutive threads: consecutive centres along
+    // x); the wave walks the neighbour offsets in step, so that a step's
+    // loads and atomic adds fall on few cache lines
+    const long long s = blockIdx.x * (long long)kThreads + threadIdx.x;
+    const float scale = (float)sc->coef_scale;
+    float4 abc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float x[3] = {0.0f, 0.0f, 0.0f};
+    int nlo[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
+    if (s < pr.n_sub) abc = coef[s];
+    if (s < pr.n_sub && !(abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f)) {
+        int pos[3] = {0, 0, 0};
+        centre_position<DIM>(pr, s, pos);
+        float fp[3] = {(float)pos[0], (float)pos[1], (float)pos[2]};
+        to_global<DIM>(pr.fixed, fp, x);
+        PState st;
+        point_state<DIM, MODE>(pr.kc, pr.weight, pr.weight_data, x, st);
+        int nhi[3] = {0, 0, 0};
+        neighbour_box<DIM>(pr, pos, (double)st.support, nlo, nhi);
+        cnt[0] = (nhi[0] - nlo[0]) / pr.subn + 1;
+        cnt[1] = (nhi[1] - nlo[1]) / pr.subn + 1;
+        cnt[2] = DIM == 3 ? (nhi[2] - nlo[2]) / pr.subn + 1 : 1;
+    }
     const int sn = pr.subn;
-    int nx = (nhi[0] - nlo[0]) / sn + 1, ny = (nhi[1] - nlo[1]) / sn + 1;
-    int nz = DIM == 3 ? (nhi[2] - nlo[2]) / sn + 1 : 1;
-    int nn = nx * ny * nz;
-    for (int t = lane; t < nn; t += 64) {
-        int fx = nlo[0] + (t % nx) * sn, fy = nlo[1] + ((t / nx) % ny) * sn;
-        int fz = DIM == 3 ? nlo[2] + (t / (nx * ny)) * sn : 0;
+    // the wave's largest neighbour box
+    int wn[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int m = cnt[d];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+        wn[d] = __builtin_amdgcn_readfirstlane(m);
+    }
+    for (int t = 0; t < wn[0] * wn[1] * wn[2]; ++t) {
+        const int kx = t % wn[0], ky = (t / wn[0]) % wn[1], kz = t / (wn[0] * wn[1]);
+        if (kx >= cnt[0] || ky >= cnt[1] || kz >= cnt[2]) continue;
+        int fx = nlo[0] + kx * sn, fy = nlo[1] + ky * sn;
+        int fz = DIM == 3 ? nlo[2] + kz * sn : 0;
         long long v = fx + (long long)pr.fixed.n[0] * (fy + (long long)pr.fixed.n[1] * fz);
         float4 g4 = dense[v];
         if (isnan(g4.w)) continue;
@@ -532,7 +548,7 @@ template <int DIM, int MODE>
 static void scatter_impl(const Problem &pr, const float4 *dense, const float4 *coef, const Scalars *sc,
                          float4 *gamma, hipStream_t s)
 {
-    unsigned g = (unsigned)((pr.n_sub + 3) / 4);
+    unsigned g = (unsigned)((pr.n_sub + kThreads - 1) / kThreads);
     k_lcc_scatter<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, sc, gamma);
 }
 
