@@ -55,7 +55,10 @@ typedef enum skm_kernel_mode {
  * kernel support, so the derivative is the exact gradient of the value.  Off
  * by default: the reference windows are used, including the 2D subsampled
  * lower bound (GpuEvaluatorMSE.cu:267-273) and the LCC centre truncation
- * (GpuEvaluatorLCC.cu:239-245). */
+ * (GpuEvaluatorLCC.cu:239-245).  For LCC, centres that fail the reference's
+ * derivative guard |u| > 1e-16 (GpuEvaluatorLCC.cu:517, a flat fixed or
+ * moving neighbourhood) add cc to the value but nothing to the derivative in
+ * both modes, so the exact gradient covers the other centres. */
 #define SKM_FLAG_EXACT_GRADIENT 0x1u
 
 /* Origin, spacing and size of an image (ImageBase, include/cuda/ImageBase.h:25-50). */

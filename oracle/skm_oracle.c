@@ -926,7 +926,9 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
                     double d1 = sffx - 2.0 * msf * sfx + cnt * msf * msf;
                     double d2 = smmx - 2.0 * msm * smx + cnt * msm * msm;
                     double v = sfmx - msm * sfx - msf * smx + cnt * msf * msm;
-                    if (pass == 1 && !(cnt > 0 && d1 > 0 && d2 > 0)) break;
+                    /* cc's condition (GpuEvaluatorLCC.cu:186) and the
+                     * derivative guard |u| > 1e-16 (:517) */
+                    if (pass == 1 && !(cnt > 0 && d1 > 0 && d2 > 0 && fabs(d1 * d2) > 1e-16)) break;
                     for (f[2] = z0; f[2] <= z1; f[2] += sn)
                         for (f[1] = lo[1]; f[1] <= hi[1]; f[1] += sn)
                             for (f[0] = lo[0]; f[0] <= hi[0]; f[0] += sn) {

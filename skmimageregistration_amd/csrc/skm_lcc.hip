@@ -168,9 +168,13 @@ __device__ __forceinline__ long long voxel_index(const Problem &pr, const int *p
 
 // Per-centre LCC (evaluateValueLCC, GpuEvaluatorLCC.cu:38-204, 536-694) from the
 // dense warp, one thread per centre.  Writes the derivative coefficients
-// (A, B, C) of the centre (zero where the centre contributes no derivative:
-// reference |u| > 1e-16 guard, :517-518, or d1 > 0 && d2 > 0 in exact mode)
-// and adds cc and the centre's inside flag to the block partials.
+// (A, B, C) of the centre and adds cc and the centre's inside flag to the
+// block partials.  The coefficients are zero where the centre contributes no
+// derivative: the reference's |u| > 1e-16 guard (:517-518) in both modes, and
+// in exact mode also d1 > 0 && d2 > 0 (the condition for cc, :186).  Without
+// the guard, a neighbourhood whose moving values are the B-spline's ringing
+// inside a zero background (d2 ~ 1e-40) gives B = 2 d1 v^2 / u^2 <= 2 / d2
+// beyond the FP32 range; with it |A| < 2e8 and B < 2e16 d1.
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const float4 *__restrict__ dense,
                                                           float4 *__restrict__ coef, double *__restrict__ partials)
@@ -226,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
             double vv = sfm - msm * sf - msf * sm + count * msf * msm;
             if (d1 > 0 && d2 > 0) cc = -vv * vv / (d1 * d2);
             double u = d1 * d2;
-            bool use = pr.exact ? (d1 > 0 && d2 > 0) : (fabs(u) > 1e-16);
+            bool use = fabs(u) > 1e-16 && (!pr.exact || (d1 > 0 && d2 > 0));
             if (use) {
                 // beta(y) = -(2 v (fv - msf) u - 2 d1 (mv - msm) v^2) / u^2
                 double inv_u = 1.0 / u;

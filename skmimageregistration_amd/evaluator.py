@@ -107,7 +107,9 @@ class GpuKernelEvaluator:
     reference runs in TransformAdapter::Initialize (itkTransformAdapter.hxx:
     435-439) runs inside the library.  cpImage is a Grid (geometry of the
     control-point image).  exact_gradient selects derivative windows that
-    cover the whole kernel support.
+    cover the whole kernel support.  LCC centres failing the reference's
+    derivative guard |u| > 1e-16 (GpuEvaluatorLCC.cu:517) add to the value
+    but not to the derivative in either mode.
     """
 
     def __init__(self, numParameters, subsample, subsampleNeighborhood, kernel, movingImage, fixedImage,
