@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-registration", action="store_true", help="skip the multi-scale registration timing")
     ap.add_argument("--no-lcc", action="store_true", help="skip the LCC timing on the sliding-motion volumes")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched time-series registration")
+    ap.add_argument("--batch-frames", type=int, default=32, help="frames of the time series (frame 0 = reference)")
+    ap.add_argument("--batch-concurrency", type=int, default=4, help="registrations in flight in the batched run")
     ap.add_argument("--registration-repeats", type=int, default=3)
     ap.add_argument("--registration-rd", type=float, default=1e-5,
                     help="RD weight of the registration (the bilateral-like regularizer)")
@@ -368,21 +371,28 @@ def lcc_timing():
     return out
 
 
+def registration_config(args):
+    """3 levels with support 4s, 2s, s, control-point spacing support/4,
+    tensors from the target with weights 400/200/100, NCC + RD, sampling
+    rates 5/4/3, 200 adaptive-step iterations per level."""
+    from skmimageregistration_amd import RegistrationConfig
+    s = args.sigma
+    return RegistrationConfig(sigma_C4=[4 * s, 2 * s, s], sigma_C0=[4 * s, 2 * s, s],
+                              tensor_weights=[args.tensor_weight, args.tensor_weight / 2, args.tensor_weight / 4]
+                              if args.kernel == "anisotropic" else [0.0],
+                              num_scales=3, num_function_evaluations=[200], initial_step_size=[1000.0],
+                              sampling_rate=[5, 4, 3], neighborhood_sampling=[1], smoothing_variance=[2, 1, 0],
+                              reg_rd=[args.registration_rd], metric="ncc")
+
+
 def registration(args, mov, fix, disp):
     """Wall time of one multi-scale registration (SKMReg's scale loop,
-    SKMReg.cxx:123-500): 3 levels with support 4s, 2s, s, control-point
-    spacing support/4, tensors from the target with weights 400/200/100,
-    NCC, sampling rates 5/4/3, 200 adaptive-step iterations per level, then
-    the dense displacement field.  Host images in, dense field out, timed
-    after one untimed run; the median of the repeats."""
-    from skmimageregistration_amd import Image, RegistrationConfig, register_multiscale
+    SKMReg.cxx:123-500) with registration_config's levels, then the dense
+    displacement field.  Host images in, dense field out, timed after one
+    untimed run; the median of the repeats."""
+    from skmimageregistration_amd import Image, register_multiscale
     s = args.sigma
-    cfg = RegistrationConfig(sigma_C4=[4 * s, 2 * s, s], sigma_C0=[4 * s, 2 * s, s],
-                             tensor_weights=[args.tensor_weight, args.tensor_weight / 2, args.tensor_weight / 4]
-                             if args.kernel == "anisotropic" else [0.0],
-                             num_scales=3, num_function_evaluations=[200], initial_step_size=[1000.0],
-                             sampling_rate=[5, 4, 3], neighborhood_sampling=[1], smoothing_variance=[2, 1, 0],
-                             reg_rd=[args.registration_rd], metric="ncc")
+    cfg = registration_config(args)
     register_multiscale(Image(mov), Image(fix), cfg)
     times, reports, df = [], None, None
     for _ in range(max(args.registration_repeats, 1)):
@@ -399,6 +409,41 @@ def registration(args, mov, fix, disp):
             "mean_abs_error_vox": round(float(np.abs(df[inner] - disp[inner]).mean()), 4),
             "mean_abs_displacement_vox": round(float(np.abs(disp[inner]).mean()), 4),
             "repeats_ms": [round(t, 2) for t in times], "reg_rd": args.registration_rd}
+
+
+def batch_timing(args):
+    """Batched time series: frames of 96^3, 128x128x64 and 160x160x80 voxels
+    (sizes drawn with seed 31) sampling one ellipsoid phantom, warped by a
+    smooth field (max 7 voxels) times the sine of the frame's phase, plus
+    N(0, 0.02); frames 1.. each registered to frame 0 with
+    registration_config's levels.  Wall time of the whole series registered
+    one at a time and with --batch-concurrency registrations in flight
+    (skm_register_batch), after one untimed pair; both runs must give the
+    same displacements bit for bit."""
+    from skmimageregistration_amd import Image, register_batch, synthetic
+    frames, disps = synthetic.time_series_3d(n_frames=args.batch_frames)
+    cfg = registration_config(args)
+    pairs = [(Image(frames[0]), Image(f)) for f in frames[1:]]
+    register_batch(pairs[:1], cfg, concurrency=1)
+    out = {"workload": f"{len(pairs)} registrations of a {len(frames)}-frame time series to frame 0, frames of "
+                       "96^3 / 128x128x64 / 160x160x80 voxels, NCC + RD, 3 levels x 200 iterations",
+           "frames": {f"{s[2]}x{s[1]}x{s[0]}": sum(f.shape == s for f in frames) for s in synthetic.SERIES_SIZES}}
+    results = {}
+    for conc in (1, args.batch_concurrency):
+        t0 = time.perf_counter()
+        results[conc] = register_batch(pairs, cfg, concurrency=conc)
+        ms = (time.perf_counter() - t0) * 1e3
+        out[f"concurrency_{conc}"] = {"ms": round(ms, 1), "ms_per_registration": round(ms / len(pairs), 2)}
+    a, b = results[1], results[args.batch_concurrency]
+    out["identical"] = all(np.array_equal(x[0], y[0]) for x, y in zip(a, b))
+    out["speedup"] = round(out["concurrency_1"]["ms"] / out[f"concurrency_{args.batch_concurrency}"]["ms"], 3)
+    inner = (slice(10, -10),) * 3
+    out["final_ncc_mean"] = round(float(np.mean([r[1][-1]["measure"] for r in b])), 6)
+    out["first_value_mean"] = round(float(np.mean([r[1][0]["first_value"] for r in b])), 6)
+    out["mean_abs_error_vox"] = round(float(np.mean([np.abs(r[0][inner] - d[inner]).mean()
+                                                     for r, d in zip(b, disps[1:])])), 4)
+    out["mean_abs_displacement_vox"] = round(float(np.mean([np.abs(d[inner]).mean() for d in disps[1:]])), 4)
+    return out
 
 
 def main():
@@ -500,6 +545,8 @@ def main():
         line["registration"] = registration(args, mov, fix, disp)
     if not args.no_lcc:
         line["lcc"] = lcc_timing()
+    if not args.no_batch:
+        line["batch"] = batch_timing(args)
     print(json.dumps(line))
 
 

@@ -420,6 +420,21 @@ SKM_API void skm_registration_desc_init(skm_registration_desc *desc);
 SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displacement_out,
                                 skm_scale_report *reports);
 
+/* Batched registration, no counterpart in the reference (whose SKMReg
+ * registers one pair per process): n independent registrations, e.g. the
+ * frames of a time series each registered to frame 0, run by up to
+ * `concurrency` host threads at once on the calling thread's device, each
+ * registration on its own streams, so that the small per-iteration launches
+ * of one frame overlap those of the others.  Every registration gives the
+ * same result as skm_register on its descriptor.  displacements_out[i] and
+ * reports[i] are as skm_register's for descs[i] (reports and reports[i] may be
+ * NULL); statuses (n entries, may be NULL) receives each registration's
+ * status.  Returns SKM_SUCCESS when all succeed, else the status of the first
+ * failing registration by index, with skm_last_error() naming it. */
+SKM_API skm_status skm_register_batch(const skm_registration_desc *descs, int32_t n, int32_t concurrency,
+                                      float *const *displacements_out, skm_scale_report *const *reports,
+                                      skm_status *statuses);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,12 +7,13 @@ interface over that ABI.
 from . import _capi
 from .evaluator import (MEASURE, ComputeAnisotropicTensor, EvaluationResult, GaussianSmoothing,
                         GpuKernelEvaluator, Grid, Image, Kernel, RegistrationConfig, control_point_grid,
-                        landmark_tre, read_image, read_landmarks, register_files, register_multiscale,
+                        landmark_tre, read_image, read_landmarks, register_batch, register_files, register_multiscale,
                         warp_image, write_image)
 
 __all__ = ["MEASURE", "ComputeAnisotropicTensor", "EvaluationResult", "GaussianSmoothing", "GpuKernelEvaluator",
            "Grid", "Image", "Kernel", "RegistrationConfig", "control_point_grid", "landmark_tre", "load_library",
-           "read_image", "read_landmarks", "register_files", "register_multiscale", "warp_image", "write_image"]
+           "read_image", "read_landmarks", "register_batch", "register_files", "register_multiscale", "warp_image",
+           "write_image"]
 
 
 def load_library():

@@ -159,6 +159,9 @@ _SIGNATURES = {
                                               c_double_p]),
     "skm_registration_desc_init": (None, [ctypes.POINTER(SkmRegistrationDesc)]),
     "skm_register": (ctypes.c_int, [ctypes.POINTER(SkmRegistrationDesc), c_float_p, ctypes.POINTER(SkmScaleReport)]),
+    "skm_register_batch": (ctypes.c_int, [ctypes.POINTER(SkmRegistrationDesc), ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.POINTER(c_float_p), ctypes.POINTER(ctypes.POINTER(SkmScaleReport)),
+                                          ctypes.POINTER(ctypes.c_int)]),
     "skm_warp_image": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p, ctypes.POINTER(SkmGrid),
                                       c_float_p, ctypes.POINTER(SkmGrid), ctypes.c_int32, ctypes.c_double,
                                       c_float_p]),

@@ -267,21 +267,30 @@ skm_status dev_alloc(T **p, size_t count)
     return SKM_SUCCESS;
 }
 
+// A device buffer holding a copy of the host array src, complete on return:
+// copied on the evaluator's stream s and waited for.  (A plain hipMemcpy runs
+// on the null stream, which the evaluator's non-blocking stream does not
+// wait for, and a copy from pageable memory may return before its transfer
+// has landed.)
 template <typename T>
-skm_status dev_upload(T **p, const T *src, size_t count)
+skm_status dev_upload(T **p, const T *src, size_t count, hipStream_t s)
 {
     SKM_TRY(dev_alloc(p, count));
-    if (src && count) SKM_HIP(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    if (src && count) {
+        SKM_HIP(hipMemcpyAsync(*p, src, sizeof(T) * count, hipMemcpyHostToDevice, s));
+        SKM_HIP(hipStreamSynchronize(s));
+    }
     return SKM_SUCCESS;
 }
 
 // A device buffer holding a copy of src (a host or, with device, a device
 // array); the copy has completed on return (the evaluator's own stream does
-// not wait for the null stream).
+// not wait for the null stream).  A device array is copied on the null
+// stream, after the work that the caller queued there to produce it.
 template <typename T>
-skm_status dev_copy_in(T **p, const T *src, size_t count, bool device)
+skm_status dev_copy_in(T **p, const T *src, size_t count, bool device, hipStream_t s)
 {
-    if (!device) return dev_upload(p, src, count);
+    if (!device) return dev_upload(p, src, count, s);
     SKM_TRY(dev_alloc(p, count));
     if (src && count) {
         SKM_HIP(hipMemcpy(*p, src, sizeof(T) * count, hipMemcpyDeviceToDevice));
@@ -340,7 +349,7 @@ skm_status build_static(skm_evaluator *ev)
         }
         long long *d_idx = nullptr;
         float *d_vals = nullptr;
-        skm_status st = dev_upload(&d_idx, idx.data(), idx.size());
+        skm_status st = dev_upload(&d_idx, idx.data(), idx.size(), ev->stream);
         if (st == SKM_SUCCESS) st = dev_alloc(&d_vals, (size_t)ncp * per);
         if (st == SKM_SUCCESS) {
             launch_gather_values(ev->d_weight, d_idx, per, ncp, d_vals, ev->stream);
@@ -419,9 +428,10 @@ skm_status build_static(skm_evaluator *ev)
             rw[d + 3] = std::min(std::max((int)((double)jc[d] + csp), 0), ds.cp.size[d] - 1);
         }
     }
-    SKM_TRY(dev_upload(&ev->d_cp_state, state.data(), state.size()));
+    SKM_TRY(dev_upload(&ev->d_cp_state, state.data(), state.size(), ev->stream));
     if (ds.kernel_mode != SKM_KERNEL_STATIONARY) {
-        SKM_TRY(dev_upload(&ev->d_cp_pstate, reinterpret_cast<const float4 *>(pstate.data()), (size_t)ncp * 2));
+        SKM_TRY(dev_upload(&ev->d_cp_pstate, reinterpret_cast<const float4 *>(pstate.data()), (size_t)ncp * 2,
+                             ev->stream));
         // largest lam a sample can look up: every weight voxel, and the
         // outside value (sigma 0 / identity covariance)
         const long long nw = grid_count(ds.weight, dim);
@@ -474,13 +484,13 @@ skm_status build_static(skm_evaluator *ev)
             const long long row = j / ds.cp.size[0];
             cp_row[row] = std::max(cp_row[row], (float)(pstate[(size_t)j * 8 + 7] * (1.0 + 1e-6)));
         }
-        SKM_TRY(dev_upload(&ev->d_cp_row_lam, cp_row.data(), cp_row.size()));
+        SKM_TRY(dev_upload(&ev->d_cp_row_lam, cp_row.data(), cp_row.size(), ev->stream));
         // each control point's share of the pair reach, pair_reach2(smin, lam, 0) in float
         const float sminf = (float)std::min(ds.sigma_c4, ds.sigma_c0);
         std::vector<float> cp_reach((size_t)ncp);
         for (long long j = 0; j < ncp; ++j)
             cp_reach[(size_t)j] = sminf * sminf * 0.5f * pstate[(size_t)j * 8 + 7] * 1.0001f;
-        SKM_TRY(dev_upload(&ev->d_cp_reach, cp_reach.data(), cp_reach.size()));
+        SKM_TRY(dev_upload(&ev->d_cp_reach, cp_reach.data(), cp_reach.size(), ev->stream));
         // lam bound per row of sample cells (my, mz), jitter included, for the
         // gather's reach; once for the evaluator's subsample and once for the
         // dense voxel grid
@@ -505,10 +515,10 @@ skm_status build_static(skm_evaluator *ev)
         release();
         if (e != hipSuccess) return fail(SKM_ERROR_HIP, std::string("row bounds: ") + hipGetErrorString(e));
     }
-    SKM_TRY(dev_upload(&ev->d_cp_window, win.data(), win.size()));
-    SKM_TRY(dev_upload(&ev->d_lcc_window, lwin.data(), lwin.size()));
-    SKM_TRY(dev_upload(&ev->d_dense_window, dwin.data(), dwin.size()));
-    SKM_TRY(dev_upload(&ev->d_reg_window, rwin.data(), rwin.size()));
+    SKM_TRY(dev_upload(&ev->d_cp_window, win.data(), win.size(), ev->stream));
+    SKM_TRY(dev_upload(&ev->d_lcc_window, lwin.data(), lwin.size(), ev->stream));
+    SKM_TRY(dev_upload(&ev->d_dense_window, dwin.data(), dwin.size(), ev->stream));
+    SKM_TRY(dev_upload(&ev->d_reg_window, rwin.data(), rwin.size(), ev->stream));
     return SKM_SUCCESS;
 }
 
@@ -743,7 +753,7 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         const long long np = (long long)(msize[0] + 3) * (msize[1] + 3) * (dim == 3 ? msize[2] + 3 : 1);
         float *raw = nullptr;
         double *work = nullptr;
-        if ((!device_data && (st = dev_upload(&raw, desc->moving_data, (size_t)nm)) != SKM_SUCCESS) ||
+        if ((!device_data && (st = dev_upload(&raw, desc->moving_data, (size_t)nm, ev->stream)) != SKM_SUCCESS) ||
             (st = dev_alloc(&work, (size_t)nm)) != SKM_SUCCESS || (st = dev_alloc(&ev->d_coeffs, (size_t)np)) != SKM_SUCCESS) {
             if (raw) (void)hipFree(raw);
             if (work) (void)hipFree(work);
@@ -755,11 +765,11 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         (void)hipFree(work);
         if (e != hipSuccess) return cleanup(fail(SKM_ERROR_HIP, std::string("prefilter: ") + hipGetErrorString(e)));
     }
-    if ((st = dev_copy_in(&ev->d_fixed, desc->fixed_data, (size_t)pr.n_vox, device_data)) != SKM_SUCCESS)
+    if ((st = dev_copy_in(&ev->d_fixed, desc->fixed_data, (size_t)pr.n_vox, device_data, ev->stream)) != SKM_SUCCESS)
         return cleanup(st);
     if (desc->kernel_mode != SKM_KERNEL_STATIONARY) {
         size_t per = desc->kernel_mode == SKM_KERNEL_WEIGHT_IMAGE ? 1 : (size_t)dim * dim;
-        if ((st = dev_copy_in(&ev->d_weight, desc->weight_data, (size_t)nw * per, device_data)) != SKM_SUCCESS)
+        if ((st = dev_copy_in(&ev->d_weight, desc->weight_data, (size_t)nw * per, device_data, ev->stream)) != SKM_SUCCESS)
             return cleanup(st);
     }
     if ((st = build_static(ev)) != SKM_SUCCESS) return cleanup(st);
@@ -783,7 +793,7 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         (st = dev_alloc(&ev->d_hreg_partials, (size_t)((pr.n_cp + kThreads - 1) / kThreads) * 4 + 4)) ||
         (st = dev_alloc(&ev->d_sc, 1)) ||
         (st = dev_alloc(&ev->d_offsets, pr.sub > 1 ? pr.n_sub * dim : 1)) ||
-        (st = dev_upload(&ev->d_rng, &desc->rng_state, 1)))
+        (st = dev_upload(&ev->d_rng, &desc->rng_state, 1, ev->stream)))
         return cleanup(st);
     pr.fixed_data = ev->d_fixed;
     pr.coeffs = ev->d_coeffs;
@@ -875,7 +885,7 @@ skm_status skm::set_displacement_field_impl(skm_evaluator *ev, const skm_grid *g
     if (!grid || !df_aos) return SKM_SUCCESS;
     SKM_TRY(validate_grid(*grid, ev->dim, "displacement field", 1));
     size_t n = (size_t)grid_count(*grid, ev->dim) * ev->dim;
-    SKM_TRY(dev_copy_in(&ev->d_df, df_aos, n, device));
+    SKM_TRY(dev_copy_in(&ev->d_df, df_aos, n, device, ev->stream));
     ev->pr.df = to_grid(*grid, ev->dim);
     ev->pr.df_data = ev->d_df;
     ev->pr.has_df = 1;

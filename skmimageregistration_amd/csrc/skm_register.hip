@@ -11,12 +11,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <new>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "skm.h"
@@ -256,7 +260,11 @@ struct DeviceArray {
     {
         skm_status st = alloc(count);
         if (st != SKM_SUCCESS) return st;
+        // waited for: the evaluators' non-blocking streams read these buffers
+        // without waiting for the null stream, and a copy from pageable
+        // memory may return before its transfer has landed
         hipError_t e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("upload: ") + hipGetErrorString(e));
         return SKM_SUCCESS;
     }
@@ -589,3 +597,59 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
 }
 
 }  // extern "C"
+
+// Batched registration: a shared counter hands out the registrations to the
+// worker threads in index order; each worker makes the caller's device
+// current and runs skm_register, whose evaluators take their own streams from
+// the device's pool.  Errors are reported per registration (the error text
+// is per thread) and the first by index is returned.
+SKM_API skm_status skm_register_batch(const skm_registration_desc *descs, int32_t n, int32_t concurrency,
+                                      float *const *displacements_out, skm_scale_report *const *reports,
+                                      skm_status *statuses)
+{
+    if (n < 0) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "n must be >= 0");
+    if (n == 0) return SKM_SUCCESS;
+    if (!descs || !displacements_out) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    if (concurrency < 1) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "concurrency must be >= 1");
+    for (int32_t i = 0; i < n; ++i)
+        if (!displacements_out[i])
+            return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "displacements_out[" + std::to_string(i) + "] is null");
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return set_last_error(SKM_ERROR_HIP, "hipGetDevice failed");
+    std::vector<skm_status> st((size_t)n, SKM_SUCCESS);
+    std::vector<std::string> msg((size_t)n);
+    std::atomic<int32_t> next{0};
+    auto worker = [&]() {
+        if (hipSetDevice(device) != hipSuccess) {
+            // no registration runs on this thread; the others take them all
+            return;
+        }
+        for (int32_t i = next.fetch_add(1); i < n; i = next.fetch_add(1)) {
+            try {
+                st[i] = skm_register(&descs[i], displacements_out[i], reports ? reports[i] : nullptr);
+                if (st[i] != SKM_SUCCESS) msg[i] = skm_last_error();
+            } catch (const std::bad_alloc &) {
+                st[i] = SKM_ERROR_OUT_OF_MEMORY;
+                msg[i] = "host allocation failed";
+            } catch (const std::exception &e) {
+                st[i] = SKM_ERROR_HIP;
+                msg[i] = e.what();
+            }
+        }
+    };
+    const int32_t workers = std::min(concurrency, n);
+    std::vector<std::thread> pool;
+    try {
+        for (int32_t w = 1; w < workers; ++w) pool.emplace_back(worker);
+    } catch (const std::system_error &) {
+        // fewer threads than asked: the ones started and this one do the work
+    }
+    worker();
+    for (auto &t : pool) t.join();
+    if (next.load() < n) return set_last_error(SKM_ERROR_HIP, "hipSetDevice failed on the worker threads");
+    if (statuses)
+        for (int32_t i = 0; i < n; ++i) statuses[i] = st[i];
+    for (int32_t i = 0; i < n; ++i)
+        if (st[i] != SKM_SUCCESS) return set_last_error(st[i], "registration " + std::to_string(i) + ": " + msg[i]);
+    return SKM_SUCCESS;
+}

@@ -466,18 +466,10 @@ class RegistrationConfig:
         return self.values[key]
 
 
-def register_multiscale(reference, target, config=None, weight_image=None, weight_tensor=None, guidance_image=None,
-                        restart_displacement=None, temp_directory=None, reference_landmarks=None,
-                        target_landmarks=None):
-    """The scale-level loop of SKMReg (src/SKMReg.cxx:84-525) through
-    skm_register.  reference (moving) and target (fixed) are Images; the
-    optional weight image / tensor / guidance image are Images on one common
-    weight grid.  With temp_directory, SKMReg's result files are written
-    there and restart_at without restart_displacement resumes from its
-    df-full file.  Landmarks ((n, D) arrays) give the TRE per level.
-    Returns (displacement on the target grid, shape target.data.shape + (D,),
-    list of per-level report dicts)."""
-    lib = _capi.load()
+def _registration_desc(lib, reference, target, config, weight_image, weight_tensor, guidance_image,
+                       restart_displacement, temp_directory, reference_landmarks, target_landmarks):
+    """skm_registration_desc of one registration, the arrays it points to
+    (kept alive by the caller), the displacement output and the reports."""
     cfg = config or RegistrationConfig()
     dim = target.dim
     d = _capi.SkmRegistrationDesc()
@@ -536,14 +528,55 @@ def register_multiscale(reference, target, config=None, weight_image=None, weigh
             d.num_landmarks = len(rl)
             d.reference_landmarks = rl.ctypes.data_as(_capi.c_double_p)
             d.target_landmarks = tl.ctypes.data_as(_capi.c_double_p)
+    keep += [ref, tar]
     out = np.empty(tar.shape + (dim,), np.float32)
     reps = (_capi.SkmScaleReport * n)()
+    return d, keep, out, reps
+
+
+def _reports(reps, dim):
+    return [{"value": r.value, "measure": r.measure, "first_value": r.first_value, "iterations": r.iterations,
+             "cp_size": tuple(r.cp_size[:dim]), "maximum_tilt": r.maximum_tilt, "ms_setup": r.ms_setup,
+             "ms_optimize": r.ms_optimize, "ms_displacement": r.ms_displacement, "tre": r.tre,
+             "zero_parameters": r.zero_parameters} for r in reps]
+
+
+def register_multiscale(reference, target, config=None, weight_image=None, weight_tensor=None, guidance_image=None,
+                        restart_displacement=None, temp_directory=None, reference_landmarks=None,
+                        target_landmarks=None):
+    """The scale-level loop of SKMReg (src/SKMReg.cxx:84-525) through
+    skm_register.  reference (moving) and target (fixed) are Images; the
+    optional weight image / tensor / guidance image are Images on one common
+    weight grid.  With temp_directory, SKMReg's result files are written
+    there and restart_at without restart_displacement resumes from its
+    df-full file.  Landmarks ((n, D) arrays) give the TRE per level.
+    Returns (displacement on the target grid, shape target.data.shape + (D,),
+    list of per-level report dicts)."""
+    lib = _capi.load()
+    d, keep, out, reps = _registration_desc(lib, reference, target, config, weight_image, weight_tensor,
+                                            guidance_image, restart_displacement, temp_directory,
+                                            reference_landmarks, target_landmarks)
     check(lib.skm_register(ctypes.byref(d), fptr(out), reps))
-    reports = [{"value": r.value, "measure": r.measure, "first_value": r.first_value, "iterations": r.iterations,
-                "cp_size": tuple(r.cp_size[:dim]), "maximum_tilt": r.maximum_tilt, "ms_setup": r.ms_setup,
-                "ms_optimize": r.ms_optimize, "ms_displacement": r.ms_displacement, "tre": r.tre,
-                "zero_parameters": r.zero_parameters} for r in reps]
-    return out, reports
+    return out, _reports(reps, target.dim)
+
+
+def register_batch(pairs, config=None, concurrency=4):
+    """Independent registrations through skm_register_batch (no counterpart
+    in the reference): pairs is a sequence of (reference, target) Images, all
+    registered with one configuration, up to `concurrency` at once on the
+    current device.  Each result equals register_multiscale's for its pair.
+    Returns a list of (displacement, per-level report dicts)."""
+    lib = _capi.load()
+    jobs = [_registration_desc(lib, r, t, config, None, None, None, None, None, None, None) for r, t in pairs]
+    n = len(jobs)
+    descs = (_capi.SkmRegistrationDesc * max(n, 1))()
+    for i, (d, _, _, _) in enumerate(jobs):
+        descs[i] = d
+    outs = (_capi.c_float_p * max(n, 1))(*[fptr(o) for _, _, o, _ in jobs])
+    reps = (ctypes.POINTER(_capi.SkmScaleReport) * max(n, 1))(
+        *[ctypes.cast(r, ctypes.POINTER(_capi.SkmScaleReport)) for _, _, _, r in jobs])
+    check(lib.skm_register_batch(descs, n, int(concurrency), outs, reps, None))
+    return [(o, _reports(r, t.dim)) for (_, _, o, r), (_, t) in zip(jobs, pairs)]
 
 
 def register_files(config):

@@ -73,3 +73,36 @@ def sliding_3d(shape=(128, 256, 256), shift=4.0, breathing=8.0, seeds=(21, 22, 2
     disp[..., 2] = breathing * np.sin(np.pi * zz / max(nz - 1, 1)) * np.exp(-((xx - nx / 2) ** 2 + (yy - ny / 2) ** 2) / (2 * (0.3 * nx) ** 2))
     target = _warp(img, disp) + np.random.default_rng(seeds[2]).normal(0.0, 0.02, size=shape)
     return img.astype(np.float32), target.astype(np.float32), disp
+
+
+# frame sizes of the time series, (z, y, x): 96^3, 128x128x64 and 160x160x80
+SERIES_SIZES = ((96, 96, 96), (64, 128, 128), (80, 160, 160))
+
+
+def time_series_3d(n_frames=32, n=20, smooth=1.5, max_disp=7.0, noise=0.02, seed=31, sizes=SERIES_SIZES):
+    """Time series of uneven-sized 3D frames.  One phantom of `n` random
+    ellipsoids (intensity U[0,1], smoothed with a Gaussian) fills the box
+    that holds every size; frame t, of a size drawn from `sizes`, samples it
+    on its own grid (origin 0, spacing 1) at x + sin(2 pi t / n_frames) d(x),
+    with d a smooth field of maximum length max_disp, plus N(0, noise).
+    Frame 0 (phase 0) is the reference the others register to.
+    Returns (frames, displacements), each frame's displacement in (x, y, z)
+    order on its grid."""
+    rng = np.random.default_rng(seed)
+    box = tuple(max(s[a] for s in sizes) for a in range(3))
+    phantom, _, _ = ellipsoids_3d(box, n=n, smooth=smooth, max_disp=0.0, noise=0.0,
+                                  seeds=(seed + 1, seed + 2, seed + 3))
+    coeffs = ndimage.spline_filter(phantom.astype(np.float64), order=3, mode="nearest")
+    field = _rbf_field(box, 8, 0.25 * min(box), max_disp, np.random.default_rng(seed + 4))
+    picks = rng.integers(0, len(sizes), size=n_frames)
+    frames, disps = [], []
+    for t in range(n_frames):
+        shape = sizes[picks[t]]
+        d = np.sin(2.0 * np.pi * t / n_frames) * field[:shape[0], :shape[1], :shape[2]]
+        grids = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
+        coords = [grids[a] + d[..., 2 - a] for a in range(3)]
+        img = ndimage.map_coordinates(coeffs, coords, order=3, mode="nearest", prefilter=False)
+        img += np.random.default_rng(seed + 100 + t).normal(0.0, noise, size=shape)
+        frames.append(img.astype(np.float32))
+        disps.append(d)
+    return frames, disps

@@ -112,3 +112,21 @@ def test_host_regularizers_match_oracle(dim):
                                            *[ctypes.byref(x) for x in ov])
         np.testing.assert_array_equal(a, b)
         assert rv.value == oreg
+
+
+def test_register_batch_rejects_bad_arguments():
+    """skm_register_batch validates before touching a device."""
+    lib = _capi.load()
+    descs = (_capi.SkmRegistrationDesc * 1)()
+    lib.skm_registration_desc_init(ctypes.byref(descs[0]))
+    buf = np.empty(8, np.float32)
+    outs = (_capi.c_float_p * 1)(buf.ctypes.data_as(_capi.c_float_p))
+    assert lib.skm_register_batch(descs, 0, 1, None, None, None) == 0  # nothing to do
+    assert lib.skm_register_batch(descs, -1, 1, outs, None, None) == _capi.SKM_ERROR_INVALID_ARGUMENT
+    assert lib.skm_register_batch(None, 1, 1, outs, None, None) == _capi.SKM_ERROR_INVALID_ARGUMENT
+    assert lib.skm_register_batch(descs, 1, 1, None, None, None) == _capi.SKM_ERROR_INVALID_ARGUMENT
+    assert lib.skm_register_batch(descs, 1, 0, outs, None, None) == _capi.SKM_ERROR_INVALID_ARGUMENT
+    assert b"concurrency" in lib.skm_last_error()
+    nulls = (_capi.c_float_p * 1)()
+    assert lib.skm_register_batch(descs, 1, 1, nulls, None, None) == _capi.SKM_ERROR_INVALID_ARGUMENT
+    assert b"displacements_out[0]" in lib.skm_last_error()
