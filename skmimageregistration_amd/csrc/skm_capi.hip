@@ -8,8 +8,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -108,6 +110,107 @@ void pack_upper(int dim, const double *sym, float sdet, float lam, float *ps)
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- device memory
+
+namespace {
+
+struct BlockCache {
+    std::mutex mutex;
+    std::unordered_map<void *, std::pair<int, size_t>> live;   // block -> (device, class bytes)
+    std::map<std::pair<int, size_t>, std::vector<void *>> idle;  // (device, class bytes) -> blocks
+    std::map<int, size_t> idle_bytes;                          // per device
+};
+
+BlockCache &block_cache()
+{
+    // never destroyed: no HIP call may run during the process's exit
+    static BlockCache *cache = new BlockCache;
+    return *cache;
+}
+
+constexpr size_t kIdleCap = size_t(16) << 30;  // idle bytes kept per device
+
+// 256 bytes, or bytes rounded up to an eighth of the largest power of two
+// below it (at most 12.5% unused)
+size_t size_class(size_t bytes)
+{
+    if (bytes <= 256) return 256;
+    size_t pow2 = 1;
+    while (pow2 <= (bytes - 1) / 2) pow2 *= 2;
+    const size_t step = std::max<size_t>(256, pow2 / 8);
+    return (bytes + step - 1) / step * step;
+}
+
+}  // namespace
+
+hipError_t skm::device_alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const size_t cls = size_class(bytes);
+    BlockCache &c = block_cache();
+    {
+        std::lock_guard<std::mutex> lock(c.mutex);
+        auto it = c.idle.find({dev, cls});
+        if (it != c.idle.end() && !it->second.empty()) {
+            *p = it->second.back();
+            it->second.pop_back();
+            c.idle_bytes[dev] -= cls;
+            c.live[*p] = {dev, cls};
+            return hipSuccess;
+        }
+    }
+    e = hipMalloc(p, cls);
+    if (e == hipErrorOutOfMemory) {
+        // give this device's idle blocks back and try once more
+        std::vector<void *> drop;
+        {
+            std::lock_guard<std::mutex> lock(c.mutex);
+            for (auto it = c.idle.begin(); it != c.idle.end();) {
+                if (it->first.first == dev) {
+                    drop.insert(drop.end(), it->second.begin(), it->second.end());
+                    it = c.idle.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            c.idle_bytes[dev] = 0;
+        }
+        for (void *q : drop) (void)hipFree(q);
+        (void)hipGetLastError();
+        e = hipMalloc(p, cls);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    std::lock_guard<std::mutex> lock(c.mutex);
+    c.live[*p] = {dev, cls};
+    return hipSuccess;
+}
+
+void skm::device_release(void *p)
+{
+    if (!p) return;
+    BlockCache &c = block_cache();
+    {
+        std::lock_guard<std::mutex> lock(c.mutex);
+        auto it = c.live.find(p);
+        if (it != c.live.end()) {
+            const std::pair<int, size_t> key = it->second;
+            c.live.erase(it);
+            if (c.idle_bytes[key.first] + key.second <= kIdleCap) {
+                c.idle[key].push_back(p);
+                c.idle_bytes[key.first] += key.second;
+                return;
+            }
+        }
+    }
+    (void)hipFree(p);
+}
 
 // ---------------------------------------------------------------- evaluator
 
@@ -217,7 +320,7 @@ struct skm_evaluator {
                         d_hreg_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng};
         for (void *p : ptrs)
-            if (p) (void)hipFree(p);
+            if (p) device_release(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
         for (auto e : event_pool) (void)hipEventDestroy(e);
         if (own_stream && stream) return_stream(stream_device, stream);
@@ -263,7 +366,7 @@ template <typename T>
 skm_status dev_alloc(T **p, size_t count)
 {
     if (count == 0) count = 1;
-    SKM_HIP(hipMalloc((void **)p, sizeof(T) * count));
+    SKM_HIP(device_alloc((void **)p, sizeof(T) * count));
     return SKM_SUCCESS;
 }
 
@@ -359,8 +462,8 @@ skm_status build_static(skm_evaluator *ev)
             if (e == hipSuccess) e = hipStreamSynchronize(ev->stream);
             if (e != hipSuccess) st = fail(SKM_ERROR_HIP, std::string("weight gather: ") + hipGetErrorString(e));
         }
-        if (d_idx) (void)hipFree(d_idx);
-        if (d_vals) (void)hipFree(d_vals);
+        if (d_idx) device_release(d_idx);
+        if (d_vals) device_release(d_vals);
         if (st != SKM_SUCCESS) return st;
     }
     for (long long j = 0; j < ncp; ++j) {
@@ -444,9 +547,10 @@ skm_status build_static(skm_evaluator *ev)
         unsigned *bbad = nullptr;
         const unsigned nb = weight_lam_blocks(nw);
         auto release = [&]() {
-            if (wlam) (void)hipFree(wlam);
-            if (bmax) (void)hipFree(bmax);
-            if (bbad) (void)hipFree(bbad);
+            (void)hipStreamSynchronize(ev->stream);  // error paths: the scan may still run
+            if (wlam) device_release(wlam);
+            if (bmax) device_release(bmax);
+            if (bbad) device_release(bbad);
         };
         skm_status st;
         if ((st = dev_alloc(&wlam, (size_t)nw)) != SKM_SUCCESS || (st = dev_alloc(&bmax, nb)) != SKM_SUCCESS ||
@@ -755,14 +859,14 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         double *work = nullptr;
         if ((!device_data && (st = dev_upload(&raw, desc->moving_data, (size_t)nm, ev->stream)) != SKM_SUCCESS) ||
             (st = dev_alloc(&work, (size_t)nm)) != SKM_SUCCESS || (st = dev_alloc(&ev->d_coeffs, (size_t)np)) != SKM_SUCCESS) {
-            if (raw) (void)hipFree(raw);
-            if (work) (void)hipFree(work);
+            if (raw) device_release(raw);
+            if (work) device_release(work);
             return cleanup(st);
         }
         launch_bspline_coefficients(dim, msize, device_data ? desc->moving_data : raw, work, ev->d_coeffs, ev->stream);
         const hipError_t e = hipStreamSynchronize(ev->stream);
-        (void)hipFree(raw);
-        (void)hipFree(work);
+        device_release(raw);
+        device_release(work);
         if (e != hipSuccess) return cleanup(fail(SKM_ERROR_HIP, std::string("prefilter: ") + hipGetErrorString(e)));
     }
     if ((st = dev_copy_in(&ev->d_fixed, desc->fixed_data, (size_t)pr.n_vox, device_data, ev->stream)) != SKM_SUCCESS)
@@ -877,7 +981,7 @@ skm_status skm::set_displacement_field_impl(skm_evaluator *ev, const skm_grid *g
     if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
     SKM_HIP(hipStreamSynchronize(ev->stream));
     if (ev->d_df) {
-        SKM_HIP(hipFree(ev->d_df));
+        device_release(ev->d_df);
         ev->d_df = nullptr;
     }
     ev->pr.has_df = 0;
@@ -1114,7 +1218,8 @@ SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *met
     if (!ev->d_opt) SKM_TRY(dev_alloc(&ev->d_opt, 1));
     if (opt->return_best && !ev->d_best) SKM_TRY(dev_alloc(&ev->d_best, np));
     if (ev->records_cap < opt->iterations) {
-        if (ev->d_records) SKM_HIP(hipFree(ev->d_records));
+        SKM_HIP(hipStreamSynchronize(s));  // the previous records may still be read
+        device_release(ev->d_records);
         ev->d_records = nullptr;
         SKM_TRY(dev_alloc(&ev->d_records, (size_t)opt->iterations));
         ev->records_cap = opt->iterations;

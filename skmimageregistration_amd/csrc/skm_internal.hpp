@@ -14,6 +14,14 @@ namespace skm {
 // Stores msg as the calling thread's last error (skm_last_error) and returns st.
 skm_status set_last_error(skm_status st, const std::string &msg);
 
+// Device memory of the library.  A released block is kept for later requests
+// of its size class on its device, so that repeated set-ups (levels, frames of
+// a batch) call neither hipMalloc nor hipFree: hipFree waits for all work on
+// the device, which would stall every other registration in flight.  A block
+// may be released only once the work that uses it has completed.
+hipError_t device_alloc(void **p, size_t bytes);
+void device_release(void *p);
+
 // Per-control-point kernel state (static for an evaluator): sig, sdet, cov[9], support.
 constexpr int kCpStateStride = 12;
 

@@ -238,17 +238,14 @@ struct DeviceArray {
     DeviceArray() = default;
     DeviceArray(const DeviceArray &) = delete;
     DeviceArray &operator=(const DeviceArray &) = delete;
-    ~DeviceArray()
-    {
-        if (p) (void)hipFree(p);
-    }
+    ~DeviceArray() { skm::device_release(p); }
     skm_status alloc(size_t count)
     {
         if (p && n == count) return SKM_SUCCESS;
-        if (p) (void)hipFree(p);
+        skm::device_release(p);
         p = nullptr;
         n = 0;
-        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+        hipError_t e = skm::device_alloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
             return set_last_error(SKM_ERROR_HIP, std::string("registration buffer: ") + hipGetErrorString(e));

@@ -376,20 +376,19 @@ __global__ void __launch_bounds__(kThreads) k_tensor_finish(const float *__restr
 #define SKM_T_HIP(expr)                                                                                            \
     do {                                                                                                           \
         hipError_t e_ = (expr);                                                                                    \
-        if (e_ != hipSuccess)                                                                                      \
-            return set_last_error(e_ == hipErrorOutOfMemory ? SKM_ERROR_OUT_OF_MEMORY : SKM_ERROR_HIP,            \
+        if (e_ != hipSuccess) {                                                                                    \
+            (void)hipStreamSynchronize(nullptr); /* temporaries are released on return */                          \
+            return set_last_error(e_ == hipErrorOutOfMemory ? SKM_ERROR_OUT_OF_MEMORY : SKM_ERROR_HIP,             \
                                   std::string(#expr) + ": " + hipGetErrorString(e_));                              \
+        }                                                                                                          \
     } while (0)
 
 // Device buffer released on every return path.
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t count) { return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)); }
+    ~DevBuf() { device_release(p); }
+    hipError_t alloc(size_t count) { return device_alloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)); }
 };
 
 int launch_blocks(long long work)

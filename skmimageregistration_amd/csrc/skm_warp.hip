@@ -203,19 +203,21 @@ __global__ __launch_bounds__(kThreads) void k_warp_bspline3(const double *__rest
 #define SKM_W_HIP(expr)                                                                                            \
     do {                                                                                                           \
         hipError_t e_ = (expr);                                                                                    \
-        if (e_ != hipSuccess)                                                                                      \
-            return set_last_error(e_ == hipErrorOutOfMemory ? SKM_ERROR_OUT_OF_MEMORY : SKM_ERROR_HIP,            \
+        if (e_ != hipSuccess) {                                                                                    \
+            (void)hipStreamSynchronize(nullptr); /* temporaries are released on return */                          \
+            return set_last_error(e_ == hipErrorOutOfMemory ? SKM_ERROR_OUT_OF_MEMORY : SKM_ERROR_HIP,             \
                                   std::string(#expr) + ": " + hipGetErrorString(e_));                              \
+        }                                                                                                          \
     } while (0)
 
 template <typename T>
 struct Buf {
     T *p = nullptr;
-    ~Buf()
+    ~Buf() { device_release(p); }
+    hipError_t alloc(long long count)
     {
-        if (p) (void)hipFree(p);
+        return device_alloc((void **)&p, (size_t)std::max<long long>(count, 1) * sizeof(T));
     }
-    hipError_t alloc(long long count) { return hipMalloc((void **)&p, (size_t)std::max<long long>(count, 1) * sizeof(T)); }
 };
 
 skm_status check_grid(const skm_grid *g, int dim, const char *what)
