@@ -267,6 +267,9 @@ void return_stream(int dev, hipStream_t s)
 
 }  // namespace
 
+hipError_t skm::take_pool_stream(hipStream_t *out) { return take_stream(out); }
+void skm::return_pool_stream(int dev, hipStream_t s) { return_stream(dev, s); }
+
 struct skm_evaluator {
     skm_desc desc{};
     int dim = 3;
@@ -777,7 +780,7 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
 
 // skm_create; with device_data the image, weight and tensor pointers of the
 // descriptor are device pointers (copied on the device, no PCIe transfer).
-skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out)
+skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out, hipStream_t stream)
 {
     if (!desc || !out) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
@@ -822,9 +825,13 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         return st;
     };
     skm_status st;
-    if (hipGetDevice(&ev->stream_device) != hipSuccess || take_stream(&ev->stream) != hipSuccess)
-        return cleanup(fail(SKM_ERROR_HIP, "hipStreamCreate failed"));
-    ev->own_stream = true;
+    if (hipGetDevice(&ev->stream_device) != hipSuccess) return cleanup(fail(SKM_ERROR_HIP, "hipGetDevice failed"));
+    if (stream) {
+        ev->stream = stream;  // the caller's (not returned to the pool)
+    } else {
+        if (take_stream(&ev->stream) != hipSuccess) return cleanup(fail(SKM_ERROR_HIP, "hipStreamCreate failed"));
+        ev->own_stream = true;
+    }
 
     Problem &pr = ev->pr;
     pr.dim = dim;
@@ -922,8 +929,12 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
     pr.cp_window = ev->d_cp_window;
     pr.reg_window = ev->d_reg_window;
     pr.offsets = ev->d_offsets;
-    if (hipMemset(ev->d_offsets, 0, sizeof(int) * (pr.sub > 1 ? pr.n_sub * dim : 1)) != hipSuccess)
-        return cleanup(fail(SKM_ERROR_HIP, "hipMemset failed"));
+    // on the evaluator's stream, complete on return: a memset on the null
+    // stream is not ordered before the stream's kernels, and could land
+    // after the first draw of the offsets
+    if (hipMemsetAsync(ev->d_offsets, 0, sizeof(int) * (pr.sub > 1 ? pr.n_sub * dim : 1), ev->stream) != hipSuccess ||
+        hipStreamSynchronize(ev->stream) != hipSuccess)
+        return cleanup(fail(SKM_ERROR_HIP, "clearing the offsets failed"));
     *out = ev;
     return SKM_SUCCESS;
 }

@@ -90,15 +90,23 @@ void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
 void launch_voxel_states(const Problem &pr, float4 *out, hipStream_t s);
 
 // skm_create with, when device_data, device pointers in the descriptor (skm_capi.hip)
-skm_status create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out);
+skm_status create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out,
+                            hipStream_t stream = nullptr);
 // skm_set_displacement_field from a device array
 skm_status set_displacement_field_device(skm_evaluator *ev, const skm_grid *grid, const float *d_df_aos);
 skm_status set_displacement_field_impl(skm_evaluator *ev, const skm_grid *grid, const float *df_aos, bool device);
-// GaussianSmoothing / ComputeAnisotropicTensor on device arrays (skm_tensor.hip)
-skm_status gaussian_smooth_device(int dim, const skm_grid &grid, const float *d_in, double variance, float *d_out);
+// GaussianSmoothing / ComputeAnisotropicTensor on device arrays (skm_tensor.hip),
+// on `stream`, complete on return
+skm_status gaussian_smooth_device(int dim, const skm_grid &grid, const float *d_in, double variance, float *d_out,
+                                  hipStream_t stream = nullptr);
 skm_status anisotropic_tensor_device(int dim, const skm_grid &grid, const float *d_image,
                                      const skm_tensor_options &opt, float *d_tensor_out, double *maximum_tilt,
-                                     double *maximum_weight, double *maximum_eigenvalue_covariance);
+                                     double *maximum_weight, double *maximum_eigenvalue_covariance,
+                                     hipStream_t stream = nullptr);
+// A synchronised non-blocking stream of the current device's pool, and its
+// return (synchronised, then kept for the next user on device `dev`).
+hipError_t take_pool_stream(hipStream_t *out);
+void return_pool_stream(int dev, hipStream_t s);
 
 // set-up (skm_setup.hip)
 // cubic B-spline coefficients (mirror boundary) in FP64, unpadded, in d_work

@@ -144,19 +144,19 @@ __global__ __launch_bounds__(kPreThreads) void k_pre_rescale(float *__restrict__
 // ThresholdImage (with thresholds) then RescaleImage(0, 1) of a device image;
 // stats holds kPreBlocks + 1 entries.
 skm_status preprocess_device(float *d_img, long long n, bool thresholds, double thresh_max, double thresh_min,
-                             PreStats *stats)
+                             PreStats *stats, hipStream_t s)
 {
     const int blocks = (int)std::min<long long>(kPreBlocks, (n + kPreThreads - 1) / kPreThreads);
     if (thresholds) {
-        k_pre_partials<<<blocks, kPreThreads>>>(d_img, n, thresh_max, thresh_min, stats);
-        k_pre_final<<<1, 64>>>(stats, blocks);
-        k_pre_threshold<<<blocks, kPreThreads>>>(d_img, n, stats + blocks);
+        k_pre_partials<<<blocks, kPreThreads, 0, s>>>(d_img, n, thresh_max, thresh_min, stats);
+        k_pre_final<<<1, 64, 0, s>>>(stats, blocks);
+        k_pre_threshold<<<blocks, kPreThreads, 0, s>>>(d_img, n, stats + blocks);
     }
-    k_pre_partials<<<blocks, kPreThreads>>>(d_img, n, 0.0, 0.0, stats);
-    k_pre_final<<<1, 64>>>(stats, blocks);
-    k_pre_rescale<<<blocks, kPreThreads>>>(d_img, n, stats + blocks);
+    k_pre_partials<<<blocks, kPreThreads, 0, s>>>(d_img, n, 0.0, 0.0, stats);
+    k_pre_final<<<1, 64, 0, s>>>(stats, blocks);
+    k_pre_rescale<<<blocks, kPreThreads, 0, s>>>(d_img, n, stats + blocks);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("preprocessing: ") + hipGetErrorString(e));
     return SKM_SUCCESS;
 }
@@ -253,25 +253,47 @@ struct DeviceArray {
         n = count;
         return SKM_SUCCESS;
     }
-    skm_status upload(const T *src, size_t count)
+    // copies on stream s, complete on return (the evaluators' streams read
+    // these buffers without waiting for s)
+    skm_status upload(const T *src, size_t count, hipStream_t s)
     {
         skm_status st = alloc(count);
         if (st != SKM_SUCCESS) return st;
-        // waited for: the evaluators' non-blocking streams read these buffers
-        // without waiting for the null stream, and a copy from pageable
-        // memory may return before its transfer has landed
-        hipError_t e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        hipError_t e = hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("upload: ") + hipGetErrorString(e));
         return SKM_SUCCESS;
     }
-    skm_status download(std::vector<T> &dst) const
+    skm_status download(std::vector<T> &dst, hipStream_t s) const
     {
         dst.resize(n);
-        hipError_t e = hipMemcpy(dst.data(), p, n * sizeof(T), hipMemcpyDeviceToHost);
+        hipError_t e = hipMemcpyAsync(dst.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("download: ") + hipGetErrorString(e));
         return SKM_SUCCESS;
     }
+};
+
+// The registration's own stream for its set-up work and copies, so that
+// registrations in flight together do not wait for each other's set-up on
+// the null stream; returned (synchronised) at the end.
+struct PoolStream {
+    hipStream_t s = nullptr;
+    int dev = 0;
+    PoolStream() = default;
+    PoolStream(const PoolStream &) = delete;
+    PoolStream &operator=(const PoolStream &) = delete;
+    ~PoolStream()
+    {
+        if (s) skm::return_pool_stream(dev, s);
+    }
+};
+
+// Waits for a stream when it goes out of scope (before the arrays declared
+// ahead of it are released).
+struct StreamWait {
+    hipStream_t s;
+    ~StreamWait() { (void)hipStreamSynchronize(s); }
 };
 
 struct EvaluatorGuard {
@@ -344,18 +366,23 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
     // host copies (result files and warped outputs only) are read back when
     // files are written
     const Output out{desc->temp_directory ? std::string(desc->temp_directory) : std::string(), dim};
+    PoolStream setup;
+    if (hipGetDevice(&setup.dev) != hipSuccess || skm::take_pool_stream(&setup.s) != hipSuccess)
+        return set_last_error(SKM_ERROR_HIP, "registration stream");
+    const hipStream_t ss = setup.s;
     DeviceArray<float> d_ref, d_tar, d_ref_s, d_tar_s, d_weight, d_guide, d_tensor, d_prev, d_cp, d_cpw;
     DeviceArray<PreStats> d_pre;
-    SKM_R_TRY(d_ref.upload(desc->reference_data, (size_t)n_ref));
-    SKM_R_TRY(d_tar.upload(desc->target_data, (size_t)n_tar));
+    StreamWait setup_wait{ss};
+    SKM_R_TRY(d_ref.upload(desc->reference_data, (size_t)n_ref, ss));
+    SKM_R_TRY(d_tar.upload(desc->target_data, (size_t)n_tar, ss));
     SKM_R_TRY(d_pre.alloc(kPreBlocks + 1));
     const bool thresholds = desc->use_intensity_thresholds != 0;
-    SKM_R_TRY(preprocess_device(d_ref.p, n_ref, thresholds, desc->threshold_max, desc->threshold_min, d_pre.p));
-    SKM_R_TRY(preprocess_device(d_tar.p, n_tar, thresholds, desc->threshold_max, desc->threshold_min, d_pre.p));
+    SKM_R_TRY(preprocess_device(d_ref.p, n_ref, thresholds, desc->threshold_max, desc->threshold_min, d_pre.p, ss));
+    SKM_R_TRY(preprocess_device(d_tar.p, n_tar, thresholds, desc->threshold_max, desc->threshold_min, d_pre.p, ss));
     std::vector<float> ref, tar;
     if (out.on()) {
-        SKM_R_TRY(d_ref.download(ref));
-        SKM_R_TRY(d_tar.download(tar));
+        SKM_R_TRY(d_ref.download(ref, ss));
+        SKM_R_TRY(d_tar.download(tar, ss));
     }
     SKM_R_TRY(out.write("reference-preprocessed.vtk", desc->reference, 1, ref.data()));
     SKM_R_TRY(out.write("target-preprocessed.vtk", desc->target, 1, tar.data()));
@@ -386,13 +413,13 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
 
     // device copies of the other inputs, made once
     if (desc->weight_image)
-        SKM_R_TRY(d_weight.upload(desc->weight_image, (size_t)n_w));
+        SKM_R_TRY(d_weight.upload(desc->weight_image, (size_t)n_w, ss));
     else if (desc->weight_tensor)
-        SKM_R_TRY(d_weight.upload(desc->weight_tensor, (size_t)n_w * dim * dim));
-    if (desc->guidance_image) SKM_R_TRY(d_guide.upload(desc->guidance_image, (size_t)n_w));
+        SKM_R_TRY(d_weight.upload(desc->weight_tensor, (size_t)n_w * dim * dim, ss));
+    if (desc->guidance_image) SKM_R_TRY(d_guide.upload(desc->guidance_image, (size_t)n_w, ss));
     bool has_prev = !prev.empty();
     if (has_prev)
-        SKM_R_TRY(d_prev.upload(prev.data(), prev.size()));
+        SKM_R_TRY(d_prev.upload(prev.data(), prev.size(), ss));
     else
         SKM_R_TRY(d_prev.alloc((size_t)n_tar * dim));
 
@@ -408,18 +435,18 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         if (P.smoothing_variance > 1e-5) {
             SKM_R_TRY(d_ref_s.alloc(d_ref.n));
             SKM_R_TRY(d_tar_s.alloc(d_tar.n));
-            SKM_R_TRY(skm::gaussian_smooth_device(dim, desc->reference, d_ref.p, P.smoothing_variance, d_ref_s.p));
-            SKM_R_TRY(skm::gaussian_smooth_device(dim, desc->target, d_tar.p, P.smoothing_variance, d_tar_s.p));
+            SKM_R_TRY(skm::gaussian_smooth_device(dim, desc->reference, d_ref.p, P.smoothing_variance, d_ref_s.p, ss));
+            SKM_R_TRY(skm::gaussian_smooth_device(dim, desc->target, d_tar.p, P.smoothing_variance, d_tar_s.p, ss));
             ref_s = d_ref_s.p;
             tar_s = d_tar_s.p;
         }
         if (out.on() && desc->verbosity >= 2) {
             const DeviceArray<float> &rs = ref_s == d_ref.p ? d_ref : d_ref_s;
             const DeviceArray<float> &ts = tar_s == d_tar.p ? d_tar : d_tar_s;
-            SKM_R_TRY(rs.download(host_tmp));
+            SKM_R_TRY(rs.download(host_tmp, ss));
             SKM_R_TRY(out.write("smoothed_reference-" + std::to_string(scale) + ".vtk", desc->reference, 1,
                                 host_tmp.data()));
-            SKM_R_TRY(ts.download(host_tmp));
+            SKM_R_TRY(ts.download(host_tmp, ss));
             SKM_R_TRY(out.write("smoothed_target-" + std::to_string(scale) + ".vtk", desc->target, 1,
                                 host_tmp.data()));
         }
@@ -474,7 +501,7 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
             skm_tensor_options to{P.sigma_c4, P.tensor_weight, 2, 0};
             double tilt = 0, mw = 0, mcov[9];
             SKM_R_TRY(skm::anisotropic_tensor_device(dim, g, guided ? d_guide.p : tar_s, to, d_tensor.p, &tilt, &mw,
-                                                     mcov));
+                                                     mcov, ss));
             ed.kernel_mode = SKM_KERNEL_WEIGHT_TENSOR;
             ed.weight = g;
             ed.weight_data = d_tensor.p;
@@ -482,7 +509,7 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
             std::memcpy(ed.maximum_eigenvalue_covariance, mcov, sizeof(mcov));
             rep.maximum_tilt = tilt;
             if (out.on()) {
-                SKM_R_TRY(d_tensor.download(host_tmp));
+                SKM_R_TRY(d_tensor.download(host_tmp, ss));
                 host_tensor = host_tmp.data();
             }
         }
@@ -502,7 +529,10 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         }
 
         EvaluatorGuard guard;
-        SKM_R_TRY(skm::create_evaluator(&ed, true, &guard.ev));
+        // the evaluator works on the registration's stream too: streams beyond
+        // the device's hardware queues share them in order, so each extra
+        // stream of a registration can queue behind another's optimizer loop
+        SKM_R_TRY(skm::create_evaluator(&ed, true, &guard.ev, ss));
         skm_evaluator *ev = guard.ev;
         if (P.reg_rkhs > 0) SKM_R_TRY(skm_set_regularizer_rkhs(ev, P.reg_rkhs));
         if (P.reg_rd > 0) SKM_R_TRY(skm_set_regularizer_rd(ev, P.reg_rd, P.reg_rd_scaling));
@@ -539,12 +569,12 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         std::vector<float> cp_aos((size_t)n_cp * dim);
         for (long long j = 0; j < n_cp; ++j)
             for (int d = 0; d < dim; ++d) cp_aos[j * dim + d] = params[d * n_cp + j];
-        SKM_R_TRY(d_cp.upload(cp_aos.data(), cp_aos.size()));
-        SKM_R_TRY(d_cpw.upload(cpw.data(), cpw.size()));
+        SKM_R_TRY(d_cp.upload(cp_aos.data(), cp_aos.size(), ss));
+        SKM_R_TRY(d_cpw.upload(cpw.data(), cpw.size(), ss));
         SKM_R_TRY(skm_evaluate_displacement_field_device(ev, d_cp.p, d_cpw.p, d_prev.p));
         SKM_R_TRY(skm_synchronize(ev));
         has_prev = true;
-        if (out.on() || landmarks) SKM_R_TRY(d_prev.download(prev));
+        if (out.on() || landmarks) SKM_R_TRY(d_prev.download(prev, ss));
         rep.ms_displacement = ms_since(t_df);
 
         // results of the level (SKMReg.cxx:459-505)
@@ -578,7 +608,7 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
     if (!has_prev)
         prev.assign((size_t)n_tar * dim, 0.0f);
     else if (prev.size() != (size_t)n_tar * dim || !(out.on() || landmarks))
-        SKM_R_TRY(d_prev.download(prev));
+        SKM_R_TRY(d_prev.download(prev, ss));
     std::memcpy(displacement_out, prev.data(), sizeof(float) * prev.size());
 
     // final results (SKMReg.cxx:507-525): the dense field and the reference

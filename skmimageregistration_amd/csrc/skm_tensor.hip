@@ -397,6 +397,13 @@ int launch_blocks(long long work)
     return (int)std::max<long long>(1, std::min<long long>(b, 1LL << 20));
 }
 
+// Waits for a stream when it goes out of scope: declared after a function's
+// temporaries, it runs before they are released, on every return path.
+struct StreamWait {
+    hipStream_t s;
+    ~StreamWait() { (void)hipStreamSynchronize(s); }
+};
+
 // Discrete Gaussian coefficients of every axis (variance / spacing^2) on the device.
 struct AxisKernels {
     DevBuf<double> coeffs;
@@ -404,7 +411,7 @@ struct AxisKernels {
     int radius[3] = {0, 0, 0};
 };
 
-skm_status make_axis_kernels(int dim, const skm_grid &g, double variance, AxisKernels &k)
+skm_status make_axis_kernels(int dim, const skm_grid &g, double variance, AxisKernels &k, hipStream_t stream)
 {
     std::vector<double> all;
     for (int a = 0; a < dim; ++a) {
@@ -420,7 +427,8 @@ skm_status make_axis_kernels(int dim, const skm_grid &g, double variance, AxisKe
         all.insert(all.end(), c.begin(), c.end());
     }
     SKM_T_HIP(k.coeffs.alloc(all.size()));
-    SKM_T_HIP(hipMemcpy(k.coeffs.p, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice));
+    SKM_T_HIP(hipMemcpyAsync(k.coeffs.p, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    SKM_T_HIP(hipStreamSynchronize(stream));
     return SKM_SUCCESS;
 }
 
@@ -498,29 +506,30 @@ SKM_API skm_status skm_anisotropic_tensor(int32_t dim, const skm_grid *grid, con
 
 // GaussianSmoothing on device arrays; d_in and d_out may be the same array.
 skm_status skm::gaussian_smooth_device(int dim, const skm_grid &grid, const float *d_in, double variance,
-                                       float *d_out)
+                                       float *d_out, hipStream_t stream)
 {
     skm_status st = validate_image(dim, &grid, d_in, d_out);
     if (st != SKM_SUCCESS) return st;
     if (!(variance >= 0.0)) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "variance must not be negative");
     const Dims d = make_dims(dim, grid);
     AxisKernels k;
-    if ((st = make_axis_kernels(dim, grid, variance, k)) != SKM_SUCCESS) return st;
     DevBuf<float> a, b;
+    StreamWait wait{stream};
+    if ((st = make_axis_kernels(dim, grid, variance, k, stream)) != SKM_SUCCESS) return st;
     SKM_T_HIP(a.alloc(d.n_vox));
     SKM_T_HIP(b.alloc(d.n_vox));
-    SKM_T_HIP(hipMemcpy(a.p, d_in, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice));
+    SKM_T_HIP(hipMemcpyAsync(a.p, d_in, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice, stream));
     float *res = nullptr;
-    if ((st = gaussian_passes(dim, d, 1, k, 1.0f, a.p, b.p, &res, nullptr)) != SKM_SUCCESS) return st;
-    SKM_T_HIP(hipMemcpy(d_out, res, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice));
-    SKM_T_HIP(hipStreamSynchronize(nullptr));  // complete for callers on other streams
+    if ((st = gaussian_passes(dim, d, 1, k, 1.0f, a.p, b.p, &res, stream)) != SKM_SUCCESS) return st;
+    SKM_T_HIP(hipMemcpyAsync(d_out, res, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    SKM_T_HIP(hipStreamSynchronize(stream));  // complete for callers on other streams
     return SKM_SUCCESS;
 }
 
 skm_status skm::anisotropic_tensor_device(int dim, const skm_grid &grid_ref, const float *d_image,
                                           const skm_tensor_options &opt_ref, float *d_tensor_out,
                                           double *maximum_tilt, double *maximum_weight,
-                                          double *maximum_eigenvalue_covariance)
+                                          double *maximum_eigenvalue_covariance, hipStream_t stream)
 {
     const skm_grid *grid = &grid_ref;
     const skm_tensor_options *opt = &opt_ref;
@@ -534,23 +543,23 @@ skm_status skm::anisotropic_tensor_device(int dim, const skm_grid &grid_ref, con
     const Dims d = make_dims(dim, *grid);
     const int nc = dim * (dim + 1) / 2;
     AxisKernels k_img, k_ten;
-    if ((st = make_axis_kernels(dim, *grid, opt->sigma, k_img)) != SKM_SUCCESS) return st;
-    if ((st = make_axis_kernels(dim, *grid, opt->sigma / 4.0, k_ten)) != SKM_SUCCESS) return st;
-
     DevBuf<float> a, b, ca, cb;
+    DevBuf<float> bmax;
+    DevBuf<unsigned> bcnt, bbad;
+    StreamWait wait{stream};
+    if ((st = make_axis_kernels(dim, *grid, opt->sigma, k_img, stream)) != SKM_SUCCESS) return st;
+    if ((st = make_axis_kernels(dim, *grid, opt->sigma / 4.0, k_ten, stream)) != SKM_SUCCESS) return st;
+
     SKM_T_HIP(a.alloc(d.n_vox));
     SKM_T_HIP(b.alloc(d.n_vox));
     SKM_T_HIP(ca.alloc(d.n_vox * nc));
     SKM_T_HIP(cb.alloc(d.n_vox * nc));
     const int fin_blocks = std::min(launch_blocks(d.n_vox), 4096);
-    DevBuf<float> bmax;
-    DevBuf<unsigned> bcnt, bbad;
     SKM_T_HIP(bmax.alloc(fin_blocks));
     SKM_T_HIP(bcnt.alloc(fin_blocks));
     SKM_T_HIP(bbad.alloc(fin_blocks));
-    SKM_T_HIP(hipMemcpy(a.p, d_image, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice));
+    SKM_T_HIP(hipMemcpyAsync(a.p, d_image, d.n_vox * sizeof(float), hipMemcpyDeviceToDevice, stream));
 
-    hipStream_t stream = nullptr;
     float *smoothed = nullptr, *comps = nullptr;
     if ((st = gaussian_passes(dim, d, 1, k_img, (float)opt->weight, a.p, b.p, &smoothed, stream)) != SKM_SUCCESS)
         return st;
@@ -574,9 +583,10 @@ skm_status skm::anisotropic_tensor_device(int dim, const skm_grid &grid_ref, con
 
     std::vector<float> hmax(fin_blocks);
     std::vector<unsigned> hcnt(fin_blocks), hbad(fin_blocks);
-    SKM_T_HIP(hipMemcpy(hmax.data(), bmax.p, fin_blocks * sizeof(float), hipMemcpyDeviceToHost));
-    SKM_T_HIP(hipMemcpy(hcnt.data(), bcnt.p, fin_blocks * sizeof(unsigned), hipMemcpyDeviceToHost));
-    SKM_T_HIP(hipMemcpy(hbad.data(), bbad.p, fin_blocks * sizeof(unsigned), hipMemcpyDeviceToHost));
+    SKM_T_HIP(hipMemcpyAsync(hmax.data(), bmax.p, fin_blocks * sizeof(float), hipMemcpyDeviceToHost, stream));
+    SKM_T_HIP(hipMemcpyAsync(hcnt.data(), bcnt.p, fin_blocks * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    SKM_T_HIP(hipMemcpyAsync(hbad.data(), bbad.p, fin_blocks * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    SKM_T_HIP(hipStreamSynchronize(stream));
     unsigned long long count = 0, bad = 0;
     float tilt = -1.0f;
     for (int i = 0; i < fin_blocks; ++i) {
