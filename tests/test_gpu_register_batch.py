@@ -67,3 +67,24 @@ def test_batch_reports_the_failing_registration():
     assert list(st) == [0, _capi.SKM_ERROR_INVALID_ARGUMENT]
     assert lib.skm_last_error().decode().startswith("registration 1: ")
     assert np.isfinite(outs[0]).all()
+
+
+def test_full_series_identical_with_four_in_flight():
+    """The 32-frame series (96^3 / 128x128x64 / 160x160x80) with the 3-level
+    NCC + RD registration: four registrations in flight give the results of
+    one at a time, bit for bit (set-up work queued on the null stream once
+    landed late inside another frame's optimizer loop)."""
+    s = 24.0
+    cfg = RegistrationConfig(sigma_C4=[4 * s, 2 * s, s], sigma_C0=[4 * s, 2 * s, s],
+                             tensor_weights=[400.0, 200.0, 100.0], num_scales=3, num_function_evaluations=[200],
+                             initial_step_size=[1000.0], sampling_rate=[5, 4, 3], neighborhood_sampling=[1],
+                             smoothing_variance=[2, 1, 0], reg_rd=[1e-5], metric="ncc")
+    frames, _ = synthetic.time_series_3d(n_frames=32)
+    pairs = [(Image(frames[0]), Image(f)) for f in frames[1:]]
+    print("full series: one at a time", flush=True)
+    one = register_batch(pairs, cfg, concurrency=1)
+    print("full series: four in flight", flush=True)
+    four = register_batch(pairs, cfg, concurrency=4)
+    for i, (a, b) in enumerate(zip(one, four)):
+        np.testing.assert_array_equal(a[0], b[0], err_msg=f"frame {i + 1}")
+        assert [r["value"] for r in a[1]] == [r["value"] for r in b[1]], i + 1
