@@ -390,6 +390,10 @@ typedef struct skm_registration_desc {
     int64_t num_landmarks;                             /* landmark pairs for the TRE */
     const double *reference_landmarks;                 /* D doubles per point       */
     const double *target_landmarks;
+    int32_t exact_gradient;                            /* 1: every level's evaluator with
+                                                          SKM_FLAG_EXACT_GRADIENT (no SKMReg
+                                                          counterpart); 0: the reference's
+                                                          derivative windows        */
 } skm_registration_desc;
 
 /* What one level did. */

@@ -107,7 +107,8 @@ class SkmRegistrationDesc(ctypes.Structure):
                 ("guidance_image", c_float_p), ("restart_at", ctypes.c_int32),
                 ("restart_displacement", c_float_p), ("temp_directory", ctypes.c_char_p),
                 ("verbosity", ctypes.c_int32), ("num_landmarks", ctypes.c_int64),
-                ("reference_landmarks", c_double_p), ("target_landmarks", c_double_p)]
+                ("reference_landmarks", c_double_p), ("target_landmarks", c_double_p),
+                ("exact_gradient", ctypes.c_int32)]
 
 
 class SkmScaleReport(ctypes.Structure):

@@ -464,6 +464,7 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         ed.sigma_c4 = P.sigma_c4;
         ed.subsample = P.sampling_rate;
         ed.subsample_neighborhood = P.neighborhood_sampling;
+        ed.flags = desc->exact_gradient ? SKM_FLAG_EXACT_GRADIENT : 0u;
         const float *host_tensor = nullptr;  // the level's weight tensor in host memory, if any
         if (desc->weight_image) {
             double mx = std::numeric_limits<double>::lowest();

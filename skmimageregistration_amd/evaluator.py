@@ -423,7 +423,9 @@ def landmark_tre(displacement, fixed_points, moving_points):
 
 class RegistrationConfig:
     """The SKMReg parameter set (include/ConfigUtils.h:29-62): per-scale lists
-    whose last entry repeats (CURRENT, src/SKMReg.cxx:48-49)."""
+    whose last entry repeats (CURRENT, src/SKMReg.cxx:48-49).  One key has no
+    SKMReg counterpart: exact_gradient (1: derivative windows covering the
+    whole kernel support at every level; 0, the default: the reference's)."""
 
     STRING_KEYS = ("metric", "reference_filename", "target_filename", "reference_landmarks", "target_landmarks",
                    "weight_filename", "weight_tensor_filename", "temp_directory")
@@ -438,7 +440,7 @@ class RegistrationConfig:
         "reg_pg_scaling": [1, 1, 1], "num_scales": [3], "restart_at": [0],
         "num_function_evaluations": [500, 500, 500], "initial_step_size": [0.1, 0.1, 0.1],
         "sampling_rate": [5, 4, 3], "neighborhood_sampling": [15, 12, 9], "intensity_thresholds": [-1500, 3500],
-        "smoothing_variance": [2, 1, 0], "metric": "mse",
+        "smoothing_variance": [2, 1, 0], "metric": "mse", "exact_gradient": [0],
     }
 
     def __init__(self, **overrides):
@@ -499,6 +501,7 @@ def _registration_desc(lib, reference, target, config, weight_image, weight_tens
         p.neighborhood_sampling = int(cfg.CURRENT("neighborhood_sampling", s))
         p.smoothing_variance = float(cfg.CURRENT("smoothing_variance", s))
     d.cp_spacing_scaler = float(cfg.CURRENT("cp_spacing_scaler", 0))
+    d.exact_gradient = int(bool(cfg.CURRENT("exact_gradient", 0)))
     d.metric = {"mse": MEASURE.MSE, "ncc": MEASURE.NCC, "lcc": MEASURE.LCC}[cfg["metric"]]
     th = cfg["intensity_thresholds"]
     d.use_intensity_thresholds = int(len(th) == 2)

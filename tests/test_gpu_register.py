@@ -161,3 +161,19 @@ def test_preprocessed_images_equal_the_host_restatement(tmp_path, thresholds):
         want = rescale_image(want)
         got = read_image(str(tmp_path / f"{name}-preprocessed.vtk")).data
         np.testing.assert_array_equal(got, want)
+
+
+def test_exact_gradient_option_reaches_every_level():
+    """exact_gradient (no SKMReg counterpart) gives the level's evaluator the
+    exact-gradient windows: the LCC registration starts from the same value
+    (the windows change only the derivative), then moves differently from the
+    reference-window one, and keeps its best iterate."""
+    mov, fix, _ = synthetic.sliding_3d((24, 32, 32), shift=2.0, breathing=2.0)
+    base = dict(sigma_C4=[12.0], sigma_C0=[12.0], tensor_weights=[0.0], num_scales=1, num_function_evaluations=[5],
+                initial_step_size=[1000.0], sampling_rate=[2], neighborhood_sampling=[4], smoothing_variance=[0],
+                reg_rd=[1e-3], metric="lcc", intensity_thresholds=[])
+    df_ref, rep_ref = register_multiscale(Image(mov), Image(fix), RegistrationConfig(**base))
+    df_x, rep_x = register_multiscale(Image(mov), Image(fix), RegistrationConfig(exact_gradient=[1], **base))
+    assert np.isfinite(df_x).all() and rep_x[0]["first_value"] == rep_ref[0]["first_value"]
+    assert not np.array_equal(df_x, df_ref)
+    assert rep_x[0]["value"] <= rep_x[0]["first_value"]
