@@ -294,6 +294,7 @@ struct skm_evaluator {
     Scalars *d_sc = nullptr;
     float *d_deriv = nullptr, *d_cp_in = nullptr, *d_cpw_in = nullptr, *d_dst = nullptr;
     float4 *d_dense = nullptr, *d_dense_q = nullptr, *d_ccoef = nullptr, *d_gamma = nullptr;
+    double *d_lcc_ref_partials = nullptr;  // chunk partials of the reference-mode LCC derivative
     float *d_params = nullptr, *d_prev_g = nullptr, *d_best = nullptr;
     OptState *d_opt = nullptr;
     skm_iteration_record *d_records = nullptr;
@@ -321,7 +322,8 @@ struct skm_evaluator {
                         d_reg_window,
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
-                        d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng};
+                        d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng,
+                        d_lcc_ref_partials};
         for (void *p : ptrs)
             if (p) device_release(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -633,6 +635,8 @@ skm_status ensure_lcc_buffers(skm_evaluator *ev)
 {
     if (!ev->d_dense) SKM_TRY(dev_alloc(&ev->d_dense, ev->pr.n_vox));
     if (!ev->d_ccoef) SKM_TRY(dev_alloc(&ev->d_ccoef, ev->pr.n_sub));
+    const long long np = lcc_reference_partials(ev->pr);
+    if (np > 0 && !ev->d_lcc_ref_partials) SKM_TRY(dev_alloc(&ev->d_lcc_ref_partials, (size_t)np));
     return SKM_SUCCESS;
 }
 
@@ -702,7 +706,7 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
                                        &b2, s);
                     dense = ev->d_dense_q;
                 }
-                launch_lcc_gather_reference(lp, dense, ev->d_ccoef, ev->d_sc, deriv, s);
+                launch_lcc_gather_reference(lp, dense, ev->d_ccoef, ev->d_sc, deriv, ev->d_lcc_ref_partials, s);
             }
         }
     } else {

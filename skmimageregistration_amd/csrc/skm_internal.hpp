@@ -130,8 +130,10 @@ void launch_result(const Scalars *sc, void *result, hipStream_t s);
 void launch_dense_warp(const Problem &pr, int quirk3d, float4 *dense_g, hipStream_t s);
 void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre_coef, double *partials,
                         int *blocks_out, hipStream_t s);
+// partials: lcc_reference_partials(pr) doubles (none needed when it is 0)
 void launch_lcc_gather_reference(const Problem &pr, const float4 *dense_g, const float4 *centre_coef,
-                                 const Scalars *sc, float *deriv_soa, hipStream_t s);
+                                 const Scalars *sc, float *deriv_soa, double *partials, hipStream_t s);
+long long lcc_reference_partials(const Problem &pr);
 void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
                         float4 *gamma, hipStream_t s);
 

@@ -270,16 +270,22 @@ template <> struct LcTile<3> { static constexpr int X = 2, Y = 2, Z = 4; };
 // neighbour offsets in step, and each pair is added to every tile point whose
 // window holds the centre.  Sums: FP32 per lane, FP32 butterfly per wave,
 // FP64 across the waves in order.
+// With chunks > 1 the centres of a tile are split over `chunks` workgroups
+// (workgroup b: tile b / chunks, every chunks-th block of kThreads centres
+// from b % chunks), which write FP64 partials per (workgroup, point, axis) for
+// k_lcc_ref_finish: a grid of few control points has too few tiles to fill
+// the GPU (13x13x7 points: 98 tiles).
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_lcc_gather_ref(Problem pr, const float4 *__restrict__ dense,
                                                              const float4 *__restrict__ coef,
                                                              const Scalars *__restrict__ sc,
-                                                             float *__restrict__ deriv)
+                                                             float *__restrict__ deriv, int chunks,
+                                                             double *__restrict__ partials)
 {
     constexpr int TX = LcTile<DIM>::X, TY = LcTile<DIM>::Y, TZ = LcTile<DIM>::Z, T = TX * TY * TZ;
     __shared__ double lds[(kThreads / 64) * T * 3];
     const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
-    const int tile = (int)blockIdx.x;
+    const int tile = (int)blockIdx.x / chunks, chunk = (int)blockIdx.x % chunks;
     const int t0 = tile % ntx, t1 = (tile / ntx) % nty, t2 = tile / (ntx * nty);
     float p[T][3];
     int lo[T][3], hi[T][3];
@@ -314,7 +320,7 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gather_ref(Problem pr, const f
     // one centre per thread (consecutive threads: consecutive centres along
     // x, so a step's neighbour loads of a wave fall on few cache lines); the
     // neighbour offsets are walked in step by the whole wave
-    for (int cb = 0; cb < ncent; cb += kThreads) {
+    for (int cb = chunk * kThreads; cb < ncent; cb += chunks * kThreads) {
         const int ci = cb + (int)threadIdx.x;
         unsigned mask = 0;
         float4 abc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -404,12 +410,37 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gather_ref(Problem pr, const f
         const int c = threadIdx.x / 3, d = threadIdx.x % 3;
         double sum = 0.0;
         for (int w = 0; w < kThreads / 64; ++w) sum += lds[w * T * 3 + threadIdx.x];
-        if (ok[c] && d < DIM) {
+        if (chunks > 1) {
+            partials[(size_t)blockIdx.x * (T * 3) + threadIdx.x] = sum;
+        } else if (ok[c] && d < DIM) {
             const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
             const long long j = jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]);
             deriv[j + d * pr.n_cp] = (float)(sum * sc->coef_scale);
         }
     }
+}
+
+// Sum of a control point's chunk partials in chunk order (k_lcc_gather_ref
+// with chunks > 1); one thread per (point, axis).
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_lcc_ref_finish(Problem pr, int chunks,
+                                                             const double *__restrict__ partials,
+                                                             const Scalars *__restrict__ sc,
+                                                             float *__restrict__ deriv)
+{
+    constexpr int TX = LcTile<DIM>::X, TY = LcTile<DIM>::Y, TZ = LcTile<DIM>::Z, T = TX * TY * TZ;
+    const long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i >= pr.n_cp * DIM) return;
+    const long long j = i % pr.n_cp;
+    const int d = (int)(i / pr.n_cp);
+    const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
+    const int j0 = (int)(j % pr.cp.n[0]), j1 = (int)((j / pr.cp.n[0]) % pr.cp.n[1]);
+    const int j2 = DIM == 3 ? (int)(j / ((long long)pr.cp.n[0] * pr.cp.n[1])) : 0;
+    const long long tile = j0 / TX + (long long)ntx * (j1 / TY + (long long)nty * (j2 / TZ));
+    const int c = j0 % TX + TX * (j1 % TY + TY * (j2 % TZ));
+    double sum = 0.0;
+    for (int k = 0; k < chunks; ++k) sum += partials[((size_t)(tile * chunks + k) * T + c) * 3 + d];
+    deriv[i] = (float)(sum * sc->coef_scale);
 }
 
 // Exact mode: every valid centre scatters beta kw gradM onto its neighbours,
@@ -532,20 +563,48 @@ void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre
     SKM_LCC_DISPATCH(pr, centres_impl, pr, dense_g, centre_coef, partials, blocks_out, s);
 }
 
-template <int DIM, int MODE>
-static void gather_ref_impl(const Problem &pr, const float4 *dense, const float4 *coef, const Scalars *sc,
-                            float *deriv, hipStream_t s)
+template <int DIM>
+static long long ref_tiles(const Problem &pr)
 {
     constexpr int TX = LcTile<DIM>::X, TY = LcTile<DIM>::Y, TZ = LcTile<DIM>::Z;
-    const long long tiles = (long long)((pr.cp.n[0] + TX - 1) / TX) * ((pr.cp.n[1] + TY - 1) / TY) *
-                            (DIM == 3 ? (pr.cp.n[2] + TZ - 1) / TZ : 1);
-    k_lcc_gather_ref<DIM, MODE><<<(unsigned)tiles, kThreads, 0, s>>>(pr, dense, coef, sc, deriv);
+    return (long long)((pr.cp.n[0] + TX - 1) / TX) * ((pr.cp.n[1] + TY - 1) / TY) *
+           (DIM == 3 ? (pr.cp.n[2] + TZ - 1) / TZ : 1);
+}
+
+// Workgroups per tile of the reference-mode derivative: enough for ~2048
+// workgroups in all, at most 64.
+static int ref_chunks(long long tiles)
+{
+    constexpr long long kWanted = 8192;
+    return (int)std::min<long long>(64, std::max<long long>(1, (kWanted + tiles - 1) / tiles));
+}
+
+long long lcc_reference_partials(const Problem &pr)
+{
+    const long long tiles = pr.dim == 3 ? ref_tiles<3>(pr) : ref_tiles<2>(pr);
+    const int chunks = ref_chunks(tiles);
+    const int t = pr.dim == 3 ? LcTile<3>::X * LcTile<3>::Y * LcTile<3>::Z : LcTile<2>::X * LcTile<2>::Y;
+    return chunks > 1 ? tiles * chunks * t * 3 : 0;
+}
+
+template <int DIM, int MODE>
+static void gather_ref_impl(const Problem &pr, const float4 *dense, const float4 *coef, const Scalars *sc,
+                            float *deriv, double *partials, hipStream_t s)
+{
+    const long long tiles = ref_tiles<DIM>(pr);
+    const int chunks = partials ? ref_chunks(tiles) : 1;
+    k_lcc_gather_ref<DIM, MODE><<<(unsigned)(tiles * chunks), kThreads, 0, s>>>(pr, dense, coef, sc, deriv, chunks,
+                                                                                partials);
+    if (chunks > 1) {
+        const unsigned g = (unsigned)((pr.n_cp * DIM + kThreads - 1) / kThreads);
+        k_lcc_ref_finish<DIM><<<g, kThreads, 0, s>>>(pr, chunks, partials, sc, deriv);
+    }
 }
 
 void launch_lcc_gather_reference(const Problem &pr, const float4 *dense_g, const float4 *centre_coef,
-                                 const Scalars *sc, float *deriv_soa, hipStream_t s)
+                                 const Scalars *sc, float *deriv_soa, double *partials, hipStream_t s)
 {
-    SKM_LCC_DISPATCH(pr, gather_ref_impl, pr, dense_g, centre_coef, sc, deriv_soa, s);
+    SKM_LCC_DISPATCH(pr, gather_ref_impl, pr, dense_g, centre_coef, sc, deriv_soa, partials, s);
 }
 
 template <int DIM, int MODE>
