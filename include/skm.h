@@ -165,6 +165,27 @@ SKM_API skm_status skm_enable_timing(skm_evaluator *ev, int32_t enable);
 /* Synchronises the stream, returns the totals since the last read and resets them. */
 SKM_API skm_status skm_read_timing(skm_evaluator *ev, skm_pass_times *out);
 
+/* Launch shapes of an evaluator's kernels (no counterpart in the reference,
+ * whose launch shapes are fixed, GpuEvaluator.cu:157-171, 424-438, 578-586).
+ * 0 in a field selects the library's own choice; results do not depend on
+ * the choice beyond FP32 rounding of the per-workgroup partial sums (the
+ * sums across workgroups are FP64, in a fixed order).
+ *   lcc_ref_chunks  reference-window LCC derivative: workgroups sharing one
+ *                   tile of control points (1..64; auto: ~8192 workgroups)
+ *   gather_split    derivative gather: sample planes per workgroup slab
+ *                   (4 or 8; auto 8)
+ *   value_split     value pass: waves sharing one brick of samples
+ *                   (1, 4, 8 or 16; auto from the sample count)
+ * skm_get_tuning returns the fields as set (0 where automatic). */
+typedef struct skm_tuning {
+    int32_t lcc_ref_chunks;
+    int32_t gather_split;
+    int32_t value_split;
+    int32_t reserved[5];
+} skm_tuning;
+SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning);
+SKM_API skm_status skm_get_tuning(skm_evaluator *ev, skm_tuning *tuning);
+
 /* Jitter offsets of the last evaluation, D ints per subsampled cell
  * (GpuEvaluator.cu:107-118); n_cells receives prod(fixed.size / subsample). */
 SKM_API skm_status skm_get_cube_offsets(skm_evaluator *ev, int32_t *out, int64_t *n_cells);

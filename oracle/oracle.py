@@ -55,6 +55,11 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def load_path(path):
+    """An oracle build at `path` (a copy of skm_oracle.c compiled elsewhere)."""
+    return _declare(ctypes.CDLL(path))
+
+
 def load(parallel=False):
     """The serial oracle; parallel=True the OpenMP build of the same source."""
     global _lib, _lib_omp
@@ -168,8 +173,10 @@ class OracleProblem:
 
     def __init__(self, fixed, moving, cp_grid, cp_aos, cpw=None, sigma_c0=1.0, sigma_c4=1.0, kernel_mode=0,
                  max_weight=1.0, max_eig_cov=None, weight=None, df=None, subsample=1, subsample_neighborhood=1,
-                 offsets=None, exact=False, fixed_geom=None, moving_geom=None, parallel=False):
-        lib = load(parallel)
+                 offsets=None, exact=False, fixed_geom=None, moving_geom=None, parallel=False, lib=None):
+        # `lib`: another build of the oracle source (the mutation tests load
+        # deliberately broken copies to show that the pins catch them)
+        lib = load(parallel) if lib is None else lib
         self.lib = lib
         fixed = np.asarray(fixed)
         dim = fixed.ndim

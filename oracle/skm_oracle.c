@@ -464,7 +464,7 @@ void skmo_displacement(const skmo_problem *pr, double *out_aos)
     int n = pr->dim;
     long long nv = grid_count(&pr->fixed, n);
     for (long long v = 0; v < nv; ++v) {
-        int i[3];
+        int i[3] = {0, 0, 0};
         long long r = v;
         for (int d = 0; d < n; ++d) { i[d] = (int)(r % pr->fixed.size[d]); r /= pr->fixed.size[d]; }
         double fi[3] = {i[0], i[1], n == 3 ? i[2] : 0}, x[3];
@@ -497,7 +497,7 @@ static double fixed_at(const skmo_problem *pr, const int *pos)
     /* A shift equal to `subsample` (nextF()==1) can leave the image; the
      * reference reads out of bounds there.  The oracle clamps, as the
      * product library does. */
-    int c[3];
+    int c[3] = {0, 0, 0};
     for (int d = 0; d < pr->dim; ++d) c[d] = imin(imax(pos[d], 0), pr->fixed.size[d] - 1);
     return pr->fixed_data[to_index(&pr->fixed, pr->dim, c)];
 }
@@ -643,8 +643,67 @@ typedef struct lcc_sums {
     int count;
 } lcc_sums;
 
+/* The warped neighbour y + u(y) and its B-spline sample depend only on the
+ * fixed voxel y (and, in the 3D derivative, on the sigma(y)-twice quirk), so
+ * they are computed once per voxel and looked up; the reference recomputes
+ * them for every (centre, neighbour) pair and gets the same numbers. */
+typedef struct lcc_cache {
+    int *in[2];
+    double *val[2];
+    double *grad[2];
+} lcc_cache;
+
+static int lcc_cache_build(const skmo_problem *pr, int with_quirk, lcc_cache *c)
+{
+    int n = pr->dim;
+    long long nv = grid_count(&pr->fixed, n);
+    memset(c, 0, sizeof(*c));
+    for (int q = 0; q <= with_quirk; ++q) {
+        c->in[q] = (int *)calloc((size_t)nv, sizeof(int));
+        c->val[q] = (double *)calloc((size_t)nv, sizeof(double));
+        c->grad[q] = (double *)calloc((size_t)nv * 3, sizeof(double));
+        if (!c->in[q] || !c->val[q] || !c->grad[q]) return -1;
+    }
+    if (!with_quirk) { c->in[1] = c->in[0]; c->val[1] = c->val[0]; c->grad[1] = c->grad[0]; }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long v = 0; v < nv; ++v) {
+        int f[3] = {0, 0, 0};
+        long long r = v;
+        for (int d = 0; d < n; ++d) { f[d] = (int)(r % pr->fixed.size[d]); r /= pr->fixed.size[d]; }
+        double ff[3] = {f[0], f[1], f[2]}, y[3];
+        to_global(&pr->fixed, n, ff, y);
+        point_state sty;
+        point_state_at(pr, y, &sty);
+        for (int q = 0; q <= with_quirk; ++q) {
+            double my[3], mv = 0.0, grad[3] = {0, 0, 0};
+            warp_neighbor(pr, y, &sty, q, my);
+            c->in[q][v] = skmo_bspline_sample(n, &pr->moving, pr->moving_coeffs, my, &mv, grad);
+            c->val[q][v] = mv;
+            for (int d = 0; d < 3; ++d) c->grad[q][v * 3 + d] = grad[d];
+        }
+    }
+    return 0;
+}
+
+static void lcc_cache_free(lcc_cache *c)
+{
+    int distinct = c->in[1] != c->in[0];
+    for (int q = 0; q < 1 + distinct; ++q) { free(c->in[q]); free(c->val[q]); free(c->grad[q]); }
+    memset(c, 0, sizeof(*c));
+}
+
+/* Sample of the warped neighbour at fixed index f (quirk: 3D derivative form). */
+static int lcc_sample(const skmo_problem *pr, const lcc_cache *c, int quirk, const int *f, double *mv, double *grad)
+{
+    long long v = to_index(&pr->fixed, pr->dim, f);
+    *mv = c->val[quirk][v];
+    for (int d = 0; d < 3; ++d) grad[d] = c->grad[quirk][v * 3 + d];
+    return c->in[quirk][v];
+}
+
 /* LCC value at one centre (GpuEvaluatorLCC.cu:94-191). */
-static double lcc_centre_value(const skmo_problem *pr, const double *x, const point_state *stx)
+static double lcc_centre_value(const skmo_problem *pr, const lcc_cache *cache, const double *x,
+                               const point_state *stx)
 {
     int n = pr->dim, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, f[3] = {0, 0, 0};
     neighbor_box(pr, x, stx->support, lo, hi);
@@ -654,13 +713,12 @@ static double lcc_centre_value(const skmo_problem *pr, const double *x, const po
     for (f[2] = z0; f[2] <= z1; f[2] += sn)
         for (f[1] = lo[1]; f[1] <= hi[1]; f[1] += sn)
             for (f[0] = lo[0]; f[0] <= hi[0]; f[0] += sn) {
-                double fi[3] = {f[0], f[1], f[2]}, y[3], my[3], mv, grad[3];
+                double fi[3] = {f[0], f[1], f[2]}, y[3], mv, grad[3];
                 to_global(&pr->fixed, n, fi, y);
                 point_state sty;
                 point_state_at(pr, y, &sty);
-                warp_neighbor(pr, y, &sty, 0, my);
                 double kw = neighborhood_weight(n, x, y, sty.support, 10.0);
-                if (kw > 0.0 && skmo_bspline_sample(n, &pr->moving, pr->moving_coeffs, my, &mv, grad)) {
+                if (kw > 0.0 && lcc_sample(pr, cache, 0, f, &mv, grad)) {
                     mv *= kw;
                     double fv = kw * pr->fixed_data[to_index(&pr->fixed, n, f)];
                     s.sf += fv; s.sm += mv; s.sff += fv * fv; s.smm += mv * mv; s.sfm += fv * mv;
@@ -682,7 +740,7 @@ static double lcc_centre_value(const skmo_problem *pr, const double *x, const po
 
 /* Reference-mode LCC derivative of one cp (GpuEvaluatorLCC.cu:206-530; 2D 696-961):
  * centres restricted to the tight cp window, w_j not applied, |u|>1e-16 guard. */
-static void lcc_derivative_reference(const skmo_problem *pr, const int *j, double *der)
+static void lcc_derivative_reference(const skmo_problem *pr, const lcc_cache *cache, const int *j, double *der)
 {
     int n = pr->dim, ns[3];
     subsampled_size(pr, ns);
@@ -725,14 +783,13 @@ static void lcc_derivative_reference(const skmo_problem *pr, const int *j, doubl
                 for (f[2] = fz0; f[2] <= fz1; f[2] += sn)
                     for (f[1] = flo[1]; f[1] <= fhi[1]; f[1] += sn)
                         for (f[0] = flo[0]; f[0] <= fhi[0]; f[0] += sn) {
-                            double ff[3] = {f[0], f[1], f[2]}, y[3], my[3], mv, grad[3];
+                            double ff[3] = {f[0], f[1], f[2]}, y[3], mv, grad[3];
                             to_global(&pr->fixed, n, ff, y);
                             point_state sty;
                             point_state_at(pr, y, &sty);
-                            warp_neighbor(pr, y, &sty, n == 3, my);
                             double kw = neighborhood_weight(n, xi, y, sty.support, 10.0);
                             double ka = kernel_ka(pr, pj, y, &sty);
-                            if (kw > 0.0 && skmo_bspline_sample(n, &pr->moving, pr->moving_coeffs, my, &mv, grad)) {
+                            if (kw > 0.0 && lcc_sample(pr, cache, n == 3, f, &mv, grad)) {
                                 mv *= kw;
                                 double fv = kw * pr->fixed_data[to_index(&pr->fixed, n, f)];
                                 sf += fv; sm += mv; sff += fv * fv; smm += mv * mv; sfm += fv * mv;
@@ -780,6 +837,12 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
     double *xs = (double *)calloc((size_t)nsub * n, sizeof(double));
     int *inside = (int *)calloc((size_t)nsub, sizeof(int));
     if (!a || !b || !g || !xs || !inside) return -1;
+    lcc_cache cache;
+    memset(&cache, 0, sizeof(cache));
+    if (measure == SKMO_LCC &&
+        lcc_cache_build(pr, with_derivative && !pr->exact && n == 3 && pr->kernel_mode != SKMO_KERNEL_STATIONARY,
+                        &cache) != 0)
+        return -1;
 
     /* value pass: one iteration per subsampled sample (e.g. GpuEvaluatorMSE.cu:29-95);
        iterations are independent (the OpenMP build runs them in parallel) */
@@ -799,7 +862,7 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
         if (measure == SKMO_LCC) {
             point_state st;
             point_state_at(pr, x, &st);
-            a[s] = lcc_centre_value(pr, x, &st);
+            a[s] = lcc_centre_value(pr, &cache, x, &st);
             continue;
         }
         if (in) {
@@ -848,7 +911,7 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
             double der[3] = {0, 0, 0};
             if (measure == SKMO_LCC) {
                 if (pr->exact) continue;  /* exact LCC gradient handled below */
-                lcc_derivative_reference(pr, j, der);
+                lcc_derivative_reference(pr, &cache, j, der);
                 for (int d = 0; d < n; ++d) deriv_soa[jj + d * ncp] = der[d] / (double)count;
                 continue;
             }
@@ -932,13 +995,12 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
                     for (f[2] = z0; f[2] <= z1; f[2] += sn)
                         for (f[1] = lo[1]; f[1] <= hi[1]; f[1] += sn)
                             for (f[0] = lo[0]; f[0] <= hi[0]; f[0] += sn) {
-                                double ff[3] = {f[0], f[1], f[2]}, y[3], my[3], mv, grad[3];
+                                double ff[3] = {f[0], f[1], f[2]}, y[3], mv, grad[3];
                                 to_global(&pr->fixed, n, ff, y);
                                 point_state sty;
                                 point_state_at(pr, y, &sty);
-                                warp_neighbor(pr, y, &sty, 0, my);
                                 double kw = neighborhood_weight(n, x, y, sty.support, 10.0);
-                                if (!(kw > 0.0 && skmo_bspline_sample(n, &pr->moving, pr->moving_coeffs, my, &mv, grad)))
+                                if (!(kw > 0.0 && lcc_sample(pr, &cache, 0, f, &mv, grad)))
                                     continue;
                                 double fv = kw * pr->fixed_data[to_index(&pr->fixed, n, f)];
                                 double mvk = kw * mv;
@@ -997,6 +1059,7 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
     }
     free(vals);
     free(a); free(b); free(g); free(xs); free(inside);
+    if (measure == SKMO_LCC) lcc_cache_free(&cache);
     return 0;
 }
 

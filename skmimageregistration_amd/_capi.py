@@ -73,6 +73,11 @@ class SkmPassTimes(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * SKM_NUM_PASSES), ("launches", ctypes.c_uint32 * SKM_NUM_PASSES)]
 
 
+class SkmTuning(ctypes.Structure):
+    _fields_ = [("lcc_ref_chunks", ctypes.c_int32), ("gather_split", ctypes.c_int32),
+                ("value_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+
 class SkmIterationRecord(ctypes.Structure):
     _fields_ = [("value", ctypes.c_double), ("measure", ctypes.c_double), ("learning_rate", ctypes.c_double),
                 ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
@@ -147,6 +152,8 @@ _SIGNATURES = {
     "skm_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
     "skm_enable_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "skm_read_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmPassTimes)]),
+    "skm_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmTuning)]),
+    "skm_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmTuning)]),
     "skm_get_cube_offsets": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, ctypes.POINTER(ctypes.c_int64)]),
     "skm_control_point_grid": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_double_p,
                                               ctypes.POINTER(SkmGrid)]),

@@ -295,6 +295,10 @@ struct skm_evaluator {
     float *d_deriv = nullptr, *d_cp_in = nullptr, *d_cpw_in = nullptr, *d_dst = nullptr;
     float4 *d_dense = nullptr, *d_dense_q = nullptr, *d_ccoef = nullptr, *d_gamma = nullptr;
     double *d_lcc_ref_partials = nullptr;  // chunk partials of the reference-mode LCC derivative
+    long long lcc_ref_partials_n = 0;       // doubles held by d_lcc_ref_partials
+    skm_tuning tuning = {};                 // launch shapes as set by skm_set_tuning
+    double *d_lcc_bound = nullptr;          // exact LCC: per-workgroup bounds of the centre pass
+    unsigned long long *d_gamma_fx = nullptr;  // exact LCC: fixed-point scatter sums, 3 per voxel
     float *d_params = nullptr, *d_prev_g = nullptr, *d_best = nullptr;
     OptState *d_opt = nullptr;
     skm_iteration_record *d_records = nullptr;
@@ -323,7 +327,7 @@ struct skm_evaluator {
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng,
-                        d_lcc_ref_partials};
+                        d_lcc_ref_partials, d_lcc_bound, d_gamma_fx};
         for (void *p : ptrs)
             if (p) device_release(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -636,7 +640,10 @@ skm_status ensure_lcc_buffers(skm_evaluator *ev)
     if (!ev->d_dense) SKM_TRY(dev_alloc(&ev->d_dense, ev->pr.n_vox));
     if (!ev->d_ccoef) SKM_TRY(dev_alloc(&ev->d_ccoef, ev->pr.n_sub));
     const long long np = lcc_reference_partials(ev->pr);
-    if (np > 0 && !ev->d_lcc_ref_partials) SKM_TRY(dev_alloc(&ev->d_lcc_ref_partials, (size_t)np));
+    if (np > 0 && !ev->d_lcc_ref_partials) {
+        SKM_TRY(dev_alloc(&ev->d_lcc_ref_partials, (size_t)np));
+        ev->lcc_ref_partials_n = np;
+    }
     return SKM_SUCCESS;
 }
 
@@ -667,7 +674,14 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
     } else if (measure == SKM_MEASURE_LCC) {
         SKM_TRY(ensure_lcc_buffers(ev));
         { PassTimer t(ev, 6); launch_dense_warp(pr, 0, ev->d_dense, s); }
-        { PassTimer t(ev, 7); launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s); }
+        const bool exact_deriv = deriv && pr.exact;
+        if (exact_deriv && !ev->d_lcc_bound)
+            SKM_TRY(dev_alloc(&ev->d_lcc_bound, (size_t)((pr.n_sub + kThreads - 1) / kThreads)));
+        {
+            PassTimer t(ev, 7);
+            launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s,
+                               exact_deriv ? ev->d_lcc_bound : nullptr);
+        }
         {
             PassTimer t(ev, 2);
             launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s, ev->d_rng, drawn, ev->rng_inc);
@@ -676,9 +690,11 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
             PassTimer t(ev, 8);
             if (pr.exact) {
                 if (!ev->d_gamma) SKM_TRY(dev_alloc(&ev->d_gamma, pr.n_vox));
-                launch_zero_words(ev->d_gamma, 4LL * pr.n_vox, s);
+                if (!ev->d_gamma_fx) SKM_TRY(dev_alloc(&ev->d_gamma_fx, (size_t)pr.n_vox * 3));
+                launch_zero_words(ev->d_gamma_fx, 6LL * pr.n_vox, s);
                 SKM_HIP(hipGetLastError());
-                launch_lcc_scatter(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_gamma, s);
+                launch_lcc_scatter(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_lcc_bound, blocks, ev->d_gamma_fx,
+                                   ev->d_gamma, s);
                 Problem dp = pr;  // gather over the dense voxel grid
                 if (!is_stationary(pr.kc.mode)) {
                     if (!ev->d_vox_pstate) {
@@ -921,8 +937,11 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
     if (desc->kernel_mode != SKM_KERNEL_STATIONARY)
         if ((st = dev_alloc(&ev->d_smp_pstate, (size_t)pr.n_sub * 2)) != SKM_SUCCESS) return cleanup(st);
     {
-        // sized for the dense voxel grid, which the exact LCC gather walks
-        const long long np = gather_partials_size(pr, pr.fixed.n[dim - 1]);
+        // sized for the dense voxel grid, which the exact LCC gather walks, and
+        // for the thinnest slab skm_set_tuning can select
+        Problem thin = pr;
+        thin.gather_split = 4;
+        const long long np = gather_partials_size(thin, pr.fixed.n[dim - 1]);
         if ((st = dev_alloc(&ev->d_gather_partials, (size_t)np)) != SKM_SUCCESS) return cleanup(st);
     }
     pr.gather_partials = ev->d_gather_partials;
@@ -1119,6 +1138,37 @@ SKM_API skm_status skm_read_timing(skm_evaluator *ev, skm_pass_times *out)
         ev->pass_ms[i] = 0.0;
         ev->pass_launches[i] = 0;
     }
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning)
+{
+    if (!ev || !tuning) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    if (tuning->lcc_ref_chunks < 0 || tuning->lcc_ref_chunks > 64)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "lcc_ref_chunks must be 0 (automatic) or 1..64");
+    if (tuning->gather_split != 0 && tuning->gather_split != 4 && tuning->gather_split != 8)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "gather_split must be 0 (automatic), 4 or 8");
+    if (tuning->value_split != 0 && tuning->value_split != 1 && tuning->value_split != 4 &&
+        tuning->value_split != 8 && tuning->value_split != 16)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "value_split must be 0 (automatic), 1, 4, 8 or 16");
+    SKM_HIP(hipStreamSynchronize(ev->stream));  // queued work may use the buffers resized below
+    ev->tuning = *tuning;
+    ev->pr.lcc_chunks = tuning->lcc_ref_chunks;
+    ev->pr.gather_split = tuning->gather_split;
+    ev->pr.value_split = tuning->value_split;
+    const long long np = lcc_reference_partials(ev->pr);
+    if (ev->d_lcc_ref_partials && np > ev->lcc_ref_partials_n) {
+        device_release(ev->d_lcc_ref_partials);
+        ev->d_lcc_ref_partials = nullptr;
+        ev->lcc_ref_partials_n = 0;
+    }
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_get_tuning(skm_evaluator *ev, skm_tuning *tuning)
+{
+    if (!ev || !tuning) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    *tuning = ev->tuning;
     return SKM_SUCCESS;
 }
 

@@ -51,6 +51,9 @@ struct Problem {
     const float *cp_row_lam;   // largest lam per row of control points (cy + n[1] cz)
     const float *smp_row_lam;  // largest lam per row of sample cells (my + nsub[1] mz)
     double *gather_partials;   // weighted-mode gather: FP64 sums per (slab, control-point tile)
+    int lcc_chunks;            // reference-window LCC derivative: workgroups per cp tile (0: automatic)
+    int gather_split;          // derivative gather: sample planes per slab, 4 or 8 (0: automatic, 8)
+    int value_split;           // value pass: waves per sample brick, 1/4/8/16 (0: automatic)
     float lam_cp_max;          // largest lam over the control points
     float lam_w_max;           // largest lam over the points a sample can look up (weight image, outside)
 };
@@ -63,6 +66,8 @@ struct Scalars {
     double mean_f, mean_m, rho, inv_denom;  // NCC derivative coefficients
     double coef_scale;     // MSE: 2/N, LCC: 1/N
     double reg[3];         // RKHS, RD, PG values
+    double fx_unit;        // exact LCC scatter: value of one fixed-point step
+    double fx_inv_unit;    // 1 / fx_unit
 };
 
 enum { kPartialsK = 8 };
@@ -128,14 +133,21 @@ void launch_result(const Scalars *sc, void *result, hipStream_t s);
 
 // LCC (skm_lcc.hip)
 void launch_dense_warp(const Problem &pr, int quirk3d, float4 *dense_g, hipStream_t s);
+// bound_blk (optional, one double per workgroup): the largest bound per
+// workgroup of |beta kw gradM| over a centre's neighbours, before the 1/N scale
 void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre_coef, double *partials,
-                        int *blocks_out, hipStream_t s);
+                        int *blocks_out, hipStream_t s, double *bound_blk = nullptr);
 // partials: lcc_reference_partials(pr) doubles (none needed when it is 0)
 void launch_lcc_gather_reference(const Problem &pr, const float4 *dense_g, const float4 *centre_coef,
                                  const Scalars *sc, float *deriv_soa, double *partials, hipStream_t s);
 long long lcc_reference_partials(const Problem &pr);
-void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
-                        float4 *gamma, hipStream_t s);
+// Exact-mode scatter in 64-bit fixed point (order-independent, so the sums
+// are the same bits on every run): the step fx_unit is chosen from the
+// centres' bounds so that no voxel's sum can overflow; then gamma = fx * unit.
+// fx: 3 long longs per fixed voxel, zeroed by the caller.
+void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, Scalars *sc,
+                        const double *bound_blk, int bound_blocks, unsigned long long *fx, float4 *gamma,
+                        hipStream_t s);
 
 // optimizer (skm_kernels.hip)
 struct OptState {

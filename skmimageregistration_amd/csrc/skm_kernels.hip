@@ -456,7 +456,7 @@ template <> struct CpTile<3> { static constexpr int X = 1, Y = 1, Z = 4; };
 template <int DIM>
 __host__ __device__ __forceinline__ int gather_slab(const Problem &pr)
 {
-    return DIM == 3 ? 8 : max(pr.fixed.n[1], 1);
+    return DIM == 3 ? (pr.gather_split == 4 ? 4 : 8) : max(pr.fixed.n[1], 1);
 }
 
 // Union of the reference windows of one control-point tile along the last
@@ -933,13 +933,15 @@ static void value_impl(const Problem &pr, int measure, float4 *sg, float *sb, do
     // workgroup share a brick and split its control-point rows, 4, 8 or 16
     // ways, the fewest that reach 4096 waves
     constexpr long long kWavesWanted = 4096;
-    if ((long long)grid.x * grid.y * grid.z * (kThreads / 64) < kWavesWanted) {
+    const bool few = (long long)grid.x * grid.y * grid.z * (kThreads / 64) < kWavesWanted;
+    if ((pr.value_split == 0 && few) || pr.value_split > 1) {
         grid = dim3((pr.nsub[0] + WaveBrick<DIM>::X - 1) / WaveBrick<DIM>::X,
                     (pr.nsub[1] + WaveBrick<DIM>::Y - 1) / WaveBrick<DIM>::Y,
                     DIM == 3 ? (pr.nsub[2] + WaveBrick<DIM>::Z - 1) / WaveBrick<DIM>::Z : 1);
         const long long bricks = (long long)grid.x * grid.y * grid.z;
         *blocks = (int)bricks;
-        const int split = bricks * 4 >= kWavesWanted ? 4 : bricks * 8 >= kWavesWanted ? 8 : 16;
+        const int split = pr.value_split > 1 ? pr.value_split
+                          : bricks * 4 >= kWavesWanted ? 4 : bricks * 8 >= kWavesWanted ? 8 : 16;
         auto go = [&](auto split_c) {
             constexpr int S = decltype(split_c)::value;
             if (measure == SKM_MEASURE_MSE)

@@ -177,9 +177,12 @@ __device__ __forceinline__ long long voxel_index(const Problem &pr, const int *p
 // beyond the FP32 range; with it |A| < 2e8 and B < 2e16 d1.
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const float4 *__restrict__ dense,
-                                                          float4 *__restrict__ coef, double *__restrict__ partials)
+                                                          float4 *__restrict__ coef, double *__restrict__ partials,
+                                                          double *__restrict__ bound_blk)
 {
     __shared__ double lds[kPartialsK * 4];
+    __shared__ double lds_max[kThreads / 64];
+    double bound = 0.0;
     long long s = blockIdx.x * (long long)kThreads + threadIdx.x;
     double acc[kPartialsK];
 #pragma unroll
@@ -194,6 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
         int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
         neighbour_box<DIM>(pr, pos, (double)st.support, lo, hi);
         double sf = 0, sm = 0, sff = 0, smm = 0, sfm = 0;
+        float fk = 0.0f, mk = 0.0f, gk = 0.0f;  // largest |kw F|, |kw M|, |gradM| component
         int count = 0;
         const int sn = pr.subn;
         const int z0 = DIM == 3 ? lo[2] : 0, z1 = DIM == 3 ? hi[2] : 0;
@@ -220,6 +224,11 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
                     double fv = (double)(kw * pr.fixed_data[v]);
                     sf += fv; sm += mv; sff += fv * fv; smm += mv * mv; sfm += fv * mv;
                     ++count;
+                    if (bound_blk) {
+                        fk = fmaxf(fk, fabsf((float)fv));
+                        mk = fmaxf(mk, fabsf((float)mv));
+                        gk = fmaxf(gk, fmaxf(fabsf(g4.x), fmaxf(fabsf(g4.y), fabsf(g4.z))));
+                    }
                 }
         double cc = 0.0;
         float A = 0.0f, B = 0.0f, C = 0.0f;
@@ -237,6 +246,9 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
                 A = (float)(-2.0 * vv * inv_u);
                 B = (float)(2.0 * d1 * vv * vv * inv_u * inv_u);
                 C = (float)(2.0 * vv * msf * inv_u - 2.0 * d1 * msm * vv * vv * inv_u * inv_u);
+                // |beta kw g| <= (|A| max|kw F| + |B| max|kw M| + |C|) max|g|, with
+                // kw <= 1; the factor covers the FP32 roundings of the scatter
+                bound = ((double)fabsf(A) * fk + (double)fabsf(B) * mk + (double)fabsf(C)) * gk * 1.001;
             }
         }
         coef[s] = make_float4(A, B, C, 0.0f);
@@ -248,6 +260,18 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
     if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < kPartialsK; ++k) partials[blockIdx.x * (long long)kPartialsK + k] = acc[k];
+    if (bound_blk) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) bound = fmax(bound, __shfl_xor(bound, off, 64));
+        if ((threadIdx.x & 63) == 0) lds_max[threadIdx.x >> 6] = bound;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+#pragma unroll
+            for (int w = 0; w < kThreads / 64; ++w) m = fmax(m, lds_max[w]);
+            bound_blk[blockIdx.x] = m;
+        }
+    }
 }
 
 // Control points per workgroup of the reference-mode derivative: the tile's
@@ -445,16 +469,55 @@ __global__ __launch_bounds__(kThreads) void k_lcc_ref_finish(Problem pr, int chu
 
 // Exact mode: every valid centre scatters beta kw gradM onto its neighbours,
 // pre-scaled by 1/N; the control points then gather sum_y w_j k(p_j, y) gamma(y).
+// The fixed-point step of the exact scatter: with every term bounded by
+// 2^e (the largest centre bound times 1/N) and at most n_sub terms per voxel
+// (each centre adds at most one term to a voxel), a step of
+// 2^(e - 62 + ceil(log2(n_sub + 1)) + 1) keeps every voxel's sum below 2^62.
+__global__ __launch_bounds__(kThreads) void k_lcc_fx_unit(const double *__restrict__ bound_blk, int blocks,
+                                                          long long n_sub, Scalars *__restrict__ sc)
+{
+    __shared__ double lds_max[kThreads / 64];
+    double m = 0.0;
+    for (int b = threadIdx.x; b < blocks; b += kThreads) m = fmax(m, bound_blk[b]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) lds_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kThreads / 64; ++w) m = fmax(m, lds_max[w]);
+        m *= sc->coef_scale;
+        int e = 0;
+        if (m > 0.0 && isfinite(m)) frexp(m, &e);  // m < 2^e
+        int h = 1;
+        while ((1LL << h) <= n_sub) ++h;            // 2^h > n_sub
+        const int shift = e - 62 + h + 1;
+        sc->fx_unit = ldexp(1.0, shift);
+        sc->fx_inv_unit = ldexp(1.0, -shift);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_lcc_fx_finish(const unsigned long long *__restrict__ fx, long long n,
+                                                            const Scalars *__restrict__ sc, float4 *__restrict__ gamma)
+{
+    const long long v = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (v >= n) return;
+    const double u = sc->fx_unit;
+    gamma[v] = make_float4((float)((double)(long long)fx[3 * v] * u), (float)((double)(long long)fx[3 * v + 1] * u),
+                           (float)((double)(long long)fx[3 * v + 2] * u), 0.0f);
+}
+
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_lcc_scatter(Problem pr, const float4 *__restrict__ dense,
                                                           const float4 *__restrict__ coef,
-                                                          const Scalars *__restrict__ sc, float4 *__restrict__ gamma)
+                                                          const Scalars *__restrict__ sc,
+                                                          unsigned long long *__restrict__ sums)
 {
     // one centre per thread (consecutive threads: consecutive centres along
     // x); the wave walks the neighbour offsets in step, so that a step's
     // loads and atomic adds fall on few cache lines
     const long long s = blockIdx.x * (long long)kThreads + threadIdx.x;
     const float scale = (float)sc->coef_scale;
+    const double inv_unit = sc->fx_inv_unit;
     float4 abc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float x[3] = {0.0f, 0.0f, 0.0f};
     int nlo[3] = {0, 0, 0}, cnt[3] = {0, 0, 0};
@@ -506,9 +569,11 @@ __global__ __launch_bounds__(kThreads) void k_lcc_scatter(Problem pr, const floa
         float fv = kw * pr.fixed_data[v];
         float mv = kw * g4.w;
         float beta = fmaf(abc.x, fv, fmaf(abc.y, mv, abc.z)) * kw * scale;
-        atomicAdd(&gamma[v].x, beta * g4.x);
-        atomicAdd(&gamma[v].y, beta * g4.y);
-        if (DIM == 3) atomicAdd(&gamma[v].z, beta * g4.z);
+        // integer adds commute: the sums do not depend on the order of the atomics
+        atomicAdd(&sums[3 * v], (unsigned long long)__double2ll_rn((double)(beta * g4.x) * inv_unit));
+        atomicAdd(&sums[3 * v + 1], (unsigned long long)__double2ll_rn((double)(beta * g4.y) * inv_unit));
+        if (DIM == 3)
+            atomicAdd(&sums[3 * v + 2], (unsigned long long)__double2ll_rn((double)(beta * g4.z) * inv_unit));
     }
 }
 
@@ -550,17 +615,17 @@ void launch_dense_warp(const Problem &pr, int quirk3d, float4 *dense_g, hipStrea
 
 template <int DIM, int MODE>
 static void centres_impl(const Problem &pr, const float4 *dense, float4 *coef, double *partials, int *blocks,
-                         hipStream_t s)
+                         double *bound_blk, hipStream_t s)
 {
     unsigned g = (unsigned)((pr.n_sub + kThreads - 1) / kThreads);
     *blocks = (int)g;
-    k_lcc_centres<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, partials);
+    k_lcc_centres<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, partials, bound_blk);
 }
 
 void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre_coef, double *partials,
-                        int *blocks_out, hipStream_t s)
+                        int *blocks_out, hipStream_t s, double *bound_blk)
 {
-    SKM_LCC_DISPATCH(pr, centres_impl, pr, dense_g, centre_coef, partials, blocks_out, s);
+    SKM_LCC_DISPATCH(pr, centres_impl, pr, dense_g, centre_coef, partials, blocks_out, bound_blk, s);
 }
 
 template <int DIM>
@@ -571,7 +636,7 @@ static long long ref_tiles(const Problem &pr)
            (DIM == 3 ? (pr.cp.n[2] + TZ - 1) / TZ : 1);
 }
 
-// Workgroups per tile of the reference-mode derivative: enough for ~2048
+// Workgroups per tile of the reference-mode derivative: enough for ~8192
 // workgroups in all, at most 64.
 static int ref_chunks(long long tiles)
 {
@@ -579,10 +644,15 @@ static int ref_chunks(long long tiles)
     return (int)std::min<long long>(64, std::max<long long>(1, (kWanted + tiles - 1) / tiles));
 }
 
+static int chunks_of(const Problem &pr, long long tiles)
+{
+    return pr.lcc_chunks > 0 ? std::min(pr.lcc_chunks, 64) : ref_chunks(tiles);
+}
+
 long long lcc_reference_partials(const Problem &pr)
 {
     const long long tiles = pr.dim == 3 ? ref_tiles<3>(pr) : ref_tiles<2>(pr);
-    const int chunks = ref_chunks(tiles);
+    const int chunks = chunks_of(pr, tiles);
     const int t = pr.dim == 3 ? LcTile<3>::X * LcTile<3>::Y * LcTile<3>::Z : LcTile<2>::X * LcTile<2>::Y;
     return chunks > 1 ? tiles * chunks * t * 3 : 0;
 }
@@ -592,7 +662,7 @@ static void gather_ref_impl(const Problem &pr, const float4 *dense, const float4
                             float *deriv, double *partials, hipStream_t s)
 {
     const long long tiles = ref_tiles<DIM>(pr);
-    const int chunks = partials ? ref_chunks(tiles) : 1;
+    const int chunks = partials ? chunks_of(pr, tiles) : 1;
     k_lcc_gather_ref<DIM, MODE><<<(unsigned)(tiles * chunks), kThreads, 0, s>>>(pr, dense, coef, sc, deriv, chunks,
                                                                                 partials);
     if (chunks > 1) {
@@ -609,16 +679,20 @@ void launch_lcc_gather_reference(const Problem &pr, const float4 *dense_g, const
 
 template <int DIM, int MODE>
 static void scatter_impl(const Problem &pr, const float4 *dense, const float4 *coef, const Scalars *sc,
-                         float4 *gamma, hipStream_t s)
+                         unsigned long long *fx, hipStream_t s)
 {
     unsigned g = (unsigned)((pr.n_sub + kThreads - 1) / kThreads);
-    k_lcc_scatter<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, sc, gamma);
+    k_lcc_scatter<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, sc, fx);
 }
 
-void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
-                        float4 *gamma, hipStream_t s)
+void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, Scalars *sc,
+                        const double *bound_blk, int bound_blocks, unsigned long long *fx, float4 *gamma,
+                        hipStream_t s)
 {
-    SKM_LCC_DISPATCH(pr, scatter_impl, pr, dense_g, centre_coef, sc, gamma, s);
+    k_lcc_fx_unit<<<1, kThreads, 0, s>>>(bound_blk, bound_blocks, pr.n_sub, sc);
+    SKM_LCC_DISPATCH(pr, scatter_impl, pr, dense_g, centre_coef, sc, fx, s);
+    const unsigned g = (unsigned)((pr.n_vox + kThreads - 1) / kThreads);
+    k_lcc_fx_finish<<<g, kThreads, 0, s>>>(fx, pr.n_vox, sc, gamma);
 }
 
 }  // namespace skm

@@ -267,6 +267,16 @@ class GpuKernelEvaluator:
     def synchronize(self):
         check(self._lib.skm_synchronize(self._h))
 
+    def set_tuning(self, lcc_ref_chunks=0, gather_split=0, value_split=0):
+        """Launch shapes (skm_set_tuning); 0 keeps the library's choice."""
+        t = _capi.SkmTuning(int(lcc_ref_chunks), int(gather_split), int(value_split))
+        check(self._lib.skm_set_tuning(self._h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = _capi.SkmTuning()
+        check(self._lib.skm_get_tuning(self._h, ctypes.byref(t)))
+        return dict(lcc_ref_chunks=t.lcc_ref_chunks, gather_split=t.gather_split, value_split=t.value_split)
+
     def cube_offsets(self):
         n = ctypes.c_int64()
         check(self._lib.skm_get_cube_offsets(self._h, None, ctypes.byref(n)))

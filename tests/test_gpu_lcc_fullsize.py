@@ -7,8 +7,9 @@ leaves ringing values far below FP32's normal range: both derivative modes
 must stay finite there (centres with |u| <= 1e-16 add no derivative) and give
 the same value.  The exact-gradient derivative is checked against central
 differences of the value on the same volumes offset by 0.5, where no
-neighbourhood is flat (offsets fixed after the first draw): within 1e-3
-relative on the four largest entries (measured: 7e-5).
+neighbourhood is flat (offsets fixed after the first draw): within 1e-4
+relative on the four largest entries (measured: 7e-5), and the same bits
+when the evaluation is repeated (the scatter sums in fixed point).
 """
 import numpy as np
 import pytest
@@ -57,6 +58,8 @@ def test_lcc_full_size_exact_derivative_against_central_differences():
     print("LCC, offset volumes, exact derivative", flush=True)
     r, d = ev.getValueAndDerivative(MEASURE.LCC, cp)
     assert ev.getValue(MEASURE.LCC, cp, do_resampling=False).measure == r.measure
+    r2, d2 = ev.getValueAndDerivative(MEASURE.LCC, cp, do_resampling=False)
+    assert r2.measure == r.measure and np.array_equal(d2, d), "exact LCC derivative differs between runs"
     d = d.astype(np.float64)
     for idx in np.argsort(-np.abs(d))[:4]:
         j, k = idx % n, idx // n
@@ -67,5 +70,5 @@ def test_lcc_full_size_exact_derivative_against_central_differences():
         fd = (ev.getValue(MEASURE.LCC, cpp, do_resampling=False).measure -
               ev.getValue(MEASURE.LCC, cpm, do_resampling=False).measure) / (2 * h)
         print(f"entry {idx}: derivative {d[idx]:.6e} central difference {fd:.6e}", flush=True)
-        assert abs(fd - d[idx]) < 1e-3 * abs(d[idx]) + 1e-12
+        assert abs(fd - d[idx]) < 1e-4 * abs(d[idx]) + 1e-12
     ev.close()

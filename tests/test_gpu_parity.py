@@ -18,8 +18,10 @@ VALUE_TOL = 1e-5
 DERIV_TOL = 1e-4
 
 
-def _compare(case, measure, regs=(0.0, 0.0, 0.0), pg_scaling=1.0, calls=1, value_only=False):
+def _compare(case, measure, regs=(0.0, 0.0, 0.0), pg_scaling=1.0, calls=1, value_only=False, tuning=None):
     ev = case.gpu()
+    if tuning:
+        ev.set_tuning(**tuning)
     if regs[0] > 0:
         ev.SetRegularizerRKHS(regs[0])
     if regs[1] > 0:
@@ -136,3 +138,13 @@ def test_value_matches_value_and_derivative():
         r2 = ev.getValue(measure, case.cp, case.cpw, do_resampling=False)
         assert r1.measure == r2.measure
     ev.close()
+
+
+@pytest.mark.parametrize("tuning", [dict(gather_split=4), dict(value_split=1), dict(value_split=4),
+                                    dict(value_split=16), dict(gather_split=4, value_split=8)],
+                         ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
+@pytest.mark.parametrize("mode", [0, 2])
+def test_launch_shapes(tuning, mode):
+    """Every launch shape skm_set_tuning can select gives the oracle's result."""
+    case = Case(dim=3, shape=(18, 20, 22), cp_spacing=5.0, sigma=12.0, mode=mode, subsample=2, seed=43)
+    _compare(case, MEASURE.NCC, tuning=tuning)
