@@ -205,6 +205,20 @@ SKM_API skm_status skm_control_point_grid(int32_t dim, const skm_grid *image, co
 SKM_API skm_status skm_gaussian_smooth(int32_t dim, const skm_grid *grid, const float *in, double variance,
                                        float *out);
 
+/* skm_gaussian_smooth on device arrays without any allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): its temporaries
+ * (the Gaussian taps and two intermediate planes) live in a fixed device
+ * array of the library (64 MiB + 32 KiB per GPU, held until the process
+ * ends).  Limit: 8,388,608 voxels (e.g. 256 x 256 x 128); larger images are
+ * rejected with SKM_ERROR_INVALID_ARGUMENT.  d_in and d_out may be the same
+ * array.  The work is enqueued on hip_stream (NULL: the null stream) and the
+ * call returns without waiting; the result equals skm_gaussian_smooth's bit
+ * for bit.  Calls that overlap on one GPU share the array and race: order
+ * them on one stream or synchronise between them. */
+SKM_API skm_status skm_gaussian_smooth_device_noalloc(int32_t dim, const skm_grid *grid, const float *d_in,
+                                                      double variance, float *d_out, void *hip_stream);
+
 /* Arguments of ComputeAnisotropicTensor (include/AnisotropicTensorUtils.h:37-39). */
 typedef struct skm_tensor_options {
     double sigma;                   /* sigma_C4 of the scale: image smoothing variance; sigma/4

@@ -162,6 +162,8 @@ _SIGNATURES = {
                                     ctypes.POINTER(SkmIterationRecord)]),
     "skm_gaussian_smooth": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p, ctypes.c_double,
                                            c_float_p]),
+    "skm_gaussian_smooth_device_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), ctypes.c_void_p,
+                                                          ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
     "skm_anisotropic_tensor": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p,
                                               ctypes.POINTER(SkmTensorOptions), c_float_p, c_double_p, c_double_p,
                                               c_double_p]),

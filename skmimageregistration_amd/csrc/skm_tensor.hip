@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -610,3 +611,116 @@ skm_status skm::anisotropic_tensor_device(int dim, const skm_grid &grid_ref, con
     return SKM_SUCCESS;
 }
 
+
+// ------------------------------------------------------------ allocation-free smoothing
+//
+// skm_gaussian_smooth_device_noalloc: the smoothing of skm_gaussian_smooth on
+// device arrays with every temporary in `scratch`, a file-scope device array
+// of fixed size, so that the call allocates nothing (graph capture, serving
+// loops).  Each GPU holds its own copy of the array at its own address,
+// looked up once per device with hipGetSymbolAddress; host code uses that
+// address, never the array's name.  All calls on one GPU share the array:
+// calls that overlap on a GPU race.  Nothing in it needs to start at zero
+// (every element is written before it is read).
+
+namespace skm {
+namespace {
+
+constexpr long long kNoallocMaxVoxels = 8388608;     // 256 x 256 x 128 floats per plane (32 MiB)
+constexpr int kNoallocTapsPerAxis = 2 * (int)kMaximumWidth + 1;
+constexpr size_t kNoallocTapBytes = 4096 * sizeof(double);
+constexpr size_t kNoallocScratchBytes = kNoallocTapBytes + 2 * (size_t)kNoallocMaxVoxels * sizeof(float);
+constexpr int kNoallocMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kNoallocMaxDevices] = {};
+
+// device address of `scratch` on the current device, looked up once per device
+skm_status scratch_address(void **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kNoallocMaxDevices) return set_last_error(SKM_ERROR_NOT_SUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_last_error(SKM_ERROR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SKM_SUCCESS;
+}
+
+// one axis's taps, passed by value so that no host-to-device copy is needed
+struct TapArg {
+    int n;
+    double c[kNoallocTapsPerAxis];
+};
+
+__global__ void __launch_bounds__(kThreads) k_write_taps(TapArg taps, double *__restrict__ dst)
+{
+    if ((int)threadIdx.x < taps.n) dst[threadIdx.x] = taps.c[threadIdx.x];
+}
+
+}  // namespace
+}  // namespace skm
+
+extern "C" {
+
+SKM_API skm_status skm_gaussian_smooth_device_noalloc(int32_t dim, const skm_grid *grid, const float *d_in,
+                                                      double variance, float *d_out, void *hip_stream)
+{
+    skm_status st = validate_image(dim, grid, d_in, d_out);
+    if (st != SKM_SUCCESS) return st;
+    if (!(variance >= 0.0)) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "variance must not be negative");
+    const Dims d = make_dims(dim, *grid);
+    if (d.n_vox > kNoallocMaxVoxels)
+        return set_last_error(SKM_ERROR_INVALID_ARGUMENT,
+                              "image of " + std::to_string(d.n_vox) + " voxels exceeds the allocation-free limit of " +
+                                  std::to_string(kNoallocMaxVoxels) + " voxels");
+    void *base = nullptr;
+    if ((st = scratch_address(&base)) != SKM_SUCCESS) return st;
+    double *taps = static_cast<double *>(base);
+    float *plane_a = reinterpret_cast<float *>(static_cast<unsigned char *>(base) + kNoallocTapBytes);
+    float *plane_b = plane_a + kNoallocMaxVoxels;
+    hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+    // taps of every axis (the coefficients of make_axis_kernels)
+    int offset[3] = {0, 0, 0}, radius[3] = {0, 0, 0}, used = 0;
+    for (int a = 0; a < dim; ++a) {
+        const double t = variance / (grid->spacing[a] * grid->spacing[a]);
+        const std::vector<double> c = gaussian_kernel(t);
+        TapArg arg{};
+        arg.n = (int)c.size();
+        if (arg.n > kNoallocTapsPerAxis) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "Gaussian kernel too wide");
+        for (int i = 0; i < arg.n; ++i) {
+            if (!std::isfinite(c[i]))
+                return set_last_error(SKM_ERROR_INVALID_ARGUMENT,
+                                      "Gaussian variance " + std::to_string(t) +
+                                          " pixels^2 is too large for the discrete kernel (exp overflows)");
+            arg.c[i] = c[i];
+        }
+        offset[a] = used;
+        radius[a] = (arg.n - 1) / 2;
+        hipLaunchKernelGGL(k_write_taps, dim3(1), dim3(kThreads), 0, stream, arg, taps + used);
+        used += arg.n;
+    }
+    // separable passes, last axis first: in -> a (-> b) -> out; out may be in
+    const int blocks = launch_blocks(d.n_vox);
+    const float *src = d_in;
+    for (int axis = dim - 1; axis >= 0; --axis) {
+        float *dst = axis == 0 ? d_out : ((dim - 1 - axis) % 2 == 0 ? plane_a : plane_b);
+        hipLaunchKernelGGL(k_smooth_axis, dim3(blocks), dim3(kThreads), 0, stream, src, dst, taps + offset[axis],
+                           radius[axis], axis, d, 1, 1.0f);
+        src = dst;
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_last_error(SKM_ERROR_HIP, std::string("smoothing launch: ") + hipGetErrorString(e));
+    return SKM_SUCCESS;
+}
+
+}  // extern "C"
