@@ -97,6 +97,7 @@ typedef struct skm_result {
     double reg_value_pg;
     uint32_t num_pixels_counted;
     uint32_t reserved;
+    double reg_value_bilateral;   /* no reference counterpart (skm_set_regularizer_bilateral) */
 } skm_result;
 
 typedef struct skm_evaluator skm_evaluator;
@@ -116,6 +117,19 @@ SKM_API const char *skm_last_error(void);
 SKM_API skm_status skm_set_regularizer_rkhs(skm_evaluator *ev, double weight);
 SKM_API skm_status skm_set_regularizer_rd(skm_evaluator *ev, double weight, double scaling);
 SKM_API skm_status skm_set_regularizer_pg(skm_evaluator *ev, double weight, double scaling);
+
+/* Bilateral regularizer, no counterpart in the reference (whose closest term
+ * is RD, GpuEvaluatorRegularizers.cu:96-166; a radiometric term appears only
+ * as a comment, itkRegularizedImageToImageMetricv4.h:204).  Per control point
+ * j over the points i within 3 spatial_sigma:
+ *   B_ij    = exp(-|p_j - p_i|^2 / (2 spatial_sigma^2))
+ *             * exp(-(I(p_j) - I(p_i))^2 / (2 intensity_sigma^2)),
+ *   value   = sum_j sum_i w_j B_ij |c_j - c_i|^2   (skm_result.reg_value_bilateral),
+ *   d/dc_j  = weight * sum_i 2 B_ij (w_j + w_i)(c_j - c_i), added to the derivative,
+ * with I the fixed image at the control point's nearest voxel (0 outside) and
+ * w the control-point weights.  weight 0 switches it off. */
+SKM_API skm_status skm_set_regularizer_bilateral(skm_evaluator *ev, double weight, double spatial_sigma,
+                                                double intensity_sigma);
 
 /* Previous-scale displacement field (the ImageNearest<VecF> argument of
  * getValue / getValueAndDerivative / evaluateDisplacementField, cached by the
@@ -389,6 +403,8 @@ typedef struct skm_scale_params {
     double initial_step_size;
     int32_t sampling_rate, neighborhood_sampling;
     double smoothing_variance;       /* > 1e-5: GaussianSmoothing of both images   */
+    double reg_bilateral;            /* skm_set_regularizer_bilateral (no SKMReg key) */
+    double reg_bilateral_spatial_sigma, reg_bilateral_intensity_sigma;
 } skm_scale_params;
 
 typedef struct skm_registration_desc {

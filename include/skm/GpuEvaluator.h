@@ -30,6 +30,7 @@ public:
         double reg_value_rd;
         double reg_value_pg;
         unsigned numPixelsCountedGpu;
+        double reg_value_bilateral;  // no reference counterpart (SetRegularizerBilateral)
     };
 
     explicit GpuKernelEvaluator(const skm_desc &desc) : dim_(desc.dim)
@@ -55,6 +56,12 @@ public:
     void SetRegularizerPG(double weight, double scaling = 1.0)
     {
         check_status(skm_set_regularizer_pg(ev_, weight, scaling), "SetRegularizerPG");
+    }
+    // the bilateral regularizer of skm_set_regularizer_bilateral (no reference counterpart)
+    void SetRegularizerBilateral(double weight, double spatial_sigma = 12.0, double intensity_sigma = 0.1)
+    {
+        check_status(skm_set_regularizer_bilateral(ev_, weight, spatial_sigma, intensity_sigma),
+                     "SetRegularizerBilateral");
     }
 
     // Previous-scale field (AoS doubles on `grid`); nullptr clears it.  The
@@ -114,7 +121,8 @@ private:
     }
     static EvaluationResult convert(const skm_result &r)
     {
-        return {r.measure, r.reg_value_rkhs, r.reg_value_rd, r.reg_value_pg, r.num_pixels_counted};
+        return {r.measure, r.reg_value_rkhs, r.reg_value_rd, r.reg_value_pg, r.num_pixels_counted,
+                r.reg_value_bilateral};
     }
 
     skm_evaluator *ev_ = nullptr;

@@ -100,6 +100,10 @@ public:
     SKM_ACCESSOR(RegularizerPG, double, pg_)
     SKM_ACCESSOR(RegularizerRDScaling, double, rd_scaling_)
     SKM_ACCESSOR(RegularizerPGScaling, double, pg_scaling_)
+    // bilateral regularizer (skm_set_regularizer_bilateral; no reference counterpart)
+    SKM_ACCESSOR(RegularizerBilateral, double, bl_)
+    SKM_ACCESSOR(RegularizerBilateralSpatialSigma, double, bl_sigma_s_)
+    SKM_ACCESSOR(RegularizerBilateralIntensitySigma, double, bl_sigma_i_)
     SKM_ACCESSOR(UseNccMetric, bool, use_ncc_)
     SKM_ACCESSOR(UseLccMetric, bool, use_lcc_)
     SKM_ACCESSOR(UseNegativeGradient, bool, negative_)
@@ -130,7 +134,7 @@ public:
     // Initialize (itkRegularizedImageToImageMetricv4.hxx:71-106)
     void Initialize()
     {
-        if (l1_ < 0 || l21_ < 0 || l2_ < 0 || rkhs_ < 0 || rd_ < 0 || pg_ < 0)
+        if (l1_ < 0 || l21_ < 0 || l2_ < 0 || rkhs_ < 0 || rd_ < 0 || pg_ < 0 || bl_ < 0)
             throw ExceptionObject("Regularization weights must be positive.");
         if (rd_scaling_ <= 0 || pg_scaling_ <= 0)
             throw ExceptionObject("Robust regularization scales must be strictly positive.");
@@ -216,6 +220,7 @@ private:
         if (rkhs_ > 0) ev->SetRegularizerRKHS(rkhs_);
         if (rd_ > 0) ev->SetRegularizerRD(rd_, rd_scaling_);
         if (pg_ > 0) ev->SetRegularizerPG(pg_, pg_scaling_);
+        if (bl_ > 0) ev->SetRegularizerBilateral(bl_, bl_sigma_s_, bl_sigma_i_);
         const auto cp = t->GetControlPointImage();
         const auto cpw = t->GetControlPointWeightImage();
         std::vector<double> cpd(cp->GetNumberOfPixels() * N), cpwd(cpw->GetBufferPointer(),
@@ -237,6 +242,7 @@ private:
         if (rkhs_ > 0) reg += rkhs_ * r.reg_value_rkhs;
         if (rd_ > 0) reg += rd_ * r.reg_value_rd;
         if (pg_ > 0) reg += pg_ * r.reg_value_pg;
+        if (bl_ > 0) reg += bl_ * r.reg_value_bilateral;
         if (verbosity_ >= 1) {
             std::stringstream s;
             s << "metric: " << r.measure + reg << ",\t value: " << r.measure;
@@ -246,6 +252,7 @@ private:
             if (rkhs_ > 0) s << ",\t rkhs: " << rkhs_ * r.reg_value_rkhs;
             if (rd_ > 0) s << ",\t rd: " << rd_ * r.reg_value_rd;
             if (pg_ > 0) s << ",\t pg: " << pg_ * r.reg_value_pg;
+            if (bl_ > 0) s << ",\t bilateral: " << bl_ * r.reg_value_bilateral;
             if (verbosity_ >= 2) std::cout << s.str() << std::endl;
             std::ofstream fs(output_filename_, std::ios::app);
             fs << s.str() << std::endl;
@@ -260,6 +267,7 @@ private:
     typename Adapter::Pointer adapter_;
     typename Summation::Pointer sum_;
     double l1_ = 0, l21_ = 0, l2_ = 0, rkhs_ = 0, rd_ = 0, pg_ = 0, rd_scaling_ = 1.0, pg_scaling_ = 1.0;
+    double bl_ = 0, bl_sigma_s_ = 12.0, bl_sigma_i_ = 0.1;
     bool use_ncc_ = false, use_lcc_ = false, negative_ = true, resampling_ = true;
     unsigned subsample_ = N == 3 ? 4 : 1, subsample_neighborhood_ = N == 3 ? 4 : 1;
     std::string output_filename_ = "/tmp/output.txt";
@@ -341,6 +349,9 @@ public:
         if (metric_->GetRegularizerRKHS() > 0) ev->SetRegularizerRKHS(metric_->GetRegularizerRKHS());
         if (metric_->GetRegularizerRD() > 0) ev->SetRegularizerRD(metric_->GetRegularizerRD(), metric_->GetRegularizerRDScaling());
         if (metric_->GetRegularizerPG() > 0) ev->SetRegularizerPG(metric_->GetRegularizerPG(), metric_->GetRegularizerPGScaling());
+        if (metric_->GetRegularizerBilateral() > 0)
+            ev->SetRegularizerBilateral(metric_->GetRegularizerBilateral(), metric_->GetRegularizerBilateralSpatialSigma(),
+                                        metric_->GetRegularizerBilateralIntensitySigma());
         check_status(skm_optimize(ev->handle(), &mo, &oo, w.data(), p.data(), rec.data()), "skm_optimize");
         position_.assign(p.begin(), p.end());
         metric_->SetParameters(position_);

@@ -109,6 +109,9 @@ def _declare(lib):
     lib.skmo_regularizer.restype = None
     lib.skmo_regularizer.argtypes = [ctypes.POINTER(Problem), ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                      c_double_p, c_double_p]
+    lib.skmo_regularizer_bilateral.restype = None
+    lib.skmo_regularizer_bilateral.argtypes = [ctypes.POINTER(Problem), ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, c_double_p, c_double_p]
     lib.skmo_host_regularizers.restype = ctypes.c_double
     lib.skmo_host_regularizers.argtypes = [ctypes.c_int, ctypes.c_longlong, c_double_p, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_int, c_double_p,
@@ -244,6 +247,14 @@ class OracleProblem:
         if per_sample:
             return res, der, samples
         return res, der
+
+    def bilateral(self, weight, sigma_s, sigma_i, with_derivative=True):
+        """(sum of the per-cp bilateral values, weight x derivative SoA)."""
+        vals = np.zeros(self.n_cp, np.float64)
+        der = np.zeros(self.n_cp * self.dim, np.float64)
+        self.lib.skmo_regularizer_bilateral(ctypes.byref(self.p), weight, sigma_s, sigma_i, dp(vals),
+                                            dp(der) if with_derivative else None)
+        return float(vals.sum()), der
 
     def displacement(self):
         n = int(np.prod(self._fixed.shape))

@@ -578,6 +578,73 @@ void skmo_regularizer(const skmo_problem *pr, int which, double weight, double p
     }
 }
 
+/* Bilateral regularizer: no counterpart in the reference, whose closest term
+ * is RD (GpuEvaluatorRegularizers.cu:96-166, sum_i k(p_j, p_i) w_j |c_j - c_i|^2
+ * with the transform kernel) and which names a radiometric term only in a
+ * comment (itkRegularizedImageToImageMetricv4.h:204).  Here the spatial
+ * factor is a Gaussian of standard deviation sigma_s cut at 3 sigma_s and
+ * the intensity factor a Gaussian of standard deviation sigma_i of the
+ * difference of the fixed image at the two control points (nearest
+ * neighbour, 0 outside):
+ *   B_ij = exp(-r_ij^2 / (2 sigma_s^2)) exp(-(I_j - I_i)^2 / (2 sigma_i^2)),  r_ij < 3 sigma_s
+ *   value_j = sum_i w_j B_ij |c_j - c_i|^2        (unweighted by `weight`, like RD)
+ *   d/dc_j (sum_j value_j) = sum_i 2 B_ij (w_j + w_i)(c_j - c_i)   (B symmetric)
+ * The derivative, times `weight`, is added to deriv_soa.  Offsets between
+ * control points are (index difference) x spacing. */
+void skmo_regularizer_bilateral(const skmo_problem *pr, double weight, double sigma_s, double sigma_i,
+                                double *values, double *deriv_soa)
+{
+    int n = pr->dim;
+    long long ncp = grid_count(&pr->cp, n);
+    double *inten = (double *)calloc((size_t)ncp, sizeof(double));
+    for (long long jj = 0; jj < ncp; ++jj) {
+        int j[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+        long long r = jj;
+        for (int d = 0; d < n; ++d) { j[d] = (int)(r % pr->cp.size[d]); r /= pr->cp.size[d]; }
+        double fj[3] = {j[0], j[1], j[2]}, pj[3];
+        to_global(&pr->cp, n, fj, pj);
+        inten[jj] = skmo_nearest_index(&pr->fixed, n, pj, f) ? pr->fixed_data[to_index(&pr->fixed, n, f)] : 0.0;
+    }
+    const double cut = 3.0 * sigma_s;
+    for (long long jj = 0; jj < ncp; ++jj) {
+        int j[3] = {0, 0, 0};
+        long long r = jj;
+        for (int d = 0; d < n; ++d) { j[d] = (int)(r % pr->cp.size[d]); r /= pr->cp.size[d]; }
+        const double wj = pr->cpw[jj];
+        const double *cj = pr->cp_aos + jj * n;
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        for (int d = 0; d < n; ++d) {
+            lo[d] = imin(imax(trunc_int((double)j[d] - cut / pr->cp.spacing[d] + 1.0), 0), pr->cp.size[d] - 1);
+            hi[d] = imin(imax(trunc_int((double)j[d] + cut / pr->cp.spacing[d]), 0), pr->cp.size[d] - 1);
+        }
+        double value = 0.0, der[3] = {0, 0, 0};
+        int i[3] = {0, 0, 0};
+        int z0 = n == 3 ? lo[2] : 0, z1 = n == 3 ? hi[2] : 0;
+        for (i[2] = z0; i[2] <= z1; ++i[2])
+            for (i[1] = lo[1]; i[1] <= hi[1]; ++i[1])
+                for (i[0] = lo[0]; i[0] <= hi[0]; ++i[0]) {
+                    double r2 = 0.0;
+                    for (int d = 0; d < n; ++d) {
+                        double v = (double)(j[d] - i[d]) * pr->cp.spacing[d];
+                        r2 += v * v;
+                    }
+                    if (!(r2 < cut * cut)) continue;
+                    long long ii = to_index(&pr->cp, n, i);
+                    double di = inten[jj] - inten[ii];
+                    double b = exp(-r2 / (2.0 * sigma_s * sigma_s)) * exp(-di * di / (2.0 * sigma_i * sigma_i));
+                    const double *ci = pr->cp_aos + ii * n;
+                    double l2 = 0.0;
+                    for (int d = 0; d < n; ++d) l2 += (cj[d] - ci[d]) * (cj[d] - ci[d]);
+                    value += wj * b * l2;
+                    for (int d = 0; d < n; ++d) der[d] += 2.0 * b * (wj + pr->cpw[ii]) * (cj[d] - ci[d]);
+                }
+        if (values) values[jj] = value;
+        if (deriv_soa)
+            for (int d = 0; d < n; ++d) deriv_soa[jj + d * ncp] += weight * der[d];
+    }
+    free(inten);
+}
+
 /* ------------------------------------------------------------------ */
 /* a9: LCC helpers (include/cuda/GpuEvaluatorLCC.cu)                    */
 /* ------------------------------------------------------------------ */

@@ -87,6 +87,9 @@ int skmo_evaluate(const skmo_problem *pr, int measure, int with_derivative,
 /* a10 regularizers alone: per-cp values (optional) and derivative increment */
 void skmo_regularizer(const skmo_problem *pr, int which, double weight, double pg_scaling,
                       double *values, double *deriv_soa);
+/* bilateral regularizer (no reference counterpart; definition in skm_oracle.c) */
+void skmo_regularizer_bilateral(const skmo_problem *pr, double weight, double sigma_s, double sigma_i,
+                                double *values, double *deriv_soa);
 
 /* a11 host regularizers and assembly */
 double skmo_host_regularizers(int dim, long long n_cp, const double *params_soa,

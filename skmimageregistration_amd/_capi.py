@@ -48,7 +48,8 @@ class SkmDesc(ctypes.Structure):
 class SkmResult(ctypes.Structure):
     _fields_ = [("measure", ctypes.c_double), ("reg_value_rkhs", ctypes.c_double),
                 ("reg_value_rd", ctypes.c_double), ("reg_value_pg", ctypes.c_double),
-                ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("reg_value_bilateral", ctypes.c_double)]
 
 
 class SkmMetricOptions(ctypes.Structure):
@@ -97,7 +98,9 @@ class SkmScaleParams(ctypes.Structure):
                 ("reg_rkhs", ctypes.c_double), ("reg_rd", ctypes.c_double), ("reg_rd_scaling", ctypes.c_double),
                 ("reg_pg", ctypes.c_double), ("reg_pg_scaling", ctypes.c_double), ("iterations", ctypes.c_int32),
                 ("initial_step_size", ctypes.c_double), ("sampling_rate", ctypes.c_int32),
-                ("neighborhood_sampling", ctypes.c_int32), ("smoothing_variance", ctypes.c_double)]
+                ("neighborhood_sampling", ctypes.c_int32), ("smoothing_variance", ctypes.c_double),
+                ("reg_bilateral", ctypes.c_double), ("reg_bilateral_spatial_sigma", ctypes.c_double),
+                ("reg_bilateral_intensity_sigma", ctypes.c_double)]
 
 
 class SkmRegistrationDesc(ctypes.Structure):
@@ -137,6 +140,8 @@ _SIGNATURES = {
     "skm_set_regularizer_rkhs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "skm_set_regularizer_rd": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "skm_set_regularizer_pg": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "skm_set_regularizer_bilateral": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                                     ctypes.c_double]),
     "skm_set_displacement_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmGrid), c_float_p]),
     "skm_get_value": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_float_p, ctypes.c_int32,
                                      ctypes.POINTER(SkmResult)]),

@@ -314,8 +314,10 @@ struct skm_evaluator {
     std::vector<hipEvent_t> event_pool;
     double pass_ms[SKM_NUM_PASSES] = {0};
     uint32_t pass_launches[SKM_NUM_PASSES] = {0};
-    double reg_w[3] = {0, 0, 0};
+    double reg_w[4] = {0, 0, 0, 0};        // RKHS, RD, PG, bilateral
     double rd_scaling = 1.0, pg_scaling = 1.0;
+    double bl_sigma_s = 1.0, bl_sigma_i = 1.0;
+    float *d_cp_intensity = nullptr;       // fixed image at the control points (bilateral term)
 
     ~skm_evaluator()
     {
@@ -327,7 +329,7 @@ struct skm_evaluator {
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng,
-                        d_lcc_ref_partials, d_lcc_bound, d_gamma_fx};
+                        d_lcc_ref_partials, d_lcc_bound, d_gamma_fx, d_cp_intensity};
         for (void *p : ptrs)
             if (p) device_release(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -728,9 +730,11 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
     } else {
         return fail(SKM_ERROR_INVALID_ARGUMENT, "unknown measure");
     }
-    if (ev->reg_w[0] > 0 || ev->reg_w[1] > 0 || ev->reg_w[2] > 0) {
+    if (ev->reg_w[0] > 0 || ev->reg_w[1] > 0 || ev->reg_w[2] > 0 || ev->reg_w[3] > 0) {
         PassTimer t(ev, 5);
-        launch_regularizers(pr, ev->reg_w, ev->pg_scaling, deriv != nullptr, deriv, ev->d_reg_partials, ev->d_sc, s);
+        const BilateralArgs bl = make_bilateral_args(pr, ev->reg_w[3], ev->bl_sigma_s, ev->bl_sigma_i);
+        launch_regularizers(pr, ev->reg_w, ev->pg_scaling, bl, deriv != nullptr, deriv, ev->d_reg_partials, ev->d_sc,
+                            s);
     }
     SKM_HIP(hipGetLastError());
     return SKM_SUCCESS;
@@ -758,6 +762,7 @@ skm_status read_result(skm_evaluator *ev, skm_result *out)
         out->reg_value_pg = sc.reg[2];
         out->num_pixels_counted = (uint32_t)sc.count;
         out->reserved = 0;
+        out->reg_value_bilateral = sc.reg[3];
     }
     return SKM_SUCCESS;
 }
@@ -995,6 +1000,50 @@ SKM_API skm_status skm_set_regularizer_pg(skm_evaluator *ev, double weight, doub
     if (!(scaling > 0.0)) return fail(SKM_ERROR_INVALID_ARGUMENT, "scaling of regularizer must be strictly positive");
     ev->reg_w[2] = weight;
     ev->pg_scaling = scaling;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_set_regularizer_bilateral(skm_evaluator *ev, double weight, double spatial_sigma,
+                                                double intensity_sigma)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null evaluator");
+    if (!(spatial_sigma > 0.0) || !(intensity_sigma > 0.0))
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "bilateral sigmas must be strictly positive");
+    if (weight > 0 && !ev->d_cp_intensity) {
+        // the fixed image at every control point's nearest voxel (ImageNearest, 0 outside)
+        const int dim = ev->dim;
+        const skm_grid &cg = ev->desc.cp, &fg = ev->desc.fixed;
+        const long long ncp = ev->pr.n_cp;
+        std::vector<long long> idx((size_t)ncp);
+        for (long long j = 0; j < ncp; ++j) {
+            long long r = j, f = 0, stride = 1;
+            bool in = true;
+            for (int d = 0; d < dim; ++d) {
+                const int jc = (int)(r % cg.size[d]);
+                r /= cg.size[d];
+                const double p = cg.origin[d] + cg.spacing[d] * jc;
+                const int i = (int)((1.0 / fg.spacing[d]) * (p - fg.origin[d]) + 0.5);
+                in = in && i >= 0 && i < fg.size[d];
+                f += stride * i;
+                stride *= fg.size[d];
+            }
+            idx[(size_t)j] = in ? f : -1;
+        }
+        long long *d_idx = nullptr;
+        skm_status st = dev_upload(&d_idx, idx.data(), idx.size(), ev->stream);
+        if (st == SKM_SUCCESS) st = dev_alloc(&ev->d_cp_intensity, (size_t)ncp);
+        if (st == SKM_SUCCESS) {
+            launch_gather_values(ev->d_fixed, d_idx, 1, ncp, ev->d_cp_intensity, ev->stream);
+            const hipError_t e = hipStreamSynchronize(ev->stream);
+            if (e != hipSuccess) st = fail(SKM_ERROR_HIP, std::string("intensity gather: ") + hipGetErrorString(e));
+        }
+        if (d_idx) device_release(d_idx);
+        if (st != SKM_SUCCESS) return st;
+        ev->pr.cp_intensity = ev->d_cp_intensity;
+    }
+    ev->reg_w[3] = weight;
+    ev->bl_sigma_s = spatial_sigma;
+    ev->bl_sigma_i = intensity_sigma;
     return SKM_SUCCESS;
 }
 

@@ -49,6 +49,7 @@ struct Problem {
     float4 *smp_pstate;        // per sample of the current evaluation, written by the value pass
     const float *cp_reach;     // per control point: pair_reach2(smin, lam, 0), its share of the pair reach
     const float *cp_row_lam;   // largest lam per row of control points (cy + n[1] cz)
+    const float *cp_intensity; // fixed image at each control point (nearest neighbour), bilateral term
     const float *smp_row_lam;  // largest lam per row of sample cells (my + nsub[1] mz)
     double *gather_partials;   // weighted-mode gather: FP64 sums per (slab, control-point tile)
     int lcc_chunks;            // reference-window LCC derivative: workgroups per cp tile (0: automatic)
@@ -65,7 +66,7 @@ struct Scalars {
     double measure;
     double mean_f, mean_m, rho, inv_denom;  // NCC derivative coefficients
     double coef_scale;     // MSE: 2/N, LCC: 1/N
-    double reg[3];         // RKHS, RD, PG values
+    double reg[4];         // RKHS, RD, PG, bilateral values
     double fx_unit;        // exact LCC scatter: value of one fixed-point step
     double fx_inv_unit;    // 1 / fx_unit
 };
@@ -87,8 +88,20 @@ void launch_prepare(const Problem &pr, int measure, const float4 *sg, const floa
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s);
 // doubles of Problem::gather_partials for a sample grid with `planes` cells along the last axis
 long long gather_partials_size(const Problem &pr, long long planes);
-void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, int with_derivative,
-                         float *deriv_soa, double *partials, Scalars *sc, hipStream_t s);
+// The bilateral term (no reference counterpart): spatial Gaussian of standard
+// deviation sigma_s cut at 3 sigma_s, intensity Gaussian of standard deviation
+// sigma_i of the fixed image's values at the two control points.
+struct BilateralArgs {
+    double weight;
+    float cut2;        // (3 sigma_s)^2
+    float inv2ss;      // 1 / (2 sigma_s^2)
+    float inv2si;      // 1 / (2 sigma_i^2)
+    float rad[3];      // 3 sigma_s / spacing per axis (window in control-point indices)
+};
+BilateralArgs make_bilateral_args(const Problem &pr, double weight, double sigma_s, double sigma_i);
+// weights: RKHS, RD, PG; bl: the bilateral term (weight 0: off)
+void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, const BilateralArgs &bl,
+                         int with_derivative, float *deriv_soa, double *partials, Scalars *sc, hipStream_t s);
 unsigned reg_blocks(long long n_cp);  // workgroups (and 4-double partial slots) of launch_regularizers
 void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
 // packed states of the weighted modes at every voxel of the dense fixed grid

@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(int measure, const double *
         sc->measure = acc[0] / n;
         sc->coef_scale = 1.0 / n;
     }
-    sc->reg[0] = sc->reg[1] = sc->reg[2] = 0.0;
+    sc->reg[0] = sc->reg[1] = sc->reg[2] = sc->reg[3] = 0.0;
     // past the offsets this evaluation drew (k_offsets has finished: the
     // value pass ran in between)
     if (rng_delta) *rng = lcg_advance(*rng, rng_delta, rng_inc);
@@ -689,9 +689,19 @@ __global__ __launch_bounds__(kThreads) void k_gather_finish(Problem pr, int n_ti
 // sums; derivatives are weighted and added to the normalised metric
 // derivative (GpuEvaluator.cu:652-711).  One wave per control point, lanes
 // over the (x, y) plane of its window, z looped; wave sums in FP64.
+//
+// The bilateral term (no reference counterpart; the reference's closest term
+// is RD above, and "radiometric" appears only in a comment,
+// itkRegularizedImageToImageMetricv4.h:204): per control point j over the
+// points i within 3 sigma_s,
+//   B_ij = exp(-r_ij^2 / 2 sigma_s^2) exp(-(I_j - I_i)^2 / 2 sigma_i^2),
+//   value_j = sum_i w_j B_ij |c_j - c_i|^2,
+//   d/dc_j = sum_i 2 B_ij (w_j + w_i)(c_j - c_i)
+// with r_ij from the index difference times the spacing and I the fixed
+// image at the control points; its value goes to the fourth partial slot.
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_regularizers(Problem pr, double w_rkhs, double w_rd, double w_pg,
-                                                           double pg_scale, int with_derivative,
+                                                           double pg_scale, BilateralArgs bl, int with_derivative,
                                                            float *__restrict__ deriv, double *__restrict__ partials)
 {
     __shared__ double lds[4 * 4];
@@ -715,10 +725,10 @@ __global__ __launch_bounds__(kThreads) void k_regularizers(Problem pr, double w_
         float wj = cj4.w;
         const int *win = pr.reg_window + j * 6;
         float s = (float)pg_scale;
-        float vr = 0.0f, vd = 0.0f, vp = 0.0f;
-        float dr[3] = {0, 0, 0}, dd[3] = {0, 0, 0}, dp[3] = {0, 0, 0};
+        float vr = 0.0f, vd = 0.0f, vp = 0.0f, vb = 0.0f;
+        float dr[3] = {0, 0, 0}, dd[3] = {0, 0, 0}, dp[3] = {0, 0, 0}, db[3] = {0, 0, 0};
         int z0 = DIM == 3 ? win[2] : 0, z1 = DIM == 3 ? win[5] : 0;
-        const int wx = win[3] - win[0] + 1, plane = wx * (win[4] - win[1] + 1);
+        const int wx = win[3] - win[0] + 1, plane = (w_rkhs > 0 || w_rd > 0 || w_pg > 0) ? wx * (win[4] - win[1] + 1) : 0;
         for (int t = lane; t < plane; t += 64)
             for (int iz = z0; iz <= z1; ++iz) {
                     const int ix = win[0] + t % wx, iy = win[1] + t / wx;
@@ -770,12 +780,41 @@ __global__ __launch_bounds__(kThreads) void k_regularizers(Problem pr, double w_
                         }
                     }
                 }
-        vals[0] = vr; vals[1] = vd; vals[2] = vp;
+        if (bl.weight > 0) {
+            int blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                blo[d] = clampi((int)((float)jc[d] - bl.rad[d] + 1.0f), 0, pr.cp.n[d] - 1);
+                bhi[d] = clampi((int)((float)jc[d] + bl.rad[d]), 0, pr.cp.n[d] - 1);
+            }
+            const float ij = pr.cp_intensity[j];
+            const int bx = bhi[0] - blo[0] + 1, bplane = bx * (bhi[1] - blo[1] + 1);
+            const int bz0 = DIM == 3 ? blo[2] : 0, bz1 = DIM == 3 ? bhi[2] : 0;
+            for (int t = lane; t < bplane; t += 64)
+                for (int iz = bz0; iz <= bz1; ++iz) {
+                    const int ix = blo[0] + t % bx, iy = blo[1] + t / bx;
+                    const float v0 = (float)(jc[0] - ix) * pr.cp.h[0], v1 = (float)(jc[1] - iy) * pr.cp.h[1];
+                    const float v2 = DIM == 3 ? (float)(jc[2] - iz) * pr.cp.h[2] : 0.0f;
+                    const float r2 = v0 * v0 + v1 * v1 + v2 * v2;
+                    if (!(r2 < bl.cut2)) continue;
+                    const long long i = ix + (long long)pr.cp.n[0] * (iy + (long long)pr.cp.n[1] * iz);
+                    const float di = ij - pr.cp_intensity[i];
+                    const float b = expf(-r2 * bl.inv2ss) * expf(-di * di * bl.inv2si);
+                    const float4 ci4 = pr.cpc[i];
+                    const float e[3] = {cj[0] - ci4.x, cj[1] - ci4.y, DIM == 3 ? cj[2] - ci4.z : 0.0f};
+                    vb = fmaf(wj * b, e[0] * e[0] + e[1] * e[1] + e[2] * e[2], vb);
+                    const float f = 2.0f * b * (wj + ci4.w);
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) db[d] = fmaf(f, e[d], db[d]);
+                }
+        }
+        vals[0] = vr; vals[1] = vd; vals[2] = vp; vals[3] = vb;
         if (with_derivative) {
             double add[3];
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
-                add[d] = wave_sum(w_rkhs * (double)dr[d] + w_rd * (double)dd[d] + w_pg * (double)dp[d]);
+                add[d] = wave_sum(w_rkhs * (double)dr[d] + w_rd * (double)dd[d] + w_pg * (double)dp[d] +
+                                  bl.weight * (double)db[d]);
             if (lane == 0)
 #pragma unroll
                 for (int d = 0; d < DIM; ++d)
@@ -796,7 +835,7 @@ __global__ __launch_bounds__(kThreads) void k_reg_reduce(const double *__restric
         for (int k = 0; k < 4; ++k) acc[k] += partials[b * 4LL + k];
     block_sum<4>(acc, lds);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 3; ++k) sc->reg[k] = acc[k];
+        for (int k = 0; k < 4; ++k) sc->reg[k] = acc[k];
 }
 
 // ------------------------------------------------------------ displacement
@@ -885,6 +924,7 @@ __global__ void k_result(const Scalars *__restrict__ sc, skm_result *__restrict_
     out->reg_value_pg = sc->reg[2];
     out->num_pixels_counted = (uint32_t)sc->count;
     out->reserved = 0;
+    out->reg_value_bilateral = sc->reg[3];
 }
 
 // ------------------------------------------------------------ launchers
@@ -1022,20 +1062,32 @@ void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStr
 }
 
 template <int DIM, int MODE>
-static void reg_impl(const Problem &pr, const double *w, double pg, int with_derivative, float *deriv,
-                     double *partials, hipStream_t s)
+static void reg_impl(const Problem &pr, const double *w, double pg, const BilateralArgs &bl, int with_derivative,
+                     float *deriv, double *partials, hipStream_t s)
 {
-    k_regularizers<DIM, MODE><<<reg_blocks(pr.n_cp), kThreads, 0, s>>>(pr, w[0], w[1], w[2], pg, with_derivative,
-                                                                       deriv, partials);
+    k_regularizers<DIM, MODE><<<reg_blocks(pr.n_cp), kThreads, 0, s>>>(pr, w[0], w[1], w[2], pg, bl,
+                                                                       with_derivative, deriv, partials);
+}
+
+BilateralArgs make_bilateral_args(const Problem &pr, double weight, double sigma_s, double sigma_i)
+{
+    BilateralArgs a{};
+    a.weight = weight;
+    const double cut = 3.0 * sigma_s;
+    a.cut2 = (float)(cut * cut);
+    a.inv2ss = (float)(1.0 / (2.0 * sigma_s * sigma_s));
+    a.inv2si = (float)(1.0 / (2.0 * sigma_i * sigma_i));
+    for (int d = 0; d < 3; ++d) a.rad[d] = d < pr.dim ? (float)(cut / (double)pr.cp.h[d]) : 0.0f;
+    return a;
 }
 
 // one wave per control point: 4 control points per workgroup
 unsigned reg_blocks(long long n_cp) { return (unsigned)((n_cp + kThreads / 64 - 1) / (kThreads / 64)); }
 
-void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, int with_derivative,
-                         float *deriv_soa, double *partials, Scalars *sc, hipStream_t s)
+void launch_regularizers(const Problem &pr, const double *weights, double pg_scaling, const BilateralArgs &bl,
+                         int with_derivative, float *deriv_soa, double *partials, Scalars *sc, hipStream_t s)
 {
-    SKM_DISPATCH_DIM_MODE(pr, reg_impl, pr, weights, pg_scaling, with_derivative, deriv_soa, partials, s);
+    SKM_DISPATCH_DIM_MODE(pr, reg_impl, pr, weights, pg_scaling, bl, with_derivative, deriv_soa, partials, s);
     k_reg_reduce<<<1, kThreads, 0, s>>>(partials, (int)reg_blocks(pr.n_cp), sc);
 }
 
@@ -1131,7 +1183,7 @@ struct StepParams {
     int kind;
     double a, A, alpha, fmax, fmin, omega;
     double l1w, l21w, l2w;
-    double rw[3];
+    double rw[4];
     int reg_blocks;
 };
 
@@ -1169,7 +1221,7 @@ __global__ __launch_bounds__(64) void k_step_scalars(StepParams sp, const double
     if (sp.l1w > 0) reg += sp.l1w * l1;
     if (sp.l21w > 0) reg += sp.l21w * l21;
     if (sp.l2w > 0) reg += sp.l2w * l2;
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < 4; ++k)
         if (sp.rw[k] > 0) reg += sp.rw[k] * sc->reg[k];
     const double value = sc->measure + reg;
     st->improved = st->iteration == 0 || value < st->best_value;
@@ -1220,7 +1272,8 @@ void launch_opt_step(const Problem &pr, int kind, double a, double A, double alp
                      float4 *cpc, hipStream_t s)
 {
     StepParams sp{kind, a, A, alpha, fmax, fmin, omega, l1w, l21w, l2w,
-                  {gpu_reg_weights[0], gpu_reg_weights[1], gpu_reg_weights[2]}, (int)blocks_for(pr.n_cp)};
+                  {gpu_reg_weights[0], gpu_reg_weights[1], gpu_reg_weights[2], gpu_reg_weights[3]},
+                  (int)blocks_for(pr.n_cp)};
     k_step_scalars<<<1, 64, 0, s>>>(sp, reg_partials, st, sc, (skm_iteration_record *)record);
     k_update<<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr.n_cp, pr.dim, params_soa, grad_soa, prev_grad, st, orthant,
                                                       best_params, cpw, cpc);

@@ -335,6 +335,9 @@ SKM_API void skm_registration_desc_init(skm_registration_desc *d)
         p.sigma_c0 = sc0[k];
         p.tensor_weight = tw[k];
         p.reg_rd_scaling = p.reg_pg_scaling = 1.0;
+        p.reg_bilateral = 0.0;
+        p.reg_bilateral_spatial_sigma = 12.0;
+        p.reg_bilateral_intensity_sigma = 0.1;
         p.iterations = 500;
         p.initial_step_size = 0.1;
         p.sampling_rate = sr[k];
@@ -538,15 +541,18 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         if (P.reg_rkhs > 0) SKM_R_TRY(skm_set_regularizer_rkhs(ev, P.reg_rkhs));
         if (P.reg_rd > 0) SKM_R_TRY(skm_set_regularizer_rd(ev, P.reg_rd, P.reg_rd_scaling));
         if (P.reg_pg > 0) SKM_R_TRY(skm_set_regularizer_pg(ev, P.reg_pg, P.reg_pg_scaling));
+        if (P.reg_bilateral > 0)
+            SKM_R_TRY(skm_set_regularizer_bilateral(ev, P.reg_bilateral, P.reg_bilateral_spatial_sigma,
+                                                    P.reg_bilateral_intensity_sigma));
         if (has_prev) SKM_R_TRY(skm::set_displacement_field_device(ev, &desc->target, d_prev.p));
         rep.ms_setup = ms_since(t_setup);
 
         // optimisation (SKMReg.cxx:375-430)
         const auto t_opt = Clock::now();
         const bool only_l1 = P.reg_l1 > 0 && P.reg_l21 <= 0 && P.reg_l2 <= 0 && P.reg_rkhs <= 0 && P.reg_rd <= 0 &&
-                             P.reg_pg <= 0;
+                             P.reg_pg <= 0 && P.reg_bilateral <= 0;
         const bool only_l21 = P.reg_l1 <= 0 && P.reg_l21 > 0 && P.reg_l2 <= 0 && P.reg_rkhs <= 0 && P.reg_rd <= 0 &&
-                              P.reg_pg <= 0;
+                              P.reg_pg <= 0 && P.reg_bilateral <= 0;
         skm_metric_options mo{desc->metric, P.reg_l1, P.reg_l21, P.reg_l2, 1, 1};
         skm_optimizer_options oo{SKM_OPT_ADAPTIVE, P.initial_step_size, 20.0, 1.0, 1e7, -1e6, 1.0,
                                  (only_l1 || only_l21) ? 1 : 0, P.iterations, 1, 0};

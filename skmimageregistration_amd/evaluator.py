@@ -92,6 +92,7 @@ class EvaluationResult:
         self.reg_value_rkhs = r.reg_value_rkhs
         self.reg_value_rd = r.reg_value_rd
         self.reg_value_pg = r.reg_value_pg
+        self.reg_value_bilateral = r.reg_value_bilateral
         self.numPixelsCountedGpu = r.num_pixels_counted
 
     def __repr__(self):
@@ -189,6 +190,11 @@ class GpuKernelEvaluator:
 
     def SetRegularizerPG(self, weight, scaling=1.0):
         check(self._lib.skm_set_regularizer_pg(self._h, float(weight), float(scaling)))
+
+    def SetRegularizerBilateral(self, weight, spatial_sigma=12.0, intensity_sigma=0.1):
+        """skm_set_regularizer_bilateral (no reference counterpart)."""
+        check(self._lib.skm_set_regularizer_bilateral(self._h, float(weight), float(spatial_sigma),
+                                                      float(intensity_sigma)))
 
     def SetDisplacementField(self, displacementField):
         """Previous-scale displacement (skm_set_displacement_field): an Image
@@ -435,7 +441,9 @@ class RegistrationConfig:
     """The SKMReg parameter set (include/ConfigUtils.h:29-62): per-scale lists
     whose last entry repeats (CURRENT, src/SKMReg.cxx:48-49).  One key has no
     SKMReg counterpart: exact_gradient (1: derivative windows covering the
-    whole kernel support at every level; 0, the default: the reference's)."""
+    whole kernel support at every level; 0, the default: the reference's),
+    and the bilateral regularizer reg_bilateral with its
+    reg_bilateral_spatial_sigma / reg_bilateral_intensity_sigma."""
 
     STRING_KEYS = ("metric", "reference_filename", "target_filename", "reference_landmarks", "target_landmarks",
                    "weight_filename", "weight_tensor_filename", "temp_directory")
@@ -451,6 +459,7 @@ class RegistrationConfig:
         "num_function_evaluations": [500, 500, 500], "initial_step_size": [0.1, 0.1, 0.1],
         "sampling_rate": [5, 4, 3], "neighborhood_sampling": [15, 12, 9], "intensity_thresholds": [-1500, 3500],
         "smoothing_variance": [2, 1, 0], "metric": "mse", "exact_gradient": [0],
+        "reg_bilateral": [0, 0, 0], "reg_bilateral_spatial_sigma": [12.0], "reg_bilateral_intensity_sigma": [0.1],
     }
 
     def __init__(self, **overrides):
@@ -503,7 +512,8 @@ def _registration_desc(lib, reference, target, config, weight_image, weight_tens
         p.sigma_c0 = cfg.CURRENT("sigma_C0", s)
         p.tensor_weight = cfg.CURRENT("tensor_weights", s)
         for key in ("reg_l1", "reg_l21", "reg_l2", "reg_rkhs", "reg_rd", "reg_rd_scaling", "reg_pg",
-                    "reg_pg_scaling"):
+                    "reg_pg_scaling", "reg_bilateral", "reg_bilateral_spatial_sigma",
+                    "reg_bilateral_intensity_sigma"):
             setattr(p, key, float(cfg.CURRENT(key, s)))
         p.iterations = int(cfg.CURRENT("num_function_evaluations", s))
         p.initial_step_size = float(cfg.CURRENT("initial_step_size", s))

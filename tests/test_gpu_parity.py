@@ -148,3 +148,36 @@ def test_launch_shapes(tuning, mode):
     """Every launch shape skm_set_tuning can select gives the oracle's result."""
     case = Case(dim=3, shape=(18, 20, 22), cp_spacing=5.0, sigma=12.0, mode=mode, subsample=2, seed=43)
     _compare(case, MEASURE.NCC, tuning=tuning)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("mode", [0, 2])
+def test_bilateral_regularizer(dim, mode):
+    """The bilateral term (no reference counterpart) against the oracle's
+    restatement: its value and the derivative it adds to MSE's, with RD on
+    too, non-uniform control-point weights and spatial sigma 2 control-point
+    spacings (cut at 6 spacings, wider than the grid in places)."""
+    shape = (40, 44) if dim == 2 else (16, 18, 20)
+    case = Case(dim=dim, shape=shape, cp_spacing=4.0, sigma=10.0, mode=mode, seed=47, coef_scale=0.8)
+    ev = case.gpu()
+    ev.SetRegularizerRD(0.2)
+    ev.SetRegularizerBilateral(0.3, 8.0, 0.1)
+    res, der = ev.getValueAndDerivative(MEASURE.MSE, case.cp, case.cpw)
+    ev.close()
+    pr = case.oracle()
+    ores, oder = pr.evaluate(int(MEASURE.MSE), True, 0.0, 0.2, 0.0)
+    bval, bder = pr.bilateral(0.3, 8.0, 0.1)
+    assert bval > 0
+    assert scalar_rel(res.reg_value_bilateral, bval) < VALUE_TOL, (res.reg_value_bilateral, bval)
+    assert scalar_rel(res.reg_value_rd, ores["reg_rd"]) < VALUE_TOL
+    assert scalar_rel(res.measure, ores["measure"]) < VALUE_TOL
+    assert rel_err(der, oder + bder) < DERIV_TOL, rel_err(der, oder + bder)
+    # the device optimizer's value assembly adds the weighted bilateral value
+    ev = case.gpu()
+    ev.SetRegularizerRD(0.2)
+    ev.SetRegularizerBilateral(0.3, 8.0, 0.1)
+    soa = np.ascontiguousarray(case.cp.T).reshape(-1)
+    _, recs = ev.optimize(soa, metric=MEASURE.MSE, iterations=1, cpwData=case.cpw, use_negative_gradient=True)
+    ev.close()
+    expected = ores["measure"] + 0.2 * ores["reg_rd"] + 0.3 * bval
+    assert scalar_rel(recs[0][0], expected) < VALUE_TOL, (recs[0][0], expected)

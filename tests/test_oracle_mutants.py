@@ -44,13 +44,18 @@ MUTANTS = {
                              "        int c = (int)(x[d] + 0.5), sn = pr->subsample_neighborhood;\n"
                              "        lo[d] = c - ((c - lo[d]) / sn) * sn;\n"
                              "    }\n}")],
+    # bilateral: intensity Gaussian without the factor 1/2
+    "bilateral_intensity_half": [("exp(-di * di / (2.0 * sigma_i * sigma_i))", "exp(-di * di / (sigma_i * sigma_i))")],
+    # bilateral: derivative with w_j only (assumes symmetric weights, as the reference's RD does)
+    "bilateral_weights": [("2.0 * b * (wj + pr->cpw[ii])", "4.0 * b * wj")],
     # LCC: cc whenever one variance is positive
     "lcc_one_variance": [("        if (d1 > 0 && d2 > 0) {\n            cc = s.sfm",
                           "        if (d1 > 0 || d2 > 0) {\n            cc = s.sfm")],
 }
 
 CHECKS = [pins.check_nonstationary_closed_forms, pins.check_anisotropic_closed_forms,
-          pins.check_symmetry_and_gram, pins.check_lcc_against_brute_force, pins.check_lcc_zero_variance]
+          pins.check_symmetry_and_gram, pins.check_lcc_against_brute_force, pins.check_lcc_zero_variance,
+          pins.check_bilateral_closed_forms, pins.check_bilateral_fd]
 
 
 def _build(tmp_path, name, subs):

@@ -194,3 +194,64 @@ def test_lcc_value_against_brute_force(case):
 
 def test_lcc_zero_moving_variance():
     check_lcc_zero_variance(_lib())
+
+
+# ------------------------------------------------------------ bilateral regularizer
+
+def _bilateral_problem(lib, dim, size, h, cp, cpw, fixed):
+    return orc.OracleProblem(fixed, fixed, (size, (0.0,) * dim, (h,) * dim), np.asarray(cp, float),
+                             cpw=np.asarray(cpw, float), sigma_c0=5.0, sigma_c4=5.0, lib=lib)
+
+
+def check_bilateral_closed_forms(lib):
+    """Two control points 5 apart with fixed-image values 0.3 and 0.55 at
+    their voxels: B = exp(-25 / (2 * 4^2)) exp(-0.25^2 / (2 * 0.2^2)) =
+    exp(-1.5625), value = (w_0 + w_1) B |c_0 - c_1|^2, derivative
+    +-2 B (w_0 + w_1)(c_0 - c_1); a pair beyond 3 sigma_s contributes nothing."""
+    fixed = np.zeros((4, 4, 8))
+    fixed[0, 0, 0], fixed[0, 0, 5] = 0.3, 0.55
+    c = np.array([[0.4, -1.0, 0.5], [-0.2, 0.3, 1.1]])
+    w = np.array([0.7, 1.6])
+    pr = _bilateral_problem(lib, 3, (2, 1, 1), 5.0, c, w, fixed)
+    val, der = pr.bilateral(2.0, 4.0, 0.2)
+    b = math.exp(-1.5625)
+    diff = c[0] - c[1]
+    assert abs(val - (w[0] + w[1]) * b * (diff @ diff)) < 1e-14, val
+    g0 = 2.0 * b * (w[0] + w[1]) * diff * 2.0  # times the weight 2
+    np.testing.assert_allclose(der.reshape(3, 2).T, np.stack([g0, -g0]), rtol=0, atol=1e-14)
+    # 13 > 3 sigma_s = 12: outside the cut
+    fixed2 = np.zeros((4, 4, 16))
+    pr = _bilateral_problem(lib, 3, (2, 1, 1), 13.0, c, w, fixed2)
+    val, der = pr.bilateral(1.0, 4.0, 0.2)
+    assert val == 0.0 and not der.any()
+
+
+def check_bilateral_fd(lib):
+    """The derivative is the exact gradient of the summed value for
+    non-uniform control-point weights (B is symmetric)."""
+    rng = np.random.default_rng(3)
+    fixed = rng.uniform(0, 1, size=(12, 13, 14))
+    size, h = (5, 4, 4), 3.0
+    n = int(np.prod(size))
+    c = rng.normal(size=(n, 3))
+    w = rng.uniform(0.5, 1.5, size=n)
+    pr = _bilateral_problem(lib, 3, size, h, c, w, fixed)
+    _, der = pr.bilateral(1.0, 2.5, 0.3)
+    eps = 1e-6
+    for idx in rng.choice(n * 3, 8, replace=False):
+        j, d = idx % n, idx // n
+        vals = []
+        for sgn in (1, -1):
+            cc = c.copy()
+            cc[j, d] += sgn * eps
+            vals.append(_bilateral_problem(lib, 3, size, h, cc, w, fixed).bilateral(1.0, 2.5, 0.3, False)[0])
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        assert abs(fd - der[idx]) < 1e-7 * max(1.0, abs(fd)), (idx, fd, der[idx])
+
+
+def test_bilateral_closed_forms():
+    check_bilateral_closed_forms(_lib())
+
+
+def test_bilateral_derivative_fd():
+    check_bilateral_fd(_lib())
