@@ -490,6 +490,51 @@ SKM_API skm_status skm_register_batch(const skm_registration_desc *descs, int32_
                                       float *const *displacements_out, skm_scale_report *const *reports,
                                       skm_status *statuses);
 
+/* ---- learned warm start ------------------------------------------------- */
+
+/* A 3D convolutional encoder-decoder that predicts a registration's initial
+ * control-point coefficients from the downsampled image pair, trained and
+ * run by the library's own kernels (no counterpart in the reference, which
+ * starts every registration from zero coefficients, src/SKMReg.cxx:375-430).
+ * Levels of 16 / 32 / 48 / 64 channels, 3x3x3 convolutions with BF16 weights
+ * and FP32 accumulation on the matrix cores, a decoder back to half the input
+ * resolution, trilinear sampling at the control points and a linear head to
+ * D = 3 coefficients per control point.
+ *   input   [batch][z][y][x][2] floats (channel 0 moving, 1 fixed), device
+ *   output  [batch][cp z][cp y][cp x][3] floats, device
+ * Control points are given in input voxel units (index i of the input is
+ * coordinate i).  Parameters: FP32 master weights, host layout as
+ * skm_ws_get_params returns them. */
+typedef struct skm_ws_desc {
+    int32_t in_size[3];      /* x, y, z of the network input; multiples of 8 */
+    int32_t cp_size[3];      /* control-point grid, x, y, z                   */
+    double cp_origin[3];     /* control point 0 in input voxel coordinates    */
+    double cp_spacing[3];    /* control-point spacing in input voxels (>= 2)  */
+    int32_t max_batch;
+} skm_ws_desc;
+typedef struct skm_ws_net skm_ws_net;
+
+SKM_API skm_status skm_ws_create(const skm_ws_desc *desc, skm_ws_net **out);
+SKM_API void skm_ws_destroy(skm_ws_net *net);
+SKM_API int64_t skm_ws_num_params(const skm_ws_net *net);
+SKM_API skm_status skm_ws_set_stream(skm_ws_net *net, void *hip_stream);
+/* host FP32 parameters in and out; set also resets the Adam moments */
+SKM_API skm_status skm_ws_set_params(skm_ws_net *net, const float *params);
+SKM_API skm_status skm_ws_get_params(skm_ws_net *net, float *params);
+/* inference on device arrays, enqueued on the network's stream */
+SKM_API skm_status skm_ws_forward(skm_ws_net *net, const float *d_input, int32_t batch, float *d_out);
+/* gradients of every parameter for the output gradient d_dout of the last
+ * forward; d_grad (device, may be NULL) receives them in the parameter layout */
+SKM_API skm_status skm_ws_backward(skm_ws_net *net, const float *d_dout, float *d_grad);
+/* output of conv `layer` (0..9, after its ReLU) of the last forward,
+ * [batch][z][y][x][channels] at that layer's resolution, copied to d_dst */
+SKM_API skm_status skm_ws_get_activation(skm_ws_net *net, int32_t layer, float *d_dst);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error
+ * between the output and d_target; *loss (may be NULL) receives the loss of
+ * the step's forward pass (the call then waits for the stream) */
+SKM_API skm_status skm_ws_train_step(skm_ws_net *net, const float *d_input, const float *d_target, int32_t batch,
+                                     double learning_rate, double *loss);
+
 #ifdef __cplusplus
 }
 #endif

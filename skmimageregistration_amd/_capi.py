@@ -79,6 +79,11 @@ class SkmTuning(ctypes.Structure):
                 ("value_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
+class SkmWsDesc(ctypes.Structure):
+    _fields_ = [("in_size", ctypes.c_int32 * 3), ("cp_size", ctypes.c_int32 * 3), ("cp_origin", ctypes.c_double * 3),
+                ("cp_spacing", ctypes.c_double * 3), ("max_batch", ctypes.c_int32)]
+
+
 class SkmIterationRecord(ctypes.Structure):
     _fields_ = [("value", ctypes.c_double), ("measure", ctypes.c_double), ("learning_rate", ctypes.c_double),
                 ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
@@ -190,6 +195,17 @@ _SIGNATURES = {
     "skm_host_regularizers": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, c_double_p, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_int32, c_double_p,
                                              c_double_p, c_double_p, c_double_p, c_double_p]),
+    "skm_ws_create": (ctypes.c_int, [ctypes.POINTER(SkmWsDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "skm_ws_destroy": (None, [ctypes.c_void_p]),
+    "skm_ws_num_params": (ctypes.c_int64, [ctypes.c_void_p]),
+    "skm_ws_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "skm_ws_set_params": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "skm_ws_get_params": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "skm_ws_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "skm_ws_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "skm_ws_get_activation": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "skm_ws_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGNATURES)
