@@ -179,6 +179,22 @@ SKM_API skm_status skm_enable_timing(skm_evaluator *ev, int32_t enable);
 /* Synchronises the stream, returns the totals since the last read and resets them. */
 SKM_API skm_status skm_read_timing(skm_evaluator *ev, skm_pass_times *out);
 
+/* Least-squares coefficients of a dense displacement in the evaluator's
+ * kernel basis (no counterpart in the reference): CGLS with `iterations`
+ * steps for min_c |K c - u|^2 + ridge |c|^2, where K c is the displacement of
+ * skm_evaluate_displacement_field with control-point weights 1 and no
+ * previous field.  d_displacement_aos: D floats per fixed voxel; d_coefficients_soa:
+ * the control points' coefficients in the derivative's SoA layout.  Device
+ * arrays; returns when done.  Stationary kernels only (SKM_ERROR_NOT_SUPPORTED
+ * otherwise): there the transpose, a gather over every voxel in a control
+ * point's support, is exactly the adjoint of K, whose per-voxel windows hold
+ * the whole support; the weighted modes' windows (the reference's support
+ * estimate of the voxel) can truncate a pair that the transpose would keep.
+ * Used to turn a registration's final field into the coefficients the warm
+ * start learns. */
+SKM_API skm_status skm_fit_coefficients_device(skm_evaluator *ev, const float *d_displacement_aos, int32_t iterations,
+                                               double ridge, float *d_coefficients_soa);
+
 /* Launch shapes of an evaluator's kernels (no counterpart in the reference,
  * whose launch shapes are fixed, GpuEvaluator.cu:157-171, 424-438, 578-586).
  * 0 in a field selects the library's own choice; results do not depend on
@@ -276,6 +292,31 @@ typedef enum skm_interpolation {
 SKM_API skm_status skm_warp_image(int32_t dim, const skm_grid *image_grid, const float *image,
                                   const skm_grid *df_grid, const float *df_aos, const skm_grid *out_grid,
                                   int32_t interpolation, double default_value, float *out);
+
+/* Synthetic 3D registration pair on the GPU (no counterpart in the
+ * reference; inputs for tests, benchmarks and the warm start's training):
+ * reference = n_ellipsoids random ellipsoids (centres U[0.15, 0.85] of the
+ * extent, semi-axes U[0.05, 0.2] of it, intensities U[0, 1], later ones over
+ * earlier ones) smoothed by a Gaussian of standard deviation smoothing_sigma;
+ * d = sum of n_rbf Gaussian RBFs (standard deviation rbf_sigma, centres
+ * uniform in the volume, weights U[-1, 1]^3) scaled so that max |d| =
+ * max_displacement, times displacement_scale; target(x) = reference(x + d(x))
+ * (cubic B-spline, 0 outside) + N(0, noise_sigma^2).  The random stream is
+ * fixed by seed.  Host outputs: reference and target (x fastest), d AoS (may
+ * be NULL). */
+typedef struct skm_synthetic_desc {
+    int32_t size[3];
+    int32_t n_ellipsoids;        /* <= 64 */
+    double smoothing_sigma;
+    int32_t n_rbf;               /* 1..32 */
+    double rbf_sigma;
+    double max_displacement;
+    double displacement_scale;
+    double noise_sigma;
+    uint64_t seed;
+} skm_synthetic_desc;
+SKM_API skm_status skm_synthetic_pair(const skm_synthetic_desc *desc, float *reference, float *target,
+                                      float *displacement_aos);
 
 /* ---- image files and landmarks ------------------------------------------ */
 
@@ -445,6 +486,18 @@ typedef struct skm_registration_desc {
                                                           SKM_FLAG_EXACT_GRADIENT (no SKMReg
                                                           counterpart); 0: the reference's
                                                           derivative windows        */
+    /* warm start (no SKMReg counterpart) */
+    const float *initial_coefficients;                 /* start point of the first level run,
+                                                          SoA on its control-point grid, or
+                                                          NULL for zeros (SKMReg's)  */
+    int32_t initial_guard;                             /* 1: start from zeros instead when the
+                                                          given coefficients (or displacement)
+                                                          evaluate worse than zeros (none) */
+    const float *initial_displacement;                 /* previous displacement of the first
+                                                          level run, AoS on the target grid
+                                                          (as restart_displacement), or NULL;
+                                                          with initial_guard dropped when it
+                                                          evaluates worse than none  */
 } skm_registration_desc;
 
 /* What one level did. */
@@ -458,6 +511,11 @@ typedef struct skm_scale_report {
     double ms_setup, ms_optimize, ms_displacement;
     double tre;                      /* CalculateTRE of df-full, -1 without landmarks */
     double zero_parameters;          /* percentage of parameters equal to 0          */
+    int32_t initial_used;            /* 1: the level started from initial_coefficients
+                                        or initial_displacement                      */
+    int32_t reserved;
+    double initial_value, zero_value;/* with initial_guard: value at the given start
+                                        and at zeros                                 */
 } skm_scale_report;
 
 /* ConfigUtils.h:29-62 defaults (3 levels, metric MSE, sigma_C4 64/32/16, ...). */
@@ -526,6 +584,10 @@ SKM_API skm_status skm_ws_forward(skm_ws_net *net, const float *d_input, int32_t
 /* gradients of every parameter for the output gradient d_dout of the last
  * forward; d_grad (device, may be NULL) receives them in the parameter layout */
 SKM_API skm_status skm_ws_backward(skm_ws_net *net, const float *d_dout, float *d_grad);
+/* sample `index` of a network input from a moving and a fixed image of
+ * twice the input size (device, x fastest): their 2 x 2 x 2 block means */
+SKM_API skm_status skm_ws_prepare_input(skm_ws_net *net, const float *d_moving, const float *d_fixed, int32_t index,
+                                        float *d_input);
 /* output of conv `layer` (0..9, after its ReLU) of the last forward,
  * [batch][z][y][x][channels] at that layer's resolution, copied to d_dst */
 SKM_API skm_status skm_ws_get_activation(skm_ws_net *net, int32_t layer, float *d_dst);

@@ -84,6 +84,12 @@ class SkmWsDesc(ctypes.Structure):
                 ("cp_spacing", ctypes.c_double * 3), ("max_batch", ctypes.c_int32)]
 
 
+class SkmSyntheticDesc(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int32 * 3), ("n_ellipsoids", ctypes.c_int32), ("smoothing_sigma", ctypes.c_double),
+                ("n_rbf", ctypes.c_int32), ("rbf_sigma", ctypes.c_double), ("max_displacement", ctypes.c_double),
+                ("displacement_scale", ctypes.c_double), ("noise_sigma", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
 class SkmIterationRecord(ctypes.Structure):
     _fields_ = [("value", ctypes.c_double), ("measure", ctypes.c_double), ("learning_rate", ctypes.c_double),
                 ("num_pixels_counted", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
@@ -121,7 +127,8 @@ class SkmRegistrationDesc(ctypes.Structure):
                 ("restart_displacement", c_float_p), ("temp_directory", ctypes.c_char_p),
                 ("verbosity", ctypes.c_int32), ("num_landmarks", ctypes.c_int64),
                 ("reference_landmarks", c_double_p), ("target_landmarks", c_double_p),
-                ("exact_gradient", ctypes.c_int32)]
+                ("exact_gradient", ctypes.c_int32), ("initial_coefficients", c_float_p),
+                ("initial_guard", ctypes.c_int32), ("initial_displacement", c_float_p)]
 
 
 class SkmScaleReport(ctypes.Structure):
@@ -129,7 +136,8 @@ class SkmScaleReport(ctypes.Structure):
                 ("iterations", ctypes.c_int32), ("cp_size", ctypes.c_int32 * 3), ("maximum_tilt", ctypes.c_double),
                 ("ms_setup", ctypes.c_double), ("ms_optimize", ctypes.c_double),
                 ("ms_displacement", ctypes.c_double), ("tre", ctypes.c_double),
-                ("zero_parameters", ctypes.c_double)]
+                ("zero_parameters", ctypes.c_double), ("initial_used", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("initial_value", ctypes.c_double), ("zero_value", ctypes.c_double)]
 
 
 class SkmImageHeader(ctypes.Structure):
@@ -203,6 +211,11 @@ _SIGNATURES = {
     "skm_ws_get_params": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
     "skm_ws_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "skm_ws_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "skm_ws_prepare_input": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.c_void_p]),
+    "skm_fit_coefficients_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                                   ctypes.c_void_p]),
+    "skm_synthetic_pair": (ctypes.c_int, [ctypes.POINTER(SkmSyntheticDesc), c_float_p, c_float_p, c_float_p]),
     "skm_ws_get_activation": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "skm_ws_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                          ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),

@@ -767,7 +767,153 @@ skm_status read_result(skm_evaluator *ev, skm_result *out)
     return SKM_SUCCESS;
 }
 
+// ---- least-squares fit of a dense displacement onto the kernel basis
+
+__global__ __launch_bounds__(kThreads) void k_aos_to_float4(const float *__restrict__ u, int dim, long long n,
+                                                            float4 *__restrict__ out)
+{
+    const long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i >= n) return;
+    out[i] = make_float4(u[i * dim], u[i * dim + 1], dim == 3 ? u[i * dim + 2] : 0.0f, 0.0f);
+}
+
+// FP64 partial sums of a.b, one per workgroup
+__global__ __launch_bounds__(kThreads) void k_dot(const float *__restrict__ a, const float *__restrict__ b,
+                                                  long long n, double *__restrict__ part)
+{
+    __shared__ double lds[kThreads / 64];
+    double s = 0.0;
+    for (long long i = blockIdx.x * (long long)kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads)
+        s += (double)a[i] * (double)b[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) t += lds[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// out = a x + b y
+__global__ __launch_bounds__(kThreads) void k_lincomb(float *__restrict__ out, double a, const float *__restrict__ x,
+                                                      double b, const float *__restrict__ y, long long n)
+{
+    const long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i < n) out[i] = (float)(a * (double)x[i] + b * (double)y[i]);
+}
+
+constexpr int kDotBlocks = 1024;
+
+skm_status dot(const float *a, const float *b, long long n, double *part, hipStream_t s, double *out)
+{
+    k_dot<<<kDotBlocks, kThreads, 0, s>>>(a, b, n, part);
+    std::vector<double> h(kDotBlocks);
+    SKM_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * kDotBlocks, hipMemcpyDeviceToHost, s));
+    SKM_HIP(hipStreamSynchronize(s));
+    double t = 0.0;
+    for (double v : h) t += v;
+    *out = t;
+    return SKM_SUCCESS;
+}
+
+unsigned grid_of(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
 }  // namespace
+
+// CGLS for min |K c - u|^2 + ridge |c|^2, with K the evaluator's kernel sum
+// (control-point weights 1, no previous displacement) from SoA coefficients
+// to the dense displacement on the fixed grid.  K^T is the derivative gather
+// over the dense voxel grid.  Deterministic: fixed-order reductions.
+skm_status skm::fit_coefficients_device(skm_evaluator *ev, const float *d_u_aos, int iterations, double ridge,
+                                        float *d_c_soa)
+{
+    if (!ev || !d_u_aos || !d_c_soa) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    if (iterations < 0 || !(ridge >= 0.0)) return fail(SKM_ERROR_INVALID_ARGUMENT, "invalid fit settings");
+    if (!is_stationary(ev->pr.kc.mode))
+        return fail(SKM_ERROR_NOT_SUPPORTED, "the coefficient fit needs a stationary kernel (exact adjoint)");
+    const int dim = ev->dim;
+    hipStream_t s = ev->stream;
+    Problem pk = ev->pr;     // K: kernel sum only
+    pk.has_df = 0;
+    Problem pt = ev->pr;     // K^T: gather over the dense voxel grid
+    if (!is_stationary(pt.kc.mode)) {
+        if (!ev->d_vox_pstate) {
+            SKM_TRY(dev_alloc(&ev->d_vox_pstate, (size_t)pt.n_vox * 2));
+            launch_voxel_states(pt, ev->d_vox_pstate, s);
+        }
+        pt.smp_pstate = ev->d_vox_pstate;
+        if (ev->d_vox_row_lam) pt.smp_row_lam = ev->d_vox_row_lam;
+    }
+    pt.sub = 1;
+    for (int d = 0; d < 3; ++d) pt.nsub[d] = pt.fixed.n[d];
+    pt.n_sub = pt.n_vox;
+    pt.cp_window = ev->d_dense_window;
+    const long long nc = pt.n_cp * dim, nv = pt.n_vox * dim;
+    struct Tmp {
+        float *r = nullptr, *q = nullptr, *p = nullptr, *sv = nullptr, *ones = nullptr;
+        float4 *r4 = nullptr;
+        double *part = nullptr;
+        ~Tmp()
+        {
+            void *ptrs[] = {r, q, p, sv, ones, r4, part};
+            for (void *x : ptrs)
+                if (x) device_release(x);
+        }
+    } t;
+    SKM_TRY(dev_alloc(&t.r, (size_t)nv));
+    SKM_TRY(dev_alloc(&t.q, (size_t)nv));
+    SKM_TRY(dev_alloc(&t.p, (size_t)nc));
+    SKM_TRY(dev_alloc(&t.sv, (size_t)nc));
+    SKM_TRY(dev_alloc(&t.ones, (size_t)pt.n_cp));
+    SKM_TRY(dev_alloc(&t.r4, (size_t)pt.n_vox));
+    SKM_TRY(dev_alloc(&t.part, (size_t)kDotBlocks));
+    // declared after the temporaries: waits for the stream before they are released, on every return
+    struct SyncOnExit {
+        hipStream_t s;
+        ~SyncOnExit() { (void)hipStreamSynchronize(s); }
+    } sync_before_release{s};
+    std::vector<float> one((size_t)pt.n_cp, 1.0f);
+    SKM_HIP(hipMemcpyAsync(t.ones, one.data(), sizeof(float) * one.size(), hipMemcpyHostToDevice, s));
+    auto apply_kt = [&](const float *r, float *out) {
+        k_aos_to_float4<<<grid_of(pt.n_vox), kThreads, 0, s>>>(r, dim, pt.n_vox, t.r4);
+        launch_gather(pt, t.r4, out, s);
+    };
+    auto apply_k = [&](const float *c, float *out) {
+        launch_soa_to_cpc(pk, c, t.ones, ev->d_cpc, s);
+        launch_displacement(pk, out, s);
+    };
+    // c = 0, r = u, s = K^T r, p = s; the packed control points carry the
+    // weights (1) that the transpose multiplies by
+    SKM_HIP(hipMemsetAsync(d_c_soa, 0, sizeof(float) * nc, s));
+    launch_soa_to_cpc(pk, d_c_soa, t.ones, ev->d_cpc, s);
+    SKM_HIP(hipMemcpyAsync(t.r, d_u_aos, sizeof(float) * nv, hipMemcpyDeviceToDevice, s));
+    apply_kt(t.r, t.sv);
+    SKM_HIP(hipMemcpyAsync(t.p, t.sv, sizeof(float) * nc, hipMemcpyDeviceToDevice, s));
+    double gamma = 0.0;
+    SKM_TRY(dot(t.sv, t.sv, nc, t.part, s, &gamma));
+    for (int it = 0; it < iterations && gamma > 0.0; ++it) {
+        apply_k(t.p, t.q);
+        double qq = 0.0, pp = 0.0;
+        SKM_TRY(dot(t.q, t.q, nv, t.part, s, &qq));
+        SKM_TRY(dot(t.p, t.p, nc, t.part, s, &pp));
+        const double delta = qq + ridge * pp;
+        if (!(delta > 0.0)) break;
+        const double alpha = gamma / delta;
+        k_lincomb<<<grid_of(nc), kThreads, 0, s>>>(d_c_soa, 1.0, d_c_soa, alpha, t.p, nc);
+        k_lincomb<<<grid_of(nv), kThreads, 0, s>>>(t.r, 1.0, t.r, -alpha, t.q, nv);
+        apply_kt(t.r, t.sv);
+        k_lincomb<<<grid_of(nc), kThreads, 0, s>>>(t.sv, 1.0, t.sv, -ridge, d_c_soa, nc);
+        double g2 = 0.0;
+        SKM_TRY(dot(t.sv, t.sv, nc, t.part, s, &g2));
+        const double beta = g2 / gamma;
+        gamma = g2;
+        k_lincomb<<<grid_of(nc), kThreads, 0, s>>>(t.p, 1.0, t.sv, beta, t.p, nc);
+    }
+    SKM_HIP(hipGetLastError());
+    SKM_HIP(hipStreamSynchronize(s));
+    return SKM_SUCCESS;
+}
 
 skm_status skm::set_last_error(skm_status st, const std::string &msg) { return fail(st, msg); }
 
@@ -1188,6 +1334,12 @@ SKM_API skm_status skm_read_timing(skm_evaluator *ev, skm_pass_times *out)
         ev->pass_launches[i] = 0;
     }
     return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_fit_coefficients_device(skm_evaluator *ev, const float *d_displacement_aos, int32_t iterations,
+                                               double ridge, float *d_coefficients_soa)
+{
+    return skm::fit_coefficients_device(ev, d_displacement_aos, iterations, ridge, d_coefficients_soa);
 }
 
 SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning)

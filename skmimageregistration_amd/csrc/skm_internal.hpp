@@ -110,6 +110,11 @@ void launch_voxel_states(const Problem &pr, float4 *out, hipStream_t s);
 // skm_create with, when device_data, device pointers in the descriptor (skm_capi.hip)
 skm_status create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out,
                             hipStream_t stream = nullptr);
+// least-squares fit of a dense displacement (AoS, fixed grid, device) onto the
+// evaluator's kernel basis: CGLS for min |K c - u|^2 + ridge |c|^2, control-point
+// weights 1; c (SoA, device) starts at zero
+skm_status fit_coefficients_device(skm_evaluator *ev, const float *d_u_aos, int iterations, double ridge,
+                                   float *d_c_soa);
 // skm_set_displacement_field from a device array
 skm_status set_displacement_field_device(skm_evaluator *ev, const skm_grid *grid, const float *d_df_aos);
 skm_status set_displacement_field_impl(skm_evaluator *ev, const skm_grid *grid, const float *df_aos, bool device);

@@ -310,6 +310,35 @@ struct EvaluatorGuard {
         if (s_ != SKM_SUCCESS) return s_;                                                                          \
     } while (0)
 
+// The metric value a level's optimizer reports at SoA parameters `params`
+// (RegularizedImageToImageMetricv4::GetValue, .hxx:160-268): the measure plus
+// the weighted device and host regularizers.  `resample` draws new jitter
+// offsets first.
+skm_status level_value(skm_evaluator *ev, int metric, const skm_scale_params &P, const std::vector<float> &params,
+                       const std::vector<float> &cpw, int resample, double *value)
+{
+    const size_t n_cp = cpw.size(), dim = params.size() / n_cp;
+    std::vector<float> aos(params.size());
+    for (size_t j = 0; j < n_cp; ++j)
+        for (size_t d = 0; d < dim; ++d) aos[j * dim + d] = params[d * n_cp + j];
+    skm_result r;
+    SKM_R_TRY(skm_get_value(ev, metric, aos.data(), cpw.data(), resample, &r));
+    double v = r.measure;
+    if (P.reg_rkhs > 0) v += P.reg_rkhs * r.reg_value_rkhs;
+    if (P.reg_rd > 0) v += P.reg_rd * r.reg_value_rd;
+    if (P.reg_pg > 0) v += P.reg_pg * r.reg_value_pg;
+    if (P.reg_bilateral > 0) v += P.reg_bilateral * r.reg_value_bilateral;
+    if (P.reg_l1 > 0 || P.reg_l21 > 0 || P.reg_l2 > 0) {
+        std::vector<double> pd(params.begin(), params.end());
+        double reg = 0.0, l1 = 0.0, l21 = 0.0, l2 = 0.0;
+        SKM_R_TRY(skm_host_regularizers((int32_t)dim, (int64_t)n_cp, pd.data(), P.reg_l1, P.reg_l21, P.reg_l2, 0,
+                                        nullptr, &reg, &l1, &l21, &l2));
+        v += reg;
+    }
+    *value = v;
+    return SKM_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -421,6 +450,11 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         SKM_R_TRY(d_weight.upload(desc->weight_tensor, (size_t)n_w * dim * dim, ss));
     if (desc->guidance_image) SKM_R_TRY(d_guide.upload(desc->guidance_image, (size_t)n_w, ss));
     bool has_prev = !prev.empty();
+    const bool warm_field = !has_prev && desc->initial_displacement != nullptr;
+    if (warm_field) {
+        prev.assign(desc->initial_displacement, desc->initial_displacement + n_tar * dim);
+        has_prev = true;
+    }
     if (has_prev)
         SKM_R_TRY(d_prev.upload(prev.data(), prev.size(), ss));
     else
@@ -557,6 +591,36 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
         skm_optimizer_options oo{SKM_OPT_ADAPTIVE, P.initial_step_size, 20.0, 1.0, 1e7, -1e6, 1.0,
                                  (only_l1 || only_l21) ? 1 : 0, P.iterations, 1, 0};
         std::vector<float> params((size_t)n_cp * dim, 0.0f), cpw((size_t)n_cp, 1.0f);
+        if (scale == start && warm_field) {
+            // warm start by a dense field: kept unless it evaluates worse than none
+            rep.initial_used = 1;
+            if (desc->initial_guard) {
+                const std::vector<float> zero(params.size(), 0.0f);
+                SKM_R_TRY(level_value(ev, desc->metric, P, zero, cpw, 1, &rep.initial_value));
+                SKM_R_TRY(skm_set_displacement_field(ev, nullptr, nullptr));
+                SKM_R_TRY(level_value(ev, desc->metric, P, zero, cpw, 0, &rep.zero_value));
+                if (rep.initial_value < rep.zero_value) {
+                    SKM_R_TRY(skm::set_displacement_field_device(ev, &desc->target, d_prev.p));
+                } else {
+                    rep.initial_used = 0;
+                    has_prev = false;
+                }
+            }
+        }
+        if (scale == start && desc->initial_coefficients) {
+            // warm start: the given coefficients, unless they evaluate worse than zeros
+            std::copy(desc->initial_coefficients, desc->initial_coefficients + params.size(), params.begin());
+            rep.initial_used = 1;
+            if (desc->initial_guard) {
+                SKM_R_TRY(level_value(ev, desc->metric, P, params, cpw, 1, &rep.initial_value));
+                const std::vector<float> zero(params.size(), 0.0f);
+                SKM_R_TRY(level_value(ev, desc->metric, P, zero, cpw, 0, &rep.zero_value));
+                if (!(rep.initial_value < rep.zero_value)) {
+                    std::fill(params.begin(), params.end(), 0.0f);
+                    rep.initial_used = 0;
+                }
+            }
+        }
         std::vector<skm_iteration_record> rec(std::max(1, P.iterations));
         SKM_R_TRY(skm_optimize(ev, &mo, &oo, cpw.data(), params.data(), rec.data()));
         rep.iterations = P.iterations;

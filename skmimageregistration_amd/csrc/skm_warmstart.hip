@@ -524,6 +524,25 @@ __global__ __launch_bounds__(256) void k_add(float *__restrict__ dst, const floa
 
 unsigned blocks_of(long long n) { return (unsigned)((n + 255) / 256); }
 
+// One sample of the network input: the 2 x 2 x 2 block means of the moving
+// and the fixed image (each 2 in_size) as channels 0 and 1.
+__global__ __launch_bounds__(256) void k_prepare_input(const float *__restrict__ moving, const float *__restrict__ fixed,
+                                                       int d, int h, int w, float *__restrict__ out)
+{
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i >= (long long)d * h * w) return;
+    const int x = (int)(i % w), y = (int)((i / w) % h), z = (int)(i / ((long long)w * h));
+    const int W = 2 * w, H = 2 * h;
+    float sm = 0.0f, sf = 0.0f;
+    for (int k = 0; k < 8; ++k) {
+        const long long v = ((long long)(2 * z + (k >> 2)) * H + (2 * y + ((k >> 1) & 1))) * W + 2 * x + (k & 1);
+        sm += moving[v];
+        sf += fixed[v];
+    }
+    out[2 * i] = 0.125f * sm;
+    out[2 * i + 1] = 0.125f * sf;
+}
+
 }  // namespace
 }  // namespace skm
 
@@ -886,6 +905,18 @@ SKM_API skm_status skm_ws_backward(skm_ws_net *net, const float *d_dout, float *
     if (st != SKM_SUCCESS) return st;
     if (d_grad)
         WS_HIP(hipMemcpyAsync(d_grad, net->g, sizeof(float) * net->n_params, hipMemcpyDeviceToDevice, net->stream));
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_ws_prepare_input(skm_ws_net *net, const float *d_moving, const float *d_fixed, int32_t index,
+                                        float *d_input)
+{
+    if (!net || !d_moving || !d_fixed || !d_input) return ws_fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    if (index < 0 || index >= net->maxb) return ws_fail(SKM_ERROR_INVALID_ARGUMENT, "index outside 0..max_batch-1");
+    const int d = net->dims[0][0], h = net->dims[0][1], w = net->dims[0][2];
+    const long long nv = (long long)d * h * w;
+    k_prepare_input<<<blocks_of(nv), 256, 0, net->stream>>>(d_moving, d_fixed, d, h, w, d_input + 2 * nv * index);
+    WS_HIP(hipGetLastError());
     return SKM_SUCCESS;
 }
 

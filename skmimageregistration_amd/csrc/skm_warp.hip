@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <vector>
 
 #include "skm.h"
 #include "skm_internal.hpp"
@@ -247,6 +248,117 @@ bool same_grid(const skm_grid &a, const skm_grid &b, int dim)
 
 unsigned blocks_of(long long n) { return (unsigned)std::max<long long>(1, (n + kThreads - 1) / kThreads); }
 
+// ---- synthetic registration pairs (random ellipsoids, smooth displacement)
+
+struct Ellipsoid {
+    float c[3], inv_ax[3], value;
+};
+struct Rbf {
+    float c[3], w[3];
+};
+constexpr int kMaxEllipsoids = 64, kMaxRbf = 32;
+struct SynthShapes {
+    Ellipsoid e[kMaxEllipsoids];
+    int ne;
+};
+struct SynthField {
+    Rbf r[kMaxRbf];
+    int nr;
+    float inv2s2;
+};
+
+// value of the last ellipsoid (in order) holding the voxel, 0 outside all
+__global__ __launch_bounds__(kThreads) void k_synth_ellipsoids(SynthShapes sh, int nx, int ny, int nz,
+                                                               float *__restrict__ out)
+{
+    const long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i >= (long long)nx * ny * nz) return;
+    const float p[3] = {(float)(i % nx), (float)((i / nx) % ny), (float)(i / ((long long)nx * ny))};
+    float v = 0.0f;
+    for (int k = 0; k < sh.ne; ++k) {
+        const Ellipsoid &e = sh.e[k];
+        float q = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float t = (p[d] - e.c[d]) * e.inv_ax[d];
+            q += t * t;
+        }
+        if (q <= 1.0f) v = e.value;
+    }
+    out[i] = v;
+}
+
+// sum of Gaussian RBFs per voxel (AoS), with the block maxima of |d|^2
+__global__ __launch_bounds__(kThreads) void k_synth_field(SynthField f, int nx, int ny, int nz, float *__restrict__ out,
+                                                          float *__restrict__ blk_max)
+{
+    __shared__ float lds[kThreads / 64];
+    const long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    float m = 0.0f;
+    if (i < (long long)nx * ny * nz) {
+        const float p[3] = {(float)(i % nx), (float)((i / nx) % ny), (float)(i / ((long long)nx * ny))};
+        double u[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < f.nr; ++k) {
+            const Rbf &r = f.r[k];
+            const float dx = p[0] - r.c[0], dy = p[1] - r.c[1], dz = p[2] - r.c[2];
+            const double g = exp(-(double)(dx * dx + dy * dy + dz * dz) * (double)f.inv2s2);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) u[d] += g * (double)r.w[d];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[i * 3 + d] = (float)u[d];
+        m = (float)(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < kThreads / 64; ++w) t = fmaxf(t, lds[w]);
+        blk_max[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scale(float *__restrict__ a, long long n, float s)
+{
+    const long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i < n) a[i] *= s;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// adds N(0, sigma^2) per voxel: Box-Muller on two uniforms hashed from (seed, voxel)
+__global__ __launch_bounds__(kThreads) void k_synth_noise(float *__restrict__ img, long long n, uint64_t seed,
+                                                          float sigma)
+{
+    const long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t h = mix64(seed ^ mix64((uint64_t)i));
+    const double u1 = ((double)(h >> 11) + 1.0) * (1.0 / 9007199254740993.0);  // (0, 1]
+    const double u2 = (double)(mix64(h) >> 11) * (1.0 / 9007199254740992.0);
+    img[i] += (float)((double)sigma * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+// host stream of uniform doubles in [0, 1) (splitmix64)
+struct HostRng {
+    uint64_t s;
+    double next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    }
+    double uniform(double a, double b) { return a + (b - a) * next(); }
+};
+
 }  // namespace
 }  // namespace skm
 
@@ -300,3 +412,68 @@ SKM_API skm_status skm_warp_image(int32_t dim, const skm_grid *image_grid, const
 }
 
 }  // extern "C"
+
+extern "C" SKM_API skm_status skm_synthetic_pair(const skm_synthetic_desc *desc, float *reference, float *target,
+                                                float *displacement_aos)
+{
+    if (!desc || !reference || !target) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    const int nx = desc->size[0], ny = desc->size[1], nz = desc->size[2];
+    if (nx < 4 || ny < 4 || nz < 4) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "volume too small");
+    if (desc->n_ellipsoids < 0 || desc->n_ellipsoids > kMaxEllipsoids || desc->n_rbf < 1 || desc->n_rbf > kMaxRbf)
+        return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "ellipsoid or RBF count out of range");
+    if (!(desc->rbf_sigma > 0.0) || !(desc->smoothing_sigma >= 0.0) || !(desc->noise_sigma >= 0.0))
+        return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "sigmas must be positive");
+    const long long n = (long long)nx * ny * nz;
+    const int size[3] = {nx, ny, nz};
+    HostRng rng{desc->seed};
+    SynthShapes sh{};
+    sh.ne = desc->n_ellipsoids;
+    for (int k = 0; k < sh.ne; ++k) {
+        for (int d = 0; d < 3; ++d) sh.e[k].c[d] = (float)(rng.uniform(0.15, 0.85) * size[d]);
+        for (int d = 0; d < 3; ++d) sh.e[k].inv_ax[d] = (float)(1.0 / (rng.uniform(0.05, 0.2) * size[d]));
+        sh.e[k].value = (float)rng.next();
+    }
+    SynthField f{};
+    f.nr = desc->n_rbf;
+    f.inv2s2 = (float)(1.0 / (2.0 * desc->rbf_sigma * desc->rbf_sigma));
+    for (int k = 0; k < f.nr; ++k) {
+        for (int d = 0; d < 3; ++d) f.r[k].c[d] = (float)rng.uniform(0.0, size[d] - 1);
+        for (int d = 0; d < 3; ++d) f.r[k].w[d] = (float)rng.uniform(-1.0, 1.0);
+    }
+    skm_grid g{};
+    for (int d = 0; d < 3; ++d) {
+        g.size[d] = size[d];
+        g.spacing[d] = 1.0;
+    }
+    const Geo geo = to_geo(g, 3);
+    const unsigned nb = blocks_of(n);
+    Buf<float> img, df, out, bmax;
+    Buf<double> coeffs;
+    SKM_W_HIP(img.alloc(n));
+    SKM_W_HIP(df.alloc(n * 3));
+    SKM_W_HIP(out.alloc(n));
+    SKM_W_HIP(bmax.alloc(nb));
+    SKM_W_HIP(coeffs.alloc(n));
+    k_synth_ellipsoids<<<nb, kThreads>>>(sh, nx, ny, nz, img.p);
+    SKM_W_HIP(hipGetLastError());
+    if (desc->smoothing_sigma > 0.0) {
+        const skm_status st =
+            gaussian_smooth_device(3, g, img.p, desc->smoothing_sigma * desc->smoothing_sigma, img.p, nullptr);
+        if (st != SKM_SUCCESS) return st;
+    }
+    k_synth_field<<<nb, kThreads>>>(f, nx, ny, nz, df.p, bmax.p);
+    std::vector<float> hmax(nb);
+    SKM_W_HIP(hipMemcpy(hmax.data(), bmax.p, sizeof(float) * nb, hipMemcpyDeviceToHost));
+    float m2 = 0.0f;
+    for (float v : hmax) m2 = std::max(m2, v);
+    const double scale = desc->max_displacement / std::max(1e-12, std::sqrt((double)m2)) * desc->displacement_scale;
+    k_scale<<<blocks_of(n * 3), kThreads>>>(df.p, n * 3, (float)scale);
+    launch_bspline_prefilter(3, size, img.p, coeffs.p, nullptr);
+    k_warp_bspline3<3><<<nb, kThreads>>>(coeffs.p, geo, df.p, geo, 0.0, out.p, n);
+    if (desc->noise_sigma > 0.0) k_synth_noise<<<nb, kThreads>>>(out.p, n, desc->seed, (float)desc->noise_sigma);
+    SKM_W_HIP(hipGetLastError());
+    SKM_W_HIP(hipMemcpy(reference, img.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    SKM_W_HIP(hipMemcpy(target, out.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (displacement_aos) SKM_W_HIP(hipMemcpy(displacement_aos, df.p, sizeof(float) * n * 3, hipMemcpyDeviceToHost));
+    return SKM_SUCCESS;
+}

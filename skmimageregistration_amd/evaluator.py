@@ -561,12 +561,44 @@ def _reports(reps, dim):
     return [{"value": r.value, "measure": r.measure, "first_value": r.first_value, "iterations": r.iterations,
              "cp_size": tuple(r.cp_size[:dim]), "maximum_tilt": r.maximum_tilt, "ms_setup": r.ms_setup,
              "ms_optimize": r.ms_optimize, "ms_displacement": r.ms_displacement, "tre": r.tre,
-             "zero_parameters": r.zero_parameters} for r in reps]
+             "zero_parameters": r.zero_parameters, "initial_used": bool(r.initial_used),
+             "initial_value": r.initial_value, "zero_value": r.zero_value} for r in reps]
+
+
+def level_control_points(reference, config, scale):
+    """The control-point grid skm_register builds for a level
+    (SKMReg.cxx:219-232): spacing min(sigma_C4, sigma_C0) / cp_spacing_scaler."""
+    cfg = config or RegistrationConfig()
+    smin = min(cfg.CURRENT("sigma_C4", scale), cfg.CURRENT("sigma_C0", scale))
+    scaler = float(cfg.CURRENT("cp_spacing_scaler", 0))
+    h = smin if scaler < 1 else smin / scaler
+    return control_point_grid(reference, (h,) * reference.dim)
+
+
+def _attach_warm_start(d, keep, reference, target, config, initial_coefficients, initial_displacement,
+                       initial_guard):
+    """Warm-start fields of a registration descriptor (no SKMReg counterpart)."""
+    cfg = config or RegistrationConfig()
+    if initial_coefficients is not None:
+        grid = level_control_points(reference, cfg, max(0, int(cfg.CURRENT("restart_at", 0)) - 1))
+        c = f32(initial_coefficients).reshape(-1)
+        if c.size != grid.count() * reference.dim:
+            raise ValueError(f"initial coefficients: expected {grid.count() * reference.dim}, got {c.size}")
+        keep.append(c)
+        d.initial_coefficients = fptr(c)
+    if initial_displacement is not None:
+        u = f32(initial_displacement).reshape(-1)
+        if u.size != target.data.size * target.dim:
+            raise ValueError("initial displacement: D floats per target voxel expected")
+        keep.append(u)
+        d.initial_displacement = fptr(u)
+    d.initial_guard = int(bool(initial_guard))
 
 
 def register_multiscale(reference, target, config=None, weight_image=None, weight_tensor=None, guidance_image=None,
                         restart_displacement=None, temp_directory=None, reference_landmarks=None,
-                        target_landmarks=None):
+                        target_landmarks=None, initial_coefficients=None, initial_displacement=None,
+                        initial_guard=False):
     """The scale-level loop of SKMReg (src/SKMReg.cxx:84-525) through
     skm_register.  reference (moving) and target (fixed) are Images; the
     optional weight image / tensor / guidance image are Images on one common
@@ -574,23 +606,36 @@ def register_multiscale(reference, target, config=None, weight_image=None, weigh
     there and restart_at without restart_displacement resumes from its
     df-full file.  Landmarks ((n, D) arrays) give the TRE per level.
     Returns (displacement on the target grid, shape target.data.shape + (D,),
-    list of per-level report dicts)."""
+    list of per-level report dicts).
+
+    Warm start (no SKMReg counterpart): initial_coefficients (SoA on the
+    first level's control points) start that level's optimizer;
+    initial_displacement (the target grid's shape + (D,)) is the first level
+    run's previous displacement; with initial_guard either is used only when
+    it evaluates better than none."""
     lib = _capi.load()
     d, keep, out, reps = _registration_desc(lib, reference, target, config, weight_image, weight_tensor,
                                             guidance_image, restart_displacement, temp_directory,
                                             reference_landmarks, target_landmarks)
+    _attach_warm_start(d, keep, reference, target, config, initial_coefficients, initial_displacement,
+                       initial_guard)
     check(lib.skm_register(ctypes.byref(d), fptr(out), reps))
     return out, _reports(reps, target.dim)
 
 
-def register_batch(pairs, config=None, concurrency=4):
+def register_batch(pairs, config=None, concurrency=4, initial_displacements=None, initial_guard=False):
     """Independent registrations through skm_register_batch (no counterpart
     in the reference): pairs is a sequence of (reference, target) Images, all
     registered with one configuration, up to `concurrency` at once on the
     current device.  Each result equals register_multiscale's for its pair.
-    Returns a list of (displacement, per-level report dicts)."""
+    Returns a list of (displacement, per-level report dicts);
+    initial_displacements is an optional per-pair sequence (see
+    register_multiscale)."""
     lib = _capi.load()
     jobs = [_registration_desc(lib, r, t, config, None, None, None, None, None, None, None) for r, t in pairs]
+    if initial_displacements is not None:
+        for (d, keep, _, _), (r, t), u in zip(jobs, pairs, initial_displacements):
+            _attach_warm_start(d, keep, r, t, config, None, u, initial_guard)
     n = len(jobs)
     descs = (_capi.SkmRegistrationDesc * max(n, 1))()
     for i, (d, _, _, _) in enumerate(jobs):

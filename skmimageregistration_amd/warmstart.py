@@ -114,3 +114,252 @@ class WarmStartNet:
         check(self._lib.skm_ws_train_step(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(target.data_ptr()),
                                           batch, float(learning_rate), ctypes.byref(loss)))
         return loss.value
+
+
+# ---------------------------------------------------------------- registration workload
+
+CFG2_SHAPE = (96, 128, 128)  # z, y, x
+
+
+def synthetic_pair(seed, shape=CFG2_SHAPE, n_ellipsoids=20, smoothing_sigma=1.5, n_rbf=8, rbf_sigma=None,
+                   max_displacement=6.0, displacement_scale=1.0, noise_sigma=0.02):
+    """skm_synthetic_pair: (reference, target, displacement) as numpy arrays,
+    shapes (z, y, x) and (z, y, x, 3); the displacement RBFs' standard
+    deviation defaults to a quarter of the smallest extent."""
+    lib = _capi.load()
+    d = _capi.SkmSyntheticDesc()
+    for a, n in enumerate(reversed(shape)):
+        d.size[a] = int(n)
+    d.n_ellipsoids = int(n_ellipsoids)
+    d.smoothing_sigma = float(smoothing_sigma)
+    d.n_rbf = int(n_rbf)
+    d.rbf_sigma = float(rbf_sigma if rbf_sigma is not None else 0.25 * min(shape))
+    d.max_displacement = float(max_displacement)
+    d.displacement_scale = float(displacement_scale)
+    d.noise_sigma = float(noise_sigma)
+    d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    ref = np.empty(shape, np.float32)
+    tar = np.empty(shape, np.float32)
+    disp = np.empty(tuple(shape) + (3,), np.float32)
+    check(lib.skm_synthetic_pair(ctypes.byref(d), _capi.fptr(ref), _capi.fptr(tar), _capi.fptr(disp)))
+    return ref, tar, disp
+
+
+def plain_config(iterations=200, bilateral=1e-5):
+    """The plain 3-level registration of the 128x128x96 workload: support
+    96 / 48 / 24, control points every support / 4, tensors from the target
+    (weights 400 / 200 / 100), NCC + bilateral (spatial sigma 12, intensity
+    sigma 0.1), sampling rates 5 / 4 / 3, adaptive steps from 1000."""
+    from .evaluator import RegistrationConfig
+
+    return RegistrationConfig(sigma_C4=[96.0, 48.0, 24.0], sigma_C0=[96.0, 48.0, 24.0], tensor_weights=[400, 200, 100],
+                              num_scales=3, num_function_evaluations=[iterations], initial_step_size=[1000.0],
+                              sampling_rate=[5, 4, 3], neighborhood_sampling=[1], smoothing_variance=[2, 1, 0],
+                              reg_bilateral=[bilateral], reg_bilateral_spatial_sigma=[12.0],
+                              reg_bilateral_intensity_sigma=[0.1], metric="ncc")
+
+
+def fine_config(iterations=50, bilateral=1e-5):
+    """The last level of plain_config alone (support 24, control points every
+    6 voxels, tensor weight 100, sampling rate 3, no smoothing)."""
+    from .evaluator import RegistrationConfig
+
+    return RegistrationConfig(sigma_C4=[24.0], sigma_C0=[24.0], tensor_weights=[100], num_scales=1,
+                              num_function_evaluations=[iterations], initial_step_size=[1000.0], sampling_rate=[3],
+                              neighborhood_sampling=[1], smoothing_variance=[0], reg_bilateral=[bilateral],
+                              reg_bilateral_spatial_sigma=[12.0], reg_bilateral_intensity_sigma=[0.1], metric="ncc")
+
+
+class WarmStart:
+    """The warm start of one volume shape.  The network predicts
+    coefficients on the fine level's control points (every 6 voxels) in a
+    stationary Wendland basis of support 24; their dense field is the
+    previous displacement of a fine-level registration (SKM's own way of
+    composing levels, a SummationTransform), refined by <= `iterations`
+    plain steps and kept only when it evaluates better than none."""
+
+    COEF_SCALE = 10.0   # network outputs are the coefficients times this
+    SUPPORT = 24.0
+    FIT_ITERATIONS, FIT_RIDGE = 60, 300.0
+
+    def __init__(self, shape=CFG2_SHAPE, max_batch=1, weights=None):
+        import torch
+
+        from .evaluator import GpuKernelEvaluator, Image, Kernel, level_control_points
+
+        self._torch = torch
+        self.shape = tuple(shape)
+        if any(n % 16 for n in self.shape):
+            raise ValueError("volume sizes must be multiples of 16 (network input = half, multiple of 8)")
+        zero = Image(np.zeros(self.shape, np.float32))
+        self.cp = level_control_points(zero, fine_config(), 0)
+        self.n_cp = self.cp.count()
+        # the stationary basis the coefficients live in
+        self.basis = GpuKernelEvaluator(self.n_cp * 3, 1, 1, Kernel(self.SUPPORT, self.SUPPORT), zero, zero, self.cp)
+        in_size = tuple(n // 2 for n in reversed(self.shape))
+        # input voxel i covers image voxels 2i, 2i + 1: image coordinate p -> (p - 0.5) / 2
+        origin = tuple((o - 0.5) / 2.0 for o in self.cp.origin)
+        spacing = tuple(h / 2.0 for h in self.cp.spacing)
+        self.net = WarmStartNet(in_size, self.cp.size(), origin, spacing, max_batch=max_batch)
+        self.in_shape = (in_size[2], in_size[1], in_size[0], 2)
+        if weights is not None:
+            self.net.load(weights)
+
+    def close(self):
+        self.basis.close()
+        self.net.close()
+
+    def prepare(self, reference, target, d_input, index=0):
+        """Network input sample `index` from host or device images."""
+        t = self._torch
+        mov = reference if isinstance(reference, t.Tensor) else t.from_numpy(np.ascontiguousarray(reference)).cuda()
+        fix = target if isinstance(target, t.Tensor) else t.from_numpy(np.ascontiguousarray(target)).cuda()
+        check(self.net._lib.skm_ws_prepare_input(self.net._h, ctypes.c_void_p(mov.data_ptr()),
+                                                 ctypes.c_void_p(fix.data_ptr()), int(index),
+                                                 ctypes.c_void_p(d_input.data_ptr())))
+
+    def fit(self, displacement):
+        """Least-squares coefficients (SoA) of a dense field (shape + (3,)) in the basis."""
+        t = self._torch
+        d_u = t.from_numpy(np.ascontiguousarray(displacement, np.float32).reshape(-1)).cuda()
+        d_c = t.empty(self.n_cp * 3, device="cuda", dtype=t.float32)
+        t.cuda.synchronize()  # the fit runs on the evaluator's own stream
+        check(self.basis._lib.skm_fit_coefficients_device(self.basis.handle, ctypes.c_void_p(d_u.data_ptr()),
+                                                          self.FIT_ITERATIONS, self.FIT_RIDGE,
+                                                          ctypes.c_void_p(d_c.data_ptr())))
+        return d_c.cpu().numpy()
+
+    def field(self, coefficients_soa):
+        """The basis' dense field (shape + (3,)) of SoA coefficients."""
+        c = np.asarray(coefficients_soa, np.float32).reshape(3, self.n_cp)
+        u = self.basis.evaluateDisplacementField(np.ascontiguousarray(c.T), np.ones(self.n_cp, np.float32))
+        return u.reshape(self.shape + (3,))
+
+    def label(self, coefficients_soa):
+        """SoA coefficients -> the network's [z][y][x][3] target."""
+        nx, ny, nz = self.cp.size()
+        c = np.asarray(coefficients_soa, np.float32).reshape(3, nz, ny, nx)
+        return np.ascontiguousarray(c.transpose(1, 2, 3, 0)) * self.COEF_SCALE
+
+    def coefficients(self, out):
+        """The network's [z][y][x][3] output -> SoA coefficients."""
+        nx, ny, nz = self.cp.size()
+        o = np.asarray(out, np.float32).reshape(nz, ny, nx, 3)
+        return np.ascontiguousarray(o.transpose(3, 0, 1, 2)).reshape(-1) / self.COEF_SCALE
+
+    def predict(self, reference, target):
+        x = self._torch.empty((1,) + self.in_shape, device="cuda", dtype=self._torch.float32)
+        self.prepare(reference, target, x)
+        return self.coefficients(self.net.forward(x).cpu().numpy())
+
+    def register(self, reference, target, iterations=50, bilateral=1e-5, guard=True, coefficients=None):
+        """Warm-started registration: the predicted (or given) coefficients'
+        field refined by `iterations` steps of the fine level.  Returns
+        (displacement, report dict of the level)."""
+        from .evaluator import Image, register_multiscale
+
+        c = self.predict(reference, target) if coefficients is None else coefficients
+        df, reps = register_multiscale(Image(reference), Image(target), fine_config(iterations, bilateral),
+                                       initial_displacement=self.field(c), initial_guard=guard)
+        return df, reps[0]
+
+
+def build_dataset(ws, seeds, concurrency=4, chunk=32, plain_iterations=200, bilateral=1e-5, log=print):
+    """Network inputs (device tensor [n, z, y, x, 2]) and labels (device
+    tensor [n, cz, cy, cx, 3]) for the synthetic pairs of `seeds`: each pair
+    registered by the plain 3-level method, its final field fitted onto the
+    warm start's basis.  Also returns the plain runs' final NCC."""
+    import time
+
+    from .evaluator import Image, register_batch
+
+    torch = ws._torch
+    n = len(seeds)
+    x = torch.empty((n,) + ws.in_shape, device="cuda", dtype=torch.float32)
+    nx, ny, nz = ws.cp.size()
+    y = torch.empty((n, nz, ny, nx, 3), device="cuda", dtype=torch.float32)
+    nccs = []
+    t0 = time.perf_counter()
+    for c0 in range(0, n, chunk):
+        pairs = []
+        for i in range(c0, min(n, c0 + chunk)):
+            ref, tar, _ = synthetic_pair(seeds[i], ws.shape)
+            pairs.append((Image(ref), Image(tar)))
+            ws.prepare(ref, tar, x[i:i + 1])
+        res = register_batch(pairs, plain_config(plain_iterations, bilateral), concurrency=concurrency)
+        for k, (df, reps) in enumerate(res):
+            y[c0 + k] = torch.from_numpy(ws.label(ws.fit(df))).cuda()
+            nccs.append(reps[-1]["measure"])
+        log(f"labels {min(n, c0 + chunk)}/{n}: {time.perf_counter() - t0:.1f} s, "
+            f"mean plain NCC {np.mean(nccs):.4f}")
+    return x, y, np.array(nccs)
+
+
+def train(ws, x, y, epochs=20, batch=8, lr=1e-3, val_fraction=0.05, seed=0, log=print):
+    """Adam on the mean squared error between the network's output and the
+    labels, cosine learning-rate decay, the last val_fraction held out.
+    Returns the per-epoch (train loss, validation loss)."""
+    import math
+
+    torch = ws._torch
+    n = x.shape[0]
+    n_val = max(1, int(round(n * val_fraction))) if n > 1 else 0
+    n_tr = n - n_val
+    rng = np.random.default_rng(seed)
+    steps = epochs * max(1, n_tr // batch)
+    step = 0
+    hist = []
+    for ep in range(epochs):
+        perm = rng.permutation(n_tr)
+        losses = []
+        for b0 in range(0, n_tr - batch + 1, batch):
+            idx = torch.from_numpy(perm[b0:b0 + batch]).cuda()
+            xb = x.index_select(0, idx).contiguous()
+            yb = y.index_select(0, idx).contiguous()
+            rate = lr * 0.5 * (1.0 + math.cos(math.pi * step / max(1, steps)))
+            losses.append(ws.net.train_step(xb, yb, rate))
+            step += 1
+        val = float("nan")
+        if n_val:
+            vl = []
+            for b0 in range(n_tr, n, batch):
+                xb = x[b0:min(n, b0 + batch)].contiguous()
+                out = ws.net.forward(xb).cpu().numpy()
+                yt = y[b0:min(n, b0 + batch)].cpu().numpy()
+                vl.append(float(np.mean((out.astype(np.float64) - yt) ** 2)) * xb.shape[0])
+            val = sum(vl) / n_val
+        hist.append((float(np.mean(losses)) if losses else float("nan"), val))
+        log(f"epoch {ep + 1}/{epochs}: train {hist[-1][0]:.5f} validation {val:.5f}")
+    return hist
+
+
+def evaluate(ws, seeds, iterations=50, plain_iterations=200, bilateral=1e-5, log=print):
+    """Per held-out pair: the plain 3-level registration, the warm-started
+    one (prediction + `iterations` fine-level steps, guarded) and the same
+    fine-level steps from zero (what the prediction adds).  Wall times
+    include the host images in and the dense field out.  Returns a list of
+    dicts."""
+    import time
+
+    from .evaluator import Image, register_multiscale
+
+    inner = (slice(10, -10),) * 3
+    rows = []
+    for seed in seeds:
+        ref, tar, disp = synthetic_pair(seed, ws.shape)
+        t0 = time.perf_counter()
+        dfp, rp = register_multiscale(Image(ref), Image(tar), plain_config(plain_iterations, bilateral))
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        dfw, rw = ws.register(ref, tar, iterations, bilateral)
+        t_warm = time.perf_counter() - t0
+        dfz, rz = register_multiscale(Image(ref), Image(tar), fine_config(iterations, bilateral))
+        row = {"seed": seed, "plain_ms": t_plain * 1e3, "warm_ms": t_warm * 1e3,
+               "plain_ncc": rp[-1]["measure"], "warm_ncc": rw["measure"], "zero_ncc": rz[0]["measure"],
+               "warm_used": rw["initial_used"], "warm_initial": rw["initial_value"], "zero_initial": rw["zero_value"],
+               "plain_err": float(np.abs(dfp[inner] - disp[inner]).mean()),
+               "warm_err": float(np.abs(dfw[inner] - disp[inner]).mean()),
+               "zero_err": float(np.abs(dfz[inner] - disp[inner]).mean())}
+        rows.append(row)
+        log(" ".join(f"{k} {v:.4f}" if isinstance(v, float) else f"{k} {v}" for k, v in row.items()))
+    return rows
