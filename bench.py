@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--lcc-iterations", type=int, default=50, help="iterations per level of the LCC registration")
     ap.add_argument("--batch-frames", type=int, default=32, help="frames of the time series (frame 0 = reference)")
     ap.add_argument("--batch-concurrency", type=int, default=4, help="registrations in flight in the batched run")
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start comparison")
+    ap.add_argument("--warm-pairs", type=int, default=8, help="held-out pairs of the warm-start comparison")
     ap.add_argument("--registration-repeats", type=int, default=3)
     ap.add_argument("--registration-rd", type=float, default=1e-5,
                     help="RD weight of the registration (the bilateral-like regularizer)")
@@ -287,42 +289,54 @@ def profiled_traffic(kernel, key):
 
 
 def cpu_baseline(args, mov, fix, ten, mcov):
-    """The FP64 oracle, OpenMP build (per-sample and per-control-point loops
-    in parallel over the host's cores, at most 16: one GPU's share of the
-    host), one GetValueAndDerivative of the same workload (whole volumes,
-    same kernel and control-point spacing; --cpu-crop z,y,x takes a corner
-    crop instead); reported in the same unit."""
+    """The FP32 multi-core CPU restatement of the reference's NCC
+    GetValueAndDerivative (oracle/cpu_fp32.c, compiled here with -O3
+    -march=native -fopenmp, OpenMP over every logical CPU this process may
+    use), one evaluation of the same workload (whole volumes, same kernel,
+    sigma, control-point spacing and RD; --cpu-crop z,y,x takes a corner crop
+    instead), median of --cpu-repeats after one untimed run; reported in the
+    same unit.  Also the CPU time of one 3-level registration estimated as
+    iterations x pairs per iteration / the measured pair rate."""
+    if args.measure != "ncc" or args.subsample != 1:
+        return None
     sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import cpu_fp32
     import oracle as orc
     avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    threads = min(16, avail, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
-    threads = orc.load(parallel=True).skmo_set_threads(threads)
+    threads = cpu_fp32.load().skmf_set_threads(avail)
     crop = tuple(int(v) for v in args.cpu_crop.split(",")) if args.cpu_crop else tuple(fix.shape)
     crop = tuple(min(c, n) for c, n in zip(crop, fix.shape))
     f = np.ascontiguousarray(fix[: crop[0], : crop[1], : crop[2]])
     m = np.ascontiguousarray(mov[: crop[0], : crop[1], : crop[2]])
     size, cs, co = geometry(crop, args.cp_spacing)
     rng = np.random.default_rng(0)
-    cp = rng.uniform(-1, 1, size=(int(np.prod(cs)), 3))
-    kw = {}
+    n_cp = int(np.prod(cs))
+    cp = rng.uniform(-1, 1, size=(n_cp, 3)).astype(np.float32)
     tc = None
     if ten is not None:
-        tc = np.ascontiguousarray(ten[: crop[0], : crop[1], : crop[2]])
-        kw = dict(kernel_mode=2, max_eig_cov=mcov, weight=(size, (0.0,) * 3, (1.0,) * 3, tc))
-    pr = orc.OracleProblem(f, m, (cs, co, (args.cp_spacing,) * 3), cp, sigma_c0=args.sigma, sigma_c4=args.sigma,
-                           parallel=True, **kw)
-    measure = {"mse": 0, "ncc": 1, "lcc": 2}[args.measure]
+        tc = np.ascontiguousarray(ten[: crop[0], : crop[1], : crop[2]]).reshape(-1, 9)
+    coeffs = orc.bspline_coefficients(m.astype(np.float64))
+    pr = cpu_fp32.CpuProblem(f, coeffs, (cs, co, (args.cp_spacing,) * 3), cp, np.ones(n_cp, np.float32), args.sigma,
+                             cov=tc, max_eig_cov=mcov)
+    pr.evaluate(args.reg_rd)
     times = []
     for _ in range(max(args.cpu_repeats, 1)):
         t0 = time.perf_counter()
-        pr.evaluate(measure, True, 0.0, args.reg_rd, 0.0)
+        pr.evaluate(args.reg_rd)
         times.append(time.perf_counter() - t0)
     dt = float(np.median(times))
-    v, d, _, _ = pair_counts(args, crop, tc, mcov, None, False)
-    return {"value": (v + d) / dt / 1e9, "unit": "GVox*ctrl/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle (OpenMP, {threads} threads), GetValueAndDerivative on "
-                      f"{crop[2]}x{crop[1]}x{crop[0]} voxels (same kernel, sigma and cp spacing), median of "
-                      f"{len(times)}: {dt:.1f} s; host {host_cpu()}"}
+    v, d, _, _ = pair_counts(args, crop, None if tc is None else tc.reshape(crop + (3, 3)), mcov, None, False)
+    rate = (v + d) / dt
+    # pairs per iteration of the registration's levels: samples / sub^3 x (2 support / h)^3 box
+    # control points, value and derivative pass; support / h = 4 on every level
+    n_vox = float(np.prod(fix.shape))
+    reg_pairs = sum(200 * 2 * n_vox / sub ** 3 * 8 ** 3 for sub in (5, 4, 3))
+    return {"value": rate / 1e9, "unit": "GVox*ctrl/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 CPU restatement (oracle/cpu_fp32.c, -O3 -march=native, OpenMP {threads} threads), "
+                      f"GetValueAndDerivative on {crop[2]}x{crop[1]}x{crop[0]} voxels (same kernel, sigma, cp "
+                      f"spacing, RD), median of {len(times)}: {dt:.2f} s; host {host_cpu()}",
+            "ms_per_evaluation": round(dt * 1e3, 1),
+            "ms_per_registration_estimate": round(reg_pairs / rate * 1e3, 0)}
 
 
 def host_cpu():
@@ -489,6 +503,49 @@ def batch_timing(args):
     return out
 
 
+def warm_start_timing(args):
+    """The learned warm start on held-out synthetic pairs of the 128x128x96
+    workload (seeds 900.., outside the training seeds): per pair the plain
+    3-level registration (3 x 200 iterations), the warm-started one (network
+    prediction + 50 fine-level iterations, guarded) and the same 50 iterations
+    from zero; then the warm-started registrations of --warm-pairs pairs run
+    --batch-concurrency at a time (skm_register_batch).  Wall times include
+    host images in and the dense field out."""
+    from skmimageregistration_amd import Image, register_batch
+    from skmimageregistration_amd.warmstart import WarmStart, evaluate, fine_config, synthetic_pair
+    weights = os.path.join(REPO, "skmimageregistration_amd", "weights", "warmstart_128x128x96.bin")
+    if not os.path.exists(weights):
+        return None
+    ws = WarmStart(weights=weights)
+    ref, tar, _ = synthetic_pair(899)
+    ws.register(ref, tar, 50)  # first-call set-up, untimed
+    rows = evaluate(ws, list(range(900, 900 + args.warm_pairs)), log=lambda m: None)
+    # batched: predictions first (one network pass per pair), then the guarded fine-level runs in flight together
+    pairs = [synthetic_pair(900 + i)[:2] for i in range(args.warm_pairs)]
+    t0 = time.perf_counter()
+    fields = [ws.field(ws.predict(r, t)) for r, t in pairs]
+    res = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), concurrency=args.batch_concurrency,
+                         initial_displacements=fields, initial_guard=True)
+    ms_batch = (time.perf_counter() - t0) * 1e3 / len(pairs)
+    ws.close()
+    plain_ms = float(np.mean([r["plain_ms"] for r in rows]))
+    warm_ms = float(np.mean([r["warm_ms"] for r in rows]))
+    return {"workload": f"{len(rows)} held-out 128x128x96 pairs (max displacement 6), NCC + bilateral 1e-5; plain = "
+                        "3 levels x 200 iterations, warm = prediction + 50 fine-level iterations",
+            "plain_ms_per_registration": round(plain_ms, 2), "warm_ms_per_registration": round(warm_ms, 2),
+            "warm_batched_ms_per_registration": round(ms_batch, 2),
+            "speedup_single": round(plain_ms / warm_ms, 2), "speedup_batched": round(plain_ms / ms_batch, 2),
+            "plain_final_ncc": round(float(np.mean([r["plain_ncc"] for r in rows])), 5),
+            "warm_final_ncc": round(float(np.mean([r["warm_ncc"] for r in rows])), 5),
+            "warm_batched_final_ncc": round(float(np.mean([r[1][0]["measure"] for r in res])), 5),
+            "zero_start_final_ncc": round(float(np.mean([r["zero_ncc"] for r in rows])), 5),
+            "max_rel_ncc_gap": round(float(max(abs(r["warm_ncc"] - r["plain_ncc"]) / abs(r["plain_ncc"])
+                                               for r in rows)), 5),
+            "predictions_used": int(sum(r["warm_used"] for r in rows)),
+            "mean_abs_error_vox": {k: round(float(np.mean([r[k + "_err"] for r in rows])), 4)
+                                   for k in ("plain", "warm", "zero")}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -560,10 +617,17 @@ def main():
         pairs = vnz if dom == "value" else dnz
         achieved = pairs * fpp / (per_kernel[dom] * 1e-3) / 1e12
         kname = "k_value" if dom == "value" else ("k_gather" if args.kernel == "stationary" else "k_gather_tile")
+        traffic = profiled_traffic(kname, workload_key(args, shape))
         roof = {"bound": "valu", "kernel": kname,
                 "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": profiled_traffic(kname, workload_key(args, shape)),
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                 "launch_ms": round(per_kernel[dom], 4), "pairs_per_launch": pairs, "flops_per_pair": fpp}
+        if traffic:
+            # the measured HBM traffic (rocprofv3 FETCH + WRITE) as a fraction of the HBM peak: when it
+            # exceeds the FP32 fraction, memory traffic co-limits the kernel
+            hbm_frac = traffic / (per_kernel[dom] * 1e-3) / (HBM_PEAK_GBS * 1e9)
+            roof["hbm_traffic_frac"] = round(hbm_frac, 4)
+            roof["limiter"] = "hbm traffic" if hbm_frac > roof["frac"] else "fp32 valu"
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, mov, fix, ten, mcov)
@@ -592,6 +656,8 @@ def main():
         line["batch"] = batch_timing(args)
     if not args.no_lcc_registration:
         line["lcc_registration"] = lcc_registration(args)
+    if not args.no_warm_start:
+        line["warm_start"] = warm_start_timing(args)
     print(json.dumps(line))
 
 

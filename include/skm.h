@@ -216,6 +216,27 @@ typedef struct skm_tuning {
 SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning);
 SKM_API skm_status skm_get_tuning(skm_evaluator *ev, skm_tuning *tuning);
 
+/* The project's autotuner (no counterpart in the reference): times every
+ * launch shape skm_set_tuning offers for `measure` on this evaluator (value
+ * split x gather slab for MSE / NCC, workgroups per tile for the reference
+ * LCC derivative, gather slab for the exact LCC derivative) with HIP events,
+ * `repeats` evaluations each (median) at the device control points given,
+ * sets the fastest and records it for the evaluator's shape class (dimension,
+ * kernel mode, windows, sampling, image and control-point grids); evaluators
+ * created later in the process with the same class start with it.  The
+ * table can be saved and loaded (one line per class). */
+typedef struct skm_autotune_report {
+    skm_tuning best;
+    double best_ms;          /* median time of one evaluation with it        */
+    double first_ms;         /* the same for the automatic shapes            */
+    int32_t candidates;
+    int32_t reserved;
+} skm_autotune_report;
+SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float *d_cp_aos, const float *d_cpw,
+                                int32_t repeats, skm_autotune_report *report);
+SKM_API skm_status skm_tuning_table_save(const char *path);
+SKM_API skm_status skm_tuning_table_load(const char *path);
+
 /* Jitter offsets of the last evaluation, D ints per subsampled cell
  * (GpuEvaluator.cu:107-118); n_cells receives prod(fixed.size / subsample). */
 SKM_API skm_status skm_get_cube_offsets(skm_evaluator *ev, int32_t *out, int64_t *n_cells);
@@ -409,6 +430,18 @@ SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *met
                                 const skm_optimizer_options *opt, const float *cpw, float *params_soa,
                                 skm_iteration_record *records);
 
+/* skm_optimize of n evaluators at once (no counterpart in the reference,
+ * which optimises one pair per process): the same metric and optimizer
+ * options, each evaluator's own cpw, start point (params_soa[i], host, SoA)
+ * and records (records[i], may be NULL; records itself may be NULL).  The
+ * first iteration runs kernel by kernel; with opt->graph the others replay
+ * one captured HIP graph holding one iteration of every evaluator, each on
+ * its own stream, so that one graph launch performs one optimizer step of
+ * all n.  Every result equals skm_optimize's on its evaluator. */
+SKM_API skm_status skm_optimize_batch(skm_evaluator *const *evaluators, int32_t n, const skm_metric_options *metric,
+                                      const skm_optimizer_options *opt, const float *const *cpw,
+                                      float *const *params_soa, skm_iteration_record *const *records);
+
 /* Adds L1 / L21 / L2 values and subgradients to a host derivative and applies
  * the sign flip exactly as the metric does on the host
  * (itkRegularizedImageToImageMetricv4.hxx:430-529).  Returns the weighted sum
@@ -493,6 +526,10 @@ typedef struct skm_registration_desc {
     int32_t initial_guard;                             /* 1: start from zeros instead when the
                                                           given coefficients (or displacement)
                                                           evaluate worse than zeros (none) */
+    int32_t autotune;                                  /* 1: each level's evaluator runs
+                                                          skm_autotune first unless its shape
+                                                          class is already tuned (no SKMReg
+                                                          counterpart)               */
     const float *initial_displacement;                 /* previous displacement of the first
                                                           level run, AoS on the target grid
                                                           (as restart_displacement), or NULL;
@@ -547,6 +584,18 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
 SKM_API skm_status skm_register_batch(const skm_registration_desc *descs, int32_t n, int32_t concurrency,
                                       float *const *displacements_out, skm_scale_report *const *reports,
                                       skm_status *statuses);
+
+/* The same batch in lockstep (no counterpart in the reference): all n
+ * registrations in flight, one host thread each, their set-up concurrent;
+ * at every level the optimisers of all registrations still running run
+ * together through skm_optimize_batch, so that each optimizer step of the
+ * whole batch is one launch of one captured HIP graph (one branch per
+ * registration; registrations whose levels have different optimizer options
+ * form separate graphs).  Results, reports and statuses as skm_register_batch;
+ * each result equals skm_register's on its descriptor. */
+SKM_API skm_status skm_register_batch_lockstep(const skm_registration_desc *descs, int32_t n,
+                                               float *const *displacements_out, skm_scale_report *const *reports,
+                                               skm_status *statuses);
 
 /* ---- learned warm start ------------------------------------------------- */
 

@@ -7,13 +7,13 @@ interface over that ABI.
 from . import _capi
 from .evaluator import (MEASURE, ComputeAnisotropicTensor, EvaluationResult, GaussianSmoothing,
                         GpuKernelEvaluator, Grid, Image, Kernel, RegistrationConfig, control_point_grid,
-                        landmark_tre, read_image, read_landmarks, register_batch, register_files, register_multiscale,
-                        warp_image, write_image)
+                        landmark_tre, optimize_batch, read_image, read_landmarks, register_batch, register_files,
+                        register_multiscale, tuning_table_load, tuning_table_save, warp_image, write_image)
 
 __all__ = ["MEASURE", "ComputeAnisotropicTensor", "EvaluationResult", "GaussianSmoothing", "GpuKernelEvaluator",
            "Grid", "Image", "Kernel", "RegistrationConfig", "control_point_grid", "landmark_tre", "load_library",
-           "read_image", "read_landmarks", "register_batch", "register_files", "register_multiscale", "warp_image",
-           "write_image"]
+           "optimize_batch", "read_image", "read_landmarks", "register_batch", "register_files", "register_multiscale",
+           "tuning_table_load", "tuning_table_save", "warp_image", "write_image"]
 
 
 def load_library():

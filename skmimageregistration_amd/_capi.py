@@ -79,6 +79,11 @@ class SkmTuning(ctypes.Structure):
                 ("value_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
+class SkmAutotuneReport(ctypes.Structure):
+    _fields_ = [("best", SkmTuning), ("best_ms", ctypes.c_double), ("first_ms", ctypes.c_double),
+                ("candidates", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class SkmWsDesc(ctypes.Structure):
     _fields_ = [("in_size", ctypes.c_int32 * 3), ("cp_size", ctypes.c_int32 * 3), ("cp_origin", ctypes.c_double * 3),
                 ("cp_spacing", ctypes.c_double * 3), ("max_batch", ctypes.c_int32)]
@@ -128,7 +133,8 @@ class SkmRegistrationDesc(ctypes.Structure):
                 ("verbosity", ctypes.c_int32), ("num_landmarks", ctypes.c_int64),
                 ("reference_landmarks", c_double_p), ("target_landmarks", c_double_p),
                 ("exact_gradient", ctypes.c_int32), ("initial_coefficients", c_float_p),
-                ("initial_guard", ctypes.c_int32), ("initial_displacement", c_float_p)]
+                ("initial_guard", ctypes.c_int32), ("autotune", ctypes.c_int32),
+                ("initial_displacement", c_float_p)]
 
 
 class SkmScaleReport(ctypes.Structure):
@@ -172,6 +178,10 @@ _SIGNATURES = {
     "skm_read_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmPassTimes)]),
     "skm_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmTuning)]),
     "skm_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmTuning)]),
+    "skm_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                    ctypes.POINTER(SkmAutotuneReport)]),
+    "skm_tuning_table_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "skm_tuning_table_load": (ctypes.c_int, [ctypes.c_char_p]),
     "skm_get_cube_offsets": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, ctypes.POINTER(ctypes.c_int64)]),
     "skm_control_point_grid": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_double_p,
                                               ctypes.POINTER(SkmGrid)]),
@@ -190,6 +200,14 @@ _SIGNATURES = {
     "skm_register_batch": (ctypes.c_int, [ctypes.POINTER(SkmRegistrationDesc), ctypes.c_int32, ctypes.c_int32,
                                           ctypes.POINTER(c_float_p), ctypes.POINTER(ctypes.POINTER(SkmScaleReport)),
                                           ctypes.POINTER(ctypes.c_int)]),
+    "skm_register_batch_lockstep": (ctypes.c_int, [ctypes.POINTER(SkmRegistrationDesc), ctypes.c_int32,
+                                                   ctypes.POINTER(c_float_p),
+                                                   ctypes.POINTER(ctypes.POINTER(SkmScaleReport)),
+                                                   ctypes.POINTER(ctypes.c_int)]),
+    "skm_optimize_batch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32,
+                                          ctypes.POINTER(SkmMetricOptions), ctypes.POINTER(SkmOptimizerOptions),
+                                          ctypes.POINTER(c_float_p), ctypes.POINTER(c_float_p),
+                                          ctypes.POINTER(ctypes.POINTER(SkmIterationRecord))]),
     "skm_warp_image": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_float_p, ctypes.POINTER(SkmGrid),
                                       c_float_p, ctypes.POINTER(SkmGrid), ctypes.c_int32, ctypes.c_double,
                                       c_float_p]),

@@ -8,8 +8,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <mutex>
+#include <sstream>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -951,6 +953,42 @@ SKM_API skm_status skm_create(const skm_desc *desc, skm_evaluator **out)
 
 // skm_create; with device_data the image, weight and tensor pointers of the
 // descriptor are device pointers (copied on the device, no PCIe transfer).
+// ---- launch-shape table (skm_autotune results), per shape class
+
+namespace {
+
+std::mutex g_tuning_mutex;
+std::map<std::string, skm_tuning> g_tuning;
+
+// what the launch shapes depend on: dimension, kernel mode, windows, sampling,
+// image and control-point grids
+std::string shape_class(const skm_evaluator *ev)
+{
+    const Problem &pr = ev->pr;
+    std::string k = std::to_string(pr.dim) + ":" + std::to_string(pr.kc.mode) + ":" + std::to_string(pr.exact) + ":" +
+                    std::to_string(pr.sub) + ":" + std::to_string(pr.subn);
+    for (int d = 0; d < pr.dim; ++d) k += ":" + std::to_string(pr.fixed.n[d]);
+    for (int d = 0; d < pr.dim; ++d) k += ":" + std::to_string(pr.cp.n[d]);
+    return k;
+}
+
+bool tuning_lookup(const std::string &key, skm_tuning *t)
+{
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    auto it = g_tuning.find(key);
+    if (it == g_tuning.end()) return false;
+    *t = it->second;
+    return true;
+}
+
+void tuning_store(const std::string &key, const skm_tuning &t)
+{
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    g_tuning[key] = t;
+}
+
+}  // namespace
+
 skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out, hipStream_t stream)
 {
     if (!desc || !out) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
@@ -1109,6 +1147,14 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
     if (hipMemsetAsync(ev->d_offsets, 0, sizeof(int) * (pr.sub > 1 ? pr.n_sub * dim : 1), ev->stream) != hipSuccess ||
         hipStreamSynchronize(ev->stream) != hipSuccess)
         return cleanup(fail(SKM_ERROR_HIP, "clearing the offsets failed"));
+    // launch shapes tuned for this shape class earlier in the process (or loaded)
+    skm_tuning t;
+    if (tuning_lookup(shape_class(ev), &t)) {
+        ev->tuning = t;
+        pr.lcc_chunks = t.lcc_ref_chunks;
+        pr.gather_split = t.gather_split;
+        pr.value_split = t.value_split;
+    }
     *out = ev;
     return SKM_SUCCESS;
 }
@@ -1342,16 +1388,25 @@ SKM_API skm_status skm_fit_coefficients_device(skm_evaluator *ev, const float *d
     return skm::fit_coefficients_device(ev, d_displacement_aos, iterations, ridge, d_coefficients_soa);
 }
 
+namespace {
+
+skm_status validate_tuning(const skm_tuning &t)
+{
+    if (t.lcc_ref_chunks < 0 || t.lcc_ref_chunks > 64)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "lcc_ref_chunks must be 0 (automatic) or 1..64");
+    if (t.gather_split != 0 && t.gather_split != 4 && t.gather_split != 8)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "gather_split must be 0 (automatic), 4 or 8");
+    if (t.value_split != 0 && t.value_split != 1 && t.value_split != 4 && t.value_split != 8 && t.value_split != 16)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "value_split must be 0 (automatic), 1, 4, 8 or 16");
+    return SKM_SUCCESS;
+}
+
+}  // namespace
+
 SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning)
 {
     if (!ev || !tuning) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
-    if (tuning->lcc_ref_chunks < 0 || tuning->lcc_ref_chunks > 64)
-        return fail(SKM_ERROR_INVALID_ARGUMENT, "lcc_ref_chunks must be 0 (automatic) or 1..64");
-    if (tuning->gather_split != 0 && tuning->gather_split != 4 && tuning->gather_split != 8)
-        return fail(SKM_ERROR_INVALID_ARGUMENT, "gather_split must be 0 (automatic), 4 or 8");
-    if (tuning->value_split != 0 && tuning->value_split != 1 && tuning->value_split != 4 &&
-        tuning->value_split != 8 && tuning->value_split != 16)
-        return fail(SKM_ERROR_INVALID_ARGUMENT, "value_split must be 0 (automatic), 1, 4, 8 or 16");
+    SKM_TRY(validate_tuning(*tuning));
     SKM_HIP(hipStreamSynchronize(ev->stream));  // queued work may use the buffers resized below
     ev->tuning = *tuning;
     ev->pr.lcc_chunks = tuning->lcc_ref_chunks;
@@ -1363,6 +1418,118 @@ SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning)
         ev->d_lcc_ref_partials = nullptr;
         ev->lcc_ref_partials_n = 0;
     }
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float *d_cp_aos, const float *d_cpw,
+                                int32_t repeats, skm_autotune_report *report)
+{
+    if (!ev || !d_cp_aos || !d_cpw) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    if (measure < SKM_MEASURE_MSE || measure > SKM_MEASURE_LCC) return fail(SKM_ERROR_INVALID_ARGUMENT, "bad measure");
+    if (ev->timing) return fail(SKM_ERROR_INVALID_ARGUMENT, "disable pass timing before tuning");
+    repeats = std::max(1, repeats);
+    hipStream_t s = ev->stream;
+    const skm_tuning start = ev->tuning;
+    std::vector<skm_tuning> cand;
+    if (measure == SKM_MEASURE_LCC && !ev->pr.exact) {
+        for (int c : {0, 1, 2, 4, 8, 16, 32, 64}) {
+            skm_tuning t = start;
+            t.lcc_ref_chunks = c;
+            cand.push_back(t);
+        }
+    } else if (measure == SKM_MEASURE_LCC) {
+        for (int g : {0, 4}) {
+            skm_tuning t = start;
+            t.gather_split = g;
+            cand.push_back(t);
+        }
+    } else {
+        for (int v : {0, 1, 4, 8, 16})
+            for (int g : {0, 4}) {
+                skm_tuning t = start;
+                t.value_split = v;
+                t.gather_split = g;
+                cand.push_back(t);
+            }
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    SKM_HIP(hipEventCreate(&a));
+    SKM_HIP(hipEventCreate(&b));
+    struct Events {
+        hipEvent_t &a, &b;
+        ~Events()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } events{a, b};
+    launch_pack_cp(ev->pr, d_cp_aos, d_cpw, 0, ev->d_cpc, s);
+    double best_ms = 0.0, default_ms = 0.0;
+    size_t best = 0;
+    for (size_t c = 0; c < cand.size(); ++c) {
+        SKM_TRY(skm_set_tuning(ev, &cand[c]));
+        SKM_TRY(run_eval(ev, measure, 0, ev->d_deriv));  // first use of the shape (allocations), untimed
+        std::vector<double> ms;
+        for (int r = 0; r < repeats; ++r) {
+            SKM_HIP(hipEventRecord(a, s));
+            SKM_TRY(run_eval(ev, measure, 0, ev->d_deriv));
+            SKM_HIP(hipEventRecord(b, s));
+            SKM_HIP(hipEventSynchronize(b));
+            float t = 0.0f;
+            SKM_HIP(hipEventElapsedTime(&t, a, b));
+            ms.push_back(t);
+        }
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[ms.size() / 2];
+        if (c == 0) default_ms = med;
+        if (c == 0 || med < best_ms) {
+            best_ms = med;
+            best = c;
+        }
+    }
+    SKM_TRY(skm_set_tuning(ev, &cand[best]));
+    tuning_store(shape_class(ev), cand[best]);
+    if (report) {
+        report->best = cand[best];
+        report->best_ms = best_ms;
+        report->first_ms = default_ms;
+        report->candidates = (int32_t)cand.size();
+    }
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_tuning_table_save(const char *path)
+{
+    if (!path) return fail(SKM_ERROR_INVALID_ARGUMENT, "null path");
+    std::ofstream f(path);
+    if (!f) return fail(SKM_ERROR_IO, std::string("cannot write ") + path);
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    f << "# skm launch shapes: shape class, lcc_ref_chunks, gather_split, value_split\n";
+    for (const auto &kv : g_tuning)
+        f << kv.first << " " << kv.second.lcc_ref_chunks << " " << kv.second.gather_split << " "
+          << kv.second.value_split << "\n";
+    return f ? SKM_SUCCESS : fail(SKM_ERROR_IO, std::string("cannot write ") + path);
+}
+
+SKM_API skm_status skm_tuning_table_load(const char *path)
+{
+    if (!path) return fail(SKM_ERROR_INVALID_ARGUMENT, "null path");
+    std::ifstream f(path);
+    if (!f) return fail(SKM_ERROR_IO, std::string("cannot read ") + path);
+    std::string line;
+    std::map<std::string, skm_tuning> read;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream in(line);
+        std::string key;
+        skm_tuning t{};
+        if (!(in >> key >> t.lcc_ref_chunks >> t.gather_split >> t.value_split))
+            return fail(SKM_ERROR_IO, "malformed tuning line: " + line);
+        if (validate_tuning(t) != SKM_SUCCESS) return fail(SKM_ERROR_IO, "launch shape out of range: " + line);
+        read[key] = t;
+    }
+    std::lock_guard<std::mutex> lock(g_tuning_mutex);
+    for (const auto &kv : read) g_tuning[kv.first] = kv.second;
     return SKM_SUCCESS;
 }
 
@@ -1467,15 +1634,22 @@ SKM_API skm_status skm_host_regularizers(int32_t dim, int64_t n_cp, const double
     return SKM_SUCCESS;
 }
 
-SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *metric,
-                                const skm_optimizer_options *opt, const float *cpw, float *params_soa,
-                                skm_iteration_record *records)
+namespace {
+
+skm_status opt_check(const skm_evaluator *ev, const skm_metric_options *metric, const skm_optimizer_options *opt,
+                     const float *cpw, const float *params_soa)
 {
     if (!ev || !metric || !opt || !cpw || !params_soa) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
     if (opt->iterations < 0) return fail(SKM_ERROR_INVALID_ARGUMENT, "iterations must be >= 0");
     if (opt->kind < 0 || opt->kind > 2) return fail(SKM_ERROR_INVALID_ARGUMENT, "bad optimizer kind");
     if (metric->reg_l1 < 0 || metric->reg_l21 < 0 || metric->reg_l2 < 0)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "Regularization weights must be positive.");
+    return SKM_SUCCESS;
+}
+
+// buffers, start point and state of a device optimisation, on the evaluator's stream
+skm_status opt_prepare(skm_evaluator *ev, const skm_optimizer_options *opt, const float *cpw, const float *params_soa)
+{
     Problem &pr = ev->pr;
     hipStream_t s = ev->stream;
     const size_t np = (size_t)pr.n_cp * pr.dim;
@@ -1494,53 +1668,34 @@ SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *met
     SKM_HIP(hipMemcpyAsync(ev->d_cpw_in, cpw, sizeof(float) * pr.n_cp, hipMemcpyHostToDevice, s));
     SKM_HIP(hipMemsetAsync(ev->d_prev_g, 0, sizeof(float) * np, s));
     SKM_HIP(hipMemsetAsync(ev->d_opt, 0, sizeof(OptState), s));
-    // one iteration: every value it needs lives on the device (parameters and
-    // their packed control points, optimizer state and iteration count,
-    // random stream), so the launches of one iteration serve every other one;
-    // the update packs the control points of the next evaluation
     launch_soa_to_cpc(pr, ev->d_params, ev->d_cpw_in, ev->d_cpc, s);
-    auto iteration = [&]() -> skm_status {
-        SKM_TRY(run_eval(ev, metric->measure, metric->do_resampling, ev->d_deriv));
-        launch_host_regs_device(pr, ev->d_params, metric->reg_l1, metric->reg_l21, metric->reg_l2,
-                                metric->use_negative_gradient, ev->d_deriv, ev->d_prev_g, ev->d_hreg_partials, s);
-        launch_opt_step(pr, opt->kind, opt->a, opt->A, opt->alpha, opt->fmax, opt->fmin, opt->omega,
-                        opt->orthant_projection, ev->d_params, ev->d_deriv, ev->d_prev_g, ev->d_opt, ev->d_sc,
-                        ev->d_hreg_partials, metric->reg_l1, metric->reg_l21, metric->reg_l2, ev->reg_w,
-                        ev->d_records, opt->return_best ? ev->d_best : nullptr, ev->d_cpw_in, ev->d_cpc, s);
-        return SKM_SUCCESS;
-    };
-    // with graph replay the first iteration runs kernel by kernel (it
-    // allocates what a measure needs on first use and draws the first
-    // offsets) and the rest replay a capture of the second one; pass timing
-    // needs the kernel-by-kernel launches
-    const bool replay = opt->graph && !ev->timing && opt->iterations > 2;
-    int it = 0;
-    for (; it < (replay ? 1 : opt->iterations); ++it) SKM_TRY(iteration());
-    if (replay) {
-        SKM_HIP(hipGetLastError());
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        SKM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const skm_status captured = iteration();
-        const hipError_t ended = hipStreamEndCapture(s, &graph);
-        auto release = [&]() {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
-        };
-        if (captured != SKM_SUCCESS) {
-            release();
-            return captured;
-        }
-        if (ended != hipSuccess) {
-            release();
-            return fail(SKM_ERROR_HIP, std::string("optimizer capture: ") + hipGetErrorString(ended));
-        }
-        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        for (; e == hipSuccess && it < opt->iterations; ++it) e = hipGraphLaunch(exec, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        release();
-        if (e != hipSuccess) return fail(SKM_ERROR_HIP, std::string("optimizer replay: ") + hipGetErrorString(e));
-    }
+    return SKM_SUCCESS;
+}
+
+// One iteration: every value it needs lives on the device (parameters and
+// their packed control points, optimizer state and iteration count, random
+// stream), so the launches of one iteration serve every other one; the
+// update packs the control points of the next evaluation.
+skm_status opt_iteration(skm_evaluator *ev, const skm_metric_options *metric, const skm_optimizer_options *opt)
+{
+    Problem &pr = ev->pr;
+    hipStream_t s = ev->stream;
+    SKM_TRY(run_eval(ev, metric->measure, metric->do_resampling, ev->d_deriv));
+    launch_host_regs_device(pr, ev->d_params, metric->reg_l1, metric->reg_l21, metric->reg_l2,
+                            metric->use_negative_gradient, ev->d_deriv, ev->d_prev_g, ev->d_hreg_partials, s);
+    launch_opt_step(pr, opt->kind, opt->a, opt->A, opt->alpha, opt->fmax, opt->fmin, opt->omega,
+                    opt->orthant_projection, ev->d_params, ev->d_deriv, ev->d_prev_g, ev->d_opt, ev->d_sc,
+                    ev->d_hreg_partials, metric->reg_l1, metric->reg_l21, metric->reg_l2, ev->reg_w, ev->d_records,
+                    opt->return_best ? ev->d_best : nullptr, ev->d_cpw_in, ev->d_cpc, s);
+    return SKM_SUCCESS;
+}
+
+// the result (best or last position) and the records, back to the host
+skm_status opt_finish(skm_evaluator *ev, const skm_optimizer_options *opt, float *params_soa,
+                      skm_iteration_record *records)
+{
+    hipStream_t s = ev->stream;
+    const size_t np = (size_t)ev->pr.n_cp * ev->pr.dim;
     SKM_HIP(hipGetLastError());
     // with return_best (and at least one evaluation) the lowest-value position
     const float *result = (opt->return_best && opt->iterations > 0) ? ev->d_best : ev->d_params;
@@ -1549,6 +1704,107 @@ SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *met
         SKM_HIP(hipMemcpyAsync(records, ev->d_records, sizeof(skm_iteration_record) * opt->iterations,
                                hipMemcpyDeviceToHost, s));
     SKM_HIP(hipStreamSynchronize(s));
+    return SKM_SUCCESS;
+}
+
+// Iterations first..iterations of every evaluator as one captured HIP graph
+// per step: the capture forks from the first evaluator's stream to every
+// other one and joins back, so the graph holds one branch per evaluator; one
+// hipGraphLaunch then runs one optimizer step of all of them.
+skm_status opt_replay(skm_evaluator *const *evs, int n, const skm_metric_options *metric,
+                      const skm_optimizer_options *opt, int first)
+{
+    hipStream_t s0 = evs[0]->stream;
+    SKM_HIP(hipGetLastError());
+    std::vector<hipEvent_t> events;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto release = [&]() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    };
+    for (int i = 0; i < 2 * n; ++i) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            release();
+            return fail(SKM_ERROR_HIP, "graph events");
+        }
+        events.push_back(e);
+    }
+    // the evaluators' streams must be idle and after s0's work before the capture joins them
+    for (int i = 1; i < n; ++i)
+        if (evs[i]->stream != s0 && hipEventRecord(events[n + i], evs[i]->stream) == hipSuccess)
+            (void)hipStreamWaitEvent(s0, events[n + i], 0);
+    SKM_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    skm_status captured = SKM_SUCCESS;
+    hipError_t e = hipEventRecord(events[0], s0);
+    for (int i = 0; i < n && e == hipSuccess && captured == SKM_SUCCESS; ++i) {
+        hipStream_t si = evs[i]->stream;
+        if (si != s0) e = hipStreamWaitEvent(si, events[0], 0);
+        if (e == hipSuccess) captured = opt_iteration(evs[i], metric, opt);
+        if (e == hipSuccess && captured == SKM_SUCCESS && si != s0) {
+            e = hipEventRecord(events[n + i], si);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s0, events[n + i], 0);
+        }
+    }
+    const hipError_t ended = hipStreamEndCapture(s0, &graph);
+    if (captured != SKM_SUCCESS) {
+        release();
+        return captured;
+    }
+    if (e != hipSuccess || ended != hipSuccess) {
+        release();
+        return fail(SKM_ERROR_HIP, std::string("optimizer capture: ") + hipGetErrorString(e != hipSuccess ? e : ended));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    for (int it = first; e == hipSuccess && it < opt->iterations; ++it) e = hipGraphLaunch(exec, s0);
+    if (e == hipSuccess) e = hipStreamSynchronize(s0);
+    release();
+    if (e != hipSuccess) return fail(SKM_ERROR_HIP, std::string("optimizer replay: ") + hipGetErrorString(e));
+    return SKM_SUCCESS;
+}
+
+}  // namespace
+
+SKM_API skm_status skm_optimize(skm_evaluator *ev, const skm_metric_options *metric,
+                                const skm_optimizer_options *opt, const float *cpw, float *params_soa,
+                                skm_iteration_record *records)
+{
+    SKM_TRY(opt_check(ev, metric, opt, cpw, params_soa));
+    SKM_TRY(opt_prepare(ev, opt, cpw, params_soa));
+    // with graph replay the first iteration runs kernel by kernel (it
+    // allocates what a measure needs on first use and draws the first
+    // offsets) and the rest replay a capture of the second one; pass timing
+    // needs the kernel-by-kernel launches
+    const bool replay = opt->graph && !ev->timing && opt->iterations > 2;
+    int it = 0;
+    for (; it < (replay ? 1 : opt->iterations); ++it) SKM_TRY(opt_iteration(ev, metric, opt));
+    if (replay) SKM_TRY(opt_replay(&ev, 1, metric, opt, 1));
+    return opt_finish(ev, opt, params_soa, records);
+}
+
+SKM_API skm_status skm_optimize_batch(skm_evaluator *const *evaluators, int32_t n, const skm_metric_options *metric,
+                                      const skm_optimizer_options *opt, const float *const *cpw,
+                                      float *const *params_soa, skm_iteration_record *const *records)
+{
+    if (n < 1 || !evaluators || !cpw || !params_soa) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    for (int32_t i = 0; i < n; ++i) {
+        SKM_TRY(opt_check(evaluators[i], metric, opt, cpw[i], params_soa[i]));
+        for (int32_t k = 0; k < i; ++k)
+            if (evaluators[k] == evaluators[i]) return fail(SKM_ERROR_INVALID_ARGUMENT, "an evaluator appears twice");
+    }
+    for (int32_t i = 0; i < n; ++i) SKM_TRY(opt_prepare(evaluators[i], opt, cpw[i], params_soa[i]));
+    bool timing = false;
+    for (int32_t i = 0; i < n; ++i) timing = timing || evaluators[i]->timing;
+    const bool replay = opt->graph && !timing && opt->iterations > 1;
+    // the first iteration kernel by kernel (first-use allocations, first offsets)
+    int it = 0;
+    for (; it < (replay ? 1 : opt->iterations); ++it)
+        for (int32_t i = 0; i < n; ++i) SKM_TRY(opt_iteration(evaluators[i], metric, opt));
+    if (replay) SKM_TRY(opt_replay(evaluators, n, metric, opt, 1));
+    for (int32_t i = 0; i < n; ++i)
+        SKM_TRY(opt_finish(evaluators[i], opt, params_soa[i], records ? records[i] : nullptr));
     return SKM_SUCCESS;
 }
 

@@ -13,6 +13,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <condition_variable>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -339,6 +341,109 @@ skm_status level_value(skm_evaluator *ev, int metric, const skm_scale_params &P,
     return SKM_SUCCESS;
 }
 
+// Registrations in lockstep (skm_register_batch_lockstep): each one, when it
+// reaches a level's optimisation, waits until every registration still
+// running has reached its own; the last to arrive runs all their optimisers
+// at once (skm_optimize_batch: one captured graph launch per step for all),
+// grouped by identical options.  A registration that ends, or fails, leaves
+// the group, so that the others never wait for it.
+struct Lockstep {
+    struct Job {
+        skm_evaluator *ev;
+        skm_metric_options mo;
+        skm_optimizer_options oo;
+        const float *cpw;
+        float *params;
+        skm_iteration_record *rec;
+        skm_status st = SKM_SUCCESS;
+        std::string msg;
+    };
+    std::mutex m;
+    std::condition_variable cv;
+    int active = 0;
+    long long generation = 0;
+    std::vector<Job *> waiting;
+
+    static bool same(const Job &a, const Job &b)
+    {
+        const skm_metric_options &x = a.mo, &y = b.mo;
+        const skm_optimizer_options &p = a.oo, &q = b.oo;
+        return x.measure == y.measure && x.reg_l1 == y.reg_l1 && x.reg_l21 == y.reg_l21 && x.reg_l2 == y.reg_l2 &&
+               x.use_negative_gradient == y.use_negative_gradient && x.do_resampling == y.do_resampling &&
+               p.kind == q.kind && p.a == q.a && p.A == q.A && p.alpha == q.alpha && p.fmax == q.fmax &&
+               p.fmin == q.fmin && p.omega == q.omega && p.orthant_projection == q.orthant_projection &&
+               p.iterations == q.iterations && p.return_best == q.return_best && p.graph == q.graph;
+    }
+
+    // runs every waiting job; called with the lock held, releases it meanwhile
+    void run(std::unique_lock<std::mutex> &lk)
+    {
+        std::vector<Job *> jobs;
+        jobs.swap(waiting);
+        lk.unlock();
+        std::vector<bool> done(jobs.size(), false);
+        for (size_t i = 0; i < jobs.size(); ++i) {
+            if (done[i]) continue;
+            std::vector<Job *> group;
+            for (size_t k = i; k < jobs.size(); ++k)
+                if (!done[k] && same(*jobs[i], *jobs[k])) {
+                    group.push_back(jobs[k]);
+                    done[k] = true;
+                }
+            std::vector<skm_evaluator *> evs;
+            std::vector<const float *> cpw;
+            std::vector<float *> params;
+            std::vector<skm_iteration_record *> rec;
+            for (Job *j : group) {
+                evs.push_back(j->ev);
+                cpw.push_back(j->cpw);
+                params.push_back(j->params);
+                rec.push_back(j->rec);
+            }
+            const skm_status st = skm_optimize_batch(evs.data(), (int32_t)evs.size(), &group[0]->mo, &group[0]->oo,
+                                                     cpw.data(), params.data(), rec.data());
+            const std::string msg = st != SKM_SUCCESS ? std::string(skm_last_error()) : std::string();
+            for (Job *j : group) {
+                j->st = st;
+                j->msg = msg;
+            }
+        }
+        lk.lock();
+        ++generation;
+        cv.notify_all();
+    }
+
+    skm_status optimize(Job &job)
+    {
+        std::unique_lock<std::mutex> lk(m);
+        waiting.push_back(&job);
+        const long long gen = generation;
+        if ((int)waiting.size() == active) run(lk);
+        else cv.wait(lk, [&] { return generation != gen; });
+        lk.unlock();
+        if (job.st != SKM_SUCCESS) return set_last_error(job.st, job.msg);
+        return SKM_SUCCESS;
+    }
+
+    void leave()
+    {
+        std::unique_lock<std::mutex> lk(m);
+        --active;
+        if (!waiting.empty() && (int)waiting.size() == active) run(lk);
+    }
+};
+
+struct LockstepMember {
+    Lockstep *g;
+    ~LockstepMember()
+    {
+        if (g) g->leave();
+    }
+};
+
+skm_status register_impl(const skm_registration_desc *desc, float *displacement_out, skm_scale_report *reports,
+                         Lockstep *group);
+
 }  // namespace
 
 extern "C" {
@@ -384,6 +489,17 @@ SKM_API void skm_registration_desc_init(skm_registration_desc *d)
 SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displacement_out,
                                 skm_scale_report *reports)
 {
+    return register_impl(desc, displacement_out, reports, nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+
+skm_status register_impl(const skm_registration_desc *desc, float *displacement_out, skm_scale_report *reports,
+                         Lockstep *group)
+{
+    LockstepMember member{group};
     if (!desc || !displacement_out) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "null argument");
     const int dim = desc->dim;
     if (dim != 2 && dim != 3) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "dim must be 2 or 3");
@@ -579,6 +695,17 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
             SKM_R_TRY(skm_set_regularizer_bilateral(ev, P.reg_bilateral, P.reg_bilateral_spatial_sigma,
                                                     P.reg_bilateral_intensity_sigma));
         if (has_prev) SKM_R_TRY(skm::set_displacement_field_device(ev, &desc->target, d_prev.p));
+        if (desc->autotune) {
+            // launch shapes of this level's shape class, tuned once per process
+            skm_tuning t{};
+            SKM_R_TRY(skm_get_tuning(ev, &t));
+            if (t.lcc_ref_chunks == 0 && t.gather_split == 0 && t.value_split == 0) {
+                std::vector<float> zero((size_t)n_cp * dim, 0.0f), one((size_t)n_cp, 1.0f);
+                SKM_R_TRY(d_cp.upload(zero.data(), zero.size(), ss));
+                SKM_R_TRY(d_cpw.upload(one.data(), one.size(), ss));
+                SKM_R_TRY(skm_autotune(ev, desc->metric, d_cp.p, d_cpw.p, 3, nullptr));
+            }
+        }
         rep.ms_setup = ms_since(t_setup);
 
         // optimisation (SKMReg.cxx:375-430)
@@ -622,7 +749,13 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
             }
         }
         std::vector<skm_iteration_record> rec(std::max(1, P.iterations));
-        SKM_R_TRY(skm_optimize(ev, &mo, &oo, cpw.data(), params.data(), rec.data()));
+        if (group) {
+            oo.graph = 1;  // one graph launch per step for every registration of the group
+            Lockstep::Job job{ev, mo, oo, cpw.data(), params.data(), rec.data()};
+            SKM_R_TRY(group->optimize(job));
+        } else {
+            SKM_R_TRY(skm_optimize(ev, &mo, &oo, cpw.data(), params.data(), rec.data()));
+        }
         rep.iterations = P.iterations;
         if (P.iterations > 0) {
             int best = 0;
@@ -694,7 +827,7 @@ SKM_API skm_status skm_register(const skm_registration_desc *desc, float *displa
     return SKM_SUCCESS;
 }
 
-}  // extern "C"
+}  // namespace
 
 // Batched registration: a shared counter hands out the registrations to the
 // worker threads in index order; each worker makes the caller's device
@@ -745,6 +878,62 @@ SKM_API skm_status skm_register_batch(const skm_registration_desc *descs, int32_
     worker();
     for (auto &t : pool) t.join();
     if (next.load() < n) return set_last_error(SKM_ERROR_HIP, "hipSetDevice failed on the worker threads");
+    if (statuses)
+        for (int32_t i = 0; i < n; ++i) statuses[i] = st[i];
+    for (int32_t i = 0; i < n; ++i)
+        if (st[i] != SKM_SUCCESS) return set_last_error(st[i], "registration " + std::to_string(i) + ": " + msg[i]);
+    return SKM_SUCCESS;
+}
+
+// Lockstep batch: one thread per registration, all in flight; every level's
+// optimisers run together through the group (one graph launch per optimizer
+// step for all registrations with the same options).
+SKM_API skm_status skm_register_batch_lockstep(const skm_registration_desc *descs, int32_t n,
+                                               float *const *displacements_out, skm_scale_report *const *reports,
+                                               skm_status *statuses)
+{
+    if (n < 0) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "n must be >= 0");
+    if (n == 0) return SKM_SUCCESS;
+    if (!descs || !displacements_out) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    for (int32_t i = 0; i < n; ++i)
+        if (!displacements_out[i])
+            return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "displacements_out[" + std::to_string(i) + "] is null");
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return set_last_error(SKM_ERROR_HIP, "hipGetDevice failed");
+    std::vector<skm_status> st((size_t)n, SKM_SUCCESS);
+    std::vector<std::string> msg((size_t)n);
+    Lockstep group;
+    group.active = n;
+    auto one = [&](int32_t i) {
+        if (hipSetDevice(device) != hipSuccess) {
+            st[i] = SKM_ERROR_HIP;
+            msg[i] = "hipSetDevice failed";
+            group.leave();
+            return;
+        }
+        try {
+            st[i] = register_impl(&descs[i], displacements_out[i], reports ? reports[i] : nullptr, &group);
+            if (st[i] != SKM_SUCCESS) msg[i] = skm_last_error();
+        } catch (const std::bad_alloc &) {
+            st[i] = SKM_ERROR_OUT_OF_MEMORY;
+            msg[i] = "host allocation failed";
+        } catch (const std::exception &e) {
+            st[i] = SKM_ERROR_HIP;
+            msg[i] = e.what();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int32_t i = 1; i < n; ++i) {
+        try {
+            pool.emplace_back(one, i);
+        } catch (const std::system_error &) {
+            st[i] = SKM_ERROR_HIP;
+            msg[i] = "could not start a thread";
+            group.leave();  // the others do not wait for it
+        }
+    }
+    one(0);
+    for (auto &t : pool) t.join();
     if (statuses)
         for (int32_t i = 0; i < n; ++i) statuses[i] = st[i];
     for (int32_t i = 0; i < n; ++i)

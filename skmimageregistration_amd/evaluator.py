@@ -278,6 +278,15 @@ class GpuKernelEvaluator:
         t = _capi.SkmTuning(int(lcc_ref_chunks), int(gather_split), int(value_split))
         check(self._lib.skm_set_tuning(self._h, ctypes.byref(t)))
 
+    def autotune_device(self, metric, d_cp_aos, d_cpw, repeats=3):
+        """skm_autotune at device control points (ints); returns the report as a dict."""
+        r = _capi.SkmAutotuneReport()
+        check(self._lib.skm_autotune(self._h, int(metric), ctypes.c_void_p(d_cp_aos), ctypes.c_void_p(d_cpw),
+                                     int(repeats), ctypes.byref(r)))
+        b = r.best
+        return {"lcc_ref_chunks": b.lcc_ref_chunks, "gather_split": b.gather_split, "value_split": b.value_split,
+                "best_ms": r.best_ms, "first_ms": r.first_ms, "candidates": r.candidates}
+
     def get_tuning(self):
         t = _capi.SkmTuning()
         check(self._lib.skm_get_tuning(self._h, ctypes.byref(t)))
@@ -297,14 +306,65 @@ class GpuKernelEvaluator:
                  cpwData=None, return_best=False, graph=False):
         params = f32(params_soa).copy()
         cpw = f32(cpwData if cpwData is not None else np.ones(self.n_cp, np.float32))
-        mo = _capi.SkmMetricOptions(int(metric), reg_l1, reg_l21, reg_l2, int(bool(use_negative_gradient)),
-                                    int(bool(do_resampling)))
-        oo = _capi.SkmOptimizerOptions(int(kind), a, A, alpha, fmax, fmin, omega, int(bool(orthant_projection)),
-                                       int(iterations), int(bool(return_best)), int(bool(graph)))
+        mo, oo = _optimizer_options(metric, reg_l1, reg_l21, reg_l2, use_negative_gradient, do_resampling, kind, a,
+                                    A, alpha, fmax, fmin, omega, orthant_projection, iterations, return_best, graph)
         recs = (_capi.SkmIterationRecord * max(1, iterations))()
         check(self._lib.skm_optimize(self._h, ctypes.byref(mo), ctypes.byref(oo), fptr(cpw), fptr(params), recs))
-        records = [(r.value, r.measure, r.learning_rate, r.num_pixels_counted) for r in recs[:iterations]]
-        return params, records
+        return params, _records(recs, iterations)
+
+
+def _optimizer_options(metric, reg_l1, reg_l21, reg_l2, use_negative_gradient, do_resampling, kind, a, A, alpha,
+                       fmax, fmin, omega, orthant_projection, iterations, return_best, graph):
+    mo = _capi.SkmMetricOptions(int(metric), reg_l1, reg_l21, reg_l2, int(bool(use_negative_gradient)),
+                                int(bool(do_resampling)))
+    oo = _capi.SkmOptimizerOptions(int(kind), a, A, alpha, fmax, fmin, omega, int(bool(orthant_projection)),
+                                   int(iterations), int(bool(return_best)), int(bool(graph)))
+    return mo, oo
+
+
+def _records(recs, iterations):
+    return [(r.value, r.measure, r.learning_rate, r.num_pixels_counted) for r in recs[:iterations]]
+
+
+def optimize_batch(evaluators, params_soa, metric=MEASURE.MSE, reg_l1=0.0, reg_l21=0.0, reg_l2=0.0,
+                   use_negative_gradient=True, do_resampling=True, kind=_capi.SKM_OPT_ADAPTIVE, a=0.1, A=20.0,
+                   alpha=1.0, fmax=1e7, fmin=-1e6, omega=1.0, orthant_projection=False, iterations=100,
+                   cpwData=None, return_best=False, graph=False):
+    """GpuKernelEvaluator.optimize of several evaluators at once through
+    skm_optimize_batch (no counterpart in the reference): with graph=True
+    every iteration after the first is one launch of one captured graph
+    holding an optimizer step of every evaluator.  params_soa and cpwData
+    (optional) are per-evaluator sequences; returns [(params, records)]."""
+    evaluators = list(evaluators)
+    n = len(evaluators)
+    if n == 0:
+        return []
+    lib = evaluators[0]._lib
+    params = [f32(p).copy() for p in params_soa]
+    cpws = [f32(c) for c in cpwData] if cpwData is not None else [np.ones(ev.n_cp, np.float32) for ev in evaluators]
+    if len(params) != n or len(cpws) != n:
+        raise ValueError("one start point (and cpw) per evaluator")
+    mo, oo = _optimizer_options(metric, reg_l1, reg_l21, reg_l2, use_negative_gradient, do_resampling, kind, a, A,
+                                alpha, fmax, fmin, omega, orthant_projection, iterations, return_best, graph)
+    recs = [(_capi.SkmIterationRecord * max(1, iterations))() for _ in range(n)]
+    hs = (ctypes.c_void_p * n)(*[ev._h for ev in evaluators])
+    cps = (_capi.c_float_p * n)(*[fptr(c) for c in cpws])
+    pps = (_capi.c_float_p * n)(*[fptr(p) for p in params])
+    rps = (ctypes.POINTER(_capi.SkmIterationRecord) * n)(
+        *[ctypes.cast(r, ctypes.POINTER(_capi.SkmIterationRecord)) for r in recs])
+    check(lib.skm_optimize_batch(hs, n, ctypes.byref(mo), ctypes.byref(oo), cps, pps, rps))
+    return [(p, _records(r, iterations)) for p, r in zip(params, recs)]
+
+
+def tuning_table_save(path):
+    """Writes the process's launch-shape table (skm_autotune results) to path."""
+    check(_capi.load().skm_tuning_table_save(str(path).encode()))
+
+
+def tuning_table_load(path):
+    """Merges a saved launch-shape table into the process's; evaluators
+    created afterwards with a listed shape class start with its shapes."""
+    check(_capi.load().skm_tuning_table_load(str(path).encode()))
 
 
 def control_point_grid(image, spacing):
@@ -442,8 +502,10 @@ class RegistrationConfig:
     whose last entry repeats (CURRENT, src/SKMReg.cxx:48-49).  One key has no
     SKMReg counterpart: exact_gradient (1: derivative windows covering the
     whole kernel support at every level; 0, the default: the reference's),
-    and the bilateral regularizer reg_bilateral with its
-    reg_bilateral_spatial_sigma / reg_bilateral_intensity_sigma."""
+    the bilateral regularizer reg_bilateral with its
+    reg_bilateral_spatial_sigma / reg_bilateral_intensity_sigma, and
+    autotune (1: each level's launch shapes from skm_autotune, once per
+    shape class and process)."""
 
     STRING_KEYS = ("metric", "reference_filename", "target_filename", "reference_landmarks", "target_landmarks",
                    "weight_filename", "weight_tensor_filename", "temp_directory")
@@ -460,6 +522,7 @@ class RegistrationConfig:
         "sampling_rate": [5, 4, 3], "neighborhood_sampling": [15, 12, 9], "intensity_thresholds": [-1500, 3500],
         "smoothing_variance": [2, 1, 0], "metric": "mse", "exact_gradient": [0],
         "reg_bilateral": [0, 0, 0], "reg_bilateral_spatial_sigma": [12.0], "reg_bilateral_intensity_sigma": [0.1],
+        "autotune": [0],
     }
 
     def __init__(self, **overrides):
@@ -522,6 +585,7 @@ def _registration_desc(lib, reference, target, config, weight_image, weight_tens
         p.smoothing_variance = float(cfg.CURRENT("smoothing_variance", s))
     d.cp_spacing_scaler = float(cfg.CURRENT("cp_spacing_scaler", 0))
     d.exact_gradient = int(bool(cfg.CURRENT("exact_gradient", 0)))
+    d.autotune = int(bool(cfg.CURRENT("autotune", 0)))
     d.metric = {"mse": MEASURE.MSE, "ncc": MEASURE.NCC, "lcc": MEASURE.LCC}[cfg["metric"]]
     th = cfg["intensity_thresholds"]
     d.use_intensity_thresholds = int(len(th) == 2)
@@ -623,14 +687,17 @@ def register_multiscale(reference, target, config=None, weight_image=None, weigh
     return out, _reports(reps, target.dim)
 
 
-def register_batch(pairs, config=None, concurrency=4, initial_displacements=None, initial_guard=False):
+def register_batch(pairs, config=None, concurrency=4, initial_displacements=None, initial_guard=False,
+                   lockstep=False):
     """Independent registrations through skm_register_batch (no counterpart
     in the reference): pairs is a sequence of (reference, target) Images, all
     registered with one configuration, up to `concurrency` at once on the
     current device.  Each result equals register_multiscale's for its pair.
     Returns a list of (displacement, per-level report dicts);
     initial_displacements is an optional per-pair sequence (see
-    register_multiscale)."""
+    register_multiscale).  lockstep=True runs skm_register_batch_lockstep
+    instead: all pairs in flight, each optimizer step of the whole batch one
+    captured-graph launch (concurrency is then ignored)."""
     lib = _capi.load()
     jobs = [_registration_desc(lib, r, t, config, None, None, None, None, None, None, None) for r, t in pairs]
     if initial_displacements is not None:
@@ -643,7 +710,10 @@ def register_batch(pairs, config=None, concurrency=4, initial_displacements=None
     outs = (_capi.c_float_p * max(n, 1))(*[fptr(o) for _, _, o, _ in jobs])
     reps = (ctypes.POINTER(_capi.SkmScaleReport) * max(n, 1))(
         *[ctypes.cast(r, ctypes.POINTER(_capi.SkmScaleReport)) for _, _, _, r in jobs])
-    check(lib.skm_register_batch(descs, n, int(concurrency), outs, reps, None))
+    if lockstep:
+        check(lib.skm_register_batch_lockstep(descs, n, outs, reps, None))
+    else:
+        check(lib.skm_register_batch(descs, n, int(concurrency), outs, reps, None))
     return [(o, _reports(r, t.dim)) for (_, _, o, r), (_, t) in zip(jobs, pairs)]
 
 

@@ -3,7 +3,9 @@
 target, 2 levels), with several registrations in flight at once.  Every
 frame's displacement and per-level report must equal those of the same
 registration run alone through skm_register, bit for bit (the kernels sum in
-a fixed order and the frames share no device state)."""
+a fixed order and the frames share no device state).  The "lockstep" case
+runs skm_register_batch_lockstep: all frames in flight, each optimizer step
+of the batch one launch of one captured graph."""
 import numpy as np
 import pytest
 
@@ -22,13 +24,16 @@ def _config():
                               reg_rd=[1e-5], metric="ncc")
 
 
-@pytest.mark.parametrize("concurrency", [1, 3, 8])
+@pytest.mark.parametrize("concurrency", [1, 3, 8, "lockstep"])
 def test_batch_equals_one_at_a_time(concurrency):
     frames, _ = synthetic.time_series_3d(n_frames=6, max_disp=3.0, sizes=SIZES)
     pairs = [(Image(frames[0]), Image(f)) for f in frames[1:]]
     cfg = _config()
     print(f"batch of {len(pairs)}, concurrency {concurrency}, sizes {[f.shape for f in frames]}", flush=True)
-    got = register_batch(pairs, cfg, concurrency=concurrency)
+    if concurrency == "lockstep":
+        got = register_batch(pairs, cfg, lockstep=True)
+    else:
+        got = register_batch(pairs, cfg, concurrency=concurrency)
     assert len(got) == len(pairs)
     for i, ((ref, tar), (df, reps)) in enumerate(zip(pairs, got)):
         want_df, want_reps = register_multiscale(ref, tar, cfg)
