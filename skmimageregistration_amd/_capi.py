@@ -76,7 +76,8 @@ class SkmPassTimes(ctypes.Structure):
 
 class SkmTuning(ctypes.Structure):
     _fields_ = [("lcc_ref_chunks", ctypes.c_int32), ("gather_split", ctypes.c_int32),
-                ("value_split", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("value_split", ctypes.c_int32), ("gather_regions", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class SkmAutotuneReport(ctypes.Structure):

@@ -989,6 +989,12 @@ void tuning_store(const std::string &key, const skm_tuning &t)
 
 }  // namespace
 
+bool skm::tuning_recorded(const skm_evaluator *ev)
+{
+    skm_tuning t;
+    return tuning_lookup(shape_class(ev), &t);
+}
+
 skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out, hipStream_t stream)
 {
     if (!desc || !out) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
@@ -1154,6 +1160,7 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         pr.lcc_chunks = t.lcc_ref_chunks;
         pr.gather_split = t.gather_split;
         pr.value_split = t.value_split;
+        pr.gather_regions = t.gather_regions;
     }
     *out = ev;
     return SKM_SUCCESS;
@@ -1398,6 +1405,8 @@ skm_status validate_tuning(const skm_tuning &t)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "gather_split must be 0 (automatic), 4 or 8");
     if (t.value_split != 0 && t.value_split != 1 && t.value_split != 4 && t.value_split != 8 && t.value_split != 16)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "value_split must be 0 (automatic), 1, 4, 8 or 16");
+    if (t.gather_regions != 0 && t.gather_regions != 1 && t.gather_regions != 8)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "gather_regions must be 0 (automatic), 1 or 8");
     return SKM_SUCCESS;
 }
 
@@ -1412,6 +1421,7 @@ SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning)
     ev->pr.lcc_chunks = tuning->lcc_ref_chunks;
     ev->pr.gather_split = tuning->gather_split;
     ev->pr.value_split = tuning->value_split;
+    ev->pr.gather_regions = tuning->gather_regions;
     const long long np = lcc_reference_partials(ev->pr);
     if (ev->d_lcc_ref_partials && np > ev->lcc_ref_partials_n) {
         device_release(ev->d_lcc_ref_partials);
@@ -1429,28 +1439,22 @@ SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float 
     if (ev->timing) return fail(SKM_ERROR_INVALID_ARGUMENT, "disable pass timing before tuning");
     repeats = std::max(1, repeats);
     hipStream_t s = ev->stream;
-    const skm_tuning start = ev->tuning;
-    std::vector<skm_tuning> cand;
+    // axes of the search, each tuned with the others at their best so far
+    // (the value pass and the gather are separate kernels: their shapes do
+    // not interact beyond the order of the evaluation's launches)
+    using Field = int32_t skm_tuning::*;
+    struct Axis {
+        Field field;
+        std::vector<int> values;  // the first is the automatic choice (0)
+    };
+    std::vector<Axis> axes;
+    const bool d3 = ev->pr.dim == 3;
     if (measure == SKM_MEASURE_LCC && !ev->pr.exact) {
-        for (int c : {0, 1, 2, 4, 8, 16, 32, 64}) {
-            skm_tuning t = start;
-            t.lcc_ref_chunks = c;
-            cand.push_back(t);
-        }
-    } else if (measure == SKM_MEASURE_LCC) {
-        for (int g : {0, 4}) {
-            skm_tuning t = start;
-            t.gather_split = g;
-            cand.push_back(t);
-        }
+        axes.push_back({&skm_tuning::lcc_ref_chunks, {0, 1, 2, 4, 8, 16, 32, 64}});
     } else {
-        for (int v : {0, 1, 4, 8, 16})
-            for (int g : {0, 4}) {
-                skm_tuning t = start;
-                t.value_split = v;
-                t.gather_split = g;
-                cand.push_back(t);
-            }
+        if (measure != SKM_MEASURE_LCC) axes.push_back({&skm_tuning::value_split, {0, 1, 4, 8, 16}});
+        axes.push_back({&skm_tuning::gather_split, {0, 4}});
+        if (d3) axes.push_back({&skm_tuning::gather_regions, {0, 8}});
     }
     hipEvent_t a = nullptr, b = nullptr;
     SKM_HIP(hipEventCreate(&a));
@@ -1464,10 +1468,8 @@ SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float 
         }
     } events{a, b};
     launch_pack_cp(ev->pr, d_cp_aos, d_cpw, 0, ev->d_cpc, s);
-    double best_ms = 0.0, default_ms = 0.0;
-    size_t best = 0;
-    for (size_t c = 0; c < cand.size(); ++c) {
-        SKM_TRY(skm_set_tuning(ev, &cand[c]));
+    auto time_of = [&](const skm_tuning &t, double *out) -> skm_status {
+        SKM_TRY(skm_set_tuning(ev, &t));
         SKM_TRY(run_eval(ev, measure, 0, ev->d_deriv));  // first use of the shape (allocations), untimed
         std::vector<double> ms;
         for (int r = 0; r < repeats; ++r) {
@@ -1475,25 +1477,40 @@ SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float 
             SKM_TRY(run_eval(ev, measure, 0, ev->d_deriv));
             SKM_HIP(hipEventRecord(b, s));
             SKM_HIP(hipEventSynchronize(b));
-            float t = 0.0f;
-            SKM_HIP(hipEventElapsedTime(&t, a, b));
-            ms.push_back(t);
+            float t_ms = 0.0f;
+            SKM_HIP(hipEventElapsedTime(&t_ms, a, b));
+            ms.push_back(t_ms);
         }
         std::sort(ms.begin(), ms.end());
-        const double med = ms[ms.size() / 2];
-        if (c == 0) default_ms = med;
-        if (c == 0 || med < best_ms) {
-            best_ms = med;
-            best = c;
+        *out = ms[ms.size() / 2];
+        return SKM_SUCCESS;
+    };
+    skm_tuning best = ev->tuning;
+    for (const Axis &ax : axes) best.*(ax.field) = 0;
+    double best_ms = 0.0;
+    SKM_TRY(time_of(best, &best_ms));
+    const double default_ms = best_ms;
+    int candidates = 1;
+    for (const Axis &ax : axes) {
+        for (size_t k = 1; k < ax.values.size(); ++k) {
+            skm_tuning t = best;
+            t.*(ax.field) = ax.values[k];
+            double ms = 0.0;
+            SKM_TRY(time_of(t, &ms));
+            ++candidates;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
         }
     }
-    SKM_TRY(skm_set_tuning(ev, &cand[best]));
-    tuning_store(shape_class(ev), cand[best]);
+    SKM_TRY(skm_set_tuning(ev, &best));
+    tuning_store(shape_class(ev), best);
     if (report) {
-        report->best = cand[best];
+        report->best = best;
         report->best_ms = best_ms;
         report->first_ms = default_ms;
-        report->candidates = (int32_t)cand.size();
+        report->candidates = candidates;
     }
     return SKM_SUCCESS;
 }
@@ -1504,10 +1521,10 @@ SKM_API skm_status skm_tuning_table_save(const char *path)
     std::ofstream f(path);
     if (!f) return fail(SKM_ERROR_IO, std::string("cannot write ") + path);
     std::lock_guard<std::mutex> lock(g_tuning_mutex);
-    f << "# skm launch shapes: shape class, lcc_ref_chunks, gather_split, value_split\n";
+    f << "# skm launch shapes: shape class, lcc_ref_chunks, gather_split, value_split, gather_regions\n";
     for (const auto &kv : g_tuning)
         f << kv.first << " " << kv.second.lcc_ref_chunks << " " << kv.second.gather_split << " "
-          << kv.second.value_split << "\n";
+          << kv.second.value_split << " " << kv.second.gather_regions << "\n";
     return f ? SKM_SUCCESS : fail(SKM_ERROR_IO, std::string("cannot write ") + path);
 }
 
@@ -1523,7 +1540,7 @@ SKM_API skm_status skm_tuning_table_load(const char *path)
         std::istringstream in(line);
         std::string key;
         skm_tuning t{};
-        if (!(in >> key >> t.lcc_ref_chunks >> t.gather_split >> t.value_split))
+        if (!(in >> key >> t.lcc_ref_chunks >> t.gather_split >> t.value_split >> t.gather_regions))
             return fail(SKM_ERROR_IO, "malformed tuning line: " + line);
         if (validate_tuning(t) != SKM_SUCCESS) return fail(SKM_ERROR_IO, "launch shape out of range: " + line);
         read[key] = t;

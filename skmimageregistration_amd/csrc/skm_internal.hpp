@@ -55,6 +55,7 @@ struct Problem {
     int lcc_chunks;            // reference-window LCC derivative: workgroups per cp tile (0: automatic)
     int gather_split;          // derivative gather: sample planes per slab, 4 or 8 (0: automatic, 8)
     int value_split;           // value pass: waves per sample brick, 1/4/8/16 (0: automatic)
+    int gather_regions;        // 3D derivative gather: 8 XCD regions of cp tiles, or 1 (0: automatic, 1)
     float lam_cp_max;          // largest lam over the control points
     float lam_w_max;           // largest lam over the points a sample can look up (weight image, outside)
 };
@@ -117,6 +118,8 @@ skm_status fit_coefficients_device(skm_evaluator *ev, const float *d_u_aos, int 
                                    float *d_c_soa);
 // skm_set_displacement_field from a device array
 skm_status set_displacement_field_device(skm_evaluator *ev, const skm_grid *grid, const float *d_df_aos);
+// whether the process's launch-shape table holds this evaluator's shape class
+bool tuning_recorded(const skm_evaluator *ev);
 skm_status set_displacement_field_impl(skm_evaluator *ev, const skm_grid *grid, const float *df_aos, bool device);
 // GaussianSmoothing / ComputeAnisotropicTensor on device arrays (skm_tensor.hip),
 // on `stream`, complete on return

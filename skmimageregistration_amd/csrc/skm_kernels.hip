@@ -459,6 +459,16 @@ __host__ __device__ __forceinline__ int gather_slab(const Problem &pr)
     return DIM == 3 ? (pr.gather_split == 4 ? 4 : 8) : max(pr.fixed.n[1], 1);
 }
 
+// Workgroup -> (slab, control-point tile) of the 3D derivative gather.  With
+// regions == 8 the tiles are split into eight regions of the (x, y) tile
+// plane, rx x (8 / rx) of cw x ch tiles each, and workgroup b works in region
+// b % 8: workgroups reach the eight XCDs in turn, so each XCD's L2 serves one
+// region's samples.  Within a region (and without regions) the order is
+// slab-major, so the workgroups in flight share a few sample planes.
+struct GatherMap {
+    int regions, rx, cw, ch, cap;  // cap: tiles per region (padded), cw * ch * tiles along z
+};
+
 // Union of the reference windows of one control-point tile along the last
 // axis (sample cells); empty when lo > hi.
 template <int DIM>
@@ -494,13 +504,30 @@ __device__ __forceinline__ void tile_plane_range(const Problem &pr, int t0, int 
 // sample's loads issued early, and than seven or eight, which spill).  Each
 // workgroup writes its FP64 sums; k_gather_finish adds the slabs in order.
 template <int DIM, int MODE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_gather_tile(Problem pr, const float4 *__restrict__ sv, int n_tiles)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_gather_tile(Problem pr, const float4 *__restrict__ sv, int n_tiles, GatherMap gm)
 {
     constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z, T = TX * TY * TZ;
     __shared__ double lds[T * 3 * 4];
     const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
-    const int tile = (int)(blockIdx.x % (unsigned)n_tiles), slab_id = (int)(blockIdx.x / (unsigned)n_tiles);
-    const int t0 = tile % ntx, t1 = (tile / ntx) % nty, t2 = tile / (ntx * nty);
+    int tile, slab_id, t0, t1, t2;
+    if (DIM == 3 && gm.regions == 8) {
+        const int region = (int)(blockIdx.x % 8u);
+        const unsigned k = blockIdx.x / 8u;
+        slab_id = (int)(k / (unsigned)gm.cap);
+        const int li = (int)(k % (unsigned)gm.cap);
+        t0 = (region % gm.rx) * gm.cw + li % gm.cw;
+        t1 = (region / gm.rx) * gm.ch + (li / gm.cw) % gm.ch;
+        t2 = li / (gm.cw * gm.ch);
+        // padding of the last regions (uniform over the workgroup)
+        if (t0 >= ntx || t1 >= nty) return;
+        tile = t0 + ntx * (t1 + nty * t2);
+    } else {
+        tile = (int)(blockIdx.x % (unsigned)n_tiles);
+        slab_id = (int)(blockIdx.x / (unsigned)n_tiles);
+        t0 = tile % ntx;
+        t1 = (tile / ntx) % nty;
+        t2 = tile / (ntx * nty);
+    }
     const int slab = gather_slab<DIM>(pr);
     int plo, phi;
     tile_plane_range<DIM>(pr, t0, t1, t2, plo, phi);
@@ -1035,7 +1062,20 @@ static void gather_impl(const Problem &pr, const float4 *sv, float *deriv, hipSt
                                 (DIM == 3 ? (pr.cp.n[2] + CpTile<DIM>::Z - 1) / CpTile<DIM>::Z : 1);
         const int slab = gather_slab<DIM>(pr);
         const long long slabs = (pr.nsub[DIM - 1] + slab - 1) / slab;
-        k_gather_tile<DIM, MODE><<<(unsigned)(tiles * slabs), kThreads, 0, s>>>(pr, sv, (int)tiles);
+        GatherMap gm{1, 1, 1, 1, 1};
+        long long blocks = tiles * slabs;
+        if (DIM == 3 && pr.gather_regions == 8) {
+            const int ntx = (pr.cp.n[0] + CpTile<DIM>::X - 1) / CpTile<DIM>::X;
+            const int nty = (pr.cp.n[1] + CpTile<DIM>::Y - 1) / CpTile<DIM>::Y;
+            const int ntz = (pr.cp.n[2] + CpTile<DIM>::Z - 1) / CpTile<DIM>::Z;
+            gm.regions = 8;
+            gm.rx = ntx >= nty ? 4 : 2;
+            gm.cw = (ntx + gm.rx - 1) / gm.rx;
+            gm.ch = (nty + 8 / gm.rx - 1) / (8 / gm.rx);
+            gm.cap = gm.cw * gm.ch * ntz;
+            blocks = 8LL * slabs * gm.cap;
+        }
+        k_gather_tile<DIM, MODE><<<(unsigned)blocks, kThreads, 0, s>>>(pr, sv, (int)tiles, gm);
         k_gather_finish<DIM><<<blocks_for(pr.n_cp), kThreads, 0, s>>>(pr, (int)tiles, deriv);
     }
 }

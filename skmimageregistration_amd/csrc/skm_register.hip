@@ -697,9 +697,8 @@ skm_status register_impl(const skm_registration_desc *desc, float *displacement_
         if (has_prev) SKM_R_TRY(skm::set_displacement_field_device(ev, &desc->target, d_prev.p));
         if (desc->autotune) {
             // launch shapes of this level's shape class, tuned once per process
-            skm_tuning t{};
-            SKM_R_TRY(skm_get_tuning(ev, &t));
-            if (t.lcc_ref_chunks == 0 && t.gather_split == 0 && t.value_split == 0) {
+            // (or loaded with the table)
+            if (!skm::tuning_recorded(ev)) {
                 std::vector<float> zero((size_t)n_cp * dim, 0.0f), one((size_t)n_cp, 1.0f);
                 SKM_R_TRY(d_cp.upload(zero.data(), zero.size(), ss));
                 SKM_R_TRY(d_cpw.upload(one.data(), one.size(), ss));

@@ -40,18 +40,18 @@ def _device(case):
 @pytest.mark.parametrize("cid,ckw,metric", CASES, ids=[c[0] for c in CASES])
 def test_autotuned_shapes_keep_results(cid, ckw, metric):
     case = Case(coef_scale=0.5, **ckw)
-    # no resampling anywhere below: the evaluator's offsets stay at their initial zeros
-    zeros = np.zeros((case.n_cells(), case.dim), np.int32) if case.sub > 1 else None
-    ores, oder = case.oracle(offsets=zeros).evaluate(int(metric), True)
+    # the first evaluation draws the offsets; none of the later ones resamples
+    ores, oder = case.oracle(offsets=case.offsets(1) if case.sub > 1 else None).evaluate(int(metric), True)
     ev = case.gpu()
     base_res, base_der = ev.getValueAndDerivative(metric, case.cp, case.cpw, do_resampling=False)
+    assert scalar_rel(base_res.measure, ores["measure"]) < 1e-5
     cp, cpw = _device(case)
     rep = ev.autotune_device(metric, cp.data_ptr(), cpw.data_ptr(), repeats=3)
     print(f"{cid}: {rep}", flush=True)
     assert rep["candidates"] >= 2 and rep["best_ms"] <= rep["first_ms"] and rep["best_ms"] > 0
     tuned = ev.get_tuning()
-    assert {k: tuned[k] for k in ("lcc_ref_chunks", "gather_split", "value_split")} == \
-        {k: rep[k] for k in ("lcc_ref_chunks", "gather_split", "value_split")}
+    keys = ("lcc_ref_chunks", "gather_split", "value_split", "gather_regions")
+    assert {k: tuned[k] for k in keys} == {k: rep[k] for k in keys}
     res, der = ev.getValueAndDerivative(metric, case.cp, case.cpw, do_resampling=False)
     assert res.numPixelsCountedGpu == base_res.numPixelsCountedGpu == ores["pixels_counted"]
     assert scalar_rel(res.measure, ores["measure"]) < 1e-5
@@ -76,14 +76,14 @@ def test_table_save_and_load(tmp_path):
     lines = [ln for ln in path.read_text().splitlines() if ln and not ln.startswith("#")]
     assert lines
     # rewrite every saved class with a fixed valid choice, load it back
-    fixed = [ln.split()[0] + " 0 4 8" for ln in lines]
+    fixed = [ln.split()[0] + " 0 4 8 1" for ln in lines]
     path.write_text("\n".join(fixed) + "\n")
     tuning_table_load(path)
     ev = case.gpu()
-    assert ev.get_tuning() == dict(lcc_ref_chunks=0, gather_split=4, value_split=8)
+    assert ev.get_tuning() == dict(lcc_ref_chunks=0, gather_split=4, value_split=8, gather_regions=1)
     ev.close()
     bad = tmp_path / "bad.txt"
-    bad.write_text(lines[0].split()[0] + " 0 5 8\n")
+    bad.write_text(lines[0].split()[0] + " 0 5 8 1\n")
     with pytest.raises(_capi.SkmError, match="out of range"):
         tuning_table_load(bad)
     with pytest.raises(_capi.SkmError):

@@ -141,7 +141,8 @@ def test_value_matches_value_and_derivative():
 
 
 @pytest.mark.parametrize("tuning", [dict(gather_split=4), dict(value_split=1), dict(value_split=4),
-                                    dict(value_split=16), dict(gather_split=4, value_split=8)],
+                                    dict(value_split=16), dict(gather_split=4, value_split=8),
+                                    dict(gather_regions=1), dict(gather_regions=8, gather_split=4)],
                          ids=lambda t: ",".join(f"{k}={v}" for k, v in t.items()))
 @pytest.mark.parametrize("mode", [0, 2])
 def test_launch_shapes(tuning, mode):
