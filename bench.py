@@ -30,6 +30,9 @@ HBM_PEAK_GBS = 8000.0
 # the weighted accumulate (32); the anisotropic kernel adds the Mahalanobis
 # distance and the F factor (75 more)
 FLOPS_PER_PAIR = {"stationary": 32, "anisotropic": 32 + 75}
+# LCC reference-window derivative: per (control point, centre, neighbour)
+# triple the isotropic Wendland ka(p_j, y) and the weighted accumulate
+FLOPS_PER_LCC_TRIPLE = 32
 
 
 def parse():
@@ -302,8 +305,7 @@ def cpu_baseline(args, mov, fix, ten, mcov):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import cpu_fp32
     import oracle as orc
-    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    threads = cpu_fp32.load().skmf_set_threads(avail)
+    threads = cpu_fp32.load().skmf_set_threads(cpu_threads())
     crop = tuple(int(v) for v in args.cpu_crop.split(",")) if args.cpu_crop else tuple(fix.shape)
     crop = tuple(min(c, n) for c, n in zip(crop, fix.shape))
     f = np.ascontiguousarray(fix[: crop[0], : crop[1], : crop[2]])
@@ -339,6 +341,17 @@ def cpu_baseline(args, mov, fix, ten, mcov):
             "ms_per_registration_estimate": round(reg_pairs / rate * 1e3, 0)}
 
 
+def cpu_threads():
+    """Logical CPUs this process may use, capped by OMP_NUM_THREADS when set
+    (the host's share of CPUs for this GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
 def host_cpu():
     """CPU model and the logical CPUs this process may use."""
     model = "unknown CPU"
@@ -354,7 +367,61 @@ def host_cpu():
     return f"{model}, {n} logical CPUs available"
 
 
-def lcc_timing():
+def lcc_triples(shape, cs, co, h, support, sub, subn):
+    """The reference-window LCC derivative's enumeration
+    (GpuEvaluatorLCC.cu:206-530): for every control point j, every centre in
+    j's sample-cell window, every neighbour on the stride-subn lattice of the
+    centre's tight support box; centres at their cell origins (the jitter
+    changes the count at the volume's faces only).  Separable per axis."""
+    size = tuple(reversed(shape))
+    lo, hi, _ = derivative_windows(size, cs, co, h, support, sub)
+    total = np.ones(len(lo), np.float64)
+    for d in range(3):
+        n = size[d]
+        pos = np.arange(n // sub) * sub
+        nlo = np.clip(np.trunc(pos - support + 1.0), 0, n - 1)
+        nhi = np.clip(np.trunc(pos + support), 0, n - 1)
+        cnt = (nhi - nlo) // subn + 1
+        pre = np.concatenate([[0.0], np.cumsum(cnt)])
+        total *= pre[hi[:, d] + 1] - pre[lo[:, d]]
+    return float(total.sum())
+
+
+def lcc_cpu_baseline(mov, fix, h, s, sub, subn, triples_full, crop=64):
+    """The OpenMP build of the FP64 oracle (oracle/skm_oracle.c, the
+    reference's LCC loops restated) timed on a crop^3 block at the centre of
+    the sliding-motion volumes (the sliding interface runs through it), with
+    the same kernel, control-point spacing, subsampling and neighbourhood;
+    reference-window derivative.  Its triple rate scales to the full volume's
+    triple count for a per-evaluation estimate."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    threads = orc.load(parallel=True).skmo_set_threads(cpu_threads())
+    c0 = [(n - crop) // 2 for n in fix.shape]
+    sl = tuple(slice(a, a + crop) for a in c0)
+    f, m = np.ascontiguousarray(fix[sl]), np.ascontiguousarray(mov[sl])
+    size, cs, co = geometry(f.shape, h)
+    n_cp = int(np.prod(cs))
+    cp = np.random.default_rng(0).uniform(-1, 1, (n_cp, 3))
+    n_cells = int(np.prod([n // sub for n in f.shape]))
+    offsets, _ = orc.cube_offsets(n_cells, 3, sub)
+    pr = orc.OracleProblem(f, m, (cs, co, (h,) * 3), cp, sigma_c0=s, sigma_c4=s, subsample=sub,
+                           subsample_neighborhood=subn, offsets=offsets, parallel=True)
+    t0 = time.perf_counter()
+    pr.evaluate(2, True, 0.0, 1e-3, 0.0)
+    dt = time.perf_counter() - t0
+    tri = lcc_triples(f.shape, cs, co, h, s, sub, subn)
+    rate = tri / dt
+    return {"value": round(triples_full / rate * 1e3, 1), "unit": "ms per LCC GetValueAndDerivative (estimate)",
+            "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (oracle/skm_oracle.c, OpenMP {threads} threads), reference-window LCC value and "
+                      f"derivative on the central {crop}^3 crop (cp {cs[0]}x{cs[1]}x{cs[2]}): {dt:.2f} s for "
+                      f"{tri:.3g} (cp, centre, neighbour) triples; scaled to the full volume's {triples_full:.3g}; "
+                      f"host {host_cpu()}",
+            "triples_per_s": round(rate, 1)}
+
+
+def lcc_timing(args):
     """One LCC GetValueAndDerivative on the sliding-motion workload
     (256x256x128 two halves shifted 4 voxels plus breathing motion,
     control-point spacing 5 -> 51x51x26, isotropic Wendland support 20, RD
@@ -363,7 +430,7 @@ def lcc_timing():
     exact gradient (every centre's neighbours scattered once, then one
     kernel-sum gather); median of 3 after one untimed evaluation."""
     from skmimageregistration_amd import MEASURE, GpuKernelEvaluator, Grid, Image, Kernel, synthetic
-    shape, h, s = (128, 256, 256), 5.0, 20.0
+    shape, h, s, sub, subn = (128, 256, 256), 5.0, 20.0, 2, 6
     mov, fix, _ = synthetic.sliding_3d(shape)
     size, cs, co = geometry(shape, h)
     n = int(np.prod(cs))
@@ -385,6 +452,29 @@ def lcc_timing():
         passes = {k: round(v[0] / max(v[1], 1), 3) for k, v in ev.read_timing().items() if v[1]}
         ev.close()
         out[name] = {"ms": round(float(np.median(times)), 2), "pass_ms": passes, "measure": round(r.measure, 6)}
+    # the reference-window derivative's own work: (cp, centre, neighbour) triples
+    tri = lcc_triples(shape, cs, co, h, s, sub, subn)
+    key = f"lcc-{size[0]}x{size[1]}x{size[2]}-h{h:g}-s{s:g}-sub{sub}-n{subn}"
+    ref_ms = out["reference_windows"]["pass_ms"]["lcc_derivative"]
+    achieved = tri * FLOPS_PER_LCC_TRIPLE / (ref_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": "k_lcc_gather_ref", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "launch_ms": ref_ms,
+            "triples_per_launch": tri, "flops_per_triple": FLOPS_PER_LCC_TRIPLE,
+            "traffic": profiled_traffic("k_lcc_gather_ref", key)}
+    if roof["traffic"]:
+        roof["hbm_traffic_frac"] = round(roof["traffic"] / (ref_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
+        roof["limiter"] = "hbm traffic" if roof["hbm_traffic_frac"] > roof["frac"] else "fp32 valu"
+    out["roofline"] = roof
+    # the exact-gradient path's streaming kernels against HBM
+    hbm = {}
+    for kname, pname in (("k_lcc_centres", "lcc_centres"), ("k_lcc_scatter", None), ("k_dense_warp", "lcc_dense")):
+        t = profiled_traffic(kname, key)
+        ms_k = out["exact_gradient"]["pass_ms"].get(pname) if pname else None
+        if t and ms_k:
+            hbm[kname] = {"traffic": t, "ms": ms_k, "GB/s": round(t / (ms_k * 1e-3) / 1e9, 1),
+                          "frac": round(t / (ms_k * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
+    out["exact_gradient"]["hbm"] = hbm or None
+    out["cpu_baseline"] = None if args.no_cpu_baseline else lcc_cpu_baseline(mov, fix, h, s, sub, subn, tri)
     return out
 
 
@@ -474,9 +564,11 @@ def batch_timing(args):
     smooth field (max 7 voxels) times the sine of the frame's phase, plus
     N(0, 0.02); frames 1.. each registered to frame 0 with
     registration_config's levels.  Wall time of the whole series registered
-    one at a time and with --batch-concurrency registrations in flight
-    (skm_register_batch), after one untimed pair; both runs must give the
-    same displacements bit for bit."""
+    one at a time, with --batch-concurrency registrations in flight
+    (skm_register_batch) and all of them in lockstep (one captured graph per
+    optimizer step of the whole batch, skm_register_batch_lockstep), after
+    one untimed pair; every run must give the same displacements bit for
+    bit."""
     from skmimageregistration_amd import Image, register_batch, synthetic
     frames, disps = synthetic.time_series_3d(n_frames=args.batch_frames)
     cfg = registration_config(args)
@@ -491,9 +583,15 @@ def batch_timing(args):
         results[conc] = register_batch(pairs, cfg, concurrency=conc)
         ms = (time.perf_counter() - t0) * 1e3
         out[f"concurrency_{conc}"] = {"ms": round(ms, 1), "ms_per_registration": round(ms / len(pairs), 2)}
+    t0 = time.perf_counter()
+    results["lockstep"] = register_batch(pairs, cfg, lockstep=True)
+    ms = (time.perf_counter() - t0) * 1e3
+    out["lockstep"] = {"ms": round(ms, 1), "ms_per_registration": round(ms / len(pairs), 2)}
     a, b = results[1], results[args.batch_concurrency]
-    out["identical"] = all(np.array_equal(x[0], y[0]) for x, y in zip(a, b))
+    out["identical"] = all(np.array_equal(x[0], y[0]) and np.array_equal(x[0], z[0])
+                           for x, y, z in zip(a, b, results["lockstep"]))
     out["speedup"] = round(out["concurrency_1"]["ms"] / out[f"concurrency_{args.batch_concurrency}"]["ms"], 3)
+    out["speedup_lockstep"] = round(out["concurrency_1"]["ms"] / out["lockstep"]["ms"], 3)
     inner = (slice(10, -10),) * 3
     out["final_ncc_mean"] = round(float(np.mean([r[1][-1]["measure"] for r in b])), 6)
     out["first_value_mean"] = round(float(np.mean([r[1][0]["first_value"] for r in b])), 6)
@@ -527,6 +625,11 @@ def warm_start_timing(args):
     res = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), concurrency=args.batch_concurrency,
                          initial_displacements=fields, initial_guard=True)
     ms_batch = (time.perf_counter() - t0) * 1e3 / len(pairs)
+    t0 = time.perf_counter()
+    fields = [ws.field(ws.predict(r, t)) for r, t in pairs]
+    res_ls = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), lockstep=True,
+                            initial_displacements=fields, initial_guard=True)
+    ms_lockstep = (time.perf_counter() - t0) * 1e3 / len(pairs)
     ws.close()
     plain_ms = float(np.mean([r["plain_ms"] for r in rows]))
     warm_ms = float(np.mean([r["warm_ms"] for r in rows]))
@@ -534,7 +637,10 @@ def warm_start_timing(args):
                         "3 levels x 200 iterations, warm = prediction + 50 fine-level iterations",
             "plain_ms_per_registration": round(plain_ms, 2), "warm_ms_per_registration": round(warm_ms, 2),
             "warm_batched_ms_per_registration": round(ms_batch, 2),
+            "warm_lockstep_ms_per_registration": round(ms_lockstep, 2),
             "speedup_single": round(plain_ms / warm_ms, 2), "speedup_batched": round(plain_ms / ms_batch, 2),
+            "speedup_lockstep": round(plain_ms / ms_lockstep, 2),
+            "lockstep_identical_to_batched": all(np.array_equal(x[0], y[0]) for x, y in zip(res, res_ls)),
             "plain_final_ncc": round(float(np.mean([r["plain_ncc"] for r in rows])), 5),
             "warm_final_ncc": round(float(np.mean([r["warm_ncc"] for r in rows])), 5),
             "warm_batched_final_ncc": round(float(np.mean([r[1][0]["measure"] for r in res])), 5),
@@ -651,7 +757,7 @@ def main():
     if not args.no_registration:
         line["registration"] = registration(args, mov, fix, disp)
     if not args.no_lcc:
-        line["lcc"] = lcc_timing()
+        line["lcc"] = lcc_timing(args)
     if not args.no_batch:
         line["batch"] = batch_timing(args)
     if not args.no_lcc_registration:
