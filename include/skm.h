@@ -60,6 +60,13 @@ typedef enum skm_kernel_mode {
  * moving neighbourhood) add cc to the value but nothing to the derivative in
  * both modes, so the exact gradient covers the other centres. */
 #define SKM_FLAG_EXACT_GRADIENT 0x1u
+/* moving_data holds the moving image's cubic B-spline coefficients as the
+ * reference's BsplineImage takes them (BsplineImage::computeCoeffs,
+ * BsplineImage.h:209-221, called by TransformAdapter::Initialize,
+ * itkTransformAdapter.hxx:435-439): prefiltered, shifted by one and padded to
+ * (n+3)^D with mirrored borders, x fastest; moving.size stays the image size
+ * n.  skm_create then copies them as they are instead of prefiltering. */
+#define SKM_FLAG_MOVING_COEFFICIENTS 0x2u
 
 /* Origin, spacing and size of an image (ImageBase, include/cuda/ImageBase.h:25-50). */
 typedef struct skm_grid {
@@ -75,7 +82,9 @@ typedef struct skm_desc {
     skm_grid fixed;  const float *fixed_data; /* target image T                        */
     skm_grid moving; const float *moving_data;/* reference image R, raw intensities;   */
                                               /* the cubic B-spline prefilter runs     */
-                                              /* inside skm_create                      */
+                                              /* inside skm_create (with               */
+                                              /* SKM_FLAG_MOVING_COEFFICIENTS: the     */
+                                              /* (n+3)^D coefficients instead)          */
     skm_grid cp;                              /* control-point grid                    */
     double sigma_c0, sigma_c4;                /* Wendland C0 / C4 supports             */
     int32_t kernel_mode;                      /* skm_kernel_mode                       */
@@ -243,6 +252,22 @@ SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float 
                                 int32_t repeats, skm_autotune_report *report);
 SKM_API skm_status skm_tuning_table_save(const char *path);
 SKM_API skm_status skm_tuning_table_load(const char *path);
+
+/* The library's device-memory cache (no counterpart in the reference, which
+ * calls cudaMalloc / cudaFree per evaluator, GpuEvaluator.cu:30-91, 93-113):
+ * blocks that skm_destroy and the registration driver give back are kept,
+ * per device, up to a limit of idle bytes, for the next evaluator (hipFree
+ * waits for the whole device, which stalls registrations running on other
+ * streams).  The limit is 4 GiB per device, or SKM_CACHE_IDLE_BYTES from the
+ * environment at the first allocation; skm_set_cache_limit changes it (idle
+ * blocks beyond a lower limit are freed at once; 0 disables the cache).
+ * skm_release_cached_memory frees every idle block of every device (a host
+ * application that needs the memory back); live blocks stay.  Sizes are
+ * rounded up to a size class (256 bytes, else an eighth of the largest power
+ * of two below the size). */
+SKM_API skm_status skm_set_cache_limit(uint64_t idle_bytes_per_device);
+SKM_API skm_status skm_get_cache_info(uint64_t *idle_bytes, uint64_t *live_bytes, uint64_t *limit);
+SKM_API skm_status skm_release_cached_memory(void);
 
 /* Jitter offsets of the last evaluation, D ints per subsampled cell
  * (GpuEvaluator.cu:107-118); n_cells receives prod(fixed.size / subsample). */

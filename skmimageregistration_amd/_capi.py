@@ -17,6 +17,7 @@ SKM_INTERPOLATION_LINEAR, SKM_INTERPOLATION_BSPLINE3 = 1, 3
 SKM_MEASURE_MSE, SKM_MEASURE_NCC, SKM_MEASURE_LCC = 0, 1, 2
 SKM_KERNEL_STATIONARY, SKM_KERNEL_WEIGHT_IMAGE, SKM_KERNEL_WEIGHT_TENSOR = 0, 1, 2
 SKM_FLAG_EXACT_GRADIENT = 0x1
+SKM_FLAG_MOVING_COEFFICIENTS = 0x2
 SKM_OPT_STANDARD, SKM_OPT_ADAPTIVE, SKM_OPT_FIXED = 0, 1, 2
 
 c_float_p = ctypes.POINTER(ctypes.c_float)
@@ -182,6 +183,10 @@ _SIGNATURES = {
     "skm_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                     ctypes.POINTER(SkmAutotuneReport)]),
     "skm_tuning_table_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "skm_set_cache_limit": (ctypes.c_int, [ctypes.c_uint64]),
+    "skm_get_cache_info": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "skm_release_cached_memory": (ctypes.c_int, []),
     "skm_tuning_table_load": (ctypes.c_int, [ctypes.c_char_p]),
     "skm_get_cube_offsets": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, ctypes.POINTER(ctypes.c_int64)]),
     "skm_control_point_grid": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(SkmGrid), c_double_p,

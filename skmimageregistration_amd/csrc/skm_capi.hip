@@ -5,8 +5,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -131,7 +133,46 @@ BlockCache &block_cache()
     return *cache;
 }
 
-constexpr size_t kIdleCap = size_t(16) << 30;  // idle bytes kept per device
+// idle bytes kept per device: SKM_CACHE_IDLE_BYTES in the environment (read
+// at the first allocation), skm_set_cache_limit, else 4 GiB
+std::atomic<long long> g_idle_cap{-1};
+
+size_t idle_cap()
+{
+    long long cap = g_idle_cap.load();
+    if (cap < 0) {
+        cap = (long long)(size_t(4) << 30);
+        if (const char *env = std::getenv("SKM_CACHE_IDLE_BYTES")) {
+            char *end = nullptr;
+            const long long v = std::strtoll(env, &end, 10);
+            if (end != env && v >= 0) cap = v;
+        }
+        long long expected = -1;
+        if (!g_idle_cap.compare_exchange_strong(expected, cap)) cap = expected;
+    }
+    return (size_t)cap;
+}
+
+// hipFree of every idle block (of one device, or all with dev < 0)
+void drop_idle(int dev)
+{
+    std::vector<void *> drop;
+    {
+        BlockCache &c = block_cache();
+        std::lock_guard<std::mutex> lock(c.mutex);
+        for (auto it = c.idle.begin(); it != c.idle.end();) {
+            if (dev < 0 || it->first.first == dev) {
+                drop.insert(drop.end(), it->second.begin(), it->second.end());
+                it = c.idle.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        for (auto &kv : c.idle_bytes)
+            if (dev < 0 || kv.first == dev) kv.second = 0;
+    }
+    for (void *q : drop) (void)hipFree(q);
+}
 
 // 256 bytes, or bytes rounded up to an eighth of the largest power of two
 // below it (at most 12.5% unused)
@@ -168,20 +209,7 @@ hipError_t skm::device_alloc(void **p, size_t bytes)
     e = hipMalloc(p, cls);
     if (e == hipErrorOutOfMemory) {
         // give this device's idle blocks back and try once more
-        std::vector<void *> drop;
-        {
-            std::lock_guard<std::mutex> lock(c.mutex);
-            for (auto it = c.idle.begin(); it != c.idle.end();) {
-                if (it->first.first == dev) {
-                    drop.insert(drop.end(), it->second.begin(), it->second.end());
-                    it = c.idle.erase(it);
-                } else {
-                    ++it;
-                }
-            }
-            c.idle_bytes[dev] = 0;
-        }
-        for (void *q : drop) (void)hipFree(q);
+        drop_idle(dev);
         (void)hipGetLastError();
         e = hipMalloc(p, cls);
     }
@@ -204,7 +232,7 @@ void skm::device_release(void *p)
         if (it != c.live.end()) {
             const std::pair<int, size_t> key = it->second;
             c.live.erase(it);
-            if (c.idle_bytes[key.first] + key.second <= kIdleCap) {
+            if (c.idle_bytes[key.first] + key.second <= idle_cap()) {
                 c.idle[key].push_back(p);
                 c.idle_bytes[key.first] += key.second;
                 return;
@@ -1005,6 +1033,8 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
     SKM_TRY(validate_grid(desc->moving, dim, "moving", 3));
     SKM_TRY(validate_grid(desc->cp, dim, "cp", 1));
     if (!desc->fixed_data || !desc->moving_data) return fail(SKM_ERROR_INVALID_ARGUMENT, "image data missing");
+    if (desc->flags & ~(SKM_FLAG_EXACT_GRADIENT | SKM_FLAG_MOVING_COEFFICIENTS))
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "unknown flags");
     if (!(desc->sigma_c0 > 0) || !(desc->sigma_c4 > 0))
         return fail(SKM_ERROR_INVALID_ARGUMENT, "Kernel parameters have to be positive.");
     if (desc->subsample < 1 || desc->subsample > 100)
@@ -1073,7 +1103,12 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
 
     // images
     int msize[3] = {desc->moving.size[0], desc->moving.size[1], dim == 3 ? desc->moving.size[2] : 1};
-    {
+    if (desc->flags & SKM_FLAG_MOVING_COEFFICIENTS) {
+        // the caller's prefiltered, padded coefficients (BsplineImage's layout)
+        const long long np = (long long)(msize[0] + 3) * (msize[1] + 3) * (dim == 3 ? msize[2] + 3 : 1);
+        if ((st = dev_copy_in(&ev->d_coeffs, desc->moving_data, (size_t)np, device_data, ev->stream)) != SKM_SUCCESS)
+            return cleanup(st);
+    } else {
         // cubic B-spline coefficients of the moving image, prefiltered on the device in FP64
         const long long nm = (long long)msize[0] * msize[1] * msize[2];
         const long long np = (long long)(msize[0] + 3) * (msize[1] + 3) * (dim == 3 ? msize[2] + 3 : 1);
@@ -1547,6 +1582,41 @@ SKM_API skm_status skm_tuning_table_load(const char *path)
     }
     std::lock_guard<std::mutex> lock(g_tuning_mutex);
     for (const auto &kv : read) g_tuning[kv.first] = kv.second;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_set_cache_limit(uint64_t idle_bytes_per_device)
+{
+    if (idle_bytes_per_device > (uint64_t)INT64_MAX) return fail(SKM_ERROR_INVALID_ARGUMENT, "limit out of range");
+    g_idle_cap.store((long long)idle_bytes_per_device);
+    // devices over the new limit give their idle blocks back
+    std::vector<int> over;
+    {
+        BlockCache &c = block_cache();
+        std::lock_guard<std::mutex> lock(c.mutex);
+        for (const auto &kv : c.idle_bytes)
+            if (kv.second > idle_bytes_per_device) over.push_back(kv.first);
+    }
+    for (int dev : over) drop_idle(dev);
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_get_cache_info(uint64_t *idle_bytes, uint64_t *live_bytes, uint64_t *limit)
+{
+    BlockCache &c = block_cache();
+    std::lock_guard<std::mutex> lock(c.mutex);
+    uint64_t idle = 0, live = 0;
+    for (const auto &kv : c.idle_bytes) idle += kv.second;
+    for (const auto &kv : c.live) live += kv.second.second;
+    if (idle_bytes) *idle_bytes = idle;
+    if (live_bytes) *live_bytes = live;
+    if (limit) *limit = idle_cap();
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_release_cached_memory(void)
+{
+    drop_idle(-1);
     return SKM_SUCCESS;
 }
 

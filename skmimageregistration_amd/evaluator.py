@@ -110,11 +110,16 @@ class GpuKernelEvaluator:
     control-point image).  exact_gradient selects derivative windows that
     cover the whole kernel support.  LCC centres failing the reference's
     derivative guard |u| > 1e-16 (GpuEvaluatorLCC.cu:517) add to the value
-    but not to the derivative in either mode.
+    but not to the derivative in either mode.  moving_coefficients, when
+    given, are the moving image's prefiltered cubic B-spline coefficients in
+    the reference BsplineImage's layout ((n+3)^D, mirrored borders, shifted by
+    one; BsplineImage.h:209-221): they are taken as they are
+    (SKM_FLAG_MOVING_COEFFICIENTS) and movingImage gives the geometry only.
     """
 
     def __init__(self, numParameters, subsample, subsampleNeighborhood, kernel, movingImage, fixedImage,
-                 cpImage, cpwImage=None, exact_gradient=False, rng_state=None, rng_sequence=None):
+                 cpImage, cpwImage=None, exact_gradient=False, rng_state=None, rng_sequence=None,
+                 moving_coefficients=None):
         self._lib = _capi.load()
         dim = fixedImage.dim
         self.dim = dim
@@ -126,7 +131,15 @@ class GpuKernelEvaluator:
         self._lib.skm_desc_init(ctypes.byref(desc))
         desc.dim = dim
         self._fixed = f32(fixedImage.data)
-        self._moving = f32(movingImage.data)
+        flags = _capi.SKM_FLAG_EXACT_GRADIENT if exact_gradient else 0
+        if moving_coefficients is not None:
+            padded = tuple(n + 3 for n in movingImage.data.shape[:dim])
+            self._moving = f32(moving_coefficients)
+            if self._moving.shape != padded:
+                raise ValueError(f"moving_coefficients must have shape {padded} (image size + 3 per axis)")
+            flags |= _capi.SKM_FLAG_MOVING_COEFFICIENTS
+        else:
+            self._moving = f32(movingImage.data)
         desc.fixed = fixedImage.grid()
         desc.fixed_data = fptr(self._fixed)
         desc.moving = movingImage.grid()
@@ -160,7 +173,7 @@ class GpuKernelEvaluator:
             desc.rng_state = int(rng_state)
         if rng_sequence is not None:
             desc.rng_sequence = int(rng_sequence)
-        desc.flags = _capi.SKM_FLAG_EXACT_GRADIENT if exact_gradient else 0
+        desc.flags = flags
         handle = ctypes.c_void_p()
         check(self._lib.skm_create(ctypes.byref(desc), ctypes.byref(handle)))
         self._h = handle
@@ -367,6 +380,24 @@ def tuning_table_load(path):
     """Merges a saved launch-shape table into the process's; evaluators
     created afterwards with a listed shape class start with its shapes."""
     check(_capi.load().skm_tuning_table_load(str(path).encode()))
+
+
+def set_cache_limit(idle_bytes_per_device):
+    """Idle device bytes the library keeps per device for later evaluators
+    (skm_set_cache_limit; 0 disables the cache)."""
+    check(_capi.load().skm_set_cache_limit(int(idle_bytes_per_device)))
+
+
+def cache_info():
+    """{"idle": bytes kept idle, "live": bytes in use, "limit": idle bytes per device}."""
+    idle, live, limit = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    check(_capi.load().skm_get_cache_info(ctypes.byref(idle), ctypes.byref(live), ctypes.byref(limit)))
+    return {"idle": idle.value, "live": live.value, "limit": limit.value}
+
+
+def release_cached_memory():
+    """Frees every idle block of the library's device cache (skm_release_cached_memory)."""
+    check(_capi.load().skm_release_cached_memory())
 
 
 def control_point_grid(image, spacing):

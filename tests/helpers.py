@@ -86,7 +86,7 @@ class Case:
             self.df = (rng.normal(size=self.shape + (dim,)) * 0.7).astype(np.float32)
 
     # ---------------------------------------------------------------- GPU
-    def gpu(self):
+    def gpu(self, **kw):
         kernel = Kernel(self.sigma_c0, self.sigma_c4,
                         useWeightImage=self.mode == 1, maximumWeight=self.max_weight,
                         wImage=Image(self.weight, self.origin, self.spacing) if self.mode == 1 else None,
@@ -96,7 +96,7 @@ class Case:
         ev = GpuKernelEvaluator(self.n_cp * self.dim, self.sub, self.subn, kernel,
                                 Image(self.moving, self.origin, self.spacing),
                                 Image(self.fixed, self.origin, self.spacing),
-                                Grid(self.cp_size, self.cp_origin, self.cp_spacing), exact_gradient=self.exact)
+                                Grid(self.cp_size, self.cp_origin, self.cp_spacing), exact_gradient=self.exact, **kw)
         return ev
 
     def df_image(self):

@@ -130,3 +130,38 @@ def test_register_batch_rejects_bad_arguments():
     nulls = (_capi.c_float_p * 1)()
     assert lib.skm_register_batch(descs, 1, 1, nulls, None, None) == _capi.SKM_ERROR_INVALID_ARGUMENT
     assert b"displacements_out[0]" in lib.skm_last_error()
+
+
+def test_create_rejects_unknown_flags():
+    lib = _capi.load()
+    d = _capi.SkmDesc()
+    lib.skm_desc_init(ctypes.byref(d))
+    img = np.zeros((8, 8), np.float32)
+    d.dim = 2
+    d.fixed = _capi.make_grid((8, 8))
+    d.moving = _capi.make_grid((8, 8))
+    d.cp = _capi.make_grid((2, 2), spacing=(4, 4))
+    d.fixed_data = _capi.fptr(img)
+    d.moving_data = _capi.fptr(img)
+    d.sigma_c0 = d.sigma_c4 = 4.0
+    d.flags = 0x80
+    h = ctypes.c_void_p()
+    assert lib.skm_create(ctypes.byref(d), ctypes.byref(h)) == _capi.SKM_ERROR_INVALID_ARGUMENT
+    assert b"unknown flags" in lib.skm_last_error()
+
+
+def test_cache_limit_and_release_without_device_blocks():
+    """The cache's host bookkeeping: the limit round-trips, releasing with
+    nothing idle succeeds, and no block is live before any evaluator."""
+    from skmimageregistration_amd import cache_info, release_cached_memory, set_cache_limit
+    before = cache_info()
+    try:
+        set_cache_limit(123 << 20)
+        info = cache_info()
+        assert info["limit"] == 123 << 20
+        release_cached_memory()
+        assert cache_info()["idle"] == 0
+        set_cache_limit(0)
+        assert cache_info()["limit"] == 0
+    finally:
+        set_cache_limit(before["limit"])

@@ -9,6 +9,7 @@ Tolerances (FP32 per pair, FP64 final sums):
 import numpy as np
 import pytest
 
+import oracle as orc
 from helpers import Case, rel_err, scalar_rel
 from skmimageregistration_amd import MEASURE
 
@@ -182,3 +183,28 @@ def test_bilateral_regularizer(dim, mode):
     ev.close()
     expected = ores["measure"] + 0.2 * ores["reg_rd"] + 0.3 * bval
     assert scalar_rel(recs[0][0], expected) < VALUE_TOL, (recs[0][0], expected)
+
+
+@pytest.mark.parametrize("dim,measure", [(2, MEASURE.MSE), (3, MEASURE.NCC), (3, MEASURE.LCC)])
+def test_moving_coefficients_as_the_reference_passes_them(dim, measure):
+    """SKM_FLAG_MOVING_COEFFICIENTS: the (n+3)^D prefiltered coefficients
+    that the reference's TransformAdapter hands to GpuKernelEvaluator
+    (itkTransformAdapter.hxx:435-461), here from the oracle's FP64 prefilter
+    rounded to FP32, give the oracle's result and the raw-image path's."""
+    case = Case(dim=dim, shape=(40, 44) if dim == 2 else (16, 18, 20), cp_spacing=5.0, sigma=12.0, seed=49,
+                subsample_neighborhood=2 if measure == MEASURE.LCC else 1)
+    coeffs = orc.bspline_coefficients(case.moving).astype(np.float32)
+    ev_raw = case.gpu()
+    ev_cf = case.gpu(moving_coefficients=coeffs)
+    r_raw, d_raw = ev_raw.getValueAndDerivative(measure, case.cp, case.cpw)
+    r_cf, d_cf = ev_cf.getValueAndDerivative(measure, case.cp, case.cpw)
+    ev_raw.close()
+    ev_cf.close()
+    ores, oder = case.oracle().evaluate(int(measure), True)
+    assert r_cf.numPixelsCountedGpu == ores["pixels_counted"]
+    assert scalar_rel(r_cf.measure, ores["measure"]) < VALUE_TOL
+    assert rel_err(d_cf, oder) < DERIV_TOL
+    assert scalar_rel(r_cf.measure, r_raw.measure) < 1e-6
+    assert rel_err(d_cf, d_raw) < 1e-5
+    with pytest.raises(ValueError):
+        case.gpu(moving_coefficients=coeffs[1:])
