@@ -707,12 +707,16 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
         SKM_TRY(ensure_lcc_buffers(ev));
         { PassTimer t(ev, 6); launch_dense_warp(pr, 0, ev->d_dense, s); }
         const bool exact_deriv = deriv && pr.exact;
-        if (exact_deriv && !ev->d_lcc_bound)
+        // stationary exact mode gathers gamma per voxel; the scatter (weighted
+        // modes) needs the centres' bounds for its fixed-point step
+        const bool gamma_gather = exact_deriv && lcc_gamma_gather_applies(pr);
+        const bool need_bound = exact_deriv && !gamma_gather;
+        if (need_bound && !ev->d_lcc_bound)
             SKM_TRY(dev_alloc(&ev->d_lcc_bound, (size_t)((pr.n_sub + kThreads - 1) / kThreads)));
         {
             PassTimer t(ev, 7);
             launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s,
-                               exact_deriv ? ev->d_lcc_bound : nullptr);
+                               need_bound ? ev->d_lcc_bound : nullptr);
         }
         {
             PassTimer t(ev, 2);
@@ -722,11 +726,15 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
             PassTimer t(ev, 8);
             if (pr.exact) {
                 if (!ev->d_gamma) SKM_TRY(dev_alloc(&ev->d_gamma, pr.n_vox));
-                if (!ev->d_gamma_fx) SKM_TRY(dev_alloc(&ev->d_gamma_fx, (size_t)pr.n_vox * 3));
-                launch_zero_words(ev->d_gamma_fx, 6LL * pr.n_vox, s);
-                SKM_HIP(hipGetLastError());
-                launch_lcc_scatter(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_lcc_bound, blocks, ev->d_gamma_fx,
-                                   ev->d_gamma, s);
+                if (gamma_gather) {
+                    launch_lcc_gamma(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_gamma, s);
+                } else {
+                    if (!ev->d_gamma_fx) SKM_TRY(dev_alloc(&ev->d_gamma_fx, (size_t)pr.n_vox * 3));
+                    launch_zero_words(ev->d_gamma_fx, 6LL * pr.n_vox, s);
+                    SKM_HIP(hipGetLastError());
+                    launch_lcc_scatter(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_lcc_bound, blocks,
+                                       ev->d_gamma_fx, ev->d_gamma, s);
+                }
                 Problem dp = pr;  // gather over the dense voxel grid
                 if (!is_stationary(pr.kc.mode)) {
                     if (!ev->d_vox_pstate) {

@@ -169,6 +169,12 @@ long long lcc_reference_partials(const Problem &pr);
 void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, Scalars *sc,
                         const double *bound_blk, int bound_blocks, unsigned long long *fx, float4 *gamma,
                         hipStream_t s);
+// Exact mode, stationary kernels: the same gamma gathered per voxel (no
+// atomics, FP64 sums in a fixed order); applies when every axis' reach of a
+// voxel fits the gather's 64-position masks (support / spacing <= ~30).
+bool lcc_gamma_gather_applies(const Problem &pr);
+void launch_lcc_gamma(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
+                      float4 *gamma, hipStream_t s);
 
 // optimizer (skm_kernels.hip)
 struct OptState {

@@ -577,6 +577,108 @@ __global__ __launch_bounds__(kThreads) void k_lcc_scatter(Problem pr, const floa
     }
 }
 
+// Exact mode with a stationary kernel: gamma(y) gathered per fixed voxel y
+// from the centres whose neighbour lattice holds y -- the transpose of
+// k_lcc_scatter, without atomics.  With one support for every centre, the
+// centres that reach y along an axis lie within [y - e - 1, y - c + 1]
+// (c = floor(1 - r), e = floor(r), r = support / spacing); each position p of
+// that range is tested with neighbour_box's own arithmetic (y inside the box
+// and on its stride lattice) and kept as a bit of a 64-bit mask, so the launch
+// needs e - c + 3 <= 64 on every axis.  A position triple is a centre
+// when its cell's jitter offsets equal p - sub floor(p / sub).  Then
+//   gamma(y) = scale gradM(y) (F(y) SA + M(y) SB + SC),
+//   SA = sum A kw^2, SB = sum B kw^2, SC = sum C kw
+// over the centres, in FP64 and in a fixed order (z, y, x positions
+// ascending), which is the scatter's sum of beta kw gradM regrouped.
+template <int DIM>
+__device__ __forceinline__ unsigned long long reach_mask(const Problem &pr, int d, int yi, double r, int lo_c,
+                                                         int &base)
+{
+    base = yi - (int)floor(r) - 1;
+    unsigned long long m = 0;
+    const int n = pr.fixed.n[d], sn = pr.subn;
+    for (int i = 0; i < 64; ++i) {
+        const int p = base + i;
+        if (p > yi - lo_c + 1) break;
+        if (p < 0 || p >= n) continue;
+        const int nlo = clampi((int)((double)p - r + 1.0), 0, n - 1);
+        const int nhi = clampi((int)((double)p + r), 0, n - 1);
+        if (yi >= nlo && yi <= nhi && (yi - nlo) % sn == 0) m |= 1ull << i;
+    }
+    return m;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4 *__restrict__ dense,
+                                                        const float4 *__restrict__ coef,
+                                                        const Scalars *__restrict__ sc, float4 *__restrict__ gamma)
+{
+    const long long v = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (v >= pr.n_vox) return;
+    const int yi[3] = {(int)(v % pr.fixed.n[0]), (int)((v / pr.fixed.n[0]) % pr.fixed.n[1]),
+                       DIM == 3 ? (int)(v / ((long long)pr.fixed.n[0] * pr.fixed.n[1])) : 0};
+    const float4 g4 = dense[v];
+    if (isnan(g4.w)) {
+        gamma[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float fy3[3] = {(float)yi[0], (float)yi[1], (float)yi[2]};
+    float y[3];
+    to_global<DIM>(pr.fixed, fy3, y);
+    unsigned long long mask[3] = {1ull, 1ull, 1ull};
+    int base[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double r = (double)pr.kc.smin / (double)pr.fixed.h[d];
+        mask[d] = reach_mask<DIM>(pr, d, yi[d], r, (int)floor(1.0 - r), base[d]);
+    }
+    const int sub = pr.sub;
+    double sa = 0.0, sb = 0.0, scc = 0.0;
+    for (unsigned long long mz = mask[2]; mz; mz &= mz - 1) {
+        const int pz = DIM == 3 ? base[2] + __builtin_ctzll(mz) : 0;
+        for (unsigned long long my = mask[1]; my; my &= my - 1) {
+            const int py = base[1] + __builtin_ctzll(my);
+            for (unsigned long long mx = mask[0]; mx; mx &= mx - 1) {
+                const int px = base[0] + __builtin_ctzll(mx);
+                const int pos[3] = {px, py, pz};
+                // the neighbourhood weight first: it needs only the position,
+                // and about half the candidates lie outside the support
+                const float fp[3] = {(float)px, (float)py, (float)pz};
+                float x[3];
+                to_global<DIM>(pr.fixed, fp, x);
+                float r2 = 0.0f;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) r2 += (x[d] - y[d]) * (x[d] - y[d]);
+                const float kw = neighborhood_weight(r2, pr.kc.smin);
+                if (!(kw > 0.0f)) continue;
+                int m[3] = {0, 0, 0};
+                bool cell_ok = true;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    m[d] = pos[d] / sub;
+                    cell_ok = cell_ok && m[d] < pr.nsub[d];
+                }
+                if (!cell_ok) continue;
+                const long long s = m[0] + (long long)pr.nsub[0] * (m[1] + (long long)pr.nsub[1] * m[2]);
+                if (sub > 1) {
+                    bool same = true;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) same = same && pr.offsets[s * DIM + d] == pos[d] - m[d] * sub;
+                    if (!same) continue;
+                }
+                const float4 abc = coef[s];
+                if (abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f) continue;
+                const double kw2 = (double)kw * (double)kw;
+                sa += (double)abc.x * kw2;
+                sb += (double)abc.y * kw2;
+                scc += (double)abc.z * (double)kw;
+            }
+        }
+    }
+    const double gam = sc->coef_scale * ((double)pr.fixed_data[v] * sa + (double)g4.w * sb + scc);
+    gamma[v] = make_float4((float)(gam * g4.x), (float)(gam * g4.y), DIM == 3 ? (float)(gam * g4.z) : 0.0f, 0.0f);
+}
+
 // ------------------------------------------------------------ launchers
 
 template <int DIM>
@@ -683,6 +785,24 @@ static void scatter_impl(const Problem &pr, const float4 *dense, const float4 *c
 {
     unsigned g = (unsigned)((pr.n_sub + kThreads - 1) / kThreads);
     k_lcc_scatter<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, sc, fx);
+}
+
+bool lcc_gamma_gather_applies(const Problem &pr)
+{
+    if (!is_stationary(pr.kc.mode)) return false;
+    for (int d = 0; d < pr.dim; ++d) {
+        const double r = (double)pr.kc.smin / (double)pr.fixed.h[d];
+        if ((std::floor(r) - std::floor(1.0 - r)) + 3.0 > 64.0) return false;
+    }
+    return true;
+}
+
+void launch_lcc_gamma(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
+                      float4 *gamma, hipStream_t s)
+{
+    const unsigned g = (unsigned)((pr.n_vox + kThreads - 1) / kThreads);
+    if (pr.dim == 2) k_lcc_gamma<2><<<g, kThreads, 0, s>>>(pr, dense_g, centre_coef, sc, gamma);
+    else k_lcc_gamma<3><<<g, kThreads, 0, s>>>(pr, dense_g, centre_coef, sc, gamma);
 }
 
 void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, Scalars *sc,
