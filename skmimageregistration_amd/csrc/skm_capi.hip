@@ -329,6 +329,7 @@ struct skm_evaluator {
     skm_tuning tuning = {};                 // launch shapes as set by skm_set_tuning
     double *d_lcc_bound = nullptr;          // exact LCC: per-workgroup bounds of the centre pass
     unsigned long long *d_gamma_fx = nullptr;  // exact LCC: fixed-point scatter sums, 3 per voxel
+    unsigned *d_lcc_cpk = nullptr;          // exact LCC gather: packed offsets + "contributes" per centre
     float *d_params = nullptr, *d_prev_g = nullptr, *d_best = nullptr;
     OptState *d_opt = nullptr;
     skm_iteration_record *d_records = nullptr;
@@ -359,7 +360,7 @@ struct skm_evaluator {
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng,
-                        d_lcc_ref_partials, d_lcc_bound, d_gamma_fx, d_cp_intensity};
+                        d_lcc_ref_partials, d_lcc_bound, d_gamma_fx, d_cp_intensity, d_lcc_cpk};
         for (void *p : ptrs)
             if (p) device_release(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -713,10 +714,11 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
         const bool need_bound = exact_deriv && !gamma_gather;
         if (need_bound && !ev->d_lcc_bound)
             SKM_TRY(dev_alloc(&ev->d_lcc_bound, (size_t)((pr.n_sub + kThreads - 1) / kThreads)));
+        if (gamma_gather && !ev->d_lcc_cpk) SKM_TRY(dev_alloc(&ev->d_lcc_cpk, (size_t)pr.n_sub));
         {
             PassTimer t(ev, 7);
             launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s,
-                               need_bound ? ev->d_lcc_bound : nullptr);
+                               need_bound ? ev->d_lcc_bound : nullptr, gamma_gather ? ev->d_lcc_cpk : nullptr);
         }
         {
             PassTimer t(ev, 2);
@@ -727,7 +729,7 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
             if (pr.exact) {
                 if (!ev->d_gamma) SKM_TRY(dev_alloc(&ev->d_gamma, pr.n_vox));
                 if (gamma_gather) {
-                    launch_lcc_gamma(pr, ev->d_dense, ev->d_ccoef, ev->d_sc, ev->d_gamma, s);
+                    launch_lcc_gamma(pr, ev->d_dense, ev->d_ccoef, ev->d_lcc_cpk, ev->d_sc, ev->d_gamma, s);
                 } else {
                     if (!ev->d_gamma_fx) SKM_TRY(dev_alloc(&ev->d_gamma_fx, (size_t)pr.n_vox * 3));
                     launch_zero_words(ev->d_gamma_fx, 6LL * pr.n_vox, s);

@@ -155,9 +155,11 @@ void launch_result(const Scalars *sc, void *result, hipStream_t s);
 // LCC (skm_lcc.hip)
 void launch_dense_warp(const Problem &pr, int quirk3d, float4 *dense_g, hipStream_t s);
 // bound_blk (optional, one double per workgroup): the largest bound per
-// workgroup of |beta kw gradM| over a centre's neighbours, before the 1/N scale
+// workgroup of |beta kw gradM| over a centre's neighbours, before the 1/N scale;
+// cpk (optional, one word per centre): its jitter offsets, one byte per axis,
+// and bit 31 set when its derivative coefficients are not all zero
 void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre_coef, double *partials,
-                        int *blocks_out, hipStream_t s, double *bound_blk = nullptr);
+                        int *blocks_out, hipStream_t s, double *bound_blk = nullptr, unsigned *cpk = nullptr);
 // partials: lcc_reference_partials(pr) doubles (none needed when it is 0)
 void launch_lcc_gather_reference(const Problem &pr, const float4 *dense_g, const float4 *centre_coef,
                                  const Scalars *sc, float *deriv_soa, double *partials, hipStream_t s);
@@ -173,8 +175,8 @@ void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *
 // atomics, FP64 sums in a fixed order); applies when every axis' reach of a
 // voxel fits the gather's 64-position masks (support / spacing <= ~30).
 bool lcc_gamma_gather_applies(const Problem &pr);
-void launch_lcc_gamma(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
-                      float4 *gamma, hipStream_t s);
+void launch_lcc_gamma(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const unsigned *cpk,
+                      const Scalars *sc, float4 *gamma, hipStream_t s);
 
 // optimizer (skm_kernels.hip)
 struct OptState {

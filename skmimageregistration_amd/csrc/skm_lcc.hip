@@ -178,7 +178,7 @@ __device__ __forceinline__ long long voxel_index(const Problem &pr, const int *p
 template <int DIM, int MODE>
 __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const float4 *__restrict__ dense,
                                                           float4 *__restrict__ coef, double *__restrict__ partials,
-                                                          double *__restrict__ bound_blk)
+                                                          double *__restrict__ bound_blk, unsigned *__restrict__ cpk)
 {
     __shared__ double lds[kPartialsK * 4];
     __shared__ double lds_max[kThreads / 64];
@@ -252,6 +252,13 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
             }
         }
         coef[s] = make_float4(A, B, C, 0.0f);
+        if (cpk) {
+            unsigned w = (A != 0.0f || B != 0.0f || C != 0.0f) ? 0x80000000u : 0u;
+            if (pr.sub > 1)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) w |= (unsigned)pr.offsets[s * DIM + d] << (8 * d);
+            cpk[s] = w;
+        }
         acc[0] = cc;
         long long cv = voxel_index<DIM>(pr, pos);
         acc[6] = isnan(dense[cv].w) ? 0.0 : 1.0;
@@ -611,6 +618,7 @@ __device__ __forceinline__ unsigned long long reach_mask(const Problem &pr, int 
 template <int DIM>
 __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4 *__restrict__ dense,
                                                         const float4 *__restrict__ coef,
+                                                        const unsigned *__restrict__ cpk,
                                                         const Scalars *__restrict__ sc, float4 *__restrict__ gamma)
 {
     const long long v = blockIdx.x * (long long)kThreads + threadIdx.x;
@@ -653,21 +661,17 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4
                 if (!(kw > 0.0f)) continue;
                 int m[3] = {0, 0, 0};
                 bool cell_ok = true;
+                unsigned want = 0x80000000u;  // a centre at pos whose coefficients are not all zero
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
                     m[d] = pos[d] / sub;
                     cell_ok = cell_ok && m[d] < pr.nsub[d];
+                    want |= (unsigned)(pos[d] - m[d] * sub) << (8 * d);
                 }
                 if (!cell_ok) continue;
                 const long long s = m[0] + (long long)pr.nsub[0] * (m[1] + (long long)pr.nsub[1] * m[2]);
-                if (sub > 1) {
-                    bool same = true;
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) same = same && pr.offsets[s * DIM + d] == pos[d] - m[d] * sub;
-                    if (!same) continue;
-                }
+                if (cpk[s] != want) continue;
                 const float4 abc = coef[s];
-                if (abc.x == 0.0f && abc.y == 0.0f && abc.z == 0.0f) continue;
                 const double kw2 = (double)kw * (double)kw;
                 sa += (double)abc.x * kw2;
                 sb += (double)abc.y * kw2;
@@ -717,17 +721,17 @@ void launch_dense_warp(const Problem &pr, int quirk3d, float4 *dense_g, hipStrea
 
 template <int DIM, int MODE>
 static void centres_impl(const Problem &pr, const float4 *dense, float4 *coef, double *partials, int *blocks,
-                         double *bound_blk, hipStream_t s)
+                         double *bound_blk, unsigned *cpk, hipStream_t s)
 {
     unsigned g = (unsigned)((pr.n_sub + kThreads - 1) / kThreads);
     *blocks = (int)g;
-    k_lcc_centres<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, partials, bound_blk);
+    k_lcc_centres<DIM, MODE><<<g, kThreads, 0, s>>>(pr, dense, coef, partials, bound_blk, cpk);
 }
 
 void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre_coef, double *partials,
-                        int *blocks_out, hipStream_t s, double *bound_blk)
+                        int *blocks_out, hipStream_t s, double *bound_blk, unsigned *cpk)
 {
-    SKM_LCC_DISPATCH(pr, centres_impl, pr, dense_g, centre_coef, partials, blocks_out, bound_blk, s);
+    SKM_LCC_DISPATCH(pr, centres_impl, pr, dense_g, centre_coef, partials, blocks_out, bound_blk, cpk, s);
 }
 
 template <int DIM>
@@ -797,12 +801,12 @@ bool lcc_gamma_gather_applies(const Problem &pr)
     return true;
 }
 
-void launch_lcc_gamma(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
-                      float4 *gamma, hipStream_t s)
+void launch_lcc_gamma(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const unsigned *cpk,
+                      const Scalars *sc, float4 *gamma, hipStream_t s)
 {
     const unsigned g = (unsigned)((pr.n_vox + kThreads - 1) / kThreads);
-    if (pr.dim == 2) k_lcc_gamma<2><<<g, kThreads, 0, s>>>(pr, dense_g, centre_coef, sc, gamma);
-    else k_lcc_gamma<3><<<g, kThreads, 0, s>>>(pr, dense_g, centre_coef, sc, gamma);
+    if (pr.dim == 2) k_lcc_gamma<2><<<g, kThreads, 0, s>>>(pr, dense_g, centre_coef, cpk, sc, gamma);
+    else k_lcc_gamma<3><<<g, kThreads, 0, s>>>(pr, dense_g, centre_coef, cpk, sc, gamma);
 }
 
 void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, Scalars *sc,
