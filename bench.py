@@ -602,6 +602,61 @@ def batch_timing(args):
     return out
 
 
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+WS_CIN = (2, 16, 16, 32, 32, 48, 48, 64, 112, 80)
+WS_COUT = (16, 16, 32, 32, 48, 48, 64, 64, 48, 32)
+WS_LEVEL = (0, 0, 1, 1, 2, 2, 3, 3, 2, 1)
+
+
+def network_timing(in_size, cp, steps=10, warmup=3, batch=8):
+    """The warm-start network's own kernels at the workload's shape (input
+    2 x 64 x 64 x 48, control points 22 x 22 x 16): one training step
+    (forward, backward, Adam) of a batch of 8 and one forward of one pair,
+    timed with HIP events on the network's stream.  Algorithmic flops of the
+    ten 3x3x3 convolutions: forward 2 x voxels x 27 x cin x cout each, a
+    training step three times that (data and weight gradients)."""
+    import torch
+
+    from skmimageregistration_amd.warmstart import WarmStartNet
+    vox0 = in_size[0] * in_size[1] * in_size[2]
+    fwd = sum(2.0 * (vox0 >> (3 * lv)) * 27 * ci * co for ci, co, lv in zip(WS_CIN, WS_COUT, WS_LEVEL))
+    origin = tuple((o - 0.5) / 2.0 for o in cp.origin)
+    spacing = tuple(h / 2.0 for h in cp.spacing)
+    net = WarmStartNet(in_size, cp.size(), origin, spacing, max_batch=batch)
+    rng = np.random.default_rng(5)
+    net.set_params(rng.normal(0.0, 0.05, net.n_params).astype(np.float32))
+    x = torch.tensor(rng.uniform(0, 1, (batch, in_size[2], in_size[1], in_size[0], 2)).astype(np.float32),
+                     device="cuda")
+    t = torch.zeros((batch, cp.size()[2], cp.size()[1], cp.size()[0], 3), device="cuda")
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        net.train_step(x, t, 1e-4)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        net.train_step(x, t, 1e-4)
+    b.record(stream)
+    torch.cuda.synchronize()
+    train_ms = a.elapsed_time(b) / steps
+    x1 = x[:1].contiguous()
+    for _ in range(warmup):
+        net.forward(x1)
+    a.record(stream)
+    for _ in range(steps):
+        net.forward(x1)
+    b.record(stream)
+    torch.cuda.synchronize()
+    fwd_ms = a.elapsed_time(b) / steps
+    net.close()
+    train_tf = 3.0 * fwd * batch / (train_ms * 1e-3) / 1e12
+    return {"train_step_ms": round(train_ms, 3), "batch": batch, "forward_ms_per_pair": round(fwd_ms, 3),
+            "flops_forward_per_pair": fwd,
+            "roofline": {"bound": "mfma", "achieved": round(train_tf, 2), "peak": BF16_DENSE_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(train_tf / BF16_DENSE_PEAK_TFLOPS, 5),
+                         "traffic": None, "what": "training step (bf16 operands, fp32 accumulation)"},
+            "forward_tflops": round(fwd / (fwd_ms * 1e-3) / 1e12, 2)}
+
+
 def warm_start_timing(args):
     """The learned warm start on held-out synthetic pairs of the 128x128x96
     workload (seeds 900.., outside the training seeds): per pair the plain
@@ -631,6 +686,7 @@ def warm_start_timing(args):
     res_ls = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), lockstep=True,
                             initial_displacements=fields, initial_guard=True)
     ms_lockstep = (time.perf_counter() - t0) * 1e3 / len(pairs)
+    net = network_timing(ws.net.in_size, ws.cp)
     ws.close()
     plain_ms = float(np.mean([r["plain_ms"] for r in rows]))
     warm_ms = float(np.mean([r["warm_ms"] for r in rows]))
@@ -650,7 +706,8 @@ def warm_start_timing(args):
                                                for r in rows)), 5),
             "predictions_used": int(sum(r["warm_used"] for r in rows)),
             "mean_abs_error_vox": {k: round(float(np.mean([r[k + "_err"] for r in rows])), 4)
-                                   for k in ("plain", "warm", "zero")}}
+                                   for k in ("plain", "warm", "zero")},
+            "network": net}
 
 
 def main():
