@@ -118,10 +118,10 @@ class WarmStartNet:
 
 # ---------------------------------------------------------------- registration workload
 
-CFG2_SHAPE = (96, 128, 128)  # z, y, x
+WORKLOAD_SHAPE = (96, 128, 128)  # z, y, x
 
 
-def synthetic_pair(seed, shape=CFG2_SHAPE, n_ellipsoids=20, smoothing_sigma=1.5, n_rbf=8, rbf_sigma=None,
+def synthetic_pair(seed, shape=WORKLOAD_SHAPE, n_ellipsoids=20, smoothing_sigma=1.5, n_rbf=8, rbf_sigma=None,
                    max_displacement=6.0, displacement_scale=1.0, noise_sigma=0.02):
     """skm_synthetic_pair: (reference, target, displacement) as numpy arrays,
     shapes (z, y, x) and (z, y, x, 3); the displacement RBFs' standard
@@ -182,7 +182,7 @@ class WarmStart:
     SUPPORT = 24.0
     FIT_ITERATIONS, FIT_RIDGE = 60, 300.0
 
-    def __init__(self, shape=CFG2_SHAPE, max_batch=1, weights=None):
+    def __init__(self, shape=WORKLOAD_SHAPE, max_batch=1, weights=None):
         import torch
 
         from .evaluator import GpuKernelEvaluator, Image, Kernel, level_control_points
