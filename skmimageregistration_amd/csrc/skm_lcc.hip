@@ -641,16 +641,56 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4
         mask[d] = reach_mask<DIM>(pr, d, yi[d], r, (int)floor(1.0 - r), base[d]);
     }
     const int sub = pr.sub;
+    const float inv_sub = 1.0f / (float)sub;
+    // cell and jitter offset of a position p >= 0: (p + 0.5) / sub is at least
+    // 0.5 / sub away from an integer, far more than the FP32 product's error
+    // for any image size below 2^22
+    auto cell_of = [&](int p, int &off) {
+        const int m = sub > 1 ? (int)(((float)p + 0.5f) * inv_sub) : p;
+        off = p - m * sub;
+        return m;
+    };
+    // bits i of a mask whose position base + i lies within R of c
+    auto bits_within = [](int base_i, int c, int R) -> unsigned long long {
+        const int a = max(c - R - base_i, 0), b = min(c + R - base_i, 63);
+        if (a > b) return 0ull;
+        const unsigned long long upto = b == 63 ? ~0ull : ((1ull << (b + 1)) - 1ull);
+        return upto & (~0ull << a);
+    };
+    // candidates beyond the support (plus a margin) are skipped before their
+    // weight is computed; the weight of the others decides
+    const float lim = pr.kc.smin * pr.kc.smin * 1.001f;
     double sa = 0.0, sb = 0.0, scc = 0.0;
     for (unsigned long long mz = mask[2]; mz; mz &= mz - 1) {
         const int pz = DIM == 3 ? base[2] + __builtin_ctzll(mz) : 0;
-        for (unsigned long long my = mask[1]; my; my &= my - 1) {
+        const float dz = DIM == 3 ? fmaf(pr.fixed.h[2], (float)pz, pr.fixed.o[2]) - y[2] : 0.0f;
+        const float remz = lim - dz * dz;
+        if (!(remz > 0.0f)) continue;
+        int oz = 0;
+        const int mzc = DIM == 3 ? cell_of(pz, oz) : 0;
+        if (DIM == 3 && mzc >= pr.nsub[2]) continue;
+        const unsigned long long ymask =
+            mask[1] & bits_within(base[1], yi[1], (int)(sqrtf(remz) * pr.fixed.ih[1]) + 1);
+        for (unsigned long long my = ymask; my; my &= my - 1) {
             const int py = base[1] + __builtin_ctzll(my);
-            for (unsigned long long mx = mask[0]; mx; mx &= mx - 1) {
+            const float dy = fmaf(pr.fixed.h[1], (float)py, pr.fixed.o[1]) - y[1];
+            const float remy = remz - dy * dy;
+            if (!(remy > 0.0f)) continue;
+            int oy = 0;
+            const int myc = cell_of(py, oy);
+            if (myc >= pr.nsub[1]) continue;
+            const long long rowbase = (long long)pr.nsub[0] * (myc + (long long)pr.nsub[1] * mzc);
+            const unsigned wyz = 0x80000000u | ((unsigned)oy << 8) | (DIM == 3 ? (unsigned)oz << 16 : 0u);
+            const unsigned long long xmask =
+                mask[0] & bits_within(base[0], yi[0], (int)(sqrtf(remy) * pr.fixed.ih[0]) + 1);
+            for (unsigned long long mx = xmask; mx; mx &= mx - 1) {
                 const int px = base[0] + __builtin_ctzll(mx);
-                const int pos[3] = {px, py, pz};
-                // the neighbourhood weight first: it needs only the position,
-                // and about half the candidates lie outside the support
+                int ox = 0;
+                const int mxc = cell_of(px, ox);
+                if (mxc >= pr.nsub[0]) continue;
+                const long long s = rowbase + mxc;
+                // a centre at this position whose coefficients are not all zero
+                if (cpk[s] != (wyz | (unsigned)ox)) continue;
                 const float fp[3] = {(float)px, (float)py, (float)pz};
                 float x[3];
                 to_global<DIM>(pr.fixed, fp, x);
@@ -659,18 +699,6 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4
                 for (int d = 0; d < DIM; ++d) r2 += (x[d] - y[d]) * (x[d] - y[d]);
                 const float kw = neighborhood_weight(r2, pr.kc.smin);
                 if (!(kw > 0.0f)) continue;
-                int m[3] = {0, 0, 0};
-                bool cell_ok = true;
-                unsigned want = 0x80000000u;  // a centre at pos whose coefficients are not all zero
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) {
-                    m[d] = pos[d] / sub;
-                    cell_ok = cell_ok && m[d] < pr.nsub[d];
-                    want |= (unsigned)(pos[d] - m[d] * sub) << (8 * d);
-                }
-                if (!cell_ok) continue;
-                const long long s = m[0] + (long long)pr.nsub[0] * (m[1] + (long long)pr.nsub[1] * m[2]);
-                if (cpk[s] != want) continue;
                 const float4 abc = coef[s];
                 const double kw2 = (double)kw * (double)kw;
                 sa += (double)abc.x * kw2;
