@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--batch-concurrency", type=int, default=4, help="registrations in flight in the batched run")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--warm-pairs", type=int, default=8, help="held-out pairs of the warm-start comparison")
+    ap.add_argument("--no-lockstep", action="store_true",
+                    help="skip the lockstep (graph-replay) runs of the batch and warm-start legs")
     ap.add_argument("--registration-repeats", type=int, default=3)
     ap.add_argument("--registration-rd", type=float, default=1e-5,
                     help="RD weight of the registration (the bilateral-like regularizer)")
@@ -584,15 +586,17 @@ def batch_timing(args):
         results[conc] = register_batch(pairs, cfg, concurrency=conc)
         ms = (time.perf_counter() - t0) * 1e3
         out[f"concurrency_{conc}"] = {"ms": round(ms, 1), "ms_per_registration": round(ms / len(pairs), 2)}
-    t0 = time.perf_counter()
-    results["lockstep"] = register_batch(pairs, cfg, lockstep=True)
-    ms = (time.perf_counter() - t0) * 1e3
-    out["lockstep"] = {"ms": round(ms, 1), "ms_per_registration": round(ms / len(pairs), 2)}
+    if not args.no_lockstep:
+        t0 = time.perf_counter()
+        results["lockstep"] = register_batch(pairs, cfg, lockstep=True)
+        ms = (time.perf_counter() - t0) * 1e3
+        out["lockstep"] = {"ms": round(ms, 1), "ms_per_registration": round(ms / len(pairs), 2)}
     a, b = results[1], results[args.batch_concurrency]
-    out["identical"] = all(np.array_equal(x[0], y[0]) and np.array_equal(x[0], z[0])
-                           for x, y, z in zip(a, b, results["lockstep"]))
+    out["identical"] = all(np.array_equal(x[0], y[0]) and ("lockstep" not in results or np.array_equal(x[0], z[0]))
+                           for x, y, z in zip(a, b, results.get("lockstep", b)))
     out["speedup"] = round(out["concurrency_1"]["ms"] / out[f"concurrency_{args.batch_concurrency}"]["ms"], 3)
-    out["speedup_lockstep"] = round(out["concurrency_1"]["ms"] / out["lockstep"]["ms"], 3)
+    if "lockstep" in out:
+        out["speedup_lockstep"] = round(out["concurrency_1"]["ms"] / out["lockstep"]["ms"], 3)
     inner = (slice(10, -10),) * 3
     out["final_ncc_mean"] = round(float(np.mean([r[1][-1]["measure"] for r in b])), 6)
     out["first_value_mean"] = round(float(np.mean([r[1][0]["first_value"] for r in b])), 6)
@@ -681,11 +685,13 @@ def warm_start_timing(args):
     res = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), concurrency=args.batch_concurrency,
                          initial_displacements=fields, initial_guard=True)
     ms_batch = (time.perf_counter() - t0) * 1e3 / len(pairs)
-    t0 = time.perf_counter()
-    fields = [ws.field(ws.predict(r, t)) for r, t in pairs]
-    res_ls = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), lockstep=True,
-                            initial_displacements=fields, initial_guard=True)
-    ms_lockstep = (time.perf_counter() - t0) * 1e3 / len(pairs)
+    res_ls, ms_lockstep = None, None
+    if not args.no_lockstep:
+        t0 = time.perf_counter()
+        fields = [ws.field(ws.predict(r, t)) for r, t in pairs]
+        res_ls = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), lockstep=True,
+                                initial_displacements=fields, initial_guard=True)
+        ms_lockstep = (time.perf_counter() - t0) * 1e3 / len(pairs)
     net = network_timing(ws.net.in_size, ws.cp)
     ws.close()
     plain_ms = float(np.mean([r["plain_ms"] for r in rows]))
@@ -694,10 +700,11 @@ def warm_start_timing(args):
                         "3 levels x 200 iterations, warm = prediction + 50 fine-level iterations",
             "plain_ms_per_registration": round(plain_ms, 2), "warm_ms_per_registration": round(warm_ms, 2),
             "warm_batched_ms_per_registration": round(ms_batch, 2),
-            "warm_lockstep_ms_per_registration": round(ms_lockstep, 2),
+            "warm_lockstep_ms_per_registration": None if ms_lockstep is None else round(ms_lockstep, 2),
             "speedup_single": round(plain_ms / warm_ms, 2), "speedup_batched": round(plain_ms / ms_batch, 2),
-            "speedup_lockstep": round(plain_ms / ms_lockstep, 2),
-            "lockstep_identical_to_batched": all(np.array_equal(x[0], y[0]) for x, y in zip(res, res_ls)),
+            "speedup_lockstep": None if ms_lockstep is None else round(plain_ms / ms_lockstep, 2),
+            "lockstep_identical_to_batched": None if res_ls is None else
+            all(np.array_equal(x[0], y[0]) for x, y in zip(res, res_ls)),
             "plain_final_ncc": round(float(np.mean([r["plain_ncc"] for r in rows])), 5),
             "warm_final_ncc": round(float(np.mean([r["warm_ncc"] for r in rows])), 5),
             "warm_batched_final_ncc": round(float(np.mean([r[1][0]["measure"] for r in res])), 5),

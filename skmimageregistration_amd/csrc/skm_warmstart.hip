@@ -203,8 +203,19 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradGeom g, const float *__
     const bool kvalid = kk < g.kdim;
     const int dzt = tap / 9, dyt = (tap / 3) % 3, dxt = tap % 3;
     f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // coordinates of this lane's first voxel of a step, decoded once and then
+    // advanced: by one voxel per element, by 32 per step
+    int ox, oy, oz, ob;
+    {
+        long long t = v0 + kq * 8;
+        ox = (int)(t % g.ow); t /= g.ow;
+        oy = (int)(t % g.oh); t /= g.oh;
+        oz = (int)(t % g.od); t /= g.od;
+        ob = (int)t;
+    }
     for (long long vb = v0; vb < v1; vb += 32) {
         bf16x8 a, bw;
+        int cx = ox, cy = oy, cz = oz, cb = ob;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const long long o = vb + kq * 8 + e;
@@ -212,20 +223,36 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(WgradGeom g, const float *__
             if (o < v1) {
                 av = dz[o * g.cout + co0 + r];
                 if (kvalid) {
-                    long long t = o;
-                    const int ox = (int)(t % g.ow); t /= g.ow;
-                    const int oy = (int)(t % g.oh); t /= g.oh;
-                    const int oz = (int)(t % g.od); t /= g.od;
-                    const int b = (int)t;
-                    const int sz = oz * g.stride + dzt - 1, sy = oy * g.stride + dyt - 1, sx = ox * g.stride + dxt - 1;
+                    const int sz = cz * g.stride + dzt - 1, sy = cy * g.stride + dyt - 1, sx = cx * g.stride + dxt - 1;
                     if (sz >= 0 && sy >= 0 && sx >= 0 && sz < g.id && sy < g.ih && sx < g.iw)
-                        bv = x[((((long long)b * g.id + sz) * g.ih + sy) * g.iw + sx) * g.cin + ci];
+                        bv = x[((((long long)cb * g.id + sz) * g.ih + sy) * g.iw + sx) * g.cin + ci];
                 }
             }
             a[e] = to_bf16(av);
             bw[e] = to_bf16(bv);
+            if (++cx == g.ow) {
+                cx = 0;
+                if (++cy == g.oh) {
+                    cy = 0;
+                    if (++cz == g.od) {
+                        cz = 0;
+                        ++cb;
+                    }
+                }
+            }
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc, 0, 0, 0);
+        ox += 32;
+        while (ox >= g.ow) {
+            ox -= g.ow;
+            if (++oy == g.oh) {
+                oy = 0;
+                if (++oz == g.od) {
+                    oz = 0;
+                    ++ob;
+                }
+            }
+        }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -245,8 +272,29 @@ __global__ __launch_bounds__(256) void k_wgrad_reduce(const float *__restrict__ 
     dw[i] = s;
 }
 
-// Bias gradient: db[c] = sum_v dz[v][c], per channel over fixed chunks of
-// voxels summed in order (one workgroup per channel, FP32 wave sums, FP64 total).
+// Bias gradient, first stage: per chunk of kWdwChunk voxels the FP32 sum of
+// dz[v][c] of every channel (lanes over 64 channels at a time, reading a
+// voxel's channels together; the four waves take every fourth voxel and are
+// added in wave order).
+__global__ __launch_bounds__(256) void k_bias_part(const float *__restrict__ dz, long long nvox, int c_count,
+                                                   float *__restrict__ part)
+{
+    __shared__ float lds[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = (int)blockIdx.y * 64 + lane;
+    const long long v0 = (long long)blockIdx.x * kWdwChunk, v1 = std::min<long long>(nvox, v0 + kWdwChunk);
+    float s = 0.0f;
+    if (c < c_count)
+        for (long long v = v0 + wave; v < v1; v += 4) s += dz[v * c_count + c];
+    lds[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && c < c_count)
+        part[(long long)blockIdx.x * c_count + c] = ((lds[0][lane] + lds[1][lane]) + lds[2][lane]) + lds[3][lane];
+}
+
+// Bias gradient, second stage (and any per-channel column sum): db[c] =
+// sum_v dz[v][c], one workgroup per channel, each thread every 256th row in
+// order, FP64 wave sums in a fixed butterfly, the four waves in order.
 __global__ __launch_bounds__(256) void k_bias_grad(const float *__restrict__ dz, long long nvox, int c_count,
                                                    float *__restrict__ db)
 {
@@ -563,8 +611,8 @@ struct skm_ws_net {
     int last_batch = 0;
     float *grad_a = nullptr, *grad_b = nullptr, *grad_skip3 = nullptr, *grad_skip2 = nullptr, *dsamp = nullptr,
           *dout = nullptr;
-    float *wpart = nullptr;
-    long long wpart_n = 0;
+    float *wpart = nullptr, *bpart = nullptr;
+    long long wpart_n = 0, bpart_n = 0;
     double *loss_part = nullptr;
     long long adam_t = 0;
     hipStream_t stream = nullptr;
@@ -573,7 +621,7 @@ struct skm_ws_net {
     {
         if (stream) (void)hipStreamSynchronize(stream);
         void *ptrs[] = {p, g, m, v, wh, cat3, cat2, samp, out, grad_a, grad_b, grad_skip3, grad_skip2, dsamp, dout,
-                        wpart, loss_part};
+                        wpart, bpart, loss_part};
         for (void *q : ptrs)
             if (q) device_release(q);
         for (int l = 0; l < kWsLayers; ++l) {
@@ -661,7 +709,9 @@ void conv_backward_weights(skm_ws_net *n, int l, const float *src, const float *
     k_conv_wgrad<<<dim3((unsigned)((tiles + 3) / 4), (unsigned)g.chunks), 256, 0, n->stream>>>(g, src, dz, n->wpart);
     k_wgrad_reduce<<<blocks_of((long long)g.cout * g.kdim), 256, 0, n->stream>>>(n->wpart, g.chunks, g.cout, g.kdim,
                                                                                 g.kpad, n->g + n->w_off[l]);
-    k_bias_grad<<<(unsigned)g.cout, 256, 0, n->stream>>>(dz, nvox, g.cout, n->g + n->b_off[l]);
+    k_bias_part<<<dim3((unsigned)g.chunks, (unsigned)((g.cout + 63) / 64)), 256, 0, n->stream>>>(dz, nvox, g.cout,
+                                                                                            n->bpart);
+    k_bias_grad<<<(unsigned)g.cout, 256, 0, n->stream>>>(n->bpart, g.chunks, g.cout, n->g + n->b_off[l]);
 }
 
 SampleGeom sample_geom(const skm_ws_net *n, int batch)
@@ -844,6 +894,7 @@ SKM_API skm_status skm_ws_create(const skm_ws_desc *desc, skm_ws_net **out)
         gmax = std::max(gmax, std::max(na, B * n->level_vox(sl) * kCin[l]));
         const long long chunks = (B * n->level_vox(kLevel[l]) + kWdwChunk - 1) / kWdwChunk;
         n->wpart_n = std::max(n->wpart_n, chunks * kCout[l] * n->kpad[l]);
+        n->bpart_n = std::max(n->bpart_n, chunks * kCout[l]);
     }
     if ((e = ws_alloc(&n->cat3, B * n->level_vox(2) * kCin[8])) || (e = ws_alloc(&n->cat2, B * n->level_vox(1) * kCin[9])) ||
         (e = ws_alloc(&n->samp, B * ncp * kHeadIn)) || (e = ws_alloc(&n->out, B * ncp * kHeadOut)) ||
@@ -851,6 +902,7 @@ SKM_API skm_status skm_ws_create(const skm_ws_desc *desc, skm_ws_net **out)
         (e = ws_alloc(&n->grad_a, gmax)) || (e = ws_alloc(&n->grad_b, gmax)) ||
         (e = ws_alloc(&n->grad_skip3, B * n->level_vox(2) * 48)) ||
         (e = ws_alloc(&n->grad_skip2, B * n->level_vox(1) * 32)) || (e = ws_alloc(&n->wpart, n->wpart_n)) ||
+        (e = ws_alloc(&n->bpart, n->bpart_n)) ||
         (e = ws_alloc(&n->loss_part, (long long)blocks_of(B * ncp * kHeadOut))))
         return bail(e);
     if ((e = hipMemset(n->p, 0, sizeof(float) * off)) || (e = hipMemset(n->m, 0, sizeof(float) * off)) ||
