@@ -37,6 +37,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWsLayers = 10;      // 3x3x3 convolutions
 constexpr int kHeadIn = 32, kHeadOut = 3;
 constexpr int kWdwChunk = 2048;    // voxels per workgroup of the weight-gradient GEMM
+constexpr int kBiasChunk = 256;    // voxels per workgroup of the bias gradient's first stage
 constexpr int kMaxColTiles = 8;    // GEMM columns per wave: 8 x 16 = 128 channels (the widest is 112)
 
 // (input channels, output channels, stride, level of the output) per conv
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(256) void k_wgrad_reduce(const float *__restrict__ 
     dw[i] = s;
 }
 
-// Bias gradient, first stage: per chunk of kWdwChunk voxels the FP32 sum of
+// Bias gradient, first stage: per chunk of kBiasChunk voxels the FP32 sum of
 // dz[v][c] of every channel (lanes over 64 channels at a time, reading a
 // voxel's channels together; the four waves take every fourth voxel and are
 // added in wave order).
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(256) void k_bias_part(const float *__restrict__ dz,
     __shared__ float lds[4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = (int)blockIdx.y * 64 + lane;
-    const long long v0 = (long long)blockIdx.x * kWdwChunk, v1 = std::min<long long>(nvox, v0 + kWdwChunk);
+    const long long v0 = (long long)blockIdx.x * kBiasChunk, v1 = std::min<long long>(nvox, v0 + kBiasChunk);
     float s = 0.0f;
     if (c < c_count)
         for (long long v = v0 + wave; v < v1; v += 4) s += dz[v * c_count + c];
@@ -709,9 +710,10 @@ void conv_backward_weights(skm_ws_net *n, int l, const float *src, const float *
     k_conv_wgrad<<<dim3((unsigned)((tiles + 3) / 4), (unsigned)g.chunks), 256, 0, n->stream>>>(g, src, dz, n->wpart);
     k_wgrad_reduce<<<blocks_of((long long)g.cout * g.kdim), 256, 0, n->stream>>>(n->wpart, g.chunks, g.cout, g.kdim,
                                                                                 g.kpad, n->g + n->w_off[l]);
-    k_bias_part<<<dim3((unsigned)g.chunks, (unsigned)((g.cout + 63) / 64)), 256, 0, n->stream>>>(dz, nvox, g.cout,
-                                                                                            n->bpart);
-    k_bias_grad<<<(unsigned)g.cout, 256, 0, n->stream>>>(n->bpart, g.chunks, g.cout, n->g + n->b_off[l]);
+    const long long bchunks = (nvox + kBiasChunk - 1) / kBiasChunk;
+    k_bias_part<<<dim3((unsigned)bchunks, (unsigned)((g.cout + 63) / 64)), 256, 0, n->stream>>>(dz, nvox, g.cout,
+                                                                                          n->bpart);
+    k_bias_grad<<<(unsigned)g.cout, 256, 0, n->stream>>>(n->bpart, bchunks, g.cout, n->g + n->b_off[l]);
 }
 
 SampleGeom sample_geom(const skm_ws_net *n, int batch)
@@ -894,7 +896,7 @@ SKM_API skm_status skm_ws_create(const skm_ws_desc *desc, skm_ws_net **out)
         gmax = std::max(gmax, std::max(na, B * n->level_vox(sl) * kCin[l]));
         const long long chunks = (B * n->level_vox(kLevel[l]) + kWdwChunk - 1) / kWdwChunk;
         n->wpart_n = std::max(n->wpart_n, chunks * kCout[l] * n->kpad[l]);
-        n->bpart_n = std::max(n->bpart_n, chunks * kCout[l]);
+        n->bpart_n = std::max(n->bpart_n, (B * n->level_vox(kLevel[l]) + kBiasChunk - 1) / kBiasChunk * kCout[l]);
     }
     if ((e = ws_alloc(&n->cat3, B * n->level_vox(2) * kCin[8])) || (e = ws_alloc(&n->cat2, B * n->level_vox(1) * kCin[9])) ||
         (e = ws_alloc(&n->samp, B * ncp * kHeadIn)) || (e = ws_alloc(&n->out, B * ncp * kHeadOut)) ||
