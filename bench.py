@@ -597,6 +597,8 @@ def batch_timing(args):
     out["speedup"] = round(out["concurrency_1"]["ms"] / out[f"concurrency_{args.batch_concurrency}"]["ms"], 3)
     if "lockstep" in out:
         out["speedup_lockstep"] = round(out["concurrency_1"]["ms"] / out["lockstep"]["ms"], 3)
+    if not args.no_warm_start:
+        out["warm_start"] = series_warm_start(args, pairs, b, out["concurrency_1"]["ms_per_registration"])
     inner = (slice(10, -10),) * 3
     out["final_ncc_mean"] = round(float(np.mean([r[1][-1]["measure"] for r in b])), 6)
     out["first_value_mean"] = round(float(np.mean([r[1][0]["first_value"] for r in b])), 6)
@@ -604,6 +606,54 @@ def batch_timing(args):
                                                      for r, d in zip(b, disps[1:])])), 4)
     out["mean_abs_displacement_vox"] = round(float(np.mean([np.abs(d[inner]).mean() for d in disps[1:]])), 4)
     return out
+
+
+def series_warm_start(args, pairs, plain, plain_single_ms):
+    """The time series with the learned warm start: per frame the network's
+    prediction (one network per frame size; its weights were trained on
+    128x128x96 pairs only) and its dense field, then the registration leg's
+    last level alone (support s, tensor weight / 4, sampling rate 3, RD) for
+    50 iterations from that field, guarded, --batch-concurrency frames in
+    flight; also the same 50 iterations from zero.  Wall time per
+    registration includes the predictions; final NCC against the plain
+    3-level runs (`plain`, the same frames)."""
+    from skmimageregistration_amd import RegistrationConfig, register_batch
+    from skmimageregistration_amd.warmstart import WarmStart
+    weights = os.path.join(REPO, "skmimageregistration_amd", "weights", "warmstart_128x128x96.bin")
+    if not os.path.exists(weights):
+        return None
+    s = args.sigma
+    fine = RegistrationConfig(sigma_C4=[s], sigma_C0=[s], tensor_weights=[args.tensor_weight / 4]
+                              if args.kernel == "anisotropic" else [0.0], num_scales=1,
+                              num_function_evaluations=[50], initial_step_size=[1000.0], sampling_rate=[3],
+                              neighborhood_sampling=[1], smoothing_variance=[0], reg_rd=[args.registration_rd],
+                              metric="ncc")
+    nets = {}
+    for _, t in pairs:
+        if t.data.shape not in nets:
+            nets[t.data.shape] = WarmStart(shape=t.data.shape, weights=weights)
+            nets[t.data.shape].predict(t.data, t.data)  # first-call set-up, untimed
+    register_batch(pairs[:1], fine, concurrency=1)
+    t0 = time.perf_counter()
+    fields = [nets[t.data.shape].field(nets[t.data.shape].predict(r.data, t.data)) for r, t in pairs]
+    warm = register_batch(pairs, fine, concurrency=args.batch_concurrency, initial_displacements=fields,
+                          initial_guard=True)
+    ms = (time.perf_counter() - t0) * 1e3 / len(pairs)
+    zero = register_batch(pairs, fine, concurrency=args.batch_concurrency)
+    for ws in nets.values():
+        ws.close()
+    gaps = [abs(w[1][0]["measure"] - p[1][-1]["measure"]) / abs(p[1][-1]["measure"]) for w, p in zip(warm, plain)]
+    used = {}
+    for (_, t), w in zip(pairs, warm):
+        k = "x".join(str(v) for v in reversed(t.data.shape))
+        used.setdefault(k, [0, 0])
+        used[k][0] += int(w[1][0]["initial_used"])
+        used[k][1] += 1
+    return {"ms_per_registration": round(ms, 2), "speedup_vs_plain_single": round(plain_single_ms / ms, 2),
+            "final_ncc_mean": round(float(np.mean([w[1][0]["measure"] for w in warm])), 6),
+            "zero_start_final_ncc_mean": round(float(np.mean([z[1][0]["measure"] for z in zero])), 6),
+            "max_rel_ncc_gap_to_plain": round(float(max(gaps)), 5),
+            "predictions_used_per_size": {k: f"{u}/{n}" for k, (u, n) in used.items()}}
 
 
 BF16_DENSE_PEAK_TFLOPS = 2500.0

@@ -259,6 +259,16 @@ def load():
         raise RuntimeError(
             f"libskm.so not found at {LIB_PATH}; build it with "
             "`python -m skmimageregistration_amd.build` (there is no fallback path)")
+    # One HIP runtime per process: PyTorch's ROCm wheel bundles its own
+    # libamdhip64.so.7, and whichever copy is loaded first serves every
+    # library that needs that soname.  Loading torch first makes libskm use
+    # torch's runtime, so that torch tensors and streams handed to libskm and
+    # torch's own later GPU use both work (the other order leaves torch on a
+    # runtime it was not built for, and it then finds no GPU).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)
