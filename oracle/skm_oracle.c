@@ -348,7 +348,7 @@ int skmo_bspline_sample(int dim, const skmo_grid *g, const double *coeffs, const
     for (int d = 0; d < dim; ++d)
         if (idx[d] < -0.5 || idx[d] + 3.5 >= g->size[d] + 3) return 0;
     int base[3] = {0, 0, 0};
-    double w[3][4], dw[3][4];
+    double w[3][4] = {{0}}, dw[3][4] = {{0}};
     for (int d = 0; d < dim; ++d) {
         base[d] = trunc_int(idx[d]);
         double t = idx[d] - base[d];
