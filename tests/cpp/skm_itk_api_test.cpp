@@ -80,7 +80,7 @@ static TransformType::Pointer make_transform(ImageType::Pointer fixed, ImageType
     return t;
 }
 
-static void fd_test(bool ncc)
+static void fd_test(bool ncc, bool regularized = false)
 {
     ImageType::SizeType size{{40, 36, 32}};
     auto fimage = blobs(size, 7, 0.0);
@@ -92,6 +92,14 @@ static void fd_test(bool ncc)
     metric->SetMovingImage(mimage);
     metric->SetTransform(transform);
     if (ncc) metric->UseNccMetricOn();
+    if (regularized) {
+        // RD and the bilateral term (spatial sigma 2 control-point spacings,
+        // intensity sigma 0.3) add their values and derivatives to the metric's
+        metric->SetRegularizerRD(2e-3);
+        metric->SetRegularizerBilateral(3e-3);
+        metric->SetRegularizerBilateralSpatialSigma(8.0);
+        metric->SetRegularizerBilateralIntensitySigma(0.3);
+    }
     metric->SetSubSample(2);
     metric->SetSubSampleNeighborhood(3);
     metric->DoResamplingOff();
@@ -130,8 +138,8 @@ static void fd_test(bool ncc)
         err_sum += err;
         EXPECT(err <= 2e-2 * maxd + 1e-7, "FD %zu: fd %.6g derivative %.6g", idx, fd, deriv[idx]);
     }
-    std::printf("%s: value %.8f, average |FD - derivative| %.3e (max |derivative| %.3e)\n", ncc ? "NCC" : "MSE", val,
-                err_sum / 20, maxd);
+    std::printf("%s%s: value %.8f, average |FD - derivative| %.3e (max |derivative| %.3e)\n", ncc ? "NCC" : "MSE",
+                regularized ? " + RD + bilateral" : "", val, err_sum / 20, maxd);
     transform->SetParameters(r);
 }
 
@@ -244,6 +252,7 @@ int main()
     validation_test();
     fd_test(false);
     fd_test(true);
+    fd_test(false, true);
     transform_point_test();
     optimizer_test();
     std::printf(failures ? "FAILED (%d)\n" : "OK\n", failures);
