@@ -70,10 +70,14 @@ def python_loop(reference, target, cfg):
                                 Image(ts, target.origin, target.spacing), cpg)
         if cfg.CURRENT("reg_rd", s) > 0:
             ev.SetRegularizerRD(cfg.CURRENT("reg_rd", s), cfg.CURRENT("reg_rd_scaling", s))
+        if cfg.CURRENT("reg_bilateral", s) > 0:
+            ev.SetRegularizerBilateral(cfg.CURRENT("reg_bilateral", s), cfg.CURRENT("reg_bilateral_spatial_sigma", s),
+                                       cfg.CURRENT("reg_bilateral_intensity_sigma", s))
         prev_img = None if prev is None else Image(prev, target.origin, target.spacing, dim=dim)
         ev.SetDisplacementField(prev_img)
         regs = {k: float(cfg.CURRENT(k, s)) for k in ("reg_l1", "reg_l21", "reg_l2")}
-        only = [k for k in ("reg_l1", "reg_l21", "reg_l2", "reg_rkhs", "reg_rd", "reg_pg") if cfg.CURRENT(k, s) > 0]
+        only = [k for k in ("reg_l1", "reg_l21", "reg_l2", "reg_rkhs", "reg_rd", "reg_pg", "reg_bilateral")
+                if cfg.CURRENT(k, s) > 0]
         orthant = only in (["reg_l1"], ["reg_l21"])
         params, recs = ev.optimize(np.zeros(cpg.count() * dim, np.float32), metric=measure, **regs,
                                    kind=_capi.SKM_OPT_ADAPTIVE, a=cfg.CURRENT("initial_step_size", s), A=20.0,
@@ -92,15 +96,16 @@ def _case3d():
     return Image(mov), Image(fix), disp
 
 
-@pytest.mark.parametrize("metric,tensor", [("ncc", 400.0), ("mse", 0.0)])
-def test_driver_matches_python_loop(metric, tensor):
+@pytest.mark.parametrize("metric,tensor,bilateral", [("ncc", 400.0, 0.0), ("mse", 0.0, 0.0), ("ncc", 400.0, 1e-3)])
+def test_driver_matches_python_loop(metric, tensor, bilateral):
     reference, target, _ = _case3d()
     # thresholds inside the intensity range so that ThresholdImage clips
     cfg = RegistrationConfig(sigma_C4=[16, 8], sigma_C0=[16, 8], tensor_weights=[tensor], num_scales=2,
                              num_function_evaluations=[15, 10], initial_step_size=[0.5], sampling_rate=[2, 1],
                              neighborhood_sampling=[1], smoothing_variance=[2, 0], reg_rd=[1e-3],
-                             intensity_thresholds=[0.05, 0.8], metric=metric)
-    print(f"driver vs python loop metric={metric} tensor={tensor}", flush=True)
+                             reg_bilateral=[bilateral], reg_bilateral_spatial_sigma=[8.0],
+                             reg_bilateral_intensity_sigma=[0.2], intensity_thresholds=[0.05, 0.8], metric=metric)
+    print(f"driver vs python loop metric={metric} tensor={tensor} bilateral={bilateral}", flush=True)
     df, reports = register_multiscale(reference, target, cfg)
     ref_df = python_loop(reference, target, cfg)
     assert np.isfinite(df).all() and np.abs(df).max() > 0
