@@ -267,7 +267,7 @@ def load():
     # runtime it was not built for, and it then finds no GPU).
     try:
         import torch  # noqa: F401
-    except ImportError:
+    except Exception:  # noqa: BLE001 -- without a usable torch the system HIP runtime serves libskm alone
         pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
