@@ -123,6 +123,17 @@ def test_lcc(dim, exact):
     _compare(case, MEASURE.LCC)
 
 
+@pytest.mark.parametrize("sub,subn,spacing", [(3, 3, (0.8, 1.25, 1.0)), (2, 1, (1.0, 1.0, 1.5)), (1, 2, (1.2, 0.9, 1.1))])
+def test_lcc_exact_geometry(sub, subn, spacing):
+    """The exact-gradient LCC derivative's per-voxel gather (stationary
+    kernels) with anisotropic voxel spacing, subsample 1, 2 and 3 and
+    neighbourhood strides 1-3: the reach masks and the support ranges per axis
+    in voxel units, the cell index of a position, the jitter match."""
+    case = Case(dim=3, shape=(14, 15, 16), cp_spacing=4.0, sigma=6.5, subsample=sub, subsample_neighborhood=subn,
+                exact=True, seed=33, coef_scale=0.8, spacing=spacing, origin=(1.5, -2.0, 0.25))
+    _compare(case, MEASURE.LCC)
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_lcc_weighted(mode):
     case = Case(dim=3, shape=(12, 13, 14), cp_spacing=4.0, sigma=7.0, mode=mode, subsample=2,
