@@ -16,7 +16,9 @@
  *   derivative pass, one iteration per control point over the voxels of its
  *     3D window (evaluateDerivativeNCC, GpuEvaluatorNCC.cu:111-190);
  *   RD regularizer per control point over its control-point window
- *     (regularizerRD, GpuEvaluatorRegularizers.cu:96-166).
+ *     (regularizerRD, GpuEvaluatorRegularizers.cu:96-166);
+ *   the bilateral regularizer (no reference counterpart; the library's
+ *   definition, RD's structure with a spatial and an intensity Gaussian).
  * Images: spacing 1, origin 0, x fastest.  Build: -O3 -march=native -fopenmp.
  */
 #include <math.h>
@@ -292,5 +294,67 @@ int skmf_value_and_derivative_ncc(const skmf_problem *pr, double rd_weight, doub
     }
     *rd_value = rd;
     free(f); free(m); free(g); free(in); free(cpcov); free(cpsup);
+    return 0;
+}
+
+/* The bilateral regularizer: per control point j over the points i within
+ * 3 sigma_s, B = exp(-r^2 / 2 sigma_s^2) exp(-(I_j - I_i)^2 / 2 sigma_i^2)
+ * with I the fixed image at the control points (nearest voxel, 0 outside),
+ * value_j = w_j sum B |c_j - c_i|^2 and d/dc_j = sum 2 B (w_j + w_i)(c_j - c_i);
+ * weight x the derivative is added to deriv_soa, the sum of the values is
+ * returned in *value.  Returns 0 on success. */
+int skmf_bilateral(const skmf_problem *pr, double weight, double sigma_s, double sigma_i, double *value,
+                   float *deriv_soa)
+{
+    const long long ncp = (long long)pr->cp_size[0] * pr->cp_size[1] * pr->cp_size[2];
+    float *inten = malloc(sizeof(float) * ncp);
+    if (!inten) return -1;
+#pragma omp parallel for schedule(static)
+    for (long long j = 0; j < ncp; ++j) {
+        int jc[3] = {(int)(j % pr->cp_size[0]), (int)((j / pr->cp_size[0]) % pr->cp_size[1]),
+                     (int)(j / ((long long)pr->cp_size[0] * pr->cp_size[1]))};
+        int v[3], inside = 1;
+        for (int d = 0; d < 3; ++d) {
+            v[d] = (int)(pr->cp_origin[d] + pr->cp_spacing[d] * jc[d] + 0.5f);
+            if (v[d] < 0 || v[d] >= pr->size[d]) inside = 0;
+        }
+        inten[j] = inside ? pr->fixed[v[0] + (long long)pr->size[0] * (v[1] + (long long)pr->size[1] * v[2])] : 0.0f;
+    }
+    const float cut = 3.0f * (float)sigma_s, inv2ss = 1.0f / (2.0f * (float)(sigma_s * sigma_s)),
+                inv2si = 1.0f / (2.0f * (float)(sigma_i * sigma_i));
+    double total = 0.0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (long long j = 0; j < ncp; ++j) {
+        int jc[3] = {(int)(j % pr->cp_size[0]), (int)((j / pr->cp_size[0]) % pr->cp_size[1]),
+                     (int)(j / ((long long)pr->cp_size[0] * pr->cp_size[1]))};
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = clampi((int)(jc[d] - cut / pr->cp_spacing[d] + 1.0f), 0, pr->cp_size[d] - 1);
+            hi[d] = clampi((int)(jc[d] + cut / pr->cp_spacing[d]), 0, pr->cp_size[d] - 1);
+        }
+        const float *cj = pr->cp_aos + 3 * j;
+        const float wj = pr->cpw[j];
+        float val = 0.0f, der[3] = {0, 0, 0};
+        for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int iy = lo[1]; iy <= hi[1]; ++iy)
+                for (int ix = lo[0]; ix <= hi[0]; ++ix) {
+                    float v0 = pr->cp_spacing[0] * (jc[0] - ix), v1 = pr->cp_spacing[1] * (jc[1] - iy),
+                          v2 = pr->cp_spacing[2] * (jc[2] - k);
+                    float r2 = v0 * v0 + v1 * v1 + v2 * v2;
+                    if (!(r2 < cut * cut)) continue;
+                    long long i = ix + (long long)pr->cp_size[0] * (iy + (long long)pr->cp_size[1] * k);
+                    float di = inten[j] - inten[i];
+                    float b = expf(-r2 * inv2ss) * expf(-di * di * inv2si);
+                    const float *ci = pr->cp_aos + 3 * i;
+                    float e[3] = {cj[0] - ci[0], cj[1] - ci[1], cj[2] - ci[2]};
+                    val += wj * b * (e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+                    float f2 = 2.0f * b * (wj + pr->cpw[i]);
+                    for (int d = 0; d < 3; ++d) der[d] += f2 * e[d];
+                }
+        total += val;
+        for (int d = 0; d < 3; ++d) deriv_soa[j + d * ncp] += (float)weight * der[d];
+    }
+    *value = total;
+    free(inten);
     return 0;
 }

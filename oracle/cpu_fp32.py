@@ -39,6 +39,9 @@ def load():
         lib.skmf_value_and_derivative_ncc.restype = ctypes.c_int
         lib.skmf_value_and_derivative_ncc.argtypes = [ctypes.POINTER(Problem), ctypes.c_double, c_double_p,
                                                       c_double_p, c_float_p]
+        lib.skmf_bilateral.restype = ctypes.c_int
+        lib.skmf_bilateral.argtypes = [ctypes.POINTER(Problem), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       c_double_p, c_float_p]
         _lib = lib
     return _lib
 
@@ -77,10 +80,18 @@ class CpuProblem:
         self.p = p
         self.n_cp = int(np.prod(size))
 
-    def evaluate(self, rd_weight=0.0):
+    def evaluate(self, rd_weight=0.0, bilateral=None):
+        """(NCC, RD value, derivative SoA) -- with bilateral = (weight, sigma_s,
+        sigma_i) also the bilateral term: (NCC, RD value, bilateral value, derivative)."""
         der = np.zeros(self.n_cp * 3, np.float32)
         measure, rd = ctypes.c_double(), ctypes.c_double()
         if self.lib.skmf_value_and_derivative_ncc(ctypes.byref(self.p), rd_weight, ctypes.byref(measure),
                                                   ctypes.byref(rd), fp(der)) != 0:
             raise RuntimeError("cpu_fp32 evaluation failed")
-        return measure.value, rd.value, der
+        if bilateral is None:
+            return measure.value, rd.value, der
+        bv = ctypes.c_double()
+        w, ss, si = bilateral
+        if self.lib.skmf_bilateral(ctypes.byref(self.p), float(w), float(ss), float(si), ctypes.byref(bv), fp(der)) != 0:
+            raise RuntimeError("cpu_fp32 bilateral failed")
+        return measure.value, rd.value, bv.value, der
