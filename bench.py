@@ -5,7 +5,8 @@ Workload (default): the 3D registration of 128x128x96 float32 volumes
 points every 6 voxels (22x22x16 = 7,744, 23,232 parameters), Wendland support
 24 voxels, anisotropic kernel tensors computed from the target's structure
 (ComputeAnisotropicTensor with tensor weight 400, alignment z, as SKMReg does
-by default), NCC metric, RD regularizer, dense sampling.  One step is one
+by default), NCC metric, bilateral regularizer (weight 1e-3, spatial sigma
+12, intensity sigma 0.1), dense sampling.  One step is one
 value-and-derivative evaluation with every input resident in HBM.
 `--kernel stationary` runs the isotropic kernel instead.
 
@@ -47,7 +48,10 @@ def parse():
     ap.add_argument("--tensor-weight", type=float, default=400.0)
     ap.add_argument("--subsample", type=int, default=1)
     ap.add_argument("--measure", default="ncc", choices=["mse", "ncc", "lcc"])
-    ap.add_argument("--reg-rd", type=float, default=1e-3)
+    ap.add_argument("--reg-rd", type=float, default=0.0)
+    ap.add_argument("--reg-bilateral", type=float, default=1e-3)
+    ap.add_argument("--bilateral-spatial-sigma", type=float, default=12.0)
+    ap.add_argument("--bilateral-intensity-sigma", type=float, default=0.1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-registration", action="store_true", help="skip the multi-scale registration timing")
     ap.add_argument("--no-lcc", action="store_true", help="skip the LCC timing on the sliding-motion volumes")
@@ -62,8 +66,8 @@ def parse():
     ap.add_argument("--no-lockstep", action="store_true",
                     help="skip the lockstep (graph-replay) runs of the batch and warm-start legs")
     ap.add_argument("--registration-repeats", type=int, default=3)
-    ap.add_argument("--registration-rd", type=float, default=1e-5,
-                    help="RD weight of the registration (the bilateral-like regularizer)")
+    ap.add_argument("--registration-bilateral", type=float, default=1e-5,
+                    help="bilateral weight of the registrations (spatial / intensity sigma as above)")
     ap.add_argument("--no-pair-count", action="store_true", help="skip the exact k != 0 pair count")
     ap.add_argument("--cpu-crop", default=None, help="z,y,x voxels of the CPU baseline sample")
     ap.add_argument("--cpu-repeats", type=int, default=3, help="timed CPU baseline evaluations (median)")
@@ -298,7 +302,7 @@ def cpu_baseline(args, mov, fix, ten, mcov):
     GetValueAndDerivative (oracle/cpu_fp32.c, compiled here with -O3
     -march=native -fopenmp, OpenMP over every logical CPU this process may
     use), one evaluation of the same workload (whole volumes, same kernel,
-    sigma, control-point spacing and RD; --cpu-crop z,y,x takes a corner crop
+    sigma, control-point spacing and regularizers; --cpu-crop z,y,x takes a corner crop
     instead), median of --cpu-repeats after one untimed run; reported in the
     same unit.  Also the CPU time of one 3-level registration estimated as
     iterations x pairs per iteration / the measured pair rate."""
@@ -322,11 +326,12 @@ def cpu_baseline(args, mov, fix, ten, mcov):
     coeffs = orc.bspline_coefficients(m.astype(np.float64))
     pr = cpu_fp32.CpuProblem(f, coeffs, (cs, co, (args.cp_spacing,) * 3), cp, np.ones(n_cp, np.float32), args.sigma,
                              cov=tc, max_eig_cov=mcov)
-    pr.evaluate(args.reg_rd)
+    bl = bilateral_args(args, args.reg_bilateral) if args.reg_bilateral > 0 else None
+    pr.evaluate(args.reg_rd, bilateral=bl)
     times = []
     for _ in range(max(args.cpu_repeats, 1)):
         t0 = time.perf_counter()
-        pr.evaluate(args.reg_rd)
+        pr.evaluate(args.reg_rd, bilateral=bl)
         times.append(time.perf_counter() - t0)
     dt = float(np.median(times))
     v, d, _, _ = pair_counts(args, crop, None if tc is None else tc.reshape(crop + (3, 3)), mcov, None, False)
@@ -338,9 +343,20 @@ def cpu_baseline(args, mov, fix, ten, mcov):
     return {"value": rate / 1e9, "unit": "GVox*ctrl/s", "cores": threads, "kind": "port",
             "sample": f"FP32 CPU restatement (oracle/cpu_fp32.c, -O3 -march=native, OpenMP {threads} threads), "
                       f"GetValueAndDerivative on {crop[2]}x{crop[1]}x{crop[0]} voxels (same kernel, sigma, cp "
-                      f"spacing, RD), median of {len(times)}: {dt:.2f} s; host {host_cpu()}",
+                      f"spacing, regularizers), median of {len(times)}: {dt:.2f} s; host {host_cpu()}",
             "ms_per_evaluation": round(dt * 1e3, 1),
             "ms_per_registration_estimate": round(reg_pairs / rate * 1e3, 0)}
+
+
+def regularizer_desc(args):
+    """' + RD w' / ' + bilateral w (spatial sigma, intensity sigma)' of the workload."""
+    out = ""
+    if args.reg_rd > 0:
+        out += f" + RD {args.reg_rd:g}"
+    if args.reg_bilateral > 0:
+        out += (f" + bilateral {args.reg_bilateral:g} (spatial sigma {args.bilateral_spatial_sigma:g}, "
+                f"intensity sigma {args.bilateral_intensity_sigma:g})")
+    return out
 
 
 def cpu_threads():
@@ -389,12 +405,13 @@ def lcc_triples(shape, cs, co, h, support, sub, subn):
     return float(total.sum())
 
 
-def lcc_cpu_baseline(mov, fix, h, s, sub, subn, triples_full, crop=64):
+def lcc_cpu_baseline(mov, fix, h, s, sub, subn, triples_full, bilateral, crop=64):
     """The OpenMP build of the FP64 oracle (oracle/skm_oracle.c, the
     reference's LCC loops restated) timed on a crop^3 block at the centre of
     the sliding-motion volumes (the sliding interface runs through it), with
-    the same kernel, control-point spacing, subsampling and neighbourhood;
-    reference-window derivative.  Its triple rate scales to the full volume's
+    the same kernel, control-point spacing, subsampling, neighbourhood and
+    bilateral term (weight, spatial sigma, intensity sigma); reference-window
+    derivative.  Its triple rate scales to the full volume's
     triple count for a per-evaluation estimate."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
@@ -410,7 +427,8 @@ def lcc_cpu_baseline(mov, fix, h, s, sub, subn, triples_full, crop=64):
     pr = orc.OracleProblem(f, m, (cs, co, (h,) * 3), cp, sigma_c0=s, sigma_c4=s, subsample=sub,
                            subsample_neighborhood=subn, offsets=offsets, parallel=True)
     t0 = time.perf_counter()
-    pr.evaluate(2, True, 0.0, 1e-3, 0.0)
+    pr.evaluate(2, True)
+    pr.bilateral(*bilateral)
     dt = time.perf_counter() - t0
     tri = lcc_triples(f.shape, cs, co, h, s, sub, subn)
     rate = tri / dt
@@ -426,8 +444,9 @@ def lcc_cpu_baseline(mov, fix, h, s, sub, subn, triples_full, crop=64):
 def lcc_timing(args):
     """One LCC GetValueAndDerivative on the sliding-motion workload
     (256x256x128 two halves shifted 4 voxels plus breathing motion,
-    control-point spacing 5 -> 51x51x26, isotropic Wendland support 20, RD
-    1e-3, subsample 2, neighbourhood stride 6 -> 7^3 neighbours per centre):
+    control-point spacing 5 -> 51x51x26, isotropic Wendland support 20,
+    bilateral 1e-3 (spatial sigma 12, intensity sigma 0.1), subsample 2,
+    neighbourhood stride 6 -> 7^3 neighbours per centre):
     the reference's windowed derivative (GpuEvaluatorLCC.cu:206-530) and the
     exact gradient (each voxel gathers the terms of the centres that reach
     it, then one kernel-sum gather); median of 3 after one untimed
@@ -439,11 +458,12 @@ def lcc_timing(args):
     n = int(np.prod(cs))
     cp = np.random.default_rng(0).uniform(-1, 1, (n, 3)).astype(np.float32)
     out = {"workload": "LCC GetValueAndDerivative, 3D 256x256x128 sliding motion, cp 51x51x26 spacing 5, "
-                       "isotropic Wendland support 20, RD 0.001, subsample 2, neighbourhood stride 6"}
+                       "isotropic Wendland support 20, bilateral 0.001 (spatial sigma 12, intensity sigma 0.1), "
+                       "subsample 2, neighbourhood stride 6"}
     for name, exact in (("exact_gradient", True), ("reference_windows", False)):
         ev = GpuKernelEvaluator(n * 3, 2, 6, Kernel(s, s), Image(mov), Image(fix), Grid(cs, co, (h,) * 3),
                                 exact_gradient=exact)
-        ev.SetRegularizerRD(1e-3)
+        ev.SetRegularizerBilateral(1e-3, args.bilateral_spatial_sigma, args.bilateral_intensity_sigma)
         ev.getValueAndDerivative(MEASURE.LCC, cp)
         times = []
         for _ in range(3):
@@ -477,7 +497,8 @@ def lcc_timing(args):
             hbm[kname] = {"traffic": t, "ms": ms_k, "GB/s": round(t / (ms_k * 1e-3) / 1e9, 1),
                           "frac": round(t / (ms_k * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)}
     out["exact_gradient"]["hbm"] = hbm or None
-    out["cpu_baseline"] = None if args.no_cpu_baseline else lcc_cpu_baseline(mov, fix, h, s, sub, subn, tri)
+    out["cpu_baseline"] = None if args.no_cpu_baseline else lcc_cpu_baseline(
+        mov, fix, h, s, sub, subn, tri, bilateral_args(args, 1e-3))
     return out
 
 
@@ -487,7 +508,8 @@ def lcc_registration(args):
     plus breathing motion): support 80/40/20, control-point spacing
     support/4 (51x51x26 at the finest level), isotropic Wendland kernel,
     LCC with a 7^3 neighbourhood (strides 23/12/6), the reference's windowed
-    derivative, RD 1e-3, sampling rates 4/3/2, --lcc-iterations
+    derivative, bilateral 1e-3 (spatial sigma 12, intensity sigma 0.1),
+    sampling rates 4/3/2, --lcc-iterations
     adaptive-step iterations per level at step 1000.  One timed run, then one
     with the exact-gradient derivative."""
     from skmimageregistration_amd import Image, RegistrationConfig, register_multiscale, synthetic
@@ -496,7 +518,7 @@ def lcc_registration(args):
     cfg = RegistrationConfig(sigma_C4=[4 * s, 2 * s, s], sigma_C0=[4 * s, 2 * s, s], tensor_weights=[0.0],
                              num_scales=3, num_function_evaluations=[args.lcc_iterations],
                              initial_step_size=[1000.0], sampling_rate=[4, 3, 2], neighborhood_sampling=[23, 12, 6],
-                             smoothing_variance=[2, 1, 0], reg_rd=[1e-3], metric="lcc")
+                             smoothing_variance=[2, 1, 0], metric="lcc", **bilateral_config(args, 1e-3))
     t0 = time.perf_counter()
     df, reports = register_multiscale(Image(mov), Image(fix), cfg)
     ms = (time.perf_counter() - t0) * 1e3
@@ -509,7 +531,7 @@ def lcc_registration(args):
              "level_ms_optimize": [round(r["ms_optimize"], 1) for r in rep_x],
              "mean_abs_error_vox": round(float(np.abs(df_x[inner] - disp[inner]).mean()), 4)}
     return {"workload": "LCC registration, 3D 256x256x128 sliding motion, 3 levels (support 80/40/20, "
-                        f"cp spacing support/4), 7^3 neighbourhood, RD 0.001, {args.lcc_iterations} iterations "
+                        f"cp spacing support/4), 7^3 neighbourhood, bilateral 0.001, {args.lcc_iterations} iterations "
                         "per level, reference windowed derivative",
             "ms_per_registration": round(ms, 1),
             "cp_size": [list(r["cp_size"]) for r in reports],
@@ -523,7 +545,7 @@ def lcc_registration(args):
 
 def registration_config(args):
     """3 levels with support 4s, 2s, s, control-point spacing support/4,
-    tensors from the target with weights 400/200/100, NCC + RD, sampling
+    tensors from the target with weights 400/200/100, NCC + bilateral, sampling
     rates 5/4/3, 200 adaptive-step iterations per level."""
     from skmimageregistration_amd import RegistrationConfig
     s = args.sigma
@@ -532,7 +554,18 @@ def registration_config(args):
                               if args.kernel == "anisotropic" else [0.0],
                               num_scales=3, num_function_evaluations=[200], initial_step_size=[1000.0],
                               sampling_rate=[5, 4, 3], neighborhood_sampling=[1], smoothing_variance=[2, 1, 0],
-                              reg_rd=[args.registration_rd], metric="ncc")
+                              metric="ncc", **bilateral_config(args, args.registration_bilateral))
+
+
+def bilateral_args(args, weight):
+    """(weight, spatial sigma, intensity sigma) of the bilateral term."""
+    return (weight, args.bilateral_spatial_sigma, args.bilateral_intensity_sigma)
+
+
+def bilateral_config(args, weight):
+    """RegistrationConfig keys of the bilateral term."""
+    return {"reg_bilateral": [weight], "reg_bilateral_spatial_sigma": [args.bilateral_spatial_sigma],
+            "reg_bilateral_intensity_sigma": [args.bilateral_intensity_sigma]}
 
 
 def registration(args, mov, fix, disp):
@@ -558,7 +591,7 @@ def registration(args, mov, fix, disp):
             "level_ms": [{k: round(r[k], 2) for k in ("ms_setup", "ms_optimize", "ms_displacement")} for r in reports],
             "mean_abs_error_vox": round(float(np.abs(df[inner] - disp[inner]).mean()), 4),
             "mean_abs_displacement_vox": round(float(np.abs(disp[inner]).mean()), 4),
-            "repeats_ms": [round(t, 2) for t in times], "reg_rd": args.registration_rd}
+            "repeats_ms": [round(t, 2) for t in times], "reg_bilateral": args.registration_bilateral}
 
 
 def batch_timing(args):
@@ -578,7 +611,7 @@ def batch_timing(args):
     pairs = [(Image(frames[0]), Image(f)) for f in frames[1:]]
     register_batch(pairs[:1], cfg, concurrency=1)
     out = {"workload": f"{len(pairs)} registrations of a {len(frames)}-frame time series to frame 0, frames of "
-                       "96^3 / 128x128x64 / 160x160x80 voxels, NCC + RD, 3 levels x 200 iterations",
+                       "96^3 / 128x128x64 / 160x160x80 voxels, NCC + bilateral, 3 levels x 200 iterations",
            "frames": {f"{s[2]}x{s[1]}x{s[0]}": sum(f.shape == s for f in frames) for s in synthetic.SERIES_SIZES}}
     results = {}
     for conc in (1, args.batch_concurrency):
@@ -611,12 +644,15 @@ def batch_timing(args):
 def series_warm_start(args, pairs, plain, plain_single_ms):
     """The time series with the learned warm start: per frame the network's
     prediction (one network per frame size; its weights were trained on
-    128x128x96 pairs only) and its dense field, then the registration leg's
-    last level alone (support s, tensor weight / 4, sampling rate 3, RD) for
+    128x128x96 pairs only) and its dense field, both computed and handed to
+    the registration in device memory, then the registration leg's
+    last level alone (support s, tensor weight / 4, sampling rate 3, bilateral) for
     50 iterations from that field, guarded, --batch-concurrency frames in
     flight; also the same 50 iterations from zero.  Wall time per
     registration includes the predictions; final NCC against the plain
     3-level runs (`plain`, the same frames)."""
+    import torch
+
     from skmimageregistration_amd import RegistrationConfig, register_batch
     from skmimageregistration_amd.warmstart import WarmStart
     weights = os.path.join(REPO, "skmimageregistration_amd", "weights", "warmstart_128x128x96.bin")
@@ -626,16 +662,17 @@ def series_warm_start(args, pairs, plain, plain_single_ms):
     fine = RegistrationConfig(sigma_C4=[s], sigma_C0=[s], tensor_weights=[args.tensor_weight / 4]
                               if args.kernel == "anisotropic" else [0.0], num_scales=1,
                               num_function_evaluations=[50], initial_step_size=[1000.0], sampling_rate=[3],
-                              neighborhood_sampling=[1], smoothing_variance=[0], reg_rd=[args.registration_rd],
-                              metric="ncc")
+                              neighborhood_sampling=[1], smoothing_variance=[0], metric="ncc",
+                              **bilateral_config(args, args.registration_bilateral))
     nets = {}
     for _, t in pairs:
         if t.data.shape not in nets:
             nets[t.data.shape] = WarmStart(shape=t.data.shape, weights=weights)
-            nets[t.data.shape].predict(t.data, t.data)  # first-call set-up, untimed
+            nets[t.data.shape].predict_field_device(t.data, t.data)  # first-call set-up, untimed
     register_batch(pairs[:1], fine, concurrency=1)
     t0 = time.perf_counter()
-    fields = [nets[t.data.shape].field(nets[t.data.shape].predict(r.data, t.data)) for r, t in pairs]
+    ref = torch.from_numpy(pairs[0][0].data).cuda()  # frame 0, the reference of every pair
+    fields = [nets[t.data.shape].predict_field_device(ref, t.data) for _, t in pairs]
     warm = register_batch(pairs, fine, concurrency=args.batch_concurrency, initial_displacements=fields,
                           initial_guard=True)
     ms = (time.perf_counter() - t0) * 1e3 / len(pairs)
@@ -731,14 +768,14 @@ def warm_start_timing(args):
     # batched: predictions first (one network pass per pair), then the guarded fine-level runs in flight together
     pairs = [synthetic_pair(900 + i)[:2] for i in range(args.warm_pairs)]
     t0 = time.perf_counter()
-    fields = [ws.field(ws.predict(r, t)) for r, t in pairs]
+    fields = [ws.predict_field_device(r, t) for r, t in pairs]
     res = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), concurrency=args.batch_concurrency,
                          initial_displacements=fields, initial_guard=True)
     ms_batch = (time.perf_counter() - t0) * 1e3 / len(pairs)
     res_ls, ms_lockstep = None, None
     if not args.no_lockstep:
         t0 = time.perf_counter()
-        fields = [ws.field(ws.predict(r, t)) for r, t in pairs]
+        fields = [ws.predict_field_device(r, t) for r, t in pairs]
         res_ls = register_batch([(Image(r), Image(t)) for r, t in pairs], fine_config(50), lockstep=True,
                                 initial_displacements=fields, initial_guard=True)
         ms_lockstep = (time.perf_counter() - t0) * 1e3 / len(pairs)
@@ -787,6 +824,8 @@ def main():
                             Grid(cs, co, (args.cp_spacing,) * 3))
     if args.reg_rd > 0:
         ev.SetRegularizerRD(args.reg_rd)
+    if args.reg_bilateral > 0:
+        ev.SetRegularizerBilateral(*bilateral_args(args, args.reg_bilateral))
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.set_stream(stream)
@@ -861,7 +900,7 @@ def main():
         "vs_baseline": None, "dtype": "f32 (FP64 final sums)", "data": "synthetic",
         "config": {"workload": f"SKM GetValueAndDerivative, 3D {size[0]}x{size[1]}x{size[2]}, cp {cs[0]}x{cs[1]}x{cs[2]} "
                                f"spacing {args.cp_spacing:g}, {kdesc} support {args.sigma:g}, "
-                               f"{args.measure.upper()} + RD {args.reg_rd:g}, subsample {args.subsample}",
+                               f"{args.measure.upper()}{regularizer_desc(args)}, subsample {args.subsample}",
                    "pairs_value": vbox, "pairs_derivative": dbox,
                    "pairs_nonzero_value": vnz, "pairs_nonzero_derivative": dnz},
         "pass_ms": {k: round(v, 4) for k, v in per_kernel.items()},

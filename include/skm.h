@@ -567,6 +567,13 @@ typedef struct skm_registration_desc {
                                                           (as restart_displacement), or NULL;
                                                           with initial_guard dropped when it
                                                           evaluates worse than none  */
+    int32_t initial_displacement_on_device;            /* 1: initial_displacement is device
+                                                          memory of the current device, its
+                                                          writes complete before the call
+                                                          (the caller synchronises); copied on
+                                                          the device, no host round trip;
+                                                          any other pointer is an
+                                                          SKM_ERROR_INVALID_ARGUMENT */
 } skm_registration_desc;
 
 /* What one level did. */

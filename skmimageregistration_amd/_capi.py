@@ -136,7 +136,7 @@ class SkmRegistrationDesc(ctypes.Structure):
                 ("reference_landmarks", c_double_p), ("target_landmarks", c_double_p),
                 ("exact_gradient", ctypes.c_int32), ("initial_coefficients", c_float_p),
                 ("initial_guard", ctypes.c_int32), ("autotune", ctypes.c_int32),
-                ("initial_displacement", c_float_p)]
+                ("initial_displacement", c_float_p), ("initial_displacement_on_device", ctypes.c_int32)]
 
 
 class SkmScaleReport(ctypes.Structure):

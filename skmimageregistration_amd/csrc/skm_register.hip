@@ -567,14 +567,32 @@ skm_status register_impl(const skm_registration_desc *desc, float *displacement_
     if (desc->guidance_image) SKM_R_TRY(d_guide.upload(desc->guidance_image, (size_t)n_w, ss));
     bool has_prev = !prev.empty();
     const bool warm_field = !has_prev && desc->initial_displacement != nullptr;
-    if (warm_field) {
-        prev.assign(desc->initial_displacement, desc->initial_displacement + n_tar * dim);
-        has_prev = true;
-    }
-    if (has_prev)
-        SKM_R_TRY(d_prev.upload(prev.data(), prev.size(), ss));
-    else
+    if (warm_field && desc->initial_displacement_on_device) {
+        // device memory of this device, copied on the device
+        hipPointerAttribute_t attr{};
+        if (hipPointerGetAttributes(&attr, desc->initial_displacement) != hipSuccess ||
+            attr.type != hipMemoryTypeDevice || attr.device != setup.dev) {
+            (void)hipGetLastError();
+            return set_last_error(SKM_ERROR_INVALID_ARGUMENT,
+                                  "initial_displacement_on_device: not device memory of the current device");
+        }
         SKM_R_TRY(d_prev.alloc((size_t)n_tar * dim));
+        hipError_t e = hipMemcpyAsync(d_prev.p, desc->initial_displacement, sizeof(float) * (size_t)n_tar * dim,
+                                      hipMemcpyDeviceToDevice, ss);
+        if (e == hipSuccess) e = hipStreamSynchronize(ss);
+        if (e != hipSuccess)
+            return set_last_error(SKM_ERROR_HIP, std::string("initial displacement: ") + hipGetErrorString(e));
+        has_prev = true;
+    } else {
+        if (warm_field) {
+            prev.assign(desc->initial_displacement, desc->initial_displacement + n_tar * dim);
+            has_prev = true;
+        }
+        if (has_prev)
+            SKM_R_TRY(d_prev.upload(prev.data(), prev.size(), ss));
+        else
+            SKM_R_TRY(d_prev.alloc((size_t)n_tar * dim));
+    }
 
     std::vector<float> host_tmp;
     for (int scale = start; scale < desc->num_scales; ++scale) {

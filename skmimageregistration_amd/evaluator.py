@@ -683,7 +683,17 @@ def _attach_warm_start(d, keep, reference, target, config, initial_coefficients,
             raise ValueError(f"initial coefficients: expected {grid.count() * reference.dim}, got {c.size}")
         keep.append(c)
         d.initial_coefficients = fptr(c)
-    if initial_displacement is not None:
+    if initial_displacement is not None and getattr(initial_displacement, "is_cuda", False):
+        # a torch tensor in device memory: handed over without a host round trip
+        u = initial_displacement
+        if str(u.dtype) != "torch.float32" or not u.is_contiguous():
+            raise ValueError("initial displacement: a contiguous float32 tensor expected")
+        if u.numel() != target.data.size * target.dim:
+            raise ValueError("initial displacement: D floats per target voxel expected")
+        keep.append(u)
+        d.initial_displacement = ctypes.cast(ctypes.c_void_p(u.data_ptr()), _capi.c_float_p)
+        d.initial_displacement_on_device = 1
+    elif initial_displacement is not None:
         u = f32(initial_displacement).reshape(-1)
         if u.size != target.data.size * target.dim:
             raise ValueError("initial displacement: D floats per target voxel expected")
@@ -707,9 +717,10 @@ def register_multiscale(reference, target, config=None, weight_image=None, weigh
 
     Warm start (no SKMReg counterpart): initial_coefficients (SoA on the
     first level's control points) start that level's optimizer;
-    initial_displacement (the target grid's shape + (D,)) is the first level
-    run's previous displacement; with initial_guard either is used only when
-    it evaluates better than none."""
+    initial_displacement (the target grid's shape + (D,), a host array or a
+    float32 torch tensor in device memory, which is copied on the device) is
+    the first level run's previous displacement; with initial_guard either is
+    used only when it evaluates better than none."""
     lib = _capi.load()
     d, keep, out, reps = _registration_desc(lib, reference, target, config, weight_image, weight_tensor,
                                             guidance_image, restart_displacement, temp_directory,

@@ -179,6 +179,10 @@ class WarmStart:
     plain steps and kept only when it evaluates better than none."""
 
     COEF_SCALE = 10.0   # network outputs are the coefficients times this
+    # outputs -> coefficients: one FP32 product by this, on the host and on
+    # the device alike (a division by COEF_SCALE rounds differently from
+    # torch's, which multiplies by the reciprocal)
+    INV_COEF_SCALE = np.float32(1.0 / COEF_SCALE)
     SUPPORT = 24.0
     FIT_ITERATIONS, FIT_RIDGE = 60, 300.0
 
@@ -245,12 +249,31 @@ class WarmStart:
         """The network's [z][y][x][3] output -> SoA coefficients."""
         nx, ny, nz = self.cp.size()
         o = np.asarray(out, np.float32).reshape(nz, ny, nx, 3)
-        return np.ascontiguousarray(o.transpose(3, 0, 1, 2)).reshape(-1) / self.COEF_SCALE
+        return np.ascontiguousarray(o.transpose(3, 0, 1, 2)).reshape(-1) * self.INV_COEF_SCALE
 
     def predict(self, reference, target):
         x = self._torch.empty((1,) + self.in_shape, device="cuda", dtype=self._torch.float32)
         self.prepare(reference, target, x)
         return self.coefficients(self.net.forward(x).cpu().numpy())
+
+    def predict_field_device(self, reference, target):
+        """The predicted coefficients' dense field as a float32 torch tensor
+        in device memory (shape + (3,), complete on return), without a host
+        round trip: the network's [z][y][x][3] output is the coefficients in
+        the control points' AoS order times COEF_SCALE.  For
+        register_multiscale / register_batch's initial displacement."""
+        t = self._torch
+        x = t.empty((1,) + self.in_shape, device="cuda", dtype=t.float32)
+        self.prepare(reference, target, x)
+        cp = (self.net.forward(x).reshape(-1) * float(self.INV_COEF_SCALE)).contiguous()
+        cpw = t.ones(self.n_cp, device="cuda", dtype=t.float32)
+        u = t.empty(self.shape + (3,), device="cuda", dtype=t.float32)
+        t.cuda.current_stream().synchronize()  # the basis evaluator has a stream of its own
+        check(self.basis._lib.skm_evaluate_displacement_field_device(
+            self.basis.handle, ctypes.c_void_p(cp.data_ptr()), ctypes.c_void_p(cpw.data_ptr()),
+            ctypes.c_void_p(u.data_ptr())))
+        check(self.basis._lib.skm_synchronize(self.basis.handle))
+        return u
 
     def register(self, reference, target, iterations=50, bilateral=1e-5, guard=True, coefficients=None):
         """Warm-started registration: the predicted (or given) coefficients'
@@ -258,9 +281,9 @@ class WarmStart:
         (displacement, report dict of the level)."""
         from .evaluator import Image, register_multiscale
 
-        c = self.predict(reference, target) if coefficients is None else coefficients
+        u = self.predict_field_device(reference, target) if coefficients is None else self.field(coefficients)
         df, reps = register_multiscale(Image(reference), Image(target), fine_config(iterations, bilateral),
-                                       initial_displacement=self.field(c), initial_guard=guard)
+                                       initial_displacement=u, initial_guard=guard)
         return df, reps[0]
 
 

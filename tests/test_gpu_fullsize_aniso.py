@@ -1,11 +1,12 @@
 """The benchmark workload itself against the FP64 oracle (OpenMP build):
 3D 128x128x96, control-point spacing 6, anisotropic Wendland kernel of
 support 24 with the tensors of the target (ComputeAnisotropicTensor, weight
-400, alignment z), NCC + RD 1e-3, one GetValueAndDerivative at random
-control-point weights in [-1, 1].
+400, alignment z), one GetValueAndDerivative at random control-point
+weights in [-1, 1]: NCC + bilateral 1e-3 (spatial sigma 12, intensity
+sigma 0.1), the benchmark's own line, and MSE + RD 1e-3.
 
-Tolerances: metric and RD values 1e-5 relative, derivative 1e-4 of its
-largest entry (FP32 pairs against FP64).
+Tolerances: metric and regularizer values 1e-5 relative, derivative 1e-4 of
+its largest entry (FP32 pairs against FP64).
 """
 import numpy as np
 import pytest
@@ -19,6 +20,7 @@ pytestmark = pytest.mark.gpu
 
 SHAPE = (96, 128, 128)
 H, SUPPORT, WEIGHT, RD = 6.0, 24.0, 400.0, 1e-3
+BILATERAL = (1e-3, 12.0, 0.1)
 
 
 def _geometry(shape, h):
@@ -28,8 +30,8 @@ def _geometry(shape, h):
     return size, cs, co
 
 
-@pytest.mark.parametrize("measure", [MEASURE.NCC, MEASURE.MSE])
-def test_bench_workload_matches_the_oracle(measure):
+@pytest.mark.parametrize("measure,reg", [(MEASURE.NCC, "bilateral"), (MEASURE.MSE, "rd")])
+def test_bench_workload_matches_the_oracle(measure, reg):
     mov, fix, _ = synthetic.ellipsoids_3d(SHAPE, n=20, smooth=1.5, max_disp=6.0, noise=0.02)
     size, cs, co = _geometry(SHAPE, H)
     n = int(np.prod(cs))
@@ -39,19 +41,30 @@ def test_bench_workload_matches_the_oracle(measure):
                     wTensor=(Grid(size, (0.0,) * 3, (1.0,) * 3), ten))
     cp = np.random.default_rng(1).uniform(-1, 1, size=(n, 3)).astype(np.float32)
     ev = GpuKernelEvaluator(n * 3, 1, 1, kernel, Image(mov), Image(fix), Grid(cs, co, (H,) * 3))
-    ev.SetRegularizerRD(RD)
-    print(f"GPU evaluation, measure {int(measure)}", flush=True)
+    rd = RD if reg == "rd" else 0.0
+    if reg == "rd":
+        ev.SetRegularizerRD(RD)
+    else:
+        ev.SetRegularizerBilateral(*BILATERAL)
+    print(f"GPU evaluation, measure {int(measure)}, {reg}", flush=True)
     res, der = ev.getValueAndDerivative(measure, cp)
     ev.close()
     print("oracle evaluation (OpenMP)", flush=True)
     op = orc.OracleProblem(fix, mov, (cs, co, (H,) * 3), cp.astype(np.float64), sigma_c0=SUPPORT, sigma_c4=SUPPORT,
                            kernel_mode=2, max_eig_cov=mcov, weight=(size, (0.0,) * 3, (1.0,) * 3, ten),
                            parallel=True)
-    want, want_der = op.evaluate(int(measure), True, 0.0, RD, 0.0)
-    print(f"measure {res.measure!r} vs {want['measure']!r}; rd {res.reg_value_rd!r} vs {want['reg_rd']!r}",
-          flush=True)
+    want, want_der = op.evaluate(int(measure), True, 0.0, rd, 0.0)
+    print(f"measure {res.measure!r} vs {want['measure']!r}", flush=True)
     assert scalar_rel(res.measure, want["measure"]) < 1e-5
-    assert scalar_rel(res.reg_value_rd, want["reg_rd"]) < 1e-5
+    if reg == "rd":
+        print(f"rd {res.reg_value_rd!r} vs {want['reg_rd']!r}", flush=True)
+        assert scalar_rel(res.reg_value_rd, want["reg_rd"]) < 1e-5
+    else:
+        bval, bder = op.bilateral(*BILATERAL)
+        print(f"bilateral {res.reg_value_bilateral!r} vs {bval!r}", flush=True)
+        assert bval > 0
+        assert scalar_rel(res.reg_value_bilateral, bval) < 1e-5
+        want_der = want_der + bder
     assert res.numPixelsCountedGpu == int(want["pixels_counted"])
     err = rel_err(der, want_der)
     print(f"derivative max error / max entry {err:.3g}", flush=True)
