@@ -78,7 +78,7 @@ class SkmPassTimes(ctypes.Structure):
 class SkmTuning(ctypes.Structure):
     _fields_ = [("lcc_ref_chunks", ctypes.c_int32), ("gather_split", ctypes.c_int32),
                 ("value_split", ctypes.c_int32), ("gather_regions", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("lcc_ref_walk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class SkmAutotuneReport(ctypes.Structure):

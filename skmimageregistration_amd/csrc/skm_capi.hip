@@ -355,7 +355,8 @@ struct skm_evaluator {
     skm_tuning tuning = {};                 // launch shapes as set by skm_set_tuning
     double *d_lcc_bound = nullptr;          // exact LCC: per-workgroup bounds of the centre pass
     unsigned long long *d_gamma_fx = nullptr;  // exact LCC: fixed-point scatter sums, 3 per voxel
-    unsigned *d_lcc_cpk = nullptr;          // exact LCC gather: packed offsets + "contributes" per centre
+    unsigned *d_lcc_cpk = nullptr;          // LCC voxel gathers: packed offsets + "contributes" per centre
+    double *d_lcc_reft_partials = nullptr;  // reference-mode LCC derivative by voxel tables: block sums
     float *d_params = nullptr, *d_prev_g = nullptr, *d_best = nullptr;
     OptState *d_opt = nullptr;
     skm_iteration_record *d_records = nullptr;
@@ -386,7 +387,8 @@ struct skm_evaluator {
                         d_lcc_window, d_dense_window, d_offsets, d_sg, d_sv, d_sb, d_partials, d_reg_partials,
                         d_hreg_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng,
-                        d_lcc_ref_partials, d_lcc_bound, d_gamma_fx, d_cp_intensity, d_lcc_cpk};
+                        d_lcc_ref_partials, d_lcc_bound, d_gamma_fx, d_cp_intensity, d_lcc_cpk,
+                        d_lcc_reft_partials};
         for (void *p : ptrs)
             if (p) device_release(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
@@ -738,13 +740,18 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
         // modes) needs the centres' bounds for its fixed-point step
         const bool gamma_gather = exact_deriv && lcc_gamma_gather_applies(pr);
         const bool need_bound = exact_deriv && !gamma_gather;
+        // reference mode: the voxel-table form of the windowed derivative
+        const bool ref_table = deriv && !pr.exact && lcc_ref_table_applies(pr);
+        const bool need_cpk = gamma_gather || ref_table;
         if (need_bound && !ev->d_lcc_bound)
             SKM_TRY(dev_alloc(&ev->d_lcc_bound, (size_t)((pr.n_sub + kThreads - 1) / kThreads)));
-        if (gamma_gather && !ev->d_lcc_cpk) SKM_TRY(dev_alloc(&ev->d_lcc_cpk, (size_t)pr.n_sub));
+        if (need_cpk && !ev->d_lcc_cpk) SKM_TRY(dev_alloc(&ev->d_lcc_cpk, (size_t)pr.n_sub));
+        if (ref_table && !ev->d_lcc_reft_partials)
+            SKM_TRY(dev_alloc(&ev->d_lcc_reft_partials, (size_t)lcc_ref_table_partials(pr)));
         {
             PassTimer t(ev, 7);
             launch_lcc_centres(pr, ev->d_dense, ev->d_ccoef, ev->d_partials, &blocks, s,
-                               need_bound ? ev->d_lcc_bound : nullptr, gamma_gather ? ev->d_lcc_cpk : nullptr);
+                               need_bound ? ev->d_lcc_bound : nullptr, need_cpk ? ev->d_lcc_cpk : nullptr);
         }
         {
             PassTimer t(ev, 2);
@@ -790,7 +797,11 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
                                        &b2, s);
                     dense = ev->d_dense_q;
                 }
-                launch_lcc_gather_reference(lp, dense, ev->d_ccoef, ev->d_sc, deriv, ev->d_lcc_ref_partials, s);
+                if (ref_table)
+                    launch_lcc_ref_table(lp, dense, ev->d_ccoef, ev->d_lcc_cpk, ev->d_sc, deriv,
+                                         ev->d_lcc_reft_partials, s);
+                else
+                    launch_lcc_gather_reference(lp, dense, ev->d_ccoef, ev->d_sc, deriv, ev->d_lcc_ref_partials, s);
             }
         }
     } else {
@@ -1232,6 +1243,7 @@ skm_status skm::create_evaluator(const skm_desc *desc, bool device_data, skm_eva
         pr.gather_split = t.gather_split;
         pr.value_split = t.value_split;
         pr.gather_regions = t.gather_regions;
+        pr.lcc_ref_walk = t.lcc_ref_walk;
     }
     *out = ev;
     return SKM_SUCCESS;
@@ -1478,6 +1490,8 @@ skm_status validate_tuning(const skm_tuning &t)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "value_split must be 0 (automatic), 1, 4, 8 or 16");
     if (t.gather_regions != 0 && t.gather_regions != 1 && t.gather_regions != 8)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "gather_regions must be 0 (automatic), 1 or 8");
+    if (t.lcc_ref_walk != 0 && t.lcc_ref_walk != 1)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "lcc_ref_walk must be 0 (automatic) or 1");
     return SKM_SUCCESS;
 }
 
@@ -1493,6 +1507,7 @@ SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning)
     ev->pr.gather_split = tuning->gather_split;
     ev->pr.value_split = tuning->value_split;
     ev->pr.gather_regions = tuning->gather_regions;
+    ev->pr.lcc_ref_walk = tuning->lcc_ref_walk;
     const long long np = lcc_reference_partials(ev->pr);
     if (ev->d_lcc_ref_partials && np > ev->lcc_ref_partials_n) {
         device_release(ev->d_lcc_ref_partials);
@@ -1521,6 +1536,7 @@ SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float 
     std::vector<Axis> axes;
     const bool d3 = ev->pr.dim == 3;
     if (measure == SKM_MEASURE_LCC && !ev->pr.exact) {
+        if (lcc_ref_table_applies(ev->pr)) axes.push_back({&skm_tuning::lcc_ref_walk, {0, 1}});
         axes.push_back({&skm_tuning::lcc_ref_chunks, {0, 1, 2, 4, 8, 16, 32, 64}});
     } else {
         if (measure != SKM_MEASURE_LCC) axes.push_back({&skm_tuning::value_split, {0, 1, 4, 8, 16}});
@@ -1592,10 +1608,11 @@ SKM_API skm_status skm_tuning_table_save(const char *path)
     std::ofstream f(path);
     if (!f) return fail(SKM_ERROR_IO, std::string("cannot write ") + path);
     std::lock_guard<std::mutex> lock(g_tuning_mutex);
-    f << "# skm launch shapes: shape class, lcc_ref_chunks, gather_split, value_split, gather_regions\n";
+    f << "# skm launch shapes: shape class, lcc_ref_chunks, gather_split, value_split, gather_regions, "
+         "lcc_ref_walk\n";
     for (const auto &kv : g_tuning)
         f << kv.first << " " << kv.second.lcc_ref_chunks << " " << kv.second.gather_split << " "
-          << kv.second.value_split << " " << kv.second.gather_regions << "\n";
+          << kv.second.value_split << " " << kv.second.gather_regions << " " << kv.second.lcc_ref_walk << "\n";
     return f ? SKM_SUCCESS : fail(SKM_ERROR_IO, std::string("cannot write ") + path);
 }
 
@@ -1611,7 +1628,8 @@ SKM_API skm_status skm_tuning_table_load(const char *path)
         std::istringstream in(line);
         std::string key;
         skm_tuning t{};
-        if (!(in >> key >> t.lcc_ref_chunks >> t.gather_split >> t.value_split >> t.gather_regions))
+        if (!(in >> key >> t.lcc_ref_chunks >> t.gather_split >> t.value_split >> t.gather_regions >>
+              t.lcc_ref_walk))
             return fail(SKM_ERROR_IO, "malformed tuning line: " + line);
         if (validate_tuning(t) != SKM_SUCCESS) return fail(SKM_ERROR_IO, "launch shape out of range: " + line);
         read[key] = t;

@@ -56,6 +56,7 @@ struct Problem {
     int gather_split;          // derivative gather: sample planes per slab, 4 or 8 (0: automatic, 8)
     int value_split;           // value pass: waves per sample brick, 1/4/8/16 (0: automatic)
     int gather_regions;        // 3D derivative gather: 8 XCD regions of cp tiles, or 1 (0: automatic, 1)
+    int lcc_ref_walk;          // reference-window LCC derivative: 1 forces the centre walk (0: automatic)
     float lam_cp_max;          // largest lam over the control points
     float lam_w_max;           // largest lam over the points a sample can look up (weight image, outside)
 };
@@ -177,6 +178,15 @@ void launch_lcc_scatter(const Problem &pr, const float4 *dense_g, const float4 *
 bool lcc_gamma_gather_applies(const Problem &pr);
 void launch_lcc_gamma(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const unsigned *cpk,
                       const Scalars *sc, float4 *gamma, hipStream_t s);
+// Reference-window derivative regrouped per neighbour voxel (stationary
+// kernels, 3D, a control-point box of at most 8 per axis around each voxel;
+// not with lcc_ref_walk): the same sum as launch_lcc_gather_reference, with
+// the kernel pair evaluated once per (control point, voxel).  cpk as from
+// launch_lcc_centres; partials: lcc_ref_table_partials(pr) doubles.
+bool lcc_ref_table_applies(const Problem &pr);
+long long lcc_ref_table_partials(const Problem &pr);
+void launch_lcc_ref_table(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const unsigned *cpk,
+                          const Scalars *sc, float *deriv_soa, double *partials, hipStream_t s);
 
 // optimizer (skm_kernels.hip)
 struct OptState {

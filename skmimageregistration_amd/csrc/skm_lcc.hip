@@ -711,6 +711,439 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4
     gamma[v] = make_float4((float)(gam * g4.x), (float)(gam * g4.y), DIM == 3 ? (float)(gam * g4.z) : 0.0f, 0.0f);
 }
 
+// ------------------------------------------------------------ reference windows by voxel tables
+
+// The reference-window derivative (k_lcc_gather_ref's sum) regrouped per
+// neighbour voxel y, for stationary kernels in 3D:
+//   d_j = sum_y ka(p_j, y) gradM(y) (F(y) SA_j(y) + M(y) SB_j(y) + SC_j(y)),
+//   SA_j = sum A kw^2, SB_j = sum B kw^2, SC_j = sum C kw
+// over the centres whose neighbour lattice holds y (found as k_lcc_gamma
+// finds them) and whose cell lies in j's window.  Along an axis, the centres
+// reaching y occupy a few sample cells (their "ranks"), and j's window, an
+// interval of cells, keeps a contiguous run of those ranks.  So one 3D prefix
+// sum of the centres' terms over (x, y, z) ranks per voxel gives every
+// control point's restricted sum as an 8-corner difference: the kernel pair
+// is evaluated once per (control point, voxel) instead of once per (control
+// point, centre, neighbour).
+//
+// One wave per block of voxels: the voxels whose position lies in one cell
+// [p_k, p_k+1) of the control-point grid.  Every control point with
+// |p_j - y| < support for such a voxel lies in the box k - A .. k - A + NB - 1
+// per axis (NB = 2 ceil(support / h), A = NB / 2 - 1), whose points have one
+// FP32 sum slot each in LDS for the whole block.  Per voxel: the reach masks
+// and ranks; the list of active slots (points within the support whose
+// windows keep a centre on every axis, about half the box); the table in
+// chunks of at most kRtCh ranks per axis (prefix along x in registers in
+// FP64, then y and z in LDS in FP32, each line's loads issued together); and
+// for the active slots, 64 at a time, an 8-corner difference of the table.
+// Each block writes its slots, k_lcc_reft_finish adds the blocks in order in
+// FP64.
+constexpr int kRtCh = 7;              // ranks per axis of one table chunk (the sliding-motion
+                                      // workload's voxels away from the faces reach 7 cells)
+constexpr int kRtT = kRtCh + 1;       // table extent per axis; index 0 is the zero border
+constexpr int kRtBox = 8;             // control points per axis of a block's box, at most
+
+struct RefTableGeom {
+    int kmin[3], nk[3];  // control-point cells of the blocks, per axis
+    int A[3], NB[3];     // the box of block k: control points k - A .. k - A + NB - 1
+};
+
+// control-point cell of voxel index i along axis d: floor((y - o_cp) / h_cp)
+__device__ __forceinline__ int rt_cell(const Problem &pr, int d, int i)
+{
+    const float y = pr.fixed.o[d] + pr.fixed.h[d] * (float)i;
+    return (int)floorf((y - pr.cp.o[d]) * pr.cp.ih[d]);
+}
+
+// bits [0, t) of a 64-bit mask (t clamped to 0..64)
+__device__ __forceinline__ unsigned long long rt_below(int t)
+{
+    return t <= 0 ? 0ull : (t >= 64 ? ~0ull : ((1ull << t) - 1ull));
+}
+
+// Reach of voxel index yi along axis d: the centre cells (relative to cbase,
+// one bit each) whose lattice holds yi, and in LDS per rank the cell and the
+// jitter offsets allowed in it.  The wave computes it together.
+__device__ __forceinline__ void rt_axis(const Problem &pr, int d, int yi, int lane, int (*cells)[64],
+                                        unsigned (*allowed)[64], unsigned long long &cm, int &cbase)
+{
+    const double r = (double)pr.kc.smin / (double)pr.fixed.h[d];
+    const int base = yi - (int)floor(r) - 1, lim = yi - (int)floor(1.0 - r) + 1;
+    const int n = pr.fixed.n[d], sn = pr.subn, sub = pr.sub;
+    // lane i: position base + i, tested with neighbour_box's arithmetic
+    const int p = base + lane;
+    bool in = false;
+    if (p >= 0 && p < n && p <= lim) {
+        const int nlo = clampi((int)((double)p - r + 1.0), 0, n - 1);
+        const int nhi = clampi((int)((double)p + r), 0, n - 1);
+        in = yi >= nlo && yi <= nhi && (yi - nlo) % sn == 0;
+    }
+    const unsigned long long pm = __ballot(in);
+    const int p0 = max(base, 0);
+    cbase = sub > 1 ? (int)(((float)p0 + 0.5f) / (float)sub) : p0;
+    // lane l: cell cbase + l, the offsets o with sub (cbase + l) + o in the mask
+    const int c = cbase + lane;
+    unsigned al = 0;
+    if (c < pr.nsub[d]) {
+        const int sh = sub * c - base;
+        for (int o = 0; o < sub; ++o) {
+            const int i = sh + o;
+            if (i >= 0 && i < 64 && ((pm >> i) & 1ull)) al |= 1u << o;
+        }
+    }
+    cm = __ballot(al != 0u);
+    if (al) {
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((unsigned)cm, 0u));
+        cells[d][rank] = c;
+        allowed[d][rank] = al;
+    }
+}
+
+// table index of ranks (z, y, x), each 0..kRtCh with 0 the zero border; rows
+// and planes padded so that lanes reading different rows fall on different
+// LDS banks
+constexpr int kRtRow = kRtT + 1, kRtPlane = kRtT * kRtRow + 3, kRtSize = kRtT * kRtPlane;
+__device__ __forceinline__ int rt_at(int z, int y, int x) { return z * kRtPlane + y * kRtRow + x; }
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_lcc_ref_table(Problem pr, const float4 *__restrict__ dense,
+                                                      const float4 *__restrict__ coef,
+                                                      const unsigned *__restrict__ cpk, RefTableGeom g,
+                                                      double *__restrict__ partials)
+{
+    __shared__ float tA[kRtSize], tB[kRtSize], tC[kRtSize];
+    __shared__ float accs[kRtBox * 64 * 3];  // per box control point (slot bz * 64 + lane) and axis
+    __shared__ short list[kRtBox * 64];      // the current voxel's active slots
+    __shared__ int cells[3][64];
+    __shared__ unsigned allowed[3][64];
+    __shared__ int wlo[3][kRtBox], whi[3][kRtBox];  // box control points' windows (sample cells)
+    __shared__ int rlo[3][kRtBox], rhi[3][kRtBox];  // their rank ranges at the current voxel
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    const int k[3] = {g.kmin[0] + b % g.nk[0], g.kmin[1] + (b / g.nk[0]) % g.nk[1],
+                      g.kmin[2] + b / (g.nk[0] * g.nk[1])};
+    for (int i = lane; i < kRtSize; i += 64) tA[i] = tB[i] = tC[i] = 0.0f;  // borders stay zero
+    for (int i = lane; i < kRtBox * 64 * 3; i += 64) accs[i] = 0.0f;
+    if (lane < 3 * kRtBox) {
+        // windows are separable for stationary kernels: axis d's from a
+        // control point on that axis
+        const int d = lane / kRtBox, bj = lane % kRtBox;
+        const int j = k[d] - g.A[d] + bj;
+        int lo = 1, hi = 0;
+        if (bj < g.NB[d] && j >= 0 && j < pr.cp.n[d]) {
+            const long long jj = d == 0 ? j : (d == 1 ? (long long)j * pr.cp.n[0]
+                                                      : (long long)j * pr.cp.n[0] * pr.cp.n[1]);
+            lo = pr.cp_window[jj * 6 + d];
+            hi = pr.cp_window[jj * 6 + 3 + d];
+        }
+        wlo[d][bj] = lo;
+        whi[d][bj] = hi;
+    }
+    // this lane's box column (bx, by); slot bz * 64 + lane
+    const int bx = lane % g.NB[0], by = lane / g.NB[0];
+    const bool lane_ok = lane < g.NB[0] * g.NB[1];
+    const int jx = k[0] - g.A[0] + bx, jy = k[1] - g.A[1] + by;
+    const bool jxy_ok = lane_ok && jx >= 0 && jx < pr.cp.n[0] && jy >= 0 && jy < pr.cp.n[1];
+    const float px = pr.cp.o[0] + pr.cp.h[0] * (float)jx, py = pr.cp.o[1] + pr.cp.h[1] * (float)jy;
+    const float s2 = pr.kc.smin * pr.kc.smin * 1.0001f;
+    // voxel range of the block per axis (one voxel of margin; rt_cell decides)
+    int vlo[3], vhi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float a = (pr.cp.o[d] + pr.cp.h[d] * (float)k[d] - pr.fixed.o[d]) * pr.fixed.ih[d];
+        const float e = (pr.cp.o[d] + pr.cp.h[d] * (float)(k[d] + 1) - pr.fixed.o[d]) * pr.fixed.ih[d];
+        vlo[d] = max(0, (int)floorf(a) - 1);
+        vhi[d] = min(pr.fixed.n[d] - 1, (int)ceilf(e) + 1);
+    }
+    __syncthreads();
+    const int sub = pr.sub;
+    unsigned long long cm[3] = {0, 0, 0};
+    int cbase[3] = {0, 0, 0};
+    for (int iz = vlo[2]; iz <= vhi[2]; ++iz) {
+        if (rt_cell(pr, 2, iz) != k[2]) continue;
+        __syncthreads();  // the previous voxel's readers of cells / allowed
+        rt_axis(pr, 2, iz, lane, cells, allowed, cm[2], cbase[2]);
+        for (int iy = vlo[1]; iy <= vhi[1]; ++iy) {
+            if (rt_cell(pr, 1, iy) != k[1]) continue;
+            __syncthreads();
+            rt_axis(pr, 1, iy, lane, cells, allowed, cm[1], cbase[1]);
+            // the row's dense values and fixed intensities, one voxel ahead
+            const long long row = (long long)pr.fixed.n[0] * (iy + (long long)pr.fixed.n[1] * iz);
+            float4 g4n = dense[row + vlo[0]];
+            float fn = pr.fixed_data[row + vlo[0]];
+            for (int ix = vlo[0]; ix <= vhi[0]; ++ix) {
+                const long long v = row + ix;
+                const float4 g4 = g4n;
+                const float F = fn;
+                if (ix < vhi[0]) {
+                    g4n = dense[v + 1];
+                    fn = pr.fixed_data[v + 1];
+                }
+                if (rt_cell(pr, 0, ix) != k[0]) continue;
+                if (isnan(g4.w)) continue;
+                __syncthreads();
+                rt_axis(pr, 0, ix, lane, cells, allowed, cm[0], cbase[0]);
+                const int R[3] = {(int)__popcll(cm[0]), (int)__popcll(cm[1]), (int)__popcll(cm[2])};
+                if (R[0] == 0 || R[1] == 0 || R[2] == 0) continue;
+                // rank ranges of the box control points' windows
+                if (lane < 3 * kRtBox) {
+                    const int d = lane / kRtBox, bj = lane % kRtBox;
+                    const unsigned long long m = d == 0 ? cm[0] : (d == 1 ? cm[1] : cm[2]);
+                    const int cb = d == 0 ? cbase[0] : (d == 1 ? cbase[1] : cbase[2]);
+                    int a = 1, e = 0;
+                    if (wlo[d][bj] <= whi[d][bj]) {
+                        a = (int)__popcll(m & rt_below(wlo[d][bj] - cb));
+                        e = (int)__popcll(m & rt_below(whi[d][bj] + 1 - cb)) - 1;
+                    }
+                    rlo[d][bj] = a;
+                    rhi[d][bj] = e;
+                }
+                const float fyv[3] = {(float)ix, (float)iy, (float)iz};
+                float y[3];
+                to_global<3>(pr.fixed, fyv, y);
+                const float M = g4.w;
+                __syncthreads();
+                // the active slots: control points within the support whose
+                // windows keep some centre on every axis, listed in slot order
+                int nact = 0;
+                {
+                    const bool xy_ok = jxy_ok && rlo[0][bx] <= rhi[0][bx] && rlo[1][by] <= rhi[1][by];
+                    const float dxy = (px - y[0]) * (px - y[0]) + (py - y[1]) * (py - y[1]);
+                    for (int bz = 0; bz < g.NB[2]; ++bz) {
+                        const int jz = k[2] - g.A[2] + bz;
+                        const float dz = pr.cp.o[2] + pr.cp.h[2] * (float)jz - y[2];
+                        const bool act = xy_ok && jz >= 0 && jz < pr.cp.n[2] && rlo[2][bz] <= rhi[2][bz] &&
+                                         dxy + dz * dz < s2;
+                        const unsigned long long bal = __ballot(act);
+                        if (act) {
+                            const int r = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                            list[nact + r] = (short)(bz * 64 + lane);
+                        }
+                        nact += (int)__popcll(bal);
+                    }
+                }
+                if (nact == 0) continue;
+                for (int c0z = 0; c0z < R[2]; c0z += kRtCh)
+                    for (int c0y = 0; c0y < R[1]; c0y += kRtCh)
+                        for (int c0x = 0; c0x < R[0]; c0x += kRtCh) {
+                            const int nx = min(kRtCh, R[0] - c0x), ny = min(kRtCh, R[1] - c0y);
+                            const int nz = min(kRtCh, R[2] - c0z);
+                            // the chunk's terms, prefix along x: lane = (y rank, z rank); the
+                            // loads of a lane's row are issued together (indices clamped into
+                            // the row, the extra entries unused)
+                            {
+                                const int ly = lane % kRtCh, lz = lane / kRtCh;
+                                if (lz < kRtCh && ly < ny && lz < nz) {
+                                    const int cy = cells[1][c0y + ly], cz = cells[2][c0z + lz];
+                                    const unsigned ay = allowed[1][c0y + ly], az = allowed[2][c0z + lz];
+                                    const unsigned rowc = (unsigned)pr.nsub[0] * (unsigned)(cy + pr.nsub[1] * cz);
+                                    unsigned cxs[kRtCh], w[kRtCh];
+                                    float4 abc[kRtCh];
+#pragma unroll
+                                    for (int lx = 0; lx < kRtCh; ++lx) cxs[lx] = (unsigned)cells[0][c0x + min(lx, nx - 1)];
+#pragma unroll
+                                    for (int lx = 0; lx < kRtCh; ++lx) w[lx] = cpk[rowc + cxs[lx]];
+#pragma unroll
+                                    for (int lx = 0; lx < kRtCh; ++lx) abc[lx] = coef[rowc + cxs[lx]];
+                                    double sa = 0.0, sb = 0.0, sc = 0.0;
+                                    const int tb = rt_at(lz + 1, ly + 1, 1);
+#pragma unroll
+                                    for (int lx = 0; lx < kRtCh; ++lx) {
+                                        const unsigned ax = allowed[0][c0x + min(lx, nx - 1)];
+                                        const unsigned ox = w[lx] & 0xffu, oy = (w[lx] >> 8) & 0xffu;
+                                        const unsigned oz = (w[lx] >> 16) & 0xffu;
+                                        if ((w[lx] & 0x80000000u) && ((ax >> ox) & 1u) && ((ay >> oy) & 1u) &&
+                                            ((az >> oz) & 1u)) {
+                                            const float fp[3] = {(float)((int)cxs[lx] * sub + (sub > 1 ? (int)ox : 0)),
+                                                                 (float)(cy * sub + (sub > 1 ? (int)oy : 0)),
+                                                                 (float)(cz * sub + (sub > 1 ? (int)oz : 0))};
+                                            float x[3];
+                                            to_global<3>(pr.fixed, fp, x);
+                                            float r2 = 0.0f;
+#pragma unroll
+                                            for (int d = 0; d < 3; ++d) r2 += (x[d] - y[d]) * (x[d] - y[d]);
+                                            const float kw = neighborhood_weight(r2, pr.kc.smin);
+                                            if (kw > 0.0f) {
+                                                const double kw2 = (double)kw * (double)kw;
+                                                sa += (double)abc[lx].x * kw2;
+                                                sb += (double)abc[lx].y * kw2;
+                                                sc += (double)abc[lx].z * (double)kw;
+                                            }
+                                        }
+                                        if (lx < nx) {
+                                            tA[tb + lx] = (float)sa;
+                                            tB[tb + lx] = (float)sb;
+                                            tC[tb + lx] = (float)sc;
+                                        }
+                                    }
+                                }
+                            }
+                            __syncthreads();
+                            // prefix along y: lane = (x rank, z rank), a line's loads together
+                            {
+                                const int lx = lane % kRtCh, lz = lane / kRtCh;
+                                if (lz < kRtCh && lx < nx && lz < nz) {
+                                    float va[kRtCh], vb[kRtCh], vc[kRtCh];
+#pragma unroll
+                                    for (int ly = 0; ly < kRtCh; ++ly) {
+                                        const int t = rt_at(lz + 1, ly + 1, lx + 1);
+                                        va[ly] = tA[t]; vb[ly] = tB[t]; vc[ly] = tC[t];
+                                    }
+#pragma unroll
+                                    for (int ly = 1; ly < kRtCh; ++ly) {
+                                        va[ly] += va[ly - 1]; vb[ly] += vb[ly - 1]; vc[ly] += vc[ly - 1];
+                                    }
+#pragma unroll
+                                    for (int ly = 1; ly < kRtCh; ++ly)
+                                        if (ly < ny) {
+                                            const int t = rt_at(lz + 1, ly + 1, lx + 1);
+                                            tA[t] = va[ly]; tB[t] = vb[ly]; tC[t] = vc[ly];
+                                        }
+                                }
+                            }
+                            __syncthreads();
+                            // prefix along z: lane = (x rank, y rank)
+                            {
+                                const int lx = lane % kRtCh, ly = lane / kRtCh;
+                                if (ly < kRtCh && lx < nx && ly < ny) {
+                                    float va[kRtCh], vb[kRtCh], vc[kRtCh];
+#pragma unroll
+                                    for (int lz = 0; lz < kRtCh; ++lz) {
+                                        const int t = rt_at(lz + 1, ly + 1, lx + 1);
+                                        va[lz] = tA[t]; vb[lz] = tB[t]; vc[lz] = tC[t];
+                                    }
+#pragma unroll
+                                    for (int lz = 1; lz < kRtCh; ++lz) {
+                                        va[lz] += va[lz - 1]; vb[lz] += vb[lz - 1]; vc[lz] += vc[lz - 1];
+                                    }
+#pragma unroll
+                                    for (int lz = 1; lz < kRtCh; ++lz)
+                                        if (lz < nz) {
+                                            const int t = rt_at(lz + 1, ly + 1, lx + 1);
+                                            tA[t] = va[lz]; tB[t] = vb[lz]; tC[t] = vc[lz];
+                                        }
+                                }
+                            }
+                            __syncthreads();
+                            // the active slots, 64 at a time: 8-corner sums of their boxes
+                            for (int e0 = 0; e0 < nact; e0 += 64) {
+                                if (e0 + lane >= nact) continue;
+                                const int slot = list[e0 + lane];
+                                const int sbz = slot >> 6, sl = slot & 63;
+                                const int sbx = sl % g.NB[0], sby = sl / g.NB[0];
+                                const int xa = max(rlo[0][sbx], c0x) - c0x, xb = min(rhi[0][sbx], c0x + nx - 1) - c0x + 1;
+                                const int ya = max(rlo[1][sby], c0y) - c0y, yb = min(rhi[1][sby], c0y + ny - 1) - c0y + 1;
+                                const int za = max(rlo[2][sbz], c0z) - c0z, zb = min(rhi[2][sbz], c0z + nz - 1) - c0z + 1;
+                                if (xa >= xb || ya >= yb || za >= zb) continue;
+                                const float qx = pr.cp.o[0] + pr.cp.h[0] * (float)(k[0] - g.A[0] + sbx);
+                                const float qy = pr.cp.o[1] + pr.cp.h[1] * (float)(k[1] - g.A[1] + sby);
+                                const float qz = pr.cp.o[2] + pr.cp.h[2] * (float)(k[2] - g.A[2] + sbz);
+                                const float vv[3] = {qx - y[0], qy - y[1], qz - y[2]};
+                                PState sty;
+                                sty.support = pr.kc.smin;
+                                const float ka = kernel_pair<3, 0>(pr.kc, vv, sty, sty);
+                                if (ka == 0.0f) continue;
+                                const int c11 = rt_at(zb, yb, 0), c10 = rt_at(zb, ya, 0);
+                                const int c01 = rt_at(za, yb, 0), c00 = rt_at(za, ya, 0);
+                                const float SA = ((tA[c11 + xb] - tA[c11 + xa]) - (tA[c10 + xb] - tA[c10 + xa])) -
+                                                 ((tA[c01 + xb] - tA[c01 + xa]) - (tA[c00 + xb] - tA[c00 + xa]));
+                                const float SB = ((tB[c11 + xb] - tB[c11 + xa]) - (tB[c10 + xb] - tB[c10 + xa])) -
+                                                 ((tB[c01 + xb] - tB[c01 + xa]) - (tB[c00 + xb] - tB[c00 + xa]));
+                                const float SC = ((tC[c11 + xb] - tC[c11 + xa]) - (tC[c10 + xb] - tC[c10 + xa])) -
+                                                 ((tC[c01 + xb] - tC[c01 + xa]) - (tC[c00 + xb] - tC[c00 + xa]));
+                                const float wgt = ka * fmaf(F, SA, fmaf(M, SB, SC));
+                                float *a = accs + slot * 3;
+                                a[0] = fmaf(wgt, g4.x, a[0]);
+                                a[1] = fmaf(wgt, g4.y, a[1]);
+                                a[2] = fmaf(wgt, g4.z, a[2]);
+                            }
+                            __syncthreads();  // the table's readers, before the next chunk or voxel
+                        }
+            }
+        }
+    }
+    // this block's sums: [block][slot][axis]
+    for (int i = lane; i < kRtBox * 64 * 3; i += 64) partials[(long long)b * (kRtBox * 64 * 3) + i] = (double)accs[i];
+}
+
+// deriv_j = coef_scale * the sum over the blocks whose box holds j, in block order.
+__global__ __launch_bounds__(kThreads) void k_lcc_reft_finish(Problem pr, RefTableGeom g,
+                                                              const double *__restrict__ partials,
+                                                              const Scalars *__restrict__ sc,
+                                                              float *__restrict__ deriv)
+{
+    const long long i = blockIdx.x * (long long)kThreads + threadIdx.x;
+    if (i >= pr.n_cp * 3) return;
+    const long long j = i % pr.n_cp;
+    const int d = (int)(i / pr.n_cp);
+    const int jc[3] = {(int)(j % pr.cp.n[0]), (int)((j / pr.cp.n[0]) % pr.cp.n[1]),
+                       (int)(j / ((long long)pr.cp.n[0] * pr.cp.n[1]))};
+    int klo[3], khi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        klo[a] = max(jc[a] + g.A[a] - g.NB[a] + 1, g.kmin[a]);
+        khi[a] = min(jc[a] + g.A[a], g.kmin[a] + g.nk[a] - 1);
+    }
+    double sum = 0.0;
+    for (int kz = klo[2]; kz <= khi[2]; ++kz)
+        for (int ky = klo[1]; ky <= khi[1]; ++ky)
+            for (int kx = klo[0]; kx <= khi[0]; ++kx) {
+                const long long b = (kx - g.kmin[0]) + (long long)g.nk[0] * ((ky - g.kmin[1]) +
+                                                                             (long long)g.nk[1] * (kz - g.kmin[2]));
+                const int bx = jc[0] - kx + g.A[0], by = jc[1] - ky + g.A[1], bz = jc[2] - kz + g.A[2];
+                sum += partials[((b * kRtBox + bz) * 64 + bx + g.NB[0] * by) * 3 + d];
+            }
+    deriv[i] = (float)(sum * sc->coef_scale);
+}
+
+// Geometry of the voxel-table launch; false when it does not apply.
+static bool ref_table_geom(const Problem &pr, RefTableGeom &g)
+{
+    if (pr.dim != 3 || !is_stationary(pr.kc.mode) || pr.sub < 1 || pr.sub > 32 || pr.subn < 1) return false;
+    for (int d = 0; d < 3; ++d) {
+        const double r = (double)pr.kc.smin / (double)pr.fixed.h[d];
+        // positions of a voxel's reach: 64 at most (the masks), cells as many
+        if ((std::floor(r) - std::floor(1.0 - r)) + 3.0 > 64.0) return false;
+        const double sh = (double)pr.kc.smin / (double)pr.cp.h[d];
+        const int nb = 2 * (int)std::ceil(sh);
+        if (nb < 2 || nb > kRtBox) return false;
+        g.NB[d] = nb;
+        g.A[d] = nb / 2 - 1;
+        // blocks: the cells of the first and last voxel, one more each side
+        const double y0 = (double)pr.fixed.o[d], y1 = y0 + (double)pr.fixed.h[d] * (pr.fixed.n[d] - 1);
+        g.kmin[d] = (int)std::floor((y0 - (double)pr.cp.o[d]) / (double)pr.cp.h[d]) - 1;
+        const int kmax = (int)std::floor((y1 - (double)pr.cp.o[d]) / (double)pr.cp.h[d]) + 1;
+        g.nk[d] = kmax - g.kmin[d] + 1;
+    }
+    if (g.NB[0] * g.NB[1] > 64) return false;
+    return (long long)g.nk[0] * g.nk[1] * g.nk[2] < (1ll << 31);
+}
+
+bool lcc_ref_table_applies(const Problem &pr)
+{
+    RefTableGeom g;
+    return pr.lcc_ref_walk == 0 && ref_table_geom(pr, g);
+}
+
+long long lcc_ref_table_partials(const Problem &pr)
+{
+    RefTableGeom g;
+    if (!ref_table_geom(pr, g)) return 0;
+    return (long long)g.nk[0] * g.nk[1] * g.nk[2] * kRtBox * 64 * 3;
+}
+
+void launch_lcc_ref_table(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const unsigned *cpk,
+                          const Scalars *sc, float *deriv_soa, double *partials, hipStream_t s)
+{
+    RefTableGeom g;
+    if (!ref_table_geom(pr, g)) return;
+    const long long blocks = (long long)g.nk[0] * g.nk[1] * g.nk[2];
+    k_lcc_ref_table<<<(unsigned)blocks, 64, 0, s>>>(pr, dense_g, centre_coef, cpk, g, partials);
+    const unsigned fb = (unsigned)((pr.n_cp * 3 + kThreads - 1) / kThreads);
+    k_lcc_reft_finish<<<fb, kThreads, 0, s>>>(pr, g, partials, sc, deriv_soa);
+}
+
 // ------------------------------------------------------------ launchers
 
 template <int DIM>

@@ -286,9 +286,10 @@ class GpuKernelEvaluator:
     def synchronize(self):
         check(self._lib.skm_synchronize(self._h))
 
-    def set_tuning(self, lcc_ref_chunks=0, gather_split=0, value_split=0, gather_regions=0):
+    def set_tuning(self, lcc_ref_chunks=0, gather_split=0, value_split=0, gather_regions=0, lcc_ref_walk=0):
         """Launch shapes (skm_set_tuning); 0 keeps the library's choice."""
-        t = _capi.SkmTuning(int(lcc_ref_chunks), int(gather_split), int(value_split), int(gather_regions))
+        t = _capi.SkmTuning(int(lcc_ref_chunks), int(gather_split), int(value_split), int(gather_regions),
+                            int(lcc_ref_walk))
         check(self._lib.skm_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune_device(self, metric, d_cp_aos, d_cpw, repeats=3):
@@ -298,14 +299,14 @@ class GpuKernelEvaluator:
                                      int(repeats), ctypes.byref(r)))
         b = r.best
         return {"lcc_ref_chunks": b.lcc_ref_chunks, "gather_split": b.gather_split, "value_split": b.value_split,
-                "gather_regions": b.gather_regions,
+                "gather_regions": b.gather_regions, "lcc_ref_walk": b.lcc_ref_walk,
                 "best_ms": r.best_ms, "first_ms": r.first_ms, "candidates": r.candidates}
 
     def get_tuning(self):
         t = _capi.SkmTuning()
         check(self._lib.skm_get_tuning(self._h, ctypes.byref(t)))
         return dict(lcc_ref_chunks=t.lcc_ref_chunks, gather_split=t.gather_split, value_split=t.value_split,
-                    gather_regions=t.gather_regions)
+                    gather_regions=t.gather_regions, lcc_ref_walk=t.lcc_ref_walk)
 
     def cube_offsets(self):
         n = ctypes.c_int64()

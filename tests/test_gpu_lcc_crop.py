@@ -1,10 +1,12 @@
 """LCC at the sliding-motion workload's own parameters (support 20, control
 points every 5 voxels, subsample 2, neighbourhood stride 6, so 7^3
 neighbours per centre) on volumes small enough for the OpenMP FP64 oracle:
-both derivative modes against the oracle, the reference-window derivative at
-forced workgroup splits (1, 2 and 64 workgroups per control-point tile, the
-split that the full-size workload's 51x51x26 grid and the coarse levels'
-grids select), and the exact-mode derivative's bits on a repeated call.
+both derivative modes against the oracle; the reference-window derivative
+in both of its forms, regrouped per neighbour voxel (the automatic choice,
+the same bits on a repeated call) and as the centre walk at forced workgroup
+splits (1, 2 and 64 workgroups per control-point tile, the split that the
+full-size workload's 51x51x26 grid and the coarse levels' grids select); and
+the exact-mode derivative's bits on a repeated call.
 
 Tolerances: value 1e-5 relative, derivative 1e-4 of its largest entry.
 """
@@ -58,9 +60,15 @@ def test_lcc_reference_windows_cfg_parameters_forced_splits():
     ores, oder = _oracle(case)
     ev = case.gpu()
     ev.getValue(MEASURE.LCC, case.cp, case.cpw)  # first draw of the offsets
+    # the automatic path: the sum regrouped per neighbour voxel, twice (same bits)
+    r1, d1 = _check(ev, case, ores, oder, "voxel tables")
+    r2, d2 = _check(ev, case, ores, oder, "voxel tables, repeated")
+    assert r1.measure == r2.measure and np.array_equal(d1, d2), "voxel-table derivative differs between runs"
+    # the centre walk at forced workgroup splits
     for chunks in (0, 1, 2, 64):
-        ev.set_tuning(lcc_ref_chunks=chunks)
-        _check(ev, case, ores, oder, f"chunks={chunks or 'auto'}")
+        ev.set_tuning(lcc_ref_chunks=chunks, lcc_ref_walk=1)
+        _, dw = _check(ev, case, ores, oder, f"centre walk, chunks={chunks or 'auto'}")
+        assert rel_err(d1, dw) < 1e-5
     ev.close()
 
 
