@@ -222,13 +222,15 @@ SKM_API skm_status skm_fit_coefficients_device(skm_evaluator *ev, const float *d
  *                   grid order (1 or 8; auto 1: at 128x128x96 the regions
  *                   measured 1.20 ms against 1.05-1.10 ms per gather)
  *   lcc_ref_walk    reference-window LCC derivative: 1 walks the centres of
- *                   each control point's window (the reference's order);
- *                   auto (0) regroups the same sum per neighbour voxel where
- *                   that applies (stationary kernels in 3D, at most 8 control
+ *                   each control point's window (the reference's order); 2
+ *                   regroups the same sum per neighbour voxel where that
+ *                   applies (stationary kernels in 3D, at most 8 control
  *                   points per axis within the support of a voxel), which
  *                   evaluates the kernel pair once per (control point,
  *                   voxel) instead of once per (control point, centre,
- *                   neighbour); results agree to FP rounding
+ *                   neighbour); auto (0) takes the form with the smaller
+ *                   expected time (the walk's triples against the voxels);
+ *                   results agree to FP rounding
  * skm_get_tuning returns the fields as set (0 where automatic). */
 typedef struct skm_tuning {
     int32_t lcc_ref_chunks;

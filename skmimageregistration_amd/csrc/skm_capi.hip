@@ -798,8 +798,7 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
                     dense = ev->d_dense_q;
                 }
                 if (ref_table)
-                    launch_lcc_ref_table(lp, dense, ev->d_ccoef, ev->d_lcc_cpk, ev->d_sc, deriv,
-                                         ev->d_lcc_reft_partials, s);
+                    launch_lcc_ref_table(lp, dense, ev->d_ccoef, ev->d_sc, deriv, ev->d_lcc_reft_partials, s);
                 else
                     launch_lcc_gather_reference(lp, dense, ev->d_ccoef, ev->d_sc, deriv, ev->d_lcc_ref_partials, s);
             }
@@ -1490,8 +1489,8 @@ skm_status validate_tuning(const skm_tuning &t)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "value_split must be 0 (automatic), 1, 4, 8 or 16");
     if (t.gather_regions != 0 && t.gather_regions != 1 && t.gather_regions != 8)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "gather_regions must be 0 (automatic), 1 or 8");
-    if (t.lcc_ref_walk != 0 && t.lcc_ref_walk != 1)
-        return fail(SKM_ERROR_INVALID_ARGUMENT, "lcc_ref_walk must be 0 (automatic) or 1");
+    if (t.lcc_ref_walk < 0 || t.lcc_ref_walk > 2)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "lcc_ref_walk must be 0 (automatic), 1 or 2");
     return SKM_SUCCESS;
 }
 
@@ -1536,7 +1535,9 @@ SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float 
     std::vector<Axis> axes;
     const bool d3 = ev->pr.dim == 3;
     if (measure == SKM_MEASURE_LCC && !ev->pr.exact) {
-        if (lcc_ref_table_applies(ev->pr)) axes.push_back({&skm_tuning::lcc_ref_walk, {0, 1}});
+        Problem tp = ev->pr;
+        tp.lcc_ref_walk = 2;
+        if (lcc_ref_table_applies(tp)) axes.push_back({&skm_tuning::lcc_ref_walk, {0, 1, 2}});
         axes.push_back({&skm_tuning::lcc_ref_chunks, {0, 1, 2, 4, 8, 16, 32, 64}});
     } else {
         if (measure != SKM_MEASURE_LCC) axes.push_back({&skm_tuning::value_split, {0, 1, 4, 8, 16}});

@@ -56,7 +56,7 @@ struct Problem {
     int gather_split;          // derivative gather: sample planes per slab, 4 or 8 (0: automatic, 8)
     int value_split;           // value pass: waves per sample brick, 1/4/8/16 (0: automatic)
     int gather_regions;        // 3D derivative gather: 8 XCD regions of cp tiles, or 1 (0: automatic, 1)
-    int lcc_ref_walk;          // reference-window LCC derivative: 1 forces the centre walk (0: automatic)
+    int lcc_ref_walk;          // reference-window LCC derivative: 1 the centre walk, 2 the voxel tables (0: automatic)
     float lam_cp_max;          // largest lam over the control points
     float lam_w_max;           // largest lam over the points a sample can look up (weight image, outside)
 };
@@ -158,7 +158,8 @@ void launch_dense_warp(const Problem &pr, int quirk3d, float4 *dense_g, hipStrea
 // bound_blk (optional, one double per workgroup): the largest bound per
 // workgroup of |beta kw gradM| over a centre's neighbours, before the 1/N scale;
 // cpk (optional, one word per centre): its jitter offsets, one byte per axis,
-// and bit 31 set when its derivative coefficients are not all zero
+// and bit 31 set when its derivative coefficients are not all zero; the word
+// is also stored in the centre's centre_coef .w (0 without cpk)
 void launch_lcc_centres(const Problem &pr, const float4 *dense_g, float4 *centre_coef, double *partials,
                         int *blocks_out, hipStream_t s, double *bound_blk = nullptr, unsigned *cpk = nullptr);
 // partials: lcc_reference_partials(pr) doubles (none needed when it is 0)
@@ -180,13 +181,15 @@ void launch_lcc_gamma(const Problem &pr, const float4 *dense_g, const float4 *ce
                       const Scalars *sc, float4 *gamma, hipStream_t s);
 // Reference-window derivative regrouped per neighbour voxel (stationary
 // kernels, 3D, a control-point box of at most 8 per axis around each voxel;
-// not with lcc_ref_walk): the same sum as launch_lcc_gather_reference, with
-// the kernel pair evaluated once per (control point, voxel).  cpk as from
-// launch_lcc_centres; partials: lcc_ref_table_partials(pr) doubles.
+// chosen by lcc_ref_walk or, automatically, where its expected time is the
+// smaller): the same sum as launch_lcc_gather_reference, with
+// the kernel pair evaluated once per (control point, voxel).  centre_coef as
+// from launch_lcc_centres given a cpk buffer (the packed word then rides in
+// each centre's .w); partials: lcc_ref_table_partials(pr) doubles.
 bool lcc_ref_table_applies(const Problem &pr);
 long long lcc_ref_table_partials(const Problem &pr);
-void launch_lcc_ref_table(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const unsigned *cpk,
-                          const Scalars *sc, float *deriv_soa, double *partials, hipStream_t s);
+void launch_lcc_ref_table(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
+                          float *deriv_soa, double *partials, hipStream_t s);
 
 // optimizer (skm_kernels.hip)
 struct OptState {

@@ -10,6 +10,8 @@
 // The derivative then only needs k(p_j, y) per (cp, centre, neighbour).
 #include "skm_internal.hpp"
 
+#include <vector>
+
 #include "skm.h"
 
 namespace skm {
@@ -251,14 +253,16 @@ __global__ __launch_bounds__(kThreads) void k_lcc_centres(Problem pr, const floa
                 bound = ((double)fabsf(A) * fk + (double)fabsf(B) * mk + (double)fabsf(C)) * gk * 1.001;
             }
         }
-        coef[s] = make_float4(A, B, C, 0.0f);
+        unsigned w = 0u;
         if (cpk) {
-            unsigned w = (A != 0.0f || B != 0.0f || C != 0.0f) ? 0x80000000u : 0u;
+            w = (A != 0.0f || B != 0.0f || C != 0.0f) ? 0x80000000u : 0u;
             if (pr.sub > 1)
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) w |= (unsigned)pr.offsets[s * DIM + d] << (8 * d);
             cpk[s] = w;
         }
+        // .w carries the packed word too, so that one load gives both
+        coef[s] = make_float4(A, B, C, __uint_as_float(w));
         acc[0] = cc;
         long long cv = voxel_index<DIM>(pr, pos);
         acc[6] = isnan(dense[cv].w) ? 0.0 : 1.0;
@@ -733,11 +737,12 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4
 // FP32 sum slot each in LDS for the whole block.  Per voxel: the reach masks
 // and ranks; the list of active slots (points within the support whose
 // windows keep a centre on every axis, about half the box); the table in
-// chunks of at most kRtCh ranks per axis (prefix along x in registers in
-// FP64, then y and z in LDS in FP32, each line's loads issued together); and
+// chunks of at most kRtCh ranks per axis (prefix along x in registers, then
+// y and z in LDS, each line's loads issued together; FP32); and
 // for the active slots, 64 at a time, an 8-corner difference of the table.
 // Each block writes its slots, k_lcc_reft_finish adds the blocks in order in
-// FP64.
+// FP64.  A control-point cell wider than ~5 voxels is split into pieces along
+// each axis, one wave each, so that coarse grids still fill the GPU.
 constexpr int kRtCh = 7;              // ranks per axis of one table chunk (the sliding-motion
                                       // workload's voxels away from the faces reach 7 cells)
 constexpr int kRtT = kRtCh + 1;       // table extent per axis; index 0 is the zero border
@@ -746,6 +751,7 @@ constexpr int kRtBox = 8;             // control points per axis of a block's bo
 struct RefTableGeom {
     int kmin[3], nk[3];  // control-point cells of the blocks, per axis
     int A[3], NB[3];     // the box of block k: control points k - A .. k - A + NB - 1
+    int np[3];           // voxel pieces per cell and axis (one wave each)
 };
 
 // control-point cell of voxel index i along axis d: floor((y - o_cp) / h_cp)
@@ -761,42 +767,66 @@ __device__ __forceinline__ unsigned long long rt_below(int t)
     return t <= 0 ? 0ull : (t >= 64 ? ~0ull : ((1ull << t) - 1ull));
 }
 
-// Reach of voxel index yi along axis d: the centre cells (relative to cbase,
-// one bit each) whose lattice holds yi, and in LDS per rank the cell and the
-// jitter offsets allowed in it.  The wave computes it together.
-__device__ __forceinline__ void rt_axis(const Problem &pr, int d, int yi, int lane, int (*cells)[64],
-                                        unsigned (*allowed)[64], unsigned long long &cm, int &cbase)
+// Reach of a voxel index yi along axis d, for every index of every axis:
+// the centre cells (relative to cbase, one bit each) whose neighbour lattice
+// holds yi, and per rank the cell and the jitter offsets allowed in it.  It
+// depends on the geometry only; one wave per (axis, index).
+struct RefReach {
+    unsigned long long *cm;  // per row (off[d] + yi): the cell mask
+    int *cbase;              // its first cell
+    int *cell;               // 64 per row: the cell of each rank
+    unsigned *allowed;       // 64 per row: its allowed offsets (bit o: offset o)
+    int off[3];
+};
+
+__global__ __launch_bounds__(64) void k_lcc_reach(Problem pr, RefReach rr)
 {
+    const int lane = threadIdx.x;
+    const int row = (int)blockIdx.x;
+    const int d = row < rr.off[1] ? 0 : (row < rr.off[2] ? 1 : 2);
+    const int yi = row - rr.off[d];
     const double r = (double)pr.kc.smin / (double)pr.fixed.h[d];
+    // centre positions that can reach yi: [base, lim] (k_lcc_gamma's range)
     const int base = yi - (int)floor(r) - 1, lim = yi - (int)floor(1.0 - r) + 1;
     const int n = pr.fixed.n[d], sn = pr.subn, sub = pr.sub;
-    // lane i: position base + i, tested with neighbour_box's arithmetic
-    const int p = base + lane;
-    bool in = false;
-    if (p >= 0 && p < n && p <= lim) {
-        const int nlo = clampi((int)((double)p - r + 1.0), 0, n - 1);
-        const int nhi = clampi((int)((double)p + r), 0, n - 1);
-        in = yi >= nlo && yi <= nhi && (yi - nlo) % sn == 0;
-    }
-    const unsigned long long pm = __ballot(in);
     const int p0 = max(base, 0);
-    cbase = sub > 1 ? (int)(((float)p0 + 0.5f) / (float)sub) : p0;
-    // lane l: cell cbase + l, the offsets o with sub (cbase + l) + o in the mask
+    const int cbase = sub > 1 ? (int)(((float)p0 + 0.5f) / (float)sub) : p0;
+    // lane l: cell cbase + l; offset o is allowed when position sub (cbase + l)
+    // + o holds yi on its lattice (neighbour_box's arithmetic)
     const int c = cbase + lane;
     unsigned al = 0;
     if (c < pr.nsub[d]) {
-        const int sh = sub * c - base;
         for (int o = 0; o < sub; ++o) {
-            const int i = sh + o;
-            if (i >= 0 && i < 64 && ((pm >> i) & 1ull)) al |= 1u << o;
+            const int p = sub * c + o;
+            if (p < p0 || p > lim || p >= n) continue;
+            const int nlo = clampi((int)((double)p - r + 1.0), 0, n - 1);
+            const int nhi = clampi((int)((double)p + r), 0, n - 1);
+            if (yi >= nlo && yi <= nhi && (yi - nlo) % sn == 0) al |= 1u << o;
         }
     }
-    cm = __ballot(al != 0u);
+    const unsigned long long cm = __ballot(al != 0u);
     if (al) {
         const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((unsigned)cm, 0u));
-        cells[d][rank] = c;
-        allowed[d][rank] = al;
+        rr.cell[row * 64 + rank] = c;
+        rr.allowed[row * 64 + rank] = al;
+    }
+    if (lane == 0) {
+        rr.cm[row] = cm;
+        rr.cbase[row] = cbase;
+    }
+}
+
+// A voxel index's reach into LDS (cells, allowed) and registers (cm, cbase).
+__device__ __forceinline__ void rt_axis(const RefReach &rr, int d, int yi, int lane, int (*cells)[64],
+                                        unsigned (*allowed)[64], unsigned long long &cm, int &cbase)
+{
+    const int row = rr.off[d] + yi;
+    cm = rr.cm[row];
+    cbase = rr.cbase[row];
+    if (lane < (int)__popcll(cm)) {
+        cells[d][lane] = rr.cell[row * 64 + lane];
+        allowed[d][lane] = rr.allowed[row * 64 + lane];
     }
 }
 
@@ -807,21 +837,26 @@ constexpr int kRtRow = kRtT + 1, kRtPlane = kRtT * kRtRow + 3, kRtSize = kRtT * 
 __device__ __forceinline__ int rt_at(int z, int y, int x) { return z * kRtPlane + y * kRtRow + x; }
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_lcc_ref_table(Problem pr, const float4 *__restrict__ dense,
-                                                      const float4 *__restrict__ coef,
-                                                      const unsigned *__restrict__ cpk, RefTableGeom g,
-                                                      double *__restrict__ partials)
+                                                      const float4 *__restrict__ coef, RefReach rr,
+                                                      RefTableGeom g, float *__restrict__ partials)
 {
     __shared__ float tA[kRtSize], tB[kRtSize], tC[kRtSize];
     __shared__ float accs[kRtBox * 64 * 3];  // per box control point (slot bz * 64 + lane) and axis
-    __shared__ short list[kRtBox * 64];      // the current voxel's active slots
+    // the current voxel's active slots: slot (9 bits), bx, by, bz (3 bits
+    // each), the z rank range z0, z1 (6 bits each)
+    __shared__ unsigned list[kRtBox * 64];
     __shared__ int cells[3][64];
     __shared__ unsigned allowed[3][64];
     __shared__ int wlo[3][kRtBox], whi[3][kRtBox];  // box control points' windows (sample cells)
     __shared__ int rlo[3][kRtBox], rhi[3][kRtBox];  // their rank ranges at the current voxel
     const int lane = threadIdx.x;
     const int b = blockIdx.x;
-    const int k[3] = {g.kmin[0] + b % g.nk[0], g.kmin[1] + (b / g.nk[0]) % g.nk[1],
-                      g.kmin[2] + b / (g.nk[0] * g.nk[1])};
+    // block b: piece (q0, q1, q2) of control-point cell (k0, k1, k2), pieces fastest
+    const int npc = g.np[0] * g.np[1] * g.np[2];
+    const int cell = b / npc, piece = b % npc;
+    const int k[3] = {g.kmin[0] + cell % g.nk[0], g.kmin[1] + (cell / g.nk[0]) % g.nk[1],
+                      g.kmin[2] + cell / (g.nk[0] * g.nk[1])};
+    const int q[3] = {piece % g.np[0], (piece / g.np[0]) % g.np[1], piece / (g.np[0] * g.np[1])};
     for (int i = lane; i < kRtSize; i += 64) tA[i] = tB[i] = tC[i] = 0.0f;  // borders stay zero
     for (int i = lane; i < kRtBox * 64 * 3; i += 64) accs[i] = 0.0f;
     if (lane < 3 * kRtBox) {
@@ -846,14 +881,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     const bool jxy_ok = lane_ok && jx >= 0 && jx < pr.cp.n[0] && jy >= 0 && jy < pr.cp.n[1];
     const float px = pr.cp.o[0] + pr.cp.h[0] * (float)jx, py = pr.cp.o[1] + pr.cp.h[1] * (float)jy;
     const float s2 = pr.kc.smin * pr.kc.smin * 1.0001f;
+    const float inv_s2 = 1.0f / (pr.kc.smin * pr.kc.smin);
     // voxel range of the block per axis (one voxel of margin; rt_cell decides)
     int vlo[3], vhi[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const float a = (pr.cp.o[d] + pr.cp.h[d] * (float)k[d] - pr.fixed.o[d]) * pr.fixed.ih[d];
         const float e = (pr.cp.o[d] + pr.cp.h[d] * (float)(k[d] + 1) - pr.fixed.o[d]) * pr.fixed.ih[d];
-        vlo[d] = max(0, (int)floorf(a) - 1);
-        vhi[d] = min(pr.fixed.n[d] - 1, (int)ceilf(e) + 1);
+        const int lo = max(0, (int)floorf(a) - 1), hi = min(pr.fixed.n[d] - 1, (int)ceilf(e) + 1);
+        // this block's piece of it
+        const int len = hi - lo + 1;
+        vlo[d] = lo + (len * q[d]) / g.np[d];
+        vhi[d] = lo + (len * (q[d] + 1)) / g.np[d] - 1;
     }
     __syncthreads();
     const int sub = pr.sub;
@@ -862,11 +901,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     for (int iz = vlo[2]; iz <= vhi[2]; ++iz) {
         if (rt_cell(pr, 2, iz) != k[2]) continue;
         __syncthreads();  // the previous voxel's readers of cells / allowed
-        rt_axis(pr, 2, iz, lane, cells, allowed, cm[2], cbase[2]);
+        rt_axis(rr, 2, iz, lane, cells, allowed, cm[2], cbase[2]);
         for (int iy = vlo[1]; iy <= vhi[1]; ++iy) {
             if (rt_cell(pr, 1, iy) != k[1]) continue;
             __syncthreads();
-            rt_axis(pr, 1, iy, lane, cells, allowed, cm[1], cbase[1]);
+            rt_axis(rr, 1, iy, lane, cells, allowed, cm[1], cbase[1]);
             // the row's dense values and fixed intensities, one voxel ahead
             const long long row = (long long)pr.fixed.n[0] * (iy + (long long)pr.fixed.n[1] * iz);
             float4 g4n = dense[row + vlo[0]];
@@ -882,7 +921,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                 if (rt_cell(pr, 0, ix) != k[0]) continue;
                 if (isnan(g4.w)) continue;
                 __syncthreads();
-                rt_axis(pr, 0, ix, lane, cells, allowed, cm[0], cbase[0]);
+                rt_axis(rr, 0, ix, lane, cells, allowed, cm[0], cbase[0]);
                 const int R[3] = {(int)__popcll(cm[0]), (int)__popcll(cm[1]), (int)__popcll(cm[2])};
                 if (R[0] == 0 || R[1] == 0 || R[2] == 0) continue;
                 // rank ranges of the box control points' windows
@@ -907,18 +946,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                 // windows keep some centre on every axis, listed in slot order
                 int nact = 0;
                 {
-                    const bool xy_ok = jxy_ok && rlo[0][bx] <= rhi[0][bx] && rlo[1][by] <= rhi[1][by];
+                    const int x0 = lane_ok ? rlo[0][bx] : 1, x1 = lane_ok ? rhi[0][bx] : 0;
+                    const int y0 = lane_ok ? rlo[1][by] : 1, y1 = lane_ok ? rhi[1][by] : 0;
+                    const bool xy_ok = jxy_ok && x0 <= x1 && y0 <= y1;
+                    const unsigned lw = (unsigned)lane | ((unsigned)bx << 9) | ((unsigned)by << 12);
                     const float dxy = (px - y[0]) * (px - y[0]) + (py - y[1]) * (py - y[1]);
                     for (int bz = 0; bz < g.NB[2]; ++bz) {
                         const int jz = k[2] - g.A[2] + bz;
                         const float dz = pr.cp.o[2] + pr.cp.h[2] * (float)jz - y[2];
-                        const bool act = xy_ok && jz >= 0 && jz < pr.cp.n[2] && rlo[2][bz] <= rhi[2][bz] &&
-                                         dxy + dz * dz < s2;
+                        const int z0 = rlo[2][bz], z1 = rhi[2][bz];
+                        const bool act = xy_ok && jz >= 0 && jz < pr.cp.n[2] && z0 <= z1 && dxy + dz * dz < s2;
                         const unsigned long long bal = __ballot(act);
                         if (act) {
                             const int r = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                            list[nact + r] = (short)(bz * 64 + lane);
+                            list[nact + r] = (lw + ((unsigned)bz << 6)) | ((unsigned)bz << 15) |
+                                             ((unsigned)(z0 & 63) << 18) | ((unsigned)(z1 & 63) << 24);
                         }
                         nact += (int)__popcll(bal);
                     }
@@ -943,10 +986,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 #pragma unroll
                                     for (int lx = 0; lx < kRtCh; ++lx) cxs[lx] = (unsigned)cells[0][c0x + min(lx, nx - 1)];
 #pragma unroll
-                                    for (int lx = 0; lx < kRtCh; ++lx) w[lx] = cpk[rowc + cxs[lx]];
-#pragma unroll
                                     for (int lx = 0; lx < kRtCh; ++lx) abc[lx] = coef[rowc + cxs[lx]];
-                                    double sa = 0.0, sb = 0.0, sc = 0.0;
+#pragma unroll
+                                    for (int lx = 0; lx < kRtCh; ++lx) w[lx] = __float_as_uint(abc[lx].w);
+                                    float sa = 0.0f, sb = 0.0f, sc = 0.0f;
                                     const int tb = rt_at(lz + 1, ly + 1, 1);
 #pragma unroll
                                     for (int lx = 0; lx < kRtCh; ++lx) {
@@ -963,18 +1006,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                                             float r2 = 0.0f;
 #pragma unroll
                                             for (int d = 0; d < 3; ++d) r2 += (x[d] - y[d]) * (x[d] - y[d]);
-                                            const float kw = neighborhood_weight(r2, pr.kc.smin);
-                                            if (kw > 0.0f) {
-                                                const double kw2 = (double)kw * (double)kw;
-                                                sa += (double)abc[lx].x * kw2;
-                                                sb += (double)abc[lx].y * kw2;
-                                                sc += (double)abc[lx].z * (double)kw;
-                                            }
+                                            // neighborhood_weight with the reciprocal of support^2
+                                            const float q = r2 * inv_s2, q2 = q * q;
+                                            const float f = fmaxf(0.0f, 1.0f - q2 * q2 * q);
+                                            const float kw = f * f;
+                                            const float kw2 = kw * kw;
+                                            sa = fmaf(abc[lx].x, kw2, sa);
+                                            sb = fmaf(abc[lx].y, kw2, sb);
+                                            sc = fmaf(abc[lx].z, kw, sc);
                                         }
                                         if (lx < nx) {
-                                            tA[tb + lx] = (float)sa;
-                                            tB[tb + lx] = (float)sb;
-                                            tC[tb + lx] = (float)sc;
+                                            tA[tb + lx] = sa;
+                                            tB[tb + lx] = sb;
+                                            tC[tb + lx] = sc;
                                         }
                                     }
                                 }
@@ -1029,12 +1073,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                             // the active slots, 64 at a time: 8-corner sums of their boxes
                             for (int e0 = 0; e0 < nact; e0 += 64) {
                                 if (e0 + lane >= nact) continue;
-                                const int slot = list[e0 + lane];
-                                const int sbz = slot >> 6, sl = slot & 63;
-                                const int sbx = sl % g.NB[0], sby = sl / g.NB[0];
+                                const unsigned ent = list[e0 + lane];
+                                const int slot = (int)(ent & 511u);
+                                const int sbx = (int)((ent >> 9) & 7u), sby = (int)((ent >> 12) & 7u);
+                                const int sbz = (int)((ent >> 15) & 7u);
                                 const int xa = max(rlo[0][sbx], c0x) - c0x, xb = min(rhi[0][sbx], c0x + nx - 1) - c0x + 1;
                                 const int ya = max(rlo[1][sby], c0y) - c0y, yb = min(rhi[1][sby], c0y + ny - 1) - c0y + 1;
-                                const int za = max(rlo[2][sbz], c0z) - c0z, zb = min(rhi[2][sbz], c0z + nz - 1) - c0z + 1;
+                                const int za = max((int)((ent >> 18) & 63u), c0z) - c0z;
+                                const int zb = min((int)((ent >> 24) & 63u), c0z + nz - 1) - c0z + 1;
                                 if (xa >= xb || ya >= yb || za >= zb) continue;
                                 const float qx = pr.cp.o[0] + pr.cp.h[0] * (float)(k[0] - g.A[0] + sbx);
                                 const float qy = pr.cp.o[1] + pr.cp.h[1] * (float)(k[1] - g.A[1] + sby);
@@ -1064,12 +1110,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
         }
     }
     // this block's sums: [block][slot][axis]
-    for (int i = lane; i < kRtBox * 64 * 3; i += 64) partials[(long long)b * (kRtBox * 64 * 3) + i] = (double)accs[i];
+    for (int i = lane; i < kRtBox * 64 * 3; i += 64) partials[(long long)b * (kRtBox * 64 * 3) + i] = accs[i];
 }
 
-// deriv_j = coef_scale * the sum over the blocks whose box holds j, in block order.
+// deriv_j = coef_scale * the sum over the blocks whose box holds j (cells,
+// then their pieces), in block order.
 __global__ __launch_bounds__(kThreads) void k_lcc_reft_finish(Problem pr, RefTableGeom g,
-                                                              const double *__restrict__ partials,
+                                                              const float *__restrict__ partials,
                                                               const Scalars *__restrict__ sc,
                                                               float *__restrict__ deriv)
 {
@@ -1085,14 +1132,17 @@ __global__ __launch_bounds__(kThreads) void k_lcc_reft_finish(Problem pr, RefTab
         klo[a] = max(jc[a] + g.A[a] - g.NB[a] + 1, g.kmin[a]);
         khi[a] = min(jc[a] + g.A[a], g.kmin[a] + g.nk[a] - 1);
     }
+    const int npc = g.np[0] * g.np[1] * g.np[2];
     double sum = 0.0;
     for (int kz = klo[2]; kz <= khi[2]; ++kz)
         for (int ky = klo[1]; ky <= khi[1]; ++ky)
             for (int kx = klo[0]; kx <= khi[0]; ++kx) {
-                const long long b = (kx - g.kmin[0]) + (long long)g.nk[0] * ((ky - g.kmin[1]) +
-                                                                             (long long)g.nk[1] * (kz - g.kmin[2]));
+                const long long cell = (kx - g.kmin[0]) + (long long)g.nk[0] * ((ky - g.kmin[1]) +
+                                                                                (long long)g.nk[1] * (kz - g.kmin[2]));
                 const int bx = jc[0] - kx + g.A[0], by = jc[1] - ky + g.A[1], bz = jc[2] - kz + g.A[2];
-                sum += partials[((b * kRtBox + bz) * 64 + bx + g.NB[0] * by) * 3 + d];
+                const long long slot = (long long)(bz * 64 + bx + g.NB[0] * by) * 3 + d;
+                for (int pc = 0; pc < npc; ++pc)
+                    sum += (double)partials[(cell * npc + pc) * (kRtBox * 64 * 3) + slot];
             }
     deriv[i] = (float)(sum * sc->coef_scale);
 }
@@ -1103,8 +1153,10 @@ static bool ref_table_geom(const Problem &pr, RefTableGeom &g)
     if (pr.dim != 3 || !is_stationary(pr.kc.mode) || pr.sub < 1 || pr.sub > 32 || pr.subn < 1) return false;
     for (int d = 0; d < 3; ++d) {
         const double r = (double)pr.kc.smin / (double)pr.fixed.h[d];
-        // positions of a voxel's reach: 64 at most (the masks), cells as many
-        if ((std::floor(r) - std::floor(1.0 - r)) + 3.0 > 64.0) return false;
+        // the centre positions reaching a voxel, [y - floor(r) - 1, y - floor(1 - r) + 1],
+        // must lie within 64 cells (the reach rows' masks)
+        const double positions = std::floor(r) - std::floor(1.0 - r) + 3.0;
+        if (std::ceil(positions / pr.sub) + 1.0 > 64.0) return false;
         const double sh = (double)pr.kc.smin / (double)pr.cp.h[d];
         const int nb = 2 * (int)std::ceil(sh);
         if (nb < 2 || nb > kRtBox) return false;
@@ -1115,33 +1167,95 @@ static bool ref_table_geom(const Problem &pr, RefTableGeom &g)
         g.kmin[d] = (int)std::floor((y0 - (double)pr.cp.o[d]) / (double)pr.cp.h[d]) - 1;
         const int kmax = (int)std::floor((y1 - (double)pr.cp.o[d]) / (double)pr.cp.h[d]) + 1;
         g.nk[d] = kmax - g.kmin[d] + 1;
+        // pieces of about 5 voxels per axis, so that large cells fill the GPU
+        const double vox = (double)pr.cp.h[d] / (double)pr.fixed.h[d];
+        g.np[d] = std::max(1, std::min(16, (int)std::lround(vox / 5.0)));
     }
     if (g.NB[0] * g.NB[1] > 64) return false;
-    return (long long)g.nk[0] * g.nk[1] * g.nk[2] < (1ll << 31);
+    return (long long)g.nk[0] * g.nk[1] * g.nk[2] * g.np[0] * g.np[1] * g.np[2] < (1ll << 31);
 }
 
+// The centre walk's (control point, centre, neighbour) triples, counted per
+// axis with each centre at its cell origin: the product over the axes of the
+// sum over the control points of the neighbours along the axis of the
+// centres in the point's window.
+static double walk_triples(const Problem &pr)
+{
+    const auto clampi = [](int v, int lo, int hi) { return std::min(std::max(v, lo), hi); };
+    double t = 1.0;
+    for (int d = 0; d < pr.dim; ++d) {
+        const int n = pr.fixed.n[d], ns = pr.nsub[d], sub = pr.sub;
+        const double r = (double)pr.kc.smin / (double)pr.fixed.h[d];
+        std::vector<double> pre((size_t)ns + 1, 0.0);
+        for (int c = 0; c < ns; ++c) {
+            const int pos = c * sub;
+            const int nlo = clampi((int)((double)pos - r + 1.0), 0, n - 1), nhi = clampi((int)((double)pos + r), 0, n - 1);
+            pre[(size_t)c + 1] = pre[(size_t)c] + (double)((nhi - nlo) / pr.subn + 1);
+        }
+        double sum = 0.0;
+        for (int j = 0; j < pr.cp.n[d]; ++j) {
+            const double idx = ((double)pr.cp.o[d] + (double)pr.cp.h[d] * j - (double)pr.fixed.o[d]) / (double)pr.fixed.h[d];
+            const int lo = clampi((int)((idx - r) / sub + 1.0), 0, ns - 1), hi = clampi((int)((idx + r) / sub), 0, ns - 1);
+            if (lo <= hi) sum += pre[(size_t)hi + 1] - pre[(size_t)lo];
+        }
+        t *= sum;
+    }
+    return t;
+}
+
+// Which form of the reference-window derivative runs: lcc_ref_walk 1 the
+// walk, 2 the voxel tables where their geometry allows, 0 the one with the
+// smaller expected time.  Rates measured on MI355X at the sliding-motion
+// registration's three levels (256x256x128, supports 80 / 40 / 20): the walk
+// 0.9-1.7e12 of walk_triples' triples per second (the lower one here), the
+// tables 2.8e8 voxels per second (1.7e8 at support 80, where the walk wins
+// by far).
 bool lcc_ref_table_applies(const Problem &pr)
 {
     RefTableGeom g;
-    return pr.lcc_ref_walk == 0 && ref_table_geom(pr, g);
+    if (pr.lcc_ref_walk == 1 || !ref_table_geom(pr, g)) return false;
+    if (pr.lcc_ref_walk == 2) return true;
+    constexpr double kWalkTriplesPerS = 0.9e12, kTableVoxelsPerS = 2.8e8;
+    return walk_triples(pr) / kWalkTriplesPerS > (double)pr.n_vox / kTableVoxelsPerS;
 }
+
+// block sums (FP32), then the reach rows (cm, cell, allowed, cbase) of every
+// index of every axis
+static long long ref_table_blocks(const RefTableGeom &g)
+{
+    return (long long)g.nk[0] * g.nk[1] * g.nk[2] * g.np[0] * g.np[1] * g.np[2];
+}
+static long long ref_reach_rows(const Problem &pr) { return (long long)pr.fixed.n[0] + pr.fixed.n[1] + pr.fixed.n[2]; }
 
 long long lcc_ref_table_partials(const Problem &pr)
 {
     RefTableGeom g;
     if (!ref_table_geom(pr, g)) return 0;
-    return (long long)g.nk[0] * g.nk[1] * g.nk[2] * kRtBox * 64 * 3;
+    const long long rows = ref_reach_rows(pr);
+    const long long bytes = ref_table_blocks(g) * kRtBox * 64 * 3 * 4 + rows * (8 + 64 * 4 + 64 * 4 + 4);
+    return (bytes + 7) / 8;
 }
 
-void launch_lcc_ref_table(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const unsigned *cpk,
-                          const Scalars *sc, float *deriv_soa, double *partials, hipStream_t s)
+void launch_lcc_ref_table(const Problem &pr, const float4 *dense_g, const float4 *centre_coef, const Scalars *sc,
+                          float *deriv_soa, double *partials, hipStream_t s)
 {
     RefTableGeom g;
     if (!ref_table_geom(pr, g)) return;
-    const long long blocks = (long long)g.nk[0] * g.nk[1] * g.nk[2];
-    k_lcc_ref_table<<<(unsigned)blocks, 64, 0, s>>>(pr, dense_g, centre_coef, cpk, g, partials);
+    const long long rows = ref_reach_rows(pr), blocks = ref_table_blocks(g);
+    float *sums = reinterpret_cast<float *>(partials);
+    RefReach rr;
+    // (the block sums are a multiple of 8 bytes: 1536 floats per block)
+    rr.cm = reinterpret_cast<unsigned long long *>(sums + blocks * kRtBox * 64 * 3);
+    rr.cell = reinterpret_cast<int *>(rr.cm + rows);
+    rr.allowed = reinterpret_cast<unsigned *>(rr.cell + rows * 64);
+    rr.cbase = reinterpret_cast<int *>(rr.allowed + rows * 64);
+    rr.off[0] = 0;
+    rr.off[1] = pr.fixed.n[0];
+    rr.off[2] = pr.fixed.n[0] + pr.fixed.n[1];
+    k_lcc_reach<<<(unsigned)rows, 64, 0, s>>>(pr, rr);
+    k_lcc_ref_table<<<(unsigned)blocks, 64, 0, s>>>(pr, dense_g, centre_coef, rr, g, sums);
     const unsigned fb = (unsigned)((pr.n_cp * 3 + kThreads - 1) / kThreads);
-    k_lcc_reft_finish<<<fb, kThreads, 0, s>>>(pr, g, partials, sc, deriv_soa);
+    k_lcc_reft_finish<<<fb, kThreads, 0, s>>>(pr, g, sums, sc, deriv_soa);
 }
 
 // ------------------------------------------------------------ launchers

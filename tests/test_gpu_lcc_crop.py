@@ -2,8 +2,8 @@
 points every 5 voxels, subsample 2, neighbourhood stride 6, so 7^3
 neighbours per centre) on volumes small enough for the OpenMP FP64 oracle:
 both derivative modes against the oracle; the reference-window derivative
-in both of its forms, regrouped per neighbour voxel (the automatic choice,
-the same bits on a repeated call) and as the centre walk at forced workgroup
+in both of its forms, regrouped per neighbour voxel (the same bits on a
+repeated call; also at the coarser levels' parameters) and as the centre walk at forced workgroup
 splits (1, 2 and 64 workgroups per control-point tile, the split that the
 full-size workload's 51x51x26 grid and the coarse levels' grids select); and
 the exact-mode derivative's bits on a repeated call.
@@ -60,7 +60,8 @@ def test_lcc_reference_windows_cfg_parameters_forced_splits():
     ores, oder = _oracle(case)
     ev = case.gpu()
     ev.getValue(MEASURE.LCC, case.cp, case.cpw)  # first draw of the offsets
-    # the automatic path: the sum regrouped per neighbour voxel, twice (same bits)
+    # the sum regrouped per neighbour voxel, twice (same bits)
+    ev.set_tuning(lcc_ref_walk=2)
     r1, d1 = _check(ev, case, ores, oder, "voxel tables")
     r2, d2 = _check(ev, case, ores, oder, "voxel tables, repeated")
     assert r1.measure == r2.measure and np.array_equal(d1, d2), "voxel-table derivative differs between runs"
@@ -69,6 +70,28 @@ def test_lcc_reference_windows_cfg_parameters_forced_splits():
         ev.set_tuning(lcc_ref_chunks=chunks, lcc_ref_walk=1)
         _, dw = _check(ev, case, ores, oder, f"centre walk, chunks={chunks or 'auto'}")
         assert rel_err(d1, dw) < 1e-5
+    ev.close()
+
+
+@pytest.mark.parametrize("support,spacing,sub,subn", [(40.0, 10.0, 3, 12), (80.0, 20.0, 4, 23)])
+def test_lcc_reference_windows_coarse_level_parameters(support, spacing, sub, subn):
+    """The sliding-motion registration's coarser levels (support 40 and 80,
+    control points every support / 4, subsample 3 and 4, strides 12 and 23):
+    reach rows wider than 64 positions and control-point cells split into
+    voxel pieces; the voxel tables and the centre walk against the oracle."""
+    case = Case(dim=3, shape=(48, 48, 48), cp_spacing=spacing, sigma=support, mode=0, subsample=sub,
+                subsample_neighborhood=subn, seed=23, exact=False, coef_scale=1.0)
+    mov, fix, _ = synthetic.sliding_3d((48, 48, 48), shift=2.0, breathing=3.0)
+    case.moving, case.fixed = mov, fix
+    print(f"LCC reference windows, 48^3, support {support:g}, stride {subn}, subsample {sub}", flush=True)
+    ores, oder = _oracle(case)
+    ev = case.gpu()
+    ev.getValue(MEASURE.LCC, case.cp, case.cpw)
+    ev.set_tuning(lcc_ref_walk=2)
+    _, d1 = _check(ev, case, ores, oder, "voxel tables")
+    ev.set_tuning(lcc_ref_walk=1)
+    _, dw = _check(ev, case, ores, oder, "centre walk")
+    assert rel_err(d1, dw) < 1e-5
     ev.close()
 
 
