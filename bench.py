@@ -34,6 +34,13 @@ FLOPS_PER_PAIR = {"stationary": 32, "anisotropic": 32 + 75}
 # LCC reference-window derivative: per (control point, centre, neighbour)
 # triple the isotropic Wendland ka(p_j, y) and the weighted accumulate
 FLOPS_PER_LCC_TRIPLE = 32
+# the same derivative regrouped per voxel (k_lcc_ref_table): per candidate
+# centre of a voxel's table its weight kw (distance, (r/s)^10, squares) and
+# three weighted terms; per (control point, voxel) pair with ka != 0 the
+# kernel ka (32), the 8-corner sums of three tables (21) and the combination
+# and accumulate (10)
+FLOPS_PER_TABLE_ENTRY = 20
+FLOPS_PER_TABLE_PAIR = 63
 
 
 def parse():
@@ -405,6 +412,33 @@ def lcc_triples(shape, cs, co, h, support, sub, subn):
     return float(total.sum())
 
 
+def lcc_reach_cells(n, r, sub, subn):
+    """Per voxel index along one axis: the sample cells holding a centre
+    position whose neighbour lattice (neighbour_box's arithmetic,
+    GpuEvaluatorLCC.cu:95-97) holds the voxel -- the ranks of its table."""
+    out = np.zeros(n, np.int64)
+    for y in range(n):
+        cells = set()
+        for p in range(max(0, y - int(np.floor(r)) - 1), min(n - 1, y - int(np.floor(1.0 - r)) + 1) + 1):
+            nlo = min(max(int(p - r + 1.0), 0), n - 1)
+            nhi = min(max(int(p + r), 0), n - 1)
+            if nlo <= y <= nhi and (y - nlo) % subn == 0 and p // sub < n // sub:
+                cells.add(p // sub)
+        out[y] = len(cells)
+    return out
+
+
+def lcc_table_work(shape, cs, co, h, s, sub, subn):
+    """(table entries, (control point, voxel) pairs with ka != 0) of one
+    voxel-table derivative: entries = sum over voxels of the product of the
+    per-axis reach cells; pairs = exact count of |p - y| < s."""
+    size = tuple(reversed(shape))
+    entries = float(np.prod([lcc_reach_cells(size[d], s, sub, subn).sum() for d in range(3)]))
+    voxels = [np.arange(size[d], dtype=np.float64) for d in range(3)]
+    cps = [co[d] + h * np.arange(cs[d]) for d in range(3)]
+    return entries, float(support_pairs(voxels, cps, s))
+
+
 def lcc_cpu_baseline(mov, fix, h, s, sub, subn, triples_full, bilateral, crop=64):
     """The OpenMP build of the FP64 oracle (oracle/skm_oracle.c, the
     reference's LCC loops restated) timed on a crop^3 block at the centre of
@@ -447,9 +481,10 @@ def lcc_timing(args):
     control-point spacing 5 -> 51x51x26, isotropic Wendland support 20,
     bilateral 1e-3 (spatial sigma 12, intensity sigma 0.1), subsample 2,
     neighbourhood stride 6 -> 7^3 neighbours per centre):
-    the reference's windowed derivative (GpuEvaluatorLCC.cu:206-530) and the
-    exact gradient (each voxel gathers the terms of the centres that reach
-    it, then one kernel-sum gather); median of 3 after one untimed
+    the reference's windowed derivative (GpuEvaluatorLCC.cu:206-530), in the
+    automatic form (regrouped per voxel at this size) and as the centre walk,
+    and the exact gradient (each voxel gathers the terms of the centres that
+    reach it, then one kernel-sum gather); median of 3 after one untimed
     evaluation."""
     from skmimageregistration_amd import MEASURE, GpuKernelEvaluator, Grid, Image, Kernel, synthetic
     shape, h, s, sub, subn = (128, 256, 256), 5.0, 20.0, 2, 6
@@ -460,10 +495,13 @@ def lcc_timing(args):
     out = {"workload": "LCC GetValueAndDerivative, 3D 256x256x128 sliding motion, cp 51x51x26 spacing 5, "
                        "isotropic Wendland support 20, bilateral 0.001 (spatial sigma 12, intensity sigma 0.1), "
                        "subsample 2, neighbourhood stride 6"}
-    for name, exact in (("exact_gradient", True), ("reference_windows", False)):
+    for name, exact, walk in (("exact_gradient", True, 0), ("reference_windows", False, 0),
+                              ("reference_windows_walk", False, 1)):
         ev = GpuKernelEvaluator(n * 3, 2, 6, Kernel(s, s), Image(mov), Image(fix), Grid(cs, co, (h,) * 3),
                                 exact_gradient=exact)
         ev.SetRegularizerBilateral(1e-3, args.bilateral_spatial_sigma, args.bilateral_intensity_sigma)
+        if walk:
+            ev.set_tuning(lcc_ref_walk=1)
         ev.getValueAndDerivative(MEASURE.LCC, cp)
         times = []
         for _ in range(3):
@@ -475,19 +513,32 @@ def lcc_timing(args):
         passes = {k: round(v[0] / max(v[1], 1), 3) for k, v in ev.read_timing().items() if v[1]}
         ev.close()
         out[name] = {"ms": round(float(np.median(times)), 2), "pass_ms": passes, "measure": round(r.measure, 6)}
-    # the reference-window derivative's own work: (cp, centre, neighbour) triples
+    # the reference windows' derivative by voxel tables (the automatic form at
+    # this size): its own work, and the reference enumeration's (cp, centre,
+    # neighbour) triples per second
     tri = lcc_triples(shape, cs, co, h, s, sub, subn)
+    entries, pairs = lcc_table_work(shape, cs, co, h, s, sub, subn)
     key = f"lcc-{size[0]}x{size[1]}x{size[2]}-h{h:g}-s{s:g}-sub{sub}-n{subn}"
     ref_ms = out["reference_windows"]["pass_ms"]["lcc_derivative"]
-    achieved = tri * FLOPS_PER_LCC_TRIPLE / (ref_ms * 1e-3) / 1e12
-    roof = {"bound": "valu", "kernel": "k_lcc_gather_ref", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+    flops = entries * FLOPS_PER_TABLE_ENTRY + pairs * FLOPS_PER_TABLE_PAIR
+    achieved = flops / (ref_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": "k_lcc_ref_table", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "launch_ms": ref_ms,
-            "triples_per_launch": tri, "flops_per_triple": FLOPS_PER_LCC_TRIPLE,
-            "traffic": profiled_traffic("k_lcc_gather_ref", key)}
+            "table_entries_per_launch": entries, "flops_per_entry": FLOPS_PER_TABLE_ENTRY,
+            "pairs_per_launch": pairs, "flops_per_pair": FLOPS_PER_TABLE_PAIR,
+            "traffic": profiled_traffic("k_lcc_ref_table", key),
+            "reference_triples_per_s": round(tri / (ref_ms * 1e-3), 1)}
     if roof["traffic"]:
         roof["hbm_traffic_frac"] = round(roof["traffic"] / (ref_ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 4)
-        roof["limiter"] = "hbm traffic" if roof["hbm_traffic_frac"] > roof["frac"] else "fp32 valu"
     out["roofline"] = roof
+    # the centre walk (the reference's enumeration order) on its own work
+    walk_ms = out["reference_windows_walk"]["pass_ms"]["lcc_derivative"]
+    walk_tf = tri * FLOPS_PER_LCC_TRIPLE / (walk_ms * 1e-3) / 1e12
+    out["reference_windows_walk"]["roofline"] = {
+        "bound": "valu", "kernel": "k_lcc_gather_ref", "achieved": round(walk_tf, 3), "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s", "frac": round(walk_tf / FP32_PEAK_TFLOPS, 4), "launch_ms": walk_ms,
+        "triples_per_launch": tri, "flops_per_triple": FLOPS_PER_LCC_TRIPLE,
+        "traffic": profiled_traffic("k_lcc_gather_ref", key)}
     # the exact-gradient path's streaming kernels against HBM
     hbm = {}
     for kname, pname in (("k_lcc_centres", "lcc_centres"), ("k_dense_warp", "lcc_dense")):
