@@ -46,7 +46,8 @@ FLOPS_PER_TABLE_PAIR = 63
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200,
+                    help="timed evaluations (200: one pyramid level's optimizer iterations)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shape", default="96,128,128", help="z,y,x voxels")
     ap.add_argument("--cp-spacing", type=float, default=6.0)
@@ -882,12 +883,33 @@ def main():
     torch.cuda.set_stream(stream)
     ev.set_stream(stream.cuda_stream)
     rng = np.random.default_rng(1)
-    cp = torch.tensor(rng.uniform(-1, 1, size=(n_cp, 3)).astype(np.float32), device=dev)
+    # two parameter sets, alternated: consecutive evaluations have different inputs
+    cps = [torch.tensor(rng.uniform(-1, 1, size=(n_cp, 3)).astype(np.float32), device=dev) for _ in range(2)]
     cpw = torch.ones(n_cp, dtype=torch.float32, device=dev)
     der = torch.empty(n_cp * 3, dtype=torch.float32, device=dev)
 
-    def step():
-        ev.get_value_and_derivative_device(measure, cp.data_ptr(), cpw.data_ptr(), der.data_ptr(), True)
+    def step(i):
+        ev.get_value_and_derivative_device(measure, cps[i & 1].data_ptr(), cpw.data_ptr(), der.data_ptr(), True)
+
+    def timed(n):
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for i in range(n):
+            step(i)
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop)
+
+    def pass_times(n):
+        ev.enable_timing(True)
+        ev.read_timing()
+        for i in range(n):
+            step(i)
+        t = ev.read_timing()
+        ev.enable_timing(False)
+        return {k: v[0] / max(v[1], 1) for k, v in t.items() if v[1] > 0}
 
     if args.calibration_copy > 0:
         src = torch.full((args.calibration_copy * (1 << 18),), 1.0, dtype=torch.float32, device=dev)
@@ -896,49 +918,62 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    for _ in range(args.warmup):
-        step()
+    for i in range(args.warmup):
+        step(i)
     torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.steps):
-        step()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    total_ms = start.elapsed_time(stop)
+    # the timed steps start without the kernel matrix: its assembly (at the
+    # first step) is inside the timed region
+    ev.release_kernel_matrix()
+    total_ms = timed(args.steps)
     ms_step = total_ms / args.steps
-    # per-pass device times: HIP events around each pass, on the evaluator's stream
-    ev.enable_timing(True)
-    ev.read_timing()
-    for _ in range(args.steps):
-        step()
-    passes = ev.read_timing()
-    ev.enable_timing(False)
+    km = ev.kernel_matrix_info()
+    per_kernel = pass_times(min(args.steps, 50))  # per-pass device times, assembled
+    steady_ms = timed(min(args.steps, 50)) / min(args.steps, 50)
+    # the same evaluations with the kernels evaluated on the fly
+    ev.set_tuning(kernel_matrix=1)
+    for i in range(2):
+        step(i)
+    n_fly = min(args.steps, 20)
+    fly_ms = timed(n_fly) / n_fly
+    fly_kernel = pass_times(n_fly)
     ev.close()
 
     exact = args.subsample == 1 and not args.no_pair_count
     vbox, dbox, vnz, dnz = pair_counts(args, shape, ten, mcov, dev, exact)
     value = (vbox + dbox) / (ms_step * 1e-3) / 1e9
-    per_kernel = {k: v[0] / max(v[1], 1) for k, v in passes.items() if v[1] > 0}
-    dom = max(("value", "gather"), key=lambda k: per_kernel.get(k, 0.0))
-    roof = None
+    key = workload_key(args, shape)
+    # the on-the-fly passes: FP32-VALU bound (kernel evaluation per pair)
+    fly_roof = None
     fpp = FLOPS_PER_PAIR[args.kernel]
     if vnz is not None:
+        dom = max(("value", "gather"), key=lambda k: fly_kernel.get(k, 0.0))
         pairs = vnz if dom == "value" else dnz
-        achieved = pairs * fpp / (per_kernel[dom] * 1e-3) / 1e12
-        kname = "k_value" if dom == "value" else ("k_gather" if args.kernel == "stationary" else "k_gather_tile")
-        traffic = profiled_traffic(kname, workload_key(args, shape))
-        roof = {"bound": "valu", "kernel": kname,
-                "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                "launch_ms": round(per_kernel[dom], 4), "pairs_per_launch": pairs, "flops_per_pair": fpp}
-        if traffic:
-            # the measured HBM traffic (rocprofv3 FETCH + WRITE) as a fraction of the HBM peak: when it
-            # exceeds the FP32 fraction, memory traffic co-limits the kernel
-            hbm_frac = traffic / (per_kernel[dom] * 1e-3) / (HBM_PEAK_GBS * 1e9)
-            roof["hbm_traffic_frac"] = round(hbm_frac, 4)
-            roof["limiter"] = "hbm traffic" if hbm_frac > roof["frac"] else "fp32 valu"
+        achieved = pairs * fpp / (fly_kernel[dom] * 1e-3) / 1e12
+        kname = "k_value" if dom == "value" else "k_gather_tile"
+        fly_roof = {"bound": "valu", "kernel": kname, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                    "traffic": profiled_traffic(kname, key), "launch_ms": round(fly_kernel[dom], 4),
+                    "pairs_per_launch": pairs, "flops_per_pair": fpp}
+    # the kernel-matrix passes: HBM bound (streaming the stored pairs)
+    roof = None
+    if km["state"] == 1:
+        n_smp = int(np.prod(shape))
+        per_smp = {"value": 4 + 4 + 16 + (4 if args.measure == "ncc" else 0),  # F, coefficients, (gradM, a), b
+                   "gather": 16}                                              # v
+        layout = {"value": km["read_value"], "gather": km["read_gather"]}
+        nnz = {"value": km["pairs_value"], "gather": km["pairs_gather"]}
+        dom = max(("value", "gather"), key=lambda k: per_kernel.get(k, 0.0))
+        t = per_kernel[dom] * 1e-3
+        algo = layout[dom] + per_smp[dom] * n_smp
+        kname = "k_km_value" if dom == "value" else "k_km_gather"
+        roof = {"bound": "hbm", "kernel": kname, "achieved": round(algo / t / 1e9, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(algo / t / 1e9 / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(kname, key), "launch_ms": round(per_kernel[dom], 4),
+                "bytes_per_launch": algo, "layout_bytes": layout[dom], "pairs_per_launch": nnz[dom],
+                # the same launch at 4 bytes per stored non-zero pair (no records, no zero slots)
+                "frac_at_4_bytes_per_pair": round((4 * nnz[dom] + per_smp[dom] * n_smp) / t / 1e9 / HBM_PEAK_GBS, 4)}
+    else:
+        roof = fly_roof  # no kernel matrix (subsampled, or refused): the timed steps ran on the fly
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, mov, fix, ten, mcov)
@@ -956,6 +991,12 @@ def main():
                    "pairs_nonzero_value": vnz, "pairs_nonzero_derivative": dnz},
         "pass_ms": {k: round(v, 4) for k, v in per_kernel.items()},
         "roofline": roof, "cpu_baseline": cpu,
+        "kernel_matrix": {"state": km["state"], "assembly_ms": round(km["build_ms"], 3),
+                          "ms_per_step_assembled": round(steady_ms, 4), "bytes": km["bytes"],
+                          "read_value": km["read_value"], "read_gather": km["read_gather"],
+                          "pairs_value": km["pairs_value"], "pairs_gather": km["pairs_gather"]},
+        "on_the_fly": {"ms_per_step": round(fly_ms, 4), "value": round((vbox + dbox) / (fly_ms * 1e-3) / 1e9, 3),
+                       "pass_ms": {k: round(v, 4) for k, v in fly_kernel.items()}, "roofline": fly_roof},
     }
     if tensor_ms is not None:
         line["tensor_generation_ms"] = round(tensor_ms, 2)

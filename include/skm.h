@@ -178,8 +178,8 @@ SKM_API skm_status skm_synchronize(skm_evaluator *ev);
  * wall-clock only, itkIterationObserver.h:59-63).  Passes: 0 jitter offsets,
  * 1 value, 2 reduction, 3 derivative preparation, 4 derivative gather,
  * 5 regularizers, 6 LCC dense warp, 7 LCC centres, 8 LCC derivative,
- * 9 displacement field. */
-#define SKM_NUM_PASSES 10
+ * 9 displacement field, 10 kernel-matrix assembly. */
+#define SKM_NUM_PASSES 11
 typedef struct skm_pass_times {
     double ms[SKM_NUM_PASSES];
     uint32_t launches[SKM_NUM_PASSES];
@@ -231,6 +231,10 @@ SKM_API skm_status skm_fit_coefficients_device(skm_evaluator *ev, const float *d
  *                   neighbour); auto (0) takes the form with the smaller
  *                   expected time (the walk's triples against the voxels);
  *                   results agree to FP rounding
+ *   kernel_matrix   MSE / NCC with subsample 1: 0 assembles the evaluator's
+ *                   kernel matrix at its first evaluation and reads it from
+ *                   then on, 1 evaluates the kernels on the fly (see
+ *                   skm_get_kernel_matrix_info)
  * skm_get_tuning returns the fields as set (0 where automatic). */
 typedef struct skm_tuning {
     int32_t lcc_ref_chunks;
@@ -238,10 +242,40 @@ typedef struct skm_tuning {
     int32_t value_split;
     int32_t gather_regions;
     int32_t lcc_ref_walk;
-    int32_t reserved[3];
+    int32_t kernel_matrix;
+    int32_t reserved[2];
 } skm_tuning;
 SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning);
 SKM_API skm_status skm_get_tuning(skm_evaluator *ev, skm_tuning *tuning);
+
+/* The kernel matrix (no counterpart in the reference, which evaluates every
+ * (voxel, control point) kernel pair in the value pass and again in the
+ * derivative pass of every GetValueAndDerivative, GpuEvaluatorMSE.cu:61-80,
+ * 134-162; GpuEvaluatorNCC.cu:61-82, 145-176).  With subsample 1 the samples
+ * are the fixed voxels and never move, and the kernel depends only on the
+ * positions and the weight image / tensor field, all fixed for an evaluator's
+ * life: its non-zero k(x_m, p_j) are assembled once in device memory, in the
+ * order each pass reads them, and every later MSE / NCC evaluation streams
+ * them instead of evaluating the kernel.  It is assembled at the first MSE /
+ * NCC evaluation unless skm_tuning.kernel_matrix is 1 or that evaluation is
+ * being captured into a graph; it is refused (the kernels then run on the fly)
+ * when it would hold more than SKM_KERNEL_MATRIX_MAX_BYTES (environment) or,
+ * by default, a quarter of the device's memory or half of its free memory.
+ * Results equal the on-the-fly passes' within FP32 rounding of the sums'
+ * order, and are the same bits on every call.  skm_release_kernel_matrix
+ * frees it (the next evaluation assembles it again). */
+typedef struct skm_kernel_matrix_info {
+    int32_t state;          /* 0 not assembled, 1 assembled, 2 refused (too large) */
+    int32_t reserved;
+    uint64_t bytes;         /* device memory it holds (or would hold, when refused) */
+    uint64_t pairs_value;   /* non-zero pairs stored for the value pass */
+    uint64_t pairs_gather;  /* non-zero pairs stored for the derivative gather */
+    uint64_t read_value;    /* bytes of the value layout one evaluation reads */
+    uint64_t read_gather;   /* bytes of the gather layout one evaluation reads */
+    double build_ms;        /* wall time of the assembly */
+} skm_kernel_matrix_info;
+SKM_API skm_status skm_get_kernel_matrix_info(skm_evaluator *ev, skm_kernel_matrix_info *info);
+SKM_API skm_status skm_release_kernel_matrix(skm_evaluator *ev);
 
 /* The project's autotuner (no counterpart in the reference): times every
  * launch shape skm_set_tuning offers for `measure` on this evaluator (value

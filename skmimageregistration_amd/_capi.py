@@ -66,9 +66,9 @@ class SkmOptimizerOptions(ctypes.Structure):
                 ("iterations", ctypes.c_int32), ("return_best", ctypes.c_int32), ("graph", ctypes.c_int32)]
 
 
-SKM_NUM_PASSES = 10
+SKM_NUM_PASSES = 11
 PASS_NAMES = ("offsets", "value", "reduce", "prepare", "gather", "regularizers", "lcc_dense", "lcc_centres",
-              "lcc_derivative", "displacement")
+              "lcc_derivative", "displacement", "kernel_matrix")
 
 
 class SkmPassTimes(ctypes.Structure):
@@ -78,7 +78,13 @@ class SkmPassTimes(ctypes.Structure):
 class SkmTuning(ctypes.Structure):
     _fields_ = [("lcc_ref_chunks", ctypes.c_int32), ("gather_split", ctypes.c_int32),
                 ("value_split", ctypes.c_int32), ("gather_regions", ctypes.c_int32),
-                ("lcc_ref_walk", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("lcc_ref_walk", ctypes.c_int32), ("kernel_matrix", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+
+
+class SkmKernelMatrixInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("reserved", ctypes.c_int32), ("bytes", ctypes.c_uint64),
+                ("pairs_value", ctypes.c_uint64), ("pairs_gather", ctypes.c_uint64), ("read_value", ctypes.c_uint64),
+                ("read_gather", ctypes.c_uint64), ("build_ms", ctypes.c_double)]
 
 
 class SkmAutotuneReport(ctypes.Structure):
@@ -180,6 +186,8 @@ _SIGNATURES = {
     "skm_read_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmPassTimes)]),
     "skm_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmTuning)]),
     "skm_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmTuning)]),
+    "skm_get_kernel_matrix_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkmKernelMatrixInfo)]),
+    "skm_release_kernel_matrix": (ctypes.c_int, [ctypes.c_void_p]),
     "skm_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                     ctypes.POINTER(SkmAutotuneReport)]),
     "skm_tuning_table_save": (ctypes.c_int, [ctypes.c_char_p]),

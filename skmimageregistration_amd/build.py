@@ -14,7 +14,7 @@ REPO_DIR = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(PKG_DIR, "csrc")
 BUILD_DIR = os.path.join(PKG_DIR, "_build")
 LIB_PATH = os.path.join(PKG_DIR, "libskm.so")
-SOURCES = ["skm_kernels.hip", "skm_lcc.hip", "skm_tensor.hip", "skm_setup.hip", "skm_capi.hip", "skm_register.hip",
+SOURCES = ["skm_kernels.hip", "skm_kmatrix.hip", "skm_lcc.hip", "skm_tensor.hip", "skm_setup.hip", "skm_capi.hip", "skm_register.hip",
            "skm_warp.hip", "skm_warmstart.hip", "skm_io.cpp"]
 ARCH = os.environ.get("SKM_OFFLOAD_ARCH", "gfx950")
 

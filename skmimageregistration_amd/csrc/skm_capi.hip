@@ -376,11 +376,13 @@ struct skm_evaluator {
     double rd_scaling = 1.0, pg_scaling = 1.0;
     double bl_sigma_s = 1.0, bl_sigma_i = 1.0;
     float *d_cp_intensity = nullptr;       // fixed image at the control points (bilateral term)
+    KernelMatrix km;                       // assembled k(x_m, p_j) (subsample 1; see use_kernel_matrix)
 
     ~skm_evaluator()
     {
         // work queued on the stream may still read these buffers
         if (stream) (void)hipStreamSynchronize(stream);
+        kernel_matrix_release(km);
         void *ptrs[] = {d_fixed, d_coeffs, d_weight, d_df, d_cpc, d_cp_state, d_cp_pstate, d_smp_pstate,
                         d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_reach, d_cp_window,
                         d_reg_window,
@@ -708,6 +710,49 @@ skm_status ensure_lcc_buffers(skm_evaluator *ev)
     return SKM_SUCCESS;
 }
 
+// Largest kernel matrix an evaluator may assemble: SKM_KERNEL_MATRIX_MAX_BYTES,
+// else a quarter of the device's memory or half of its free memory.
+long long kernel_matrix_limit()
+{
+    if (const char *env = std::getenv("SKM_KERNEL_MATRIX_MAX_BYTES")) {
+        char *end = nullptr;
+        const long long v = std::strtoll(env, &end, 10);
+        if (end && end != env && v >= 0) return v;
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+    return (long long)std::min(total_b / 4, free_b / 2);
+}
+
+// Assembles the kernel matrix now (on the evaluator's stream, timed as pass 10).
+skm_status assemble_kernel_matrix(skm_evaluator *ev)
+{
+    PassTimer t(ev, 10);
+    return kernel_matrix_build(ev->pr, ev->km, ev->stream, kernel_matrix_limit());
+}
+
+// Whether this MSE / NCC evaluation reads the kernel matrix, assembling it
+// first unless skm_tuning.kernel_matrix is 1 (never), it was refused, or the
+// stream is being captured into a graph (the assembly waits for its own
+// counts; the captured evaluation then runs on the fly).  A matrix whose gather
+// slab no longer matches the tuning is assembled again.
+skm_status use_kernel_matrix(skm_evaluator *ev, bool *use)
+{
+    *use = false;
+    if (ev->tuning.kernel_matrix == 1 || !kernel_matrix_applies(ev->pr) || ev->km.state == 2) return SKM_SUCCESS;
+    const int slab = ev->pr.dim == 3 ? (ev->pr.gather_split == 4 ? 4 : 8) : std::max(ev->pr.fixed.n[1], 1);
+    if (ev->km.state == 1 && ev->km.slab == slab) {
+        *use = true;
+        return SKM_SUCCESS;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    SKM_HIP(hipStreamIsCapturing(ev->stream, &cap));
+    if (cap != hipStreamCaptureStatusNone) return SKM_SUCCESS;
+    SKM_TRY(assemble_kernel_matrix(ev));
+    *use = ev->km.state == 1;
+    return SKM_SUCCESS;
+}
+
 // One evaluation on the device: value (+ derivative) of the measure and the
 // regularizers, everything left in d_sc / deriv (GpuEvaluator.cu:94-357, 361-714).
 skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *deriv)
@@ -723,14 +768,24 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
     }
     int blocks = 0;
     if (measure == SKM_MEASURE_MSE || measure == SKM_MEASURE_NCC) {
-        { PassTimer t(ev, 1); launch_value(pr, measure, ev->d_sg, ev->d_sb, ev->d_partials, &blocks, s); }
+        bool km = false;
+        SKM_TRY(use_kernel_matrix(ev, &km));
+        {
+            PassTimer t(ev, 1);
+            if (km) launch_value_km(pr, ev->km, measure, ev->d_sg, ev->d_sb, ev->d_partials, &blocks, s);
+            else launch_value(pr, measure, ev->d_sg, ev->d_sb, ev->d_partials, &blocks, s);
+        }
         {
             PassTimer t(ev, 2);
             launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s, ev->d_rng, drawn, ev->rng_inc);
         }
         if (deriv) {
             { PassTimer t(ev, 3); launch_prepare(pr, measure, ev->d_sg, ev->d_sb, ev->d_sc, ev->d_sv, s); }
-            { PassTimer t(ev, 4); launch_gather(pr, ev->d_sv, deriv, s); }
+            {
+                PassTimer t(ev, 4);
+                if (km) launch_gather_km(pr, ev->km, ev->d_sv, deriv, s);
+                else launch_gather(pr, ev->d_sv, deriv, s);
+            }
         }
     } else if (measure == SKM_MEASURE_LCC) {
         SKM_TRY(ensure_lcc_buffers(ev));
@@ -1491,6 +1546,8 @@ skm_status validate_tuning(const skm_tuning &t)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "gather_regions must be 0 (automatic), 1 or 8");
     if (t.lcc_ref_walk < 0 || t.lcc_ref_walk > 2)
         return fail(SKM_ERROR_INVALID_ARGUMENT, "lcc_ref_walk must be 0 (automatic), 1 or 2");
+    if (t.kernel_matrix < 0 || t.kernel_matrix > 1)
+        return fail(SKM_ERROR_INVALID_ARGUMENT, "kernel_matrix must be 0 (automatic) or 1 (never)");
     return SKM_SUCCESS;
 }
 
@@ -1501,6 +1558,9 @@ SKM_API skm_status skm_set_tuning(skm_evaluator *ev, const skm_tuning *tuning)
     if (!ev || !tuning) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
     SKM_TRY(validate_tuning(*tuning));
     SKM_HIP(hipStreamSynchronize(ev->stream));  // queued work may use the buffers resized below
+    // a new policy, or a new gather slab (the gather layout follows the slabs), drops the kernel matrix
+    if (tuning->kernel_matrix != ev->tuning.kernel_matrix || tuning->gather_split != ev->tuning.gather_split)
+        kernel_matrix_release(ev->km);
     ev->tuning = *tuning;
     ev->pr.lcc_chunks = tuning->lcc_ref_chunks;
     ev->pr.gather_split = tuning->gather_split;
@@ -1540,6 +1600,8 @@ SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float 
         if (lcc_ref_table_applies(tp)) axes.push_back({&skm_tuning::lcc_ref_walk, {0, 1, 2}});
         axes.push_back({&skm_tuning::lcc_ref_chunks, {0, 1, 2, 4, 8, 16, 32, 64}});
     } else {
+        if (measure != SKM_MEASURE_LCC && kernel_matrix_applies(ev->pr))
+            axes.push_back({&skm_tuning::kernel_matrix, {0, 1}});
         if (measure != SKM_MEASURE_LCC) axes.push_back({&skm_tuning::value_split, {0, 1, 4, 8, 16}});
         axes.push_back({&skm_tuning::gather_split, {0, 4}});
         if (d3) axes.push_back({&skm_tuning::gather_regions, {0, 8}});
@@ -1558,7 +1620,8 @@ SKM_API skm_status skm_autotune(skm_evaluator *ev, int32_t measure, const float 
     launch_pack_cp(ev->pr, d_cp_aos, d_cpw, 0, ev->d_cpc, s);
     auto time_of = [&](const skm_tuning &t, double *out) -> skm_status {
         SKM_TRY(skm_set_tuning(ev, &t));
-        SKM_TRY(run_eval(ev, measure, 0, ev->d_deriv));  // first use of the shape (allocations), untimed
+        // first use of the shape (allocations, the kernel matrix), untimed
+        SKM_TRY(run_eval(ev, measure, 0, ev->d_deriv));
         std::vector<double> ms;
         for (int r = 0; r < repeats; ++r) {
             SKM_HIP(hipEventRecord(a, s));
@@ -1672,6 +1735,28 @@ SKM_API skm_status skm_get_cache_info(uint64_t *idle_bytes, uint64_t *live_bytes
 SKM_API skm_status skm_release_cached_memory(void)
 {
     drop_idle(-1);
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_get_kernel_matrix_info(skm_evaluator *ev, skm_kernel_matrix_info *info)
+{
+    if (!ev || !info) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    *info = skm_kernel_matrix_info{};
+    info->state = ev->km.state;
+    info->bytes = (uint64_t)ev->km.bytes;
+    info->pairs_value = (uint64_t)ev->km.nnz_v;
+    info->pairs_gather = (uint64_t)ev->km.nnz_g;
+    info->read_value = (uint64_t)ev->km.read_v;
+    info->read_gather = (uint64_t)ev->km.read_g;
+    info->build_ms = ev->km.build_ms;
+    return SKM_SUCCESS;
+}
+
+SKM_API skm_status skm_release_kernel_matrix(skm_evaluator *ev)
+{
+    if (!ev) return fail(SKM_ERROR_INVALID_ARGUMENT, "null argument");
+    SKM_HIP(hipStreamSynchronize(ev->stream));  // queued evaluations may still read it
+    kernel_matrix_release(ev->km);
     return SKM_SUCCESS;
 }
 

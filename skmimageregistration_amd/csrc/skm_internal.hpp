@@ -88,6 +88,8 @@ void launch_reduce(const Problem &pr, int measure, const double *partials, int b
 void launch_prepare(const Problem &pr, int measure, const float4 *sg, const float *sb, const Scalars *sc,
                     float4 *sv, hipStream_t s);
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s);
+// deriv_j = w_j * the sum of j's (slab, tile) partials in slab order
+void launch_gather_finish(const Problem &pr, float *deriv_soa, hipStream_t s);
 // doubles of Problem::gather_partials for a sample grid with `planes` cells along the last axis
 long long gather_partials_size(const Problem &pr, long long planes);
 // The bilateral term (no reference counterpart): spatial Gaussian of standard
@@ -108,6 +110,56 @@ unsigned reg_blocks(long long n_cp);  // workgroups (and 4-double partial slots)
 void launch_displacement(const Problem &pr, float *dst, hipStream_t s);
 // packed states of the weighted modes at every voxel of the dense fixed grid
 void launch_voxel_states(const Problem &pr, float4 *out, hipStream_t s);
+
+// The kernel matrix of an evaluator whose samples do not move between
+// evaluations (subsample 1: sample m is voxel m): every non-zero k(x_m, p_j)
+// that the value pass and the derivative gather use, assembled once in device
+// memory (skm_kmatrix.hip).  k depends on the sample and control-point
+// positions and on the weight image / tensor field only, none of which changes
+// over an evaluator's life; the coefficients c_j do.  Each evaluation then reads
+// the stored values in place of evaluating the kernel twice per pair.
+//   value layout: per sample brick (one wave of the value pass), the control
+//     points of the walk (in its order) with a pair passing the window and
+//     reach tests, in groups of four: one record {lane mask lo, lane mask hi,
+//     point 0, point 1}, {point 2, point 3, 0, 0} per group, followed in v_val
+//     by one float4 per lane of the mask: the lane's k with the four points (0
+//     where a pair is zero or outside the lane's window);
+//   gather layout: per wave of a (slab, control-point tile) workgroup of the
+//     derivative gather, one record {lane mask lo, lane mask hi, first sample, 0}
+//     per sample block with a non-zero pair, followed in g_val by one float4 per
+//     lane of the mask: k of the four tile points (0 where a pair is zero or
+//     outside the point's window).
+struct KernelMatrix {
+    int state = 0;  // 0 not built, 1 built, 2 refused (larger than the limit); see kernel_matrix_build
+    int slab = 0;   // gather slab of the gather layout
+    long long units_v = 0, units_g = 0;
+    long long cap_ent_v = 0, cap_val_v = 0, cap_ent_g = 0, cap_val_g = 0;  // allocated records and float4 slots
+    long long nnz_v = 0, nnz_g = 0;    // non-zero pairs stored
+    long long read_v = 0, read_g = 0;  // bytes of each layout an evaluation reads (records and slots)
+    long long bytes = 0;
+    double build_ms = 0.0;
+    uint4 *v_ent = nullptr;  // two per record
+    float4 *v_val = nullptr;
+    long long *v_ent_off = nullptr, *v_val_off = nullptr;
+    int *v_cnt = nullptr;
+    uint4 *g_ent = nullptr;
+    float4 *g_val = nullptr;
+    long long *g_ent_off = nullptr, *g_val_off = nullptr;
+    int *g_cnt = nullptr;
+};
+// whether the evaluator's problem has a kernel matrix (subsample 1, sizes within 32-bit indices)
+bool kernel_matrix_applies(const Problem &pr);
+// Assembles km on stream s (returns when done).  Refuses (state 2, SKM_SUCCESS)
+// when its device memory would exceed max_bytes; errors leave state 0.
+skm_status kernel_matrix_build(const Problem &pr, KernelMatrix &km, hipStream_t s, long long max_bytes);
+// frees the buffers (the caller has synchronised the work that reads them)
+void kernel_matrix_release(KernelMatrix &km);
+// value pass and derivative gather from the kernel matrix (same outputs as
+// launch_value / launch_gather)
+void launch_value_km(const Problem &pr, const KernelMatrix &km, int measure, float4 *sg, float *sb,
+                     double *partials, int *blocks_out, hipStream_t s);
+void launch_gather_km(const Problem &pr, const KernelMatrix &km, const float4 *sv, float *deriv_soa,
+                      hipStream_t s);
 
 // skm_create with, when device_data, device pointers in the descriptor (skm_capi.hip)
 skm_status create_evaluator(const skm_desc *desc, bool device_data, skm_evaluator **out,

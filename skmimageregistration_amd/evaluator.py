@@ -286,11 +286,23 @@ class GpuKernelEvaluator:
     def synchronize(self):
         check(self._lib.skm_synchronize(self._h))
 
-    def set_tuning(self, lcc_ref_chunks=0, gather_split=0, value_split=0, gather_regions=0, lcc_ref_walk=0):
-        """Launch shapes (skm_set_tuning); 0 keeps the library's choice."""
+    def set_tuning(self, lcc_ref_chunks=0, gather_split=0, value_split=0, gather_regions=0, lcc_ref_walk=0,
+                   kernel_matrix=0):
+        """Launch shapes (skm_set_tuning); 0 keeps the library's choice.
+        kernel_matrix=1 evaluates the kernels on the fly at subsample 1."""
         t = _capi.SkmTuning(int(lcc_ref_chunks), int(gather_split), int(value_split), int(gather_regions),
-                            int(lcc_ref_walk))
+                            int(lcc_ref_walk), int(kernel_matrix))
         check(self._lib.skm_set_tuning(self._h, ctypes.byref(t)))
+
+    def kernel_matrix_info(self):
+        """skm_get_kernel_matrix_info as a dict (state 0 none, 1 assembled, 2 refused)."""
+        i = _capi.SkmKernelMatrixInfo()
+        check(self._lib.skm_get_kernel_matrix_info(self._h, ctypes.byref(i)))
+        return dict(state=i.state, bytes=i.bytes, pairs_value=i.pairs_value, pairs_gather=i.pairs_gather,
+                    read_value=i.read_value, read_gather=i.read_gather, build_ms=i.build_ms)
+
+    def release_kernel_matrix(self):
+        check(self._lib.skm_release_kernel_matrix(self._h))
 
     def autotune_device(self, metric, d_cp_aos, d_cpw, repeats=3):
         """skm_autotune at device control points (ints); returns the report as a dict."""
@@ -300,13 +312,13 @@ class GpuKernelEvaluator:
         b = r.best
         return {"lcc_ref_chunks": b.lcc_ref_chunks, "gather_split": b.gather_split, "value_split": b.value_split,
                 "gather_regions": b.gather_regions, "lcc_ref_walk": b.lcc_ref_walk,
-                "best_ms": r.best_ms, "first_ms": r.first_ms, "candidates": r.candidates}
+                "kernel_matrix": b.kernel_matrix, "best_ms": r.best_ms, "first_ms": r.first_ms, "candidates": r.candidates}
 
     def get_tuning(self):
         t = _capi.SkmTuning()
         check(self._lib.skm_get_tuning(self._h, ctypes.byref(t)))
         return dict(lcc_ref_chunks=t.lcc_ref_chunks, gather_split=t.gather_split, value_split=t.value_split,
-                    gather_regions=t.gather_regions, lcc_ref_walk=t.lcc_ref_walk)
+                    gather_regions=t.gather_regions, lcc_ref_walk=t.lcc_ref_walk, kernel_matrix=t.kernel_matrix)
 
     def cube_offsets(self):
         n = ctypes.c_int64()
