@@ -1673,10 +1673,11 @@ SKM_API skm_status skm_tuning_table_save(const char *path)
     if (!f) return fail(SKM_ERROR_IO, std::string("cannot write ") + path);
     std::lock_guard<std::mutex> lock(g_tuning_mutex);
     f << "# skm launch shapes: shape class, lcc_ref_chunks, gather_split, value_split, gather_regions, "
-         "lcc_ref_walk\n";
+         "lcc_ref_walk, kernel_matrix\n";
     for (const auto &kv : g_tuning)
         f << kv.first << " " << kv.second.lcc_ref_chunks << " " << kv.second.gather_split << " "
-          << kv.second.value_split << " " << kv.second.gather_regions << " " << kv.second.lcc_ref_walk << "\n";
+          << kv.second.value_split << " " << kv.second.gather_regions << " " << kv.second.lcc_ref_walk << " "
+          << kv.second.kernel_matrix << "\n";
     return f ? SKM_SUCCESS : fail(SKM_ERROR_IO, std::string("cannot write ") + path);
 }
 
@@ -1695,6 +1696,8 @@ SKM_API skm_status skm_tuning_table_load(const char *path)
         if (!(in >> key >> t.lcc_ref_chunks >> t.gather_split >> t.value_split >> t.gather_regions >>
               t.lcc_ref_walk))
             return fail(SKM_ERROR_IO, "malformed tuning line: " + line);
+        // kernel_matrix: an optional last column (tables written before it hold five)
+        if (!(in >> t.kernel_matrix)) t.kernel_matrix = 0;
         if (validate_tuning(t) != SKM_SUCCESS) return fail(SKM_ERROR_IO, "launch shape out of range: " + line);
         read[key] = t;
     }

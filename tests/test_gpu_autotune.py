@@ -50,7 +50,7 @@ def test_autotuned_shapes_keep_results(cid, ckw, metric):
     print(f"{cid}: {rep}", flush=True)
     assert rep["candidates"] >= 2 and rep["best_ms"] <= rep["first_ms"] and rep["best_ms"] > 0
     tuned = ev.get_tuning()
-    keys = ("lcc_ref_chunks", "gather_split", "value_split", "gather_regions", "lcc_ref_walk")
+    keys = ("lcc_ref_chunks", "gather_split", "value_split", "gather_regions", "lcc_ref_walk", "kernel_matrix")
     assert {k: tuned[k] for k in keys} == {k: rep[k] for k in keys}
     res, der = ev.getValueAndDerivative(metric, case.cp, case.cpw, do_resampling=False)
     assert res.numPixelsCountedGpu == base_res.numPixelsCountedGpu == ores["pixels_counted"]
@@ -81,7 +81,7 @@ def test_table_save_and_load(tmp_path):
     tuning_table_load(path)
     ev = case.gpu()
     assert ev.get_tuning() == dict(lcc_ref_chunks=0, gather_split=4, value_split=8, gather_regions=1,
-                                   lcc_ref_walk=0)
+                                   lcc_ref_walk=0, kernel_matrix=0)
     ev.close()
     bad = tmp_path / "bad.txt"
     bad.write_text(lines[0].split()[0] + " 0 5 8 1 0\n")
