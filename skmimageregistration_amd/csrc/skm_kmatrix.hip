@@ -34,7 +34,7 @@ namespace {
 // with empty records (mask 0) to a whole number of batches, so that a batch's
 // loads are unconditional.  Values are allocated with kEntryPad to spare (lanes
 // without a pair read their unit's first value).
-constexpr int kValueBatch = 8, kGatherBatch = 4;
+constexpr int kValueBatch = 8, kGatherBatch = 2;
 constexpr int kEntryPad = 16;
 
 __host__ __device__ constexpr int round_up(int n, int b) { return (n + b - 1) / b * b; }
@@ -344,52 +344,89 @@ template <int DIM> struct GBlock;
 template <> struct GBlock<2> { static constexpr int X = 8, Y = 8, Z = 1; };
 template <> struct GBlock<3> { static constexpr int X = 4, Y = 4, Z = 4; };
 
-// Tile control points of a gather workgroup (CpTile), as k_gather_tile loads them.
-template <int DIM, int MODE>
-struct TilePoints {
-    static constexpr int T = CpTile<DIM>::X * CpTile<DIM>::Y * CpTile<DIM>::Z;
-    bool ok[T];
-    float px[T], py[T], pz[T], lam[T];
-    int lo[T][3], hi[T][3];
-    PState st[T];
+// A gather workgroup of the kernel-matrix form takes a 2 x 2 group of the
+// derivative gather's control-point tiles (CpTile: 4 points each), 16 points:
+// each stored sample block then serves all 16 with one load of its samples' v.
+constexpr int kGroupTiles = 4;
+
+struct GroupMap {
+    int ntx, nty, ngx, ngy, n_groups, n_tiles;
 };
 
-// Workgroup -> (slab, tile) of the derivative gather (slab-major, no XCD
-// regions) and the tile's sample planes; false when the slab lies outside
-// every window of the tile (k_gather_tile writes no partials there either).
 template <int DIM>
-__device__ __forceinline__ bool gather_unit(const Problem &pr, int n_tiles, int &tile, int &slab_id, int &t0,
-                                            int &t1, int &t2)
+__host__ __device__ __forceinline__ GroupMap group_map(const Problem &pr)
 {
-    constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y;
-    const int ntx = (pr.cp.n[0] + TX - 1) / TX, nty = (pr.cp.n[1] + TY - 1) / TY;
-    tile = (int)(blockIdx.x % (unsigned)n_tiles);
-    slab_id = (int)(blockIdx.x / (unsigned)n_tiles);
-    t0 = tile % ntx;
-    t1 = (tile / ntx) % nty;
-    t2 = tile / (ntx * nty);
+    GroupMap g;
+    g.ntx = (pr.cp.n[0] + CpTile<DIM>::X - 1) / CpTile<DIM>::X;
+    g.nty = (pr.cp.n[1] + CpTile<DIM>::Y - 1) / CpTile<DIM>::Y;
+    const int ntz = DIM == 3 ? (pr.cp.n[2] + CpTile<DIM>::Z - 1) / CpTile<DIM>::Z : 1;
+    g.ngx = (g.ntx + 1) / 2;
+    g.ngy = (g.nty + 1) / 2;
+    g.n_groups = g.ngx * g.ngy * ntz;
+    g.n_tiles = g.ntx * g.nty * ntz;
+    return g;
+}
+
+// Workgroup -> (slab, group of tiles), slab-major.  tile[s] is the s-th tile
+// of the group (x, y offsets s & 1, s >> 1) or -1 past the grid; false when
+// the slab lies outside the windows of every tile of the group (k_gather_tile
+// writes no partials for such a slab of a tile either).
+template <int DIM>
+__device__ __forceinline__ bool group_unit(const Problem &pr, const GroupMap &g, int &slab_id, int *tile, int *tc)
+{
+    const int grp = (int)(blockIdx.x % (unsigned)g.n_groups);
+    slab_id = (int)(blockIdx.x / (unsigned)g.n_groups);
+    const int gx = grp % g.ngx, gy = (grp / g.ngx) % g.ngy, gz = grp / (g.ngx * g.ngy);
     const int slab = gather_slab<DIM>(pr);
-    int plo, phi;
-    tile_plane_range<DIM>(pr, t0, t1, t2, plo, phi);
-    return !(plo > phi || slab_id < plo / slab || slab_id > phi / slab);
+    bool live = false;
+#pragma unroll
+    for (int s = 0; s < kGroupTiles; ++s) {
+        const int t0 = 2 * gx + (s & 1), t1 = 2 * gy + (s >> 1);
+        tile[s] = -1;
+        tc[3 * s] = t0;
+        tc[3 * s + 1] = t1;
+        tc[3 * s + 2] = gz;
+        if (t0 >= g.ntx || t1 >= g.nty) continue;
+        tile[s] = t0 + g.ntx * (t1 + g.nty * gz);
+        int plo, phi;
+        tile_plane_range<DIM>(pr, t0, t1, gz, plo, phi);
+        live = live || !(plo > phi || slab_id < plo / slab || slab_id > phi / slab);
+    }
+    return live;
+}
+
+// Control point c (0-3) of tile (t0, t1, t2); false past the grid.
+template <int DIM>
+__device__ __forceinline__ bool tile_point(const Problem &pr, int t0, int t1, int t2, int c, long long &j, int *jc)
+{
+    constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z;
+    jc[0] = t0 * TX + c % TX;
+    jc[1] = t1 * TY + (c / TX) % TY;
+    jc[2] = DIM == 3 ? t2 * TZ + c / (TX * TY) : 0;
+    const bool ok = jc[0] < pr.cp.n[0] && jc[1] < pr.cp.n[1] && (DIM == 2 || jc[2] < pr.cp.n[2]);
+    j = ok ? jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]) : 0;
+    return ok;
 }
 
 // Walk of the derivative gather over aligned sample blocks of the union of
-// the tile's windows inside the slab (evaluateDerivativeMSE / NCC,
+// the group's windows inside the slab (evaluateDerivativeMSE / NCC,
 // GpuEvaluatorMSE.cu:134-162, GpuEvaluatorNCC.cu:145-176: a pair counts when
 // the sample lies in the control point's window and, here, within the pair's
-// reach).  sstate: packed states of the samples (weighted modes).
+// reach).  Per block: one lane mask per tile (its four points' non-zero pairs)
+// and per lane of a tile's mask the float4 of its four points' k.  The tile
+// points' positions, windows and states are read per block (scalar cache).
+// sstate: packed states of the samples (weighted modes).
 template <int DIM, int MODE, bool FILL>
-__global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, int n_tiles,
+__global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, GroupMap gm,
                                                               const float4 *__restrict__ sstate, BuildArgs ba)
 {
-    constexpr int TX = CpTile<DIM>::X, TY = CpTile<DIM>::Y, TZ = CpTile<DIM>::Z, T = TX * TY * TZ;
+    constexpr int T = CpTile<DIM>::X * CpTile<DIM>::Y * CpTile<DIM>::Z;
     constexpr int BX = GBlock<DIM>::X, BY = GBlock<DIM>::Y, BZ = GBlock<DIM>::Z;
-    static_assert(T == 4, "the gather layout holds the four tile points' k in one float4 per lane");
+    static_assert(T == 4, "one float4 of k per lane, tile and block");
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const long long unit = (long long)blockIdx.x * (kThreads / 64) + wave;
-    int tile, slab_id, t0, t1, t2;
-    const bool live = gather_unit<DIM>(pr, n_tiles, tile, slab_id, t0, t1, t2);
+    int slab_id, tile[kGroupTiles], tc[3 * kGroupTiles];
+    const bool live = group_unit<DIM>(pr, gm, slab_id, tile, tc);
     int ne = 0;
     long long nv = 0, npairs = 0;
     bool full = false;
@@ -402,31 +439,17 @@ __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, int n_
     }
     if (live) {
         const int slab = gather_slab<DIM>(pr);
-        TilePoints<DIM, MODE> tp;
         int ulo[3] = {INT_MAX, INT_MAX, DIM == 3 ? INT_MAX : 0}, uhi[3] = {INT_MIN, INT_MIN, DIM == 3 ? INT_MIN : 0};
-#pragma unroll
-        for (int c = 0; c < T; ++c) {
-            const int jc[3] = {t0 * TX + c % TX, t1 * TY + (c / TX) % TY, t2 * TZ + c / (TX * TY)};
-            tp.ok[c] = jc[0] < pr.cp.n[0] && jc[1] < pr.cp.n[1] && (DIM == 2 || jc[2] < pr.cp.n[2]);
-            const long long j = tp.ok[c] ? jc[0] + (long long)pr.cp.n[0] * (jc[1] + (long long)pr.cp.n[1] * jc[2]) : 0;
-            const float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]};
-            float p[3] = {0.0f, 0.0f, 0.0f};
-            to_global<DIM>(pr.cp, fj, p);
-            tp.px[c] = p[0];
-            tp.py[c] = p[1];
-            tp.pz[c] = p[2];
-            tp.lam[c] = is_stationary(MODE) ? 1.0f : cp_state_load<DIM, MODE>(pr, j, tp.st[c]);
-            const int *win = pr.cp_window + j * 6;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                tp.lo[c][d] = d < DIM ? win[d] : 0;
-                tp.hi[c][d] = d < DIM ? win[d + 3] : 0;
-            }
-            if (tp.ok[c]) {
+        for (int s = 0; s < kGroupTiles; ++s) {
+            if (tile[s] < 0) continue;
+            for (int c = 0; c < T; ++c) {
+                long long j;
+                int jc[3];
+                if (!tile_point<DIM>(pr, tc[3 * s], tc[3 * s + 1], tc[3 * s + 2], c, j, jc)) continue;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
-                    ulo[d] = min(ulo[d], tp.lo[c][d]);
-                    uhi[d] = max(uhi[d], tp.hi[c][d]);
+                    ulo[d] = min(ulo[d], pr.cp_window[j * 6 + d]);
+                    uhi[d] = max(uhi[d], pr.cp_window[j * 6 + 3 + d]);
                 }
             }
         }
@@ -454,53 +477,72 @@ __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, int n_
             const float fpos[3] = {(float)mm[0], (float)mm[1], (float)mm[2]};
             float xs[3];
             to_global<DIM>(pr.fixed, fpos, xs);
-            bool pass[T];
-            unsigned long long pm_any = 0;
-#pragma unroll
-            for (int c = 0; c < T; ++c) {
-                const float v0 = tp.px[c] - xs[0], v1 = tp.py[c] - xs[1], v2 = DIM == 3 ? tp.pz[c] - xs[2] : 0.0f;
-                const float d2 = v0 * v0 + v1 * v1 + (DIM == 3 ? v2 * v2 : 0.0f);
-                bool win = inside && tp.ok[c];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) win = win && mm[d] >= tp.lo[c][d] && mm[d] <= tp.hi[c][d];
-                pass[c] = win && d2 < smin2h * (tp.lam[c] + lam_s);
-                pm_any |= __ballot(pass[c]);
-            }
-            if (!pm_any) continue;
-            if (!FILL) {
-                ++ne;
-                nv += __popcll(pm_any);
-                continue;
-            }
-            float k[T];
-            unsigned long long any = 0;
+            unsigned long long sm[kGroupTiles];
+            float k[kGroupTiles][T];
             int pairs = 0;
 #pragma unroll
-            for (int c = 0; c < T; ++c) {
-                k[c] = 0.0f;
-                if (pass[c]) {
-                    const float v[3] = {tp.px[c] - xs[0], tp.py[c] - xs[1], DIM == 3 ? tp.pz[c] - xs[2] : 0.0f};
-                    k[c] = kernel_pair<DIM, MODE>(pr.kc, v, tp.st[c], sst);
+            for (int s = 0; s < kGroupTiles; ++s) {
+                sm[s] = 0;
+#pragma unroll
+                for (int c = 0; c < T; ++c) {
+                    k[s][c] = 0.0f;
+                    long long j;
+                    int jc[3];
+                    if (tile[s] < 0 || !tile_point<DIM>(pr, tc[3 * s], tc[3 * s + 1], tc[3 * s + 2], c, j, jc))
+                        continue;
+                    const float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]};
+                    float p[3] = {0.0f, 0.0f, 0.0f};
+                    to_global<DIM>(pr.cp, fj, p);
+                    PState cst;
+                    const float lam_c = is_stationary(MODE) ? 1.0f : cp_state_load<DIM, MODE>(pr, j, cst);
+                    const float v[3] = {p[0] - xs[0], p[1] - xs[1], DIM == 3 ? p[2] - xs[2] : 0.0f};
+                    const float d2 = v[0] * v[0] + v[1] * v[1] + (DIM == 3 ? v[2] * v[2] : 0.0f);
+                    bool pass = inside;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d)
+                        pass = pass && mm[d] >= pr.cp_window[j * 6 + d] && mm[d] <= pr.cp_window[j * 6 + 3 + d];
+                    pass = pass && d2 < smin2h * (lam_c + lam_s);
+                    if (!FILL) {
+                        sm[s] |= __ballot(pass);
+                        continue;
+                    }
+                    if (!__ballot(pass)) continue;
+                    float kv = 0.0f;
+                    if (pass) kv = kernel_pair<DIM, MODE>(pr.kc, v, cst, sst);
+                    const unsigned long long km = __ballot(pass && kv != 0.0f);
+                    k[s][c] = lane_bit(km, lane) ? kv : 0.0f;
+                    sm[s] |= km;
+                    pairs += __popcll(km);
                 }
-                const unsigned long long km = __ballot(pass[c] && k[c] != 0.0f);
-                if (!lane_bit(km, lane)) k[c] = 0.0f;
-                any |= km;
-                pairs += __popcll(km);
             }
-            if (!any) continue;
-            const int slots = __popcll(any);
-            if (ne >= cap_e || nv + slots > cap_v) {
-                full = true;
-                break;
+            int slots = 0;
+#pragma unroll
+            for (int s = 0; s < kGroupTiles; ++s) slots += __popcll(sm[s]);
+            if (!slots) continue;
+            if (FILL) {
+                if (ne >= cap_e || nv + slots > cap_v) {
+                    full = true;
+                    break;
+                }
+                if (lane == 0) {
+                    const int org = (int)(bo[0] + (long long)pr.nsub[0] * (bo[1] + (long long)pr.nsub[1] * bo[2]));
+                    ba.ent[3 * (ebase + ne)] = make_uint4((unsigned)sm[0], (unsigned)(sm[0] >> 32), (unsigned)sm[1],
+                                                          (unsigned)(sm[1] >> 32));
+                    ba.ent[3 * (ebase + ne) + 1] = make_uint4((unsigned)sm[2], (unsigned)(sm[2] >> 32),
+                                                              (unsigned)sm[3], (unsigned)(sm[3] >> 32));
+                    ba.ent[3 * (ebase + ne) + 2] = make_uint4((unsigned)org, 0u, 0u, 0u);
+                }
+                long long vb = vbase + nv;
+#pragma unroll
+                for (int s = 0; s < kGroupTiles; ++s) {
+                    if (lane_bit(sm[s], lane))
+                        ba.val[vb + lane_rank(sm[s])] = make_float4(k[s][0], k[s][1], k[s][2], k[s][3]);
+                    vb += __popcll(sm[s]);
+                }
+                npairs += pairs;
             }
-            if (lane == 0) {
-                const int org = (int)(bo[0] + (long long)pr.nsub[0] * (bo[1] + (long long)pr.nsub[1] * bo[2]));
-                ba.ent[ebase + ne] = make_uint4((unsigned)any, (unsigned)(any >> 32), (unsigned)org, 0u);
-            }
-            if (lane_bit(any, lane)) ba.val[vbase + nv + lane_rank(any)] = make_float4(k[0], k[1], k[2], k[3]);
             ++ne;
             nv += slots;
-            npairs += pairs;
         }
     }
     if (lane != 0) return;
@@ -508,7 +550,11 @@ __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, int n_
         ba.cap_ent[unit] = round_up(ne, kGatherBatch);
         ba.cap_val[unit] = (int)nv;
     } else {
-        for (int e = ne; e < round_up(ne, kGatherBatch); ++e) ba.ent[ebase + e] = make_uint4(0u, 0u, 0u, 0u);
+        for (int e = ne; e < round_up(ne, kGatherBatch); ++e) {
+            ba.ent[3 * (ebase + e)] = make_uint4(0u, 0u, 0u, 0u);
+            ba.ent[3 * (ebase + e) + 1] = make_uint4(0u, 0u, 0u, 0u);
+            ba.ent[3 * (ebase + e) + 2] = make_uint4(0u, 0u, 0u, 0u);
+        }
         ba.cnt[unit] = round_up(ne, kGatherBatch);
         ba.nnz[unit] = (int)npairs;
         ba.cap_val[unit] = (int)nv;  // the slots stored (the capacity was read above)
@@ -517,70 +563,107 @@ __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, int n_
 }
 
 // Derivative gather from the gather layout: per stored block every lane with
-// a pair loads its sample's v and the four tile points' k (one float4 each)
-// and adds k v to the four sums; entries are wave-uniform (scalar loads),
-// taken four at a time so that their loads are in flight together.  The
-// workgroup's sums then go through gather_tile_store, the same FP64 partials
-// that k_gather_finish adds in slab order.
+// a pair loads its sample's v once and, per tile whose mask holds it, the
+// float4 of that tile's four points' k, and adds k v to their sums; records
+// are wave-uniform (scalar loads), two per batch, the next batch's loaded
+// before this batch's sums.  The sums then go to the FP64 (slab, tile)
+// partials that k_gather_finish adds in slab order (fixed butterfly per wave,
+// the four waves in FP64), for every tile of the group.
 template <int DIM>
-__global__ __launch_bounds__(kThreads) void k_km_gather(Problem pr, const float4 *__restrict__ sv, int n_tiles,
+__global__ __launch_bounds__(kThreads) void k_km_gather(Problem pr, const float4 *__restrict__ sv, GroupMap gm,
                                                         const uint4 *__restrict__ ent,
                                                         const float4 *__restrict__ val,
                                                         const long long *__restrict__ ent_off,
                                                         const long long *__restrict__ val_off,
                                                         const int *__restrict__ cnt)
 {
-    constexpr int T = CpTile<DIM>::X * CpTile<DIM>::Y * CpTile<DIM>::Z;
+    constexpr int T = CpTile<DIM>::X * CpTile<DIM>::Y * CpTile<DIM>::Z, G = kGroupTiles;
     constexpr int BX = GBlock<DIM>::X, BY = GBlock<DIM>::Y;
     constexpr int B = kGatherBatch;
-    static_assert(T == 4, "one float4 of k per lane and entry");
-    __shared__ double lds[T * 3 * 4];
-    int tile, slab_id, t0, t1, t2;
-    if (!gather_unit<DIM>(pr, n_tiles, tile, slab_id, t0, t1, t2)) return;
+    static_assert(T == 4, "one float4 of k per lane, tile and block");
+    __shared__ double lds[G * T * 3 * 4];
+    int slab_id, tile[G], tc[3 * G];
+    if (!group_unit<DIM>(pr, gm, slab_id, tile, tc)) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const long long unit = (long long)blockIdx.x * (kThreads / 64) + wave;
     const int tx = lane % BX, ty = (lane / BX) % BY, tz = lane / (BX * BY);
     const long long loff = tx + (long long)pr.nsub[0] * (ty + (long long)pr.nsub[1] * tz);
-    float acc[T][3];
+    float acc[G][T][3];
 #pragma unroll
-    for (int c = 0; c < T; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0f;
-    const uint4 *E = ent + ent_off[unit];
+    for (int s = 0; s < G; ++s)
+#pragma unroll
+        for (int c = 0; c < T; ++c) acc[s][c][0] = acc[s][c][1] = acc[s][c][2] = 0.0f;
+    const uint4 *E = ent + 3 * ent_off[unit];
     const float4 *V = val + val_off[unit];
     const int ne = cnt[unit];  // a whole number of batches
-    uint4 en[B];
+    uint4 en[B][3];
 #pragma unroll
-    for (int i = 0; i < B; ++i) en[i] = ne > 0 ? E[i] : make_uint4(0u, 0u, 0u, 0u);
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+        for (int w = 0; w < 3; ++w) en[i][w] = ne > 0 ? E[3 * i + w] : make_uint4(0u, 0u, 0u, 0u);
     for (int e0 = 0; e0 < ne; e0 += B) {
         // lanes without a pair read element 0 and use zeros
-        float4 cv[B], kk[B];
+        float4 cv[B], kk[B][G];
         int off = 0;
 #pragma unroll
         for (int i = 0; i < B; ++i) {
-            const unsigned long long any = en[i].x | ((unsigned long long)en[i].y << 32);
-            const bool on = lane_bit(any, lane);
-            const float4 a = sv[on ? (int)en[i].z + loff : 0];
-            const float4 b = V[on ? off + (int)lane_rank(any) : 0];
+            const unsigned long long sm[G] = {en[i][0].x | ((unsigned long long)en[i][0].y << 32),
+                                              en[i][0].z | ((unsigned long long)en[i][0].w << 32),
+                                              en[i][1].x | ((unsigned long long)en[i][1].y << 32),
+                                              en[i][1].z | ((unsigned long long)en[i][1].w << 32)};
+            const bool on = lane_bit(sm[0] | sm[1] | sm[2] | sm[3], lane);
+            const float4 a = sv[on ? (int)en[i][2].x + loff : 0];
             cv[i] = make_float4(on ? a.x : 0.0f, on ? a.y : 0.0f, on ? a.z : 0.0f, 0.0f);
-            kk[i] = make_float4(on ? b.x : 0.0f, on ? b.y : 0.0f, on ? b.z : 0.0f, on ? b.w : 0.0f);
-            off += __popcll(any);
+#pragma unroll
+            for (int s = 0; s < G; ++s) {
+                const bool ons = lane_bit(sm[s], lane);
+                const float4 b = V[ons ? off + (int)lane_rank(sm[s]) : 0];
+                kk[i][s] = make_float4(ons ? b.x : 0.0f, ons ? b.y : 0.0f, ons ? b.z : 0.0f, ons ? b.w : 0.0f);
+                off += __popcll(sm[s]);
+            }
         }
         if (e0 + B < ne) {
 #pragma unroll
-            for (int i = 0; i < B; ++i) en[i] = E[e0 + B + i];
+            for (int i = 0; i < B; ++i)
+#pragma unroll
+                for (int w = 0; w < 3; ++w) en[i][w] = E[3 * (e0 + B + i) + w];
         }
 #pragma unroll
-        for (int i = 0; i < B; ++i) {
-            const float k4[T] = {kk[i].x, kk[i].y, kk[i].z, kk[i].w};
+        for (int i = 0; i < B; ++i)
 #pragma unroll
-            for (int c = 0; c < T; ++c) {
-                acc[c][0] = fmaf(k4[c], cv[i].x, acc[c][0]);
-                acc[c][1] = fmaf(k4[c], cv[i].y, acc[c][1]);
-                if (DIM == 3) acc[c][2] = fmaf(k4[c], cv[i].z, acc[c][2]);
+            for (int s = 0; s < G; ++s) {
+                const float k4[T] = {kk[i][s].x, kk[i][s].y, kk[i][s].z, kk[i][s].w};
+#pragma unroll
+                for (int c = 0; c < T; ++c) {
+                    acc[s][c][0] = fmaf(k4[c], cv[i].x, acc[s][c][0]);
+                    acc[s][c][1] = fmaf(k4[c], cv[i].y, acc[s][c][1]);
+                    if (DIM == 3) acc[s][c][2] = fmaf(k4[c], cv[i].z, acc[s][c][2]);
+                }
             }
-        }
         V += off;
     }
-    gather_tile_store<DIM>(pr, acc, lds, t0, t1, t2, tile, slab_id, n_tiles);
+    // wave sums (fixed butterfly, FP32), then the four waves in FP64
+#pragma unroll
+    for (int s = 0; s < G; ++s)
+#pragma unroll
+        for (int c = 0; c < T; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                float v = acc[s][c][d];
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                if (lane == 0) lds[wave * G * T * 3 + (s * T + c) * 3 + d] = v;
+            }
+    __syncthreads();
+    if (threadIdx.x < G * T * 3) {
+        const int s = threadIdx.x / (T * 3), c = (threadIdx.x / 3) % T, d = threadIdx.x % 3;
+        double sum = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) sum += lds[w * G * T * 3 + threadIdx.x];
+        long long j;
+        int jc[3];
+        if (d < DIM && tile[s] >= 0 && tile_point<DIM>(pr, tc[3 * s], tc[3 * s + 1], tc[3 * s + 2], c, j, jc))
+            pr.gather_partials[((long long)slab_id * gm.n_tiles + tile[s]) * (T * 3) + c * 3 + d] = sum;
+    }
 }
 
 // Exclusive prefix sums of n per-unit capacities (one workgroup, in a fixed
@@ -644,7 +727,7 @@ template <int DIM>
 long long gather_workgroups(const Problem &pr)
 {
     const int slab = gather_slab<DIM>(pr);
-    return gather_tiles<DIM>(pr) * ((pr.nsub[DIM - 1] + slab - 1) / slab);
+    return (long long)group_map<DIM>(pr).n_groups * ((pr.nsub[DIM - 1] + slab - 1) / slab);
 }
 
 template <typename T>
@@ -661,7 +744,7 @@ void build_count(const Problem &pr, const float4 *sstate, const BuildArgs &bv, c
 {
     k_km_value_build<DIM, MODE, false><<<value_grid<DIM>(pr), kThreads, 0, s>>>(pr, bv);
     k_km_gather_build<DIM, MODE, false>
-        <<<(unsigned)gather_workgroups<DIM>(pr), kThreads, 0, s>>>(pr, (int)gather_tiles<DIM>(pr), sstate, bg);
+        <<<(unsigned)gather_workgroups<DIM>(pr), kThreads, 0, s>>>(pr, group_map<DIM>(pr), sstate, bg);
 }
 
 template <int DIM, int MODE>
@@ -669,7 +752,7 @@ void build_fill(const Problem &pr, const float4 *sstate, const BuildArgs &bv, co
 {
     k_km_value_build<DIM, MODE, true><<<value_grid<DIM>(pr), kThreads, 0, s>>>(pr, bv);
     k_km_gather_build<DIM, MODE, true>
-        <<<(unsigned)gather_workgroups<DIM>(pr), kThreads, 0, s>>>(pr, (int)gather_tiles<DIM>(pr), sstate, bg);
+        <<<(unsigned)gather_workgroups<DIM>(pr), kThreads, 0, s>>>(pr, group_map<DIM>(pr), sstate, bg);
 }
 
 #define SKM_KM_DISPATCH(pr, FN, ...)                                                                              \
@@ -760,7 +843,7 @@ skm_status kernel_matrix_build(const Problem &pr, KernelMatrix &km, hipStream_t 
     km.cap_val_v = tot[1];
     km.cap_ent_g = tot[2];
     km.cap_val_g = tot[3];
-    const long long bytes = km.cap_ent_v * 32 + km.cap_val_v * 16 + km.cap_ent_g * 16 + km.cap_val_g * 16 +
+    const long long bytes = km.cap_ent_v * 32 + km.cap_val_v * 16 + km.cap_ent_g * 48 + km.cap_val_g * 16 +
                             (km.units_v + km.units_g) * 20;
     km.bytes = bytes;
     if (bytes > max_bytes) {
@@ -774,7 +857,7 @@ skm_status kernel_matrix_build(const Problem &pr, KernelMatrix &km, hipStream_t 
     // value, which for a unit without values is one past the last
     if ((st = km_alloc(&km.v_ent, 2 * km.cap_ent_v)) != SKM_SUCCESS) return done(st);
     if ((st = km_alloc(&km.v_val, km.cap_val_v + kEntryPad)) != SKM_SUCCESS) return done(st);
-    if ((st = km_alloc(&km.g_ent, km.cap_ent_g)) != SKM_SUCCESS) return done(st);
+    if ((st = km_alloc(&km.g_ent, 3 * km.cap_ent_g)) != SKM_SUCCESS) return done(st);
     if ((st = km_alloc(&km.g_val, km.cap_val_g + kEntryPad)) != SKM_SUCCESS) return done(st);
     bv.ent = km.v_ent;
     bv.val = km.v_val;
@@ -800,7 +883,7 @@ skm_status kernel_matrix_build(const Problem &pr, KernelMatrix &km, hipStream_t 
     km.nnz_g = used[1];
     // bytes one evaluation reads from each layout: records and float4 slots
     km.read_v = used[4] * 32 + used[2] * 16;
-    km.read_g = used[5] * 16 + used[3] * 16;
+    km.read_g = used[5] * 48 + used[3] * 16;
     km.state = 1;
     km.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return done(SKM_SUCCESS);
@@ -829,10 +912,10 @@ void launch_gather_km(const Problem &pr, const KernelMatrix &km, const float4 *s
 {
     if (pr.dim == 2) {
         k_km_gather<2><<<(unsigned)gather_workgroups<2>(pr), kThreads, 0, s>>>(
-            pr, sv, (int)gather_tiles<2>(pr), km.g_ent, km.g_val, km.g_ent_off, km.g_val_off, km.g_cnt);
+            pr, sv, group_map<2>(pr), km.g_ent, km.g_val, km.g_ent_off, km.g_val_off, km.g_cnt);
     } else {
         k_km_gather<3><<<(unsigned)gather_workgroups<3>(pr), kThreads, 0, s>>>(
-            pr, sv, (int)gather_tiles<3>(pr), km.g_ent, km.g_val, km.g_ent_off, km.g_val_off, km.g_cnt);
+            pr, sv, group_map<3>(pr), km.g_ent, km.g_val, km.g_ent_off, km.g_val_off, km.g_cnt);
     }
     launch_gather_finish(pr, deriv_soa, s);
 }
