@@ -18,6 +18,7 @@
 // the results are the same bits on every call.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <chrono>
 #include <cstdlib>
@@ -413,20 +414,46 @@ __device__ __forceinline__ bool tile_point(const Problem &pr, int t0, int t1, in
 // GpuEvaluatorMSE.cu:134-162, GpuEvaluatorNCC.cu:145-176: a pair counts when
 // the sample lies in the control point's window and, here, within the pair's
 // reach).  Per block: one lane mask per tile (its four points' non-zero pairs)
-// and per lane of a tile's mask the float4 of its four points' k.  The tile
-// points' positions, windows and states are read per block (scalar cache).
+// and per lane of a tile's mask the float4 of its four points' k.  The
+// group's 16 points (position, reach share, window, state) are staged in LDS
+// once; a point whose window misses a block's box, or whose reach with the
+// largest lam any sample has cannot meet the box, is skipped for that block.
 // sstate: packed states of the samples (weighted modes).
 template <int DIM, int MODE, bool FILL>
 __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, GroupMap gm,
                                                               const float4 *__restrict__ sstate, BuildArgs ba)
 {
-    constexpr int T = CpTile<DIM>::X * CpTile<DIM>::Y * CpTile<DIM>::Z;
+    constexpr int T = CpTile<DIM>::X * CpTile<DIM>::Y * CpTile<DIM>::Z, G = kGroupTiles, NP = G * T;
     constexpr int BX = GBlock<DIM>::X, BY = GBlock<DIM>::Y, BZ = GBlock<DIM>::Z;
     static_assert(T == 4, "one float4 of k per lane, tile and block");
+    __shared__ float4 lp[NP];      // position, lam (the point's reach share as in the on-the-fly gather)
+    __shared__ int4 lwa[NP], lwb[NP];  // window lo (x, y, z, ok), hi (x, y, z, 0)
+    __shared__ float4 lsa[NP], lsb[NP];  // packed state (weighted modes)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const long long unit = (long long)blockIdx.x * (kThreads / 64) + wave;
-    int slab_id, tile[kGroupTiles], tc[3 * kGroupTiles];
+    int slab_id, tile[G], tc[3 * G];
     const bool live = group_unit<DIM>(pr, gm, slab_id, tile, tc);
+    if (live && threadIdx.x < NP) {
+        const int sidx = threadIdx.x / T, c = threadIdx.x % T;
+        long long j = 0;
+        int jc[3] = {0, 0, 0};
+        const bool ok = tile[sidx] >= 0 && tile_point<DIM>(pr, tc[3 * sidx], tc[3 * sidx + 1], tc[3 * sidx + 2], c, j, jc);
+        const float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]};
+        float p[3] = {0.0f, 0.0f, 0.0f};
+        to_global<DIM>(pr.cp, fj, p);
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        if (!is_stationary(MODE)) {
+            a = pr.cp_pstate[2 * j];
+            b = pr.cp_pstate[2 * j + 1];
+        }
+        lp[threadIdx.x] = make_float4(p[0], p[1], p[2], is_stationary(MODE) ? 1.0f : b.w);
+        const int *w = pr.cp_window + j * 6;
+        lwa[threadIdx.x] = make_int4(w[0], DIM > 1 ? w[1] : 0, DIM == 3 ? w[2] : 0, ok ? 1 : 0);
+        lwb[threadIdx.x] = make_int4(w[3], DIM > 1 ? w[4] : 0, DIM == 3 ? w[5] : 0, 0);
+        lsa[threadIdx.x] = a;
+        lsb[threadIdx.x] = b;
+    }
+    __syncthreads();
     int ne = 0;
     long long nv = 0, npairs = 0;
     bool full = false;
@@ -440,17 +467,14 @@ __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, GroupM
     if (live) {
         const int slab = gather_slab<DIM>(pr);
         int ulo[3] = {INT_MAX, INT_MAX, DIM == 3 ? INT_MAX : 0}, uhi[3] = {INT_MIN, INT_MIN, DIM == 3 ? INT_MIN : 0};
-        for (int s = 0; s < kGroupTiles; ++s) {
-            if (tile[s] < 0) continue;
-            for (int c = 0; c < T; ++c) {
-                long long j;
-                int jc[3];
-                if (!tile_point<DIM>(pr, tc[3 * s], tc[3 * s + 1], tc[3 * s + 2], c, j, jc)) continue;
+        for (int q = 0; q < NP; ++q) {
+            const int4 wa = lwa[q], wb = lwb[q];
+            if (!wa.w) continue;
+            const int lo3[3] = {wa.x, wa.y, wa.z}, hi3[3] = {wb.x, wb.y, wb.z};
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) {
-                    ulo[d] = min(ulo[d], pr.cp_window[j * 6 + d]);
-                    uhi[d] = max(uhi[d], pr.cp_window[j * 6 + 3 + d]);
-                }
+            for (int d = 0; d < DIM; ++d) {
+                ulo[d] = min(ulo[d], lo3[d]);
+                uhi[d] = max(uhi[d], hi3[d]);
             }
         }
         ulo[DIM - 1] = max(ulo[DIM - 1], slab_id * slab);
@@ -460,10 +484,23 @@ __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, GroupM
         const int nb2 = any ? (DIM == 3 ? (uhi[2] - ulo[2] + BZ) / BZ : 1) : 0;
         const int tx = lane % BX, ty = (lane / BX) % BY, tz = lane / (BX * BY);
         const float smin2h = 0.5f * pr.kc.smin * pr.kc.smin * 1.0001f;
+        // the largest lam a sample can have (stationary modes: 1)
+        const float lam_smax = is_stationary(MODE) ? 1.0f : pr.lam_w_max;
         const int nblk = nb0 * nb1 * nb2;
         for (int blk = wave; blk < nblk && !full; blk += kThreads / 64) {
             const int bo[3] = {ulo[0] + (blk % nb0) * BX, ulo[1] + ((blk / nb0) % nb1) * BY,
                                DIM == 3 ? ulo[2] + (blk / (nb0 * nb1)) * BZ : 0};
+            const int be[3] = {min(bo[0] + BX - 1, uhi[0]), min(bo[1] + BY - 1, uhi[1]),
+                               DIM == 3 ? min(bo[2] + BZ - 1, uhi[2]) : 0};
+            // the block's box in physical coordinates (uniform)
+            float bl[3] = {0.0f, 0.0f, 0.0f}, bh[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const float e0 = pr.fixed.o[d] + pr.fixed.h[d] * (float)bo[d];
+                const float e1 = pr.fixed.o[d] + pr.fixed.h[d] * (float)be[d];
+                bl[d] = fminf(e0, e1);
+                bh[d] = fmaxf(e0, e1);
+            }
             const int mm[3] = {bo[0] + tx, bo[1] + ty, bo[2] + tz};
             const bool inside = mm[0] <= uhi[0] && mm[1] <= uhi[1] && (DIM == 2 || mm[2] <= uhi[2]);
             const long long sidx = inside ? mm[0] + (long long)pr.nsub[0] * (mm[1] + (long long)pr.nsub[1] * mm[2]) : 0;
@@ -477,47 +514,60 @@ __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, GroupM
             const float fpos[3] = {(float)mm[0], (float)mm[1], (float)mm[2]};
             float xs[3];
             to_global<DIM>(pr.fixed, fpos, xs);
-            unsigned long long sm[kGroupTiles];
-            float k[kGroupTiles][T];
+            unsigned long long sm[G];
+            float k[G][T];
             int pairs = 0;
+            // an index the compiler cannot see through: the points' LDS reads
+            // stay inside the block loop instead of 16 points' data held in VGPRs
+            int q0 = 0;
+            asm volatile("" : "+v"(q0));
 #pragma unroll
-            for (int s = 0; s < kGroupTiles; ++s) {
-                sm[s] = 0;
+            for (int sg = 0; sg < G; ++sg) {
+                sm[sg] = 0;
 #pragma unroll
                 for (int c = 0; c < T; ++c) {
-                    k[s][c] = 0.0f;
-                    long long j;
-                    int jc[3];
-                    if (tile[s] < 0 || !tile_point<DIM>(pr, tc[3 * s], tc[3 * s + 1], tc[3 * s + 2], c, j, jc))
-                        continue;
-                    const float fj[3] = {(float)jc[0], (float)jc[1], (float)jc[2]};
-                    float p[3] = {0.0f, 0.0f, 0.0f};
-                    to_global<DIM>(pr.cp, fj, p);
-                    PState cst;
-                    const float lam_c = is_stationary(MODE) ? 1.0f : cp_state_load<DIM, MODE>(pr, j, cst);
-                    const float v[3] = {p[0] - xs[0], p[1] - xs[1], DIM == 3 ? p[2] - xs[2] : 0.0f};
+                    k[sg][c] = 0.0f;
+                    const int q = q0 + sg * T + c;
+                    const int4 wa = lwa[q], wb = lwb[q];
+                    const float4 pp = lp[q];
+                    // the block's box against the point's window and largest reach (uniform)
+                    const int wlo[3] = {wa.x, wa.y, wa.z}, whi[3] = {wb.x, wb.y, wb.z};
+                    const float pc[3] = {pp.x, pp.y, pp.z};
+                    bool meet = wa.w != 0;
+                    float dbox = 0.0f;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        meet = meet && bo[d] <= whi[d] && be[d] >= wlo[d];
+                        const float g = fmaxf(0.0f, fmaxf(bl[d] - pc[d], pc[d] - bh[d]));
+                        dbox = fmaf(g, g, dbox);
+                    }
+                    if (!meet || !(dbox < smin2h * (pp.w + lam_smax))) continue;
+                    const float v[3] = {pc[0] - xs[0], pc[1] - xs[1], DIM == 3 ? pc[2] - xs[2] : 0.0f};
                     const float d2 = v[0] * v[0] + v[1] * v[1] + (DIM == 3 ? v[2] * v[2] : 0.0f);
                     bool pass = inside;
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d)
-                        pass = pass && mm[d] >= pr.cp_window[j * 6 + d] && mm[d] <= pr.cp_window[j * 6 + 3 + d];
-                    pass = pass && d2 < smin2h * (lam_c + lam_s);
+                    for (int d = 0; d < DIM; ++d) pass = pass && mm[d] >= wlo[d] && mm[d] <= whi[d];
+                    pass = pass && d2 < smin2h * (pp.w + lam_s);
                     if (!FILL) {
-                        sm[s] |= __ballot(pass);
+                        sm[sg] |= __ballot(pass);
                         continue;
                     }
                     if (!__ballot(pass)) continue;
                     float kv = 0.0f;
-                    if (pass) kv = kernel_pair<DIM, MODE>(pr.kc, v, cst, sst);
+                    if (pass) {
+                        PState cst;
+                        if (!is_stationary(MODE)) pstate_unpack<DIM, MODE>(lsa[q], lsb[q], cst);
+                        kv = kernel_pair<DIM, MODE>(pr.kc, v, cst, sst);
+                    }
                     const unsigned long long km = __ballot(pass && kv != 0.0f);
-                    k[s][c] = lane_bit(km, lane) ? kv : 0.0f;
-                    sm[s] |= km;
+                    k[sg][c] = lane_bit(km, lane) ? kv : 0.0f;
+                    sm[sg] |= km;
                     pairs += __popcll(km);
                 }
             }
             int slots = 0;
 #pragma unroll
-            for (int s = 0; s < kGroupTiles; ++s) slots += __popcll(sm[s]);
+            for (int sg = 0; sg < G; ++sg) slots += __popcll(sm[sg]);
             if (!slots) continue;
             if (FILL) {
                 if (ne >= cap_e || nv + slots > cap_v) {
@@ -534,10 +584,10 @@ __global__ __launch_bounds__(kThreads) void k_km_gather_build(Problem pr, GroupM
                 }
                 long long vb = vbase + nv;
 #pragma unroll
-                for (int s = 0; s < kGroupTiles; ++s) {
-                    if (lane_bit(sm[s], lane))
-                        ba.val[vb + lane_rank(sm[s])] = make_float4(k[s][0], k[s][1], k[s][2], k[s][3]);
-                    vb += __popcll(sm[s]);
+                for (int sg = 0; sg < G; ++sg) {
+                    if (lane_bit(sm[sg], lane))
+                        ba.val[vb + lane_rank(sm[sg])] = make_float4(k[sg][0], k[sg][1], k[sg][2], k[sg][3]);
+                    vb += __popcll(sm[sg]);
                 }
                 npairs += pairs;
             }
@@ -693,6 +743,50 @@ __global__ __launch_bounds__(1024) void k_km_scan(const int *__restrict__ cap, l
     if (t == 1023) *total = part[1023];
 }
 
+// Block-parallel form of the same exclusive scan: each workgroup scans 1024
+// capacities (four per thread) and writes its total; the totals are scanned
+// by k_km_scan; k_km_scan_add adds each workgroup's offset.
+constexpr int kScanBlock = 1024;
+
+__global__ __launch_bounds__(256) void k_km_scan_local(const int *__restrict__ cap, long long n,
+                                                       long long *__restrict__ off, int *__restrict__ bsum)
+{
+    __shared__ long long part[256];
+    const long long base = (long long)blockIdx.x * kScanBlock + 4 * threadIdx.x;
+    int v[4];
+    long long sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v[i] = base + i < n ? cap[base + i] : 0;
+        sum += v[i];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const long long add = (int)threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    long long run = part[threadIdx.x] - sum;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (base + i < n) off[base + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == 255) bsum[blockIdx.x] = (int)part[255];
+}
+
+__global__ __launch_bounds__(256) void k_km_scan_add(long long *__restrict__ off, long long n,
+                                                     const long long *__restrict__ boff)
+{
+    const long long i = (long long)blockIdx.x * kScanBlock + threadIdx.x;
+    const long long add = boff[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i + 256 * k < n) off[i + 256 * k] += add;
+}
+
 // Sum of n ints (one workgroup, fixed order).
 __global__ __launch_bounds__(1024) void k_km_total(const int *__restrict__ v, long long n, long long *__restrict__ total)
 {
@@ -830,10 +924,31 @@ skm_status kernel_matrix_build(const Problem &pr, KernelMatrix &km, hipStream_t 
     BuildArgs bg{capg, capg + km.units_g, km.g_ent_off, km.g_val_off, km.g_cnt, nnz + km.units_v, nullptr, nullptr,
                  flag};
     SKM_KM_DISPATCH(pr, build_count, pr, sstate, bv, bg, s);
-    k_km_scan<<<1, 1024, 0, s>>>(bv.cap_ent, km.units_v, km.v_ent_off, totals + 0);
-    k_km_scan<<<1, 1024, 0, s>>>(bv.cap_val, km.units_v, km.v_val_off, totals + 1);
-    k_km_scan<<<1, 1024, 0, s>>>(bg.cap_ent, km.units_g, km.g_ent_off, totals + 2);
-    k_km_scan<<<1, 1024, 0, s>>>(bg.cap_val, km.units_g, km.g_val_off, totals + 3);
+    {
+        // exclusive offsets of the four capacity arrays
+        const long long nbv = (km.units_v + kScanBlock - 1) / kScanBlock, nbg = (km.units_g + kScanBlock - 1) / kScanBlock;
+        int *bsum = nullptr;
+        long long *boff = nullptr;
+        if ((st = km_alloc(&bsum, 4 * std::max(nbv, nbg))) != SKM_SUCCESS) return done(st);
+        if ((st = km_alloc(&boff, 4 * std::max(nbv, nbg))) != SKM_SUCCESS) {
+            device_release(bsum);
+            return done(st);
+        }
+        const int *caps[4] = {bv.cap_ent, bv.cap_val, bg.cap_ent, bg.cap_val};
+        long long *offs[4] = {km.v_ent_off, km.v_val_off, km.g_ent_off, km.g_val_off};
+        const long long ns[4] = {km.units_v, km.units_v, km.units_g, km.units_g};
+        const long long mb = std::max(nbv, nbg);
+        for (int i = 0; i < 4; ++i) {
+            const long long nb = (ns[i] + kScanBlock - 1) / kScanBlock;
+            k_km_scan_local<<<(unsigned)nb, 256, 0, s>>>(caps[i], ns[i], offs[i], bsum + i * mb);
+            k_km_scan<<<1, 1024, 0, s>>>(bsum + i * mb, nb, boff + i * mb, totals + i);
+            k_km_scan_add<<<(unsigned)nb, 256, 0, s>>>(offs[i], ns[i], boff + i * mb);
+        }
+        // released with the stream's work (the counts are copied back below, after a sync)
+        (void)hipStreamSynchronize(s);
+        device_release(bsum);
+        device_release(boff);
+    }
     launch_zero_words(flag, 1, s);
     long long tot[4] = {0, 0, 0, 0};
     if (hipMemcpyAsync(tot, totals, sizeof(tot), hipMemcpyDeviceToHost, s) != hipSuccess ||
