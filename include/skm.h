@@ -231,9 +231,12 @@ SKM_API skm_status skm_fit_coefficients_device(skm_evaluator *ev, const float *d
  *                   neighbour); auto (0) takes the form with the smaller
  *                   expected time (the walk's triples against the voxels);
  *                   results agree to FP rounding
- *   kernel_matrix   MSE / NCC with subsample 1: 0 assembles the evaluator's
+ *   kernel_matrix   MSE / NCC with subsample 1: 2 assembles the evaluator's
  *                   kernel matrix at its first evaluation and reads it from
- *                   then on, 1 evaluates the kernels on the fly (see
+ *                   then on, 1 evaluates the kernels on the fly, 0 (auto) the
+ *                   former for the tensor kernel and the latter for the
+ *                   stationary and weight-image kernels, which cost less to
+ *                   evaluate than their stored values to stream (see
  *                   skm_get_kernel_matrix_info)
  * skm_get_tuning returns the fields as set (0 where automatic). */
 typedef struct skm_tuning {
@@ -257,8 +260,9 @@ SKM_API skm_status skm_get_tuning(skm_evaluator *ev, skm_tuning *tuning);
  * life: its non-zero k(x_m, p_j) are assembled once in device memory, in the
  * order each pass reads them, and every later MSE / NCC evaluation streams
  * them instead of evaluating the kernel.  It is assembled at the first MSE /
- * NCC evaluation unless skm_tuning.kernel_matrix is 1 or that evaluation is
- * being captured into a graph; it is refused (the kernels then run on the fly)
+ * NCC evaluation when skm_tuning.kernel_matrix asks for it (automatically for
+ * the tensor kernel), unless that evaluation is being captured into a graph;
+ * it is refused (the kernels then run on the fly)
  * when it would hold more than SKM_KERNEL_MATRIX_MAX_BYTES (environment) or,
  * by default, a quarter of the device's memory or half of its free memory.
  * Results equal the on-the-fly passes' within FP32 rounding of the sums'

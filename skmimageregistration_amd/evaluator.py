@@ -289,7 +289,8 @@ class GpuKernelEvaluator:
     def set_tuning(self, lcc_ref_chunks=0, gather_split=0, value_split=0, gather_regions=0, lcc_ref_walk=0,
                    kernel_matrix=0):
         """Launch shapes (skm_set_tuning); 0 keeps the library's choice.
-        kernel_matrix=1 evaluates the kernels on the fly at subsample 1."""
+        kernel_matrix at subsample 1: 2 reads the assembled kernel matrix, 1
+        evaluates the kernels on the fly, 0 the former for the tensor kernel."""
         t = _capi.SkmTuning(int(lcc_ref_chunks), int(gather_split), int(value_split), int(gather_regions),
                             int(lcc_ref_walk), int(kernel_matrix))
         check(self._lib.skm_set_tuning(self._h, ctypes.byref(t)))

@@ -2,6 +2,8 @@
 from the assembled k(x_m, p_j) against the same evaluator's on-the-fly
 kernels (skm_tuning.kernel_matrix = 1) and against the FP64 oracle.
 
+The automatic policy assembles it for the tensor kernel only.
+
 Tolerances: against the oracle, the parity bar (value 1e-5 relative,
 derivative 1e-4 of its largest entry); between the two GPU forms, which sum
 the same FP32 pair terms in different orders, value 1e-6 and derivative 1e-5.
@@ -35,7 +37,7 @@ def _eval(case, measure, kernel_matrix, gather_split=0, regs=None):
 def _check(case, measure, gather_split=0):
     print(f"case dim={case.dim} shape={case.shape} mode={case.mode} measure={int(measure)} "
           f"exact={case.exact} df={case.df is not None} split={gather_split}", flush=True)
-    ev_km, r_km, d_km, info = _eval(case, measure, 0, gather_split)
+    ev_km, r_km, d_km, info = _eval(case, measure, 2, gather_split)
     assert info["state"] == 1 and info["pairs_value"] > 0 and info["pairs_gather"] > 0, info
     # a second evaluation from the matrix: the same bits
     r2, d2 = ev_km.getValueAndDerivative(measure, case.cp, case.cpw, case.df_image())
@@ -88,7 +90,7 @@ def test_kernel_matrix_refused_above_the_limit():
     case = Case(dim=3, shape=(16, 18, 20), cp_spacing=5.0, sigma=12.0, mode=2, seed=51)
     os.environ["SKM_KERNEL_MATRIX_MAX_BYTES"] = "1000"
     try:
-        ev, res, der, info = _eval(case, MEASURE.NCC, 0)
+        ev, res, der, info = _eval(case, MEASURE.NCC, 2)
     finally:
         del os.environ["SKM_KERNEL_MATRIX_MAX_BYTES"]
     ev.close()
@@ -100,7 +102,7 @@ def test_kernel_matrix_refused_above_the_limit():
 
 def test_kernel_matrix_release_and_reassembly():
     case = Case(dim=3, shape=(16, 18, 20), cp_spacing=5.0, sigma=12.0, mode=2, seed=53)
-    ev, r1, d1, info = _eval(case, MEASURE.NCC, 0)
+    ev, r1, d1, info = _eval(case, MEASURE.NCC, 2)
     ev.release_kernel_matrix()
     assert ev.kernel_matrix_info()["state"] == 0
     r2, d2 = ev.getValueAndDerivative(MEASURE.NCC, case.cp, case.cpw)
@@ -119,18 +121,30 @@ def test_kernel_matrix_optimizer(graph):
     start = np.zeros(case.n_cp * 3, np.float32)
     opts = dict(metric=MEASURE.NCC, iterations=20, kind=SKM_OPT_FIXED, a=50.0)
     out = {}
-    for km in (0, 1):
+    for km in (2, 1):
         ev = case.gpu()
         ev.set_tuning(kernel_matrix=km)
         p, rec = ev.optimize(start, graph=graph, **opts)
-        assert ev.kernel_matrix_info()["state"] == (1 if km == 0 else 0)
-        if km == 0 and graph:
+        assert ev.kernel_matrix_info()["state"] == (1 if km == 2 else 0)
+        if km == 2 and graph:
             p_eager, rec_eager = ev.optimize(start, graph=False, **opts)
             np.testing.assert_array_equal(p, p_eager)
             assert [r[0] for r in rec] == [r[0] for r in rec_eager]
         ev.close()
         out[km] = (p, rec)
-    (p0, r0), (p1, r1) = out[0], out[1]
+    (p0, r0), (p1, r1) = out[2], out[1]
     assert np.abs(p0).max() > 0
     assert rel_err(p0, p1) < 1e-4, rel_err(p0, p1)
     assert abs(r0[-1][1] - r1[-1][1]) < 1e-5 * abs(r1[-1][1])
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_kernel_matrix_automatic_policy(mode):
+    """kernel_matrix 0: assembled for the tensor kernel, on the fly otherwise."""
+    case = Case(dim=3, shape=(16, 18, 20), cp_spacing=5.0, sigma=12.0, mode=mode, seed=57)
+    ev, res, der, info = _eval(case, MEASURE.NCC, 0)
+    ev.close()
+    assert info["state"] == (1 if mode == 2 else 0)
+    ores, oder = case.oracle().evaluate(int(MEASURE.NCC), True, 0.0, 0.0, 0.0)
+    assert scalar_rel(res.measure, ores["measure"]) < VALUE_TOL
+    assert rel_err(der, oder) < DERIV_TOL
