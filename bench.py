@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--cp-spacing", type=float, default=6.0)
     ap.add_argument("--sigma", type=float, default=24.0)
     ap.add_argument("--kernel", default="anisotropic", choices=["anisotropic", "stationary"])
+    ap.add_argument("--gather-split", type=int, default=0, choices=[0, 4, 8],
+                    help="sample planes per derivative-gather slab (skm_tuning.gather_split; 0: automatic)")
     ap.add_argument("--tensor-weight", type=float, default=400.0)
     ap.add_argument("--subsample", type=int, default=1)
     ap.add_argument("--measure", default="ncc", choices=["mse", "ncc", "lcc"])
@@ -878,6 +880,8 @@ def main():
         ev.SetRegularizerRD(args.reg_rd)
     if args.reg_bilateral > 0:
         ev.SetRegularizerBilateral(*bilateral_args(args, args.reg_bilateral))
+    if args.gather_split:
+        ev.set_tuning(gather_split=args.gather_split)
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
     torch.cuda.set_stream(stream)
@@ -930,7 +934,7 @@ def main():
     per_kernel = pass_times(min(args.steps, 50))  # per-pass device times, assembled
     steady_ms = timed(min(args.steps, 50)) / min(args.steps, 50)
     # the same evaluations with the kernels evaluated on the fly
-    ev.set_tuning(kernel_matrix=1)
+    ev.set_tuning(gather_split=args.gather_split, kernel_matrix=1)
     for i in range(2):
         step(i)
     n_fly = min(args.steps, 20)

@@ -746,8 +746,7 @@ skm_status use_kernel_matrix(skm_evaluator *ev, bool *use)
     if (policy == 1 || (policy == 0 && ev->pr.kc.mode != 2) || !kernel_matrix_applies(ev->pr) ||
         ev->km.state == 2)
         return SKM_SUCCESS;
-    const int slab = ev->pr.dim == 3 ? (ev->pr.gather_split == 4 ? 4 : 8) : std::max(ev->pr.fixed.n[1], 1);
-    if (ev->km.state == 1 && ev->km.slab == slab) {
+    if (ev->km.state == 1 && ev->km.slab == kernel_matrix_slab(ev->pr)) {
         *use = true;
         return SKM_SUCCESS;
     }

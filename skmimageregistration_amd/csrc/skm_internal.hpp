@@ -149,6 +149,8 @@ struct KernelMatrix {
 };
 // whether the evaluator's problem has a kernel matrix (subsample 1, sizes within 32-bit indices)
 bool kernel_matrix_applies(const Problem &pr);
+// sample planes per slab of the kernel-matrix gather for this problem's tuning
+int kernel_matrix_slab(const Problem &pr);
 // Assembles km on stream s (returns when done).  Refuses (state 2, SKM_SUCCESS)
 // when its device memory would exceed max_bytes; errors leave state 0.
 skm_status kernel_matrix_build(const Problem &pr, KernelMatrix &km, hipStream_t s, long long max_bytes);

@@ -866,6 +866,22 @@ void build_fill(const Problem &pr, const float4 *sstate, const BuildArgs &bv, co
 
 }  // namespace
 
+// The kernel-matrix gather's problem: slabs of 4 sample planes unless the
+// tuning asks for 8 (4 measured faster: 0.49 against 0.53 ms at 128x128x96,
+// each slab's samples then share an XCD's L2 better).
+static Problem km_gather_problem(const Problem &pr)
+{
+    Problem p = pr;
+    if (p.dim == 3 && p.gather_split != 8) p.gather_split = 4;
+    return p;
+}
+
+int kernel_matrix_slab(const Problem &pr)
+{
+    const Problem p = km_gather_problem(pr);
+    return p.dim == 2 ? gather_slab<2>(p) : gather_slab<3>(p);
+}
+
 bool kernel_matrix_applies(const Problem &pr)
 {
     return pr.sub == 1 && pr.n_sub > 0 && pr.n_sub < (1LL << 31) && pr.n_cp < (1LL << 31);
@@ -880,8 +896,9 @@ void kernel_matrix_release(KernelMatrix &km)
     km = KernelMatrix{};
 }
 
-skm_status kernel_matrix_build(const Problem &pr, KernelMatrix &km, hipStream_t s, long long max_bytes)
+skm_status kernel_matrix_build(const Problem &pr_in, KernelMatrix &km, hipStream_t s, long long max_bytes)
 {
+    const Problem pr = km_gather_problem(pr_in);
     const auto t_start = std::chrono::steady_clock::now();
     kernel_matrix_release(km);
     if (!kernel_matrix_applies(pr)) return set_last_error(SKM_ERROR_INVALID_ARGUMENT, "no kernel matrix: subsample > 1");
@@ -1022,9 +1039,10 @@ void launch_value_km(const Problem &pr, const KernelMatrix &km, int measure, flo
 #undef SKM_KM_VALUE
 }
 
-void launch_gather_km(const Problem &pr, const KernelMatrix &km, const float4 *sv, float *deriv_soa,
+void launch_gather_km(const Problem &pr_in, const KernelMatrix &km, const float4 *sv, float *deriv_soa,
                       hipStream_t s)
 {
+    const Problem pr = km_gather_problem(pr_in);
     if (pr.dim == 2) {
         k_km_gather<2><<<(unsigned)gather_workgroups<2>(pr), kThreads, 0, s>>>(
             pr, sv, group_map<2>(pr), km.g_ent, km.g_val, km.g_ent_off, km.g_val_off, km.g_cnt);

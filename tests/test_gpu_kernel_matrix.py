@@ -80,10 +80,12 @@ def test_kernel_matrix_exact_windows_and_previous_field(dim):
            MEASURE.NCC)
 
 
-def test_kernel_matrix_geometry_and_gather_slab():
+@pytest.mark.parametrize("split", [0, 8])
+def test_kernel_matrix_geometry_and_gather_slab(split):
+    # the kernel-matrix gather takes 4-plane slabs unless 8 is asked for
     case = Case(dim=3, shape=(17, 21, 23), cp_spacing=6.0, sigma=14.0, mode=1, origin=(-3.5, 2.25, 1.0),
                 spacing=(1.25, 0.8, 1.5), seed=49)
-    _check(case, MEASURE.MSE, gather_split=4)
+    _check(case, MEASURE.MSE, gather_split=split)
 
 
 def test_kernel_matrix_refused_above_the_limit():
