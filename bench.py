@@ -76,6 +76,9 @@ def parse():
     ap.add_argument("--no-lockstep", action="store_true",
                     help="skip the lockstep (graph-replay) runs of the batch and warm-start legs")
     ap.add_argument("--registration-repeats", type=int, default=3)
+    ap.add_argument("--registration-autotune", type=int, default=1, choices=[0, 1],
+                    help="1: the registration's levels take their launch shapes from the autotuner "
+                         "(tuned in the untimed first run; 0: the automatic shapes)")
     ap.add_argument("--registration-bilateral", type=float, default=1e-5,
                     help="bilateral weight of the registrations (spatial / intensity sigma as above)")
     ap.add_argument("--no-pair-count", action="store_true", help="skip the exact k != 0 pair count")
@@ -608,7 +611,8 @@ def registration_config(args):
                               if args.kernel == "anisotropic" else [0.0],
                               num_scales=3, num_function_evaluations=[200], initial_step_size=[1000.0],
                               sampling_rate=[5, 4, 3], neighborhood_sampling=[1], smoothing_variance=[2, 1, 0],
-                              metric="ncc", **bilateral_config(args, args.registration_bilateral))
+                              metric="ncc", autotune=[int(getattr(args, "registration_autotune", 0))],
+                              **bilateral_config(args, args.registration_bilateral))
 
 
 def bilateral_args(args, weight):
@@ -645,7 +649,8 @@ def registration(args, mov, fix, disp):
             "level_ms": [{k: round(r[k], 2) for k in ("ms_setup", "ms_optimize", "ms_displacement")} for r in reports],
             "mean_abs_error_vox": round(float(np.abs(df[inner] - disp[inner]).mean()), 4),
             "mean_abs_displacement_vox": round(float(np.abs(disp[inner]).mean()), 4),
-            "repeats_ms": [round(t, 2) for t in times], "reg_bilateral": args.registration_bilateral}
+            "repeats_ms": [round(t, 2) for t in times], "reg_bilateral": args.registration_bilateral,
+            "autotune": int(getattr(args, "registration_autotune", 0))}
 
 
 def batch_timing(args):
