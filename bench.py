@@ -600,7 +600,7 @@ def lcc_registration(args):
             "exact_gradient": exact}
 
 
-def registration_config(args):
+def registration_config(args, autotune=0):
     """3 levels with support 4s, 2s, s, control-point spacing support/4,
     tensors from the target with weights 400/200/100, NCC + bilateral, sampling
     rates 5/4/3, 200 adaptive-step iterations per level."""
@@ -611,7 +611,7 @@ def registration_config(args):
                               if args.kernel == "anisotropic" else [0.0],
                               num_scales=3, num_function_evaluations=[200], initial_step_size=[1000.0],
                               sampling_rate=[5, 4, 3], neighborhood_sampling=[1], smoothing_variance=[2, 1, 0],
-                              metric="ncc", autotune=[int(getattr(args, "registration_autotune", 0))],
+                              metric="ncc", autotune=[int(autotune)],
                               **bilateral_config(args, args.registration_bilateral))
 
 
@@ -633,7 +633,8 @@ def registration(args, mov, fix, disp):
     untimed run; the median of the repeats."""
     from skmimageregistration_amd import Image, register_multiscale
     s = args.sigma
-    cfg = registration_config(args)
+    # with autotune the untimed first run tunes each level's launch shapes
+    cfg = registration_config(args, autotune=args.registration_autotune)
     register_multiscale(Image(mov), Image(fix), cfg)
     times, reports, df = [], None, None
     for _ in range(max(args.registration_repeats, 1)):
