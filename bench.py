@@ -36,11 +36,11 @@ FLOPS_PER_PAIR = {"stationary": 32, "anisotropic": 32 + 75}
 FLOPS_PER_LCC_TRIPLE = 32
 # the same derivative regrouped per voxel (k_lcc_ref_table): per candidate
 # centre of a voxel's table its weight kw (distance, (r/s)^10, squares) and
-# three weighted terms; per (control point, voxel) pair with ka != 0 the
-# kernel ka (32), the 8-corner sums of three tables (21) and the combination
-# and accumulate (10)
+# its term kw (kw (F A + M B) + C) added to the row's prefix; per (control
+# point, voxel) pair with ka != 0 the kernel ka (32), the 8-corner sum of the
+# table (7) and the weighted accumulate (7)
 FLOPS_PER_TABLE_ENTRY = 20
-FLOPS_PER_TABLE_PAIR = 63
+FLOPS_PER_TABLE_PAIR = 46
 
 
 def parse():

@@ -722,7 +722,9 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4
 //   d_j = sum_y ka(p_j, y) gradM(y) (F(y) SA_j(y) + M(y) SB_j(y) + SC_j(y)),
 //   SA_j = sum A kw^2, SB_j = sum B kw^2, SC_j = sum C kw
 // over the centres whose neighbour lattice holds y (found as k_lcc_gamma
-// finds them) and whose cell lies in j's window.  Along an axis, the centres
+// finds them) and whose cell lies in j's window.  F(y) and M(y) are fixed for
+// the voxel, so one sum T_j = sum kw (kw (F A + M B) + C) stands for the
+// bracket.  Along an axis, the centres
 // reaching y occupy a few sample cells (their "ranks"), and j's window, an
 // interval of cells, keeps a contiguous run of those ranks.  So one 3D prefix
 // sum of the centres' terms over (x, y, z) ranks per voxel gives every
@@ -736,10 +738,11 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4
 // per axis (NB = 2 ceil(support / h), A = NB / 2 - 1), whose points have one
 // FP32 sum slot each in LDS for the whole block.  Per voxel: the reach masks
 // and ranks; the list of active slots (points within the support whose
-// windows keep a centre on every axis, about half the box); the table in
-// chunks of at most kRtCh ranks per axis (prefix along x in registers, then
-// y and z in LDS, each line's loads issued together; FP32); and
-// for the active slots, 64 at a time, an 8-corner difference of the table.
+// windows keep a centre on every axis, about half the box); the table of the
+// centres' terms kw (kw (F A + M B) + C) in chunks of at most kRtCh ranks per
+// axis (prefix along x in registers, then y and z in LDS, each line's loads
+// issued together; FP32); and for the active slots, 64 at a time, an 8-corner
+// difference of the table.
 // Each block writes its slots, k_lcc_reft_finish adds the blocks in order in
 // FP64.  A control-point cell wider than ~5 voxels is split into pieces along
 // each axis, one wave each, so that coarse grids still fill the GPU.
@@ -840,7 +843,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                                                       const float4 *__restrict__ coef, RefReach rr,
                                                       RefTableGeom g, float *__restrict__ partials)
 {
-    __shared__ float tA[kRtSize], tB[kRtSize], tC[kRtSize];
+    __shared__ float tT[kRtSize];
     __shared__ float accs[kRtBox * 64 * 3];  // per box control point (slot bz * 64 + lane) and axis
     // the current voxel's active slots: slot (9 bits), bx, by, bz (3 bits
     // each), the z rank range z0, z1 (6 bits each)
@@ -857,7 +860,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     const int k[3] = {g.kmin[0] + cell % g.nk[0], g.kmin[1] + (cell / g.nk[0]) % g.nk[1],
                       g.kmin[2] + cell / (g.nk[0] * g.nk[1])};
     const int q[3] = {piece % g.np[0], (piece / g.np[0]) % g.np[1], piece / (g.np[0] * g.np[1])};
-    for (int i = lane; i < kRtSize; i += 64) tA[i] = tB[i] = tC[i] = 0.0f;  // borders stay zero
+    for (int i = lane; i < kRtSize; i += 64) tT[i] = 0.0f;  // borders stay zero
     for (int i = lane; i < kRtBox * 64 * 3; i += 64) accs[i] = 0.0f;
     if (lane < 3 * kRtBox) {
         // windows are separable for stationary kernels: axis d's from a
@@ -989,7 +992,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                                     for (int lx = 0; lx < kRtCh; ++lx) abc[lx] = coef[rowc + cxs[lx]];
 #pragma unroll
                                     for (int lx = 0; lx < kRtCh; ++lx) w[lx] = __float_as_uint(abc[lx].w);
-                                    float sa = 0.0f, sb = 0.0f, sc = 0.0f;
+                                    float st = 0.0f;
                                     const int tb = rt_at(lz + 1, ly + 1, 1);
 #pragma unroll
                                     for (int lx = 0; lx < kRtCh; ++lx) {
@@ -1010,16 +1013,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                                             const float q = r2 * inv_s2, q2 = q * q;
                                             const float f = fmaxf(0.0f, 1.0f - q2 * q2 * q);
                                             const float kw = f * f;
-                                            const float kw2 = kw * kw;
-                                            sa = fmaf(abc[lx].x, kw2, sa);
-                                            sb = fmaf(abc[lx].y, kw2, sb);
-                                            sc = fmaf(abc[lx].z, kw, sc);
+                                            st = fmaf(kw, fmaf(kw, fmaf(F, abc[lx].x, M * abc[lx].y), abc[lx].z), st);
                                         }
-                                        if (lx < nx) {
-                                            tA[tb + lx] = sa;
-                                            tB[tb + lx] = sb;
-                                            tC[tb + lx] = sc;
-                                        }
+                                        if (lx < nx) tT[tb + lx] = st;
                                     }
                                 }
                             }
@@ -1028,22 +1024,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                             {
                                 const int lx = lane % kRtCh, lz = lane / kRtCh;
                                 if (lz < kRtCh && lx < nx && lz < nz) {
-                                    float va[kRtCh], vb[kRtCh], vc[kRtCh];
+                                    float va[kRtCh];
 #pragma unroll
-                                    for (int ly = 0; ly < kRtCh; ++ly) {
-                                        const int t = rt_at(lz + 1, ly + 1, lx + 1);
-                                        va[ly] = tA[t]; vb[ly] = tB[t]; vc[ly] = tC[t];
-                                    }
+                                    for (int ly = 0; ly < kRtCh; ++ly) va[ly] = tT[rt_at(lz + 1, ly + 1, lx + 1)];
 #pragma unroll
-                                    for (int ly = 1; ly < kRtCh; ++ly) {
-                                        va[ly] += va[ly - 1]; vb[ly] += vb[ly - 1]; vc[ly] += vc[ly - 1];
-                                    }
+                                    for (int ly = 1; ly < kRtCh; ++ly) va[ly] += va[ly - 1];
 #pragma unroll
                                     for (int ly = 1; ly < kRtCh; ++ly)
-                                        if (ly < ny) {
-                                            const int t = rt_at(lz + 1, ly + 1, lx + 1);
-                                            tA[t] = va[ly]; tB[t] = vb[ly]; tC[t] = vc[ly];
-                                        }
+                                        if (ly < ny) tT[rt_at(lz + 1, ly + 1, lx + 1)] = va[ly];
                                 }
                             }
                             __syncthreads();
@@ -1051,22 +1039,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                             {
                                 const int lx = lane % kRtCh, ly = lane / kRtCh;
                                 if (ly < kRtCh && lx < nx && ly < ny) {
-                                    float va[kRtCh], vb[kRtCh], vc[kRtCh];
+                                    float va[kRtCh];
 #pragma unroll
-                                    for (int lz = 0; lz < kRtCh; ++lz) {
-                                        const int t = rt_at(lz + 1, ly + 1, lx + 1);
-                                        va[lz] = tA[t]; vb[lz] = tB[t]; vc[lz] = tC[t];
-                                    }
+                                    for (int lz = 0; lz < kRtCh; ++lz) va[lz] = tT[rt_at(lz + 1, ly + 1, lx + 1)];
 #pragma unroll
-                                    for (int lz = 1; lz < kRtCh; ++lz) {
-                                        va[lz] += va[lz - 1]; vb[lz] += vb[lz - 1]; vc[lz] += vc[lz - 1];
-                                    }
+                                    for (int lz = 1; lz < kRtCh; ++lz) va[lz] += va[lz - 1];
 #pragma unroll
                                     for (int lz = 1; lz < kRtCh; ++lz)
-                                        if (lz < nz) {
-                                            const int t = rt_at(lz + 1, ly + 1, lx + 1);
-                                            tA[t] = va[lz]; tB[t] = vb[lz]; tC[t] = vc[lz];
-                                        }
+                                        if (lz < nz) tT[rt_at(lz + 1, ly + 1, lx + 1)] = va[lz];
                                 }
                             }
                             __syncthreads();
@@ -1092,13 +1072,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                                 if (ka == 0.0f) continue;
                                 const int c11 = rt_at(zb, yb, 0), c10 = rt_at(zb, ya, 0);
                                 const int c01 = rt_at(za, yb, 0), c00 = rt_at(za, ya, 0);
-                                const float SA = ((tA[c11 + xb] - tA[c11 + xa]) - (tA[c10 + xb] - tA[c10 + xa])) -
-                                                 ((tA[c01 + xb] - tA[c01 + xa]) - (tA[c00 + xb] - tA[c00 + xa]));
-                                const float SB = ((tB[c11 + xb] - tB[c11 + xa]) - (tB[c10 + xb] - tB[c10 + xa])) -
-                                                 ((tB[c01 + xb] - tB[c01 + xa]) - (tB[c00 + xb] - tB[c00 + xa]));
-                                const float SC = ((tC[c11 + xb] - tC[c11 + xa]) - (tC[c10 + xb] - tC[c10 + xa])) -
-                                                 ((tC[c01 + xb] - tC[c01 + xa]) - (tC[c00 + xb] - tC[c00 + xa]));
-                                const float wgt = ka * fmaf(F, SA, fmaf(M, SB, SC));
+                                const float ST = ((tT[c11 + xb] - tT[c11 + xa]) - (tT[c10 + xb] - tT[c10 + xa])) -
+                                                 ((tT[c01 + xb] - tT[c01 + xa]) - (tT[c00 + xb] - tT[c00 + xa]));
+                                const float wgt = ka * ST;
                                 float *a = accs + slot * 3;
                                 a[0] = fmaf(wgt, g4.x, a[0]);
                                 a[1] = fmaf(wgt, g4.y, a[1]);
@@ -1208,14 +1184,14 @@ static double walk_triples(const Problem &pr)
 // smaller expected time.  Rates measured on MI355X at the sliding-motion
 // registration's three levels (256x256x128, supports 80 / 40 / 20): the walk
 // 0.9-1.7e12 of walk_triples' triples per second (the lower one here), the
-// tables 2.8e8 voxels per second (1.7e8 at support 80, where the walk wins
-// by far).
+// tables 4.2e8 voxels per second at support 20 (fewer at support 80, where
+// the walk wins by far).
 bool lcc_ref_table_applies(const Problem &pr)
 {
     RefTableGeom g;
     if (pr.lcc_ref_walk == 1 || !ref_table_geom(pr, g)) return false;
     if (pr.lcc_ref_walk == 2) return true;
-    constexpr double kWalkTriplesPerS = 0.9e12, kTableVoxelsPerS = 2.8e8;
+    constexpr double kWalkTriplesPerS = 0.9e12, kTableVoxelsPerS = 4.2e8;
     return walk_triples(pr) / kWalkTriplesPerS > (double)pr.n_vox / kTableVoxelsPerS;
 }
 
