@@ -83,7 +83,8 @@ def parse():
                     help="bilateral weight of the registrations (spatial / intensity sigma as above)")
     ap.add_argument("--no-pair-count", action="store_true", help="skip the exact k != 0 pair count")
     ap.add_argument("--cpu-crop", default=None, help="z,y,x voxels of the CPU baseline sample")
-    ap.add_argument("--cpu-repeats", type=int, default=3, help="timed CPU baseline evaluations (median)")
+    ap.add_argument("--cpu-repeats", type=int, default=5,
+                    help="timed CPU baseline evaluations after one untimed one (median)")
     ap.add_argument("--calibration-copy", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of MIB MiB first (reference dispatch for HBM counters)")
     return ap.parse_args()

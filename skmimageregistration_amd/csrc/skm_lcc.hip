@@ -736,13 +736,15 @@ __global__ __launch_bounds__(kThreads) void k_lcc_gamma(Problem pr, const float4
 // [p_k, p_k+1) of the control-point grid.  Every control point with
 // |p_j - y| < support for such a voxel lies in the box k - A .. k - A + NB - 1
 // per axis (NB = 2 ceil(support / h), A = NB / 2 - 1), whose points have one
-// FP32 sum slot each in LDS for the whole block.  Per voxel: the reach masks
-// and ranks; the list of active slots (points within the support whose
-// windows keep a centre on every axis, about half the box); the table of the
-// centres' terms kw (kw (F A + M B) + C) in chunks of at most kRtCh ranks per
-// axis (prefix along x in registers, then y and z in LDS, each line's loads
-// issued together; FP32); and for the active slots, 64 at a time, an 8-corner
-// difference of the table.
+// FP32 sum slot each for the whole block: lane l holds the slots of box
+// column (l % NB0, l / NB0) in registers, one per layer bz.  Per voxel: the
+// reach masks and ranks; each column's active layers (points within the
+// support whose windows keep a centre on every axis, about half the box); the
+// table of the centres' terms kw (kw (F A + M B) + C) in chunks of at most
+// kRtCh ranks per axis (prefix along x in registers, then y and z in LDS,
+// each line's loads issued together; FP32); and per active slot an 8-corner
+// difference of the table.  The LDS left (4.3 KB per wave) lets 4 waves run
+// per SIMD, which the registers are held to.
 // Each block writes its slots, k_lcc_reft_finish adds the blocks in order in
 // FP64.  A control-point cell wider than ~5 voxels is split into pieces along
 // each axis, one wave each, so that coarse grids still fill the GPU.
@@ -839,15 +841,11 @@ __device__ __forceinline__ void rt_axis(const RefReach &rr, int d, int yi, int l
 constexpr int kRtRow = kRtT + 1, kRtPlane = kRtT * kRtRow + 3, kRtSize = kRtT * kRtPlane;
 __device__ __forceinline__ int rt_at(int z, int y, int x) { return z * kRtPlane + y * kRtRow + x; }
 
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_lcc_ref_table(Problem pr, const float4 *__restrict__ dense,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_lcc_ref_table(Problem pr, const float4 *__restrict__ dense,
                                                       const float4 *__restrict__ coef, RefReach rr,
                                                       RefTableGeom g, float *__restrict__ partials)
 {
     __shared__ float tT[kRtSize];
-    __shared__ float accs[kRtBox * 64 * 3];  // per box control point (slot bz * 64 + lane) and axis
-    // the current voxel's active slots: slot (9 bits), bx, by, bz (3 bits
-    // each), the z rank range z0, z1 (6 bits each)
-    __shared__ unsigned list[kRtBox * 64];
     __shared__ int cells[3][64];
     __shared__ unsigned allowed[3][64];
     __shared__ int wlo[3][kRtBox], whi[3][kRtBox];  // box control points' windows (sample cells)
@@ -861,7 +859,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                       g.kmin[2] + cell / (g.nk[0] * g.nk[1])};
     const int q[3] = {piece % g.np[0], (piece / g.np[0]) % g.np[1], piece / (g.np[0] * g.np[1])};
     for (int i = lane; i < kRtSize; i += 64) tT[i] = 0.0f;  // borders stay zero
-    for (int i = lane; i < kRtBox * 64 * 3; i += 64) accs[i] = 0.0f;
+    // the sums of this lane's box column, slot bz * 64 + lane, per axis
+    float acc[kRtBox][3];
+#pragma unroll
+    for (int bz = 0; bz < kRtBox; ++bz) acc[bz][0] = acc[bz][1] = acc[bz][2] = 0.0f;
     if (lane < 3 * kRtBox) {
         // windows are separable for stationary kernels: axis d's from a
         // control point on that axis
@@ -945,31 +946,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                 to_global<3>(pr.fixed, fyv, y);
                 const float M = g4.w;
                 __syncthreads();
-                // the active slots: control points within the support whose
-                // windows keep some centre on every axis, listed in slot order
-                int nact = 0;
+                // the active slots of this lane's column (bit bz): control
+                // points within the support whose windows keep some centre on
+                // every axis
+                const int x0 = lane_ok ? rlo[0][bx] : 1, x1 = lane_ok ? rhi[0][bx] : 0;
+                const int y0 = lane_ok ? rlo[1][by] : 1, y1 = lane_ok ? rhi[1][by] : 0;
+                unsigned amask = 0u;
                 {
-                    const int x0 = lane_ok ? rlo[0][bx] : 1, x1 = lane_ok ? rhi[0][bx] : 0;
-                    const int y0 = lane_ok ? rlo[1][by] : 1, y1 = lane_ok ? rhi[1][by] : 0;
                     const bool xy_ok = jxy_ok && x0 <= x1 && y0 <= y1;
-                    const unsigned lw = (unsigned)lane | ((unsigned)bx << 9) | ((unsigned)by << 12);
                     const float dxy = (px - y[0]) * (px - y[0]) + (py - y[1]) * (py - y[1]);
-                    for (int bz = 0; bz < g.NB[2]; ++bz) {
+#pragma unroll
+                    for (int bz = 0; bz < kRtBox; ++bz) {
                         const int jz = k[2] - g.A[2] + bz;
                         const float dz = pr.cp.o[2] + pr.cp.h[2] * (float)jz - y[2];
                         const int z0 = rlo[2][bz], z1 = rhi[2][bz];
-                        const bool act = xy_ok && jz >= 0 && jz < pr.cp.n[2] && z0 <= z1 && dxy + dz * dz < s2;
-                        const unsigned long long bal = __ballot(act);
-                        if (act) {
-                            const int r = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                            list[nact + r] = (lw + ((unsigned)bz << 6)) | ((unsigned)bz << 15) |
-                                             ((unsigned)(z0 & 63) << 18) | ((unsigned)(z1 & 63) << 24);
-                        }
-                        nact += (int)__popcll(bal);
+                        if (bz < g.NB[2] && xy_ok && jz >= 0 && jz < pr.cp.n[2] && z0 <= z1 && dxy + dz * dz < s2)
+                            amask |= 1u << bz;
                     }
                 }
-                if (nact == 0) continue;
+                if (__ballot(amask != 0u) == 0ull) continue;
                 for (int c0z = 0; c0z < R[2]; c0z += kRtCh)
                     for (int c0y = 0; c0y < R[1]; c0y += kRtCh)
                         for (int c0x = 0; c0x < R[0]; c0x += kRtCh) {
@@ -1050,35 +1045,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
                                 }
                             }
                             __syncthreads();
-                            // the active slots, 64 at a time: 8-corner sums of their boxes
-                            for (int e0 = 0; e0 < nact; e0 += 64) {
-                                if (e0 + lane >= nact) continue;
-                                const unsigned ent = list[e0 + lane];
-                                const int slot = (int)(ent & 511u);
-                                const int sbx = (int)((ent >> 9) & 7u), sby = (int)((ent >> 12) & 7u);
-                                const int sbz = (int)((ent >> 15) & 7u);
-                                const int xa = max(rlo[0][sbx], c0x) - c0x, xb = min(rhi[0][sbx], c0x + nx - 1) - c0x + 1;
-                                const int ya = max(rlo[1][sby], c0y) - c0y, yb = min(rhi[1][sby], c0y + ny - 1) - c0y + 1;
-                                const int za = max((int)((ent >> 18) & 63u), c0z) - c0z;
-                                const int zb = min((int)((ent >> 24) & 63u), c0z + nz - 1) - c0z + 1;
-                                if (xa >= xb || ya >= yb || za >= zb) continue;
-                                const float qx = pr.cp.o[0] + pr.cp.h[0] * (float)(k[0] - g.A[0] + sbx);
-                                const float qy = pr.cp.o[1] + pr.cp.h[1] * (float)(k[1] - g.A[1] + sby);
-                                const float qz = pr.cp.o[2] + pr.cp.h[2] * (float)(k[2] - g.A[2] + sbz);
-                                const float vv[3] = {qx - y[0], qy - y[1], qz - y[2]};
-                                PState sty;
-                                sty.support = pr.kc.smin;
-                                const float ka = kernel_pair<3, 0>(pr.kc, vv, sty, sty);
-                                if (ka == 0.0f) continue;
-                                const int c11 = rt_at(zb, yb, 0), c10 = rt_at(zb, ya, 0);
-                                const int c01 = rt_at(za, yb, 0), c00 = rt_at(za, ya, 0);
-                                const float ST = ((tT[c11 + xb] - tT[c11 + xa]) - (tT[c10 + xb] - tT[c10 + xa])) -
-                                                 ((tT[c01 + xb] - tT[c01 + xa]) - (tT[c00 + xb] - tT[c00 + xa]));
-                                const float wgt = ka * ST;
-                                float *a = accs + slot * 3;
-                                a[0] = fmaf(wgt, g4.x, a[0]);
-                                a[1] = fmaf(wgt, g4.y, a[1]);
-                                a[2] = fmaf(wgt, g4.z, a[2]);
+                            // the column's active slots: 8-corner sums of their boxes
+                            const int xa = max(x0, c0x) - c0x, xb = min(x1, c0x + nx - 1) - c0x + 1;
+                            const int ya = max(y0, c0y) - c0y, yb = min(y1, c0y + ny - 1) - c0y + 1;
+                            if (amask != 0u && xa < xb && ya < yb) {
+                                const float qx = pr.cp.o[0] + pr.cp.h[0] * (float)(k[0] - g.A[0] + bx);
+                                const float qy = pr.cp.o[1] + pr.cp.h[1] * (float)(k[1] - g.A[1] + by);
+#pragma unroll
+                                for (int bz = 0; bz < kRtBox; ++bz) {
+                                    if (!((amask >> bz) & 1u)) continue;
+                                    const int za = max(rlo[2][bz], c0z) - c0z;
+                                    const int zb = min(rhi[2][bz], c0z + nz - 1) - c0z + 1;
+                                    if (za >= zb) continue;
+                                    const float qz = pr.cp.o[2] + pr.cp.h[2] * (float)(k[2] - g.A[2] + bz);
+                                    const float vv[3] = {qx - y[0], qy - y[1], qz - y[2]};
+                                    PState sty;
+                                    sty.support = pr.kc.smin;
+                                    const float ka = kernel_pair<3, 0>(pr.kc, vv, sty, sty);
+                                    if (ka == 0.0f) continue;
+                                    const int c11 = rt_at(zb, yb, 0), c10 = rt_at(zb, ya, 0);
+                                    const int c01 = rt_at(za, yb, 0), c00 = rt_at(za, ya, 0);
+                                    const float ST = ((tT[c11 + xb] - tT[c11 + xa]) - (tT[c10 + xb] - tT[c10 + xa])) -
+                                                     ((tT[c01 + xb] - tT[c01 + xa]) - (tT[c00 + xb] - tT[c00 + xa]));
+                                    const float wgt = ka * ST;
+                                    acc[bz][0] = fmaf(wgt, g4.x, acc[bz][0]);
+                                    acc[bz][1] = fmaf(wgt, g4.y, acc[bz][1]);
+                                    acc[bz][2] = fmaf(wgt, g4.z, acc[bz][2]);
+                                }
                             }
                             __syncthreads();  // the table's readers, before the next chunk or voxel
                         }
@@ -1086,7 +1079,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
         }
     }
     // this block's sums: [block][slot][axis]
-    for (int i = lane; i < kRtBox * 64 * 3; i += 64) partials[(long long)b * (kRtBox * 64 * 3) + i] = accs[i];
+    float *out = partials + (long long)b * (kRtBox * 64 * 3);
+#pragma unroll
+    for (int bz = 0; bz < kRtBox; ++bz) {
+        out[(bz * 64 + lane) * 3] = acc[bz][0];
+        out[(bz * 64 + lane) * 3 + 1] = acc[bz][1];
+        out[(bz * 64 + lane) * 3 + 2] = acc[bz][2];
+    }
 }
 
 // deriv_j = coef_scale * the sum over the blocks whose box holds j (cells,
