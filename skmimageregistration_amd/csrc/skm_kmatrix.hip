@@ -235,18 +235,16 @@ __global__ __launch_bounds__(kThreads) void k_km_value_build(Problem pr, BuildAr
     }
 }
 
-__device__ __forceinline__ float read_lane(float v, int l)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
 // Value pass from the value layout: the same per-sample prologue and tail as
 // k_value (evaluateValueMSE / evaluateValueNCC), u accumulated from the brick's
 // stored groups in the walk's order.  Eight records (32 control points) per
 // batch: lanes 0-31 load the batch's control-point indices and gather their
 // (c, w) with one vector load, lanes 0-7 the records' masks; the values of all
 // eight records are loaded together, and the next batch's indices and masks
-// are loaded before this batch's sums.
+// are loaded before this batch's sums.  The points' (c, w) reach every lane
+// through LDS, one broadcast read per point (a v_readlane per component would
+// be four VALU instructions per point, each with a wait before its SGPR is
+// read).
 template <int DIM, int MEASURE>
 __global__ __launch_bounds__(kThreads) void k_km_value(Problem pr, const float4 *__restrict__ cpc,
                                                        const uint4 *__restrict__ ent,
@@ -259,6 +257,7 @@ __global__ __launch_bounds__(kThreads) void k_km_value(Problem pr, const float4 
     constexpr int R = kValueBatch;
     static_assert(4 * R <= 64, "one lane per control point of a batch");
     __shared__ double lds[kPartialsK * (kThreads / 64)];
+    __shared__ float4 lcw[kThreads / 64][4 * R];  // per wave: the batch's points' (c, w)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     int l[3];
     tile_coords<DIM>(threadIdx.x, l);
@@ -313,17 +312,18 @@ __global__ __launch_bounds__(kThreads) void k_km_value(Problem pr, const float4 
             mlo = Wn[8 * rm];
             mhi = Wn[8 * rm + 1];
         }
+        if (lane < 4 * R) lcw[wave][lane] = cq;
         // a lane's k is 0 where it has no pair with the point: u + 0 c = u
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const float k4[4] = {kk[r].x, kk[r].y, kk[r].z, kk[r].w};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int src = 4 * r + q;
-                const float wk = read_lane(cq.w, src) * k4[q];
-                u[0] = fmaf(wk, read_lane(cq.x, src), u[0]);
-                u[1] = fmaf(wk, read_lane(cq.y, src), u[1]);
-                if (DIM == 3) u[2] = fmaf(wk, read_lane(cq.z, src), u[2]);
+                const float4 c4 = lcw[wave][4 * r + q];
+                const float wk = c4.w * k4[q];
+                u[0] = fmaf(wk, c4.x, u[0]);
+                u[1] = fmaf(wk, c4.y, u[1]);
+                if (DIM == 3) u[2] = fmaf(wk, c4.z, u[2]);
             }
         }
         V += o;
