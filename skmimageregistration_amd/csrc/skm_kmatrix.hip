@@ -47,6 +47,23 @@ __device__ __forceinline__ unsigned lane_rank(unsigned long long mask)
 
 __device__ __forceinline__ bool lane_bit(unsigned long long mask, int lane) { return (mask >> lane) & 1ull; }
 
+// Loads of the layouts, which every evaluation reads once: non-temporal, so
+// that they do not push the samples' and control points' data, which are read
+// again by other units, out of the L2.
+typedef float kmF4 __attribute__((ext_vector_type(4)));
+typedef unsigned kmU4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 stream_load(const float4 *p)
+{
+    const kmF4 v = __builtin_nontemporal_load(reinterpret_cast<const kmF4 *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 stream_load(const uint4 *p)
+{
+    const kmU4 v = __builtin_nontemporal_load(reinterpret_cast<const kmU4 *>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ unsigned stream_load(const unsigned *p) { return __builtin_nontemporal_load(p); }
+
 // Where one phase of a build writes: COUNT phases store capacities per unit
 // (entries, values: every lane whose pair passes the window and reach tests),
 // FILL phases the entries and values at the scanned offsets and the entry
@@ -283,9 +300,9 @@ __global__ __launch_bounds__(kThreads) void k_km_value(Problem pr, const float4 
     const int rz = lane < 4 * R ? lane >> 2 : 0, rm = lane < R ? lane : 0;
     unsigned zq = 0, mlo = 0, mhi = 0;
     if (ne > 0) {
-        zq = W[8 * rz + 2 + (lane & 3)];
-        mlo = W[8 * rm];
-        mhi = W[8 * rm + 1];
+        zq = stream_load(W + 8 * rz + 2 + (lane & 3));
+        mlo = stream_load(W + 8 * rm);
+        mhi = stream_load(W + 8 * rm + 1);
     }
     for (int e0 = 0; e0 < ne; e0 += R) {
         const float4 cq = cpc[zq];
@@ -303,14 +320,14 @@ __global__ __launch_bounds__(kThreads) void k_km_value(Problem pr, const float4 
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const bool on = lane_bit(any[r], lane);
-            const float4 kv = V[on ? off[r] + (int)lane_rank(any[r]) : 0];
+            const float4 kv = stream_load(V + (on ? off[r] + (int)lane_rank(any[r]) : 0));
             kk[r] = make_float4(on ? kv.x : 0.0f, on ? kv.y : 0.0f, on ? kv.z : 0.0f, on ? kv.w : 0.0f);
         }
         if (e0 + R < ne) {
             const unsigned *Wn = W + 8 * (e0 + R);
-            zq = Wn[8 * rz + 2 + (lane & 3)];
-            mlo = Wn[8 * rm];
-            mhi = Wn[8 * rm + 1];
+            zq = stream_load(Wn + 8 * rz + 2 + (lane & 3));
+            mlo = stream_load(Wn + 8 * rm);
+            mhi = stream_load(Wn + 8 * rm + 1);
         }
         if (lane < 4 * R) lcw[wave][lane] = cq;
         // a lane's k is 0 where it has no pair with the point: u + 0 c = u
@@ -650,7 +667,7 @@ __global__ __launch_bounds__(kThreads) void k_km_gather(Problem pr, const float4
 #pragma unroll
     for (int i = 0; i < B; ++i)
 #pragma unroll
-        for (int w = 0; w < 3; ++w) en[i][w] = ne > 0 ? E[3 * i + w] : make_uint4(0u, 0u, 0u, 0u);
+        for (int w = 0; w < 3; ++w) en[i][w] = ne > 0 ? stream_load(E + 3 * i + w) : make_uint4(0u, 0u, 0u, 0u);
     for (int e0 = 0; e0 < ne; e0 += B) {
         // lanes without a pair read element 0 and use zeros
         float4 cv[B], kk[B][G];
@@ -667,7 +684,7 @@ __global__ __launch_bounds__(kThreads) void k_km_gather(Problem pr, const float4
 #pragma unroll
             for (int s = 0; s < G; ++s) {
                 const bool ons = lane_bit(sm[s], lane);
-                const float4 b = V[ons ? off + (int)lane_rank(sm[s]) : 0];
+                const float4 b = stream_load(V + (ons ? off + (int)lane_rank(sm[s]) : 0));
                 kk[i][s] = make_float4(ons ? b.x : 0.0f, ons ? b.y : 0.0f, ons ? b.z : 0.0f, ons ? b.w : 0.0f);
                 off += __popcll(sm[s]);
             }
@@ -676,7 +693,7 @@ __global__ __launch_bounds__(kThreads) void k_km_gather(Problem pr, const float4
 #pragma unroll
             for (int i = 0; i < B; ++i)
 #pragma unroll
-                for (int w = 0; w < 3; ++w) en[i][w] = E[3 * (e0 + B + i) + w];
+                for (int w = 0; w < 3; ++w) en[i][w] = stream_load(E + 3 * (e0 + B + i) + w);
         }
 #pragma unroll
         for (int i = 0; i < B; ++i)
