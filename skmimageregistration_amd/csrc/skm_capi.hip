@@ -377,11 +377,16 @@ struct skm_evaluator {
     double bl_sigma_s = 1.0, bl_sigma_i = 1.0;
     float *d_cp_intensity = nullptr;       // fixed image at the control points (bilateral term)
     KernelMatrix km;                       // assembled k(x_m, p_j) (subsample 1; see use_kernel_matrix)
+    // the regularizers' second stream (MSE / NCC: beside the measure's passes)
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
+    double *d_reg_add = nullptr;           // their derivative terms (SoA, FP64), added by the gather's finish
 
     ~skm_evaluator()
     {
-        // work queued on the stream may still read these buffers
+        // work queued on the streams may still read these buffers
         if (stream) (void)hipStreamSynchronize(stream);
+        if (side) (void)hipStreamSynchronize(side);
         kernel_matrix_release(km);
         void *ptrs[] = {d_fixed, d_coeffs, d_weight, d_df, d_cpc, d_cp_state, d_cp_pstate, d_smp_pstate,
                         d_vox_pstate, d_cp_row_lam, d_smp_row_lam, d_vox_row_lam, d_cp_reach, d_cp_window,
@@ -390,11 +395,14 @@ struct skm_evaluator {
                         d_hreg_partials, d_gather_partials, d_sc, d_deriv, d_cp_in, d_cpw_in, d_dst, d_dense,
                         d_dense_q, d_ccoef, d_gamma, d_params, d_prev_g, d_best, d_opt, d_records, d_rng,
                         d_lcc_ref_partials, d_lcc_bound, d_gamma_fx, d_cp_intensity, d_lcc_cpk,
-                        d_lcc_reft_partials};
+                        d_lcc_reft_partials, d_reg_add};
         for (void *p : ptrs)
             if (p) device_release(p);
         for (auto &p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
         for (auto e : event_pool) (void)hipEventDestroy(e);
+        if (side_fork) (void)hipEventDestroy(side_fork);
+        if (side_join) (void)hipEventDestroy(side_join);
+        if (side) return_stream(stream_device, side);
         if (own_stream && stream) return_stream(stream_device, stream);
     }
 };
@@ -758,12 +766,38 @@ skm_status use_kernel_matrix(skm_evaluator *ev, bool *use)
     return SKM_SUCCESS;
 }
 
+// The regularizers' second stream and its fork / join events (and, with a
+// derivative, the buffer of their terms), made at the first use; false (and
+// the regularizers run in sequence) while the evaluator's stream is being
+// captured into a graph, or where they cannot be made.  (A capture holding
+// the fork ended the process in the runtime during skm_optimize_batch's
+// capture of three evaluators; captures keep the sequential order.)
+bool side_stream(skm_evaluator *ev, bool with_derivative)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ev->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
+    if (ev->side && ev->side_fork && ev->side_join && (!with_derivative || ev->d_reg_add)) return true;
+    if (with_derivative && !ev->d_reg_add &&
+        dev_alloc(&ev->d_reg_add, (size_t)ev->pr.n_cp * ev->pr.dim) != SKM_SUCCESS) {
+        ev->d_reg_add = nullptr;
+        return false;
+    }
+    if (!ev->side && take_stream(&ev->side) != hipSuccess) ev->side = nullptr;
+    if (!ev->side_fork && hipEventCreateWithFlags(&ev->side_fork, hipEventDisableTiming) != hipSuccess)
+        ev->side_fork = nullptr;
+    if (!ev->side_join && hipEventCreateWithFlags(&ev->side_join, hipEventDisableTiming) != hipSuccess)
+        ev->side_join = nullptr;
+    return ev->side && ev->side_fork && ev->side_join;
+}
+
 // One evaluation on the device: value (+ derivative) of the measure and the
 // regularizers, everything left in d_sc / deriv (GpuEvaluator.cu:94-357, 361-714).
 skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *deriv)
 {
     Problem &pr = ev->pr;
     hipStream_t s = ev->stream;
+    const bool regs = ev->reg_w[0] > 0 || ev->reg_w[1] > 0 || ev->reg_w[2] > 0 || ev->reg_w[3] > 0;
+    bool regs_done = false;
     uint64_t drawn = 0;  // random numbers drawn for the offsets of this evaluation
     if (pr.sub > 1 && (!ev->evaluated_once || do_resampling)) {
         PassTimer t(ev, 0);
@@ -775,6 +809,26 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
     if (measure == SKM_MEASURE_MSE || measure == SKM_MEASURE_NCC) {
         bool km = false;
         SKM_TRY(use_kernel_matrix(ev, &km));
+        // The regularizers read the control points only: on the second stream,
+        // beside the value pass, joined before the gather (whose finish adds
+        // their derivative terms, in the same FP64 sum the regularizers would
+        // make) or, without a derivative, at the end.  For evaluations of at
+        // least kBesideSamples samples: below, the fork and join cost more than
+        // they hide (12k-25k samples: 7-10 % slower per iteration); in
+        // sequence while the passes are timed.
+        constexpr long long kBesideSamples = 1ll << 18;
+        Problem gp = pr;
+        const bool beside = regs && !ev->timing && pr.n_sub >= kBesideSamples && side_stream(ev, deriv != nullptr);
+        if (beside) {
+            gp.reg_add = deriv ? ev->d_reg_add : nullptr;
+            SKM_HIP(hipEventRecord(ev->side_fork, s));
+            SKM_HIP(hipStreamWaitEvent(ev->side, ev->side_fork, 0));
+            const BilateralArgs bl = make_bilateral_args(pr, ev->reg_w[3], ev->bl_sigma_s, ev->bl_sigma_i);
+            launch_regularizers(gp, ev->reg_w, ev->pg_scaling, bl, deriv != nullptr, deriv, ev->d_reg_partials,
+                                ev->d_sc, ev->side);
+            SKM_HIP(hipEventRecord(ev->side_join, ev->side));
+            regs_done = true;
+        }
         {
             PassTimer t(ev, 1);
             if (km) launch_value_km(pr, ev->km, measure, ev->d_sg, ev->d_sb, ev->d_partials, &blocks, s);
@@ -782,15 +836,18 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
         }
         {
             PassTimer t(ev, 2);
-            launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s, ev->d_rng, drawn, ev->rng_inc);
+            launch_reduce(pr, measure, ev->d_partials, blocks, ev->d_sc, s, ev->d_rng, drawn, ev->rng_inc, beside);
         }
         if (deriv) {
             { PassTimer t(ev, 3); launch_prepare(pr, measure, ev->d_sg, ev->d_sb, ev->d_sc, ev->d_sv, s); }
+            if (beside) SKM_HIP(hipStreamWaitEvent(s, ev->side_join, 0));
             {
                 PassTimer t(ev, 4);
-                if (km) launch_gather_km(pr, ev->km, ev->d_sv, deriv, s);
-                else launch_gather(pr, ev->d_sv, deriv, s);
+                if (km) launch_gather_km(gp, ev->km, ev->d_sv, deriv, s);
+                else launch_gather(gp, ev->d_sv, deriv, s);
             }
+        } else if (beside) {
+            SKM_HIP(hipStreamWaitEvent(s, ev->side_join, 0));
         }
     } else if (measure == SKM_MEASURE_LCC) {
         SKM_TRY(ensure_lcc_buffers(ev));
@@ -866,7 +923,7 @@ skm_status run_eval(skm_evaluator *ev, int measure, int do_resampling, float *de
     } else {
         return fail(SKM_ERROR_INVALID_ARGUMENT, "unknown measure");
     }
-    if (ev->reg_w[0] > 0 || ev->reg_w[1] > 0 || ev->reg_w[2] > 0 || ev->reg_w[3] > 0) {
+    if (regs && !regs_done) {
         PassTimer t(ev, 5);
         const BilateralArgs bl = make_bilateral_args(pr, ev->reg_w[3], ev->bl_sigma_s, ev->bl_sigma_i);
         launch_regularizers(pr, ev->reg_w, ev->pg_scaling, bl, deriv != nullptr, deriv, ev->d_reg_partials, ev->d_sc,

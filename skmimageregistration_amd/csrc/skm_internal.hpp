@@ -52,6 +52,9 @@ struct Problem {
     const float *cp_intensity; // fixed image at each control point (nearest neighbour), bilateral term
     const float *smp_row_lam;  // largest lam per row of sample cells (my + nsub[1] mz)
     double *gather_partials;   // weighted-mode gather: FP64 sums per (slab, control-point tile)
+    double *reg_add;           // regularizers run beside the measure's passes: their derivative terms
+                               // (SoA, FP64), added to deriv by the gather's finish; nullptr: the
+                               // regularizers add them to deriv themselves
     int lcc_chunks;            // reference-window LCC derivative: workgroups per cp tile (0: automatic)
     int gather_split;          // derivative gather: sample planes per slab, 4 or 8 (0: automatic, 8)
     int value_split;           // value pass: waves per sample brick, 1/4/8/16 (0: automatic)
@@ -82,13 +85,17 @@ void launch_offsets(const Problem &pr, int *offsets, uint64_t *state, uint64_t i
 void launch_pack_cp(const Problem &pr, const float *cp, const float *cpw, int soa, float4 *cpc, hipStream_t s);
 void launch_value(const Problem &pr, int measure, float4 *sg, float *sb, double *partials, int *blocks_out,
                   hipStream_t s);
-// Also moves the PCG32 state at *rng rng_delta draws ahead (rng_delta 0: untouched).
+// Also moves the PCG32 state at *rng rng_delta draws ahead (rng_delta 0: untouched),
+// and sets the regularizer values to 0 unless keep_reg (the regularizers'
+// reduction writes them in this evaluation).
 void launch_reduce(const Problem &pr, int measure, const double *partials, int blocks, Scalars *sc,
-                   hipStream_t s, uint64_t *rng = nullptr, uint64_t rng_delta = 0, uint64_t rng_inc = 1);
+                   hipStream_t s, uint64_t *rng = nullptr, uint64_t rng_delta = 0, uint64_t rng_inc = 1,
+                   bool keep_reg = false);
 void launch_prepare(const Problem &pr, int measure, const float4 *sg, const float *sb, const Scalars *sc,
                     float4 *sv, hipStream_t s);
 void launch_gather(const Problem &pr, const float4 *sv, float *deriv_soa, hipStream_t s);
-// deriv_j = w_j * the sum of j's (slab, tile) partials in slab order
+// deriv_j = w_j * the sum of j's (slab, tile) partials in slab order (plus
+// pr.reg_add's term, where set, as the regularizers would add it)
 void launch_gather_finish(const Problem &pr, float *deriv_soa, hipStream_t s);
 // doubles of Problem::gather_partials for a sample grid with `planes` cells along the last axis
 long long gather_partials_size(const Problem &pr, long long planes);

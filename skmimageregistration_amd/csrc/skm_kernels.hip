@@ -250,7 +250,8 @@ __global__ __launch_bounds__(SPLIT > 1 ? 64 * SPLIT : kThreads) __attribute__((a
 // Host reduction of GpuEvaluator.cu:192-200 (MSE), 229-262 / 501-534 (NCC) and
 // 287-297 (LCC), done by one block in a fixed order.
 __global__ __launch_bounds__(kThreads) void k_reduce(int measure, const double *__restrict__ partials, int blocks,
-                                                     Scalars *sc, uint64_t *rng, uint64_t rng_delta, uint64_t rng_inc)
+                                                     Scalars *sc, uint64_t *rng, uint64_t rng_delta, uint64_t rng_inc,
+                                                     int keep_reg)
 {
     __shared__ double lds[kPartialsK * 4];
     double acc[kPartialsK];
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(int measure, const double *
         sc->measure = acc[0] / n;
         sc->coef_scale = 1.0 / n;
     }
-    sc->reg[0] = sc->reg[1] = sc->reg[2] = sc->reg[3] = 0.0;
+    if (!keep_reg) sc->reg[0] = sc->reg[1] = sc->reg[2] = sc->reg[3] = 0.0;
     // past the offsets this evaluation drew (k_offsets has finished: the
     // value pass ran in between)
     if (rng_delta) *rng = lcg_advance(*rng, rng_delta, rng_inc);
@@ -518,7 +519,10 @@ __global__ __launch_bounds__(kThreads) void k_gather_finish(Problem pr, int n_ti
     }
     const double w = pr.cpc[j].w;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) deriv[j + d * pr.n_cp] = (float)(w * sum[d]);
+    for (int d = 0; d < DIM; ++d) {
+        const float g = (float)(w * sum[d]);
+        deriv[j + d * pr.n_cp] = pr.reg_add ? (float)((double)g + pr.reg_add[j + d * pr.n_cp]) : g;
+    }
 }
 
 // ------------------------------------------------------------ regularizers
@@ -656,8 +660,10 @@ __global__ __launch_bounds__(kThreads) void k_regularizers(Problem pr, double w_
                                   bl.weight * (double)db[d]);
             if (lane == 0)
 #pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                    deriv[j + d * pr.n_cp] = (float)((double)deriv[j + d * pr.n_cp] + add[d]);
+                for (int d = 0; d < DIM; ++d) {
+                    if (pr.reg_add) pr.reg_add[j + d * pr.n_cp] = add[d];
+                    else deriv[j + d * pr.n_cp] = (float)((double)deriv[j + d * pr.n_cp] + add[d]);
+                }
         }
     }
     block_sum<4>(vals, lds);
@@ -847,9 +853,9 @@ void launch_value(const Problem &pr, int measure, float4 *sg, float *sb, double 
 }
 
 void launch_reduce(const Problem &pr, int measure, const double *partials, int blocks, Scalars *sc, hipStream_t s,
-                   uint64_t *rng, uint64_t rng_delta, uint64_t rng_inc)
+                   uint64_t *rng, uint64_t rng_delta, uint64_t rng_inc, bool keep_reg)
 {
-    k_reduce<<<1, kThreads, 0, s>>>(measure, partials, blocks, sc, rng, rng_delta, rng_inc);
+    k_reduce<<<1, kThreads, 0, s>>>(measure, partials, blocks, sc, rng, rng_delta, rng_inc, keep_reg ? 1 : 0);
 }
 
 void launch_prepare(const Problem &pr, int measure, const float4 *sg, const float *sb, const Scalars *sc,
