@@ -777,6 +777,9 @@ bool side_stream(skm_evaluator *ev, bool with_derivative)
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(ev->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
     if (ev->side && ev->side_fork && ev->side_join && (!with_derivative || ev->d_reg_add)) return true;
+    // made on the evaluator's device only (the pool hands out the current device's streams)
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != ev->stream_device) return false;
     if (with_derivative && !ev->d_reg_add &&
         dev_alloc(&ev->d_reg_add, (size_t)ev->pr.n_cp * ev->pr.dim) != SKM_SUCCESS) {
         ev->d_reg_add = nullptr;
