@@ -35,7 +35,7 @@ namespace {
 // with empty records (mask 0) to a whole number of batches, so that a batch's
 // loads are unconditional.  Values are allocated with kEntryPad to spare (lanes
 // without a pair read their unit's first value).
-constexpr int kValueBatch = 8, kGatherBatch = 2;
+constexpr int kValueBatch = 16, kGatherBatch = 2;
 constexpr int kEntryPad = 16;
 
 __host__ __device__ constexpr int round_up(int n, int b) { return (n + b - 1) / b * b; }
@@ -254,10 +254,10 @@ __global__ __launch_bounds__(kThreads) void k_km_value_build(Problem pr, BuildAr
 
 // Value pass from the value layout: the same per-sample prologue and tail as
 // k_value (evaluateValueMSE / evaluateValueNCC), u accumulated from the brick's
-// stored groups in the walk's order.  Eight records (32 control points) per
-// batch: lanes 0-31 load the batch's control-point indices and gather their
-// (c, w) with one vector load, lanes 0-7 the records' masks; the values of all
-// eight records are loaded together, and the next batch's indices and masks
+// stored groups in the walk's order.  Sixteen records (64 control points) per
+// batch: each lane loads one of the batch's control-point indices and gathers
+// its (c, w) with one vector load, lanes 0-15 the records' masks; the values of
+// all sixteen records are loaded together, and the next batch's indices and masks
 // are loaded before this batch's sums.  The points' (c, w) reach every lane
 // through LDS, one broadcast read per point (a v_readlane per component would
 // be four VALU instructions per point, each with a wait before its SGPR is
