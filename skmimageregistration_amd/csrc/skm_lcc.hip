@@ -979,22 +979,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                                     const int cy = cells[1][c0y + ly], cz = cells[2][c0z + lz];
                                     const unsigned ay = allowed[1][c0y + ly], az = allowed[2][c0z + lz];
                                     const unsigned rowc = (unsigned)pr.nsub[0] * (unsigned)(cy + pr.nsub[1] * cz);
-                                    unsigned cxs[kRtCh], w[kRtCh];
-                                    float4 abc[kRtCh];
+                                    unsigned cxs[kRtCh];
 #pragma unroll
                                     for (int lx = 0; lx < kRtCh; ++lx) cxs[lx] = (unsigned)cells[0][c0x + min(lx, nx - 1)];
-#pragma unroll
-                                    for (int lx = 0; lx < kRtCh; ++lx) abc[lx] = coef[rowc + cxs[lx]];
-#pragma unroll
-                                    for (int lx = 0; lx < kRtCh; ++lx) w[lx] = __float_as_uint(abc[lx].w);
                                     float st = 0.0f;
                                     const int tb = rt_at(lz + 1, ly + 1, 1);
-#pragma unroll
-                                    for (int lx = 0; lx < kRtCh; ++lx) {
+                                    // the candidates in groups of four (four, then three), each
+                                    // group's coefficients loaded together: fewer registers held
+                                    // at once than with all seven (16 bytes of spill per lane at
+                                    // four waves per SIMD instead of 60)
+                                    auto term = [&](int lx, const float4 &a4) {
+                                        const unsigned wv = __float_as_uint(a4.w);
                                         const unsigned ax = allowed[0][c0x + min(lx, nx - 1)];
-                                        const unsigned ox = w[lx] & 0xffu, oy = (w[lx] >> 8) & 0xffu;
-                                        const unsigned oz = (w[lx] >> 16) & 0xffu;
-                                        if ((w[lx] & 0x80000000u) && ((ax >> ox) & 1u) && ((ay >> oy) & 1u) &&
+                                        const unsigned ox = wv & 0xffu, oy = (wv >> 8) & 0xffu;
+                                        const unsigned oz = (wv >> 16) & 0xffu;
+                                        if ((wv & 0x80000000u) && ((ax >> ox) & 1u) && ((ay >> oy) & 1u) &&
                                             ((az >> oz) & 1u)) {
                                             const float fp[3] = {(float)((int)cxs[lx] * sub + (sub > 1 ? (int)ox : 0)),
                                                                  (float)(cy * sub + (sub > 1 ? (int)oy : 0)),
@@ -1008,9 +1007,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                                             const float q = r2 * inv_s2, q2 = q * q;
                                             const float f = fmaxf(0.0f, 1.0f - q2 * q2 * q);
                                             const float kw = f * f;
-                                            st = fmaf(kw, fmaf(kw, fmaf(F, abc[lx].x, M * abc[lx].y), abc[lx].z), st);
+                                            st = fmaf(kw, fmaf(kw, fmaf(F, a4.x, M * a4.y), a4.z), st);
                                         }
                                         if (lx < nx) tT[tb + lx] = st;
+                                    };
+                                    constexpr int kG = 4;  // candidates per group
+#pragma unroll
+                                    for (int g0 = 0; g0 < kRtCh; g0 += kG) {
+                                        float4 abc[kG];
+#pragma unroll
+                                        for (int i = 0; i < kG; ++i)
+                                            if (g0 + i < kRtCh) abc[i] = coef[rowc + cxs[g0 + i]];
+#pragma unroll
+                                        for (int i = 0; i < kG; ++i)
+                                            if (g0 + i < kRtCh) term(g0 + i, abc[i]);
+                                        __builtin_amdgcn_sched_barrier(0);
                                     }
                                 }
                             }
